@@ -1,0 +1,715 @@
+/* eh_device.h -- per-particle element maps for gfx950, and the element
+ * descriptor the host prepares for them.
+ *
+ * Every map keeps the operation order of the elegant routine it implements
+ * (cited per function) so that, compiled with -ffp-contract=off, a particle's
+ * six coordinates come out bit-identical to the CPU routine.  The maps work on
+ * one particle held in registers; loss bookkeeping writes exactly the values the
+ * CPU routine leaves in a lost row (before its swap to the top of the array).
+ *
+ * Per-element constants that the reference recomputes per particle from
+ * element parameters (e.g. KnL/n_parts * kickFrac[step], drift*driftFrac[step])
+ * are computed once on the host with the same expression, which gives the same
+ * double.
+ */
+#ifndef EH_DEVICE_H
+#define EH_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "eh_math.h"
+
+#define EH_MAXSUB 8     /* integrator sub-steps (order 6) */
+#define EH_MAXFTERMS 66 /* field terms per component (expansion order 10) */
+
+#define EH_DBL_MAX 1.7976931348623157e308
+
+/* loss stages inside one element, in the order the reference compacts them */
+#define EH_STAGE_ELEMENT 0 /* swap-to-top inside the element routine */
+#define EH_STAGE_MAXAMP 1  /* limit_amplitudes with MAXAMP limits after the element */
+#define EH_STAGE_NAN 2     /* limit_amplitudes(DBL_MAX) NaN sweep */
+
+typedef struct {
+  int32_t i, j;
+  double coef;
+} eh_fterm;
+
+/* element descriptor: everything the maps need, precomputed on the host */
+typedef struct {
+  int32_t type; /* eh_type */
+  int32_t counted; /* counts as a particle-element unit (not MARK/MAXAMP) */
+  /* in-element aperture (MAXAMP passed to the routine) for pass 0 / later passes */
+  double ap_xmax[2], ap_ymax[2];
+  /* post-element MAXAMP sweep limits for pass 0 / later passes; extrapolate for DRIF */
+  double post_xmax[2], post_ymax[2];
+  int32_t post_ap[2];
+  int32_t extrap;
+  double z_start, z_end; /* last_z and z of do_tracking for this element */
+  double Po;
+
+  /* ---- drifts ---- */
+  double length;
+  double T11; /* DRIF second-order coefficient length/2 (0 for order 1) */
+
+  /* ---- kick multipoles ---- */
+  int32_t nslices, nsub, expand;
+  int32_t mp_order[3], mp_skew[3];
+  int32_t maxOrder;
+  int32_t has_xkick, has_ykick;
+  int32_t has_offset, has_tilt, ap_rtilt;
+  double mdrift; /* drift/n_parts */
+  double dsh[EH_MAXSUB];
+  double kick[EH_MAXSUB][3];
+  double xk[EH_MAXSUB], yk[EH_MAXSUB];
+  double mdx, mdy, mdz; /* element offsets */
+  double ncos, nsin;   /* entry rotation (angle tilt) */
+  double xcos, xsin;   /* exit rotation (angle -tilt) */
+  double ap_cs0, ap_cs1; /* aperture reverse-tilt rotation */
+
+  /* ---- CSBEND ---- */
+  int32_t csb_nslices;
+  int32_t csb_field; /* hasSkew || hasNormal */
+  int32_t csb_nfx, csb_nfy;
+  int32_t csb_edge1, csb_edge2;
+  double rho0, rho_actual;
+  double csb_dsh[EH_MAXSUB], csb_dshr[EH_MAXSUB], csb_ds[EH_MAXSUB];
+  double csb_dzl; /* n * s */
+  double cos_tt, sin_tt;
+  double dxi, dyi, dzi, dxf, dyf, dzf;
+  double tan_e1, tan_e2, e1, e2, psi1, psi2;
+  eh_fterm fx[EH_MAXFTERMS], fy[EH_MAXFTERMS];
+
+  /* ---- RFCA ---- */
+  int32_t rf_matrix, rf_nkicks, rf_same, rf_standing, rf_end1, rf_end2, rf_zero_volt;
+  double rf_len;  /* length / nKicks */
+  double rf_flen; /* full element length (volt == 0 drift) */
+  double rf_volt, rf_omega, rf_phase, rf_dtLight, rf_t0, rf_dx, rf_dy, rf_dgamma_same;
+} eh_desc;
+
+typedef struct {
+  double x, xp, y, yp, s, dp;
+} eh_p6;
+
+EH_FN double eh_sqr(double x) { return x * x; }
+
+__device__ __forceinline__ bool eh_isnan(double x) { return x != x; }
+__device__ __forceinline__ bool eh_isinf(double x) { return eh_fabs(x) == __builtin_inf(); }
+
+/* ------------------------------------------------------------------------- */
+/* DRIF through drift_matrix + generic order-2 loop (src/compute_matrices.c:391-410,
+ * src/matrix.c:370-385).  For finite inputs the generic loop reduces exactly to
+ * the expressions below (terms with zero matrix entries add signed zeros, which
+ * only normalise -0 to +0 where written as 0.0 + ...); any non-finite input makes
+ * every output NaN through 0*inf or 0*NaN. */
+__device__ __forceinline__ void eh_drif(eh_p6 &p, double L, double T) {
+  bool finite = !(eh_isnan(p.x) || eh_isnan(p.xp) || eh_isnan(p.y) || eh_isnan(p.yp) || eh_isnan(p.s) ||
+                  eh_isnan(p.dp) || eh_isinf(p.x) || eh_isinf(p.xp) || eh_isinf(p.y) || eh_isinf(p.yp) ||
+                  eh_isinf(p.s) || eh_isinf(p.dp));
+  if (!finite) {
+    double n = eh_nan();
+    p.x = p.xp = p.y = p.yp = p.s = p.dp = n;
+    return;
+  }
+  double x = (0.0 + L * p.xp) + p.x;
+  double xp = 0.0 + p.xp;
+  double y = (0.0 + L * p.yp) + p.y;
+  double yp = 0.0 + p.yp;
+  double s = ((L + p.s) + (0.0 + T * p.yp) * p.yp) + (0.0 + T * p.xp) * p.xp;
+  double dp = 0.0 + p.dp;
+  p.x = x;
+  p.xp = xp;
+  p.y = y;
+  p.yp = yp;
+  p.s = s;
+  p.dp = dp;
+}
+
+/* exactDrift, src/csbend.c:3811-3815 */
+__device__ __forceinline__ void eh_edrift(eh_p6 &p, double L) {
+  p.x += p.xp * L;
+  p.y += p.yp * L;
+  p.s += L * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+}
+
+/* ------------------------------------------------------------------------- */
+/* limit_amplitudes for one particle, src/limit_amplitudes.c:226-284 (openCode 0).
+ * Returns true if the particle is lost and applies the loss assignments. */
+__device__ __forceinline__ bool eh_limit_one(eh_p6 &p, double xmax, double ymax, double z, double Po, int extrap) {
+  int is_out = 0;
+  if (xmax > 0 && eh_fabs(p.x) > xmax)
+    is_out += 1;
+  if (ymax > 0 && eh_fabs(p.y) > ymax)
+    is_out += 2;
+  if (eh_isinf(p.x) || eh_isinf(p.y) || eh_isnan(p.x) || eh_isnan(p.y))
+    is_out += 4;
+  if (!is_out)
+    return false;
+  double dz = 0;
+  if (!(is_out & 4) && extrap) {
+    double dzx = -EH_DBL_MAX, dzy = -EH_DBL_MAX;
+    if ((is_out & 1) && p.xp != 0)
+      dzx = (p.x - (p.xp < 0 ? -1.0 : 1.0) * xmax) / p.xp;
+    if ((is_out & 2) && p.yp != 0)
+      dzy = (p.y - (p.yp < 0 ? -1.0 : 1.0) * ymax) / p.yp;
+    if (dzx > dzy)
+      dz = -dzx;
+    else
+      dz = -dzy;
+    if (dz == -EH_DBL_MAX)
+      dz = 0;
+  }
+  p.x += dz * p.xp;
+  p.y += dz * p.yp;
+  p.s = z + dz;
+  p.dp = Po * (1 + p.dp);
+  return true;
+}
+
+/* ------------------------------------------------------------------------- */
+/* rectangular in-element aperture, checkMultAperture (src/multipole.c:1751-1798) */
+__device__ __forceinline__ bool eh_ap_ok(double xIn, double yIn, double xmax, double ymax, int rtilt, double cs0,
+                                         double cs1) {
+  double x = xIn - 0.0, y = yIn - 0.0;
+  if (rtilt) {
+    double x0 = x, y0 = y;
+    x = x0 * cs0 + y0 * cs1;
+    y = -x0 * cs1 + y0 * cs0;
+  }
+  if ((xmax > 0 && eh_fabs(x) > xmax) || (ymax > 0 && eh_fabs(y) > ymax))
+    return false;
+  return true;
+}
+
+/* apply_canonical_multipole_kicks_ret/_noret, src/multipole.h:391-445, with the
+ * (x+iy)^n/n! coefficient table of expansion_coefficients (src/multipole.h:76-145) */
+template <int ORDER>
+__device__ __forceinline__ void eh_mkick(double &qx, double &qy, const double *xpow, const double *ypow, double K,
+                                         int skew) {
+  double sum_Fx = 0, sum_Fy = 0;
+#pragma unroll
+  for (int i = 0; i <= ORDER; i++) {
+    double coef;
+    if (ORDER == 0)
+      coef = 1.0;
+    else if (ORDER == 1)
+      coef = 1.0;
+    else if (ORDER == 2)
+      coef = (i == 0) ? 1. / 2 : (i == 1 ? 1.0 : -1. / 2);
+    else
+      coef = (((i / 2) & 1) ? -1.0 : 1.0) /
+             ((i == 0 ? 1.0 : (i == 1 ? 1.0 : (i == 2 ? 2.0 : 6.0))) *
+              ((ORDER - i) == 0 ? 1.0 : ((ORDER - i) == 1 ? 1.0 : ((ORDER - i) == 2 ? 2.0 : 6.0))));
+    double f = coef * xpow[ORDER - i] * ypow[i];
+    if (i & 1)
+      sum_Fx += f;
+    else
+      sum_Fy += f;
+  }
+  if (skew) {
+    double t = sum_Fx;
+    sum_Fx = sum_Fy;
+    sum_Fy = t;
+    sum_Fx = -sum_Fx;
+  }
+  qx -= K * sum_Fy;
+  qy += K * sum_Fx;
+}
+
+__device__ __forceinline__ void eh_mkick_n(int order, double &qx, double &qy, const double *xpow, const double *ypow,
+                                           double K, int skew) {
+  switch (order) {
+  case 0:
+    eh_mkick<0>(qx, qy, xpow, ypow, K, skew);
+    break;
+  case 1:
+    eh_mkick<1>(qx, qy, xpow, ypow, K, skew);
+    break;
+  case 2:
+    eh_mkick<2>(qx, qy, xpow, ypow, K, skew);
+    break;
+  default:
+    eh_mkick<3>(qx, qy, xpow, ypow, K, skew);
+    break;
+  }
+}
+
+/* convertMomentaToSlopes, src/multipole.h:466-486 */
+__device__ __forceinline__ bool eh_m2s(double &xp, double &yp, double qx, double qy, double dp, int expand) {
+  if (expand) {
+    xp = qx / (1 + dp);
+    yp = qy / (1 + dp);
+    return true;
+  }
+  double denom = eh_sqr(1 + dp) - eh_sqr(qx) - eh_sqr(qy);
+  if (denom <= 0)
+    return false;
+  denom = eh_sqrt(denom);
+  xp = qx / denom;
+  yp = qy / denom;
+  return true;
+}
+
+/* KQUAD/KSEXT/KOCT: multipole_tracking2 (src/multipole.c:660-1084) for one particle,
+ * including the misalignment passes around the particle loop (method 0,
+ * src/malign_mat.c:257-267, :580-590, tilt_matrices.c:156-187) and the loss
+ * assignments of :1032-1047.  integrate_kick_multipole_ordn, src/multipole.c:1099-1456.
+ * Returns true if lost. */
+__device__ __forceinline__ bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
+  /* face 1 */
+  if (d.has_offset) {
+    p.s += d.mdz * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+    p.x = p.x - d.mdx + d.mdz * p.xp;
+    p.y = p.y - d.mdy + d.mdz * p.yp;
+  }
+  if (d.has_tilt) {
+    double x = p.x, xp = p.xp, y = p.y, yp = p.yp;
+    p.x = x * d.ncos + y * d.nsin;
+    p.y = -x * d.nsin + y * d.ncos;
+    p.xp = xp * d.ncos + yp * d.nsin;
+    p.yp = -xp * d.nsin + yp * d.ncos;
+  }
+  const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
+  double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, dp = p.dp, qx, qy;
+  double dzLoss = 0;
+  bool lost = false;
+  int how = 0; /* 0: coord untouched, 1: x,y written, 2: all written */
+  if (eh_isnan(x) || eh_isnan(xp) || eh_isnan(y) || eh_isnan(yp) || eh_fabs(x) > 10.0 || eh_fabs(y) > 10.0 ||
+      eh_fabs(xp) > 1.0 || eh_fabs(yp) > 1.0) {
+    lost = true;
+  } else {
+    /* convertSlopesToMomenta, src/multipole.h:448-464 */
+    if (d.expand) {
+      qx = (1 + dp) * xp;
+      qy = (1 + dp) * yp;
+    } else {
+      double denom = eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
+      qx = (1 + dp) * xp / denom;
+      qy = (1 + dp) * yp / denom;
+    }
+    if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
+      lost = true;
+    } else {
+      double xpow[4], ypow[4];
+      for (int ik = 0; ik < d.nslices && !lost; ik++) {
+        if (!eh_ap_ok(x + d.mdx, y + d.mdy, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1)) {
+          lost = true;
+          how = 1;
+          break;
+        }
+        for (int step = 0; step < d.nsub; step++) {
+          if (d.mdrift != 0) {
+            double dsh = d.dsh[step];
+            x += xp * dsh;
+            y += yp * dsh;
+            if (d.expand)
+              s += dsh * (1 + (eh_sqr(xp) + eh_sqr(yp)) / 2);
+            else
+              s += dsh * eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
+            dzLoss += dsh;
+          }
+          if (step == d.nsub - 1)
+            break;
+          xpow[0] = 1;
+          ypow[0] = 1;
+          xpow[1] = xpow[0] * x;
+          ypow[1] = ypow[0] * y;
+          xpow[2] = xpow[1] * x;
+          ypow[2] = ypow[1] * y;
+          xpow[3] = xpow[2] * x;
+          ypow[3] = ypow[2] * y;
+          for (int io = 0; io < 3; io++)
+            if (d.mp_order[io])
+              eh_mkick_n(d.mp_order[io], qx, qy, xpow, ypow, d.kick[step][io], d.mp_skew[io]);
+          if (d.has_xkick)
+            eh_mkick<0>(qx, qy, xpow, ypow, d.xk[step], 0);
+          if (d.has_ykick)
+            eh_mkick<0>(qx, qy, xpow, ypow, d.yk[step], 1);
+          if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
+            lost = true;
+            break;
+          }
+        }
+      }
+      if (!lost) {
+        if (!eh_ap_ok(x + d.mdx, y + d.mdy, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1)) {
+          lost = true;
+          how = 1;
+        } else if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
+          lost = true;
+        } else {
+          how = 2;
+          if (eh_isnan(x) || eh_isnan(xp) || eh_isnan(y) || eh_isnan(yp) || eh_fabs(x) > 10.0 ||
+              eh_fabs(y) > 10.0 || eh_fabs(xp) > 1.0 || eh_fabs(yp) > 1.0)
+            lost = true;
+        }
+      }
+    }
+  }
+  if (how == 1) {
+    p.x = x;
+    p.y = y;
+  } else if (how == 2) {
+    p.x = x;
+    p.xp = xp;
+    p.y = y;
+    p.yp = yp;
+    p.s += s;
+    p.dp = dp;
+  }
+  if (lost) {
+    p.s = d.z_start + dzLoss;
+    p.dp = d.Po * (1 + p.dp);
+  }
+  /* face 2: applied to every row, lost ones included */
+  if (d.has_offset || d.has_tilt) {
+    if (d.has_tilt) {
+      double x0 = p.x, xp0 = p.xp, y0 = p.y, yp0 = p.yp;
+      p.x = x0 * d.xcos + y0 * d.xsin;
+      p.y = -x0 * d.xsin + y0 * d.xcos;
+      p.xp = xp0 * d.xcos + yp0 * d.xsin;
+      p.yp = -xp0 * d.xsin + yp0 * d.xcos;
+    }
+    if (d.has_offset) {
+      p.s += (-d.mdz) * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+      p.x = p.x - (-d.mdx) + (-d.mdz) * p.xp;
+      p.y = p.y - (-d.mdy) + (-d.mdz) * p.yp;
+    }
+  }
+  return lost;
+}
+
+/* ------------------------------------------------------------------------- */
+/* computeCSBENDFields, src/csbend.c:99-172, over the non-zero terms in the
+ * reference's summation order (i outer, j inner) */
+__device__ __forceinline__ void eh_csb_fields(const eh_desc &d, double x, double y, double &Fx, double &Fy) {
+  if (!d.csb_field) {
+    Fx = 0;
+    Fy = 1;
+    return;
+  }
+  double sumFx = 0, sumFy = 0;
+  double xt = 1;
+  int ic = 0;
+  for (int t = 0; t < d.csb_nfx; t++) {
+    int ti = d.fx[t].i, tj = d.fx[t].j;
+    while (ic < ti) {
+      xt *= x;
+      ic++;
+    }
+    double yj = 1;
+    for (int k = 0; k < tj; k++)
+      yj = yj * y;
+    sumFx += d.fx[t].coef * xt * yj;
+  }
+  xt = 1;
+  ic = 0;
+  for (int t = 0; t < d.csb_nfy; t++) {
+    int ti = d.fy[t].i, tj = d.fy[t].j;
+    while (ic < ti) {
+      xt *= x;
+      ic++;
+    }
+    double yj = 1;
+    for (int k = 0; k < tj; k++)
+      yj = yj * y;
+    sumFy += d.fy[t].coef * xt * yj;
+  }
+  Fx = sumFx;
+  Fy = sumFy;
+}
+
+/* CSBEND: track_through_csbend (src/csbend.c:1088-1403) for one particle, with
+ * integrate_csbend_ordn (src/csbend.c:1459-1720) and the canonical conversions
+ * (:1436-1454).  Returns true if lost. */
+__device__ __forceinline__ bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi) {
+  const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
+  const double rho0 = d.rho0, rho_actual = d.rho_actual;
+  double x, xp, y, yp, s, dp;
+  p.s += d.dzi * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+  p.x = p.x + d.dxi + d.dzi * p.xp;
+  p.y = p.y + d.dyi + d.dzi * p.yp;
+  x = p.x * d.cos_tt + p.y * d.sin_tt;
+  y = -p.x * d.sin_tt + p.y * d.cos_tt;
+  xp = p.xp * d.cos_tt + p.yp * d.sin_tt;
+  yp = -p.xp * d.sin_tt + p.yp * d.cos_tt;
+  s = p.s;
+  dp = p.dp;
+  if (d.csb_edge1) {
+    double rho = (1 + dp) * rho_actual;
+    double delta_xp = d.tan_e1 / rho * x;
+    xp += delta_xp;
+    yp -= eh_tan(d.e1 - d.psi1 / (1 + dp)) / rho * y;
+  }
+  /* convertToDipoleCanonicalCoordinates */
+  double X = x, QX, Y = y, QY, DP = dp, S4 = 0;
+  {
+    double f = (1 + DP) / eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
+    QX = xp * f;
+    QY = yp * f;
+  }
+  /* integrate_csbend_ordn */
+  double dist = 0, dz_lost = 0;
+  bool lost = false;
+  for (int i = 0; i < d.csb_nslices && !lost; i++) {
+    if (!eh_ap_ok(X, Y, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1)) {
+      lost = true;
+      break;
+    }
+    for (int j = 0; j < d.nsub; j++) {
+      double f = eh_sqr(1 + DP) - eh_sqr(QY);
+      if (f <= 0) {
+        lost = true;
+        break;
+      }
+      f = eh_sqrt(f);
+      if (eh_fabs(QX / f) > 1) {
+        lost = true;
+        break;
+      }
+      double sin_phi = QX / f;
+      double phi = eh_asin(sin_phi);
+      double sine, cosi;
+      eh_sincos(d.csb_dshr[j] + phi, &sine, &cosi);
+      if (cosi == 0) {
+        lost = true;
+        break;
+      }
+      double tang = sine / cosi;
+      double cos_phi = eh_cos(phi);
+      QX = f * sine;
+      double factor = (rho0 + X) * cos_phi / f * (tang - sin_phi / cos_phi);
+      Y += QY * factor;
+      dist += factor * (1 + DP);
+      dz_lost += d.csb_dsh[j];
+      f = cos_phi / cosi;
+      X = rho0 * (f - 1) + f * X;
+      if (j == d.nsub - 1)
+        break;
+      double ds = d.csb_ds[j];
+      double Fx, Fy;
+      eh_csb_fields(d, X, Y, Fx, Fy);
+      QX += -ds * (1 + X / rho0) * Fy / rho_actual;
+      QY += ds * (1 + X / rho0) * Fx / rho_actual;
+    }
+  }
+  if (!lost) {
+    dz_lost = d.csb_dzl;
+    if (!eh_ap_ok(X, Y, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1))
+      lost = true;
+    else
+      S4 += dist;
+  }
+  /* convertFromDipoleCanonicalCoordinates */
+  {
+    double f = 1 / eh_sqrt(eh_sqr(1 + DP) - eh_sqr(QX) - eh_sqr(QY));
+    QX *= f;
+    QY *= f;
+  }
+  if (lost) {
+    /* memcpy(part, Qf) then convertFromCSBendCoords (src/csbend.c:4527-4550) */
+    p.x = X * d.cos_tt - Y * d.sin_tt;
+    p.y = X * d.sin_tt + Y * d.cos_tt;
+    p.xp = QX * d.cos_tt - QY * d.sin_tt;
+    p.yp = QX * d.sin_tt + QY * d.cos_tt;
+    p.s = d.z_start + dz_lost;
+    p.dp = d.Po * (1 + DP);
+    return true;
+  }
+  s += S4;
+  x = X;
+  xp = QX;
+  y = Y;
+  yp = QY;
+  dp = DP;
+  if (d.csb_edge2) {
+    double rho = (1 + dp) * rho_actual;
+    double delta_xp = d.tan_e2 / rho * x;
+    xp += delta_xp;
+    yp -= eh_tan(d.e2 - d.psi2 / (1 + dp)) / rho * y;
+  }
+  p.x = x * d.cos_tt - y * d.sin_tt + 0.0;
+  p.y = x * d.sin_tt + y * d.cos_tt + 0.0;
+  p.xp = xp * d.cos_tt - yp * d.sin_tt + 0.0;
+  p.yp = xp * d.sin_tt + yp * d.cos_tt + 0.0;
+  p.s = s + 0.0;
+  p.dp = dp;
+  p.x += d.dxf + d.dzf * p.xp;
+  p.y += d.dyf + d.dzf * p.yp;
+  p.s += d.dzf * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+  return false;
+}
+
+/* ------------------------------------------------------------------------- */
+/* RFCA for one particle: trackRfCavityWithWakes (src/simple_rfca.c:295-985,
+ * matrix method :824-967, kick method :567-660, add_to_particle_energy
+ * :987-1006) followed by removeInvalidParticles (src/limit_amplitudes.c:289-330).
+ * Returns true if lost. */
+__device__ __forceinline__ bool eh_rfca(eh_p6 &p, const eh_desc &d) {
+  const double c_mks = 2.99792458e8;
+  const double Pc = d.Po;
+  if (d.rf_zero_volt) {
+    double L = d.rf_flen;
+    if (L) {
+      p.x += p.xp * L;
+      p.y += p.yp * L;
+      p.s += L * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+    }
+    return false;
+  }
+  const double length = d.rf_len, volt = d.rf_volt, omega = d.rf_omega, phase = d.rf_phase;
+  p.x -= d.rf_dx;
+  p.y -= d.rf_dy;
+  if (!d.rf_matrix) {
+    double inverseF = 0;
+    for (int ik = 0; ik < d.rf_nkicks; ik++) {
+      if (!(p.dp == -1)) {
+        double dc4 = length ? length / 2 * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp)) : 0;
+        double P = Pc * (1 + p.dp);
+        double gamma = eh_sqrt(eh_sqr(P) + 1);
+        double beta_i = P / gamma;
+        double t = (p.s + dc4) / (c_mks * beta_i) - 0.0;
+        double dgamma = d.rf_dgamma_same;
+        if (!d.rf_same) {
+          if (d.rf_standing)
+            dgamma = volt * eh_sin(omega * (t - 0.0) + phase) * 1.0;
+          else {
+            double phi = omega * (t - 0.0 - ik * d.rf_dtLight) + phase;
+            dgamma = volt * eh_sin(phi) * 1.0;
+          }
+        }
+        if (length) {
+          if (d.rf_end1 && ik == 0) {
+            inverseF = dgamma / (2 * gamma * length);
+            p.xp -= p.x * inverseF;
+            p.yp -= p.y * inverseF;
+          }
+          p.x += p.xp * length / 2;
+          p.y += p.yp * length / 2;
+          p.s += length / 2 * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+        }
+        {
+          /* add_to_particle_energy */
+          double P0 = Pc * (1 + p.dp);
+          double gam = eh_sqrt(P0 * P0 + 1);
+          double gamma1 = gam + dgamma;
+          if (gamma1 <= 1)
+            gamma1 = 1 + 1e-7;
+          double P1 = eh_sqrt(gamma1 * gamma1 - 1);
+          p.dp = (P1 - Pc) / Pc;
+          p.s = t * c_mks * P1 / gamma1;
+          double Pz = P0 / eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+          double Pz1 = eh_sqrt(Pz * Pz + gamma1 * gamma1 - gam * gam);
+          double PRatio = Pz / Pz1;
+          p.xp *= PRatio;
+          p.yp *= PRatio;
+        }
+        double g1 = gamma + dgamma;
+        if (g1 <= 1)
+          p.dp = -1;
+        else
+          inverseF = -dgamma / (2 * g1 * length);
+      }
+      if (length) {
+        p.x += p.xp * length / 2;
+        p.y += p.yp * length / 2;
+        p.s += length / 2 * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+        if (d.rf_end2 && (ik == d.rf_nkicks - 1)) {
+          p.xp -= p.x * inverseF;
+          p.yp -= p.y * inverseF;
+        }
+      }
+    }
+  } else {
+    const double R11 = 1, R21 = 0;
+    double P = Pc * (1 + p.dp);
+    double gamma = eh_sqrt(eh_sqr(P) + 1);
+    double beta_i = P / gamma;
+    double ds1 = length / 2 * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+    double t = (p.s + ds1) / (c_mks * beta_i) - 0.0;
+    double dgamma = d.rf_dgamma_same;
+    if (!d.rf_same) {
+      double sin_phase = eh_sin(omega * (t - 0.0 - 0 * d.rf_dtLight) + phase);
+      dgamma = (volt * 1.0) * sin_phase;
+    }
+    if (d.rf_end1 && length) {
+      double inverseF = dgamma / (2 * gamma * length);
+      p.xp -= p.x * inverseF;
+      p.yp -= p.y * inverseF;
+    }
+    double dP = eh_sqrt(eh_sqr(gamma + dgamma) - 1) - P;
+    double R22 = 1 / (1 + dP / P);
+    double R12;
+    if (eh_fabs(dP / P) > 1e-14)
+      R12 = length * (P / dP * eh_log(1 + dP / P));
+    else
+      R12 = length;
+    p.s += ds1;
+    double x = p.x, xp = p.xp;
+    p.x = x * R11 + xp * R12;
+    p.xp = x * R21 + xp * R22;
+    x = p.y;
+    xp = p.yp;
+    p.y = x * R11 + xp * R12;
+    p.yp = x * R21 + xp * R22;
+    p.s += length / 2 * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+    p.dp = (P + dP - Pc) / Pc;
+    if ((gamma += dgamma) <= 1)
+      p.dp = -1;
+    if (d.rf_end2 && length) {
+      double inverseF = -dgamma / (2 * gamma * length);
+      p.xp -= p.x * inverseF;
+      p.yp -= p.y * inverseF;
+    }
+    p.s = (P + dP) / gamma * p.s / beta_i;
+  }
+  p.x += d.rf_dx;
+  p.y += d.rf_dy;
+  /* removeInvalidParticles */
+  if (eh_isnan(p.x) || eh_isnan(p.xp) || eh_isnan(p.y) || eh_isnan(p.yp) || eh_isnan(p.s) || eh_isnan(p.dp) ||
+      p.dp <= -1) {
+    p.s = d.z_end;
+    p.dp = Pc * (1 + p.dp);
+    return true;
+  }
+  return false;
+}
+
+/* one element followed by do_tracking's post-element sweeps
+ * (src/do_tracking.c:2506-2524, :2675-2684).  Returns -1 if the particle
+ * survives, else the loss stage. */
+__device__ __forceinline__ int eh_apply_element(eh_p6 &p, const eh_desc &d, int pi) {
+  bool lost = false;
+  switch (d.type) {
+  case 1: /* DRIF */
+    eh_drif(p, d.length, d.T11);
+    break;
+  case 2: /* EDRIFT */
+    eh_edrift(p, d.length);
+    break;
+  case 3:
+  case 4:
+  case 5:
+    lost = eh_multipole(p, d, pi);
+    break;
+  case 6:
+    lost = eh_csbend(p, d, pi);
+    break;
+  case 7:
+    lost = eh_rfca(p, d);
+    break;
+  default: /* MARK, MAXAMP */
+    break;
+  }
+  if (lost)
+    return EH_STAGE_ELEMENT;
+  if (d.type == 0)
+    return -1; /* MARK: NO_APERTURE, and nothing can be NaN-lost here that survived */
+  if (d.post_ap[pi] && eh_limit_one(p, d.post_xmax[pi], d.post_ymax[pi], d.z_end, d.Po, d.extrap))
+    return EH_STAGE_MAXAMP;
+  if (eh_limit_one(p, EH_DBL_MAX, EH_DBL_MAX, d.z_end, d.Po, 0))
+    return EH_STAGE_NAN;
+  return -1;
+}
+
+#endif

@@ -1,0 +1,1593 @@
+/* eh_lib.hip -- HIP kernels and C ABI of the gfx950 element-tracking library.
+ *
+ * Residency: particles stay on the GPU between calls as structure-of-arrays
+ * (x, x', y, y', s, delta as separate double arrays, plus the particle's initial
+ * row and its loss record).  Survivors live in one of two "alive" buffers;
+ * lost particles are appended to a graveyard.  Compaction moves survivors with
+ * a wavefront ballot + mbcnt prefix and one atomic per wavefront, so the device
+ * order is not the reference's order.  The reference's row order (swap-to-top
+ * compaction in every element, src/multipole.c:1032-1047,
+ * src/limit_amplitudes.c:262-283) is reconstructed exactly on the host when it
+ * is needed -- for downloads and for the RFCA TMEAN fiducial, whose Kahan sum
+ * runs in row order -- by replaying the recorded loss events.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "elegant_hip.h"
+#include "eh_csbend_terms.h"
+#include "eh_device.h"
+
+/* ------------------------------------------------------------------------- */
+/* errors */
+static char g_err[512];
+extern "C" const char *eh_last_error(void) { return g_err; }
+static long fail(long code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+#define HIPCHK(call)                                                                                                  \
+  do {                                                                                                                \
+    hipError_t e_ = (call);                                                                                           \
+    if (e_ != hipSuccess)                                                                                             \
+      return fail(EH_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));                                        \
+  } while (0)
+
+static const double EH_PI = 3.14159265358979323846;
+
+/* ------------------------------------------------------------------------- */
+/* integrator tables, src/multipole.c:1087-1141 and src/csbend.c:1478-1500 */
+#define EH_BETA 1.25992104989487316477
+static const double driftFrac2[2] = {0.5, 0.5};
+static const double kickFrac2[2] = {1.0, 0.0};
+static const double driftFrac4[4] = {0.5 / (2 - EH_BETA), (1 - EH_BETA) / (2 - EH_BETA) / 2,
+                                     (1 - EH_BETA) / (2 - EH_BETA) / 2, 0.5 / (2 - EH_BETA)};
+static const double kickFrac4[4] = {1. / (2 - EH_BETA), -EH_BETA / (2 - EH_BETA), 1 / (2 - EH_BETA), 0};
+static const double driftFrac6[8] = {0.39225680523878,   0.5100434119184585, -0.47105338540975655,
+                                     0.0687531682525181, 0.0687531682525181, -0.47105338540975655,
+                                     0.5100434119184585, 0.39225680523878};
+static const double kickFrac6[8] = {0.784513610477560,  0.235573213359357, -1.17767998417887, 1.3151863206839063,
+                                    -1.17767998417887, 0.235573213359357, 0.784513610477560,  0};
+
+static int integ_tables(long order, const double **drift, const double **kick, int *nsub) {
+  switch (order) {
+  case 2:
+    *drift = driftFrac2;
+    *kick = kickFrac2;
+    *nsub = 2;
+    return 1;
+  case 4:
+    *drift = driftFrac4;
+    *kick = kickFrac4;
+    *nsub = 4;
+    return 1;
+  case 6:
+    *drift = driftFrac6;
+    *kick = kickFrac6;
+    *nsub = 8;
+    return 1;
+  }
+  return 0;
+}
+
+/* cos/sin of a rotation angle with the exact special angles of
+ * rotateBeamCoordinatesForMisalignment, src/tilt_matrices.c:156-175 */
+static void rot_cs(double angle, double *c, double *s) {
+  if (std::fabs(std::fabs(angle) - EH_PI) < 1e-12) {
+    *c = -1;
+    *s = 0;
+  } else if (std::fabs(angle - EH_PI / 2) < 1e-12) {
+    *c = 0;
+    *s = 1;
+  } else if (std::fabs(angle + EH_PI / 2) < 1e-12) {
+    *c = 0;
+    *s = -1;
+  } else {
+    *c = eh_cos(angle);
+    *s = eh_sin(angle);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* CSBEND field table: computeCSBENDFieldCoefficients (src/csbend.c:175-652),
+ * evaluated term by term from the generated Maxwell-recursion table */
+extern "C" long eh_csbend_field_table(const double *b, const double *c, double h1, int nonlinear,
+                                      int expansionOrder, double *Fx, double *Fy, int *hasNormal, int *hasSkew) {
+  long i;
+  double h[20];
+  if (expansionOrder == 0) {
+    for (i = 8; i >= 0; i--)
+      if (b[i] || c[i])
+        break;
+    if ((expansionOrder = (int)(i + 2)) < 4)
+      expansionOrder = 4;
+  }
+  long eo1 = expansionOrder + 1;
+  if (eo1 > 11)
+    return fail(EH_ERR_UNSUPPORTED, "expansion order >10 for CSBEND");
+  int hn = 0, hs = 0;
+  for (i = 0; i < 9; i++) {
+    if (b[i])
+      hn = 1;
+    if (c[i])
+      hs = 1;
+  }
+  if (hasNormal)
+    *hasNormal = hn;
+  if (hasSkew)
+    *hasSkew = hs;
+  std::memset(Fx, 0, sizeof(double) * 121);
+  std::memset(Fy, 0, sizeof(double) * 121);
+  h[0] = 1;
+  for (i = 1; i < 20; i++)
+    h[i] = h[i - 1] * h1;
+  Fx[0] = c[0];
+  Fy[0] = 1 - b[0];
+  int prev_w = -1, prev_i = -1, prev_j = -1;
+  for (long t = 0; t < EH_CSBEND_NTERMS; t++) {
+    int w = eh_csbend_terms[t].which, ii = eh_csbend_terms[t].i, jj = eh_csbend_terms[t].j;
+    bool first = !(w == prev_w && ii == prev_i && jj == prev_j);
+    prev_w = w;
+    prev_i = ii;
+    prev_j = jj;
+    if (!nonlinear && ii + jj > 1)
+      continue;
+    const double *v = eh_csbend_terms[t].kind ? c + 1 : b + 1;
+    double term = ((eh_csbend_terms[t].num * h[eh_csbend_terms[t].hpow]) * v[eh_csbend_terms[t].m]) /
+                  eh_csbend_terms[t].den;
+    double *dst = (w ? Fy : Fx) + ii * 11 + jj;
+    if (first)
+      *dst = term;
+    else
+      *dst += term;
+  }
+  return eo1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* descriptor construction.  ap[pi] is the MAXAMP passed to the element routine
+ * and post[pi] the limits of the sweep that follows it, for pass 0 (pi = 0) and
+ * later passes (pi = 1). */
+struct eh_ctx {
+  double Po, z_start, z_end;
+  const EH_MAXAMP *ap[2];
+  double post_x[2], post_y[2];
+  int post_on[2];
+};
+
+static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c) {
+  double KnL[3] = {0, 0, 0};
+  long order[3] = {0, 0, 0};
+  int skew[3] = {0, 0, 0};
+  double dx = 0, dy = 0, dz = 0, tilt = 0, xkick = 0, ykick = 0, drift = 0;
+  long nSlices = 0, n_kicks = 0, integ_order = 0;
+  int expand = 0;
+  switch (e->type) {
+  case EH_T_KQUAD: {
+    const EH_KQUAD *q = (const EH_KQUAD *)e->p_elem;
+    nSlices = q->nSlices;
+    n_kicks = q->n_kicks;
+    expand = q->expandHamiltonian;
+    order[0] = 1;
+    KnL[0] = q->k1 * q->length * (1 + q->fse);
+    drift = q->length;
+    tilt = q->tilt;
+    dx = q->dx;
+    dy = q->dy;
+    dz = q->dz;
+    xkick = q->xkick * q->xKickCalibration;
+    ykick = q->ykick * q->yKickCalibration;
+    integ_order = q->integration_order;
+    break;
+  }
+  case EH_T_KSEXT: {
+    const EH_KSEXT *q = (const EH_KSEXT *)e->p_elem;
+    nSlices = q->nSlices;
+    n_kicks = q->n_kicks;
+    expand = q->expandHamiltonian;
+    order[0] = 2;
+    KnL[0] = q->k2 * q->length * (1 + q->fse);
+    drift = q->length;
+    tilt = q->tilt;
+    dx = q->dx;
+    dy = q->dy;
+    dz = q->dz;
+    xkick = q->xkick * q->xKickCalibration;
+    ykick = q->ykick * q->yKickCalibration;
+    integ_order = q->integration_order;
+    if (q->k1) {
+      KnL[1] = q->k1 * q->length;
+      order[1] = 1;
+    }
+    if (q->j1) {
+      KnL[2] = q->j1 * q->length;
+      order[2] = 1;
+      skew[2] = 1;
+    }
+    break;
+  }
+  case EH_T_KOCT: {
+    const EH_KOCT *q = (const EH_KOCT *)e->p_elem;
+    nSlices = q->nSlices;
+    n_kicks = q->n_kicks;
+    expand = q->expandHamiltonian;
+    order[0] = 3;
+    KnL[0] = q->k3 * q->length * (1 + q->fse);
+    drift = q->length;
+    tilt = q->tilt;
+    dx = q->dx;
+    dy = q->dy;
+    dz = q->dz;
+    integ_order = q->integration_order;
+    break;
+  }
+  default:
+    return fail(EH_ERR_UNSUPPORTED, "multipole_tracking2: element type not supported");
+  }
+  const double *driftFrac, *kickFrac;
+  int nsub;
+  if (!integ_tables(integ_order, &driftFrac, &kickFrac, &nsub))
+    return fail(EH_ERR_UNSUPPORTED, "multipole integration_order must be 2, 4, or 6");
+  if (n_kicks <= 0) {
+    if (nSlices <= 0)
+      return fail(EH_ERR_UNSUPPORTED, "N_KICKS<=0 and N_SLICES<=0 in multipole tracking");
+  } else {
+    if (integ_order > 2) {
+      if ((nSlices = (long)std::ceil(n_kicks / (1.0 * integ_order))) < 1)
+        nSlices = 1;
+    } else
+      nSlices = n_kicks;
+  }
+  d->nslices = (int32_t)nSlices;
+  d->nsub = nsub;
+  d->expand = expand;
+  double mdrift = drift / nSlices;
+  d->mdrift = mdrift;
+  double xk = xkick / nSlices, yk = ykick / nSlices;
+  d->has_xkick = xk != 0;
+  d->has_ykick = yk != 0;
+  double KnLp[3];
+  for (int i = 0; i < 3; i++) {
+    KnLp[i] = KnL[i] / nSlices;
+    d->mp_order[i] = KnL[i] ? (int32_t)order[i] : 0;
+    d->mp_skew[i] = skew[i];
+  }
+  for (int s = 0; s < nsub; s++) {
+    d->dsh[s] = mdrift * driftFrac[s];
+    for (int i = 0; i < 3; i++)
+      d->kick[s][i] = KnLp[i] * kickFrac[s];
+    d->xk[s] = -xk * kickFrac[s];
+    d->yk[s] = -yk * kickFrac[s];
+  }
+  d->mdx = dx;
+  d->mdy = dy;
+  d->mdz = dz;
+  d->has_offset = (dx || dy || dz);
+  d->has_tilt = tilt != 0;
+  rot_cs(tilt + 0.0, &d->ncos, &d->nsin);
+  rot_cs(-(tilt + 0.0), &d->xcos, &d->xsin);
+  double rt = -tilt;
+  d->ap_rtilt = rt != 0;
+  d->ap_cs0 = eh_cos(rt);
+  d->ap_cs1 = eh_sin(rt);
+  d->length = drift;
+  return EH_OK;
+}
+
+static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c) {
+  if (cb->angle == 0) {
+    d->type = EH_T_EDRIFT;
+    d->length = cb->length;
+    return EH_OK;
+  }
+  const double *driftFrac, *kickFrac;
+  int nsub;
+  if (!integ_tables(cb->integration_order, &driftFrac, &kickFrac, &nsub))
+    return fail(EH_ERR_UNSUPPORTED, "CSBEND integration_order is invalid");
+  if (cb->malignMethod != 0 || cb->etilt != 0 || cb->expandHamiltonian || cb->edge_order > 1)
+    return fail(EH_ERR_UNSUPPORTED, "CSBEND option not supported");
+  if ((cb->edge1_effects != 0 && cb->edge1_effects != 1) || (cb->edge2_effects != 0 && cb->edge2_effects != 1))
+    return fail(EH_ERR_UNSUPPORTED, "CSBEND edge method not supported");
+  if (cb->nSlices < 1)
+    return fail(EH_ERR_UNSUPPORTED, "invalid number of steps (integrate_csbend_ordn)");
+  double b[9], cc[9];
+  double rho0 = cb->length / cb->angle;
+  b[0] = 0;
+  if (cb->use_bn) {
+    b[1] = cb->b1;
+    b[2] = cb->b2;
+    b[3] = cb->b3;
+    b[4] = cb->b4;
+    b[5] = cb->b5;
+    b[6] = cb->b6;
+    b[7] = cb->b7;
+    b[8] = cb->b8;
+  } else {
+    b[1] = cb->k1 * rho0;
+    b[2] = cb->k2 * rho0;
+    b[3] = cb->k3 * rho0;
+    b[4] = cb->k4 * rho0;
+    b[5] = cb->k5 * rho0;
+    b[6] = cb->k6 * rho0;
+    b[7] = cb->k7 * rho0;
+    b[8] = cb->k8 * rho0;
+  }
+  for (int j = 0; j < 9; j++)
+    cc[j] = 0;
+  b[1] *= (1 + cb->fse + cb->fseQuadrupole) / (1 + cb->fse + cb->fseDipole);
+  cc[1] *= (1 + cb->fse + cb->fseQuadrupole) / (1 + cb->fse + cb->fseDipole);
+  b[2] *= (1 + cb->fse) / (1 + cb->fse + cb->fseDipole);
+  cc[2] *= (1 + cb->fse) / (1 + cb->fse + cb->fseDipole);
+  b[0] = cb->xKick / cb->angle;
+  cc[0] = cb->yKick / cb->angle;
+  double angle, e1, e2, tilt, etilt;
+  if (cb->angle < 0) {
+    angle = -cb->angle;
+    e1 = -cb->e1;
+    e2 = -cb->e2;
+    etilt = cb->etilt * cb->etiltSign;
+    tilt = cb->tilt + EH_PI;
+    rho0 = cb->length / angle;
+    for (int j = 1; j < 9; j += 2) {
+      b[j] *= -1;
+      cc[j] *= -1;
+    }
+  } else {
+    angle = cb->angle;
+    e1 = cb->e1;
+    e2 = cb->e2;
+    etilt = cb->etilt * cb->etiltSign;
+    tilt = cb->tilt;
+    rho0 = cb->length / angle;
+  }
+  (void)angle;
+  double rt = -tilt;
+  d->ap_rtilt = rt != 0;
+  d->ap_cs0 = eh_cos(rt);
+  d->ap_cs1 = eh_sin(rt);
+  if (std::fabs(rho0) > 1e6)
+    return fail(EH_ERR_UNSUPPORTED, "CSBEND with radius > 1e6 not supported");
+  double fse = cb->fse + cb->fseDipole;
+  double h = 1 / rho0;
+  if (std::fabs(fse + 1) < 1e-12)
+    fse = -1 + 1e-12;
+  double rho_actual = 1 / ((1 + fse) * h);
+  double sgn = rho0 < 0 ? -1.0 : (rho0 > 0 ? 1.0 : 0.0);
+  double Kg1 = 2 * cb->hgap * cb->fint * sgn;
+  double psi1 = Kg1 / std::fabs(rho_actual) / eh_cos(e1) * (1 + eh_sqr(eh_sin(e1)));
+  double Kg2 = 2 * cb->hgap * cb->fint * sgn;
+  double psi2 = Kg2 / std::fabs(rho_actual) / eh_cos(e2) * (1 + eh_sqr(eh_sin(e2)));
+  if (cb->length < 0) {
+    psi1 *= -1;
+    psi2 *= -1;
+  }
+  double Fx[121], Fy[121];
+  int hn, hs;
+  long eo1 = eh_csbend_field_table(b, cc, h, cb->nonlinear, cb->expansionOrder, Fx, Fy, &hn, &hs);
+  if (eo1 < 0)
+    return eo1;
+  d->csb_field = hn || hs;
+  int nfx = 0, nfy = 0;
+  for (int i = 0; i < eo1; i++)
+    for (int j = 0; j < eo1 - i; j++) {
+      double fxv = Fx[i * 11 + j], fyv = Fy[i * 11 + j];
+      if (hs) {
+        if (fxv != 0)
+          d->fx[nfx++] = {i, j, fxv};
+        if (fyv != 0)
+          d->fy[nfy++] = {i, j, fyv};
+      } else if (j & 1) {
+        if (fxv != 0)
+          d->fx[nfx++] = {i, j, fxv};
+      } else {
+        if (fyv != 0)
+          d->fy[nfy++] = {i, j, fyv};
+      }
+    }
+  d->csb_nfx = nfx;
+  d->csb_nfy = nfy;
+  double ttilt = tilt + etilt, cos_tt, sin_tt;
+  if (ttilt == 0) {
+    cos_tt = 1;
+    sin_tt = 0;
+  } else if (std::fabs(std::fabs(ttilt) - EH_PI) < 1e-12) {
+    cos_tt = -1;
+    sin_tt = 0;
+  } else if (std::fabs(ttilt - EH_PI / 2) < 1e-12) {
+    cos_tt = 0;
+    sin_tt = 1;
+  } else if (std::fabs(ttilt + EH_PI / 2) < 1e-12) {
+    cos_tt = 0;
+    sin_tt = -1;
+  } else {
+    cos_tt = eh_cos(ttilt);
+    sin_tt = eh_sin(ttilt);
+  }
+  d->cos_tt = cos_tt;
+  d->sin_tt = sin_tt;
+  d->dxi = -cb->dx;
+  d->dzi = cb->dz;
+  d->dyi = -cb->dy;
+  d->dxf = cb->dx * eh_cos(cb->angle) + cb->dz * eh_sin(cb->angle);
+  d->dzf = cb->dx * eh_sin(cb->angle) - cb->dz * eh_cos(cb->angle);
+  d->dyf = cb->dy;
+  d->rho0 = rho0;
+  d->rho_actual = rho_actual;
+  d->csb_nslices = cb->nSlices;
+  d->nsub = nsub;
+  double s = cb->length / cb->nSlices;
+  for (int j = 0; j < nsub; j++) {
+    d->csb_dsh[j] = s * driftFrac[j];
+    d->csb_dshr[j] = d->csb_dsh[j] / rho0;
+    d->csb_ds[j] = s * kickFrac[j];
+  }
+  d->csb_dzl = cb->nSlices * s;
+  d->csb_edge1 = (cb->edgeFlags & EH_BEND_EDGE1_EFFECTS) && cb->edge1_effects == 1;
+  d->csb_edge2 = (cb->edgeFlags & EH_BEND_EDGE2_EFFECTS) && cb->edge2_effects == 1;
+  d->e1 = e1;
+  d->e2 = e2;
+  d->tan_e1 = eh_tan(e1);
+  d->tan_e2 = eh_tan(e2);
+  d->psi1 = psi1;
+  d->psi2 = psi2;
+  d->length = cb->length;
+  return EH_OK;
+}
+
+/* RFCA constants, src/simple_rfca.c:378-470, once the fiducial phase is known */
+static long build_rfca(eh_desc *d, const EH_RFCA *rf) {
+  if (rf->Q != 0 || rf->change_t || rf->change_p0 || rf->linearize || rf->lockPhase)
+    return fail(EH_ERR_UNSUPPORTED, "RFCA option not supported");
+  d->rf_flen = rf->length;
+  if (rf->volt == 0) {
+    d->rf_zero_volt = 1;
+    return EH_OK;
+  }
+  double length = rf->length;
+  double omega = 2 * EH_PI * rf->freq;
+  double volt = rf->volt / (1e6 * EH_ME_MEV * 1.0);
+  long nKicks = length ? rf->nKicks : 1;
+  int matrix = 0;
+  if (nKicks <= 0) {
+    matrix = 1;
+    nKicks = 1;
+  }
+  length /= nKicks;
+  volt /= nKicks;
+  d->rf_matrix = matrix;
+  d->rf_nkicks = (int32_t)nKicks;
+  d->rf_len = length;
+  d->rf_volt = volt;
+  d->rf_omega = omega;
+  d->rf_dtLight = length / EH_C_MKS;
+  double phase = rf->phase_fiducial;
+  d->rf_t0 = omega ? -rf->phase_fiducial / omega : 0;
+  phase += rf->phase * EH_PI / 180.0;
+  d->rf_phase = phase;
+  d->rf_same = (omega == 0);
+  d->rf_dgamma_same = d->rf_same ? volt * eh_sin(phase) : 0;
+  d->rf_standing = rf->standingWave;
+  d->rf_end1 = rf->end1Focus;
+  d->rf_end2 = rf->end2Focus;
+  d->rf_dx = rf->dx;
+  d->rf_dy = rf->dy;
+  return EH_OK;
+}
+
+static double elem_length(const EH_ELEMENT_LIST *e) {
+  if (e->type == EH_T_MARK || e->type == EH_T_MAXAMP)
+    return 0;
+  return *(const double *)e->p_elem;
+}
+
+static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c) {
+  std::memset(d, 0, sizeof *d);
+  d->type = e->type;
+  d->counted = !(e->type == EH_T_MARK || e->type == EH_T_MAXAMP);
+  d->Po = c.Po;
+  d->z_start = c.z_start;
+  d->z_end = c.z_end;
+  for (int pi = 0; pi < 2; pi++) {
+    if (c.ap[pi] && c.ap[pi]->elliptical)
+      return fail(EH_ERR_UNSUPPORTED, "elliptical MAXAMP not supported");
+    d->ap_xmax[pi] = c.ap[pi] ? c.ap[pi]->x_max : 0;
+    d->ap_ymax[pi] = c.ap[pi] ? c.ap[pi]->y_max : 0;
+    d->post_ap[pi] = c.post_on[pi];
+    d->post_xmax[pi] = c.post_x[pi];
+    d->post_ymax[pi] = c.post_y[pi];
+  }
+  d->extrap = e->type == EH_T_DRIF;
+  switch (e->type) {
+  case EH_T_MARK:
+  case EH_T_MAXAMP:
+    return EH_OK;
+  case EH_T_DRIF: {
+    const EH_DRIFT *dr = (const EH_DRIFT *)e->p_elem;
+    long order = dr->order ? dr->order : 2;
+    d->length = dr->length;
+    d->T11 = order > 1 ? dr->length / 2 : 0;
+    return EH_OK;
+  }
+  case EH_T_EDRIFT:
+    d->length = ((const EH_EDRIFT *)e->p_elem)->length;
+    return EH_OK;
+  case EH_T_KQUAD:
+  case EH_T_KSEXT:
+  case EH_T_KOCT:
+    return build_multipole(d, e, c);
+  case EH_T_CSBEND:
+    return build_csbend(d, (const EH_CSBEND *)e->p_elem, c);
+  case EH_T_RFCA:
+    d->length = ((const EH_RFCA *)e->p_elem)->length;
+    return build_rfca(d, (const EH_RFCA *)e->p_elem);
+  }
+  return fail(EH_ERR_UNSUPPORTED, "element type %d not supported", e->type);
+}
+
+/* ------------------------------------------------------------------------- */
+/* device storage */
+struct eh_soa {
+  double *x, *xp, *y, *yp, *s, *dp;
+  int32_t *row0;   /* row index at upload: the particle's identity */
+  int32_t *lpass;  /* pass of loss (lost particles) */
+  int32_t *levent; /* loss event number, -1 while alive */
+};
+
+struct eh_beam {
+  long n_total = 0, n_alive = 0, n_lost = 0;
+  eh_soa buf[2] = {}; /* alive particles, double-buffered for compaction */
+  int cur = 0;
+  eh_soa grave = {};  /* lost particles, append-only */
+  void *mem = nullptr;
+  unsigned long long *d_cnt = nullptr; /* [0] alive out, [1] graveyard append, [2] lost in place, [3] effort */
+  eh_desc *d_desc = nullptr;
+  long desc_cap = 0;
+  std::vector<double> cols; /* columns 6..9 of each uploaded row, by row0 */
+  long pass = 0;
+  int32_t event = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double kernel_ms = 0;
+  long launches = 0;
+};
+
+static int g_device = -1;
+
+extern "C" int eh_init(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return (int)fail(EH_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= n)
+    return (int)fail(EH_ERR_ARG, "device %d out of range (%d devices)", device, n);
+  if (hipSetDevice(device) != hipSuccess)
+    return (int)fail(EH_ERR_HIP, "hipSetDevice(%d) failed", device);
+  g_device = device;
+  return EH_OK;
+}
+
+static size_t soa_bytes(long n) { return (size_t)n * (6 * sizeof(double) + 3 * sizeof(int32_t)); }
+
+static void soa_carve(eh_soa *s, char *base, long n) {
+  double *d = (double *)base;
+  s->x = d;
+  s->xp = d + n;
+  s->y = d + 2 * n;
+  s->yp = d + 3 * n;
+  s->s = d + 4 * n;
+  s->dp = d + 5 * n;
+  int32_t *i = (int32_t *)(d + 6 * n);
+  s->row0 = i;
+  s->lpass = i + n;
+  s->levent = i + 2 * n;
+}
+
+extern "C" eh_beam *eh_beam_create(long n_capacity) {
+  if (g_device < 0 && eh_init(0) != EH_OK)
+    return nullptr;
+  if (n_capacity < 1)
+    n_capacity = 1;
+  eh_beam *b = new eh_beam();
+  size_t per = soa_bytes(n_capacity);
+  per = (per + 255) & ~(size_t)255;
+  if (hipMalloc(&b->mem, 3 * per) != hipSuccess) {
+    fail(EH_ERR_NOMEM, "hipMalloc of %zu bytes failed", 3 * per);
+    delete b;
+    return nullptr;
+  }
+  soa_carve(&b->buf[0], (char *)b->mem, n_capacity);
+  soa_carve(&b->buf[1], (char *)b->mem + per, n_capacity);
+  soa_carve(&b->grave, (char *)b->mem + 2 * per, n_capacity);
+  if (hipMalloc(&b->d_cnt, 8 * sizeof(unsigned long long)) != hipSuccess ||
+      hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+      hipEventCreate(&b->ev1) != hipSuccess) {
+    fail(EH_ERR_HIP, "beam setup failed");
+    (void)hipFree(b->mem);
+    delete b;
+    return nullptr;
+  }
+  b->n_total = n_capacity;
+  return b;
+}
+
+extern "C" void eh_beam_free(eh_beam *b) {
+  if (!b)
+    return;
+  if (b->stream)
+    (void)hipStreamSynchronize(b->stream);
+  (void)hipFree(b->mem);
+  (void)hipFree(b->d_cnt);
+  if (b->d_desc)
+    (void)hipFree(b->d_desc);
+  if (b->ev0)
+    (void)hipEventDestroy(b->ev0);
+  if (b->ev1)
+    (void)hipEventDestroy(b->ev1);
+  if (b->stream)
+    (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" long eh_beam_n_alive(const eh_beam *b) { return b ? b->n_alive : EH_ERR_ARG; }
+extern "C" long eh_beam_n_total(const eh_beam *b) { return b ? b->n_total : EH_ERR_ARG; }
+extern "C" void eh_beam_set_pass(eh_beam *b, long pass) {
+  if (b)
+    b->pass = pass;
+}
+
+extern "C" long eh_beam_upload(eh_beam *b, double **coord, long n_part) {
+  if (!b || !coord || n_part < 0 || n_part > b->n_total)
+    return fail(EH_ERR_ARG, "eh_beam_upload: bad arguments (n_part=%ld, capacity=%ld)", n_part,
+                b ? b->n_total : 0L);
+  std::vector<double> h(6 * (size_t)n_part);
+  std::vector<int32_t> r(n_part), m1(n_part, -1), z(n_part, 0);
+  b->cols.assign(4 * (size_t)n_part, 0.0);
+  for (long i = 0; i < n_part; i++) {
+    for (int k = 0; k < 6; k++)
+      h[k * (size_t)n_part + i] = coord[i][k];
+    for (int k = 0; k < 4; k++)
+      b->cols[4 * (size_t)i + k] = coord[i][6 + k];
+    r[i] = (int32_t)i;
+  }
+  eh_soa &a = b->buf[0];
+  b->cur = 0;
+  double *dst[6] = {a.x, a.xp, a.y, a.yp, a.s, a.dp};
+  for (int k = 0; k < 6; k++)
+    HIPCHK(hipMemcpy(dst[k], h.data() + k * (size_t)n_part, sizeof(double) * n_part, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(a.row0, r.data(), sizeof(int32_t) * n_part, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(a.levent, m1.data(), sizeof(int32_t) * n_part, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(a.lpass, z.data(), sizeof(int32_t) * n_part, hipMemcpyHostToDevice));
+  b->n_alive = n_part;
+  b->n_lost = 0;
+  b->event = 0;
+  b->pass = 0;
+  b->kernel_ms = 0;
+  b->launches = 0;
+  return n_part;
+}
+
+/* ------------------------------------------------------------------------- */
+/* kernels */
+#define EH_BLOCK 256
+
+__device__ __forceinline__ eh_p6 load_p(const eh_soa &a, long i) {
+  eh_p6 p;
+  p.x = a.x[i];
+  p.xp = a.xp[i];
+  p.y = a.y[i];
+  p.yp = a.yp[i];
+  p.s = a.s[i];
+  p.dp = a.dp[i];
+  return p;
+}
+
+__device__ __forceinline__ void store_p(const eh_soa &a, long i, const eh_p6 &p) {
+  a.x[i] = p.x;
+  a.xp[i] = p.xp;
+  a.y[i] = p.y;
+  a.yp[i] = p.yp;
+  a.s[i] = p.s;
+  a.dp[i] = p.dp;
+}
+
+/* wavefront-aggregated atomic add: one atomic per wavefront */
+__device__ __forceinline__ unsigned long long wave_add(unsigned long long *ctr, unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, 64);
+  unsigned long long base = 0;
+  if (__lane_id() == 0 && v)
+    base = atomicAdd(ctr, v);
+  return base;
+}
+
+/* exclusive prefix of a predicate within the wavefront, and the wavefront total */
+__device__ __forceinline__ unsigned wave_prefix(bool pred, unsigned *total) {
+  unsigned long long m = __ballot(pred);
+  *total = (unsigned)__popcll(m);
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+/* one element routine (op 0), one limit_amplitudes sweep (op 1) or one
+ * removeInvalidParticles sweep (op 2) over the alive particles, in place; losses
+ * are marked with (pass, event) */
+__global__ void __launch_bounds__(EH_BLOCK) k_plain(eh_soa a, long n, const eh_desc *__restrict__ desc, int op,
+                                                     double xmax, double ymax, double z, double Po, int extrap,
+                                                     int32_t pass, int32_t event, unsigned long long *cnt) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool lost = false;
+  if (i < n) {
+    eh_p6 p = load_p(a, i);
+    if (op == 0) {
+      const eh_desc &d = *desc;
+      switch (d.type) {
+      case EH_T_DRIF:
+        eh_drif(p, d.length, d.T11);
+        break;
+      case EH_T_EDRIFT:
+        eh_edrift(p, d.length);
+        break;
+      case EH_T_KQUAD:
+      case EH_T_KSEXT:
+      case EH_T_KOCT:
+        lost = eh_multipole(p, d, 0);
+        break;
+      case EH_T_CSBEND:
+        lost = eh_csbend(p, d, 0);
+        break;
+      case EH_T_RFCA:
+        lost = eh_rfca(p, d);
+        break;
+      default:
+        break;
+      }
+    } else if (op == 1) {
+      lost = eh_limit_one(p, xmax, ymax, z, Po, extrap);
+    } else {
+      /* removeInvalidParticles, src/limit_amplitudes.c:289-330 */
+      if (eh_isnan(p.x) || eh_isnan(p.xp) || eh_isnan(p.y) || eh_isnan(p.yp) || eh_isnan(p.s) || eh_isnan(p.dp) ||
+          p.dp <= -1) {
+        p.s = z;
+        p.dp = Po * (1 + p.dp);
+        lost = true;
+      }
+    }
+    store_p(a, i, p);
+    if (lost) {
+      a.lpass[i] = pass;
+      a.levent[i] = event;
+    }
+  }
+  unsigned tot;
+  wave_prefix(lost, &tot);
+  wave_add(&cnt[2], tot);
+}
+
+/* move survivors of a to the front of b and lost particles to the graveyard */
+__global__ void __launch_bounds__(EH_BLOCK) k_compact(eh_soa a, long n, eh_soa b, eh_soa g,
+                                                       unsigned long long *cnt) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool in = i < n;
+  bool alive = in && a.levent[i] < 0;
+  bool lost = in && !alive;
+  unsigned na, nl;
+  unsigned pa = wave_prefix(alive, &na);
+  unsigned pl = wave_prefix(lost, &nl);
+  unsigned long long ba = 0, bl = 0;
+  if (__lane_id() == 0) {
+    if (na)
+      ba = atomicAdd(&cnt[0], (unsigned long long)na);
+    if (nl)
+      bl = atomicAdd(&cnt[1], (unsigned long long)nl);
+  }
+  ba = __shfl(ba, 0, 64);
+  bl = __shfl(bl, 0, 64);
+  if (alive) {
+    long j = (long)(ba + pa);
+    store_p(b, j, load_p(a, i));
+    b.row0[j] = a.row0[i];
+    b.lpass[j] = 0;
+    b.levent[j] = -1;
+  } else if (lost) {
+    long j = (long)(bl + pl);
+    store_p(g, j, load_p(a, i));
+    g.row0[j] = a.row0[i];
+    g.lpass[j] = a.lpass[i];
+    g.levent[j] = a.levent[i];
+  }
+}
+
+/* fused tracking: every particle runs the element sequence from (p0, e0) up to
+ * (p1, e1) exclusive in registers, then survivors are compacted into b and lost
+ * particles appended to the graveyard.  Events are event_base + ((p - p0) * nelem
+ * + e) * 3 + stage, in the order the reference compacts. */
+__global__ void __launch_bounds__(EH_BLOCK) k_fused(eh_soa a, long n, eh_soa b, eh_soa g,
+                                                     const eh_desc *__restrict__ descs, int nelem, int p0, int e0,
+                                                     int p1, int e1, int32_t event_base, unsigned long long *cnt) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool in = i < n;
+  bool alive = in;
+  eh_p6 p = {0, 0, 0, 0, 0, 0};
+  if (in)
+    p = load_p(a, i);
+  int32_t lpass = 0, levent = -1;
+  unsigned long long units = 0;
+  int pp = p0, ee = e0;
+  while (pp < p1 || (pp == p1 && ee < e1)) {
+    if (__ballot(alive) == 0)
+      break;
+    const eh_desc *d = descs + ee;
+    int pi = pp > 0 ? 1 : 0;
+    if (alive) {
+      units += d->counted;
+      int st = eh_apply_element(p, *d, pi);
+      if (st >= 0) {
+        alive = false;
+        lpass = pp;
+        levent = event_base + ((pp - p0) * nelem + ee) * 3 + st;
+      }
+    }
+    if (++ee == nelem) {
+      ee = 0;
+      pp++;
+    }
+  }
+  wave_add(&cnt[3], units);
+  bool lost = in && !alive;
+  unsigned na, nl;
+  unsigned pa = wave_prefix(alive, &na);
+  unsigned pl = wave_prefix(lost, &nl);
+  unsigned long long ba = 0, bl = 0;
+  if (__lane_id() == 0) {
+    if (na)
+      ba = atomicAdd(&cnt[0], (unsigned long long)na);
+    if (nl)
+      bl = atomicAdd(&cnt[1], (unsigned long long)nl);
+  }
+  ba = __shfl(ba, 0, 64);
+  bl = __shfl(bl, 0, 64);
+  if (alive) {
+    long j = (long)(ba + pa);
+    store_p(b, j, p);
+    b.row0[j] = a.row0[i];
+    b.lpass[j] = 0;
+    b.levent[j] = -1;
+  } else if (lost) {
+    long j = (long)(bl + pl);
+    store_p(g, j, p);
+    g.row0[j] = a.row0[i];
+    g.lpass[j] = lpass;
+    g.levent[j] = levent;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* host helpers */
+static long ensure_desc(eh_beam *b, long n) {
+  if (b->desc_cap >= n)
+    return EH_OK;
+  if (b->d_desc)
+    HIPCHK(hipFree(b->d_desc));
+  b->d_desc = nullptr;
+  HIPCHK(hipMalloc(&b->d_desc, sizeof(eh_desc) * n));
+  b->desc_cap = n;
+  return EH_OK;
+}
+
+static unsigned grid_for(long n) { return (unsigned)((n + EH_BLOCK - 1) / EH_BLOCK); }
+
+static long timed_begin(eh_beam *b) {
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  return EH_OK;
+}
+
+static long timed_end(eh_beam *b) {
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->kernel_ms += ms;
+  b->launches++;
+  return EH_OK;
+}
+
+static int32_t next_event(eh_beam *b) { return b->event++; }
+
+/* compact after an in-place kernel that lost `lost` particles */
+static long compact(eh_beam *b, long lost) {
+  if (lost <= 0)
+    return b->n_alive;
+  unsigned long long init[2] = {0, (unsigned long long)b->n_lost};
+  HIPCHK(hipMemcpyAsync(b->d_cnt, init, sizeof init, hipMemcpyHostToDevice, b->stream));
+  eh_soa &src = b->buf[b->cur], &dst = b->buf[1 - b->cur];
+  long n = b->n_alive;
+  hipLaunchKernelGGL(k_compact, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, src, n, dst, b->grave, b->d_cnt);
+  HIPCHK(hipGetLastError());
+  unsigned long long out[2];
+  HIPCHK(hipMemcpyAsync(out, b->d_cnt, sizeof out, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->cur = 1 - b->cur;
+  b->n_alive = (long)out[0];
+  b->n_lost = (long)out[1];
+  return b->n_alive;
+}
+
+/* run one plain op over the alive particles; returns survivors */
+static long run_plain(eh_beam *b, const eh_desc *hd, int op, double xmax, double ymax, double z, double Po,
+                      int extrap) {
+  long n = b->n_alive;
+  int32_t ev = next_event(b);
+  if (n == 0)
+    return 0;
+  if (op == 0) {
+    long r = ensure_desc(b, 1);
+    if (r < 0)
+      return r;
+    HIPCHK(hipMemcpyAsync(b->d_desc, hd, sizeof(eh_desc), hipMemcpyHostToDevice, b->stream));
+  }
+  unsigned long long zero = 0;
+  HIPCHK(hipMemcpyAsync(b->d_cnt + 2, &zero, sizeof zero, hipMemcpyHostToDevice, b->stream));
+  long r = timed_begin(b);
+  if (r < 0)
+    return r;
+  hipLaunchKernelGGL(k_plain, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, b->buf[b->cur], n, b->d_desc, op,
+                     xmax, ymax, z, Po, extrap, (int32_t)b->pass, ev, b->d_cnt);
+  HIPCHK(hipGetLastError());
+  r = timed_end(b);
+  if (r < 0)
+    return r;
+  unsigned long long lost = 0;
+  HIPCHK(hipMemcpy(&lost, b->d_cnt + 2, sizeof lost, hipMemcpyDeviceToHost));
+  long before_lost = b->n_lost;
+  long na = compact(b, (long)lost);
+  if (na < 0)
+    return na;
+  if (lost && (b->n_lost - before_lost != (long)lost || na != n - (long)lost))
+    return fail(EH_ERR_STATE, "compaction count mismatch");
+  return na;
+}
+
+static long check_np(eh_beam *b, long np) {
+  if (!b)
+    return fail(EH_ERR_ARG, "null beam");
+  if (np != b->n_alive)
+    return fail(EH_ERR_ARG, "n_part=%ld but %ld particles are resident", np, b->n_alive);
+  return EH_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* per-element entry points */
+extern "C" long eh_multipole_tracking2(eh_beam *b, long n_part, const EH_ELEMENT_LIST *elem, double p_error,
+                                       double Po, double z_start, const EH_MAXAMP *maxamp, long iSlice) {
+  (void)p_error;
+  long r = check_np(b, n_part);
+  if (r < 0)
+    return r;
+  if (iSlice >= 0)
+    return fail(EH_ERR_UNSUPPORTED, "slice-by-slice integration (iSlice>=0) not supported");
+  eh_desc d;
+  eh_ctx c = {};
+  c.Po = Po;
+  c.z_start = z_start;
+  c.z_end = z_start + elem_length(elem);
+  c.ap[0] = c.ap[1] = maxamp;
+  r = build_desc(&d, elem, c);
+  if (r < 0)
+    return r;
+  return run_plain(b, &d, 0, 0, 0, 0, Po, 0);
+}
+
+extern "C" long eh_track_through_csbend(eh_beam *b, long n_part, const EH_CSBEND *csbend, double p_error, double Po,
+                                        double z_start, const EH_MAXAMP *maxamp, long iSlice) {
+  (void)p_error;
+  long r = check_np(b, n_part);
+  if (r < 0)
+    return r;
+  if (iSlice >= 0)
+    return fail(EH_ERR_UNSUPPORTED, "slice-by-slice integration (iSlice>=0) not supported");
+  EH_ELEMENT_LIST e = {EH_T_CSBEND, 0, "CSBEND", (void *)csbend};
+  eh_desc d;
+  eh_ctx c = {};
+  c.Po = Po;
+  c.z_start = z_start;
+  c.z_end = z_start + csbend->length;
+  c.ap[0] = c.ap[1] = maxamp;
+  r = build_desc(&d, &e, c);
+  if (r < 0)
+    return r;
+  return run_plain(b, &d, 0, 0, 0, 0, Po, 0);
+}
+
+extern "C" long eh_exactDrift(eh_beam *b, long np, double length) {
+  long r = check_np(b, np);
+  if (r < 0)
+    return r;
+  eh_desc d;
+  std::memset(&d, 0, sizeof d);
+  d.type = EH_T_EDRIFT;
+  d.length = length;
+  return run_plain(b, &d, 0, 0, 0, 0, 0, 0);
+}
+
+extern "C" long eh_drift_beam(eh_beam *b, long np, double length, long order) {
+  long r = check_np(b, np);
+  if (r < 0)
+    return r;
+  if (!length)
+    return np; /* drift_beam does nothing for zero length, src/do_tracking.c:3759 */
+  eh_desc d;
+  std::memset(&d, 0, sizeof d);
+  d.type = EH_T_DRIF;
+  d.length = length;
+  if (order <= 0)
+    order = 2;
+  d.T11 = order > 1 ? length / 2 : 0;
+  return run_plain(b, &d, 0, 0, 0, 0, 0, 0);
+}
+
+extern "C" long eh_limit_amplitudes(eh_beam *b, double xmax, double ymax, long np, double z, double Po,
+                                    long extrapolate_z, long openCode) {
+  long r = check_np(b, np);
+  if (r < 0)
+    return r;
+  if (openCode)
+    return fail(EH_ERR_UNSUPPORTED, "open-sided apertures not supported");
+  if (xmax < 0 && ymax < 0)
+    return np;
+  return run_plain(b, nullptr, 1, xmax, ymax, z, Po, (int)extrapolate_z);
+}
+
+/* Kahan-compensated TMEAN fiducial over the alive particles in reference row
+ * order (findFiducialTime, src/simple_rfca.c:179-254; KahanPlus, src/kahan.c:27-35) */
+static long reference_order(eh_beam *b, std::vector<int32_t> &slot_at_row, std::vector<int32_t> &alive_row0,
+                            std::vector<int32_t> &dev_index_alive);
+static long fiducial_tmean(eh_beam *b, double sOffset, double p0, double *tFid) {
+  std::vector<int32_t> slot_at_row, alive_row0, idx;
+  long r = reference_order(b, slot_at_row, alive_row0, idx);
+  if (r < 0)
+    return r;
+  long n = b->n_alive;
+  std::vector<double> s(n), dp(n);
+  eh_soa &a = b->buf[b->cur];
+  HIPCHK(hipMemcpy(s.data(), a.s, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(dp.data(), a.dp, sizeof(double) * n, hipMemcpyDeviceToHost));
+  /* idx[row0] = device index of the alive particle with that identity */
+  double tsum = 0, error = 0.0;
+  long nsum = 0;
+  for (long row = 0; row < n; row++) {
+    int32_t id = slot_at_row[row];
+    long k = idx[id];
+    if (k < 0)
+      return fail(EH_ERR_STATE, "replay produced a lost particle among survivors");
+    double p = p0;
+    p *= 1 + dp[k];
+    double beta = p / std::sqrt(p * p + 1);
+    double term = (s[k] + sOffset) / (EH_C_MKS * beta);
+    double Y = term + error;
+    double sum = tsum + Y;
+    error = Y - (sum - tsum);
+    tsum = sum;
+    nsum++;
+  }
+  *tFid = nsum > 0 ? tsum / nsum : 0.0;
+  return EH_OK;
+}
+
+/* first-encounter fiducial, src/simple_rfca.c:414-463 */
+static long rfca_fiducial(eh_beam *b, EH_RFCA *rf, double Pc, double zEnd) {
+  if (rf->fiducial_seen || rf->volt == 0)
+    return EH_OK;
+  double length = rf->length;
+  long nKicks = length ? rf->nKicks : 1;
+  if (nKicks <= 0)
+    nKicks = 1;
+  length /= nKicks;
+  double omega = 2 * EH_PI * rf->freq, t0;
+  if (rf->tReference != -1)
+    t0 = rf->tReference;
+  else if (rf->fiducial_mode == EH_FID_LIGHT)
+    t0 = ((zEnd - rf->length) + length / 2) / EH_C_MKS;
+  else {
+    long r = fiducial_tmean(b, length / 2, Pc, &t0);
+    if (r < 0)
+      return r;
+  }
+  rf->phase_fiducial = -omega * t0;
+  rf->fiducial_seen = 1;
+  return EH_OK;
+}
+
+extern "C" long eh_simple_rf_cavity(eh_beam *b, long np, EH_RFCA *rfca, double *P_central, double zEnd) {
+  long r = check_np(b, np);
+  if (r < 0)
+    return r;
+  if (np <= 0)
+    return np;
+  if (rfca->Q != 0 || rfca->change_t || rfca->change_p0 || rfca->linearize || rfca->lockPhase)
+    return fail(EH_ERR_UNSUPPORTED, "RFCA option not supported");
+  r = rfca_fiducial(b, rfca, *P_central, zEnd);
+  if (r < 0)
+    return r;
+  if (rfca->volt != 0 && rfca->freq != 0)
+    rfca->t_fiducial = -rfca->phase_fiducial / (2 * EH_PI * rfca->freq);
+  EH_ELEMENT_LIST e = {EH_T_RFCA, 0, "RFCA", (void *)rfca};
+  eh_desc d;
+  eh_ctx c = {};
+  c.Po = *P_central;
+  c.z_start = zEnd - rfca->length;
+  c.z_end = zEnd;
+  r = build_desc(&d, &e, c);
+  if (r < 0)
+    return r;
+  return run_plain(b, &d, 0, 0, 0, 0, *P_central, 0);
+}
+
+extern "C" long eh_removeInvalidParticles(eh_beam *b, long np, double z, double Po) {
+  long r = check_np(b, np);
+  if (r < 0)
+    return r;
+  return run_plain(b, nullptr, 2, 0, 0, z, Po, 0);
+}
+
+/* ------------------------------------------------------------------------- */
+/* reference row order: replay of swap-to-top compaction
+ * (src/multipole.c:1013-1049, src/csbend.c:1088-1267, src/limit_amplitudes.c:226-285,
+ * :294-327).  Within one compaction event the reference scans rows upward and
+ * swaps every lost row with the current top row, re-examining the row it just
+ * filled; the chain below reproduces that sequence touching only O(lost) rows. */
+static void replay_order(long N, std::vector<std::pair<int32_t, int32_t>> &lost, std::vector<int32_t> &slot_at) {
+  slot_at.resize(N);
+  std::vector<int32_t> pos(N);
+  for (long r = 0; r < N; r++)
+    slot_at[r] = pos[r] = (int32_t)r;
+  std::vector<char> lostnow(N, 0);
+  std::stable_sort(lost.begin(), lost.end(),
+                   [](const std::pair<int32_t, int32_t> &a, const std::pair<int32_t, int32_t> &b) {
+                     return a.first < b.first;
+                   });
+  long nTo = N;
+  std::vector<int32_t> rows;
+  size_t g0 = 0;
+  while (g0 < lost.size()) {
+    size_t g1 = g0;
+    while (g1 < lost.size() && lost[g1].first == lost[g0].first)
+      g1++;
+    rows.clear();
+    for (size_t k = g0; k < g1; k++) {
+      int32_t id = lost[k].second;
+      lostnow[id] = 1;
+      rows.push_back(pos[id]);
+    }
+    std::sort(rows.begin(), rows.end());
+    long top = nTo - 1;
+    for (size_t k = 0; k < rows.size(); k++) {
+      long i = rows[k];
+      if (i > top)
+        break;
+      for (;;) {
+        if (i == top) {
+          top--;
+          break;
+        }
+        int32_t a = slot_at[i], c = slot_at[top];
+        slot_at[i] = c;
+        slot_at[top] = a;
+        pos[c] = (int32_t)i;
+        pos[a] = (int32_t)top;
+        top--;
+        if (!lostnow[slot_at[i]])
+          break;
+      }
+      if (i > top)
+        break;
+    }
+    for (size_t k = g0; k < g1; k++)
+      lostnow[lost[k].second] = 0;
+    nTo -= (long)(g1 - g0);
+    g0 = g1;
+  }
+}
+
+/* slot_at_row[r] = identity (upload row) of the particle in row r of the
+ * reference's arrangement; idx[identity] = index in the alive buffer or -1 */
+static long reference_order(eh_beam *b, std::vector<int32_t> &slot_at_row, std::vector<int32_t> &alive_row0,
+                            std::vector<int32_t> &idx) {
+  long N = b->n_alive + b->n_lost;
+  alive_row0.resize(b->n_alive);
+  std::vector<int32_t> grow0(b->n_lost), gev(b->n_lost);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->n_alive)
+    HIPCHK(hipMemcpy(alive_row0.data(), b->buf[b->cur].row0, sizeof(int32_t) * b->n_alive, hipMemcpyDeviceToHost));
+  if (b->n_lost) {
+    HIPCHK(hipMemcpy(grow0.data(), b->grave.row0, sizeof(int32_t) * b->n_lost, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(gev.data(), b->grave.levent, sizeof(int32_t) * b->n_lost, hipMemcpyDeviceToHost));
+  }
+  idx.assign(N, -1);
+  for (long k = 0; k < b->n_alive; k++) {
+    if (alive_row0[k] < 0 || alive_row0[k] >= N)
+      return fail(EH_ERR_STATE, "corrupt particle identity");
+    idx[alive_row0[k]] = (int32_t)k;
+  }
+  std::vector<std::pair<int32_t, int32_t>> lost(b->n_lost);
+  for (long k = 0; k < b->n_lost; k++) {
+    if (grow0[k] < 0 || grow0[k] >= N)
+      return fail(EH_ERR_STATE, "corrupt particle identity");
+    lost[k] = std::make_pair(gev[k], grow0[k]);
+  }
+  replay_order(N, lost, slot_at_row);
+  return EH_OK;
+}
+
+extern "C" long eh_beam_download(eh_beam *b, double **coord, long n_part) {
+  if (!b || !coord)
+    return fail(EH_ERR_ARG, "eh_beam_download: null argument");
+  long N = b->n_alive + b->n_lost;
+  if (n_part != N)
+    return fail(EH_ERR_ARG, "eh_beam_download: n_part=%ld but the beam holds %ld particles", n_part, N);
+  std::vector<int32_t> slot_at_row, alive_row0, idx;
+  long r = reference_order(b, slot_at_row, alive_row0, idx);
+  if (r < 0)
+    return r;
+  long na = b->n_alive, nl = b->n_lost;
+  std::vector<double> A(6 * (size_t)na), G(6 * (size_t)nl);
+  std::vector<int32_t> grow0(nl), gpass(nl);
+  const eh_soa &a = b->buf[b->cur], &g = b->grave;
+  const double *as[6] = {a.x, a.xp, a.y, a.yp, a.s, a.dp}, *gs[6] = {g.x, g.xp, g.y, g.yp, g.s, g.dp};
+  for (int k = 0; k < 6; k++) {
+    if (na)
+      HIPCHK(hipMemcpy(A.data() + k * (size_t)na, as[k], sizeof(double) * na, hipMemcpyDeviceToHost));
+    if (nl)
+      HIPCHK(hipMemcpy(G.data() + k * (size_t)nl, gs[k], sizeof(double) * nl, hipMemcpyDeviceToHost));
+  }
+  if (nl) {
+    HIPCHK(hipMemcpy(grow0.data(), g.row0, sizeof(int32_t) * nl, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(gpass.data(), g.lpass, sizeof(int32_t) * nl, hipMemcpyDeviceToHost));
+  }
+  std::vector<int32_t> gidx(N, -1);
+  for (long k = 0; k < nl; k++)
+    gidx[grow0[k]] = (int32_t)k;
+  for (long row = 0; row < N; row++) {
+    int32_t id = slot_at_row[row];
+    double *c = coord[row];
+    for (int k = 0; k < 4; k++)
+      c[6 + k] = b->cols[4 * (size_t)id + k];
+    if (idx[id] >= 0) {
+      long k = idx[id];
+      for (int q = 0; q < 6; q++)
+        c[q] = A[q * (size_t)na + k];
+    } else {
+      long k = gidx[id];
+      if (k < 0)
+        return fail(EH_ERR_STATE, "particle %d missing on download", id);
+      for (int q = 0; q < 6; q++)
+        c[q] = G[q * (size_t)nl + k];
+      c[EH_LOSS_PASS_INDEX] = gpass[k];
+    }
+  }
+  return na;
+}
+
+/* ------------------------------------------------------------------------- */
+/* do_tracking */
+static long track_plain(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes, double *Pc, double *effort) {
+  long nToTrack = b->n_alive, nLeft;
+  const EH_MAXAMP *maxamp = nullptr;
+  double x_max = 0, y_max = 0;
+  b->pass = 0;
+  nLeft = eh_limit_amplitudes(b, EH_DBL_MAX, EH_DBL_MAX, nToTrack, 0.0, *Pc, 0, 0);
+  if (nLeft < 0)
+    return nLeft;
+  nToTrack = nLeft;
+  for (long pass = 0; pass < n_passes; pass++) {
+    b->pass = pass;
+    double z = 0, z_travel = 0, last_z = 0;
+    for (long ie = 0; ie < ne; ie++) {
+      EH_ELEMENT_LIST *e = bl + ie;
+      double len = elem_length(e);
+      last_z = z;
+      z += len;
+      z_travel += len;
+      nLeft = nToTrack;
+      if (e->type != EH_T_MARK && e->type != EH_T_MAXAMP)
+        *effort += nToTrack;
+      switch (e->type) {
+      case EH_T_MARK:
+        break;
+      case EH_T_MAXAMP:
+        maxamp = (const EH_MAXAMP *)e->p_elem;
+        if (maxamp->elliptical)
+          return fail(EH_ERR_UNSUPPORTED, "elliptical MAXAMP not supported");
+        x_max = maxamp->x_max;
+        y_max = maxamp->y_max;
+        break;
+      case EH_T_DRIF: {
+        const EH_DRIFT *d = (const EH_DRIFT *)e->p_elem;
+        eh_desc dd;
+        std::memset(&dd, 0, sizeof dd);
+        dd.type = EH_T_DRIF;
+        dd.length = d->length;
+        long order = d->order ? d->order : 2;
+        dd.T11 = order > 1 ? d->length / 2 : 0;
+        nLeft = run_plain(b, &dd, 0, 0, 0, 0, 0, 0);
+        break;
+      }
+      case EH_T_EDRIFT:
+        nLeft = eh_exactDrift(b, nToTrack, ((const EH_EDRIFT *)e->p_elem)->length);
+        break;
+      case EH_T_KQUAD:
+      case EH_T_KSEXT:
+      case EH_T_KOCT:
+        nLeft = eh_multipole_tracking2(b, nToTrack, e, 0.0, *Pc, last_z, maxamp, -1);
+        break;
+      case EH_T_CSBEND:
+        nLeft = eh_track_through_csbend(b, nToTrack, (const EH_CSBEND *)e->p_elem, 0.0, *Pc, last_z, maxamp, -1);
+        break;
+      case EH_T_RFCA:
+        nLeft = eh_simple_rf_cavity(b, nToTrack, (EH_RFCA *)e->p_elem, Pc, z_travel);
+        break;
+      default:
+        return fail(EH_ERR_UNSUPPORTED, "element type %d not supported", e->type);
+      }
+      if (nLeft < 0)
+        return nLeft;
+      if (e->type != EH_T_MARK && (x_max || y_max)) {
+        nLeft = eh_limit_amplitudes(b, x_max, y_max, nLeft, z, *Pc, e->type == EH_T_DRIF, 0);
+        if (nLeft < 0)
+          return nLeft;
+      }
+      nLeft = eh_limit_amplitudes(b, EH_DBL_MAX, EH_DBL_MAX, nLeft, z, *Pc, 0, 0);
+      if (nLeft < 0)
+        return nLeft;
+      nToTrack = nLeft;
+    }
+  }
+  return nToTrack;
+}
+
+static long launch_fused(eh_beam *b, long ne, int p0, int e0, int p1, int e1, double *effort) {
+  long n = b->n_alive;
+  long span = ((long)(p1 - p0) * ne + ne) * 3 + 3;
+  if ((long)b->event + span >= 2147483647L)
+    return fail(EH_ERR_STATE, "loss event counter overflow");
+  int32_t base = b->event;
+  b->event += (int32_t)span;
+  if (n == 0 || (p0 == p1 && e0 >= e1))
+    return b->n_alive;
+  unsigned long long init[4] = {0, (unsigned long long)b->n_lost, 0, 0};
+  HIPCHK(hipMemcpyAsync(b->d_cnt, init, sizeof init, hipMemcpyHostToDevice, b->stream));
+  long r = timed_begin(b);
+  if (r < 0)
+    return r;
+  hipLaunchKernelGGL(k_fused, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, b->buf[b->cur], n, b->buf[1 - b->cur],
+                     b->grave, b->d_desc, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
+  HIPCHK(hipGetLastError());
+  r = timed_end(b);
+  if (r < 0)
+    return r;
+  unsigned long long out[4];
+  HIPCHK(hipMemcpy(out, b->d_cnt, sizeof out, hipMemcpyDeviceToHost));
+  if ((long)out[0] + (long)out[1] != n + b->n_lost)
+    return fail(EH_ERR_STATE, "fused compaction count mismatch");
+  b->cur = 1 - b->cur;
+  b->n_alive = (long)out[0];
+  b->n_lost = (long)out[1];
+  *effort += (double)out[3];
+  return b->n_alive;
+}
+
+static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes, double *Pc, int tpl,
+                        double *effort) {
+  if (tpl < 1)
+    tpl = 1;
+  long r = ensure_desc(b, ne);
+  if (r < 0)
+    return r;
+  std::vector<eh_desc> hd(ne);
+  std::vector<double> zs(ne), ze(ne);
+  /* positions (wrap_around resets z every pass) and apertures for pass 0 / later */
+  const EH_MAXAMP *last_amp = nullptr;
+  {
+    double z = 0;
+    for (long ie = 0; ie < ne; ie++) {
+      zs[ie] = z;
+      z += elem_length(bl + ie);
+      ze[ie] = z;
+      if (bl[ie].type == EH_T_MAXAMP)
+        last_amp = (const EH_MAXAMP *)bl[ie].p_elem;
+    }
+  }
+  std::vector<const EH_MAXAMP *> amp0(ne), amp1(ne);
+  {
+    const EH_MAXAMP *cur0 = nullptr, *cur1 = last_amp;
+    for (long ie = 0; ie < ne; ie++) {
+      if (bl[ie].type == EH_T_MAXAMP)
+        cur0 = cur1 = (const EH_MAXAMP *)bl[ie].p_elem;
+      amp0[ie] = cur0;
+      amp1[ie] = cur1;
+    }
+  }
+  auto build = [&](long ie) -> long {
+    eh_ctx c = {};
+    c.Po = *Pc;
+    c.z_start = zs[ie];
+    c.z_end = ze[ie];
+    c.ap[0] = amp0[ie];
+    c.ap[1] = amp1[ie];
+    for (int pi = 0; pi < 2; pi++) {
+      const EH_MAXAMP *m = pi ? amp1[ie] : amp0[ie];
+      c.post_x[pi] = m ? m->x_max : 0;
+      c.post_y[pi] = m ? m->y_max : 0;
+      c.post_on[pi] = (bl[ie].type != EH_T_MARK) && m && (m->x_max || m->y_max);
+      if (m && m->elliptical)
+        return fail(EH_ERR_UNSUPPORTED, "elliptical MAXAMP not supported");
+    }
+    return build_desc(&hd[ie], bl + ie, c);
+  };
+  for (long ie = 0; ie < ne; ie++) {
+    if (bl[ie].type == EH_T_RFCA && !((EH_RFCA *)bl[ie].p_elem)->fiducial_seen)
+      continue; /* built once its fiducial is known */
+    r = build(ie);
+    if (r < 0)
+      return r;
+  }
+  /* initial NaN sweep, src/do_tracking.c:478-483 */
+  b->pass = 0;
+  r = eh_limit_amplitudes(b, EH_DBL_MAX, EH_DBL_MAX, b->n_alive, 0.0, *Pc, 0, 0);
+  if (r < 0)
+    return r;
+  HIPCHK(hipMemcpy(b->d_desc, hd.data(), sizeof(eh_desc) * ne, hipMemcpyHostToDevice));
+  int p = 0, e = 0;
+  if (n_passes > 0) {
+    /* pass 0: stop before each RFCA whose fiducial is still unknown */
+    for (long ie = 0; ie < ne; ie++) {
+      if (bl[ie].type != EH_T_RFCA)
+        continue;
+      EH_RFCA *rf = (EH_RFCA *)bl[ie].p_elem;
+      if (rf->fiducial_seen)
+        continue;
+      r = launch_fused(b, ne, p, e, 0, (int)ie, effort);
+      if (r < 0)
+        return r;
+      r = rfca_fiducial(b, rf, *Pc, ze[ie]);
+      if (r < 0)
+        return r;
+      if (rf->volt != 0 && rf->freq != 0)
+        rf->t_fiducial = -rf->phase_fiducial / (2 * EH_PI * rf->freq);
+      for (long je = 0; je < ne; je++)
+        if (bl[je].p_elem == bl[ie].p_elem) {
+          r = build(je);
+          if (r < 0)
+            return r;
+          HIPCHK(hipMemcpy(b->d_desc + je, &hd[je], sizeof(eh_desc), hipMemcpyHostToDevice));
+        }
+      e = (int)ie;
+    }
+  }
+  while (p < n_passes) {
+    int p1 = p + tpl;
+    if (p1 > n_passes)
+      p1 = (int)n_passes;
+    int pe = p1, ee = 0;
+    if (e != 0) { /* finish the partial pass first */
+      pe = p;
+      ee = (int)ne;
+      p1 = p + 1;
+    }
+    r = launch_fused(b, ne, p, e, pe, ee, effort);
+    if (r < 0)
+      return r;
+    p = p1;
+    e = 0;
+  }
+  b->pass = n_passes > 0 ? n_passes - 1 : 0;
+  return b->n_alive;
+}
+
+extern "C" long eh_do_tracking(eh_beam *b, EH_ELEMENT_LIST *beamline, long n_elements, long n_passes,
+                               double *P_central, int mode, int turns_per_launch, eh_track_stats *stats) {
+  if (!b || !beamline || !P_central || n_elements < 0 || n_passes < 0)
+    return fail(EH_ERR_ARG, "eh_do_tracking: bad arguments");
+  double effort = 0;
+  double ms0 = b->kernel_ms;
+  long l0 = b->launches;
+  long r;
+  if (mode == EH_MODE_PLAIN)
+    r = track_plain(b, beamline, n_elements, n_passes, P_central, &effort);
+  else if (mode == EH_MODE_FUSED)
+    r = track_fused(b, beamline, n_elements, n_passes, P_central, turns_per_launch, &effort);
+  else
+    return fail(EH_ERR_ARG, "unknown mode %d", mode);
+  if (stats) {
+    stats->particle_element_turns = effort;
+    stats->kernel_ms = b->kernel_ms - ms0;
+    stats->launches = b->launches - l0;
+    stats->n_alive = b->n_alive;
+  }
+  return r;
+}
+
+/* contiguous AoS variants: rows of EH_NPROP doubles, as czarray_2d lays them out */
+extern "C" long eh_beam_upload_flat(eh_beam *b, const double *flat, long n_part) {
+  if (!flat || n_part < 0)
+    return fail(EH_ERR_ARG, "eh_beam_upload_flat: bad arguments");
+  std::vector<double *> rows(n_part > 0 ? n_part : 1);
+  for (long i = 0; i < n_part; i++)
+    rows[i] = const_cast<double *>(flat) + i * EH_NPROP;
+  return eh_beam_upload(b, rows.data(), n_part);
+}
+
+extern "C" long eh_beam_download_flat(eh_beam *b, double *flat, long n_part) {
+  if (!flat || n_part < 0)
+    return fail(EH_ERR_ARG, "eh_beam_download_flat: bad arguments");
+  std::vector<double *> rows(n_part > 0 ? n_part : 1);
+  for (long i = 0; i < n_part; i++)
+    rows[i] = flat + i * EH_NPROP;
+  return eh_beam_download(b, rows.data(), n_part);
+}
+
+/* ------------------------------------------------------------------------- */
+/* diagnostics */
+
+/* reference row order for n particles given the loss event and identity of each
+ * lost particle (host routine; exposed for testing the replay) */
+extern "C" long eh_replay_order(long n, const int32_t *lost_event, const int32_t *lost_id, long n_lost,
+                                int32_t *slot_at_row) {
+  if (n < 0 || n_lost < 0 || n_lost > n || (n_lost && (!lost_event || !lost_id)) || !slot_at_row)
+    return fail(EH_ERR_ARG, "eh_replay_order: bad arguments");
+  std::vector<std::pair<int32_t, int32_t>> lost(n_lost);
+  for (long k = 0; k < n_lost; k++) {
+    if (lost_id[k] < 0 || lost_id[k] >= n)
+      return fail(EH_ERR_ARG, "eh_replay_order: identity out of range");
+    lost[k] = std::make_pair(lost_event[k], lost_id[k]);
+  }
+  std::vector<int32_t> s;
+  replay_order(n, lost, s);
+  std::memcpy(slot_at_row, s.data(), sizeof(int32_t) * n);
+  return EH_OK;
+}
+
+__global__ void k_selftest(const double *x, const double *y, long n, double *out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  double a = x[i], b = y[i];
+  out[i] = eh_sin(a);
+  out[n + i] = eh_cos(a);
+  out[2 * n + i] = eh_asin(b);
+  out[3 * n + i] = eh_log(eh_fabs(a));
+  out[4 * n + i] = eh_sqrt(eh_fabs(a));
+  out[5 * n + i] = a / b;
+  out[6 * n + i] = eh_tan(a);
+}
+
+/* evaluates sin(x), cos(x), asin(y), log|x|, sqrt|x|, x/y, tan(x) on the GPU into
+ * out[7*n] (for checking that device arithmetic matches the host bit for bit) */
+extern "C" long eh_selftest_math(const double *x, const double *y, long n, double *out) {
+  if (g_device < 0 && eh_init(0) != EH_OK)
+    return EH_ERR_HIP;
+  if (n <= 0)
+    return 0;
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, sizeof(double) * 9 * n));
+  hipError_t e = hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(d + n, y, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_selftest, dim3(grid_for(n)), dim3(EH_BLOCK), 0, 0, d, d + n, n, d + 2 * n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(out, d + 2 * n, sizeof(double) * 7 * n, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess)
+    return fail(EH_ERR_HIP, "eh_selftest_math: %s", hipGetErrorString(e));
+  return n;
+}

@@ -1,0 +1,249 @@
+/* eh_math.h -- FP64 sin, cos, tan, asin and log used identically on the host and
+ * on gfx950.
+ *
+ * Why this exists: the tracking maps call transcendental functions (CSBEND's
+ * exact curvilinear drift uses asin, sin and cos per sub-step; RFCA uses sin and
+ * log).  The C library on the host and the device math library on the GPU do not
+ * return bit-identical results, so a particle near an aperture edge could be lost
+ * on one side and kept on the other.  Both the CPU checker and the HIP kernels
+ * therefore call these routines, which use only IEEE-754 double +, -, *, / and
+ * sqrt (all correctly rounded on both targets) in a fixed order.  Every file that
+ * includes this header must be compiled without floating-point contraction
+ * (-ffp-contract=off) and without fast-math.
+ *
+ * Methods (all published, none copied from a library):
+ *  - sin/cos: Cody & Waite argument reduction (W. J. Cody, W. Waite, "Software
+ *    Manual for the Elementary Functions", 1980) with pi/2 split into four
+ *    24-bit pieces plus a tail, so k*piece is exact for |k| < 2^29; the reduced
+ *    argument |r| <= ~pi/4 is fed to the Taylor series of sin (to r^21) and cos
+ *    (to r^20), evaluated by Horner's rule.
+ *  - asin: Taylor series sum_n (2n)!/(4^n (n!)^2 (2n+1)) x^(2n+1) for |x| <= 1/2
+ *    (10 terms for |x| <= 1/8, 40 terms above), and asin(x) = pi/2 - 2 asin(
+ *    sqrt((1-|x|)/2)) for |x| > 1/2.
+ *  - log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
+ *    s = (m-1)/(m+1), atanh series to s^25, plus e*ln2 split in hi/lo parts.
+ * Constants come from eh_math_consts.h, generated from exact rational arithmetic
+ * by gen_math_consts.py.
+ *
+ * Accuracy: a few ulp against correctly rounded results for |x| <= 2^29 (sin,
+ * cos); beyond 2^50 (or for non-finite x) sin and cos return NaN.
+ */
+#ifndef EH_MATH_H
+#define EH_MATH_H
+
+#include "eh_math_consts.h"
+
+#if defined(__HIPCC__)
+#define EH_FN static __host__ __device__ inline
+#else
+#define EH_FN static inline
+#endif
+
+#ifdef __cplusplus
+#include <cstdint>
+#else
+#include <stdint.h>
+#endif
+
+EH_FN uint64_t eh_bits(double x) {
+  uint64_t u;
+  __builtin_memcpy(&u, &x, sizeof u);
+  return u;
+}
+
+EH_FN double eh_from_bits(uint64_t u) {
+  double x;
+  __builtin_memcpy(&x, &u, sizeof x);
+  return x;
+}
+
+EH_FN double eh_fabs(double x) { return eh_from_bits(eh_bits(x) & 0x7fffffffffffffffULL); }
+
+EH_FN double eh_nan(void) { return eh_from_bits(0x7ff8000000000000ULL); }
+
+EH_FN double eh_sqrt(double x) { return __builtin_sqrt(x); }
+
+/* Taylor polynomial of sin(r) for |r| <= pi/4 */
+EH_FN double eh_sin_poly(double r) {
+  double z = r * r;
+  double p = EH_SIN_10;
+  p = EH_SIN_9 + z * p;
+  p = EH_SIN_8 + z * p;
+  p = EH_SIN_7 + z * p;
+  p = EH_SIN_6 + z * p;
+  p = EH_SIN_5 + z * p;
+  p = EH_SIN_4 + z * p;
+  p = EH_SIN_3 + z * p;
+  p = EH_SIN_2 + z * p;
+  p = EH_SIN_1 + z * p;
+  return r + (r * z) * p;
+}
+
+/* Taylor polynomial of cos(r) for |r| <= pi/4 */
+EH_FN double eh_cos_poly(double r) {
+  double z = r * r;
+  double p = EH_COS_10;
+  p = EH_COS_9 + z * p;
+  p = EH_COS_8 + z * p;
+  p = EH_COS_7 + z * p;
+  p = EH_COS_6 + z * p;
+  p = EH_COS_5 + z * p;
+  p = EH_COS_4 + z * p;
+  p = EH_COS_3 + z * p;
+  p = EH_COS_2 + z * p;
+  /* 1 - z/2 + z^2 p, with the z/2 term kept separate for accuracy */
+  return (1.0 - 0.5 * z) + (z * z) * p;
+}
+
+/* Reduce x to r = x - k*pi/2; returns the quadrant k mod 4, or -1 when x is
+ * outside the supported domain (non-finite or |x| >= 2^50). */
+EH_FN int eh_reduce_pio2(double x, double *r) {
+  if (!(eh_fabs(x) < 0x1p50)) {
+    *r = eh_nan();
+    return -1;
+  }
+  double kf = (x * EH_TWO_OVER_PI + 0x1.8p52) - 0x1.8p52;
+  double t = x - kf * EH_PIO2_1;
+  t = t - kf * EH_PIO2_2;
+  t = t - kf * EH_PIO2_3;
+  t = t - kf * EH_PIO2_4;
+  t = t - kf * EH_PIO2_5;
+  *r = t;
+  int64_t k = (int64_t)kf;
+  return (int)(k & 3);
+}
+
+EH_FN void eh_sincos(double x, double *s, double *c) {
+  double r;
+  int q = eh_reduce_pio2(x, &r);
+  if (q < 0) {
+    *s = *c = eh_nan();
+    return;
+  }
+  double ps = eh_sin_poly(r), pc = eh_cos_poly(r);
+  switch (q) {
+  case 0:
+    *s = ps;
+    *c = pc;
+    break;
+  case 1:
+    *s = pc;
+    *c = -ps;
+    break;
+  case 2:
+    *s = -ps;
+    *c = -pc;
+    break;
+  default:
+    *s = -pc;
+    *c = ps;
+    break;
+  }
+}
+
+EH_FN double eh_sin(double x) {
+  double s, c;
+  eh_sincos(x, &s, &c);
+  return s;
+}
+
+EH_FN double eh_cos(double x) {
+  double s, c;
+  eh_sincos(x, &s, &c);
+  return c;
+}
+
+EH_FN double eh_tan(double x) {
+  double s, c;
+  eh_sincos(x, &s, &c);
+  return s / c;
+}
+
+/* asin Taylor series; z = x*x, valid for |x| <= 1/8 */
+EH_FN double eh_asin_small(double x) {
+  double z = x * x;
+  double p = EH_ASIN_10;
+  p = EH_ASIN_9 + z * p;
+  p = EH_ASIN_8 + z * p;
+  p = EH_ASIN_7 + z * p;
+  p = EH_ASIN_6 + z * p;
+  p = EH_ASIN_5 + z * p;
+  p = EH_ASIN_4 + z * p;
+  p = EH_ASIN_3 + z * p;
+  p = EH_ASIN_2 + z * p;
+  p = EH_ASIN_1 + z * p;
+  return x + (x * z) * p;
+}
+
+/* asin Taylor series valid for |x| <= 1/2 (40 terms) */
+EH_FN double eh_asin_mid(double x) {
+  const double a[40] = {EH_ASIN_1,  EH_ASIN_2,  EH_ASIN_3,  EH_ASIN_4,  EH_ASIN_5,  EH_ASIN_6,  EH_ASIN_7,
+                        EH_ASIN_8,  EH_ASIN_9,  EH_ASIN_10, EH_ASIN_11, EH_ASIN_12, EH_ASIN_13, EH_ASIN_14,
+                        EH_ASIN_15, EH_ASIN_16, EH_ASIN_17, EH_ASIN_18, EH_ASIN_19, EH_ASIN_20, EH_ASIN_21,
+                        EH_ASIN_22, EH_ASIN_23, EH_ASIN_24, EH_ASIN_25, EH_ASIN_26, EH_ASIN_27, EH_ASIN_28,
+                        EH_ASIN_29, EH_ASIN_30, EH_ASIN_31, EH_ASIN_32, EH_ASIN_33, EH_ASIN_34, EH_ASIN_35,
+                        EH_ASIN_36, EH_ASIN_37, EH_ASIN_38, EH_ASIN_39, EH_ASIN_40};
+  double z = x * x;
+  double p = a[39];
+  for (int n = 38; n >= 0; n--)
+    p = a[n] + z * p;
+  return x + (x * z) * p;
+}
+
+EH_FN double eh_asin_half(double x) {
+  return eh_fabs(x) <= 0.125 ? eh_asin_small(x) : eh_asin_mid(x);
+}
+
+EH_FN double eh_asin(double x) {
+  double ax = eh_fabs(x);
+  if (ax <= 0.5)
+    return eh_asin_half(x);
+  if (!(ax <= 1.0))
+    return eh_nan();
+  double w = eh_sqrt((1.0 - ax) * 0.5);
+  double r = (EH_PIO2_HI - 2.0 * eh_asin_half(w)) + EH_PIO2_LO;
+  return x < 0 ? -r : r;
+}
+
+EH_FN double eh_log(double x) {
+  if (x != x || x < 0)
+    return eh_nan();
+  if (x == 0)
+    return -eh_from_bits(0x7ff0000000000000ULL);
+  uint64_t u = eh_bits(x);
+  int e = (int)((u >> 52) & 0x7ff);
+  if (e == 0x7ff)
+    return x; /* +inf */
+  if (e == 0) { /* subnormal: scale into the normal range */
+    x = x * 0x1p54;
+    u = eh_bits(x);
+    e = (int)((u >> 52) & 0x7ff) - 54;
+  }
+  e -= 1023;
+  /* mantissa in [1, 2) */
+  double m = eh_from_bits((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+  if (m > 0x1.6a09e667f3bcdp+0) { /* > sqrt(2): use m/2 in [sqrt(1/2), 1) */
+    m = m * 0.5;
+    e += 1;
+  }
+  double f = m - 1.0;
+  double s = f / (2.0 + f);
+  double z = s * s;
+  double p = EH_LOG_12;
+  p = EH_LOG_11 + z * p;
+  p = EH_LOG_10 + z * p;
+  p = EH_LOG_9 + z * p;
+  p = EH_LOG_8 + z * p;
+  p = EH_LOG_7 + z * p;
+  p = EH_LOG_6 + z * p;
+  p = EH_LOG_5 + z * p;
+  p = EH_LOG_4 + z * p;
+  p = EH_LOG_3 + z * p;
+  p = EH_LOG_2 + z * p;
+  p = EH_LOG_1 + z * p;
+  double lm = 2.0 * s + s * (z * p);
+  double de = (double)e;
+  return de * EH_LN2_HI + (lm + de * EH_LN2_LO);
+}
+
+#endif

@@ -1,0 +1,142 @@
+"""ctypes mirrors of the element structs in include/elegant_hip.h.
+
+Field names follow elegant's PARAMETER tables (src/track_data.c), so a lattice
+written with elegant's parameter names maps one-to-one.  Defaults are the
+reference defaults (src/track_data.c:1125, :1169, :1227, :1898, :579, :994).
+"""
+import ctypes as C
+
+NPROP = 10
+ID_INDEX = 6
+LOSS_PASS_INDEX = 7
+
+T_MARK, T_DRIF, T_EDRIFT, T_KQUAD, T_KSEXT, T_KOCT, T_CSBEND, T_RFCA, T_MAXAMP = range(9)
+TYPE_NAMES = {T_MARK: "MARK", T_DRIF: "DRIF", T_EDRIFT: "EDRIFT", T_KQUAD: "KQUAD", T_KSEXT: "KSEXT",
+              T_KOCT: "KOCT", T_CSBEND: "CSBEND", T_RFCA: "RFCA", T_MAXAMP: "MAXAMP"}
+TYPE_CODES = {v: k for k, v in TYPE_NAMES.items()}
+
+BEND_EDGE1_EFFECTS = 1
+BEND_EDGE2_EFFECTS = 2
+FID_TMEAN, FID_LIGHT = 0, 1
+
+C_MKS = 2.99792458e8
+ME_MEV = 0.51099895000
+
+i32 = C.c_int32
+dbl = C.c_double
+
+
+class EH_DRIFT(C.Structure):
+    _fields_ = [("length", dbl), ("order", i32), ("pad_", i32)]
+
+
+class EH_EDRIFT(C.Structure):
+    _fields_ = [("length", dbl)]
+
+
+class EH_KQUAD(C.Structure):
+    _fields_ = [(n, dbl) for n in ("length", "k1", "tilt", "dx", "dy", "dz", "fse", "xkick", "ykick",
+                                   "xKickCalibration", "yKickCalibration")] + \
+               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian")]
+
+
+class EH_KSEXT(C.Structure):
+    _fields_ = [(n, dbl) for n in ("length", "k2", "k1", "j1", "tilt", "dx", "dy", "dz", "fse", "xkick", "ykick",
+                                   "xKickCalibration", "yKickCalibration")] + \
+               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian")]
+
+
+class EH_KOCT(C.Structure):
+    _fields_ = [(n, dbl) for n in ("length", "k3", "tilt", "dx", "dy", "dz", "fse")] + \
+               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian")]
+
+
+class EH_CSBEND(C.Structure):
+    _fields_ = [(n, dbl) for n in ("length", "angle", "k1", "k2", "k3", "k4", "k5", "k6", "k7", "k8",
+                                   "e1", "e2", "tilt", "h1", "h2", "hgap", "fint",
+                                   "dx", "dy", "dz", "xKick", "yKick",
+                                   "fse", "fseDipole", "fseQuadrupole", "etilt",
+                                   "b1", "b2", "b3", "b4", "b5", "b6", "b7", "b8")] + \
+               [(n, i32) for n in ("nSlices", "nonlinear", "edge1_effects", "edge2_effects", "edge_order",
+                                   "integration_order", "expandHamiltonian", "use_bn", "expansionOrder",
+                                   "etiltSign", "malignMethod", "edgeFlags")]
+
+
+class EH_RFCA(C.Structure):
+    _fields_ = [(n, dbl) for n in ("length", "volt", "phase", "freq", "Q", "dx", "dy", "tReference")] + \
+               [(n, i32) for n in ("nKicks", "change_p0", "change_t", "end1Focus", "end2Focus", "standingWave",
+                                   "linearize", "lockPhase", "fiducial_mode", "fiducial_seen")] + \
+               [("phase_fiducial", dbl), ("t_fiducial", dbl)]
+
+
+class EH_MAXAMP(C.Structure):
+    _fields_ = [("x_max", dbl), ("y_max", dbl), ("elliptical", i32), ("exponent", i32)]
+
+
+class EH_ELEMENT_LIST(C.Structure):
+    _fields_ = [("type", i32), ("occurence", i32), ("name", C.c_char_p), ("p_elem", C.c_void_p)]
+
+
+class eh_track_stats(C.Structure):
+    _fields_ = [("particle_element_turns", dbl), ("kernel_ms", dbl), ("launches", C.c_long), ("n_alive", C.c_long)]
+
+
+STRUCT_OF = {T_DRIF: EH_DRIFT, T_EDRIFT: EH_EDRIFT, T_KQUAD: EH_KQUAD, T_KSEXT: EH_KSEXT, T_KOCT: EH_KOCT,
+             T_CSBEND: EH_CSBEND, T_RFCA: EH_RFCA, T_MAXAMP: EH_MAXAMP}
+
+# reference defaults (src/track_data.c)
+DEFAULTS = {
+    T_DRIF: dict(order=0),
+    T_KQUAD: dict(xKickCalibration=1.0, yKickCalibration=1.0, n_kicks=0, nSlices=1, integration_order=4),
+    T_KSEXT: dict(xKickCalibration=1.0, yKickCalibration=1.0, n_kicks=0, nSlices=1, integration_order=4),
+    T_KOCT: dict(n_kicks=4, nSlices=4, integration_order=4),
+    T_CSBEND: dict(fint=0.5, nSlices=4, nonlinear=1, edge1_effects=1, edge2_effects=1, edge_order=1,
+                   integration_order=4, etiltSign=1),
+    T_RFCA: dict(freq=500.0e6, tReference=-1.0),
+    T_MAXAMP: dict(exponent=2),
+}
+
+# elegant parameter names -> struct fields where they differ
+ALIASES = {
+    T_KQUAD: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "HKICK": "xkick", "VKICK": "ykick",
+              "HCALIBRATION": "xKickCalibration", "VCALIBRATION": "yKickCalibration",
+              "INTEGRATION_ORDER": "integration_order", "EXPAND_HAMILTONIAN": "expandHamiltonian"},
+    T_KSEXT: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "HKICK": "xkick", "VKICK": "ykick",
+              "HCALIBRATION": "xKickCalibration", "VCALIBRATION": "yKickCalibration",
+              "INTEGRATION_ORDER": "integration_order", "EXPAND_HAMILTONIAN": "expandHamiltonian"},
+    T_KOCT: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "INTEGRATION_ORDER": "integration_order",
+             "EXPAND_HAMILTONIAN": "expandHamiltonian"},
+    T_CSBEND: {"N_KICKS": "nSlices", "N_SLICES": "nSlices", "XKICK": "xKick", "YKICK": "yKick",
+               "FSE_DIPOLE": "fseDipole", "FSE_QUADRUPOLE": "fseQuadrupole",
+               "EDGE1_EFFECTS": "edge1_effects", "EDGE2_EFFECTS": "edge2_effects", "EDGE_ORDER": "edge_order",
+               "INTEGRATION_ORDER": "integration_order", "EXPAND_HAMILTONIAN": "expandHamiltonian",
+               "USE_BN": "use_bn", "EXPANSION_ORDER": "expansionOrder", "ETILT_SIGN": "etiltSign",
+               "MALIGN_METHOD": "malignMethod", "NONLINEAR": "nonlinear"},
+    T_RFCA: {"N_KICKS": "nKicks", "CHANGE_P0": "change_p0", "CHANGE_T": "change_t", "END1_FOCUS": "end1Focus",
+             "END2_FOCUS": "end2Focus", "STANDING_WAVE": "standingWave", "LINEARIZE": "linearize",
+             "LOCK_PHASE": "lockPhase", "T_REFERENCE": "tReference", "VOLT": "volt"},
+    T_MAXAMP: {},
+    T_DRIF: {},
+    T_EDRIFT: {},
+}
+
+
+def make_struct(etype, params):
+    """Build the ctypes struct for an element from elegant-style parameters."""
+    cls = STRUCT_OF[etype]
+    s = cls()
+    vals = dict(DEFAULTS.get(etype, {}))
+    alias = ALIASES.get(etype, {})
+    fields = {f[0] for f in cls._fields_}
+    for k, v in params.items():
+        key = alias.get(k.upper(), None)
+        if key is None:
+            key = k if k in fields else k.lower()
+            if key == "l":
+                key = "length"
+        if key not in fields:
+            raise KeyError("parameter %s not supported for %s" % (k, TYPE_NAMES[etype]))
+        vals[key] = v
+    for k, v in vals.items():
+        setattr(s, k, v)
+    return s

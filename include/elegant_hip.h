@@ -1,0 +1,260 @@
+/* elegant_hip.h -- C ABI of the MI355X (gfx950) element-tracking library.
+ *
+ * The library replaces the per-particle element tracking that elegant's
+ * do_tracking() loop performs on the CPU for DRIF, EDRIFT, KQUAD, KSEXT, KOCT,
+ * CSBEND and RFCA, plus the MAXAMP aperture and NaN sweeps that follow every
+ * element.  Particles live on the GPU as structure-of-arrays between calls
+ * (the residency model of elegant's HAVE_GPU hooks: gpuBaseInit at
+ * src/do_tracking.c:411-414, forceParticlesToCpu, gpuBaseDealloc at
+ * src/do_tracking.c:3172-3179).  Every element entry point keeps the argument
+ * meaning and order of the CPU routine it replaces, minus the particle array,
+ * exactly like the gpu_* hooks elegant calls (e.g. gpu_multipole_tracking2 at
+ * src/multipole.c:698-709).
+ *
+ * Element parameter structs below carry the PARAMETER-table names of
+ * src/track_data.c (KQUAD :1227, KSEXT :1125, KOCT :1169, CSBEND :1898,
+ * RFCA :579, MAXAMP :994, DRIF/EDRIFT) for the subset of fields the path
+ * honours; a value outside that subset makes the call return EH_ERR_UNSUPPORTED
+ * so that the caller can stay on its CPU routine.
+ *
+ * Coordinates follow src/track.h:81-98: rows of EH_NPROP doubles,
+ * (x, x', y, y', s, delta, particleID, lossPass, bunch, weight).
+ * All arithmetic is IEEE-754 double without contraction.
+ *
+ * Error convention: functions returning long return a particle count >= 0 or a
+ * negative EH_ERR_* code; eh_last_error() gives a message.
+ */
+#ifndef ELEGANT_HIP_H
+#define ELEGANT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EH_NPROP 10          /* totalPropertiesPerParticle, src/track_data.c:62 */
+#define EH_ID_INDEX 6        /* particleIDIndex, src/track.h:88 */
+#define EH_LOSS_PASS_INDEX 7 /* lossPassIndex */
+
+enum eh_error {
+  EH_OK = 0,
+  EH_ERR_UNSUPPORTED = -1, /* parameter value outside the supported subset */
+  EH_ERR_ARG = -2,         /* inconsistent argument (e.g. n_part != resident count) */
+  EH_ERR_HIP = -3,         /* HIP runtime failure */
+  EH_ERR_NOMEM = -4,
+  EH_ERR_STATE = -5 /* call sequence error (e.g. no beam resident) */
+};
+
+/* element type codes (subset of elegant's T_* codes, by meaning only) */
+enum eh_type {
+  EH_T_MARK = 0,
+  EH_T_DRIF = 1,
+  EH_T_EDRIFT = 2,
+  EH_T_KQUAD = 3,
+  EH_T_KSEXT = 4,
+  EH_T_KOCT = 5,
+  EH_T_CSBEND = 6,
+  EH_T_RFCA = 7,
+  EH_T_MAXAMP = 8
+};
+
+/* DRIF: src/track.h:1364-1368 (length, order; order 0 means run_setup default_order=2) */
+typedef struct {
+  double length;
+  int32_t order;
+  int32_t pad_;
+} EH_DRIFT;
+
+/* EDRIFT: exact drift, src/csbend.c:3792 */
+typedef struct {
+  double length;
+} EH_EDRIFT;
+
+/* KQUAD: src/track.h:2204-2228, defaults src/track_data.c:1227 */
+typedef struct {
+  double length, k1, tilt, dx, dy, dz, fse;
+  double xkick, ykick, xKickCalibration, yKickCalibration; /* HKICK, VKICK, H/VCALIBRATION */
+  int32_t n_kicks, nSlices;                               /* N_KICKS (deprecated alias path), N_SLICES */
+  int32_t integration_order;                              /* 2, 4 or 6 */
+  int32_t expandHamiltonian;
+} EH_KQUAD;
+
+/* KSEXT: src/track.h:2149-2166, defaults src/track_data.c:1125 */
+typedef struct {
+  double length, k2, k1, j1, tilt, dx, dy, dz, fse;
+  double xkick, ykick, xKickCalibration, yKickCalibration;
+  int32_t n_kicks, nSlices;
+  int32_t integration_order;
+  int32_t expandHamiltonian;
+} EH_KSEXT;
+
+/* KOCT: src/track.h:2171-2182, defaults src/track_data.c:1169 (N_KICKS=4) */
+typedef struct {
+  double length, k3, tilt, dx, dy, dz, fse;
+  int32_t n_kicks, nSlices;
+  int32_t integration_order;
+  int32_t expandHamiltonian;
+} EH_KOCT;
+
+/* CSBEND: src/track.h:2462-2502, defaults src/track_data.c:1898.
+ * N_KICKS is an alias of N_SLICES (src/track_data.c:1943-1944). */
+typedef struct {
+  double length, angle;
+  double k1, k2, k3, k4, k5, k6, k7, k8;
+  double e1, e2, tilt, h1, h2, hgap, fint; /* FINT (both edges) */
+  double dx, dy, dz, xKick, yKick;
+  double fse, fseDipole, fseQuadrupole, etilt;
+  double b1, b2, b3, b4, b5, b6, b7, b8; /* used when use_bn */
+  int32_t nSlices;
+  int32_t nonlinear;          /* default 1 */
+  int32_t edge1_effects;      /* default 1; 0 or 1 supported */
+  int32_t edge2_effects;
+  int32_t edge_order;         /* default 1; <=1 supported */
+  int32_t integration_order;  /* 2, 4, 6 */
+  int32_t expandHamiltonian;  /* 0 supported */
+  int32_t use_bn;
+  int32_t expansionOrder;     /* 0 = auto */
+  int32_t etiltSign;          /* default 1 */
+  int32_t malignMethod;       /* 0 supported */
+  int32_t edgeFlags;          /* EH_BEND_EDGE1/2 as determined for the beamline */
+} EH_CSBEND;
+
+#define EH_BEND_EDGE1_EFFECTS 1
+#define EH_BEND_EDGE2_EFFECTS 2
+
+/* fiducial modes, src/simple_rfca.c:35-48 */
+enum eh_fid_mode { EH_FID_TMEAN = 0, EH_FID_LIGHT = 1 };
+
+/* RFCA: src/track.h:1494-1508, defaults src/track_data.c:579 */
+typedef struct {
+  double length, volt, phase, freq, Q;
+  double dx, dy;
+  double tReference; /* default -1 */
+  int32_t nKicks;    /* 0 = matrix method */
+  int32_t change_p0, change_t, end1Focus, end2Focus, standingWave;
+  int32_t linearize, lockPhase;
+  int32_t fiducial_mode; /* eh_fid_mode */
+  /* state mutated by tracking, like the reference element (src/simple_rfca.c:460-475) */
+  int32_t fiducial_seen;
+  double phase_fiducial, t_fiducial;
+} EH_RFCA;
+
+/* MAXAMP: src/track_data.c:994 (rectangular, no open side) */
+typedef struct {
+  double x_max, y_max;
+  int32_t elliptical, exponent;
+} EH_MAXAMP;
+
+/* one beamline entry; mirrors ELEMENT_LIST's (name, type, p_elem) */
+typedef struct {
+  int32_t type; /* eh_type */
+  int32_t occurence;
+  const char *name;
+  void *p_elem; /* EH_DRIFT*, EH_KQUAD*, ... according to type */
+} EH_ELEMENT_LIST;
+
+/* particle charge/mass constants used on the path (electrons, CODATA 2018) */
+#define EH_C_MKS 2.99792458e8
+#define EH_ME_MEV 0.51099895000
+
+const char *eh_last_error(void);
+
+/* ---------------- device / residency --------------------------------------- */
+typedef struct eh_beam eh_beam;
+
+/* select the HIP device; replaces gpuBaseInit's device setup */
+int eh_init(int device);
+
+/* allocate resident storage for up to n_capacity particles */
+eh_beam *eh_beam_create(long n_capacity);
+void eh_beam_free(eh_beam *beam);
+
+/* copy rows [0, n_part) of an AoS array (EH_NPROP doubles per row) to the GPU;
+ * the row order becomes the reference order for later loss replay */
+long eh_beam_upload(eh_beam *beam, double **coord, long n_part);
+
+/* copy the beam back in the exact row order the CPU routines would have
+ * produced: survivors in [0, n_alive), lost particles above, with the loss
+ * pass in column EH_LOSS_PASS_INDEX.  Returns n_alive. */
+long eh_beam_download(eh_beam *beam, double **coord, long n_part);
+
+/* the same for a contiguous array of n_part rows of EH_NPROP doubles (the layout
+ * czarray_2d gives the reference's particle arrays, e.g. src/elegant.c:3183) */
+long eh_beam_upload_flat(eh_beam *beam, const double *flat, long n_part);
+long eh_beam_download_flat(eh_beam *beam, double *flat, long n_part);
+
+long eh_beam_n_alive(const eh_beam *beam);
+long eh_beam_n_total(const eh_beam *beam);
+
+/* pass number that subsequent losses are recorded with (do_tracking's i_pass) */
+void eh_beam_set_pass(eh_beam *beam, long pass);
+
+/* ---------------- per-element entry points (the plain drop-in path) -------- */
+/* Each returns the number of surviving particles, like the CPU routine. */
+
+/* replaces multipole_tracking2, src/multipole.c:660-677 (KQUAD/KSEXT/KOCT) */
+long eh_multipole_tracking2(eh_beam *beam, long n_part, const EH_ELEMENT_LIST *elem, double p_error, double Po,
+                            double z_start, const EH_MAXAMP *maxamp, long iSlice);
+
+/* replaces track_through_csbend, src/csbend.c:654-663 */
+long eh_track_through_csbend(eh_beam *beam, long n_part, const EH_CSBEND *csbend, double p_error, double Po,
+                             double z_start, const EH_MAXAMP *maxamp, long iSlice);
+
+/* replaces simple_rf_cavity, src/simple_rfca.c:264-293 (mutates rfca state) */
+long eh_simple_rf_cavity(eh_beam *beam, long np, EH_RFCA *rfca, double *P_central, double zEnd);
+
+/* replaces exactDrift, src/csbend.c:3792 */
+long eh_exactDrift(eh_beam *beam, long np, double length);
+
+/* replaces drift_beam / track_particles with drift_matrix, src/do_tracking.c:3754,
+ * src/matrix.c:212, src/compute_matrices.c:391 */
+long eh_drift_beam(eh_beam *beam, long np, double length, long order);
+
+/* replaces limit_amplitudes, src/limit_amplitudes.c:197-199 (openCode must be 0) */
+long eh_limit_amplitudes(eh_beam *beam, double xmax, double ymax, long np, double z, double Po, long extrapolate_z,
+                         long openCode);
+
+/* replaces removeInvalidParticles, src/limit_amplitudes.c:289-291 */
+long eh_removeInvalidParticles(eh_beam *beam, long np, double z, double Po);
+
+/* ---------------- whole tracking loop -------------------------------------- */
+enum eh_track_mode {
+  EH_MODE_PLAIN = 0, /* one kernel launch per element call, like the per-element hooks */
+  EH_MODE_FUSED = 1  /* runs of elements and turns fused into persistent launches */
+};
+
+typedef struct {
+  double particle_element_turns; /* surviving particles x counted elements, summed over passes */
+  double kernel_ms;              /* GPU time inside tracking kernels */
+  long launches;
+  long n_alive;
+} eh_track_stats;
+
+/* do_tracking over a beamline for n_passes (src/do_tracking.c:144, pass loop :498,
+ * element loop :829, post-element sweeps :2506-2524, :2675-2684).  The beam must be
+ * resident (eh_beam_upload).  Returns the number of surviving particles. */
+long eh_do_tracking(eh_beam *beam, EH_ELEMENT_LIST *beamline, long n_elements, long n_passes, double *P_central,
+                    int mode, int turns_per_launch, eh_track_stats *stats);
+
+/* host utility: CSBEND field-expansion table (computeCSBENDFieldCoefficients,
+ * src/csbend.c:175-652) for b[9], c[9] (index 0 = dipole term) and curvature h.
+ * Writes 11x11 row-major Fx, Fy; returns expansionOrder+1 or a negative code. */
+long eh_csbend_field_table(const double *b, const double *c, double h, int nonlinear, int expansionOrder, double *Fx,
+                           double *Fy, int *hasNormal, int *hasSkew);
+
+/* ---------------- diagnostics ---------------------------------------------- */
+/* reference row order after the given loss events (no reference counterpart:
+ * it reproduces the row permutation of the reference's swap-to-top compaction,
+ * e.g. src/limit_amplitudes.c:262-283, for testing) */
+long eh_replay_order(long n, const int32_t *lost_event, const int32_t *lost_id, long n_lost, int32_t *slot_at_row);
+
+/* GPU evaluation of sin, cos, asin, log, sqrt, division and tan for checking
+ * bit-identity with the host (no reference counterpart) */
+long eh_selftest_math(const double *x, const double *y, long n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

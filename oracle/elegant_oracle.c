@@ -1,0 +1,1364 @@
+/* elegant_oracle.c -- CPU restatement of elegant's per-particle element tracking.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP path is
+ * compared against; nothing in the product links, loads or calls it.  It keeps
+ * the reference's data layout (AoS rows of EH_NPROP doubles addressed through
+ * double **), its per-element passes (element routine, MAXAMP sweep, NaN sweep),
+ * its swap-to-top loss compaction and its operation order, so that with the same
+ * inputs it produces the same doubles as elegant compiled with gcc -O3 and no
+ * FMA contraction (Makefile.rules:218).
+ *
+ * Each function names the reference lines it restates.  Transcendentals come
+ * from eh_math.h by default (bit-identical with the GPU build); compiled with
+ * -DOR_LIBM they come from the C library instead, which is what an unmodified
+ * elegant build would call -- used to measure the drift between the two.
+ *
+ * Parameter subset: see include/elegant_hip.h.  Unsupported values set an error
+ * and make the routine return a negative count.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "elegant_hip.h"
+
+#ifdef OR_LIBM
+#define OR_SIN sin
+#define OR_COS cos
+#define OR_TAN tan
+#define OR_ASIN asin
+#define OR_LOG log
+#else
+#include "eh_math.h"
+#define OR_SIN eh_sin
+#define OR_COS eh_cos
+#define OR_TAN eh_tan
+#define OR_ASIN eh_asin
+#define OR_LOG eh_log
+#endif
+
+#include "eh_csbend_terms.h"
+
+#define PI 3.14159265358979323846
+#define PIx2 (2 * PI)
+#define PIo2 (PI / 2)
+#define c_mks EH_C_MKS
+#define particleMassMV EH_ME_MEV
+#define particleRelSign 1.0
+#define COORD_LIMIT 10.0 /* src/track.h:1251, src/track_data.c:52 */
+#define SLOPE_LIMIT 1.0  /* src/track.h:1250, src/track_data.c:51 */
+
+static double sqr(double x) { return x * x; }
+
+static char or_err[256];
+const char *or_last_error(void) { return or_err; }
+static long or_fail(const char *msg) {
+  snprintf(or_err, sizeof or_err, "%s", msg);
+  return EH_ERR_UNSUPPORTED;
+}
+
+/* swapParticles, src/elegant.c:2677-2684 */
+static void swapParticles(double *p1, double *p2) {
+  double buffer[EH_NPROP];
+  if (p1 == p2)
+    return;
+  memcpy(buffer, p1, sizeof buffer);
+  memcpy(p1, p2, sizeof buffer);
+  memcpy(p2, buffer, sizeof buffer);
+}
+
+static double SIGN(double x) { return x < 0 ? -1.0 : (x > 0 ? 1.0 : 0.0); }
+
+/* ------------------------------------------------------------------------- */
+/* limit_amplitudes, src/limit_amplitudes.c:197-287 (openCode 0) */
+long or_limit_amplitudes(double **coord, double xmax, double ymax, long np, double z, double Po, long extrapolate_z) {
+  long ip, itop, is_out;
+  double *part;
+  double dz, dzx, dzy;
+
+  if (xmax < 0 && ymax < 0)
+    return np;
+  itop = np - 1;
+  for (ip = 0; ip < np; ip++) {
+    part = coord[ip];
+    is_out = 0;
+    if (xmax > 0 && fabs(part[0]) > xmax)
+      is_out += 1;
+    if (ymax > 0 && fabs(part[2]) > ymax)
+      is_out += 2;
+    if (isinf(part[0]) || isinf(part[2]) || isnan(part[0]) || isnan(part[2]))
+      is_out += 4;
+    dz = 0;
+    if (is_out && !(is_out & 4) && extrapolate_z) {
+      dzx = dzy = -DBL_MAX;
+      if (is_out & 1 && part[1] != 0)
+        dzx = (part[0] - SIGN(part[1]) * xmax) / part[1];
+      if (is_out & 2 && part[3] != 0)
+        dzy = (part[2] - SIGN(part[3]) * ymax) / part[3];
+      if (dzx > dzy)
+        dz = -dzx;
+      else
+        dz = -dzy;
+      if (dz == -DBL_MAX)
+        dz = 0;
+    }
+    if (is_out) {
+      if (ip != itop)
+        swapParticles(coord[ip], coord[itop]);
+      coord[itop][0] += dz * coord[itop][1];
+      coord[itop][2] += dz * coord[itop][3];
+      coord[itop][4] = z + dz;
+      coord[itop][5] = Po * (1 + coord[itop][5]);
+      --itop;
+      --ip;
+      np--;
+    }
+  }
+  return np;
+}
+
+/* removeInvalidParticles, src/limit_amplitudes.c:289-330 */
+long or_removeInvalidParticles(double **coord, long np, double z, double Po) {
+  long ip, itop, is_out, ic;
+  double *part;
+  itop = np - 1;
+  for (ip = 0; ip < np; ip++) {
+    part = coord[ip];
+    is_out = 0;
+    for (ic = 0; ic < 6; ic++)
+      if (isnan(part[ic])) {
+        is_out = 1;
+        break;
+      }
+    if (part[5] <= -1)
+      is_out = 1;
+    if (is_out) {
+      swapParticles(coord[ip], coord[itop]);
+      coord[itop][4] = z;
+      coord[itop][5] = Po * (1 + coord[itop][5]);
+      --itop;
+      --ip;
+      np--;
+    }
+  }
+  return np;
+}
+
+/* exactDrift, src/csbend.c:3792-3816 */
+void or_exactDrift(double **part, long np, double length) {
+  long i;
+  double *coord;
+  for (i = 0; i < np; i++) {
+    coord = part[i];
+    coord[0] += coord[1] * length;
+    coord[2] += coord[3] * length;
+    coord[4] += length * sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* second-order transport map: drift_matrix (src/compute_matrices.c:391-410)
+ * applied by the generic order-2 loop of track_particles (src/matrix.c:334-381,
+ * TURBO_FASTMATTRACK=3 branch :373-385) */
+typedef struct {
+  double C[6], R[6][6], T[6][6][6];
+} or_vmatrix2;
+
+static void or_drift_matrix(or_vmatrix2 *M, double length, long order) {
+  memset(M, 0, sizeof *M);
+  M->C[4] = length;
+  M->R[0][0] = M->R[1][1] = M->R[2][2] = M->R[3][3] = M->R[4][4] = M->R[5][5] = 1;
+  M->R[0][1] = M->R[2][3] = length;
+  if (order > 1)
+    M->T[4][1][1] = M->T[4][3][3] = length / 2;
+}
+
+static void or_track_particles2(double **part, const or_vmatrix2 *M, long n_part) {
+  double temp[6], sum, sum1;
+  long i, j, k, i_part;
+  for (i_part = 0; i_part < n_part; i_part++) {
+    double *fin = part[i_part], *ini = part[i_part];
+    for (i = 5; i >= 0; i--) {
+      sum = M->C[i];
+      for (j = 5; j >= 0; j--) {
+        sum1 = M->R[i][j];
+        for (k = j; k >= 0; k--)
+          sum1 += M->T[i][j][k] * ini[k];
+        sum += sum1 * ini[j];
+      }
+      temp[i] = sum;
+    }
+    for (i = 5; i >= 0; i--)
+      fin[i] = temp[i];
+  }
+}
+
+/* drift_beam, src/do_tracking.c:3754-3767; also the DRIF element path
+ * (matrix tracking, src/do_tracking.c:1239-1311) with order 0 -> default_order 2 */
+void or_drift_beam(double **part, long np, double length, long order) {
+  or_vmatrix2 M;
+  if (order <= 0)
+    order = 2; /* run_setup default_order, src/elegant.nl:43 */
+  or_drift_matrix(&M, length, order > 2 ? 2 : order);
+  or_track_particles2(part, &M, np);
+}
+
+/* ------------------------------------------------------------------------- */
+/* misalignment method 0: src/malign_mat.c:257-267, :565-588; tilt_matrices.c:156-187 */
+static void offsetBeamCoordinatesForMisalignment(double **coord, long np, double dx, double dy, double dz) {
+  long ip;
+  double *part;
+  for (ip = np - 1; ip >= 0; ip--) {
+    part = coord[ip];
+    part[4] += dz * sqrt(1 + sqr(part[1]) + sqr(part[3]));
+    part[0] = part[0] - dx + dz * part[1];
+    part[2] = part[2] - dy + dz * part[3];
+  }
+}
+
+static void rotateBeamCoordinatesForMisalignment(double **part, long np, double angle) {
+  double x, xp, y, yp, *coord, sin_a, cos_a;
+  long i;
+  if (!angle || fabs(fabs(angle) - PIx2) < 1e-12)
+    return;
+  if (fabs(fabs(angle) - PI) < 1e-12) {
+    cos_a = -1;
+    sin_a = 0;
+  } else if (fabs(angle - PIo2) < 1e-12) {
+    cos_a = 0;
+    sin_a = 1;
+  } else if (fabs(angle + PIo2) < 1e-12) {
+    cos_a = 0;
+    sin_a = -1;
+  } else {
+    cos_a = OR_COS(angle);
+    sin_a = OR_SIN(angle);
+  }
+  for (i = 0; i < np; i++) {
+    coord = part[i];
+    x = coord[0];
+    xp = coord[1];
+    y = coord[2];
+    yp = coord[3];
+    coord[0] = x * cos_a + y * sin_a;
+    coord[2] = -x * sin_a + y * cos_a;
+    coord[1] = xp * cos_a + yp * sin_a;
+    coord[3] = -xp * sin_a + yp * cos_a;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* aperture data for in-element checks: setupMultApertureData / checkMultAperture,
+ * src/multipole.c:1690-1798, rectangular MAXAMP only */
+typedef struct {
+  double xMax, yMax, reverseTilt, cs0, cs1;
+} or_apdata;
+
+static void or_setup_ap(or_apdata *ap, double reverseTilt, const EH_MAXAMP *maxamp) {
+  ap->reverseTilt = reverseTilt;
+  ap->cs0 = OR_COS(reverseTilt);
+  ap->cs1 = OR_SIN(reverseTilt);
+  ap->xMax = ap->yMax = 0;
+  if (maxamp) {
+    ap->xMax = maxamp->x_max;
+    ap->yMax = maxamp->y_max;
+  }
+}
+
+static long or_check_ap(double xInput, double yInput, const or_apdata *ap) {
+  double x = xInput - 0.0, y = yInput - 0.0; /* xCen = yCen = 0 */
+  if (ap->reverseTilt) {
+    double x0 = x, y0 = y;
+    x = x0 * ap->cs0 + y0 * ap->cs1;
+    y = -x0 * ap->cs1 + y0 * ap->cs0;
+  }
+  if ((ap->xMax > 0 && fabs(x) > ap->xMax) || (ap->yMax > 0 && fabs(y) > ap->yMax))
+    return 0;
+  return 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* multipole kicks: expansion_coefficients (src/multipole.h:76-145) and
+ * apply_canonical_multipole_kicks_ret/_noret (src/multipole.h:391-445) */
+static double dblfactorial(long n) {
+  double prod = 1;
+  while (n > 1)
+    prod *= n--;
+  return prod;
+}
+
+static void or_coefs(long n, double *coef) {
+  long i;
+  if (n == 0) {
+    coef[0] = 1;
+  } else if (n == 1) {
+    coef[0] = 1;
+    coef[1] = 1;
+  } else if (n == 2) {
+    coef[0] = 1. / 2;
+    coef[1] = 1;
+    coef[2] = -1. / 2;
+  } else {
+    for (i = 0; i <= n; i++)
+      coef[i] = (((i / 2) & 1) ? -1.0 : 1.0) / (dblfactorial(i) * dblfactorial(n - i));
+  }
+}
+
+static void or_kick(double *qx, double *qy, const double *xpow, const double *ypow, long order, double KnL, long skew) {
+  double coef[16], sum_Fx = 0, sum_Fy = 0, tmp;
+  long i;
+  or_coefs(order, coef);
+  for (i = 0; i <= order; i++) {
+    double f = coef[i] * xpow[order - i] * ypow[i];
+    if (i & 1)
+      sum_Fx += f;
+    else
+      sum_Fy += f;
+  }
+  if (skew) {
+    tmp = sum_Fx;
+    sum_Fx = sum_Fy;
+    sum_Fy = tmp;
+    sum_Fx = -sum_Fx;
+  }
+  *qx -= KnL * sum_Fy;
+  *qy += KnL * sum_Fx;
+}
+
+static void fillPowerArray(double x, double *xpow, long order) {
+  long i;
+  xpow[0] = 1;
+  for (i = 1; i <= order; i++)
+    xpow[i] = xpow[i - 1] * x;
+}
+
+/* convertSlopesToMomenta / convertMomentaToSlopes, src/multipole.h:448-486 */
+static int slopes_to_momenta(double *qx, double *qy, double xp, double yp, double delta, int expand) {
+  if (expand) {
+    *qx = (1 + delta) * xp;
+    *qy = (1 + delta) * yp;
+  } else {
+    double denom = sqrt(1 + sqr(xp) + sqr(yp));
+    *qx = (1 + delta) * xp / denom;
+    *qy = (1 + delta) * yp / denom;
+  }
+  return 1;
+}
+
+static int momenta_to_slopes(double *xp, double *yp, double qx, double qy, double delta, int expand) {
+  if (expand) {
+    *xp = qx / (1 + delta);
+    *yp = qy / (1 + delta);
+  } else {
+    double denom = sqr(1 + delta) - sqr(qx) - sqr(qy);
+    if (denom <= 0)
+      return 0;
+    denom = sqrt(denom);
+    *xp = qx / denom;
+    *yp = qy / denom;
+  }
+  return 1;
+}
+
+#define BETA 1.25992104989487316477
+static const double driftFrac2[2] = {0.5, 0.5};
+static const double kickFrac2[2] = {1.0, 0.0};
+static const double driftFrac4[4] = {0.5 / (2 - BETA), (1 - BETA) / (2 - BETA) / 2, (1 - BETA) / (2 - BETA) / 2,
+                                     0.5 / (2 - BETA)};
+static const double kickFrac4[4] = {1. / (2 - BETA), -BETA / (2 - BETA), 1 / (2 - BETA), 0};
+static const double driftFrac6[8] = {0.39225680523878,      0.5100434119184585, -0.47105338540975655,
+                                     0.0687531682525181,    0.0687531682525181, -0.47105338540975655,
+                                     0.5100434119184585,    0.39225680523878};
+static const double kickFrac6[8] = {0.784513610477560,  0.235573213359357,  -1.17767998417887, 1.3151863206839063,
+                                    -1.17767998417887, 0.235573213359357, 0.784513610477560,  0};
+
+static int or_integrator_tables(long order, const double **drift, const double **kick, long *nsub) {
+  switch (order) {
+  case 2:
+    *drift = driftFrac2;
+    *kick = kickFrac2;
+    *nsub = 2;
+    return 1;
+  case 4:
+    *drift = driftFrac4;
+    *kick = kickFrac4;
+    *nsub = 4;
+    return 1;
+  case 6:
+    *drift = driftFrac6;
+    *kick = kickFrac6;
+    *nsub = 8;
+    return 1;
+  }
+  return 0;
+}
+
+/* integrate_kick_multipole_ordn, src/multipole.c:1099-1456, for i_part = -1, no
+ * radiation, no error/edge/steering multipoles, non-radial */
+static int or_integrate_kick_multipole(double *coord, double dx, double dy, double xkick, double ykick, double Po,
+                                       const long *order, const double *KnL, const short *skew, long n_parts,
+                                       double drift, long integration_order, const or_apdata *apData, double *dzLoss,
+                                       int expand) {
+  double qx, qy, s, dp;
+  double x, y, xp, yp, xpow[16], ypow[16], KnLp[3];
+  long i_kick, step, iOrder, maxOrder, nSubsteps, i;
+  double dsh;
+  const double *driftFrac, *kickFrac;
+
+  or_integrator_tables(integration_order, &driftFrac, &kickFrac, &nSubsteps);
+  drift = drift / n_parts;
+  xkick = xkick / n_parts;
+  ykick = ykick / n_parts;
+
+  x = coord[0];
+  xp = coord[1];
+  y = coord[2];
+  yp = coord[3];
+  s = 0;
+  dp = coord[5];
+
+  if (isnan(x) || isnan(xp) || isnan(y) || isnan(yp))
+    return 0;
+  if (fabs(x) > COORD_LIMIT || fabs(y) > COORD_LIMIT || fabs(xp) > SLOPE_LIMIT || fabs(yp) > SLOPE_LIMIT)
+    return 0;
+
+  slopes_to_momenta(&qx, &qy, xp, yp, dp, expand);
+  maxOrder = order[0] > (order[1] > order[2] ? order[1] : order[2]) ? order[0] : (order[1] > order[2] ? order[1] : order[2]);
+  for (i = 0; i < 3; i++)
+    KnLp[i] = KnL[i] / n_parts;
+
+  if (!momenta_to_slopes(&xp, &yp, qx, qy, dp, expand))
+    return 0;
+
+  *dzLoss = 0;
+  for (i_kick = 0; i_kick < n_parts; i_kick++) {
+    if (!or_check_ap(x + dx, y + dy, apData)) {
+      coord[0] = x;
+      coord[2] = y;
+      return 0;
+    }
+    for (step = 0; step < nSubsteps; step++) {
+      if (drift) {
+        dsh = drift * driftFrac[step];
+        x += xp * dsh;
+        y += yp * dsh;
+        if (expand)
+          s += dsh * (1 + (sqr(xp) + sqr(yp)) / 2);
+        else
+          s += dsh * sqrt(1 + sqr(xp) + sqr(yp));
+        *dzLoss += dsh;
+      }
+      if (!kickFrac[step])
+        break;
+      fillPowerArray(x, xpow, maxOrder);
+      fillPowerArray(y, ypow, maxOrder);
+      for (iOrder = 0; iOrder < 3; iOrder++)
+        if (KnL[iOrder])
+          or_kick(&qx, &qy, xpow, ypow, order[iOrder], KnLp[iOrder] * kickFrac[step], skew[iOrder]);
+      if (xkick)
+        or_kick(&qx, &qy, xpow, ypow, 0, -xkick * kickFrac[step], 0);
+      if (ykick)
+        or_kick(&qx, &qy, xpow, ypow, 0, -ykick * kickFrac[step], 1);
+      if (!momenta_to_slopes(&xp, &yp, qx, qy, dp, expand))
+        return 0;
+    }
+  }
+  if (!or_check_ap(x + dx, y + dy, apData)) {
+    coord[0] = x;
+    coord[2] = y;
+    return 0;
+  }
+  if (!momenta_to_slopes(&xp, &yp, qx, qy, dp, expand))
+    return 0;
+  coord[0] = x;
+  coord[1] = xp;
+  coord[2] = y;
+  coord[3] = yp;
+  coord[4] += s;
+  coord[5] = dp;
+  if (isnan(x) || isnan(xp) || isnan(y) || isnan(yp))
+    return 0;
+  if (fabs(x) > COORD_LIMIT || fabs(y) > COORD_LIMIT || fabs(xp) > SLOPE_LIMIT || fabs(yp) > SLOPE_LIMIT)
+    return 0;
+  return 1;
+}
+
+/* multipole_tracking2, src/multipole.c:660-1084 (KQUAD/KSEXT/KOCT, iSlice = -1,
+ * malignMethod 0, no bore/B, no radiation, no multipole files, no fringe) */
+long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIST *elem, double Po, double z_start,
+                            const EH_MAXAMP *maxamp) {
+  double KnL[3] = {0, 0, 0};
+  long order[3] = {0, 0, 0};
+  short skew[3] = {0, 0, 0};
+  double dx = 0, dy = 0, dz = 0, tilt = 0, xkick = 0, ykick = 0, drift = 0, dzLoss = 0;
+  long nSlices = 0, n_kicks = 0, integ_order = 0, i_part, i_top;
+  int expand = 0;
+  or_apdata ap;
+
+  switch (elem->type) {
+  case EH_T_KQUAD: {
+    const EH_KQUAD *q = (const EH_KQUAD *)elem->p_elem;
+    nSlices = q->nSlices;
+    n_kicks = q->n_kicks;
+    expand = q->expandHamiltonian;
+    order[0] = 1;
+    KnL[0] = q->k1 * q->length * (1 + q->fse);
+    drift = q->length;
+    tilt = q->tilt;
+    dx = q->dx;
+    dy = q->dy;
+    dz = q->dz;
+    xkick = q->xkick * q->xKickCalibration;
+    ykick = q->ykick * q->yKickCalibration;
+    integ_order = q->integration_order;
+    break;
+  }
+  case EH_T_KSEXT: {
+    const EH_KSEXT *q = (const EH_KSEXT *)elem->p_elem;
+    n_kicks = q->n_kicks;
+    nSlices = q->nSlices;
+    expand = q->expandHamiltonian;
+    order[0] = 2;
+    KnL[0] = q->k2 * q->length * (1 + q->fse);
+    drift = q->length;
+    tilt = q->tilt;
+    dx = q->dx;
+    dy = q->dy;
+    dz = q->dz;
+    xkick = q->xkick * q->xKickCalibration;
+    ykick = q->ykick * q->yKickCalibration;
+    integ_order = q->integration_order;
+    if (q->k1) {
+      KnL[1] = q->k1 * q->length;
+      order[1] = 1;
+    }
+    if (q->j1) {
+      KnL[2] = q->j1 * q->length;
+      order[2] = 1;
+      skew[2] = 1;
+    }
+    break;
+  }
+  case EH_T_KOCT: {
+    const EH_KOCT *q = (const EH_KOCT *)elem->p_elem;
+    n_kicks = q->n_kicks;
+    nSlices = q->nSlices;
+    expand = q->expandHamiltonian;
+    order[0] = 3;
+    KnL[0] = q->k3 * q->length * (1 + q->fse);
+    drift = q->length;
+    tilt = q->tilt;
+    dx = q->dx;
+    dy = q->dy;
+    dz = q->dz;
+    integ_order = q->integration_order;
+    break;
+  }
+  default:
+    return or_fail("multipole_tracking2: element type not supported");
+  }
+  if (integ_order != 2 && integ_order != 4 && integ_order != 6)
+    return or_fail("multipole integration_order must be 2, 4, or 6");
+  if (maxamp && maxamp->elliptical)
+    return or_fail("elliptical MAXAMP not supported");
+
+  i_top = n_part - 1;
+  if (n_kicks <= 0) {
+    if (nSlices <= 0)
+      return or_fail("N_KICKS<=0 and N_SLICES<=0 in multipole tracking");
+  } else {
+    if (integ_order > 2) {
+      if ((nSlices = (long)ceil(n_kicks / (1.0 * integ_order))) < 1)
+        nSlices = 1;
+      n_kicks = nSlices * integ_order;
+    } else
+      nSlices = n_kicks;
+  }
+
+  or_setup_ap(&ap, -tilt, maxamp);
+
+  /* offsetParticlesForMisalignment(0, ..., face 1), src/malign_mat.c:580-585 */
+  if (dx || dy || dz)
+    offsetBeamCoordinatesForMisalignment(particle, n_part, dx, dy, dz);
+  if (tilt)
+    rotateBeamCoordinatesForMisalignment(particle, n_part, tilt + 0.0);
+
+  for (i_part = 0; i_part <= i_top; i_part++) {
+    if (!or_integrate_kick_multipole(particle[i_part], dx, dy, xkick, ykick, Po, order, KnL, skew, nSlices, drift,
+                                     integ_order, &ap, &dzLoss, expand)) {
+      swapParticles(particle[i_part], particle[i_top]);
+      particle[i_top][4] = z_start + dzLoss;
+      particle[i_top][5] = Po * (1 + particle[i_top][5]);
+      i_top--;
+      i_part--;
+      continue;
+    }
+  }
+
+  /* face 2, src/multipole.c:1069-1070, src/malign_mat.c:586-590: all n_part rows */
+  if (dx || dy || dz || tilt) {
+    if (tilt)
+      rotateBeamCoordinatesForMisalignment(particle, n_part, -(tilt + 0.0));
+    if (dx || dy || dz)
+      offsetBeamCoordinatesForMisalignment(particle, n_part, -dx, -dy, -dz);
+  }
+  return i_top + 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* CSBEND field expansion (computeCSBENDFieldCoefficients, src/csbend.c:175-652):
+ * coefficients evaluated term by term from the generated table, whose entries
+ * equal the reference's formulas */
+typedef struct {
+  double Fx[11][11], Fy[11][11];
+  long expansionOrder1;
+  int hasSkew, hasNormal;
+} or_csbend_fields;
+
+static long or_field_coefficients(or_csbend_fields *F, const double *b, const double *c, double h1, long nonlinear,
+                                  long expansionOrder) {
+  long i, t;
+  double h[20];
+  if (expansionOrder == 0) {
+    for (i = 8; i >= 0; i--)
+      if (b[i] || c[i])
+        break;
+    if ((expansionOrder = i + 2) < 4)
+      expansionOrder = 4;
+  }
+  F->expansionOrder1 = expansionOrder + 1;
+  if (F->expansionOrder1 > 11)
+    return or_fail("expansion order >10 for CSBEND");
+  F->hasSkew = F->hasNormal = 0;
+  for (i = 0; i < 9; i++) {
+    if (b[i])
+      F->hasNormal = 1;
+    if (c[i])
+      F->hasSkew = 1;
+  }
+  memset(F->Fx, 0, sizeof F->Fx);
+  memset(F->Fy, 0, sizeof F->Fy);
+  h[0] = 1;
+  for (i = 1; i < 20; i++)
+    h[i] = h[i - 1] * h1;
+  F->Fx[0][0] = c[0];
+  F->Fy[0][0] = 1 - b[0];
+  for (t = 0; t < EH_CSBEND_NTERMS; t++) {
+    int ii = eh_csbend_terms[t].i, jj = eh_csbend_terms[t].j;
+    /* the linear block (entries of degree 1) is set regardless of NONLINEAR */
+    if (!nonlinear && ii + jj > 1)
+      continue;
+    if (ii + jj >= 11)
+      continue;
+    const double *v = eh_csbend_terms[t].kind ? c + 1 : b + 1;
+    double term = ((eh_csbend_terms[t].num * h[eh_csbend_terms[t].hpow]) * v[eh_csbend_terms[t].m]) /
+                  eh_csbend_terms[t].den;
+    double *dst = eh_csbend_terms[t].which ? &F->Fy[ii][jj] : &F->Fx[ii][jj];
+    /* first term of an entry initialises it (the reference writes a single expression) */
+    if (t == 0 || eh_csbend_terms[t - 1].which != eh_csbend_terms[t].which || eh_csbend_terms[t - 1].i != ii ||
+        eh_csbend_terms[t - 1].j != jj)
+      *dst = term;
+    else
+      *dst += term;
+  }
+  return F->expansionOrder1;
+}
+
+/* computeCSBENDFields, src/csbend.c:99-172 */
+static void or_csbend_fields_eval(const or_csbend_fields *F, double *Fx, double *Fy, double x, double y) {
+  double yp[11], sumFx = 0, sumFy = 0, xt = 1;
+  long i, j;
+  if (!F->hasSkew && !F->hasNormal) {
+    *Fx = 0;
+    *Fy = 1;
+    return;
+  }
+  yp[0] = 1;
+  for (i = 1; i < F->expansionOrder1; i++)
+    yp[i] = yp[i - 1] * y;
+  if (F->hasSkew) {
+    for (i = 0; i < F->expansionOrder1; i++) {
+      for (j = 0; j < F->expansionOrder1 - i; j++) {
+        sumFx += F->Fx[i][j] * xt * yp[j];
+        sumFy += F->Fy[i][j] * xt * yp[j];
+      }
+      xt *= x;
+    }
+  } else {
+    for (i = 0; i < F->expansionOrder1; i++) {
+      for (j = 0; j < F->expansionOrder1 - i; j++) {
+        if (j & 1)
+          sumFx += F->Fx[i][j] * xt * yp[j];
+        else
+          sumFy += F->Fy[i][j] * xt * yp[j];
+      }
+      xt *= x;
+    }
+  }
+  *Fx = sumFx;
+  *Fy = sumFy;
+}
+
+long or_csbend_field_table(const double *b, const double *c, double h, int nonlinear, int expansionOrder, double *Fx,
+                           double *Fy) {
+  or_csbend_fields F;
+  long r = or_field_coefficients(&F, b, c, h, nonlinear, expansionOrder);
+  if (r < 0)
+    return r;
+  memcpy(Fx, F.Fx, sizeof F.Fx);
+  memcpy(Fy, F.Fy, sizeof F.Fy);
+  return r;
+}
+
+/* integrate_csbend_ordn, src/csbend.c:1459-1720 (no radiation, no reference
+ * trajectory correction, iSlice = -1) */
+static long or_integrate_csbend(double *Qf, const double *Qi, double s, long n, double rho0, double rho_actual,
+                                double *dz_lost, const or_apdata *apData, long integration_order,
+                                const or_csbend_fields *F) {
+  long i, j, nSubsteps;
+  double factor, f, phi, ds, dsh, dist, Fx, Fy, x, y, sine, cosi, tang, sin_phi, cos_phi;
+  const double *driftFrac, *kickFrac;
+  or_integrator_tables(integration_order, &driftFrac, &kickFrac, &nSubsteps);
+#define X Qf[0]
+#define QX Qf[1]
+#define Y Qf[2]
+#define QY Qf[3]
+#define DPoP Qf[5]
+  memcpy(Qf, Qi, sizeof(*Qi) * 6);
+  dist = 0;
+  s /= n;
+  *dz_lost = 0;
+  for (i = 0; i < n; i++) {
+    if (!or_check_ap(X, Y, apData))
+      return 0;
+    for (j = 0; j < nSubsteps; j++) {
+      dsh = s * driftFrac[j];
+      f = sqr(1 + DPoP) - sqr(QY);
+      if (f <= 0)
+        return 0;
+      f = sqrt(f);
+      if (fabs(QX / f) > 1)
+        return 0;
+      sin_phi = QX / f;
+      phi = OR_ASIN(sin_phi);
+      sine = OR_SIN(dsh / rho0 + phi);
+      cosi = OR_COS(dsh / rho0 + phi);
+      if (cosi == 0)
+        return 0;
+      tang = sine / cosi;
+      cos_phi = OR_COS(phi);
+      QX = f * sine;
+      factor = (rho0 + X) * cos_phi / f * (tang - sin_phi / cos_phi);
+      Y += QY * factor;
+      dist += factor * (1 + DPoP);
+      *dz_lost += dsh;
+      f = cos_phi / cosi;
+      X = rho0 * (f - 1) + f * X;
+      if (kickFrac[j] == 0)
+        break;
+      ds = s * kickFrac[j];
+      x = X;
+      y = Y;
+      or_csbend_fields_eval(F, &Fx, &Fy, x, y);
+      QX += -ds * (1 + X / rho0) * Fy / rho_actual;
+      QY += ds * (1 + X / rho0) * Fx / rho_actual;
+    }
+  }
+  *dz_lost = n * s;
+  if (!or_check_ap(X, Y, apData))
+    return 0;
+  Qf[4] += dist;
+  return 1;
+#undef X
+#undef QX
+#undef Y
+#undef QY
+#undef DPoP
+}
+
+/* convertToDipoleCanonicalCoordinates / convertFrom..., src/csbend.c:1436-1454 */
+static void toDipoleCanonical(double *Qi) {
+  double f = (1 + Qi[5]) / sqrt(1 + sqr(Qi[1]) + sqr(Qi[3]));
+  Qi[1] *= f;
+  Qi[3] *= f;
+}
+
+static void fromDipoleCanonical(double *Qi) {
+  double f = 1 / sqrt(sqr(1 + Qi[5]) - sqr(Qi[1]) - sqr(Qi[3]));
+  Qi[1] *= f;
+  Qi[3] *= f;
+}
+
+/* track_through_csbend, src/csbend.c:654-1434 (malignMethod 0, ETILT 0, no
+ * radiation, EDGE*_EFFECTS 0 or 1 with EDGE_ORDER 1, no reference correction) */
+long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend, double Po, double z_start,
+                             const EH_MAXAMP *maxamp) {
+  double b[9], c[9], h, rho0, rho_actual, fse, angle, e1, e2, tilt, etilt, ttilt, cos_ttilt, sin_ttilt;
+  double psi1, psi2, Kg1, Kg2, dxi, dyi, dzi, dxf, dyf, dzf, dz_lost, rho, delta_xp;
+  double x, xp, y, yp, s, dp, Qi[6], Qf[6], *coord;
+  long i_part, i_top, j, particle_lost;
+  or_csbend_fields F;
+  or_apdata ap;
+
+  if (csbend->angle == 0) {
+    or_exactDrift(part, n_part, csbend->length);
+    return n_part;
+  }
+  if (csbend->integration_order != 2 && csbend->integration_order != 4 && csbend->integration_order != 6)
+    return or_fail("CSBEND integration_order is invalid");
+  if (csbend->malignMethod != 0 || csbend->etilt != 0 || csbend->expandHamiltonian || csbend->edge_order > 1)
+    return or_fail("CSBEND option not supported");
+  if ((csbend->edge1_effects != 0 && csbend->edge1_effects != 1) ||
+      (csbend->edge2_effects != 0 && csbend->edge2_effects != 1))
+    return or_fail("CSBEND edge method not supported");
+  if (maxamp && maxamp->elliptical)
+    return or_fail("elliptical MAXAMP not supported");
+
+  rho0 = csbend->length / csbend->angle;
+  b[0] = 0;
+  if (csbend->use_bn) {
+    b[1] = csbend->b1;
+    b[2] = csbend->b2;
+    b[3] = csbend->b3;
+    b[4] = csbend->b4;
+    b[5] = csbend->b5;
+    b[6] = csbend->b6;
+    b[7] = csbend->b7;
+    b[8] = csbend->b8;
+  } else {
+    b[1] = csbend->k1 * rho0;
+    b[2] = csbend->k2 * rho0;
+    b[3] = csbend->k3 * rho0;
+    b[4] = csbend->k4 * rho0;
+    b[5] = csbend->k5 * rho0;
+    b[6] = csbend->k6 * rho0;
+    b[7] = csbend->k7 * rho0;
+    b[8] = csbend->k8 * rho0;
+  }
+  for (j = 0; j < 9; j++)
+    c[j] = 0;
+  b[1] *= (1 + csbend->fse + csbend->fseQuadrupole) / (1 + csbend->fse + csbend->fseDipole);
+  c[1] *= (1 + csbend->fse + csbend->fseQuadrupole) / (1 + csbend->fse + csbend->fseDipole);
+  b[2] *= (1 + csbend->fse) / (1 + csbend->fse + csbend->fseDipole);
+  c[2] *= (1 + csbend->fse) / (1 + csbend->fse + csbend->fseDipole);
+  b[0] = csbend->xKick / csbend->angle;
+  c[0] = csbend->yKick / csbend->angle;
+
+  if (csbend->angle < 0) {
+    angle = -csbend->angle;
+    e1 = -csbend->e1;
+    e2 = -csbend->e2;
+    etilt = csbend->etilt * csbend->etiltSign;
+    tilt = csbend->tilt + PI;
+    rho0 = csbend->length / angle;
+    for (j = 1; j < 9; j += 2) {
+      b[j] *= -1;
+      c[j] *= -1;
+    }
+  } else {
+    angle = csbend->angle;
+    e1 = csbend->e1;
+    e2 = csbend->e2;
+    etilt = csbend->etilt * csbend->etiltSign;
+    tilt = csbend->tilt;
+    rho0 = csbend->length / angle;
+  }
+  or_setup_ap(&ap, -tilt, maxamp);
+  if (fabs(rho0) > 1e6)
+    return or_fail("CSBEND with radius > 1e6 not supported");
+
+  fse = csbend->fse + csbend->fseDipole;
+  h = 1 / rho0;
+  if (fabs(fse + 1) < 1e-12)
+    fse = -1 + 1e-12;
+  rho_actual = 1 / ((1 + fse) * h);
+  Kg1 = 2 * csbend->hgap * csbend->fint * SIGN(rho0);
+  psi1 = Kg1 / fabs(rho_actual) / OR_COS(e1) * (1 + sqr(OR_SIN(e1)));
+  Kg2 = 2 * csbend->hgap * csbend->fint * SIGN(rho0);
+  psi2 = Kg2 / fabs(rho_actual) / OR_COS(e2) * (1 + sqr(OR_SIN(e2)));
+  if (csbend->length < 0) {
+    psi1 *= -1;
+    psi2 *= -1;
+  }
+  if (or_field_coefficients(&F, b, c, h, csbend->nonlinear, csbend->expansionOrder) < 0)
+    return EH_ERR_UNSUPPORTED;
+
+  ttilt = tilt + etilt;
+  if (ttilt == 0) {
+    cos_ttilt = 1;
+    sin_ttilt = 0;
+  } else if (fabs(fabs(ttilt) - PI) < 1e-12) {
+    cos_ttilt = -1;
+    sin_ttilt = 0;
+  } else if (fabs(ttilt - PIo2) < 1e-12) {
+    cos_ttilt = 0;
+    sin_ttilt = 1;
+  } else if (fabs(ttilt + PIo2) < 1e-12) {
+    cos_ttilt = 0;
+    sin_ttilt = -1;
+  } else {
+    cos_ttilt = OR_COS(ttilt);
+    sin_ttilt = OR_SIN(ttilt);
+  }
+  /* computeEtiltCentroidOffset with etilt = 0 gives zero offsets */
+  dxi = -csbend->dx;
+  dzi = csbend->dz;
+  dyi = -csbend->dy;
+  dxf = csbend->dx * OR_COS(csbend->angle) + csbend->dz * OR_SIN(csbend->angle);
+  dzf = csbend->dx * OR_SIN(csbend->angle) - csbend->dz * OR_COS(csbend->angle);
+  dyf = csbend->dy;
+
+  i_top = n_part - 1;
+  for (i_part = 0; i_part <= i_top; i_part++) {
+    coord = part[i_part];
+    coord[4] += dzi * sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
+    coord[0] = coord[0] + dxi + dzi * coord[1];
+    coord[2] = coord[2] + dyi + dzi * coord[3];
+    x = coord[0] * cos_ttilt + coord[2] * sin_ttilt;
+    y = -coord[0] * sin_ttilt + coord[2] * cos_ttilt;
+    xp = coord[1] * cos_ttilt + coord[3] * sin_ttilt;
+    yp = -coord[1] * sin_ttilt + coord[3] * cos_ttilt;
+    s = coord[4];
+    dp = coord[5];
+
+    if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) && csbend->edge1_effects == 1) {
+      rho = (1 + dp) * rho_actual;
+      delta_xp = OR_TAN(e1) / rho * x;
+      xp += delta_xp;
+      yp -= OR_TAN(e1 - psi1 / (1 + dp)) / rho * y;
+    }
+    Qi[0] = x;
+    Qi[1] = xp;
+    Qi[2] = y;
+    Qi[3] = yp;
+    Qi[4] = 0;
+    Qi[5] = dp;
+    toDipoleCanonical(Qi);
+    particle_lost = !or_integrate_csbend(Qf, Qi, csbend->length, csbend->nSlices, rho0, rho_actual, &dz_lost, &ap,
+                                         csbend->integration_order, &F);
+    fromDipoleCanonical(Qf);
+    if (particle_lost) {
+      double X0 = Qf[0], XP0 = Qf[1], Y0 = Qf[2], YP0 = Qf[3];
+      memcpy(part[i_part], Qf, sizeof(double) * 6);
+      /* convertFromCSBendCoords, src/csbend.c:4527-4550 */
+      coord[0] = X0 * cos_ttilt - Y0 * sin_ttilt;
+      coord[2] = X0 * sin_ttilt + Y0 * cos_ttilt;
+      coord[1] = XP0 * cos_ttilt - YP0 * sin_ttilt;
+      coord[3] = XP0 * sin_ttilt + YP0 * cos_ttilt;
+      swapParticles(part[i_part], part[i_top]);
+      part[i_top][4] = z_start + dz_lost;
+      part[i_top][5] = Po * (1 + part[i_top][5]);
+      i_top--;
+      i_part--;
+      continue;
+    }
+    s += Qf[4];
+    x = Qf[0];
+    xp = Qf[1];
+    y = Qf[2];
+    yp = Qf[3];
+    dp = Qf[5];
+    if ((csbend->edgeFlags & EH_BEND_EDGE2_EFFECTS) && csbend->edge2_effects == 1) {
+      rho = (1 + dp) * rho_actual;
+      delta_xp = OR_TAN(e2) / rho * x;
+      xp += delta_xp;
+      yp -= OR_TAN(e2 - psi2 / (1 + dp)) / rho * y;
+    }
+    coord[0] = x * cos_ttilt - y * sin_ttilt + 0.0;
+    coord[2] = x * sin_ttilt + y * cos_ttilt + 0.0;
+    coord[1] = xp * cos_ttilt - yp * sin_ttilt + 0.0;
+    coord[3] = xp * sin_ttilt + yp * cos_ttilt + 0.0;
+    coord[4] = s + 0.0;
+    coord[5] = dp;
+    coord[0] += dxf + dzf * coord[1];
+    coord[2] += dyf + dzf * coord[3];
+    coord[4] += dzf * sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
+  }
+  return i_top + 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* RFCA */
+static double KahanPlus(double oldSum, double b, double *error) { /* src/kahan.c:27-35 */
+  double sum, Y;
+  Y = b + *error;
+  sum = oldSum + Y;
+  *error = Y - (sum - oldSum);
+  return sum;
+}
+
+static double beta_from_delta(double p, double delta) { /* src/do_tracking.c:95-98 */
+  p *= 1 + delta;
+  return (p / sqrt(p * p + 1));
+}
+
+/* findFiducialTime, src/simple_rfca.c:62-262 (TMEAN with Kahan, LIGHT) */
+double or_findFiducialTime(double **part, long np, double s0, double sOffset, double p0, int mode) {
+  double tFid = 0.0;
+  if (mode == EH_FID_LIGHT) {
+    tFid = (s0 + sOffset) / c_mks;
+  } else {
+    double tsum = 0, error = 0.0;
+    long ip, nsum = 0;
+    for (ip = 0; ip < np; ip++) {
+      tsum = KahanPlus(tsum, (part[ip][4] + sOffset) / (c_mks * beta_from_delta(p0, part[ip][5])), &error);
+      nsum++;
+    }
+    if (nsum > 0)
+      tFid = tsum / nsum;
+  }
+  return tFid;
+}
+
+/* add_to_particle_energy, src/simple_rfca.c:987-1006 */
+static void add_to_particle_energy(double *coord, double timeOfFlight, double Po, double dgamma) {
+  double gamma, gamma1, PRatio, P, P1, Pz1, Pz;
+  P = Po * (1 + coord[5]);
+  gamma1 = (gamma = sqrt(P * P + 1)) + dgamma;
+  if (gamma1 <= 1)
+    gamma1 = 1 + 1e-7;
+  P1 = sqrt(gamma1 * gamma1 - 1);
+  coord[5] = (P1 - Po) / Po;
+  coord[4] = timeOfFlight * c_mks * P1 / gamma1;
+  Pz = P / sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
+  Pz1 = sqrt(Pz * Pz + gamma1 * gamma1 - gamma * gamma);
+  PRatio = Pz / Pz1;
+  coord[1] *= PRatio;
+  coord[3] *= PRatio;
+}
+
+/* simple_rf_cavity -> trackRfCavityWithWakes, src/simple_rfca.c:264-985 (no wakes,
+ * no LSC, Q = 0, change_t = 0, change_p0 = 0, no linearize/lockPhase, no backtrack,
+ * body focus model none) */
+long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_central, double zEnd) {
+  long ip, nKicks, ik;
+  double length, omega, volt, tau, dtLight, phase, t0, To, timeOffset, dgamma = 0, dgammaMax = 0;
+  double P, gamma, gamma1, beta_i, t, dt, x, xp, dc4, *coord;
+  int matrixMethod = 0, same_dgamma = 0;
+
+  if (rfca->Q != 0 || rfca->change_t || rfca->change_p0 || rfca->linearize || rfca->lockPhase)
+    return or_fail("RFCA option not supported");
+  if (np <= 0)
+    return np;
+  length = rfca->length;
+  if (rfca->volt == 0) {
+    if (rfca->length) {
+      for (ip = 0; ip < np; ip++) {
+        coord = part[ip];
+        coord[0] += coord[1] * length;
+        coord[2] += coord[3] * length;
+        coord[4] += length * sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
+      }
+    }
+    return np;
+  }
+  omega = PIx2 * rfca->freq;
+  volt = rfca->volt / (1e6 * particleMassMV * particleRelSign);
+  tau = 0;
+  nKicks = length ? rfca->nKicks : 1;
+  if (nKicks <= 0) {
+    matrixMethod = 1;
+    nKicks = 1;
+  }
+  length /= nKicks;
+  volt /= nKicks;
+  dtLight = length / c_mks;
+
+  if (!rfca->fiducial_seen) {
+    if (rfca->tReference != -1)
+      t0 = rfca->tReference;
+    else
+      t0 = or_findFiducialTime(part, np, zEnd - rfca->length, length / 2, *P_central, rfca->fiducial_mode);
+    rfca->phase_fiducial = -omega * t0;
+    rfca->fiducial_seen = 1;
+  }
+  phase = rfca->phase_fiducial;
+  if (omega) {
+    t0 = -rfca->phase_fiducial / omega;
+    rfca->t_fiducial = t0;
+    To = PIx2 / omega;
+  } else
+    t0 = To = 0;
+  (void)To;
+  phase += rfca->phase * PI / 180.0;
+
+  if (omega == 0 && tau == 0) {
+    dgamma = volt * OR_SIN(phase);
+    dgammaMax = volt;
+    same_dgamma = 1;
+  }
+  timeOffset = 0;
+
+  for (ip = 0; ip < np; ip++) {
+    coord = part[ip];
+    coord[0] -= rfca->dx;
+    coord[2] -= rfca->dy;
+  }
+
+  if (!matrixMethod) {
+    double *inverseF = calloc(sizeof(*inverseF), np > 0 ? np : 1);
+    for (ik = 0; ik < nKicks; ik++) {
+      for (ip = 0; ip < np; ip++) {
+        coord = part[ip];
+        if (coord[5] == -1)
+          continue;
+        if (length)
+          dc4 = length / 2 * sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
+        else
+          dc4 = 0;
+        P = *P_central * (1 + coord[5]);
+        beta_i = P / (gamma = sqrt(sqr(P) + 1));
+        t = (coord[4] + dc4) / (c_mks * beta_i) - timeOffset;
+        if ((dt = t - t0) < 0)
+          dt = 0;
+        (void)dt;
+        if (!same_dgamma) {
+          if (rfca->standingWave)
+            dgamma = volt * OR_SIN(omega * (t - 0.0) + phase) * 1.0;
+          else {
+            double phi = omega * (t - 0.0 - ik * dtLight) + phase;
+            dgamma = volt * OR_SIN(phi) * 1.0;
+          }
+        }
+        if (length) {
+          if (rfca->end1Focus && ik == 0) {
+            inverseF[ip] = dgamma / (2 * gamma * length);
+            coord[1] -= coord[0] * inverseF[ip];
+            coord[3] -= coord[2] * inverseF[ip];
+          }
+          coord[0] += coord[1] * length / 2;
+          coord[2] += coord[3] * length / 2;
+          coord[4] += length / 2 * sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
+        }
+        add_to_particle_energy(coord, t, *P_central, dgamma);
+        if ((gamma1 = gamma + dgamma) <= 1)
+          coord[5] = -1;
+        else
+          inverseF[ip] = -dgamma / (2 * gamma1 * length);
+      }
+      if (length) {
+        for (ip = 0; ip < np; ip++) {
+          coord = part[ip];
+          coord[0] += coord[1] * length / 2;
+          coord[2] += coord[3] * length / 2;
+          coord[4] += length / 2 * sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
+          if (rfca->end2Focus && (ik == nKicks - 1)) {
+            coord[1] -= coord[0] * inverseF[ip];
+            coord[3] -= coord[2] * inverseF[ip];
+          }
+        }
+      }
+    }
+    free(inverseF);
+  } else {
+    double sin_phase = 0.0, inverseF;
+    double R11 = 1, R21 = 0, R22, R12, dP, ds1;
+    for (ik = 0; ik < nKicks; ik++) {
+      for (ip = 0; ip < np; ip++) {
+        coord = part[ip];
+        P = *P_central * (1 + coord[5]);
+        beta_i = P / (gamma = sqrt(sqr(P) + 1));
+        ds1 = length / 2 * sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
+        t = (coord[4] + ds1) / (c_mks * beta_i) - timeOffset;
+        if ((dt = t - t0) < 0)
+          dt = 0;
+        if (!same_dgamma) {
+          sin_phase = OR_SIN(omega * (t - 0.0 - ik * dtLight) + phase);
+          dgamma = (dgammaMax = volt * 1.0) * sin_phase;
+        }
+        if (rfca->end1Focus && length && ik == 0) {
+          inverseF = dgamma / (2 * gamma * length);
+          coord[1] -= coord[0] * inverseF;
+          coord[3] -= coord[2] * inverseF;
+        }
+        dP = sqrt(sqr(gamma + dgamma) - 1) - P;
+        R22 = 1 / (1 + dP / P);
+        if (fabs(dP / P) > 1e-14)
+          R12 = length * (P / dP * OR_LOG(1 + dP / P));
+        else
+          R12 = length;
+        coord[4] += ds1;
+        x = coord[0];
+        xp = coord[1];
+        coord[0] = x * R11 + xp * R12;
+        coord[1] = x * R21 + xp * R22;
+        x = coord[2];
+        xp = coord[3];
+        coord[2] = x * R11 + xp * R12;
+        coord[3] = x * R21 + xp * R22;
+        coord[4] += length / 2 * sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
+        coord[5] = (P + dP - (*P_central)) / (*P_central);
+        if ((gamma += dgamma) <= 1)
+          coord[5] = -1;
+        if (rfca->end2Focus && length && ik == (nKicks - 1)) {
+          inverseF = -dgamma / (2 * gamma * length);
+          coord[1] -= coord[0] * inverseF;
+          coord[3] -= coord[2] * inverseF;
+        }
+        coord[4] = (P + dP) / gamma * coord[4] / beta_i;
+      }
+    }
+    (void)dgammaMax;
+  }
+  for (ip = 0; ip < np; ip++) {
+    coord = part[ip];
+    coord[0] += rfca->dx;
+    coord[2] += rfca->dy;
+  }
+  return or_removeInvalidParticles(part, np, zEnd, *P_central);
+}
+
+/* ------------------------------------------------------------------------- */
+/* do_tracking's pass and element loops for this element set,
+ * src/do_tracking.c:144 (initial NaN sweep :478-483, pass loop :498, z bookkeeping
+ * :1101-1108, dispatch :1352-2495, MAXAMP sweep :2506-2517, NaN sweep and loss
+ * recording :2675-2684, nToTrack = nLeft :2750).  wrap_around = 1. */
+static double or_elem_length(const EH_ELEMENT_LIST *e) {
+  switch (e->type) {
+  case EH_T_MARK:
+  case EH_T_MAXAMP:
+    return 0;
+  default:
+    return *(const double *)e->p_elem; /* length is the first member, as in the reference */
+  }
+}
+
+static void recordLostParticles(double **coord, long nLeft, long nToTrack, long pass) {
+  long i;
+  for (i = nLeft; i < nToTrack; i++)
+    coord[i][EH_LOSS_PASS_INDEX] = pass;
+}
+
+long or_do_tracking(double **coord, long np, const EH_ELEMENT_LIST *beamline, long n_elements, long n_passes,
+                    double *P_central, double *effort) {
+  long nToTrack = np, nLeft, i_pass, ie;
+  const EH_MAXAMP *maxamp = NULL;
+  double x_max = 0, y_max = 0, z, last_z, z_travel;
+
+  if (effort)
+    *effort = 0;
+  nLeft = or_limit_amplitudes(coord, DBL_MAX, DBL_MAX, nToTrack, 0.0, *P_central, 0);
+  if (nLeft != nToTrack)
+    recordLostParticles(coord, nLeft, nToTrack, 0);
+  nToTrack = nLeft;
+
+  for (i_pass = 0; i_pass < n_passes; i_pass++) {
+    z = z_travel = last_z = 0;
+    for (ie = 0; ie < n_elements; ie++) {
+      const EH_ELEMENT_LIST *eptr = beamline + ie;
+      double len = or_elem_length(eptr);
+      last_z = z;
+      z += len;
+      z_travel += len;
+      nLeft = nToTrack;
+      if (effort && eptr->type != EH_T_MARK && eptr->type != EH_T_MAXAMP)
+        *effort += nToTrack;
+      switch (eptr->type) {
+      case EH_T_MARK:
+        break;
+      case EH_T_MAXAMP:
+        maxamp = (const EH_MAXAMP *)eptr->p_elem;
+        if (maxamp->elliptical)
+          return or_fail("elliptical MAXAMP not supported");
+        x_max = maxamp->x_max;
+        y_max = maxamp->y_max;
+        break;
+      case EH_T_DRIF: {
+        const EH_DRIFT *d = (const EH_DRIFT *)eptr->p_elem;
+        or_drift_beam(coord, nToTrack, d->length, d->order);
+        break;
+      }
+      case EH_T_EDRIFT:
+        or_exactDrift(coord, nToTrack, ((const EH_EDRIFT *)eptr->p_elem)->length);
+        break;
+      case EH_T_KQUAD:
+      case EH_T_KSEXT:
+      case EH_T_KOCT:
+        nLeft = or_multipole_tracking2(coord, nToTrack, eptr, *P_central, last_z, maxamp);
+        break;
+      case EH_T_CSBEND:
+        nLeft = or_track_through_csbend(coord, nToTrack, (const EH_CSBEND *)eptr->p_elem, *P_central, last_z, maxamp);
+        break;
+      case EH_T_RFCA:
+        nLeft = or_simple_rf_cavity(coord, nToTrack, (EH_RFCA *)eptr->p_elem, P_central, z_travel);
+        break;
+      default:
+        return or_fail("element type not supported");
+      }
+      if (nLeft < 0)
+        return nLeft;
+      if (eptr->type != EH_T_MARK) {
+        if (x_max || y_max)
+          nLeft = or_limit_amplitudes(coord, x_max, y_max, nLeft, z, *P_central, eptr->type == EH_T_DRIF);
+      }
+      nLeft = or_limit_amplitudes(coord, DBL_MAX, DBL_MAX, nLeft, z, *P_central, 0);
+      if (nLeft != nToTrack)
+        recordLostParticles(coord, nLeft, nToTrack, i_pass);
+      nToTrack = nLeft;
+    }
+  }
+  return nToTrack;
+}
+
+/* contiguous AoS wrapper for ctypes: rows of EH_NPROP doubles */
+static double **or_rows(double *flat, long n) {
+  double **rows = malloc(sizeof(double *) * (n > 0 ? n : 1));
+  long i;
+  for (i = 0; i < n; i++)
+    rows[i] = flat + i * EH_NPROP;
+  return rows;
+}
+
+long or_do_tracking_flat(double *flat, long np, const EH_ELEMENT_LIST *beamline, long n_elements, long n_passes,
+                         double *P_central, double *effort) {
+  double **rows = or_rows(flat, np);
+  long r = or_do_tracking(rows, np, beamline, n_elements, n_passes, P_central, effort);
+  /* rows were permuted through pointer swaps of row contents, so flat is already in row order */
+  free(rows);
+  return r;
+}
+
+/* single-element helpers for tests */
+long or_element_flat(double *flat, long np, const EH_ELEMENT_LIST *eptr, double *P_central, double z_start,
+                     double z_end, const EH_MAXAMP *maxamp) {
+  double **rows = or_rows(flat, np);
+  long r = np;
+  switch (eptr->type) {
+  case EH_T_DRIF:
+    or_drift_beam(rows, np, ((const EH_DRIFT *)eptr->p_elem)->length, ((const EH_DRIFT *)eptr->p_elem)->order);
+    break;
+  case EH_T_EDRIFT:
+    or_exactDrift(rows, np, ((const EH_EDRIFT *)eptr->p_elem)->length);
+    break;
+  case EH_T_KQUAD:
+  case EH_T_KSEXT:
+  case EH_T_KOCT:
+    r = or_multipole_tracking2(rows, np, eptr, *P_central, z_start, maxamp);
+    break;
+  case EH_T_CSBEND:
+    r = or_track_through_csbend(rows, np, (const EH_CSBEND *)eptr->p_elem, *P_central, z_start, maxamp);
+    break;
+  case EH_T_RFCA:
+    r = or_simple_rf_cavity(rows, np, (EH_RFCA *)eptr->p_elem, P_central, z_end);
+    break;
+  default:
+    r = or_fail("element type not supported");
+  }
+  free(rows);
+  return r;
+}
+
+long or_limit_amplitudes_flat(double *flat, double xmax, double ymax, long np, double z, double Po, long extrapolate_z) {
+  double **rows = or_rows(flat, np);
+  long r = or_limit_amplitudes(rows, xmax, ymax, np, z, Po, extrapolate_z);
+  free(rows);
+  return r;
+}
+
+/* math entry points exposed for tests */
+double or_sin(double x) { return OR_SIN(x); }
+double or_cos(double x) { return OR_COS(x); }
+double or_asin(double x) { return OR_ASIN(x); }
+double or_log(double x) { return OR_LOG(x); }

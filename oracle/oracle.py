@@ -1,0 +1,87 @@
+"""ctypes wrapper of the CPU checker (liboracle.so / liboracle_libm.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product path.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_libs = {}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load(libm=False):
+    name = "liboracle_libm.so" if libm else "liboracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    lib.or_do_tracking_flat.argtypes = [vp, C.c_long, vp, C.c_long, C.c_long, dp, dp]
+    lib.or_do_tracking_flat.restype = C.c_long
+    lib.or_element_flat.argtypes = [vp, C.c_long, vp, dp, C.c_double, C.c_double, vp]
+    lib.or_element_flat.restype = C.c_long
+    lib.or_limit_amplitudes_flat.argtypes = [vp, C.c_double, C.c_double, C.c_long, C.c_double, C.c_double, C.c_long]
+    lib.or_limit_amplitudes_flat.restype = C.c_long
+    lib.or_csbend_field_table.argtypes = [dp, dp, C.c_double, C.c_int, C.c_int, dp, dp]
+    lib.or_csbend_field_table.restype = C.c_long
+    lib.or_last_error.restype = C.c_char_p
+    for f in ("or_sin", "or_cos", "or_asin", "or_log"):
+        getattr(lib, f).argtypes = [C.c_double]
+        getattr(lib, f).restype = C.c_double
+    _libs[name] = lib
+    return lib
+
+
+def _check(lib, r):
+    if r < 0:
+        raise RuntimeError("oracle error %d: %s" % (r, lib.or_last_error().decode()))
+    return r
+
+
+def do_tracking(coords, lattice, n_passes, p_central, libm=False):
+    """Track a copy of coords (n, 10); returns (rows, n_alive, effort, p_central)."""
+    lib = load(libm)
+    rows = np.array(coords, dtype=np.float64, order="C", copy=True)
+    arr, keep = lattice.c_beamline()
+    pc = C.c_double(p_central)
+    eff = C.c_double(0)
+    n = _check(lib, lib.or_do_tracking_flat(rows.ctypes.data, rows.shape[0], C.cast(arr, C.c_void_p), len(lattice),
+                                            n_passes, C.byref(pc), C.byref(eff)))
+    del keep
+    return rows, n, eff.value, pc.value
+
+
+def element(coords, lattice, index, p_central, z_start, z_end, maxamp=None, libm=False):
+    """Apply one element routine (no sweeps) to a copy of coords."""
+    lib = load(libm)
+    rows = np.array(coords, dtype=np.float64, order="C", copy=True)
+    arr, keep = lattice.c_beamline()
+    pc = C.c_double(p_central)
+    amp = C.cast(C.pointer(maxamp), C.c_void_p) if maxamp is not None else None
+    el = C.cast(C.byref(arr, index * C.sizeof(arr._type_)), C.c_void_p)
+    n = _check(lib, lib.or_element_flat(rows.ctypes.data, rows.shape[0], el, C.byref(pc), z_start, z_end, amp))
+    del keep
+    return rows, n
+
+
+def field_table(b, c, h, nonlinear=1, expansion_order=0):
+    lib = load()
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    Fx = np.zeros(121)
+    Fy = np.zeros(121)
+    dp = C.POINTER(C.c_double)
+    r = _check(lib, lib.or_csbend_field_table(b.ctypes.data_as(dp), c.ctypes.data_as(dp), h, nonlinear,
+                                              expansion_order, Fx.ctypes.data_as(dp), Fy.ctypes.data_as(dp)))
+    return Fx.reshape(11, 11), Fy.reshape(11, 11), r

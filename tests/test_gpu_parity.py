@@ -1,0 +1,165 @@
+"""GPU parity: the HIP path against the CPU oracle on the same seeded inputs.
+
+Bar: bit-identical doubles (NaN == NaN) for every row in the reference's row
+order, including lost rows and their loss pass; the GPU and the oracle share the
+transcendental library (eh_math.h), so any difference is a bug.
+"""
+import ctypes as C
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from elegant_amd import configs, structs as S, tracking
+from elegant_amd.lattice import Element, Lattice
+from tests.helpers import assert_rows_identical, bit_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_device_math_bit_identical(ehlib, oracle_mod):
+    _say("device math self-test")
+    rng = np.random.default_rng(7)
+    n = 200000
+    x = np.concatenate([rng.uniform(-4, 4, n // 4), rng.uniform(-1e7, 1e7, n // 4), rng.uniform(-1e-3, 1e-3, n // 4),
+                        np.exp(rng.uniform(-700, 700, n // 4))])
+    y = np.concatenate([rng.uniform(-1, 1, n // 2), rng.uniform(-0.01, 0.01, n // 4), rng.uniform(0.5, 1.0, n // 4)])
+    y[y == 0] = 0.25
+    out = np.empty(7 * n)
+    r = ehlib.eh_selftest_math(x.ctypes.data, y.ctypes.data, n, out.ctypes.data)
+    assert r == n, ehlib.eh_last_error()
+    out = out.reshape(7, n)
+    lib = oracle_mod.load()
+    host_sin = np.array([lib.or_sin(v) for v in x])
+    host_cos = np.array([lib.or_cos(v) for v in x])
+    host_asin = np.array([lib.or_asin(v) for v in y])
+    host_log = np.array([lib.or_log(abs(v)) for v in x])
+    assert bit_equal(out[0], host_sin).all(), "sin differs"
+    assert bit_equal(out[1], host_cos).all(), "cos differs"
+    assert bit_equal(out[2], host_asin).all(), "asin differs"
+    assert bit_equal(out[3], host_log).all(), "log differs"
+    # sqrt and division must be correctly rounded on the device, as on the host
+    assert bit_equal(out[4], np.sqrt(np.abs(x))).all(), "sqrt not correctly rounded"
+    assert bit_equal(out[5], x / y).all(), "division not correctly rounded"
+
+
+def _beam_rows(n, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    rows = np.zeros((n, S.NPROP))
+    rows[:, 0] = rng.normal(0, 3e-3, n) * scale
+    rows[:, 1] = rng.normal(0, 3e-4, n) * scale
+    rows[:, 2] = rng.normal(0, 2e-3, n) * scale
+    rows[:, 3] = rng.normal(0, 2e-4, n) * scale
+    rows[:, 4] = rng.normal(0, 1e-3, n)
+    rows[:, 5] = rng.normal(0, 1e-3, n)
+    rows[:, 6] = np.arange(1, n + 1)
+    return rows
+
+
+ELEMENT_CASES = [
+    ("KQUAD", dict(L=0.3, K1=1.2, N_KICKS=8)),
+    ("KQUAD", dict(L=0.25, K1=-2.1, N_SLICES=3, DX=1e-4, DY=-2e-4, DZ=1e-3, TILT=0.01, FSE=1e-3, HKICK=1e-5,
+                   VKICK=-2e-5)),
+    ("KQUAD", dict(L=0.3, K1=0.9, N_KICKS=8, INTEGRATION_ORDER=2)),
+    ("KQUAD", dict(L=0.3, K1=0.9, N_KICKS=16, INTEGRATION_ORDER=6)),
+    ("KSEXT", dict(L=0.2, K2=15.0, N_KICKS=4)),
+    ("KSEXT", dict(L=0.2, K2=-40.0, K1=0.05, J1=0.02, N_KICKS=4, DX=3e-5, DY=-3e-5)),
+    ("KOCT", dict(L=0.2, K3=1000.0)),
+    ("KOCT", dict(L=0.2, K3=-3e4, TILT=math.pi / 2)),
+    ("CSBEND", dict(L=1.0, ANGLE=2 * math.pi / 20, N_KICKS=8, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0)),
+    ("CSBEND", dict(L=0.8, ANGLE=0.0314, K1=-0.5, N_KICKS=16, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0)),
+    ("CSBEND", dict(L=0.8, ANGLE=-0.05, K1=0.3, K2=2.0, N_KICKS=8, E1=0.02, E2=0.01, HGAP=0.01, DX=1e-4,
+                    DZ=2e-4, FSE=1e-3)),
+    ("CSBEND", dict(L=0.8, ANGLE=0.04, K3=50.0, N_KICKS=4, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0)),
+    ("EDRIFT", dict(L=0.7)),
+    ("DRIF", dict(L=1.6)),
+    ("DRIF", dict(L=1.6, ORDER=1)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(ELEMENT_CASES)))
+def test_single_element(case, ehlib, oracle_mod):
+    etype, params = ELEMENT_CASES[case]
+    _say("single element %d %s %r" % (case, etype, params))
+    lat = Lattice([Element("E", etype, **params)])
+    rows = _beam_rows(4096, seed=case, scale=3.0)
+    rows[:50, 0] = np.linspace(-0.03, 0.03, 50)  # some outside the 20 mm aperture
+    Po = configs.p_central_from_mev(3000.0)
+    amp = S.EH_MAXAMP(0.02, 0.02, 0, 2)
+    L = lat.elements[0].length()
+    ref, nref = oracle_mod.element(rows, lat, 0, Po, 1.0, 1.0 + L, maxamp=amp)
+    beam = tracking.Beam(rows)
+    arr, keep = lat.c_beamline()
+    s = lat.structs()[0]
+    code = S.TYPE_CODES[etype]
+    if code in (S.T_KQUAD, S.T_KSEXT, S.T_KOCT):
+        n = ehlib.eh_multipole_tracking2(beam.handle, len(rows), arr, 0.0, Po, 1.0, C.byref(amp), -1)
+    elif code == S.T_CSBEND:
+        n = ehlib.eh_track_through_csbend(beam.handle, len(rows), C.byref(s), 0.0, Po, 1.0, C.byref(amp), -1)
+    elif code == S.T_EDRIFT:
+        n = ehlib.eh_exactDrift(beam.handle, len(rows), L)
+    else:
+        n = ehlib.eh_drift_beam(beam.handle, len(rows), L, params.get("ORDER", 0))
+    assert n >= 0, ehlib.eh_last_error()
+    got = beam.download()
+    assert_rows_identical(got, ref, n, nref, "%s case %d" % (etype, case))
+
+
+@pytest.mark.parametrize("method", ["matrix", "kick"])
+def test_rfca(method, ehlib, oracle_mod):
+    _say("rfca %s" % method)
+    Po = configs.p_central_from_mev(3000.0)
+    params = configs.mba_rf(Po)
+    if method == "kick":
+        params["N_KICKS"] = 4
+    lat = Lattice([Element("RF", "RFCA", **params)])
+    rows = _beam_rows(5000, seed=11)
+    rows[:, 4] += 123.456
+    ref, nref = oracle_mod.element(rows, lat, 0, Po, 100.0, 100.3)
+    lat.reset_state()
+    beam = tracking.Beam(rows)
+    s = lat.structs()[0]
+    pc = C.c_double(Po)
+    n = ehlib.eh_simple_rf_cavity(beam.handle, len(rows), C.byref(s), C.byref(pc), 100.3)
+    assert n >= 0, ehlib.eh_last_error()
+    assert_rows_identical(beam.download(), ref, n, nref, "RFCA " + method)
+
+
+def _track_both(lat, rows, n_passes, Po, mode, tpl=4):
+    ref_lat = lat.copy()
+    ref, nref, eff_ref, _ = __import__("oracle.oracle", fromlist=["x"]).do_tracking(rows, ref_lat, n_passes, Po)
+    gl = lat.copy()
+    beam = tracking.Beam(rows)
+    n, stats, _ = tracking.track(beam, gl, n_passes, Po, mode=mode, turns_per_launch=tpl)
+    got = beam.download()
+    return got, n, ref, nref, stats, eff_ref
+
+
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+def test_fodo_ring(mode, ehlib, oracle_mod):
+    _say("fodo ring %s" % mode)
+    lat = configs.fodo_ring()
+    rows = configs.fodo_beam(2000, seed=3)
+    rows[:20, 0] *= 40  # a few particles outside the aperture from the start
+    rows[20:40, 1] *= 30
+    Po = configs.p_central_from_mev(1000.0)
+    got, n, ref, nref, stats, eff = _track_both(lat, rows, 30, Po, mode)
+    assert_rows_identical(got, ref, n, nref, "fodo " + mode)
+    assert stats["particle_element_turns"] == eff
+
+
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+def test_mba_da_grid(mode, ehlib, oracle_mod):
+    _say("mba da grid %s" % mode)
+    Po = configs.p_central_from_mev(3000.0)
+    lat = configs.mba_ring(Po)
+    rows = configs.da_grid(33, 17, xmax=0.03, ymax=0.012)
+    got, n, ref, nref, stats, eff = _track_both(lat, rows, 3, Po, mode, tpl=2)
+    assert_rows_identical(got, ref, n, nref, "mba " + mode)
+    assert stats["particle_element_turns"] == eff
