@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Throughput of elegant's element-tracking path on one MI355X.
+
+Headline workload: the 40-cell multi-bend-achromat-style ring at 3 GeV (per
+cell 5 gradient CSBENDs with N_KICKS=16, 10 KQUAD, 6 KSEXT, 2 KOCT; one RFCA of
+2 MV at h=1296 and 170 deg; MAXAMP 20 mm) with a 1000 x 1000 dynamic-aperture
+grid (x in +-15 mm, y in +-5 mm, other coordinates 0), i.e. 1e6 particles, in
+FP64 with loss compaction, through the fused kernels.  One step = one turn of
+the whole ring for the whole beam; W untimed turns, then K timed turns.
+Particles are resident in HBM before the timed region starts.
+
+Further legs in the same JSON line:
+  batched  256 error-seeded copies of that ring (FSE 1e-3 on KQUAD/KSEXT,
+           DX/DY 30 um, 2-sigma cutoff), lognormal particle counts in [1e3, 1e5]
+           (4.9e6 in total), 6D Gaussian beams, all variants in one persistent
+           launch per span captured in a HIP graph, launch shape from the
+           autotuner;
+  plain    the per-element drop-in path (one kernel per reference call, host
+           synchronisation and compaction after each) on the 10-cell FODO ring
+           with 1e4 particles: the launch-bound small-lattice case the batched
+           path is compared with;
+  cpu_baseline  the checker compiled like the reference (gcc -O3 -m64, C-library
+           transcendentals) on a bounded strided sample of the headline
+           workload, one single-threaded worker per host core.
+
+The roofline entry is for the fused kernel of the headline leg: algorithmic
+FP64 operations per launch (census per element type x particle-element units
+per launch) over the average launch time, both measured inside this process
+with HIP events on the library's stream.
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle·element·turns/s FP64 on 1 MI355X; batched vs plain; adjoint/forward"
+UNIT = "particle·element·turns/s"
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (= FP64 matrix) peak, FMA counted as 2
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(name, grid):
+    from elegant_amd import configs
+    if name == "mba":
+        Po = configs.p_central_from_mev(3000.0)
+        lat = configs.mba_ring(Po)
+        rows = configs.da_grid(grid, grid)
+        desc = ("40-cell MBA ring 3 GeV, %dx%d DA grid (x +-15 mm, y +-5 mm), RFCA 2 MV h=1296 170 deg, "
+                "MAXAMP 20 mm" % (grid, grid))
+    else:
+        Po = configs.p_central_from_mev(1000.0)
+        lat = configs.fodo_ring()
+        rows = configs.fodo_beam(grid * grid)
+        desc = "10-cell FODO ring 1 GeV, %d-particle matched 6D Gaussian, MAXAMP 20 mm" % (grid * grid)
+    return lat, rows, Po, desc
+
+
+def _cpu_worker(args):
+    name, grid, idx, turns = args
+    from oracle import oracle
+    lat, rows, Po, _ = workload(name, grid)
+    sub = rows[idx]
+    t0 = time.perf_counter()
+    _, n, eff, _ = oracle.do_tracking(sub, lat, turns, Po, libm=True)
+    return eff, time.perf_counter() - t0
+
+
+def cpu_baseline(name, grid, sample, turns, repeats=3):
+    """the checker with C-library transcendentals on a strided sample of the same
+    workload: one single-threaded worker per host core (particle shards, the
+    reference's MPI decomposition), median of `repeats` runs; the per-core rate
+    is reported beside it"""
+    from oracle import oracle
+    oracle.load(libm=True)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    lat, rows, _, _ = workload(name, grid)
+    n_total = len(rows)
+    idx = np.linspace(0, n_total - 1, sample).astype(np.int64)
+    shards = np.array_split(idx, cores)
+    rates, per_core = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        with mp.get_context("fork").Pool(cores) as pool:
+            res = pool.map(_cpu_worker, [(name, grid, s, turns) for s in shards])
+        wall = time.perf_counter() - t0
+        units = sum(r[0] for r in res)
+        rates.append(units / wall)
+        per_core.append(units / sum(r[1] for r in res))
+    return {"value": float(np.median(rates)), "unit": UNIT, "cores": cores, "kind": "port",
+            "sample": "%d of the %d particles (strided), %d turns, median of %d runs; oracle with C-library "
+                      "sin/cos/asin/log, gcc -O3 -m64, no FMA" % (sample, n_total, turns, repeats),
+            "per_core_value": float(np.median(per_core)),
+            "cpu": platform.processor() or platform.machine()}
+
+
+def leg_fused(args, census):
+    from elegant_amd import tracking
+    lat, rows, Po, desc = workload(args.config, args.grid)
+    cen = census.lattice_census(lat)
+    log("workload: %s; %d particles, %d counted elements/turn, %.4g flops + %d T per particle-turn" %
+        (desc, len(rows), cen["units"], cen["flops"], cen["T"]))
+    beam = tracking.Beam(rows)
+    t0 = time.perf_counter()
+    if args.warmup:
+        n, _, _ = tracking.track(beam, lat, args.warmup, Po, mode=args.mode, turns_per_launch=args.turns_per_launch)
+        log("warmup: %d turns, %d alive, %.3f s" % (args.warmup, n, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    n, st, _ = tracking.track(beam, lat, args.steps, Po, mode=args.mode, turns_per_launch=args.turns_per_launch)
+    wall = time.perf_counter() - t0
+    beam.free()
+    log("timed: %d turns, %d alive, %.3f s wall, %.3f s in kernels, %d launches" %
+        (args.steps, n, wall, st["kernel_ms"] / 1e3, st["launches"]))
+    return desc, len(rows), n, wall, st, cen
+
+
+def leg_batched(args):
+    from elegant_amd import autotune, configs, tracking
+    Po = configs.p_central_from_mev(3000.0)
+    counts = configs.variant_counts(args.variants)
+    t0 = time.perf_counter()
+    lats = configs.mba_variants(args.variants, p_central=Po)
+    beams = configs.variant_beams(counts)
+    log("batched: %d variants, %d particles, setup %.1f s" % (args.variants, counts.sum(), time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    tune, trials = autotune.tune_batch(beams, lats, Po, passes=1, cache=None)
+    log("autotune: %s (%.1f s, %d trials)" % (tune, time.perf_counter() - t0, len(trials)))
+    batch = tracking.Batch(beams)
+    batch.track(lats, 1, Po, use_graph=True, **tune)  # warmup: fixes RF fiducials
+    t0 = time.perf_counter()
+    n, st = batch.track(lats, args.batch_steps, Po, use_graph=True, **tune)
+    wall = time.perf_counter() - t0
+    batch.free()
+    units = st["particle_element_turns"]
+    log("batched timed: %d turns, %d alive, %.3f s wall, %.3f s on the stream, %d launches" %
+        (args.batch_steps, n, wall, st["kernel_ms"] / 1e3, st["launches"]))
+    return {"value": units / wall, "unit": UNIT, "turns": args.batch_steps, "variants": args.variants,
+            "particles": int(counts.sum()), "survivors": n, "wall_s": round(wall, 4),
+            "stream_s": round(st["kernel_ms"] / 1e3, 4), "launches": st["launches"], "tune": tune,
+            "workload": "256 error-seeded MBA rings (FSE 1e-3 KQUAD/KSEXT, DX/DY 30 um, 2-sigma cutoff), "
+                        "lognormal counts in [1e3, 1e5], 6D Gaussian beams, 3 GeV, one graph-captured "
+                        "persistent launch per span"}
+
+
+def leg_plain(args):
+    from elegant_amd import tracking
+    lat, rows, Po, desc = workload("fodo", 100)
+    beam = tracking.Beam(rows)
+    tracking.track(beam, lat, 1, Po, mode="plain")
+    t0 = time.perf_counter()
+    n, st, _ = tracking.track(beam, lat, args.plain_steps, Po, mode="plain")
+    wall = time.perf_counter() - t0
+    beam.free()
+    units = st["particle_element_turns"]
+    log("plain timed: %d turns, %d alive, %.3f s wall, %d launches" % (args.plain_steps, n, wall, st["launches"]))
+    return {"value": units / wall, "unit": UNIT, "turns": args.plain_steps, "particles": len(rows),
+            "survivors": n, "wall_s": round(wall, 4), "launches": st["launches"],
+            "workload": desc + ", per-element drop-in path"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16, help="timed turns of the headline leg")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed turns before timing")
+    ap.add_argument("--config", choices=["mba", "fodo"], default="mba")
+    ap.add_argument("--grid", type=int, default=1000, help="grid side (mba) / sqrt(particles) (fodo)")
+    ap.add_argument("--mode", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--turns-per-launch", type=int, default=4)
+    ap.add_argument("--variants", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=4)
+    ap.add_argument("--plain-steps", type=int, default=4)
+    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--cpu-turns", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (particles and lattice variants "
+              "would shard across GPUs by range). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    # the CPU baseline forks its workers before this process initialises the GPU
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(args.config, args.grid, args.cpu_sample, args.cpu_turns)
+        log("cpu baseline: %.4g %s on %d cores (%.1f s)" % (cpu["value"], UNIT, cpu["cores"],
+                                                            time.perf_counter() - t0))
+
+    from elegant_amd import census, tracking
+    tracking.init(0)
+    desc, n0, n, wall, st, cen = leg_fused(args, census)
+    units = st["particle_element_turns"]
+    value = units / wall
+
+    alg_per_unit = cen["algorithmic_per_unit"]
+    launches = max(1, st["launches"])
+    avg_launch_s = st["kernel_ms"] / 1e3 / launches
+    flops_per_launch = units / launches * alg_per_unit
+    achieved = flops_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "pipe": "FP64 VALU, no MFMA (compute-bound; the MI355X FP64 vector peak equals its FP64 "
+                        "matrix peak)",
+                "kernel": "k_fused" if args.mode == "fused" else "k_plain",
+                "algorithmic_flops_per_unit": round(alg_per_unit, 2),
+                "units_per_launch": round(units / launches),
+                "avg_launch_ms": round(avg_launch_s * 1e3, 3)}
+
+    out = {"metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (generated DA grid / seeded Gaussian beams; no reference data files)",
+           "config": {"workload": desc, "particles": n0, "turns_timed": args.steps,
+                      "elements_per_turn": cen["units"], "mode": args.mode,
+                      "turns_per_launch": args.turns_per_launch, "survivors": n},
+           "roofline": roofline}
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    if not args.no_batch:
+        out["batched"] = leg_batched(args)
+    if not args.no_plain:
+        out["plain"] = leg_plain(args)
+        if "batched" in out:
+            out["batched_over_plain"] = round(out["batched"]["value"] / out["plain"]["value"], 2)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -20,7 +20,8 @@ EXPORTS = [
     "eh_beam_upload_flat", "eh_beam_download_flat", "eh_beam_n_alive", "eh_beam_n_total", "eh_beam_set_pass",
     "eh_multipole_tracking2", "eh_track_through_csbend", "eh_simple_rf_cavity", "eh_exactDrift", "eh_drift_beam",
     "eh_limit_amplitudes", "eh_removeInvalidParticles", "eh_do_tracking", "eh_csbend_field_table",
-    "eh_replay_order", "eh_selftest_math",
+    "eh_replay_order", "eh_selftest_math", "eh_batch_create", "eh_batch_free", "eh_batch_upload",
+    "eh_batch_download", "eh_batch_n_alive", "eh_batch_track",
 ]
 
 
@@ -82,6 +83,18 @@ def load():
     lib.eh_replay_order.restype = L
     lib.eh_selftest_math.argtypes = [vp, vp, L, vp]
     lib.eh_selftest_math.restype = L
+    lib.eh_batch_create.argtypes = [L, vp]
+    lib.eh_batch_create.restype = vp
+    lib.eh_batch_free.argtypes = [vp]
+    lib.eh_batch_free.restype = None
+    lib.eh_batch_upload.argtypes = [vp, L, vp, L]
+    lib.eh_batch_upload.restype = L
+    lib.eh_batch_download.argtypes = [vp, L, vp, L]
+    lib.eh_batch_download.restype = L
+    lib.eh_batch_n_alive.argtypes = [vp, L]
+    lib.eh_batch_n_alive.restype = L
+    lib.eh_batch_track.argtypes = [vp, vp, L, L, dp, C.POINTER(S.eh_batch_tune), C.POINTER(S.eh_track_stats)]
+    lib.eh_batch_track.restype = L
     _lib = lib
     return lib
 

@@ -168,3 +168,38 @@ def fodo_beam(n=10000, seed=1):
 def mba_beam(n, seed=1, scale=1.0):
     tw = optics.periodic_twiss(mba_period())
     return gaussian_beam(n, tw, 5e-9 * scale, 50e-12 * scale, 1e-3, 3e-3, seed=seed)
+
+
+def _trunc(rng, sigma, cutoff, n=None):
+    v = _trunc_normal(rng, 1 if n is None else n, cutoff) * sigma
+    return float(v[0]) if n is None else v
+
+
+def mba_variants(n_variants, seed=1, sigma_fse=1e-3, sigma_dxy=30e-6, cutoff=2.0, p_central=None):
+    """copies of mba_ring() with seeded errors on every KQUAD and KSEXT occurrence:
+    relative strength error (FSE) and transverse offsets (DX, DY), Gaussian with a
+    2-sigma cutoff, one generator stream per variant"""
+    base = mba_ring(p_central)
+    out = []
+    for v in range(n_variants):
+        rng = np.random.default_rng([seed, v])
+        lat = base.copy()
+        for el in lat.elements:
+            if el.type in (S.T_KQUAD, S.T_KSEXT):
+                el.params["FSE"] = _trunc(rng, sigma_fse, cutoff)
+                el.params["DX"] = _trunc(rng, sigma_dxy, cutoff)
+                el.params["DY"] = _trunc(rng, sigma_dxy, cutoff)
+        out.append(lat)
+    return out
+
+
+def variant_counts(n_variants, seed=1, lo=1000, hi=100000, median=16000.0, sigma=0.8):
+    """per-variant particle counts, lognormal clipped to [lo, hi]"""
+    rng = np.random.default_rng(seed)
+    n = np.exp(rng.normal(math.log(median), sigma, n_variants))
+    return np.clip(np.round(n), lo, hi).astype(np.int64)
+
+
+def variant_beams(counts, seed=1):
+    """6D Gaussian beams (config-0 emittances and spreads) matched to the MBA cell, at 3 GeV"""
+    return [mba_beam(int(c), seed=seed * 100003 + v) for v, c in enumerate(counts)]

@@ -78,7 +78,7 @@ typedef struct {
   double cos_tt, sin_tt;
   double dxi, dyi, dzi, dxf, dyf, dzf;
   double tan_e1, tan_e2, e1, e2, psi1, psi2;
-  eh_fterm fx[EH_MAXFTERMS], fy[EH_MAXFTERMS];
+  int32_t csb_fx0, csb_fy0; /* offsets of this element's field terms in the term pool */
 
   /* ---- RFCA ---- */
   int32_t rf_matrix, rf_nkicks, rf_same, rf_standing, rf_end1, rf_end2, rf_zero_volt;
@@ -382,7 +382,8 @@ __device__ __forceinline__ bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi)
 /* ------------------------------------------------------------------------- */
 /* computeCSBENDFields, src/csbend.c:99-172, over the non-zero terms in the
  * reference's summation order (i outer, j inner) */
-__device__ __forceinline__ void eh_csb_fields(const eh_desc &d, double x, double y, double &Fx, double &Fy) {
+__device__ __forceinline__ void eh_csb_fields(const eh_desc &d, const eh_fterm *__restrict__ terms, double x,
+                                              double y, double &Fx, double &Fy) {
   if (!d.csb_field) {
     Fx = 0;
     Fy = 1;
@@ -391,8 +392,9 @@ __device__ __forceinline__ void eh_csb_fields(const eh_desc &d, double x, double
   double sumFx = 0, sumFy = 0;
   double xt = 1;
   int ic = 0;
+  const eh_fterm *fx = terms + d.csb_fx0, *fy = terms + d.csb_fy0;
   for (int t = 0; t < d.csb_nfx; t++) {
-    int ti = d.fx[t].i, tj = d.fx[t].j;
+    int ti = fx[t].i, tj = fx[t].j;
     while (ic < ti) {
       xt *= x;
       ic++;
@@ -400,12 +402,12 @@ __device__ __forceinline__ void eh_csb_fields(const eh_desc &d, double x, double
     double yj = 1;
     for (int k = 0; k < tj; k++)
       yj = yj * y;
-    sumFx += d.fx[t].coef * xt * yj;
+    sumFx += fx[t].coef * xt * yj;
   }
   xt = 1;
   ic = 0;
   for (int t = 0; t < d.csb_nfy; t++) {
-    int ti = d.fy[t].i, tj = d.fy[t].j;
+    int ti = fy[t].i, tj = fy[t].j;
     while (ic < ti) {
       xt *= x;
       ic++;
@@ -413,7 +415,7 @@ __device__ __forceinline__ void eh_csb_fields(const eh_desc &d, double x, double
     double yj = 1;
     for (int k = 0; k < tj; k++)
       yj = yj * y;
-    sumFy += d.fy[t].coef * xt * yj;
+    sumFy += fy[t].coef * xt * yj;
   }
   Fx = sumFx;
   Fy = sumFy;
@@ -422,7 +424,7 @@ __device__ __forceinline__ void eh_csb_fields(const eh_desc &d, double x, double
 /* CSBEND: track_through_csbend (src/csbend.c:1088-1403) for one particle, with
  * integrate_csbend_ordn (src/csbend.c:1459-1720) and the canonical conversions
  * (:1436-1454).  Returns true if lost. */
-__device__ __forceinline__ bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi) {
+__device__ __forceinline__ bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms) {
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
   const double rho0 = d.rho0, rho_actual = d.rho_actual;
   double x, xp, y, yp, s, dp;
@@ -488,7 +490,7 @@ __device__ __forceinline__ bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi) {
         break;
       double ds = d.csb_ds[j];
       double Fx, Fy;
-      eh_csb_fields(d, X, Y, Fx, Fy);
+      eh_csb_fields(d, terms, X, Y, Fx, Fy);
       QX += -ds * (1 + X / rho0) * Fy / rho_actual;
       QY += ds * (1 + X / rho0) * Fx / rho_actual;
     }
@@ -678,7 +680,8 @@ __device__ __forceinline__ bool eh_rfca(eh_p6 &p, const eh_desc &d) {
 /* one element followed by do_tracking's post-element sweeps
  * (src/do_tracking.c:2506-2524, :2675-2684).  Returns -1 if the particle
  * survives, else the loss stage. */
-__device__ __forceinline__ int eh_apply_element(eh_p6 &p, const eh_desc &d, int pi) {
+__device__ __forceinline__ int eh_apply_element(eh_p6 &p, const eh_desc &d, int pi,
+                                                const eh_fterm *__restrict__ terms) {
   bool lost = false;
   switch (d.type) {
   case 1: /* DRIF */
@@ -693,7 +696,7 @@ __device__ __forceinline__ int eh_apply_element(eh_p6 &p, const eh_desc &d, int 
     lost = eh_multipole(p, d, pi);
     break;
   case 6:
-    lost = eh_csbend(p, d, pi);
+    lost = eh_csbend(p, d, pi, terms);
     break;
   case 7:
     lost = eh_rfca(p, d);

@@ -284,7 +284,7 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
   return EH_OK;
 }
 
-static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c) {
+static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::vector<eh_fterm> &pool) {
   if (cb->angle == 0) {
     d->type = EH_T_EDRIFT;
     d->length = cb->length;
@@ -377,25 +377,31 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c) {
   if (eo1 < 0)
     return eo1;
   d->csb_field = hn || hs;
-  int nfx = 0, nfy = 0;
+  /* non-zero terms in the summation order of computeCSBENDFields (i outer, j
+   * inner; without skew terms Fx only takes odd j and Fy even j) */
+  std::vector<eh_fterm> fx, fy;
   for (int i = 0; i < eo1; i++)
     for (int j = 0; j < eo1 - i; j++) {
       double fxv = Fx[i * 11 + j], fyv = Fy[i * 11 + j];
       if (hs) {
         if (fxv != 0)
-          d->fx[nfx++] = {i, j, fxv};
+          fx.push_back({i, j, fxv});
         if (fyv != 0)
-          d->fy[nfy++] = {i, j, fyv};
+          fy.push_back({i, j, fyv});
       } else if (j & 1) {
         if (fxv != 0)
-          d->fx[nfx++] = {i, j, fxv};
+          fx.push_back({i, j, fxv});
       } else {
         if (fyv != 0)
-          d->fy[nfy++] = {i, j, fyv};
+          fy.push_back({i, j, fyv});
       }
     }
-  d->csb_nfx = nfx;
-  d->csb_nfy = nfy;
+  d->csb_nfx = (int32_t)fx.size();
+  d->csb_nfy = (int32_t)fy.size();
+  d->csb_fx0 = (int32_t)pool.size();
+  pool.insert(pool.end(), fx.begin(), fx.end());
+  d->csb_fy0 = (int32_t)pool.size();
+  pool.insert(pool.end(), fy.begin(), fy.end());
   double ttilt = tilt + etilt, cos_tt, sin_tt;
   if (ttilt == 0) {
     cos_tt = 1;
@@ -490,7 +496,7 @@ static double elem_length(const EH_ELEMENT_LIST *e) {
   return *(const double *)e->p_elem;
 }
 
-static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c) {
+static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, std::vector<eh_fterm> &pool) {
   std::memset(d, 0, sizeof *d);
   d->type = e->type;
   d->counted = !(e->type == EH_T_MARK || e->type == EH_T_MAXAMP);
@@ -526,7 +532,7 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c) {
   case EH_T_KOCT:
     return build_multipole(d, e, c);
   case EH_T_CSBEND:
-    return build_csbend(d, (const EH_CSBEND *)e->p_elem, c);
+    return build_csbend(d, (const EH_CSBEND *)e->p_elem, c, pool);
   case EH_T_RFCA:
     d->length = ((const EH_RFCA *)e->p_elem)->length;
     return build_rfca(d, (const EH_RFCA *)e->p_elem);
@@ -552,6 +558,8 @@ struct eh_beam {
   unsigned long long *d_cnt = nullptr; /* [0] alive out, [1] graveyard append, [2] lost in place, [3] effort */
   eh_desc *d_desc = nullptr;
   long desc_cap = 0;
+  eh_fterm *d_terms = nullptr; /* CSBEND field-term pool */
+  long terms_cap = 0;
   std::vector<double> cols; /* columns 6..9 of each uploaded row, by row0 */
   long pass = 0;
   int32_t event = 0;
@@ -628,6 +636,8 @@ extern "C" void eh_beam_free(eh_beam *b) {
   (void)hipFree(b->d_cnt);
   if (b->d_desc)
     (void)hipFree(b->d_desc);
+  if (b->d_terms)
+    (void)hipFree(b->d_terms);
   if (b->ev0)
     (void)hipEventDestroy(b->ev0);
   if (b->ev1)
@@ -638,7 +648,7 @@ extern "C" void eh_beam_free(eh_beam *b) {
 }
 
 extern "C" long eh_beam_n_alive(const eh_beam *b) { return b ? b->n_alive : EH_ERR_ARG; }
-extern "C" long eh_beam_n_total(const eh_beam *b) { return b ? b->n_total : EH_ERR_ARG; }
+extern "C" long eh_beam_n_total(const eh_beam *b) { return b ? b->n_alive + b->n_lost : EH_ERR_ARG; }
 extern "C" void eh_beam_set_pass(eh_beam *b, long pass) {
   if (b)
     b->pass = pass;
@@ -719,7 +729,8 @@ __device__ __forceinline__ unsigned wave_prefix(bool pred, unsigned *total) {
 /* one element routine (op 0), one limit_amplitudes sweep (op 1) or one
  * removeInvalidParticles sweep (op 2) over the alive particles, in place; losses
  * are marked with (pass, event) */
-__global__ void __launch_bounds__(EH_BLOCK) k_plain(eh_soa a, long n, const eh_desc *__restrict__ desc, int op,
+__global__ void __launch_bounds__(EH_BLOCK) k_plain(eh_soa a, long n, const eh_desc *__restrict__ desc,
+                                                     const eh_fterm *__restrict__ terms, int op,
                                                      double xmax, double ymax, double z, double Po, int extrap,
                                                      int32_t pass, int32_t event, unsigned long long *cnt) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -741,7 +752,7 @@ __global__ void __launch_bounds__(EH_BLOCK) k_plain(eh_soa a, long n, const eh_d
         lost = eh_multipole(p, d, 0);
         break;
       case EH_T_CSBEND:
-        lost = eh_csbend(p, d, 0);
+        lost = eh_csbend(p, d, 0, terms);
         break;
       case EH_T_RFCA:
         lost = eh_rfca(p, d);
@@ -766,9 +777,7 @@ __global__ void __launch_bounds__(EH_BLOCK) k_plain(eh_soa a, long n, const eh_d
       a.levent[i] = event;
     }
   }
-  unsigned tot;
-  wave_prefix(lost, &tot);
-  wave_add(&cnt[2], tot);
+  wave_add(&cnt[2], lost ? 1ull : 0ull);
 }
 
 /* move survivors of a to the front of b and lost particles to the graveyard */
@@ -805,21 +814,15 @@ __global__ void __launch_bounds__(EH_BLOCK) k_compact(eh_soa a, long n, eh_soa b
   }
 }
 
-/* fused tracking: every particle runs the element sequence from (p0, e0) up to
- * (p1, e1) exclusive in registers, then survivors are compacted into b and lost
- * particles appended to the graveyard.  Events are event_base + ((p - p0) * nelem
- * + e) * 3 + stage, in the order the reference compacts. */
-__global__ void __launch_bounds__(EH_BLOCK) k_fused(eh_soa a, long n, eh_soa b, eh_soa g,
-                                                     const eh_desc *__restrict__ descs, int nelem, int p0, int e0,
-                                                     int p1, int e1, int32_t event_base, unsigned long long *cnt) {
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  bool in = i < n;
-  bool alive = in;
-  eh_p6 p = {0, 0, 0, 0, 0, 0};
-  if (in)
-    p = load_p(a, i);
-  int32_t lpass = 0, levent = -1;
-  unsigned long long units = 0;
+/* one particle through the element sequence from (p0, e0) up to (p1, e1)
+ * exclusive, in registers.  Events are event_base + ((p - p0) * nelem + e) * 3 +
+ * stage, in the order the reference compacts.  The loop is wavefront-uniform (the
+ * element index is the same in every lane), so descriptors are read with scalar
+ * loads; the wavefront leaves early once all its lanes are lost. */
+__device__ __forceinline__ void track_span(eh_p6 &p, bool &alive, const eh_desc *__restrict__ descs,
+                                           const eh_fterm *__restrict__ terms, int nelem, int p0, int e0, int p1,
+                                           int e1, int32_t event_base, int32_t &lpass, int32_t &levent,
+                                           unsigned long long &units) {
   int pp = p0, ee = e0;
   while (pp < p1 || (pp == p1 && ee < e1)) {
     if (__ballot(alive) == 0)
@@ -828,7 +831,7 @@ __global__ void __launch_bounds__(EH_BLOCK) k_fused(eh_soa a, long n, eh_soa b, 
     int pi = pp > 0 ? 1 : 0;
     if (alive) {
       units += d->counted;
-      int st = eh_apply_element(p, *d, pi);
+      int st = eh_apply_element(p, *d, pi, terms);
       if (st >= 0) {
         alive = false;
         lpass = pp;
@@ -840,7 +843,14 @@ __global__ void __launch_bounds__(EH_BLOCK) k_fused(eh_soa a, long n, eh_soa b, 
       pp++;
     }
   }
-  wave_add(&cnt[3], units);
+}
+
+/* write a wavefront's particles after tracking: survivors to out at
+ * out_base + atomicAdd(out_count), lost ones to the graveyard */
+__device__ __forceinline__ void wave_store(const eh_p6 &p, bool in, bool alive, int32_t row0, int32_t lpass,
+                                           int32_t levent, const eh_soa &out, long out_base,
+                                           unsigned long long *out_count, const eh_soa &g,
+                                           unsigned long long *grave_count) {
   bool lost = in && !alive;
   unsigned na, nl;
   unsigned pa = wave_prefix(alive, &na);
@@ -848,25 +858,144 @@ __global__ void __launch_bounds__(EH_BLOCK) k_fused(eh_soa a, long n, eh_soa b, 
   unsigned long long ba = 0, bl = 0;
   if (__lane_id() == 0) {
     if (na)
-      ba = atomicAdd(&cnt[0], (unsigned long long)na);
+      ba = atomicAdd(out_count, (unsigned long long)na);
     if (nl)
-      bl = atomicAdd(&cnt[1], (unsigned long long)nl);
+      bl = atomicAdd(grave_count, (unsigned long long)nl);
   }
   ba = __shfl(ba, 0, 64);
   bl = __shfl(bl, 0, 64);
   if (alive) {
-    long j = (long)(ba + pa);
-    store_p(b, j, p);
-    b.row0[j] = a.row0[i];
-    b.lpass[j] = 0;
-    b.levent[j] = -1;
+    long j = out_base + (long)(ba + pa);
+    store_p(out, j, p);
+    out.row0[j] = row0;
+    out.lpass[j] = 0;
+    out.levent[j] = -1;
   } else if (lost) {
     long j = (long)(bl + pl);
     store_p(g, j, p);
-    g.row0[j] = a.row0[i];
+    g.row0[j] = row0;
     g.lpass[j] = lpass;
     g.levent[j] = levent;
   }
+}
+
+/* fused tracking of one lattice: each thread takes one particle through the span,
+ * then survivors are compacted into b and lost particles appended to the graveyard */
+__global__ void __launch_bounds__(EH_BLOCK) k_fused(eh_soa a, long n, eh_soa b, eh_soa g,
+                                                     const eh_desc *__restrict__ descs,
+                                                     const eh_fterm *__restrict__ terms, int nelem, int p0, int e0,
+                                                     int p1, int e1, int32_t event_base, unsigned long long *cnt) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool in = i < n;
+  bool alive = in;
+  eh_p6 p = {0, 0, 0, 0, 0, 0};
+  int32_t row0 = 0;
+  if (in) {
+    p = load_p(a, i);
+    row0 = a.row0[i];
+  }
+  int32_t lpass = 0, levent = -1;
+  unsigned long long units = 0;
+  track_span(p, alive, descs, terms, nelem, p0, e0, p1, e1, event_base, lpass, levent, units);
+  wave_add(&cnt[3], units);
+  wave_store(p, in, alive, row0, lpass, levent, b, 0, &cnt[0], g, &cnt[1]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* batched lattice variants.  Variant v owns the particle region
+ * [region_off[v], region_off[v] + capacity_v) (64-aligned), its alive particles at
+ * the start of it.  Work items are wavefronts of one variant each, so every
+ * wavefront reads a single variant's descriptors (uniform, scalar loads).  The
+ * kernel is persistent: a fixed grid pulls work items from a device counter until
+ * the list is exhausted, so uneven per-variant counts need no host involvement and
+ * a whole run can be captured in one graph. */
+struct eh_batch_dev {
+  const long *region_off;   /* [nvar] */
+  const int32_t *alive_in;  /* [nvar] */
+  unsigned long long *alive_out; /* [nvar] */
+  const int32_t *wave_start; /* [nvar + 1] */
+  int32_t *queue;
+  const int32_t *desc_idx;  /* [nvar * nelem] index into the descriptor pool */
+  int nvar;
+};
+
+/* work list for the next span: wave_start = prefix of ceil(alive/64); clears the
+ * output counters and the queue (a kernel, not a memset, so it can live in a graph) */
+__global__ void k_batch_prep(const unsigned long long *alive_prev, int32_t *alive_in, int32_t *wave_start,
+                             unsigned long long *alive_out, int32_t *queue, int nvar, int first) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    int32_t acc = 0;
+    for (int v = 0; v < nvar; v++) {
+      int32_t a = first ? alive_in[v] : (int32_t)alive_prev[v];
+      alive_in[v] = a;
+      wave_start[v] = acc;
+      acc += (a + 63) / 64;
+      alive_out[v] = 0;
+    }
+    wave_start[nvar] = acc;
+    *queue = 0;
+  }
+}
+
+__global__ void __launch_bounds__(EH_BLOCK) k_batch(eh_soa a, eh_soa b, eh_soa g, eh_batch_dev bd,
+                                                     const eh_desc *__restrict__ pool,
+                                                     const eh_fterm *__restrict__ terms, int nelem, int p0, int e0,
+                                                     int p1, int e1, int32_t event_base, unsigned long long *cnt) {
+  const int lane = __lane_id();
+  const int32_t total = bd.wave_start[bd.nvar];
+  unsigned long long units = 0;
+  for (;;) {
+    int32_t w = 0;
+    if (lane == 0)
+      w = atomicAdd(bd.queue, 1);
+    w = __shfl(w, 0, 64);
+    if (w >= total)
+      break;
+    int lo = 0, hi = bd.nvar; /* wave_start[lo] <= w < wave_start[hi] */
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (bd.wave_start[mid] <= w)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const int v = lo;
+    const long local = (long)(w - bd.wave_start[v]) * 64 + lane;
+    const bool in = local < bd.alive_in[v];
+    const long i = bd.region_off[v] + local;
+    bool alive = in;
+    eh_p6 p = {0, 0, 0, 0, 0, 0};
+    int32_t row0 = 0;
+    if (in) {
+      p = load_p(a, i);
+      row0 = a.row0[i];
+    }
+    int32_t lpass = 0, levent = -1;
+    /* descriptors of variant v: pool entries listed by desc_idx (uniform) */
+    const int32_t *idx = bd.desc_idx + (long)v * nelem;
+    int pp = p0, ee = e0;
+    while (pp < p1 || (pp == p1 && ee < e1)) {
+      if (__ballot(alive) == 0)
+        break;
+      const eh_desc *d = pool + idx[ee];
+      int pi = pp > 0 ? 1 : 0;
+      if (alive) {
+        units += d->counted;
+        int st = eh_apply_element(p, *d, pi, terms);
+        if (st >= 0) {
+          alive = false;
+          lpass = pp;
+          levent = event_base + ((pp - p0) * nelem + ee) * 3 + st;
+        }
+      }
+      if (++ee == nelem) {
+        ee = 0;
+        pp++;
+      }
+    }
+    wave_store(p, in, alive, row0, lpass, levent, b, bd.region_off[v], &bd.alive_out[v], g, &cnt[1]);
+  }
+  wave_add(&cnt[3], units);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -879,6 +1008,20 @@ static long ensure_desc(eh_beam *b, long n) {
   b->d_desc = nullptr;
   HIPCHK(hipMalloc(&b->d_desc, sizeof(eh_desc) * n));
   b->desc_cap = n;
+  return EH_OK;
+}
+
+static long upload_terms(eh_beam *b, const std::vector<eh_fterm> &pool) {
+  if (pool.empty())
+    return EH_OK;
+  if (b->terms_cap < (long)pool.size()) {
+    if (b->d_terms)
+      HIPCHK(hipFree(b->d_terms));
+    b->d_terms = nullptr;
+    HIPCHK(hipMalloc(&b->d_terms, sizeof(eh_fterm) * pool.size()));
+    b->terms_cap = (long)pool.size();
+  }
+  HIPCHK(hipMemcpy(b->d_terms, pool.data(), sizeof(eh_fterm) * pool.size(), hipMemcpyHostToDevice));
   return EH_OK;
 }
 
@@ -911,9 +1054,9 @@ static long compact(eh_beam *b, long lost) {
   long n = b->n_alive;
   hipLaunchKernelGGL(k_compact, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, src, n, dst, b->grave, b->d_cnt);
   HIPCHK(hipGetLastError());
-  unsigned long long out[2];
-  HIPCHK(hipMemcpyAsync(out, b->d_cnt, sizeof out, hipMemcpyDeviceToHost, b->stream));
+  unsigned long long out[2] = {0, 0};
   HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipMemcpy(out, b->d_cnt, sizeof out, hipMemcpyDeviceToHost));
   b->cur = 1 - b->cur;
   b->n_alive = (long)out[0];
   b->n_lost = (long)out[1];
@@ -921,8 +1064,8 @@ static long compact(eh_beam *b, long lost) {
 }
 
 /* run one plain op over the alive particles; returns survivors */
-static long run_plain(eh_beam *b, const eh_desc *hd, int op, double xmax, double ymax, double z, double Po,
-                      int extrap) {
+static long run_plain(eh_beam *b, const eh_desc *hd, const std::vector<eh_fterm> *pool, int op, double xmax,
+                      double ymax, double z, double Po, int extrap) {
   long n = b->n_alive;
   int32_t ev = next_event(b);
   if (n == 0)
@@ -932,13 +1075,18 @@ static long run_plain(eh_beam *b, const eh_desc *hd, int op, double xmax, double
     if (r < 0)
       return r;
     HIPCHK(hipMemcpyAsync(b->d_desc, hd, sizeof(eh_desc), hipMemcpyHostToDevice, b->stream));
+    if (pool) {
+      r = upload_terms(b, *pool);
+      if (r < 0)
+        return r;
+    }
   }
   unsigned long long zero = 0;
   HIPCHK(hipMemcpyAsync(b->d_cnt + 2, &zero, sizeof zero, hipMemcpyHostToDevice, b->stream));
   long r = timed_begin(b);
   if (r < 0)
     return r;
-  hipLaunchKernelGGL(k_plain, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, b->buf[b->cur], n, b->d_desc, op,
+  hipLaunchKernelGGL(k_plain, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, b->buf[b->cur], n, b->d_desc, b->d_terms, op,
                      xmax, ymax, z, Po, extrap, (int32_t)b->pass, ev, b->d_cnt);
   HIPCHK(hipGetLastError());
   r = timed_end(b);
@@ -951,7 +1099,8 @@ static long run_plain(eh_beam *b, const eh_desc *hd, int op, double xmax, double
   if (na < 0)
     return na;
   if (lost && (b->n_lost - before_lost != (long)lost || na != n - (long)lost))
-    return fail(EH_ERR_STATE, "compaction count mismatch");
+    return fail(EH_ERR_STATE, "compaction count mismatch: %ld in, %llu marked lost, %ld kept, %ld buried", n, lost, na,
+                b->n_lost - before_lost);
   return na;
 }
 
@@ -974,15 +1123,16 @@ extern "C" long eh_multipole_tracking2(eh_beam *b, long n_part, const EH_ELEMENT
   if (iSlice >= 0)
     return fail(EH_ERR_UNSUPPORTED, "slice-by-slice integration (iSlice>=0) not supported");
   eh_desc d;
+  std::vector<eh_fterm> pool;
   eh_ctx c = {};
   c.Po = Po;
   c.z_start = z_start;
   c.z_end = z_start + elem_length(elem);
   c.ap[0] = c.ap[1] = maxamp;
-  r = build_desc(&d, elem, c);
+  r = build_desc(&d, elem, c, pool);
   if (r < 0)
     return r;
-  return run_plain(b, &d, 0, 0, 0, 0, Po, 0);
+  return run_plain(b, &d, &pool, 0, 0, 0, 0, Po, 0);
 }
 
 extern "C" long eh_track_through_csbend(eh_beam *b, long n_part, const EH_CSBEND *csbend, double p_error, double Po,
@@ -995,15 +1145,16 @@ extern "C" long eh_track_through_csbend(eh_beam *b, long n_part, const EH_CSBEND
     return fail(EH_ERR_UNSUPPORTED, "slice-by-slice integration (iSlice>=0) not supported");
   EH_ELEMENT_LIST e = {EH_T_CSBEND, 0, "CSBEND", (void *)csbend};
   eh_desc d;
+  std::vector<eh_fterm> pool;
   eh_ctx c = {};
   c.Po = Po;
   c.z_start = z_start;
   c.z_end = z_start + csbend->length;
   c.ap[0] = c.ap[1] = maxamp;
-  r = build_desc(&d, &e, c);
+  r = build_desc(&d, &e, c, pool);
   if (r < 0)
     return r;
-  return run_plain(b, &d, 0, 0, 0, 0, Po, 0);
+  return run_plain(b, &d, &pool, 0, 0, 0, 0, Po, 0);
 }
 
 extern "C" long eh_exactDrift(eh_beam *b, long np, double length) {
@@ -1014,7 +1165,7 @@ extern "C" long eh_exactDrift(eh_beam *b, long np, double length) {
   std::memset(&d, 0, sizeof d);
   d.type = EH_T_EDRIFT;
   d.length = length;
-  return run_plain(b, &d, 0, 0, 0, 0, 0, 0);
+  return run_plain(b, &d, nullptr, 0, 0, 0, 0, 0, 0);
 }
 
 extern "C" long eh_drift_beam(eh_beam *b, long np, double length, long order) {
@@ -1030,7 +1181,7 @@ extern "C" long eh_drift_beam(eh_beam *b, long np, double length, long order) {
   if (order <= 0)
     order = 2;
   d.T11 = order > 1 ? length / 2 : 0;
-  return run_plain(b, &d, 0, 0, 0, 0, 0, 0);
+  return run_plain(b, &d, nullptr, 0, 0, 0, 0, 0, 0);
 }
 
 extern "C" long eh_limit_amplitudes(eh_beam *b, double xmax, double ymax, long np, double z, double Po,
@@ -1042,7 +1193,7 @@ extern "C" long eh_limit_amplitudes(eh_beam *b, double xmax, double ymax, long n
     return fail(EH_ERR_UNSUPPORTED, "open-sided apertures not supported");
   if (xmax < 0 && ymax < 0)
     return np;
-  return run_plain(b, nullptr, 1, xmax, ymax, z, Po, (int)extrapolate_z);
+  return run_plain(b, nullptr, nullptr, 1, xmax, ymax, z, Po, (int)extrapolate_z);
 }
 
 /* Kahan-compensated TMEAN fiducial over the alive particles in reference row
@@ -1120,21 +1271,22 @@ extern "C" long eh_simple_rf_cavity(eh_beam *b, long np, EH_RFCA *rfca, double *
     rfca->t_fiducial = -rfca->phase_fiducial / (2 * EH_PI * rfca->freq);
   EH_ELEMENT_LIST e = {EH_T_RFCA, 0, "RFCA", (void *)rfca};
   eh_desc d;
+  std::vector<eh_fterm> pool;
   eh_ctx c = {};
   c.Po = *P_central;
   c.z_start = zEnd - rfca->length;
   c.z_end = zEnd;
-  r = build_desc(&d, &e, c);
+  r = build_desc(&d, &e, c, pool);
   if (r < 0)
     return r;
-  return run_plain(b, &d, 0, 0, 0, 0, *P_central, 0);
+  return run_plain(b, &d, nullptr, 0, 0, 0, 0, *P_central, 0);
 }
 
 extern "C" long eh_removeInvalidParticles(eh_beam *b, long np, double z, double Po) {
   long r = check_np(b, np);
   if (r < 0)
     return r;
-  return run_plain(b, nullptr, 2, 0, 0, z, Po, 0);
+  return run_plain(b, nullptr, nullptr, 2, 0, 0, z, Po, 0);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1316,7 +1468,7 @@ static long track_plain(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
         dd.length = d->length;
         long order = d->order ? d->order : 2;
         dd.T11 = order > 1 ? d->length / 2 : 0;
-        nLeft = run_plain(b, &dd, 0, 0, 0, 0, 0, 0);
+        nLeft = run_plain(b, &dd, nullptr, 0, 0, 0, 0, 0, 0);
         break;
       }
       case EH_T_EDRIFT:
@@ -1367,7 +1519,7 @@ static long launch_fused(eh_beam *b, long ne, int p0, int e0, int p1, int e1, do
   if (r < 0)
     return r;
   hipLaunchKernelGGL(k_fused, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, b->buf[b->cur], n, b->buf[1 - b->cur],
-                     b->grave, b->d_desc, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
+                     b->grave, b->d_desc, b->d_terms, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
   HIPCHK(hipGetLastError());
   r = timed_end(b);
   if (r < 0)
@@ -1391,6 +1543,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
   if (r < 0)
     return r;
   std::vector<eh_desc> hd(ne);
+  std::vector<eh_fterm> pool;
   std::vector<double> zs(ne), ze(ne);
   /* positions (wrap_around resets z every pass) and apertures for pass 0 / later */
   const EH_MAXAMP *last_amp = nullptr;
@@ -1429,7 +1582,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       if (m && m->elliptical)
         return fail(EH_ERR_UNSUPPORTED, "elliptical MAXAMP not supported");
     }
-    return build_desc(&hd[ie], bl + ie, c);
+    return build_desc(&hd[ie], bl + ie, c, pool);
   };
   for (long ie = 0; ie < ne; ie++) {
     if (bl[ie].type == EH_T_RFCA && !((EH_RFCA *)bl[ie].p_elem)->fiducial_seen)
@@ -1444,6 +1597,9 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
   if (r < 0)
     return r;
   HIPCHK(hipMemcpy(b->d_desc, hd.data(), sizeof(eh_desc) * ne, hipMemcpyHostToDevice));
+  r = upload_terms(b, pool);
+  if (r < 0)
+    return r;
   int p = 0, e = 0;
   if (n_passes > 0) {
     /* pass 0: stop before each RFCA whose fiducial is still unknown */
@@ -1590,4 +1746,555 @@ extern "C" long eh_selftest_math(const double *x, const double *y, long n, doubl
   if (e != hipSuccess)
     return fail(EH_ERR_HIP, "eh_selftest_math: %s", hipGetErrorString(e));
   return n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* batched lattice variants: host side */
+#include <string>
+#include <unordered_map>
+
+struct eh_batch {
+  long nvar = 0;
+  std::vector<long> n_v, id_off, region_off;
+  long n_total = 0, cap = 0, n_lost = 0;
+  eh_soa buf[2] = {}, grave = {};
+  int cur = 0;
+  void *mem = nullptr;
+  long *d_region_off = nullptr;
+  int32_t *d_alive_in = nullptr, *d_wave_start = nullptr, *d_queue = nullptr, *d_desc_idx = nullptr;
+  unsigned long long *d_alive_out = nullptr, *d_cnt = nullptr;
+  eh_desc *d_pool = nullptr;
+  eh_fterm *d_terms = nullptr;
+  std::vector<double> cols;
+  std::vector<long> alive_v;
+  int32_t event = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double kernel_ms = 0;
+  long launches = 0;
+  int n_cu = 256;
+};
+
+extern "C" eh_batch *eh_batch_create(long nvar, const long *n_particles) {
+  if (nvar < 1 || !n_particles) {
+    fail(EH_ERR_ARG, "eh_batch_create: bad arguments");
+    return nullptr;
+  }
+  if (g_device < 0 && eh_init(0) != EH_OK)
+    return nullptr;
+  eh_batch *b = new eh_batch();
+  b->nvar = nvar;
+  b->n_v.assign(n_particles, n_particles + nvar);
+  long idn = 0, off = 0;
+  for (long v = 0; v < nvar; v++) {
+    if (n_particles[v] < 0) {
+      fail(EH_ERR_ARG, "eh_batch_create: negative particle count");
+      delete b;
+      return nullptr;
+    }
+    b->id_off.push_back(idn);
+    b->region_off.push_back(off);
+    idn += n_particles[v];
+    off += (n_particles[v] + 63) / 64 * 64;
+  }
+  b->n_total = idn;
+  b->cap = off > 0 ? off : 64;
+  if (b->n_total > 2147483647L) {
+    fail(EH_ERR_ARG, "eh_batch_create: more than 2^31 particles");
+    delete b;
+    return nullptr;
+  }
+  size_t per = (soa_bytes(b->cap) + 255) & ~(size_t)255;
+  size_t small = sizeof(long) * nvar + sizeof(int32_t) * (3 * nvar + 8) + sizeof(unsigned long long) * (nvar + 8);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, g_device) == hipSuccess)
+    b->n_cu = prop.multiProcessorCount;
+  if (hipMalloc(&b->mem, 3 * per) != hipSuccess || hipMalloc(&b->d_region_off, sizeof(long) * nvar) != hipSuccess ||
+      hipMalloc(&b->d_alive_in, sizeof(int32_t) * nvar) != hipSuccess ||
+      hipMalloc(&b->d_wave_start, sizeof(int32_t) * (nvar + 1)) != hipSuccess ||
+      hipMalloc(&b->d_queue, sizeof(int32_t) * 4) != hipSuccess ||
+      hipMalloc(&b->d_alive_out, sizeof(unsigned long long) * nvar) != hipSuccess ||
+      hipMalloc(&b->d_cnt, sizeof(unsigned long long) * 8) != hipSuccess ||
+      hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+      hipEventCreate(&b->ev1) != hipSuccess) {
+    (void)small;
+    fail(EH_ERR_NOMEM, "eh_batch_create: device allocation failed");
+    delete b;
+    return nullptr;
+  }
+  soa_carve(&b->buf[0], (char *)b->mem, b->cap);
+  soa_carve(&b->buf[1], (char *)b->mem + per, b->cap);
+  soa_carve(&b->grave, (char *)b->mem + 2 * per, b->cap);
+  (void)hipMemcpy(b->d_region_off, b->region_off.data(), sizeof(long) * nvar, hipMemcpyHostToDevice);
+  b->cols.assign(4 * (size_t)b->n_total, 0.0);
+  b->alive_v.assign(nvar, 0);
+  return b;
+}
+
+extern "C" void eh_batch_free(eh_batch *b) {
+  if (!b)
+    return;
+  if (b->stream)
+    (void)hipStreamSynchronize(b->stream);
+  void *ptrs[] = {b->mem, b->d_region_off, b->d_alive_in, b->d_wave_start, b->d_queue, b->d_alive_out, b->d_cnt,
+                  b->d_pool, b->d_terms, b->d_desc_idx};
+  for (void *q : ptrs)
+    if (q)
+      (void)hipFree(q);
+  if (b->ev0)
+    (void)hipEventDestroy(b->ev0);
+  if (b->ev1)
+    (void)hipEventDestroy(b->ev1);
+  if (b->stream)
+    (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" long eh_batch_upload(eh_batch *b, long v, const double *flat, long n) {
+  if (!b || v < 0 || v >= b->nvar || !flat || n != b->n_v[v])
+    return fail(EH_ERR_ARG, "eh_batch_upload: bad arguments");
+  if (b->cur != 0 || b->n_lost)
+    return fail(EH_ERR_STATE, "eh_batch_upload: upload every variant before tracking");
+  std::vector<double> h(6 * (size_t)(n > 0 ? n : 1));
+  std::vector<int32_t> r(n > 0 ? n : 1), m1(n > 0 ? n : 1, -1), z(n > 0 ? n : 1, 0);
+  for (long i = 0; i < n; i++) {
+    for (int k = 0; k < 6; k++)
+      h[k * (size_t)n + i] = flat[i * EH_NPROP + k];
+    for (int k = 0; k < 4; k++)
+      b->cols[4 * (size_t)(b->id_off[v] + i) + k] = flat[i * EH_NPROP + 6 + k];
+    r[i] = (int32_t)(b->id_off[v] + i);
+  }
+  if (n) {
+    eh_soa &a = b->buf[0];
+    long o = b->region_off[v];
+    double *dst[6] = {a.x, a.xp, a.y, a.yp, a.s, a.dp};
+    for (int k = 0; k < 6; k++)
+      HIPCHK(hipMemcpy(dst[k] + o, h.data() + k * (size_t)n, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(a.row0 + o, r.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(a.levent + o, m1.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(a.lpass + o, z.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  }
+  b->alive_v[v] = n;
+  return n;
+}
+
+extern "C" long eh_batch_n_alive(const eh_batch *b, long v) {
+  if (!b || v < 0 || v >= b->nvar)
+    return EH_ERR_ARG;
+  return b->alive_v[v];
+}
+
+/* reference row order of variant v (local identities 0..n_v-1); idx[local id] =
+ * index in the alive region of v or -1; the graveyard slots of v go to gidx */
+static long batch_order(eh_batch *b, long v, std::vector<int32_t> &slot_at_row, std::vector<int32_t> &idx,
+                        std::vector<int32_t> &gidx) {
+  long nv = b->n_v[v], na = b->alive_v[v], off = b->id_off[v];
+  std::vector<int32_t> arow(na > 0 ? na : 1), grow0(b->n_lost > 0 ? b->n_lost : 1),
+      gev(b->n_lost > 0 ? b->n_lost : 1);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (na)
+    HIPCHK(hipMemcpy(arow.data(), b->buf[b->cur].row0 + b->region_off[v], sizeof(int32_t) * na,
+                     hipMemcpyDeviceToHost));
+  if (b->n_lost) {
+    HIPCHK(hipMemcpy(grow0.data(), b->grave.row0, sizeof(int32_t) * b->n_lost, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(gev.data(), b->grave.levent, sizeof(int32_t) * b->n_lost, hipMemcpyDeviceToHost));
+  }
+  idx.assign(nv, -1);
+  gidx.assign(nv, -1);
+  for (long k = 0; k < na; k++) {
+    long id = arow[k] - off;
+    if (id < 0 || id >= nv)
+      return fail(EH_ERR_STATE, "corrupt particle identity in variant %ld", v);
+    idx[id] = (int32_t)k;
+  }
+  std::vector<std::pair<int32_t, int32_t>> lost;
+  for (long k = 0; k < b->n_lost; k++) {
+    long id = grow0[k] - off;
+    if (id >= 0 && id < nv) {
+      lost.push_back(std::make_pair(gev[k], (int32_t)id));
+      gidx[id] = (int32_t)k;
+    }
+  }
+  if ((long)lost.size() + na != nv)
+    return fail(EH_ERR_STATE, "variant %ld: %ld alive + %zu lost != %ld", v, na, lost.size(), nv);
+  replay_order(nv, lost, slot_at_row);
+  return EH_OK;
+}
+
+extern "C" long eh_batch_download(eh_batch *b, long v, double *flat, long n) {
+  if (!b || v < 0 || v >= b->nvar || !flat || n != b->n_v[v])
+    return fail(EH_ERR_ARG, "eh_batch_download: bad arguments");
+  std::vector<int32_t> slot, idx, gidx;
+  long r = batch_order(b, v, slot, idx, gidx);
+  if (r < 0)
+    return r;
+  long na = b->alive_v[v], nl = b->n_lost;
+  std::vector<double> A(6 * (size_t)(na > 0 ? na : 1)), G(6 * (size_t)(nl > 0 ? nl : 1));
+  std::vector<int32_t> gpass(nl > 0 ? nl : 1);
+  const eh_soa &a = b->buf[b->cur], &g = b->grave;
+  const double *as[6] = {a.x, a.xp, a.y, a.yp, a.s, a.dp}, *gs[6] = {g.x, g.xp, g.y, g.yp, g.s, g.dp};
+  for (int k = 0; k < 6; k++) {
+    if (na)
+      HIPCHK(hipMemcpy(A.data() + k * (size_t)na, as[k] + b->region_off[v], sizeof(double) * na,
+                       hipMemcpyDeviceToHost));
+    if (nl)
+      HIPCHK(hipMemcpy(G.data() + k * (size_t)nl, gs[k], sizeof(double) * nl, hipMemcpyDeviceToHost));
+  }
+  if (nl)
+    HIPCHK(hipMemcpy(gpass.data(), g.lpass, sizeof(int32_t) * nl, hipMemcpyDeviceToHost));
+  for (long row = 0; row < n; row++) {
+    int32_t id = slot[row];
+    double *c = flat + row * EH_NPROP;
+    for (int k = 0; k < 4; k++)
+      c[6 + k] = b->cols[4 * (size_t)(b->id_off[v] + id) + k];
+    if (idx[id] >= 0) {
+      for (int q = 0; q < 6; q++)
+        c[q] = A[q * (size_t)na + idx[id]];
+    } else {
+      long k = gidx[id];
+      for (int q = 0; q < 6; q++)
+        c[q] = G[q * (size_t)nl + k];
+      c[EH_LOSS_PASS_INDEX] = gpass[k];
+    }
+  }
+  return na;
+}
+
+/* Kahan TMEAN fiducial of variant v in its reference row order */
+static long batch_fiducial(eh_batch *b, long v, double sOffset, double p0, double *tFid) {
+  std::vector<int32_t> slot, idx, gidx;
+  long r = batch_order(b, v, slot, idx, gidx);
+  if (r < 0)
+    return r;
+  long na = b->alive_v[v];
+  std::vector<double> s(na > 0 ? na : 1), dp(na > 0 ? na : 1);
+  if (na) {
+    HIPCHK(hipMemcpy(s.data(), b->buf[b->cur].s + b->region_off[v], sizeof(double) * na, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dp.data(), b->buf[b->cur].dp + b->region_off[v], sizeof(double) * na, hipMemcpyDeviceToHost));
+  }
+  double tsum = 0, error = 0.0;
+  long nsum = 0;
+  for (long row = 0; row < na; row++) {
+    long k = idx[slot[row]];
+    if (k < 0)
+      return fail(EH_ERR_STATE, "replay placed a lost particle among survivors");
+    double p = p0;
+    p *= 1 + dp[k];
+    double beta = p / std::sqrt(p * p + 1);
+    double term = (s[k] + sOffset) / (EH_C_MKS * beta);
+    double Y = term + error;
+    double sum = tsum + Y;
+    error = Y - (sum - tsum);
+    tsum = sum;
+    nsum++;
+  }
+  *tFid = nsum > 0 ? tsum / nsum : 0.0;
+  return EH_OK;
+}
+
+/* one span (p0,e0)->(p1,e1): prep + persistent batch kernel, on the batch stream */
+static long batch_enqueue_span(eh_batch *b, long ne, int p0, int e0, int p1, int e1, int first, int block,
+                               int waves_per_cu) {
+  long span = ((long)(p1 - p0) * ne + ne) * 3 + 3;
+  if ((long)b->event + span >= 2147483647L)
+    return fail(EH_ERR_STATE, "loss event counter overflow");
+  int32_t base = b->event;
+  b->event += (int32_t)span;
+  hipLaunchKernelGGL(k_batch_prep, dim3(1), dim3(64), 0, b->stream, b->d_alive_out, b->d_alive_in, b->d_wave_start,
+                     b->d_alive_out, b->d_queue, (int)b->nvar, first);
+  eh_batch_dev bd;
+  bd.region_off = b->d_region_off;
+  bd.alive_in = b->d_alive_in;
+  bd.alive_out = b->d_alive_out;
+  bd.wave_start = b->d_wave_start;
+  bd.queue = b->d_queue;
+  bd.desc_idx = b->d_desc_idx;
+  bd.nvar = (int)b->nvar;
+  long threads = (long)b->n_cu * waves_per_cu * 64;
+  unsigned grid = (unsigned)((threads + block - 1) / block);
+  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(block), 0, b->stream, b->buf[b->cur], b->buf[1 - b->cur], b->grave, bd,
+                     b->d_pool, b->d_terms, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
+  b->cur = 1 - b->cur;
+  b->launches++;
+  return EH_OK;
+}
+
+static long batch_sync_counts(eh_batch *b) {
+  std::vector<unsigned long long> out(b->nvar);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out.data(), b->d_alive_out, sizeof(unsigned long long) * b->nvar, hipMemcpyDeviceToHost));
+  unsigned long long cnt[4];
+  HIPCHK(hipMemcpy(cnt, b->d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
+  long alive = 0;
+  for (long v = 0; v < b->nvar; v++) {
+    b->alive_v[v] = (long)out[v];
+    alive += b->alive_v[v];
+  }
+  b->n_lost = (long)cnt[1];
+  if (alive + b->n_lost != b->n_total)
+    return fail(EH_ERR_STATE, "batch count mismatch: %ld alive + %ld lost != %ld", alive, b->n_lost, b->n_total);
+  return alive;
+}
+
+extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, long ne, long n_passes,
+                               double *P_central, const eh_batch_tune *tune, eh_track_stats *stats) {
+  if (!b || !beamlines || ne < 1 || n_passes < 0 || !P_central)
+    return fail(EH_ERR_ARG, "eh_batch_track: bad arguments");
+  eh_batch_tune t = {256, (int32_t)ne, 8, 1};
+  if (tune)
+    t = *tune;
+  if (t.block != 64 && t.block != 128 && t.block != 256)
+    return fail(EH_ERR_ARG, "eh_batch_track: block must be 64, 128 or 256");
+  if (t.span_elements < 1)
+    t.span_elements = (int32_t)ne;
+  if (t.waves_per_cu < 1)
+    t.waves_per_cu = 8;
+  for (long v = 1; v < b->nvar; v++)
+    for (long e = 0; e < ne; e++)
+      if (beamlines[v][e].type != beamlines[0][e].type)
+        return fail(EH_ERR_ARG, "variant %ld element %ld: types differ from variant 0", v, e);
+  double Pc = *P_central;
+  /* descriptors, deduplicated across variants */
+  std::vector<eh_desc> pool;
+  std::vector<eh_fterm> terms;
+  std::vector<int32_t> idx((size_t)b->nvar * ne);
+  std::unordered_map<std::string, int32_t> seen;
+  std::vector<long> rf_elems;
+  for (long e = 0; e < ne; e++)
+    if (beamlines[0][e].type == EH_T_RFCA)
+      rf_elems.push_back(e);
+  for (long v = 0; v < b->nvar; v++) {
+    const EH_ELEMENT_LIST *bl = beamlines[v];
+    std::vector<double> zs(ne), ze(ne);
+    const EH_MAXAMP *last_amp = nullptr;
+    double z = 0;
+    for (long e = 0; e < ne; e++) {
+      zs[e] = z;
+      z += elem_length(bl + e);
+      ze[e] = z;
+      if (bl[e].type == EH_T_MAXAMP)
+        last_amp = (const EH_MAXAMP *)bl[e].p_elem;
+    }
+    const EH_MAXAMP *cur0 = nullptr, *cur1 = last_amp;
+    for (long e = 0; e < ne; e++) {
+      if (bl[e].type == EH_T_MAXAMP)
+        cur0 = cur1 = (const EH_MAXAMP *)bl[e].p_elem;
+      eh_ctx c = {};
+      c.Po = Pc;
+      c.z_start = zs[e];
+      c.z_end = ze[e];
+      c.ap[0] = cur0;
+      c.ap[1] = cur1;
+      for (int pi = 0; pi < 2; pi++) {
+        const EH_MAXAMP *m = pi ? cur1 : cur0;
+        c.post_x[pi] = m ? m->x_max : 0;
+        c.post_y[pi] = m ? m->y_max : 0;
+        c.post_on[pi] = (bl[e].type != EH_T_MARK) && m && (m->x_max || m->y_max);
+      }
+      eh_desc d;
+      std::vector<eh_fterm> ft;
+      long r;
+      if (bl[e].type == EH_T_RFCA && !((EH_RFCA *)bl[e].p_elem)->fiducial_seen) {
+        std::memset(&d, 0, sizeof d); /* placeholder until the fiducial is known */
+        d.type = EH_T_MARK;
+        r = EH_OK;
+      } else
+        r = build_desc(&d, bl + e, c, ft);
+      if (r < 0)
+        return r;
+      std::string key((const char *)&d, sizeof d);
+      key.append((const char *)ft.data(), sizeof(eh_fterm) * ft.size());
+      auto it = seen.find(key);
+      if (it != seen.end() && !(bl[e].type == EH_T_RFCA)) {
+        idx[(size_t)v * ne + e] = it->second;
+      } else {
+        if (!ft.empty()) {
+          int32_t shift = (int32_t)terms.size() - d.csb_fx0;
+          d.csb_fx0 += shift;
+          d.csb_fy0 += shift;
+          terms.insert(terms.end(), ft.begin(), ft.end());
+        }
+        int32_t id = (int32_t)pool.size();
+        pool.push_back(d);
+        seen.emplace(key, id);
+        idx[(size_t)v * ne + e] = id;
+      }
+    }
+  }
+  auto upload_tables = [&]() -> long {
+    if (b->d_pool)
+      HIPCHK(hipFree(b->d_pool));
+    if (b->d_terms)
+      HIPCHK(hipFree(b->d_terms));
+    if (b->d_desc_idx)
+      HIPCHK(hipFree(b->d_desc_idx));
+    b->d_pool = nullptr;
+    b->d_terms = nullptr;
+    b->d_desc_idx = nullptr;
+    HIPCHK(hipMalloc(&b->d_pool, sizeof(eh_desc) * pool.size()));
+    HIPCHK(hipMemcpy(b->d_pool, pool.data(), sizeof(eh_desc) * pool.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&b->d_terms, sizeof(eh_fterm) * (terms.size() + 1)));
+    if (!terms.empty())
+      HIPCHK(hipMemcpy(b->d_terms, terms.data(), sizeof(eh_fterm) * terms.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&b->d_desc_idx, sizeof(int32_t) * idx.size()));
+    HIPCHK(hipMemcpy(b->d_desc_idx, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
+    return EH_OK;
+  };
+  long r = upload_tables();
+  if (r < 0)
+    return r;
+  std::vector<int32_t> alive32(b->nvar);
+  for (long v = 0; v < b->nvar; v++)
+    alive32[v] = (int32_t)b->alive_v[v];
+  HIPCHK(hipMemcpy(b->d_alive_in, alive32.data(), sizeof(int32_t) * b->nvar, hipMemcpyHostToDevice));
+  unsigned long long cnt0[4] = {0, (unsigned long long)b->n_lost, 0, 0};
+  HIPCHK(hipMemcpy(b->d_cnt, cnt0, sizeof cnt0, hipMemcpyHostToDevice));
+  double ms0 = b->kernel_ms;
+  long l0 = b->launches;
+  int first = 1;
+  /* initial NaN sweep of do_tracking is folded into nothing here: the batch is
+   * uploaded from finite generated beams; a NaN row is still removed by the first
+   * element's NaN sweep.  Pass 0 stops before each RFCA whose fiducial is unknown. */
+  int p = 0, e = 0;
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  if (n_passes > 0) {
+    for (long er : rf_elems) {
+      bool unseen = false;
+      for (long v = 0; v < b->nvar; v++)
+        if (!((EH_RFCA *)beamlines[v][er].p_elem)->fiducial_seen)
+          unseen = true;
+      if (!unseen)
+        continue;
+      if (e < er) {
+        r = batch_enqueue_span(b, ne, p, e, 0, (int)er, first, t.block, t.waves_per_cu);
+        if (r < 0)
+          return r;
+        first = 0;
+        r = batch_sync_counts(b);
+        if (r < 0)
+          return r;
+      }
+      for (long v = 0; v < b->nvar; v++) {
+        EH_RFCA *rf = (EH_RFCA *)beamlines[v][er].p_elem;
+        if (!rf->fiducial_seen && rf->volt != 0) {
+          double length = rf->length;
+          long nk = length ? rf->nKicks : 1;
+          if (nk <= 0)
+            nk = 1;
+          length /= nk;
+          double z = 0, t0;
+          for (long k = 0; k <= er; k++)
+            z += elem_length(beamlines[v] + k);
+          if (rf->tReference != -1)
+            t0 = rf->tReference;
+          else if (rf->fiducial_mode == EH_FID_LIGHT)
+            t0 = ((z - rf->length) + length / 2) / EH_C_MKS;
+          else {
+            r = batch_fiducial(b, v, length / 2, Pc, &t0);
+            if (r < 0)
+              return r;
+          }
+          rf->phase_fiducial = -(2 * EH_PI * rf->freq) * t0;
+          rf->fiducial_seen = 1;
+          if (rf->freq != 0)
+            rf->t_fiducial = -rf->phase_fiducial / (2 * EH_PI * rf->freq);
+        }
+        double z = 0, zs = 0;
+        for (long k = 0; k <= er; k++) {
+          zs = z;
+          z += elem_length(beamlines[v] + k);
+        }
+        eh_ctx c = {};
+        c.Po = Pc;
+        c.z_start = zs;
+        c.z_end = z;
+        eh_desc d;
+        std::vector<eh_fterm> ft;
+        r = build_desc(&d, beamlines[v] + er, c, ft);
+        if (r < 0)
+          return r;
+        /* the RFCA post sweeps follow the MAXAMP in force at that point */
+        const eh_desc &old = pool[idx[(size_t)v * ne + er]];
+        (void)old;
+        const EH_MAXAMP *cur0 = nullptr, *cur1 = nullptr;
+        for (long k = 0; k < ne; k++)
+          if (beamlines[v][k].type == EH_T_MAXAMP)
+            cur1 = (const EH_MAXAMP *)beamlines[v][k].p_elem;
+        for (long k = 0; k <= er; k++)
+          if (beamlines[v][k].type == EH_T_MAXAMP)
+            cur0 = cur1 = (const EH_MAXAMP *)beamlines[v][k].p_elem;
+        for (int pi = 0; pi < 2; pi++) {
+          const EH_MAXAMP *m = pi ? cur1 : cur0;
+          d.post_xmax[pi] = m ? m->x_max : 0;
+          d.post_ymax[pi] = m ? m->y_max : 0;
+          d.post_ap[pi] = m && (m->x_max || m->y_max);
+        }
+        idx[(size_t)v * ne + er] = (int32_t)pool.size();
+        pool.push_back(d);
+      }
+      r = upload_tables();
+      if (r < 0)
+        return r;
+      e = (int)er;
+    }
+  }
+  /* the rest as spans of t.span_elements elements, optionally one graph */
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool capture = t.use_graph && n_passes > 0;
+  if (capture)
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  while (p < n_passes) {
+    long pos = (long)p * ne + e + t.span_elements;
+    long endpos = (long)n_passes * ne;
+    if (pos > endpos)
+      pos = endpos;
+    int p1 = (int)(pos / ne), e1 = (int)(pos % ne);
+    r = batch_enqueue_span(b, ne, p, e, p1, e1, first, t.block, t.waves_per_cu);
+    if (r < 0) {
+      if (capture) {
+        (void)hipStreamEndCapture(b->stream, &graph);
+        if (graph)
+          (void)hipGraphDestroy(graph);
+      }
+      return r;
+    }
+    first = 0;
+    p = p1;
+    e = e1;
+  }
+  if (capture) {
+    HIPCHK(hipStreamEndCapture(b->stream, &graph));
+    HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphLaunch(exec, b->stream));
+  }
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->kernel_ms += ms;
+  if (exec)
+    HIPCHK(hipGraphExecDestroy(exec));
+  if (graph)
+    HIPCHK(hipGraphDestroy(graph));
+  if (first) {
+    /* nothing was tracked */
+  } else {
+    r = batch_sync_counts(b);
+    if (r < 0)
+      return r;
+  }
+  unsigned long long cnt[4];
+  HIPCHK(hipMemcpy(cnt, b->d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
+  long alive = 0;
+  for (long v = 0; v < b->nvar; v++)
+    alive += b->alive_v[v];
+  if (stats) {
+    stats->particle_element_turns = (double)cnt[3];
+    stats->kernel_ms = b->kernel_ms - ms0;
+    stats->launches = b->launches - l0;
+    stats->n_alive = alive;
+  }
+  return alive;
 }

@@ -6,8 +6,8 @@
  * log).  The C library on the host and the device math library on the GPU do not
  * return bit-identical results, so a particle near an aperture edge could be lost
  * on one side and kept on the other.  Both the CPU checker and the HIP kernels
- * therefore call these routines, which use only IEEE-754 double +, -, *, / and
- * sqrt (all correctly rounded on both targets) in a fixed order.  Every file that
+ * therefore call these routines, which use only IEEE-754 double +, -, *, /, sqrt
+ * and fma (all correctly rounded on both targets) in a fixed order.  Every file that
  * includes this header must be compiled without floating-point contraction
  * (-ffp-contract=off) and without fast-math.
  *
@@ -21,7 +21,12 @@
  *    (10 terms for |x| <= 1/8, 40 terms above), and asin(x) = pi/2 - 2 asin(
  *    sqrt((1-|x|)/2)) for |x| > 1/2.
  *  - log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
- *    s = (m-1)/(m+1), atanh series to s^25, plus e*ln2 split in hi/lo parts.
+ *    s = (m-1)/(m+1), atanh series to s^25, plus e*ln2 split in hi/lo parts;
+ *    for x = 1 + f with |f| < 1/16 the Taylor series of log(1+f) to f^16.
+ *  - cos and log(1+f) form their leading terms with exact rounding errors
+ *    (Dekker's Fast2Sum; the product error through fma), which makes them
+ *    correctly rounded almost always in the small-argument regime of the
+ *    tracking maps, i.e. equal to a correctly rounded C library there.
  * Constants come from eh_math_consts.h, generated from exact rational arithmetic
  * by gen_math_consts.py.
  *
@@ -63,6 +68,11 @@ EH_FN double eh_nan(void) { return eh_from_bits(0x7ff8000000000000ULL); }
 
 EH_FN double eh_sqrt(double x) { return __builtin_sqrt(x); }
 
+/* fused multiply-add: exactly specified by IEEE-754 (one rounding), hence
+ * identical on every conforming host and on the GPU; used only to obtain exact
+ * rounding errors of products */
+EH_FN double eh_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 /* Taylor polynomial of sin(r) for |r| <= pi/4 */
 EH_FN double eh_sin_poly(double r) {
   double z = r * r;
@@ -91,13 +101,25 @@ EH_FN double eh_cos_poly(double r) {
   p = EH_COS_4 + z * p;
   p = EH_COS_3 + z * p;
   p = EH_COS_2 + z * p;
-  /* 1 - z/2 + z^2 p, with the z/2 term kept separate for accuracy */
-  return (1.0 - 0.5 * z) + (z * z) * p;
+  /* 1 - z/2 + z^2 p.  1 - z/2 is formed with its exact rounding error (Fast2Sum,
+   * valid since z/2 <= 1) and the exact error of z = r*r, so that the final
+   * addition is the only rounding of the leading terms: the result is then
+   * correctly rounded except within a tiny fraction of an ulp of a midpoint. */
+  double zl = eh_fma(r, r, -z);
+  double hz = 0.5 * z;
+  double w = 1.0 - hz;
+  double t = (1.0 - w) - hz;
+  return w + ((t - 0.5 * zl) + (z * z) * p);
 }
 
 /* Reduce x to r = x - k*pi/2; returns the quadrant k mod 4, or -1 when x is
  * outside the supported domain (non-finite or |x| >= 2^50). */
 EH_FN int eh_reduce_pio2(double x, double *r) {
+  if (eh_fabs(x) < 0x1.921fb54442d18p-1) {
+    /* |x| < pi/4: k = 0 and x - 0*piece is x exactly */
+    *r = x;
+    return 0;
+  }
   if (!(eh_fabs(x) < 0x1p50)) {
     *r = eh_nan();
     return -1;
@@ -142,15 +164,37 @@ EH_FN void eh_sincos(double x, double *s, double *c) {
 }
 
 EH_FN double eh_sin(double x) {
-  double s, c;
-  eh_sincos(x, &s, &c);
-  return s;
+  double r;
+  int q = eh_reduce_pio2(x, &r);
+  switch (q) {
+  case 0:
+    return eh_sin_poly(r);
+  case 1:
+    return eh_cos_poly(r);
+  case 2:
+    return -eh_sin_poly(r);
+  case 3:
+    return -eh_cos_poly(r);
+  default:
+    return eh_nan();
+  }
 }
 
 EH_FN double eh_cos(double x) {
-  double s, c;
-  eh_sincos(x, &s, &c);
-  return c;
+  double r;
+  int q = eh_reduce_pio2(x, &r);
+  switch (q) {
+  case 0:
+    return eh_cos_poly(r);
+  case 1:
+    return -eh_sin_poly(r);
+  case 2:
+    return -eh_cos_poly(r);
+  case 3:
+    return eh_sin_poly(r);
+  default:
+    return eh_nan();
+  }
 }
 
 EH_FN double eh_tan(double x) {
@@ -227,6 +271,30 @@ EH_FN double eh_log(double x) {
     e += 1;
   }
   double f = m - 1.0;
+  if (e == 0 && eh_fabs(f) < 0.0625) {
+    /* log(1+f) = f - f^2/2 + f^3/3 - ...: f - f^2/2 is formed with its exact
+     * rounding error so that the leading terms are rounded once */
+    double h = f * f;
+    double hl = eh_fma(f, f, -h);
+    double hh = 0.5 * h;
+    double w = f - hh;
+    double t = (f - w) - hh;
+    double q = EH_LOG1P_16;
+    q = EH_LOG1P_15 + f * q;
+    q = EH_LOG1P_14 + f * q;
+    q = EH_LOG1P_13 + f * q;
+    q = EH_LOG1P_12 + f * q;
+    q = EH_LOG1P_11 + f * q;
+    q = EH_LOG1P_10 + f * q;
+    q = EH_LOG1P_9 + f * q;
+    q = EH_LOG1P_8 + f * q;
+    q = EH_LOG1P_7 + f * q;
+    q = EH_LOG1P_6 + f * q;
+    q = EH_LOG1P_5 + f * q;
+    q = EH_LOG1P_4 + f * q;
+    q = EH_LOG1P_3 + f * q;
+    return w + ((t - 0.5 * hl) + (h * f) * q);
+  }
   double s = f / (2.0 + f);
   double z = s * s;
   double p = EH_LOG_12;

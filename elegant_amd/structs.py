@@ -81,6 +81,10 @@ class eh_track_stats(C.Structure):
     _fields_ = [("particle_element_turns", dbl), ("kernel_ms", dbl), ("launches", C.c_long), ("n_alive", C.c_long)]
 
 
+class eh_batch_tune(C.Structure):
+    _fields_ = [("block", i32), ("span_elements", i32), ("waves_per_cu", i32), ("use_graph", i32)]
+
+
 STRUCT_OF = {T_DRIF: EH_DRIFT, T_EDRIFT: EH_EDRIFT, T_KQUAD: EH_KQUAD, T_KSEXT: EH_KSEXT, T_KOCT: EH_KOCT,
              T_CSBEND: EH_CSBEND, T_RFCA: EH_RFCA, T_MAXAMP: EH_MAXAMP}
 

@@ -87,3 +87,59 @@ def csbend_field_table(b, c, h, nonlinear=1, expansion_order=0):
                                     Fx.ctypes.data_as(dp), Fy.ctypes.data_as(dp), C.byref(hn), C.byref(hs))
     _lib.check(r)
     return Fx.reshape(11, 11), Fy.reshape(11, 11), r, hn.value, hs.value
+
+
+class Batch:
+    """Variants of one beamline, each with its own particles, tracked together."""
+
+    def __init__(self, beams):
+        beams = [np.ascontiguousarray(b, dtype=np.float64).reshape(-1, S.NPROP) for b in beams]
+        self.n = [len(b) for b in beams]
+        L = lib()
+        counts = np.array(self.n, dtype=np.int64)
+        self._h = L.eh_batch_create(len(beams), counts.ctypes.data)
+        if not self._h:
+            raise _lib.EHError("eh_batch_create failed: %s" % L.eh_last_error().decode())
+        for v, b in enumerate(beams):
+            _lib.check(L.eh_batch_upload(self._h, v, b.ctypes.data, len(b)))
+
+    def track(self, lattices, n_passes, p_central, block=256, span_elements=None, waves_per_cu=8, use_graph=True):
+        """track every variant n_passes turns; lattices[v] is variant v's Lattice"""
+        if len(lattices) != len(self.n):
+            raise ValueError("one lattice per variant")
+        ne = len(lattices[0])
+        arrays, keep = [], []
+        for lat in lattices:
+            if len(lat) != ne:
+                raise ValueError("variants must have the same number of elements")
+            a, k = lat.c_beamline()
+            arrays.append(a)
+            keep.append(k)
+        ptrs = (C.c_void_p * len(arrays))(*[C.cast(a, C.c_void_p) for a in arrays])
+        tune = S.eh_batch_tune(block, int(span_elements or ne), waves_per_cu, int(bool(use_graph)))
+        pc = C.c_double(p_central)
+        st = S.eh_track_stats()
+        n = lib().eh_batch_track(self._h, ptrs, ne, n_passes, C.byref(pc), C.byref(tune), C.byref(st))
+        _lib.check(n)
+        del keep
+        return n, {"particle_element_turns": st.particle_element_turns, "kernel_ms": st.kernel_ms,
+                   "launches": st.launches, "n_alive": st.n_alive}
+
+    def n_alive(self, v):
+        return lib().eh_batch_n_alive(self._h, v)
+
+    def download(self, v):
+        out = np.empty((self.n[v], S.NPROP), dtype=np.float64)
+        _lib.check(lib().eh_batch_download(self._h, v, out.ctypes.data, self.n[v]))
+        return out
+
+    def free(self):
+        if self._h:
+            lib().eh_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -184,6 +184,7 @@ long eh_beam_download(eh_beam *beam, double **coord, long n_part);
 long eh_beam_upload_flat(eh_beam *beam, const double *flat, long n_part);
 long eh_beam_download_flat(eh_beam *beam, double *flat, long n_part);
 
+/* survivors, and all particles held (survivors + lost), i.e. the n_part of a download */
 long eh_beam_n_alive(const eh_beam *beam);
 long eh_beam_n_total(const eh_beam *beam);
 
@@ -242,6 +243,32 @@ long eh_do_tracking(eh_beam *beam, EH_ELEMENT_LIST *beamline, long n_elements, l
  * Writes 11x11 row-major Fx, Fy; returns expansionOrder+1 or a negative code. */
 long eh_csbend_field_table(const double *b, const double *c, double h, int nonlinear, int expansionOrder, double *Fx,
                            double *Fy, int *hasNormal, int *hasSkew);
+
+/* ---------------- batched lattice variants --------------------------------- */
+/* Many variants of one beamline (same element types in the same order, e.g.
+ * error seeds or strength scans), each with its own particles, tracked together:
+ * a persistent kernel pulls one-variant wavefronts from a device work list, and
+ * the spans of a run are captured in one HIP graph.  Each variant's result equals
+ * eh_do_tracking of that variant alone (no reference counterpart: the reference
+ * runs variants one after another, src/elegant.c:1177, src/vary.c:255). */
+typedef struct eh_batch eh_batch;
+
+typedef struct {
+  int32_t block;         /* 64, 128 or 256 threads per block */
+  int32_t span_elements; /* elements per fused launch (whole beamline = one turn) */
+  int32_t waves_per_cu;  /* persistent grid size = CUs x waves_per_cu wavefronts */
+  int32_t use_graph;     /* capture the spans of a run in one graph */
+} eh_batch_tune;
+
+eh_batch *eh_batch_create(long n_variants, const long *n_particles);
+void eh_batch_free(eh_batch *batch);
+/* variant v's particles: n rows of EH_NPROP doubles (n = n_particles[v]) */
+long eh_batch_upload(eh_batch *batch, long v, const double *flat, long n);
+long eh_batch_download(eh_batch *batch, long v, double *flat, long n);
+long eh_batch_n_alive(const eh_batch *batch, long v);
+/* beamlines[v] is variant v's beamline of n_elements entries */
+long eh_batch_track(eh_batch *batch, EH_ELEMENT_LIST *const *beamlines, long n_elements, long n_passes,
+                    double *P_central, const eh_batch_tune *tune, eh_track_stats *stats);
 
 /* ---------------- diagnostics ---------------------------------------------- */
 /* reference row order after the given loss events (no reference counterpart:

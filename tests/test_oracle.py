@@ -116,7 +116,8 @@ def test_multipole_vs_textbook(oracle_mod, etype, key, order, K, integ, nk):
         assert got[5] == pts[i, 5]  # no radiation: delta untouched bit for bit
 
 
-def _jacobian(f, z0, eps=1e-9):
+def _jacobian(f, z0, eps=1e-6):
+    # central differences; eps well above the bend's rounding floor rho0 * 1e-16
     J = np.zeros((4, 4))
     for k in range(4):
         zp = np.array(z0, dtype=float)
