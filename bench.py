@@ -108,6 +108,21 @@ def cpu_baseline(name, grid, sample, turns, repeats=3):
             "cpu": platform.processor() or platform.machine()}
 
 
+COUNTERS = os.path.join(ROOT, "profiles", "k_fused_mba_1e6_counters.md")
+
+
+def counter_keys():
+    """per-unit hardware FP64 operations and per-particle HBM traffic of k_fused,
+    from the committed rocprofv3 counter summary (measured on the headline shape)"""
+    try:
+        for line in open(COUNTERS):
+            if line.strip().startswith("bench-keys:"):
+                return {k: float(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].split())}
+    except OSError:
+        pass
+    return {}
+
+
 def leg_fused(args, census):
     from elegant_amd import tracking
     lat, rows, Po, desc = workload(args.config, args.grid)
@@ -172,6 +187,52 @@ def leg_plain(args):
             "workload": desc + ", per-element drop-in path"}
 
 
+def leg_adjoint(args):
+    """adjoint/forward cost: L = mean over survivors of x^2/beta_x + y^2/beta_y after
+    the turns, gradient with respect to every KQUAD K1, KSEXT K2 and KOCT K3 of the
+    MBA ring (720 strengths), checkpoints every --adj-ckpt turns"""
+    from elegant_amd import adjoint, configs, optics, tracking
+    Po = configs.p_central_from_mev(3000.0)
+    lat = configs.mba_ring(Po)
+    tw = optics.periodic_twiss(configs.mba_period())
+    bx, by = tw["betax"], tw["betay"]
+    rows = configs.mba_beam(args.adj_particles, seed=21, scale=args.adj_scale)
+    spec = adjoint.ParamSpec.strengths(lat)
+    turns = args.adj_turns
+    adj = adjoint.AdjointTracker(lat, spec, Po, n_turns=turns, checkpoint_every=args.adj_ckpt)
+    adj.fix_rf(rows)
+    # plain forward tracking of the same beam, same turns: the reference cost
+    beam = tracking.Beam(rows)
+    t0 = time.perf_counter()
+    n_f, st_f, _ = tracking.track(beam, lat, turns, Po, mode="fused", turns_per_launch=args.turns_per_launch)
+    t_fwd = time.perf_counter() - t0
+    beam.free()
+    t0 = time.perf_counter()
+    final, alive = adj.forward(rows)
+    t_rec = time.perf_counter() - t0
+    n = alive.sum()
+    G = np.zeros((len(rows), 6))
+    G[alive, 0] = 2 * final[alive, 0] / bx / n
+    G[alive, 2] = 2 * final[alive, 2] / by / n
+    t0 = time.perf_counter()
+    gp, _ = adj.backward(G)
+    t_bwd = time.perf_counter() - t0
+    st = adj.stats
+    adj.free()
+    log("adjoint: %d turns, %d particles, %d survivors; forward %.2f s, recorded forward %.2f s, backward %.2f s "
+        "(GPU: %.2f fwd, %.2f re-track, %.2f reverse); recon err %.2e" % (
+            turns, len(rows), n, t_fwd, t_rec, t_bwd, st["forward_ms"] / 1e3, st["retrack_ms"] / 1e3,
+            st["backward_ms"] / 1e3, st["max_recon_err"]))
+    return {"ratio": round((t_rec + t_bwd) / t_fwd, 3), "backward_over_forward": round(t_bwd / t_fwd, 3),
+            "forward_s": round(t_fwd, 3), "recorded_forward_s": round(t_rec, 3), "backward_s": round(t_bwd, 3),
+            "gpu_forward_s": round(st["forward_ms"] / 1e3, 3), "gpu_retrack_s": round(st["retrack_ms"] / 1e3, 3),
+            "gpu_reverse_s": round(st["backward_ms"] / 1e3, 3), "turns": turns, "particles": len(rows),
+            "survivors": int(n), "parameters": len(spec), "checkpoint_every": args.adj_ckpt,
+            "max_recon_err": st["max_recon_err"], "grad_norm": float(np.linalg.norm(gp)),
+            "workload": "MBA ring 3 GeV, 6D Gaussian beam (emittances x%g of 5 nm / 50 pm), L = mean(x^2/bx + "
+                        "y^2/by) after the turns, d/d(720 K1/K2/K3)" % args.adj_scale}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +250,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-turns", type=int, default=100)
+    ap.add_argument("--adj-particles", type=int, default=100000)
+    ap.add_argument("--adj-ckpt", type=int, default=50)
+    ap.add_argument("--adj-scale", type=float, default=400.0, help="beam emittance scale (amplitude ~ DA)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
+                    help="first run a plain copy kernel over N doubles (known HBM traffic, for counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -204,24 +272,42 @@ def main():
         log("cpu baseline: %.4g %s on %d cores (%.1f s)" % (cpu["value"], UNIT, cpu["cores"],
                                                             time.perf_counter() - t0))
 
-    from elegant_amd import census, tracking
+    from elegant_amd import _lib, census, tracking
     tracking.init(0)
+    if args.copy_calibration:
+        _lib.check(tracking.lib().eh_diag_copy(args.copy_calibration, 1))
+        log("copy kernel: %d doubles (%d bytes read, %d written)" % (
+            args.copy_calibration, 8 * args.copy_calibration, 8 * args.copy_calibration))
     desc, n0, n, wall, st, cen = leg_fused(args, census)
     units = st["particle_element_turns"]
     value = units / wall
 
     alg_per_unit = cen["algorithmic_per_unit"]
-    launches = max(1, st["launches"])
-    avg_launch_s = st["kernel_ms"] / 1e3 / launches
-    flops_per_launch = units / launches * alg_per_unit
+    launches = max(1, st["main_launches"])
+    avg_launch_s = st["main_ms"] / 1e3 / launches
+    flops_per_launch = st["main_units"] / launches * alg_per_unit
     achieved = flops_per_launch / avg_launch_s / 1e12 if avg_launch_s > 0 else 0.0
+    keys = counter_keys() if (args.config == "mba" and args.mode == "fused") else {}
+    traffic = None
+    hw = None
+    if keys:
+        traffic = round(keys["traffic_bytes_per_particle_launch"] * n0)
+        hw_t = keys["hw_fp64_ops_per_unit"] * st["main_units"] / launches / avg_launch_s / 1e12
+        hw = {"achieved": round(hw_t, 3), "frac": round(hw_t / FP64_PEAK_TFLOPS, 4),
+              "source": "hardware FP64 ops per unit from profiles/k_fused_mba_1e6_counters.md (FMA = 2) x "
+                        "units per launch / live launch time"}
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
+                "traffic_source": "FETCH_SIZE + WRITE_SIZE scaled by a copy kernel, "
+                                  "profiles/k_fused_mba_1e6_counters.md" if traffic else None,
+                "algorithmic_traffic": 112 * n0,
+                "hw_fp64": hw,
                 "pipe": "FP64 VALU, no MFMA (compute-bound; the MI355X FP64 vector peak equals its FP64 "
                         "matrix peak)",
                 "kernel": "k_fused" if args.mode == "fused" else "k_plain",
                 "algorithmic_flops_per_unit": round(alg_per_unit, 2),
-                "units_per_launch": round(units / launches),
+                "units_per_launch": round(st["main_units"] / launches),
+                "launches": st["main_launches"],
                 "avg_launch_ms": round(avg_launch_s * 1e3, 3)}
 
     out = {"metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": args.steps,
@@ -240,6 +326,9 @@ def main():
         out["plain"] = leg_plain(args)
         if "batched" in out:
             out["batched_over_plain"] = round(out["batched"]["value"] / out["plain"]["value"], 2)
+    if not args.no_adjoint:
+        out["adjoint"] = leg_adjoint(args)
+        out["adjoint_over_forward"] = out["adjoint"]["ratio"]
     print(json.dumps(out))
 
 

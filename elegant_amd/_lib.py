@@ -21,7 +21,8 @@ EXPORTS = [
     "eh_multipole_tracking2", "eh_track_through_csbend", "eh_simple_rf_cavity", "eh_exactDrift", "eh_drift_beam",
     "eh_limit_amplitudes", "eh_removeInvalidParticles", "eh_do_tracking", "eh_csbend_field_table",
     "eh_replay_order", "eh_selftest_math", "eh_batch_create", "eh_batch_free", "eh_batch_upload",
-    "eh_batch_download", "eh_batch_n_alive", "eh_batch_track",
+    "eh_batch_download", "eh_batch_n_alive", "eh_batch_track", "eh_diag_copy", "eh_adjoint_create",
+    "eh_adjoint_free", "eh_adjoint_forward", "eh_adjoint_backward",
 ]
 
 
@@ -83,6 +84,8 @@ def load():
     lib.eh_replay_order.restype = L
     lib.eh_selftest_math.argtypes = [vp, vp, L, vp]
     lib.eh_selftest_math.restype = L
+    lib.eh_diag_copy.argtypes = [L, C.c_int]
+    lib.eh_diag_copy.restype = L
     lib.eh_batch_create.argtypes = [L, vp]
     lib.eh_batch_create.restype = vp
     lib.eh_batch_free.argtypes = [vp]

@@ -20,6 +20,9 @@
 
 #include "eh_math.h"
 
+/* the per-particle maps compile for the GPU and, for test harnesses, the host */
+#define EH_HD __host__ __device__ __forceinline__
+
 #define EH_MAXSUB 8     /* integrator sub-steps (order 6) */
 #define EH_MAXFTERMS 66 /* field terms per component (expansion order 10) */
 
@@ -85,6 +88,13 @@ typedef struct {
   double rf_len;  /* length / nKicks */
   double rf_flen; /* full element length (volt == 0 drift) */
   double rf_volt, rf_omega, rf_phase, rf_dtLight, rf_t0, rf_dx, rf_dy, rf_dgamma_same;
+
+  /* ---- adjoint (reverse-mode) extras ---- */
+  int32_t adj_order0;              /* order of the main multipole term, also when its strength is 0 */
+  int32_t adj_pk, adj_pdx, adj_pdy; /* gradient slots of strength, DX, DY (-1: not a parameter) */
+  int32_t adj_anchor;              /* slot of the stored entry state (RF cavity), -1 if none */
+  double adj_kfac[EH_MAXSUB];      /* d kick[step][0] / d strength */
+  double adj_sin_dshr[EH_MAXSUB];  /* sin of the CSBEND sub-step angles */
 } eh_desc;
 
 typedef struct {
@@ -93,8 +103,8 @@ typedef struct {
 
 EH_FN double eh_sqr(double x) { return x * x; }
 
-__device__ __forceinline__ bool eh_isnan(double x) { return x != x; }
-__device__ __forceinline__ bool eh_isinf(double x) { return eh_fabs(x) == __builtin_inf(); }
+EH_HD bool eh_isnan(double x) { return x != x; }
+EH_HD bool eh_isinf(double x) { return eh_fabs(x) == __builtin_inf(); }
 
 /* ------------------------------------------------------------------------- */
 /* DRIF through drift_matrix + generic order-2 loop (src/compute_matrices.c:391-410,
@@ -102,7 +112,7 @@ __device__ __forceinline__ bool eh_isinf(double x) { return eh_fabs(x) == __buil
  * the expressions below (terms with zero matrix entries add signed zeros, which
  * only normalise -0 to +0 where written as 0.0 + ...); any non-finite input makes
  * every output NaN through 0*inf or 0*NaN. */
-__device__ __forceinline__ void eh_drif(eh_p6 &p, double L, double T) {
+EH_HD void eh_drif(eh_p6 &p, double L, double T) {
   bool finite = !(eh_isnan(p.x) || eh_isnan(p.xp) || eh_isnan(p.y) || eh_isnan(p.yp) || eh_isnan(p.s) ||
                   eh_isnan(p.dp) || eh_isinf(p.x) || eh_isinf(p.xp) || eh_isinf(p.y) || eh_isinf(p.yp) ||
                   eh_isinf(p.s) || eh_isinf(p.dp));
@@ -126,7 +136,7 @@ __device__ __forceinline__ void eh_drif(eh_p6 &p, double L, double T) {
 }
 
 /* exactDrift, src/csbend.c:3811-3815 */
-__device__ __forceinline__ void eh_edrift(eh_p6 &p, double L) {
+EH_HD void eh_edrift(eh_p6 &p, double L) {
   p.x += p.xp * L;
   p.y += p.yp * L;
   p.s += L * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
@@ -135,7 +145,7 @@ __device__ __forceinline__ void eh_edrift(eh_p6 &p, double L) {
 /* ------------------------------------------------------------------------- */
 /* limit_amplitudes for one particle, src/limit_amplitudes.c:226-284 (openCode 0).
  * Returns true if the particle is lost and applies the loss assignments. */
-__device__ __forceinline__ bool eh_limit_one(eh_p6 &p, double xmax, double ymax, double z, double Po, int extrap) {
+EH_HD bool eh_limit_one(eh_p6 &p, double xmax, double ymax, double z, double Po, int extrap) {
   int is_out = 0;
   if (xmax > 0 && eh_fabs(p.x) > xmax)
     is_out += 1;
@@ -168,7 +178,7 @@ __device__ __forceinline__ bool eh_limit_one(eh_p6 &p, double xmax, double ymax,
 
 /* ------------------------------------------------------------------------- */
 /* rectangular in-element aperture, checkMultAperture (src/multipole.c:1751-1798) */
-__device__ __forceinline__ bool eh_ap_ok(double xIn, double yIn, double xmax, double ymax, int rtilt, double cs0,
+EH_HD bool eh_ap_ok(double xIn, double yIn, double xmax, double ymax, int rtilt, double cs0,
                                          double cs1) {
   double x = xIn - 0.0, y = yIn - 0.0;
   if (rtilt) {
@@ -184,7 +194,7 @@ __device__ __forceinline__ bool eh_ap_ok(double xIn, double yIn, double xmax, do
 /* apply_canonical_multipole_kicks_ret/_noret, src/multipole.h:391-445, with the
  * (x+iy)^n/n! coefficient table of expansion_coefficients (src/multipole.h:76-145) */
 template <int ORDER>
-__device__ __forceinline__ void eh_mkick(double &qx, double &qy, const double *xpow, const double *ypow, double K,
+EH_HD void eh_mkick(double &qx, double &qy, const double *xpow, const double *ypow, double K,
                                          int skew) {
   double sum_Fx = 0, sum_Fy = 0;
 #pragma unroll
@@ -216,7 +226,7 @@ __device__ __forceinline__ void eh_mkick(double &qx, double &qy, const double *x
   qy += K * sum_Fx;
 }
 
-__device__ __forceinline__ void eh_mkick_n(int order, double &qx, double &qy, const double *xpow, const double *ypow,
+EH_HD void eh_mkick_n(int order, double &qx, double &qy, const double *xpow, const double *ypow,
                                            double K, int skew) {
   switch (order) {
   case 0:
@@ -235,7 +245,7 @@ __device__ __forceinline__ void eh_mkick_n(int order, double &qx, double &qy, co
 }
 
 /* convertMomentaToSlopes, src/multipole.h:466-486 */
-__device__ __forceinline__ bool eh_m2s(double &xp, double &yp, double qx, double qy, double dp, int expand) {
+EH_HD bool eh_m2s(double &xp, double &yp, double qx, double qy, double dp, int expand) {
   if (expand) {
     xp = qx / (1 + dp);
     yp = qy / (1 + dp);
@@ -255,7 +265,7 @@ __device__ __forceinline__ bool eh_m2s(double &xp, double &yp, double qx, double
  * src/malign_mat.c:257-267, :580-590, tilt_matrices.c:156-187) and the loss
  * assignments of :1032-1047.  integrate_kick_multipole_ordn, src/multipole.c:1099-1456.
  * Returns true if lost. */
-__device__ __forceinline__ bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
+EH_HD bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
   /* face 1 */
   if (d.has_offset) {
     p.s += d.mdz * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
@@ -382,7 +392,7 @@ __device__ __forceinline__ bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi)
 /* ------------------------------------------------------------------------- */
 /* computeCSBENDFields, src/csbend.c:99-172, over the non-zero terms in the
  * reference's summation order (i outer, j inner) */
-__device__ __forceinline__ void eh_csb_fields(const eh_desc &d, const eh_fterm *__restrict__ terms, double x,
+EH_HD void eh_csb_fields(const eh_desc &d, const eh_fterm *__restrict__ terms, double x,
                                               double y, double &Fx, double &Fy) {
   if (!d.csb_field) {
     Fx = 0;
@@ -424,7 +434,7 @@ __device__ __forceinline__ void eh_csb_fields(const eh_desc &d, const eh_fterm *
 /* CSBEND: track_through_csbend (src/csbend.c:1088-1403) for one particle, with
  * integrate_csbend_ordn (src/csbend.c:1459-1720) and the canonical conversions
  * (:1436-1454).  Returns true if lost. */
-__device__ __forceinline__ bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms) {
+EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms) {
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
   const double rho0 = d.rho0, rho_actual = d.rho_actual;
   double x, xp, y, yp, s, dp;
@@ -547,7 +557,7 @@ __device__ __forceinline__ bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, co
  * matrix method :824-967, kick method :567-660, add_to_particle_energy
  * :987-1006) followed by removeInvalidParticles (src/limit_amplitudes.c:289-330).
  * Returns true if lost. */
-__device__ __forceinline__ bool eh_rfca(eh_p6 &p, const eh_desc &d) {
+EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d) {
   const double c_mks = 2.99792458e8;
   const double Pc = d.Po;
   if (d.rf_zero_volt) {
@@ -680,7 +690,7 @@ __device__ __forceinline__ bool eh_rfca(eh_p6 &p, const eh_desc &d) {
 /* one element followed by do_tracking's post-element sweeps
  * (src/do_tracking.c:2506-2524, :2675-2684).  Returns -1 if the particle
  * survives, else the loss stage. */
-__device__ __forceinline__ int eh_apply_element(eh_p6 &p, const eh_desc &d, int pi,
+EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pi,
                                                 const eh_fterm *__restrict__ terms) {
   bool lost = false;
   switch (d.type) {

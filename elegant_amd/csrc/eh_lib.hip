@@ -262,7 +262,13 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
     d->mp_order[i] = KnL[i] ? (int32_t)order[i] : 0;
     d->mp_skew[i] = skew[i];
   }
+  d->adj_order0 = (int32_t)order[0];
+  const double kfac = drift * (1 + (e->type == EH_T_KQUAD   ? ((const EH_KQUAD *)e->p_elem)->fse
+                                    : e->type == EH_T_KSEXT ? ((const EH_KSEXT *)e->p_elem)->fse
+                                                            : ((const EH_KOCT *)e->p_elem)->fse)) /
+                      nSlices;
   for (int s = 0; s < nsub; s++) {
+    d->adj_kfac[s] = kfac * kickFrac[s];
     d->dsh[s] = mdrift * driftFrac[s];
     for (int i = 0; i < 3; i++)
       d->kick[s][i] = KnLp[i] * kickFrac[s];
@@ -435,6 +441,7 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
   for (int j = 0; j < nsub; j++) {
     d->csb_dsh[j] = s * driftFrac[j];
     d->csb_dshr[j] = d->csb_dsh[j] / rho0;
+    d->adj_sin_dshr[j] = eh_sin(d->csb_dshr[j]);
     d->csb_ds[j] = s * kickFrac[j];
   }
   d->csb_dzl = cb->nSlices * s;
@@ -500,6 +507,7 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
   std::memset(d, 0, sizeof *d);
   d->type = e->type;
   d->counted = !(e->type == EH_T_MARK || e->type == EH_T_MAXAMP);
+  d->adj_pk = d->adj_pdx = d->adj_pdy = d->adj_anchor = -1;
   d->Po = c.Po;
   d->z_start = c.z_start;
   d->z_end = c.z_end;
@@ -567,6 +575,9 @@ struct eh_beam {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double kernel_ms = 0;
   long launches = 0;
+  double last_ms = 0; /* the most recent timed launch */
+  double main_ms = 0, main_units = 0;
+  long main_launches = 0;
 };
 
 static int g_device = -1;
@@ -1038,6 +1049,7 @@ static long timed_end(eh_beam *b) {
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->kernel_ms += ms;
+  b->last_ms = ms;
   b->launches++;
   return EH_OK;
 }
@@ -1092,6 +1104,11 @@ static long run_plain(eh_beam *b, const eh_desc *hd, const std::vector<eh_fterm>
   r = timed_end(b);
   if (r < 0)
     return r;
+  if (op == 0 && hd && hd->counted) {
+    b->main_ms += b->last_ms;
+    b->main_launches++;
+    b->main_units += (double)n;
+  }
   unsigned long long lost = 0;
   HIPCHK(hipMemcpy(&lost, b->d_cnt + 2, sizeof lost, hipMemcpyDeviceToHost));
   long before_lost = b->n_lost;
@@ -1532,6 +1549,9 @@ static long launch_fused(eh_beam *b, long ne, int p0, int e0, int p1, int e1, do
   b->n_alive = (long)out[0];
   b->n_lost = (long)out[1];
   *effort += (double)out[3];
+  b->main_ms += b->last_ms;
+  b->main_launches++;
+  b->main_units += (double)out[3];
   return b->n_alive;
 }
 
@@ -1652,8 +1672,8 @@ extern "C" long eh_do_tracking(eh_beam *b, EH_ELEMENT_LIST *beamline, long n_ele
   if (!b || !beamline || !P_central || n_elements < 0 || n_passes < 0)
     return fail(EH_ERR_ARG, "eh_do_tracking: bad arguments");
   double effort = 0;
-  double ms0 = b->kernel_ms;
-  long l0 = b->launches;
+  double ms0 = b->kernel_ms, mms0 = b->main_ms, mu0 = b->main_units;
+  long l0 = b->launches, ml0 = b->main_launches;
   long r;
   if (mode == EH_MODE_PLAIN)
     r = track_plain(b, beamline, n_elements, n_passes, P_central, &effort);
@@ -1666,6 +1686,9 @@ extern "C" long eh_do_tracking(eh_beam *b, EH_ELEMENT_LIST *beamline, long n_ele
     stats->kernel_ms = b->kernel_ms - ms0;
     stats->launches = b->launches - l0;
     stats->n_alive = b->n_alive;
+    stats->main_ms = b->main_ms - mms0;
+    stats->main_launches = b->main_launches - ml0;
+    stats->main_units = b->main_units - mu0;
   }
   return r;
 }
@@ -1748,6 +1771,34 @@ extern "C" long eh_selftest_math(const double *x, const double *y, long n, doubl
   return n;
 }
 
+__global__ void __launch_bounds__(EH_BLOCK) k_copy(const double *__restrict__ a, double *__restrict__ b, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    b[i] = a[i];
+}
+
+/* copies n doubles device-to-device with a plain kernel, reps times: a launch of
+ * known HBM traffic (8n bytes read, 8n written) for calibrating memory counters */
+extern "C" long eh_diag_copy(long n, int reps) {
+  if (g_device < 0 && eh_init(0) != EH_OK)
+    return EH_ERR_HIP;
+  if (n <= 0 || reps < 1)
+    return fail(EH_ERR_ARG, "eh_diag_copy: n and reps must be positive");
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, sizeof(double) * 2 * n));
+  hipError_t e = hipMemset(d, 0, sizeof(double) * 2 * n);
+  for (int k = 0; k < reps && e == hipSuccess; k++) {
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(EH_BLOCK), 0, 0, d, d + n, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipDeviceSynchronize();
+  (void)hipFree(d);
+  if (e != hipSuccess)
+    return fail(EH_ERR_HIP, "eh_diag_copy: %s", hipGetErrorString(e));
+  return n;
+}
+
 /* ------------------------------------------------------------------------- */
 /* batched lattice variants: host side */
 #include <string>
@@ -1773,7 +1824,53 @@ struct eh_batch {
   double kernel_ms = 0;
   long launches = 0;
   int n_cu = 256;
+  /* descriptor tables of the last call, reused while the beamlines are unchanged */
+  uint64_t tables_hash = 0;
+  long tables_ne = 0;
 };
+
+static size_t elem_struct_size(int32_t type) {
+  switch (type) {
+  case EH_T_DRIF:
+    return sizeof(EH_DRIFT);
+  case EH_T_EDRIFT:
+    return sizeof(EH_EDRIFT);
+  case EH_T_KQUAD:
+    return sizeof(EH_KQUAD);
+  case EH_T_KSEXT:
+    return sizeof(EH_KSEXT);
+  case EH_T_KOCT:
+    return sizeof(EH_KOCT);
+  case EH_T_CSBEND:
+    return sizeof(EH_CSBEND);
+  case EH_T_RFCA:
+    return sizeof(EH_RFCA);
+  case EH_T_MAXAMP:
+    return sizeof(EH_MAXAMP);
+  default:
+    return 0;
+  }
+}
+
+/* FNV-1a over every element struct of every variant, and P_central */
+static uint64_t beamlines_hash(EH_ELEMENT_LIST *const *bl, long nvar, long ne, double Pc) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void *p, size_t n) {
+    const unsigned char *c = (const unsigned char *)p;
+    for (size_t i = 0; i < n; i++) {
+      h ^= c[i];
+      h *= 1099511628211ull;
+    }
+  };
+  mix(&Pc, sizeof Pc);
+  for (long v = 0; v < nvar; v++)
+    for (long e = 0; e < ne; e++) {
+      mix(&bl[v][e].type, sizeof(int32_t));
+      if (bl[v][e].p_elem)
+        mix(bl[v][e].p_elem, elem_struct_size(bl[v][e].type));
+    }
+  return h;
+}
 
 extern "C" eh_batch *eh_batch_create(long nvar, const long *n_particles) {
   if (nvar < 1 || !n_particles) {
@@ -2064,7 +2161,9 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
   for (long e = 0; e < ne; e++)
     if (beamlines[0][e].type == EH_T_RFCA)
       rf_elems.push_back(e);
-  for (long v = 0; v < b->nvar; v++) {
+  const uint64_t hash = beamlines_hash(beamlines, b->nvar, ne, Pc);
+  const bool reuse = b->d_pool && b->tables_ne == ne && b->tables_hash == hash;
+  for (long v = 0; v < b->nvar && !reuse; v++) {
     const EH_ELEMENT_LIST *bl = beamlines[v];
     std::vector<double> zs(ne), ze(ne);
     const EH_MAXAMP *last_amp = nullptr;
@@ -2123,6 +2222,7 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
     }
   }
   auto upload_tables = [&]() -> long {
+    b->tables_hash = 0; /* valid again only once this call completes */
     if (b->d_pool)
       HIPCHK(hipFree(b->d_pool));
     if (b->d_terms)
@@ -2141,7 +2241,7 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
     HIPCHK(hipMemcpy(b->d_desc_idx, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
     return EH_OK;
   };
-  long r = upload_tables();
+  long r = reuse ? EH_OK : upload_tables();
   if (r < 0)
     return r;
   std::vector<int32_t> alive32(b->nvar);
@@ -2157,7 +2257,15 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
    * uploaded from finite generated beams; a NaN row is still removed by the first
    * element's NaN sweep.  Pass 0 stops before each RFCA whose fiducial is unknown. */
   int p = 0, e = 0;
-  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  double span_ms = 0; /* stream time of the tracking launches, host work excluded */
+  auto elapsed = [&]() -> long {
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    span_ms += ms;
+    return EH_OK;
+  };
   if (n_passes > 0) {
     for (long er : rf_elems) {
       bool unseen = false;
@@ -2167,10 +2275,14 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
       if (!unseen)
         continue;
       if (e < er) {
+        HIPCHK(hipEventRecord(b->ev0, b->stream));
         r = batch_enqueue_span(b, ne, p, e, 0, (int)er, first, t.block, t.waves_per_cu);
         if (r < 0)
           return r;
         first = 0;
+        r = elapsed();
+        if (r < 0)
+          return r;
         r = batch_sync_counts(b);
         if (r < 0)
           return r;
@@ -2245,6 +2357,8 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
   bool capture = t.use_graph && n_passes > 0;
   if (capture)
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  else
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
   while (p < n_passes) {
     long pos = (long)p * ne + e + t.span_elements;
     long endpos = (long)n_passes * ne;
@@ -2267,13 +2381,13 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
   if (capture) {
     HIPCHK(hipStreamEndCapture(b->stream, &graph));
     HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
     HIPCHK(hipGraphLaunch(exec, b->stream));
   }
-  HIPCHK(hipEventRecord(b->ev1, b->stream));
-  HIPCHK(hipEventSynchronize(b->ev1));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-  b->kernel_ms += ms;
+  r = elapsed();
+  if (r < 0)
+    return r;
+  b->kernel_ms += span_ms;
   if (exec)
     HIPCHK(hipGraphExecDestroy(exec));
   if (graph)
@@ -2290,11 +2404,634 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
   long alive = 0;
   for (long v = 0; v < b->nvar; v++)
     alive += b->alive_v[v];
+  /* the tables now on the device describe the beamlines as they stand (RF fiducials set) */
+  b->tables_hash = beamlines_hash(beamlines, b->nvar, ne, Pc);
+  b->tables_ne = ne;
   if (stats) {
     stats->particle_element_turns = (double)cnt[3];
     stats->kernel_ms = b->kernel_ms - ms0;
     stats->launches = b->launches - l0;
     stats->n_alive = alive;
+    stats->main_ms = stats->kernel_ms; /* spans run back to back in one graph */
+    stats->main_launches = stats->launches;
+    stats->main_units = stats->particle_element_turns;
   }
   return alive;
+}
+
+/* ------------------------------------------------------------------------- */
+/* reverse-mode adjoint of multi-turn tracking                                 */
+/*                                                                           */
+/* Forward: particles stay in fixed rows (no compaction); every C turns the   */
+/* turn-start state is kept (checkpoint).  Backward, segment by segment from  */
+/* the last: the survivors are re-tracked from the segment's checkpoint while */
+/* the entry states of anchor elements (turn start, RF cavities) are stored,  */
+/* then one kernel walks the segment's turns and elements in reverse,         */
+/* recovering each element's entry state from its exit state with the        */
+/* inverse maps (re-anchored at every anchor) and pulling the adjoint back.   */
+/* With C = 1 the forward pass stores the anchors itself and nothing is       */
+/* re-tracked.  Parameter gradients are summed per wavefront in LDS, then per */
+/* block, then over blocks in a fixed order: the result is deterministic.     */
+#include "eh_adjoint.h"
+
+#define EH_ADJ_BLOCK 64
+
+/* forward over turns [t0, t1) for particles i < n in fixed rows; anchors (if
+ * given) receive the entry state of element 0 (slot 0) and of each anchor
+ * element (slot d.adj_anchor) for every turn, at index
+ * ((t - t0) * n_anchor + slot) * 6 * a_ld + c * a_ld + i */
+__global__ void __launch_bounds__(EH_BLOCK) k_adj_fwd(double *z, long ld, int32_t *alive, long n,
+                                                       const eh_desc *__restrict__ descs,
+                                                       const eh_fterm *__restrict__ terms, int ne, int t0, int t1,
+                                                       double *anchors, int n_anchor, long a_ld,
+                                                       unsigned long long *units) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool in = i < n;
+  bool live = in && alive[i];
+  eh_p6 p = {0, 0, 0, 0, 0, 0};
+  if (live)
+    p = {z[i], z[ld + i], z[2 * ld + i], z[3 * ld + i], z[4 * ld + i], z[5 * ld + i]};
+  unsigned long long u = 0;
+  for (int t = t0; t < t1; t++) {
+    if (__ballot(live) == 0)
+      break;
+    const int pi = t > 0 ? 1 : 0;
+    for (int e = 0; e < ne; e++) {
+      const eh_desc *d = descs + e;
+      if (live) {
+        int slot = e == 0 ? 0 : d->adj_anchor;
+        if (anchors && slot >= 0) {
+          double *a = anchors + ((long)(t - t0) * n_anchor + slot) * 6 * a_ld + i;
+          a[0] = p.x;
+          a[a_ld] = p.xp;
+          a[2 * a_ld] = p.y;
+          a[3 * a_ld] = p.yp;
+          a[4 * a_ld] = p.s;
+          a[5 * a_ld] = p.dp;
+        }
+        u += d->counted;
+        if (eh_apply_element(p, *d, pi, terms) >= 0)
+          live = false;
+      }
+    }
+  }
+  if (in) {
+    if (alive[i] && !live)
+      alive[i] = 0;
+    z[i] = p.x;
+    z[ld + i] = p.xp;
+    z[2 * ld + i] = p.y;
+    z[3 * ld + i] = p.yp;
+    z[4 * ld + i] = p.s;
+    z[5 * ld + i] = p.dp;
+  }
+  wave_add(units, u);
+}
+
+/* out[c*m + i] = in[c*ld + rows[i]] */
+__global__ void k_adj_gather(const double *in, long ld, const int32_t *rows, long m, double *out) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m)
+    for (int c = 0; c < 6; c++)
+      out[c * m + i] = in[c * ld + rows[i]];
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1)
+    v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ eh_p6 load_anchor(const double *anchors, long idx, long a_ld, long i) {
+  const double *a = anchors + idx * 6 * a_ld + i;
+  return {a[0], a[a_ld], a[2 * a_ld], a[3 * a_ld], a[4 * a_ld], a[5 * a_ld]};
+}
+
+/* reverse sweep over turns [t0, t1) for m survivors, one wavefront per block.
+ * zb/lam hold the state and adjoint at the end of turn t1-1 on entry and at
+ * the start of turn t0 on exit.  Survivor i's anchors sit at column arow[i]
+ * of turn (t - ta0). */
+__global__ void __launch_bounds__(EH_ADJ_BLOCK) k_adj_bwd(double *zb, double *lam, long m, const int32_t *arow,
+                                                           const double *__restrict__ anchors, int n_anchor,
+                                                           long a_ld, const eh_desc *__restrict__ descs,
+                                                           const eh_fterm *__restrict__ terms, int ne, int t0,
+                                                           int t1, int ta0, int n_params, double *gpart,
+                                                           double *errpart) {
+  extern __shared__ double g_lds[];
+  const int lane = threadIdx.x;
+  for (int q = lane; q < n_params; q += EH_ADJ_BLOCK)
+    g_lds[q] = 0;
+  __syncthreads();
+  long i = (long)blockIdx.x * EH_ADJ_BLOCK + lane;
+  const bool act = i < m;
+  const long ii = act ? i : 0; /* idle lanes follow survivor 0 with a zero adjoint */
+  eh_p6 z = {zb[ii], zb[m + ii], zb[2 * m + ii], zb[3 * m + ii], zb[4 * m + ii], zb[5 * m + ii]};
+  eh_p6 l = {0, 0, 0, 0, 0, 0};
+  if (act)
+    l = {lam[i], lam[m + i], lam[2 * m + i], lam[3 * m + i], lam[4 * m + i], lam[5 * m + i]};
+  const long col = arow[ii];
+  double err = 0, err_s = 0;
+  for (int t = t1 - 1; t >= t0; t--) {
+    const long tb = (long)(t - ta0) * n_anchor;
+    for (int e = ne - 1; e >= 0; e--) {
+      const eh_desc &d = descs[e];
+      switch (d.type) {
+      case EH_T_DRIF:
+        eh_adj_drif(z, l, d.length, d.T11);
+        break;
+      case EH_T_EDRIFT:
+        eh_adj_edrift(z, l, d.length);
+        break;
+      case EH_T_KQUAD:
+      case EH_T_KSEXT:
+      case EH_T_KOCT: {
+        eh_pgrad g = {0, 0, 0};
+        eh_adj_multipole(z, l, d, g);
+        if (d.adj_pk >= 0) {
+          double v = wave_sum(g.k);
+          if (lane == 0)
+            g_lds[d.adj_pk] += v;
+        }
+        if (d.adj_pdx >= 0) {
+          double v = wave_sum(g.dx);
+          if (lane == 0)
+            g_lds[d.adj_pdx] += v;
+        }
+        if (d.adj_pdy >= 0) {
+          double v = wave_sum(g.dy);
+          if (lane == 0)
+            g_lds[d.adj_pdy] += v;
+        }
+        break;
+      }
+      case EH_T_CSBEND:
+        eh_adj_csbend(z, l, d, terms);
+        break;
+      case EH_T_RFCA: {
+        eh_p6 zin = load_anchor(anchors, tb + (e == 0 ? 0 : d.adj_anchor), a_ld, col);
+        eh_adj_rfca(z, l, zin, d);
+        break;
+      }
+      default:
+        break;
+      }
+      if (e == 0) {
+        eh_p6 za = load_anchor(anchors, tb, a_ld, col);
+        err = fmax(err, fmax(fmax(eh_fabs(z.x - za.x), eh_fabs(z.xp - za.xp)),
+                             fmax(fmax(eh_fabs(z.y - za.y), eh_fabs(z.yp - za.yp)), eh_fabs(z.dp - za.dp))));
+        err_s = fmax(err_s, eh_fabs(z.s - za.s));
+        z = za;
+      }
+    }
+  }
+  if (act) {
+    zb[i] = z.x;
+    zb[m + i] = z.xp;
+    zb[2 * m + i] = z.y;
+    zb[3 * m + i] = z.yp;
+    zb[4 * m + i] = z.s;
+    zb[5 * m + i] = z.dp;
+    lam[i] = l.x;
+    lam[m + i] = l.xp;
+    lam[2 * m + i] = l.y;
+    lam[3 * m + i] = l.yp;
+    lam[4 * m + i] = l.s;
+    lam[5 * m + i] = l.dp;
+  }
+  err = wave_max(err);
+  err_s = wave_max(err_s);
+  __syncthreads();
+  double *gp = gpart + (long)blockIdx.x * n_params;
+  for (int q = lane; q < n_params; q += EH_ADJ_BLOCK)
+    gp[q] += g_lds[q];
+  if (lane == 0) {
+    errpart[2 * blockIdx.x] = fmax(errpart[2 * blockIdx.x], err);
+    errpart[2 * blockIdx.x + 1] = fmax(errpart[2 * blockIdx.x + 1], err_s);
+  }
+}
+
+/* out[p] = sum over blocks b (in order) of part[b * np + p] */
+__global__ void k_adj_reduce(const double *part, long nb, int np, double *out) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < np) {
+    double acc = 0;
+    for (long b = 0; b < nb; b++)
+      acc += part[b * np + q];
+    out[q] = acc;
+  }
+}
+
+__global__ void k_fill(double *p, long n, double v) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = v;
+}
+
+struct eh_adjoint {
+  long n = 0, ne = 0, n_params = 0, n_turns = 0, C = 1;
+  int n_anchor = 1;
+  bool store_all = false;
+  std::vector<eh_desc> hd;
+  std::vector<eh_fterm> pool;
+  double *d_z = nullptr;       /* 6 x n, fixed rows */
+  int32_t *d_alive = nullptr;  /* n */
+  double *d_ckpt = nullptr;    /* (n_ckpt) x 6 x n turn-start states */
+  double *d_anchor = nullptr;  /* anchors: all turns x n (store_all) or one segment x survivors */
+  size_t anchor_bytes = 0;
+  eh_desc *d_desc = nullptr;
+  eh_fterm *d_terms = nullptr;
+  unsigned long long *d_cnt = nullptr;
+  std::vector<double> z_final; /* 6 x n */
+  std::vector<int32_t> alive;
+  bool have_forward = false;
+  double fwd_ms = 0;
+  double units = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static void adj_release(eh_adjoint *a) {
+  void *ptrs[] = {a->d_z, a->d_alive, a->d_ckpt, a->d_anchor, a->d_desc, a->d_terms, a->d_cnt};
+  for (void *p : ptrs)
+    if (p)
+      (void)hipFree(p);
+  a->d_z = nullptr;
+  a->d_alive = nullptr;
+  a->d_ckpt = nullptr;
+  a->d_anchor = nullptr;
+  a->d_desc = nullptr;
+  a->d_terms = nullptr;
+  a->d_cnt = nullptr;
+  a->anchor_bytes = 0;
+  a->have_forward = false;
+}
+
+extern "C" eh_adjoint *eh_adjoint_create(void) {
+  if (g_device < 0 && eh_init(0) != EH_OK)
+    return nullptr;
+  eh_adjoint *a = new eh_adjoint();
+  if (hipStreamCreate(&a->stream) != hipSuccess || hipEventCreate(&a->ev0) != hipSuccess ||
+      hipEventCreate(&a->ev1) != hipSuccess) {
+    fail(EH_ERR_HIP, "eh_adjoint_create: stream setup failed");
+    delete a;
+    return nullptr;
+  }
+  return a;
+}
+
+extern "C" void eh_adjoint_free(eh_adjoint *a) {
+  if (!a)
+    return;
+  adj_release(a);
+  if (a->ev0)
+    (void)hipEventDestroy(a->ev0);
+  if (a->ev1)
+    (void)hipEventDestroy(a->ev1);
+  if (a->stream)
+    (void)hipStreamDestroy(a->stream);
+  delete a;
+}
+
+static long adj_timed(eh_adjoint *a, double *acc) {
+  HIPCHK(hipEventRecord(a->ev1, a->stream));
+  HIPCHK(hipEventSynchronize(a->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, a->ev0, a->ev1));
+  *acc += ms;
+  return EH_OK;
+}
+
+/* descriptors of a beamline as one do_tracking call sees them */
+static long build_line(EH_ELEMENT_LIST *bl, long ne, double Pc, std::vector<eh_desc> &hd,
+                       std::vector<eh_fterm> &pool) {
+  hd.assign(ne, eh_desc());
+  pool.clear();
+  std::vector<double> zs(ne), ze(ne);
+  const EH_MAXAMP *last_amp = nullptr;
+  double z = 0;
+  for (long ie = 0; ie < ne; ie++) {
+    zs[ie] = z;
+    z += elem_length(bl + ie);
+    ze[ie] = z;
+    if (bl[ie].type == EH_T_MAXAMP)
+      last_amp = (const EH_MAXAMP *)bl[ie].p_elem;
+  }
+  const EH_MAXAMP *cur0 = nullptr, *cur1 = last_amp;
+  for (long ie = 0; ie < ne; ie++) {
+    if (bl[ie].type == EH_T_MAXAMP)
+      cur0 = cur1 = (const EH_MAXAMP *)bl[ie].p_elem;
+    eh_ctx c = {};
+    c.Po = Pc;
+    c.z_start = zs[ie];
+    c.z_end = ze[ie];
+    c.ap[0] = cur0;
+    c.ap[1] = cur1;
+    for (int pi = 0; pi < 2; pi++) {
+      const EH_MAXAMP *m = pi ? cur1 : cur0;
+      c.post_x[pi] = m ? m->x_max : 0;
+      c.post_y[pi] = m ? m->y_max : 0;
+      c.post_on[pi] = (bl[ie].type != EH_T_MARK) && m && (m->x_max || m->y_max);
+      if (m && m->elliptical)
+        return fail(EH_ERR_UNSUPPORTED, "elliptical MAXAMP not supported");
+    }
+    long r = build_desc(&hd[ie], bl + ie, c, pool);
+    if (r < 0)
+      return r;
+  }
+  return EH_OK;
+}
+
+extern "C" long eh_adjoint_forward(eh_adjoint *a, const double *coord, long n, EH_ELEMENT_LIST *beamline, long ne,
+                                   const int32_t *param_slots, long n_params, long n_turns, double P_central,
+                                   long checkpoint_every, double *final_coord, int32_t *alive_out) {
+  if (!a || !coord || n < 1 || !beamline || ne < 1 || n_turns < 1 || n_params < 0 || checkpoint_every < 1)
+    return fail(EH_ERR_ARG, "eh_adjoint_forward: bad arguments");
+  if (n > 2147483647L)
+    return fail(EH_ERR_ARG, "eh_adjoint_forward: too many particles");
+  if (n_params > 8000)
+    return fail(EH_ERR_ARG, "eh_adjoint_forward: at most 8000 parameters (one LDS slot each)");
+  adj_release(a);
+  long r = build_line(beamline, ne, P_central, a->hd, a->pool);
+  if (r < 0)
+    return r;
+  int n_anchor = 1;
+  for (long e = 0; e < ne; e++) {
+    eh_desc &d = a->hd[e];
+    const int t = beamline[e].type;
+    if (t == EH_T_RFCA) {
+      const EH_RFCA *rf = (const EH_RFCA *)beamline[e].p_elem;
+      if (!rf->fiducial_seen && rf->volt != 0)
+        return fail(EH_ERR_ARG, "element %ld: RFCA fiducial unknown; track one pass first to fix it", e);
+      if (!d.rf_zero_volt && !d.rf_matrix)
+        return fail(EH_ERR_UNSUPPORTED, "element %ld: adjoint supports the RFCA matrix method only", e);
+      if (e > 0)
+        d.adj_anchor = n_anchor++;
+    }
+    if ((t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT) && d.expand)
+      return fail(EH_ERR_UNSUPPORTED, "element %ld: adjoint needs EXPAND_HAMILTONIAN=0", e);
+    for (int k = 0; k < 3; k++) {
+      int32_t slot = param_slots ? param_slots[3 * e + k] : -1;
+      if (slot < -1 || slot >= n_params)
+        return fail(EH_ERR_ARG, "element %ld: parameter slot %d out of range", e, slot);
+      if (slot >= 0 && !(t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT))
+        return fail(EH_ERR_UNSUPPORTED, "element %ld: parameters only for KQUAD/KSEXT/KOCT", e);
+      (k == 0 ? d.adj_pk : k == 1 ? d.adj_pdx : d.adj_pdy) = slot;
+    }
+  }
+  a->n = n;
+  a->ne = ne;
+  a->n_params = n_params;
+  a->n_turns = n_turns;
+  a->C = checkpoint_every;
+  a->n_anchor = n_anchor;
+  a->store_all = checkpoint_every == 1;
+  const long n_ckpt = (n_turns + a->C - 1) / a->C;
+  HIPCHK(hipMalloc(&a->d_z, sizeof(double) * 6 * n));
+  HIPCHK(hipMalloc(&a->d_alive, sizeof(int32_t) * n));
+  HIPCHK(hipMalloc(&a->d_cnt, sizeof(unsigned long long) * 4));
+  HIPCHK(hipMalloc(&a->d_desc, sizeof(eh_desc) * ne));
+  HIPCHK(hipMalloc(&a->d_terms, sizeof(eh_fterm) * (a->pool.size() + 1)));
+  if (a->store_all) {
+    a->anchor_bytes = sizeof(double) * 6 * (size_t)n * n_anchor * n_turns;
+    HIPCHK(hipMalloc(&a->d_anchor, a->anchor_bytes));
+  } else {
+    HIPCHK(hipMalloc(&a->d_ckpt, sizeof(double) * 6 * (size_t)n * n_ckpt));
+  }
+  HIPCHK(hipMemcpy(a->d_desc, a->hd.data(), sizeof(eh_desc) * ne, hipMemcpyHostToDevice));
+  if (!a->pool.empty())
+    HIPCHK(hipMemcpy(a->d_terms, a->pool.data(), sizeof(eh_fterm) * a->pool.size(), hipMemcpyHostToDevice));
+  {
+    std::vector<double> h(6 * (size_t)n);
+    for (long i = 0; i < n; i++)
+      for (int c = 0; c < 6; c++)
+        h[c * (size_t)n + i] = coord[10 * i + c];
+    HIPCHK(hipMemcpy(a->d_z, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    std::vector<int32_t> al(n, 1);
+    /* the initial NaN sweep of do_tracking (src/do_tracking.c:478-483) */
+    for (long i = 0; i < n; i++)
+      for (int c = 0; c < 6; c++)
+        if (std::isnan(h[c * (size_t)n + i]) || std::isinf(h[c * (size_t)n + i]))
+          al[i] = 0;
+    HIPCHK(hipMemcpy(a->d_alive, al.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemset(a->d_cnt, 0, sizeof(unsigned long long) * 4));
+  a->fwd_ms = 0;
+  HIPCHK(hipEventRecord(a->ev0, a->stream));
+  for (long k = 0; k < n_ckpt; k++) {
+    int t0 = (int)(k * a->C), t1 = (int)std::min(n_turns, (k + 1) * a->C);
+    double *anc = nullptr;
+    if (a->store_all)
+      anc = a->d_anchor + (size_t)t0 * n_anchor * 6 * n;
+    else
+      HIPCHK(hipMemcpyAsync(a->d_ckpt + (size_t)k * 6 * n, a->d_z, sizeof(double) * 6 * n, hipMemcpyDeviceToDevice,
+                            a->stream));
+    hipLaunchKernelGGL(k_adj_fwd, dim3(grid_for(n)), dim3(EH_BLOCK), 0, a->stream, a->d_z, n, a->d_alive, n,
+                       a->d_desc, a->d_terms, (int)ne, t0, t1, anc, n_anchor, n, a->d_cnt);
+    HIPCHK(hipGetLastError());
+  }
+  r = adj_timed(a, &a->fwd_ms);
+  if (r < 0)
+    return r;
+  unsigned long long cnt[4];
+  HIPCHK(hipMemcpy(cnt, a->d_cnt, sizeof cnt, hipMemcpyDeviceToHost));
+  a->units = (double)cnt[0];
+  a->z_final.resize(6 * (size_t)n);
+  a->alive.resize(n);
+  HIPCHK(hipMemcpy(a->z_final.data(), a->d_z, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(a->alive.data(), a->d_alive, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  long n_alive = 0;
+  for (long i = 0; i < n; i++) {
+    n_alive += a->alive[i] != 0;
+    if (final_coord)
+      for (int c = 0; c < 6; c++)
+        final_coord[6 * i + c] = a->z_final[c * (size_t)n + i];
+    if (alive_out)
+      alive_out[i] = a->alive[i];
+  }
+  a->have_forward = true;
+  return n_alive;
+}
+
+extern "C" long eh_adjoint_backward(eh_adjoint *a, const double *grad_final, double *grad_params,
+                                    double *grad_initial, eh_adjoint_stats *stats) {
+  if (!a || !grad_final || (a->n_params > 0 && !grad_params))
+    return fail(EH_ERR_ARG, "eh_adjoint_backward: bad arguments");
+  if (!a->have_forward)
+    return fail(EH_ERR_STATE, "eh_adjoint_backward: no forward pass recorded");
+  const long n = a->n, ne = a->ne;
+  std::vector<int32_t> surv;
+  for (long i = 0; i < n; i++)
+    if (a->alive[i])
+      surv.push_back((int32_t)i);
+  const long m = (long)surv.size();
+  const int np = (int)a->n_params;
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->n_survivors = m;
+    stats->forward_ms = a->fwd_ms;
+    stats->forward_units = a->units;
+  }
+  if (grad_initial)
+    std::fill(grad_initial, grad_initial + 6 * n, 0.0);
+  if (m == 0) {
+    if (np)
+      std::fill(grad_params, grad_params + np, 0.0);
+    return 0;
+  }
+  const long nb = (m + EH_ADJ_BLOCK - 1) / EH_ADJ_BLOCK;
+  const int npl = np > 0 ? np : 1;
+  double *d_zb = nullptr, *d_lam = nullptr, *d_gpart = nullptr, *d_err = nullptr, *d_grad = nullptr,
+         *d_seg = nullptr;
+  int32_t *d_surv = nullptr, *d_ident = nullptr;
+  auto cleanup = [&]() {
+    void *ptrs[] = {d_zb, d_lam, d_gpart, d_err, d_grad, d_seg, d_surv, d_ident};
+    for (void *p : ptrs)
+      if (p)
+        (void)hipFree(p);
+  };
+  long r = EH_OK;
+  do {
+    std::vector<double> h(6 * (size_t)m);
+    if (hipMalloc(&d_zb, sizeof(double) * 6 * m) != hipSuccess || hipMalloc(&d_lam, sizeof(double) * 6 * m) ||
+        hipMalloc(&d_gpart, sizeof(double) * nb * npl) || hipMalloc(&d_err, sizeof(double) * 2 * nb) ||
+        hipMalloc(&d_grad, sizeof(double) * npl) || hipMalloc(&d_surv, sizeof(int32_t) * m) ||
+        hipMalloc(&d_ident, sizeof(int32_t) * m)) {
+      r = fail(EH_ERR_NOMEM, "eh_adjoint_backward: device allocation failed");
+      break;
+    }
+    for (long j = 0; j < m; j++)
+      for (int c = 0; c < 6; c++)
+        h[c * (size_t)m + j] = a->z_final[c * (size_t)n + surv[j]];
+    if (hipMemcpy(d_zb, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      r = fail(EH_ERR_HIP, "eh_adjoint_backward: upload failed");
+      break;
+    }
+    for (long j = 0; j < m; j++)
+      for (int c = 0; c < 6; c++)
+        h[c * (size_t)m + j] = grad_final[6 * (size_t)surv[j] + c];
+    std::vector<int32_t> ident(m);
+    for (long j = 0; j < m; j++)
+      ident[j] = (int32_t)j;
+    if (hipMemcpy(d_lam, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_surv, surv.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_ident, ident.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice) != hipSuccess) {
+      r = fail(EH_ERR_HIP, "eh_adjoint_backward: upload failed");
+      break;
+    }
+    if (!a->store_all) {
+      size_t seg = sizeof(double) * 6 * (size_t)m * a->n_anchor * a->C;
+      if (hipMalloc(&d_seg, seg) != hipSuccess) {
+        r = fail(EH_ERR_NOMEM, "eh_adjoint_backward: %zu bytes for the segment anchors", seg);
+        break;
+      }
+      a->anchor_bytes = seg;
+    }
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((nb * npl + 255) / 256)), dim3(256), 0, a->stream, d_gpart, nb * npl,
+                       0.0);
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((2 * nb + 255) / 256)), dim3(256), 0, a->stream, d_err, 2 * nb, 0.0);
+    const long n_ckpt = (a->n_turns + a->C - 1) / a->C;
+    double bwd_ms = 0, rec_ms = 0;
+    const size_t lds = sizeof(double) * npl;
+    for (long k = n_ckpt - 1; k >= 0 && r == EH_OK; k--) {
+      int t0 = (int)(k * a->C), t1 = (int)std::min(a->n_turns, (k + 1) * a->C);
+      const double *anc;
+      const int32_t *arow;
+      long a_ld;
+      int ta0;
+      if (a->store_all) {
+        anc = a->d_anchor;
+        arow = d_surv;
+        a_ld = n;
+        ta0 = 0;
+      } else {
+        /* re-track the survivors over this segment, storing their anchors */
+        HIPCHK(hipEventRecord(a->ev0, a->stream));
+        double *zs = d_seg; /* gather into the first slot, then track in place there */
+        double *ztmp = nullptr;
+        if (hipMalloc(&ztmp, sizeof(double) * 6 * m) != hipSuccess) {
+          r = fail(EH_ERR_NOMEM, "eh_adjoint_backward: re-tracking buffer");
+          break;
+        }
+        int32_t *al1 = nullptr;
+        if (hipMalloc(&al1, sizeof(int32_t) * m) != hipSuccess) {
+          (void)hipFree(ztmp);
+          r = fail(EH_ERR_NOMEM, "eh_adjoint_backward: re-tracking buffer");
+          break;
+        }
+        (void)zs;
+        std::vector<int32_t> ones(m, 1);
+        hipError_t e1 = hipMemcpyAsync(al1, ones.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, a->stream);
+        if (e1 == hipSuccess) {
+          hipLaunchKernelGGL(k_adj_gather, dim3(grid_for(m)), dim3(EH_BLOCK), 0, a->stream,
+                             a->d_ckpt + (size_t)k * 6 * n, n, d_surv, m, ztmp);
+          hipLaunchKernelGGL(k_adj_fwd, dim3(grid_for(m)), dim3(EH_BLOCK), 0, a->stream, ztmp, m, al1, m,
+                             a->d_desc, a->d_terms, (int)ne, t0, t1, d_seg, a->n_anchor, m, a->d_cnt + 1);
+          e1 = hipGetLastError();
+        }
+        if (e1 == hipSuccess)
+          e1 = hipStreamSynchronize(a->stream);
+        (void)hipFree(ztmp);
+        (void)hipFree(al1);
+        if (e1 != hipSuccess) {
+          r = fail(EH_ERR_HIP, "eh_adjoint_backward: re-tracking failed: %s", hipGetErrorString(e1));
+          break;
+        }
+        r = adj_timed(a, &rec_ms);
+        if (r < 0)
+          break;
+        anc = d_seg;
+        arow = d_ident;
+        a_ld = m;
+        ta0 = t0;
+      }
+      HIPCHK(hipEventRecord(a->ev0, a->stream));
+      hipLaunchKernelGGL(k_adj_bwd, dim3((unsigned)nb), dim3(EH_ADJ_BLOCK), lds, a->stream, d_zb, d_lam, m, arow, anc,
+                         a->n_anchor, a_ld, a->d_desc, a->d_terms, (int)ne, t0, t1, ta0, np, d_gpart, d_err);
+      hipError_t e2 = hipGetLastError();
+      if (e2 != hipSuccess) {
+        r = fail(EH_ERR_HIP, "k_adj_bwd launch: %s", hipGetErrorString(e2));
+        break;
+      }
+      r = adj_timed(a, &bwd_ms);
+    }
+    if (r < 0)
+      break;
+    if (np > 0) {
+      hipLaunchKernelGGL(k_adj_reduce, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, a->stream, d_gpart, nb, np,
+                         d_grad);
+      if (hipMemcpyAsync(grad_params, d_grad, sizeof(double) * np, hipMemcpyDeviceToHost, a->stream) != hipSuccess ||
+          hipStreamSynchronize(a->stream) != hipSuccess) {
+        r = fail(EH_ERR_HIP, "eh_adjoint_backward: gradient download failed");
+        break;
+      }
+    }
+    std::vector<double> errs(2 * nb);
+    if (hipMemcpy(errs.data(), d_err, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(h.data(), d_lam, sizeof(double) * 6 * m, hipMemcpyDeviceToHost) != hipSuccess) {
+      r = fail(EH_ERR_HIP, "eh_adjoint_backward: download failed");
+      break;
+    }
+    if (grad_initial)
+      for (long j = 0; j < m; j++)
+        for (int c = 0; c < 6; c++)
+          grad_initial[6 * (size_t)surv[j] + c] = h[c * (size_t)m + j];
+    if (stats) {
+      stats->backward_ms = bwd_ms;
+      stats->retrack_ms = rec_ms;
+      for (long b = 0; b < nb; b++) {
+        stats->max_recon_err = std::max(stats->max_recon_err, errs[2 * b]);
+        stats->max_recon_err_s = std::max(stats->max_recon_err_s, errs[2 * b + 1]);
+      }
+      stats->anchor_bytes = (double)a->anchor_bytes;
+      stats->checkpoint_bytes = a->store_all ? 0.0 : (double)(sizeof(double) * 6 * (size_t)n * n_ckpt);
+    }
+  } while (0);
+  cleanup();
+  return r < 0 ? r : m;
 }

@@ -35,10 +35,12 @@ class Lattice:
     def __init__(self, elements=None):
         self.elements = list(elements or [])
         self._structs = None
+        self._cbl = None
 
     def append(self, el):
         self.elements.append(el)
         self._structs = None
+        self._cbl = None
         return self
 
     def extend(self, els):
@@ -102,6 +104,19 @@ class Lattice:
             self._structs = out
         return self._structs
 
+    def set_param(self, i, name, value):
+        """change one parameter of element i, in its struct too (tracking sees it)"""
+        el = self.elements[i]
+        el.params[name] = value
+        if self._structs is not None and self._structs[i] is not None:
+            new = S.make_struct(el.type, el.params)
+            if el.type == S.T_CSBEND:
+                new.edgeFlags = self._edge_flags(i)
+            if el.type == S.T_RFCA:  # keep the fiducial state the run has set
+                for f in ("fiducial_seen", "phase_fiducial", "t_fiducial"):
+                    setattr(new, f, getattr(self._structs[i], f))
+            C.memmove(C.addressof(self._structs[i]), C.addressof(new), C.sizeof(new))
+
     def reset_state(self):
         """forget RFCA fiducials (a new run)"""
         for s in self.structs():
@@ -112,7 +127,13 @@ class Lattice:
 
     def c_beamline(self):
         """(EH_ELEMENT_LIST array, keepalive) for the C ABI; the same structs are shared,
-        so state written by tracking (RFCA fiducials) persists in this Lattice."""
+        so state written by tracking (RFCA fiducials) persists in this Lattice.  Built
+        once and cached (the structs are updated in place by set_param)."""
+        if self._cbl is None or self._structs is None:
+            self._cbl = self._build_c_beamline()
+        return self._cbl
+
+    def _build_c_beamline(self):
         structs = self.structs()
         arr = (S.EH_ELEMENT_LIST * max(1, len(self.elements)))()
         keep = [arr]

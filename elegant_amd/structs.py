@@ -78,7 +78,8 @@ class EH_ELEMENT_LIST(C.Structure):
 
 
 class eh_track_stats(C.Structure):
-    _fields_ = [("particle_element_turns", dbl), ("kernel_ms", dbl), ("launches", C.c_long), ("n_alive", C.c_long)]
+    _fields_ = [("particle_element_turns", dbl), ("kernel_ms", dbl), ("launches", C.c_long), ("n_alive", C.c_long),
+                ("main_ms", dbl), ("main_launches", C.c_long), ("main_units", dbl)]
 
 
 class eh_batch_tune(C.Structure):

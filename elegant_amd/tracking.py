@@ -71,8 +71,11 @@ def track(beam, lattice, n_passes, p_central, mode="fused", turns_per_launch=16)
                              int(turns_per_launch), C.byref(st))
     _lib.check(n)
     del keep
-    return n, {"particle_element_turns": st.particle_element_turns, "kernel_ms": st.kernel_ms,
-               "launches": st.launches, "n_alive": st.n_alive}, pc.value
+    return n, stats_dict(st), pc.value
+
+
+def stats_dict(st):
+    return {k: getattr(st, k) for k, _ in S.eh_track_stats._fields_}
 
 
 def csbend_field_table(b, c, h, nonlinear=1, expansion_order=0):
@@ -122,8 +125,7 @@ class Batch:
         n = lib().eh_batch_track(self._h, ptrs, ne, n_passes, C.byref(pc), C.byref(tune), C.byref(st))
         _lib.check(n)
         del keep
-        return n, {"particle_element_turns": st.particle_element_turns, "kernel_ms": st.kernel_ms,
-                   "launches": st.launches, "n_alive": st.n_alive}
+        return n, stats_dict(st)
 
     def n_alive(self, v):
         return lib().eh_batch_n_alive(self._h, v)

@@ -227,9 +227,14 @@ enum eh_track_mode {
 
 typedef struct {
   double particle_element_turns; /* surviving particles x counted elements, summed over passes */
-  double kernel_ms;              /* GPU time inside tracking kernels */
+  double kernel_ms;              /* GPU time inside all tracking kernels (incl. sweeps, compaction) */
   long launches;
   long n_alive;
+  /* the element-tracking kernel alone (k_fused, k_plain element calls, k_batch spans):
+   * stream time, launches and the particle-element units those launches processed */
+  double main_ms;
+  long main_launches;
+  double main_units;
 } eh_track_stats;
 
 /* do_tracking over a beamline for n_passes (src/do_tracking.c:144, pass loop :498,
@@ -270,6 +275,42 @@ long eh_batch_n_alive(const eh_batch *batch, long v);
 long eh_batch_track(eh_batch *batch, EH_ELEMENT_LIST *const *beamlines, long n_elements, long n_passes,
                     double *P_central, const eh_batch_tune *tune, eh_track_stats *stats);
 
+/* ---------------- reverse-mode adjoint --------------------------------------- */
+/* No reference counterpart: gradients of multi-turn tracking with respect to
+ * element strengths and offsets, for gradient-based tuning in place of the
+ * derivative-free loops of optimize.c (src/optimize.c:912, :2002). */
+typedef struct eh_adjoint eh_adjoint;
+
+typedef struct {
+  double forward_ms;      /* GPU time of the recorded forward pass */
+  double backward_ms;     /* GPU time of the reverse sweeps */
+  double retrack_ms;      /* GPU time re-tracking segments between checkpoints */
+  double forward_units;   /* particle-element units of the forward pass */
+  double max_recon_err;   /* largest |recovered - stored| state (x, xp, y, yp, dp) at an anchor */
+  double max_recon_err_s; /* same for s */
+  double anchor_bytes;    /* device memory for anchor states */
+  double checkpoint_bytes;
+  long n_survivors;
+} eh_adjoint_stats;
+
+eh_adjoint *eh_adjoint_create(void);
+void eh_adjoint_free(eh_adjoint *adj);
+/* Track n particles (rows of 10 doubles, elegant layout) n_turns through the
+ * beamline, exactly as eh_do_tracking does for the survivors, keeping
+ * checkpoints every checkpoint_every turns (1: keep every anchor, no
+ * re-tracking).  param_slots[3*e + k] is the gradient slot of element e's
+ * strength (k=0: K1/K2/K3 of KQUAD/KSEXT/KOCT), DX (k=1) or DY (k=2), or -1.
+ * RF cavity fiducials must already be fixed (track one pass first).  Writes the
+ * final coordinates (n x 6) and survival flags; returns the number of survivors. */
+long eh_adjoint_forward(eh_adjoint *adj, const double *coord, long n, EH_ELEMENT_LIST *beamline, long n_elements,
+                        const int32_t *param_slots, long n_params, long n_turns, double P_central,
+                        long checkpoint_every, double *final_coord, int32_t *alive);
+/* Given dL/d(final coordinates) (n x 6; rows of lost particles are ignored),
+ * writes dL/d(parameters) and, if grad_initial is not NULL, dL/d(initial
+ * coordinates) (n x 6, zero for lost particles).  Returns the survivor count. */
+long eh_adjoint_backward(eh_adjoint *adj, const double *grad_final, double *grad_params, double *grad_initial,
+                         eh_adjoint_stats *stats);
+
 /* ---------------- diagnostics ---------------------------------------------- */
 /* reference row order after the given loss events (no reference counterpart:
  * it reproduces the row permutation of the reference's swap-to-top compaction,
@@ -279,6 +320,10 @@ long eh_replay_order(long n, const int32_t *lost_event, const int32_t *lost_id, 
 /* GPU evaluation of sin, cos, asin, log, sqrt, division and tan for checking
  * bit-identity with the host (no reference counterpart) */
 long eh_selftest_math(const double *x, const double *y, long n, double *out);
+
+/* n doubles copied device-to-device by a plain kernel, reps times: a launch of
+ * known HBM traffic for calibrating memory counters (no reference counterpart) */
+long eh_diag_copy(long n, int reps);
 
 #ifdef __cplusplus
 }

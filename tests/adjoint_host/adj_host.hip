@@ -1,0 +1,101 @@
+/* Test harness (not part of the library): runs the library's per-particle maps
+ * and their reverse-mode adjoints on the host CPU, with the same turn/element
+ * loop, anchors and re-anchoring as the GPU kernels k_adj_fwd / k_adj_bwd, so
+ * that the adjoint arithmetic can be checked against PyTorch autograd without a
+ * GPU.  It includes the library source to reach its descriptor builder. */
+#include "eh_lib.hip"
+
+extern "C" long adjh_gradient(const double *coord, long n, EH_ELEMENT_LIST *bl, long ne, const int32_t *slots,
+                              long np, long turns, double Pc, const double *grad_final, double *final_out,
+                              int32_t *alive_out, double *grad_params, double *grad_z0, double *max_err) {
+  std::vector<eh_desc> hd;
+  std::vector<eh_fterm> pool;
+  long r = build_line(bl, ne, Pc, hd, pool);
+  if (r < 0)
+    return r;
+  int n_anchor = 1;
+  for (long e = 0; e < ne; e++) {
+    if (bl[e].type == EH_T_RFCA && e > 0)
+      hd[e].adj_anchor = n_anchor++;
+    hd[e].adj_pk = slots[3 * e];
+    hd[e].adj_pdx = slots[3 * e + 1];
+    hd[e].adj_pdy = slots[3 * e + 2];
+  }
+  std::fill(grad_params, grad_params + np, 0.0);
+  *max_err = 0;
+  std::vector<eh_p6> anchors((size_t)turns * n_anchor);
+  for (long i = 0; i < n; i++) {
+    eh_p6 p = {coord[10 * i], coord[10 * i + 1], coord[10 * i + 2], coord[10 * i + 3], coord[10 * i + 4],
+               coord[10 * i + 5]};
+    bool live = true;
+    for (long t = 0; t < turns && live; t++)
+      for (long e = 0; e < ne && live; e++) {
+        int slot = e == 0 ? 0 : hd[e].adj_anchor;
+        if (slot >= 0)
+          anchors[t * n_anchor + slot] = p;
+        if (eh_apply_element(p, hd[e], t > 0 ? 1 : 0, pool.data()) >= 0)
+          live = false;
+      }
+    alive_out[i] = live;
+    final_out[6 * i] = p.x;
+    final_out[6 * i + 1] = p.xp;
+    final_out[6 * i + 2] = p.y;
+    final_out[6 * i + 3] = p.yp;
+    final_out[6 * i + 4] = p.s;
+    final_out[6 * i + 5] = p.dp;
+    for (int c = 0; c < 6; c++)
+      grad_z0[6 * i + c] = 0;
+    if (!live)
+      continue;
+    eh_p6 z = p;
+    const double *g = grad_final + 6 * i;
+    eh_p6 l = {g[0], g[1], g[2], g[3], g[4], g[5]};
+    for (long t = turns - 1; t >= 0; t--)
+      for (long e = ne - 1; e >= 0; e--) {
+        const eh_desc &d = hd[e];
+        switch (d.type) {
+        case EH_T_DRIF:
+          eh_adj_drif(z, l, d.length, d.T11);
+          break;
+        case EH_T_EDRIFT:
+          eh_adj_edrift(z, l, d.length);
+          break;
+        case EH_T_KQUAD:
+        case EH_T_KSEXT:
+        case EH_T_KOCT: {
+          eh_pgrad pg = {0, 0, 0};
+          eh_adj_multipole(z, l, d, pg);
+          if (d.adj_pk >= 0)
+            grad_params[d.adj_pk] += pg.k;
+          if (d.adj_pdx >= 0)
+            grad_params[d.adj_pdx] += pg.dx;
+          if (d.adj_pdy >= 0)
+            grad_params[d.adj_pdy] += pg.dy;
+          break;
+        }
+        case EH_T_CSBEND:
+          eh_adj_csbend(z, l, d, pool.data());
+          break;
+        case EH_T_RFCA:
+          eh_adj_rfca(z, l, anchors[t * n_anchor + (e == 0 ? 0 : d.adj_anchor)], d);
+          break;
+        default:
+          break;
+        }
+        if (e == 0) {
+          const eh_p6 &za = anchors[t * n_anchor];
+          double err = std::max(std::max(std::fabs(z.x - za.x), std::fabs(z.xp - za.xp)),
+                                std::max(std::fabs(z.y - za.y), std::fabs(z.yp - za.yp)));
+          *max_err = std::max(*max_err, err);
+          z = za;
+        }
+      }
+    grad_z0[6 * i] = l.x;
+    grad_z0[6 * i + 1] = l.xp;
+    grad_z0[6 * i + 2] = l.y;
+    grad_z0[6 * i + 3] = l.yp;
+    grad_z0[6 * i + 4] = l.s;
+    grad_z0[6 * i + 5] = l.dp;
+  }
+  return n;
+}

@@ -95,22 +95,29 @@ def _gpu(name, idx, turns, fid):
 
 
 def compare(gpu, ng, ref, nr):
+    """deviation stats of survivors, max |ds| of survivors, the number of
+    differences in the loss mask (ID, loss pass) and max |ds| at loss over
+    particles lost on the same pass in both"""
     stats, common = deviation_stats(gpu, ng, ref, nr)
     A, B = by_id(gpu, ng), by_id(ref, nr)
     ids = sorted(set(A) & set(B))
     ds = max((abs(A[i][4] - B[i][4]) for i in ids), default=0.0)
-    mg, mr = loss_mask(gpu, ng), loss_mask(ref, nr)
-    return stats, common, ds, len(mg ^ mr), ng, nr
+    mg = {(i, p): s for (i, p, s) in loss_mask(gpu, ng)}
+    mr = {(i, p): s for (i, p, s) in loss_mask(ref, nr)}
+    both = set(mg) & set(mr)
+    dsl = max((abs(mg[k] - mr[k]) for k in both), default=0.0)
+    return stats, common, ds, len(set(mg) ^ set(mr)), ng, nr, dsl
 
 
 def _table(title, rows):
     lines = ["### " + title, "",
              "| turns | oracle math | survivors gpu/cpu | " + " | ".join("%s max / rms" % c for c in COORD_NAMES) +
-             " | max abs Δs [m] | loss-set diffs |",
-             "|---|---|---|" + "---|" * len(COORD_NAMES) + "---|---|"]
-    for turns, kind, (stats, common, ds, ndiff, ng, nr) in rows:
+             " | max abs Δs [m] | loss-mask diffs (ID, pass) | max abs Δ(loss s) [m] |",
+             "|---|---|---|" + "---|" * len(COORD_NAMES) + "---|---|---|"]
+    for turns, kind, (stats, common, ds, ndiff, ng, nr, dsl) in rows:
         cells = ["%.2e / %.2e" % stats[c] if c in stats else "-" for c in COORD_NAMES]
-        lines.append("| %d | %s | %d / %d | %s | %.2e | %d |" % (turns, kind, ng, nr, " | ".join(cells), ds, ndiff))
+        lines.append("| %d | %s | %d / %d | %s | %.2e | %d | %.2e |" % (turns, kind, ng, nr, " | ".join(cells), ds,
+                                                                        ndiff, dsl))
     return "\n".join(lines) + "\n"
 
 

@@ -1,0 +1,125 @@
+"""The adjoint arithmetic (elegant_amd/csrc/eh_adjoint.h) compiled for the host
+CPU by a test harness (tests/adjoint_host), checked against PyTorch autograd
+through the independent restatement in tests/torch_model.py.  The GPU kernels
+run the same functions; tests/test_gpu_adjoint.py repeats these checks there."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+from elegant_amd import adjoint, configs
+from elegant_amd.lattice import Element, Lattice
+from oracle import oracle
+from tests.torch_model import TorchRing
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "adjoint_host")
+_h = None
+
+
+def harness():
+    global _h
+    if _h is None:
+        subprocess.run(["make", "-s", "-C", HERE], check=True)
+        _h = C.CDLL(os.path.join(HERE, "libadj_host.so"))
+        vp, L = C.c_void_p, C.c_long
+        _h.adjh_gradient.argtypes = [vp, L, vp, L, vp, L, L, C.c_double, vp, vp, vp, vp, vp, vp]
+        _h.adjh_gradient.restype = L
+    return _h
+
+
+def host_gradient(lat, spec, Po, rows, turns, W):
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    n = len(rows)
+    slots = np.ascontiguousarray(spec.slot_array(lat), dtype=np.int32)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    final = np.zeros((n, 6))
+    alive = np.zeros(n, dtype=np.int32)
+    gp = np.zeros(max(1, len(spec)))
+    gz = np.zeros((n, 6))
+    err = C.c_double()
+    arr, keep = lat.c_beamline()
+    r = harness().adjh_gradient(rows.ctypes.data, n, C.cast(arr, C.c_void_p), len(lat), slots.ctypes.data,
+                                len(spec), turns, Po, W.ctypes.data, final.ctypes.data, alive.ctypes.data,
+                                gp.ctypes.data, gz.ctypes.data, C.byref(err))
+    assert r == n
+    return final, alive.astype(bool), gp[:len(spec)], gz, err.value
+
+
+def torch_gradient(lat, spec, Po, rows, turns, W):
+    ring = TorchRing(lat, Po, spec.slot_array(lat), len(spec))
+    theta = torch.tensor(spec.values(lat), requires_grad=True)
+    z0 = torch.tensor(rows[:, :6], requires_grad=True)
+    out = ring.track(z0, theta, turns)
+    (out * torch.tensor(W)).sum().backward()
+    return theta.grad.numpy(), z0.grad.numpy(), out.detach().numpy()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def mix_ring():
+    return Lattice([
+        Element("Q", "KQUAD", L=0.3, K1=1.1, N_KICKS=8, DX=1e-4, DY=-2e-4, DZ=3e-4, TILT=0.02, FSE=1e-3,
+                HKICK=2e-5, VKICK=-1e-5),
+        Element("D", "DRIF", L=0.5),
+        Element("S", "KSEXT", L=0.2, K2=-30.0, K1=0.05, J1=0.03, N_KICKS=4, DX=-5e-5),
+        Element("B", "CSBEND", L=0.9, ANGLE=0.06, K1=-0.3, K2=3.0, N_KICKS=8, E1=0.02, E2=0.03, HGAP=0.012,
+                FINT=0.6),
+        Element("E", "EDRIFT", L=0.4),
+        Element("O", "KOCT", L=0.2, K3=2000.0, TILT=-0.1),
+        Element("QD", "KQUAD", L=0.3, K1=-1.0, N_KICKS=8),
+        Element("D2", "DRIF", L=0.3, ORDER=1),
+    ])
+
+
+WEIGHTS = np.array([1e3, 1e4, 1e3, 1e4, 1.0, 1e3])
+
+
+def test_host_forward_equals_oracle():
+    """the maps compiled for the host reproduce the oracle bit for bit"""
+    Po = configs.p_central_from_mev(2000.0)
+    lat = mix_ring()
+    spec = adjoint.ParamSpec.strengths(lat)
+    rows = configs.fodo_beam(50, seed=8)
+    final, alive, _, _, _ = host_gradient(lat, spec, Po, rows, 4, np.zeros((50, 6)))
+    ref, n, _, _ = oracle.do_tracking(rows, lat, 4, Po)
+    got = {int(r[6]): r[:6] for r in ref[:n]}
+    assert alive.sum() == n
+    for i in np.flatnonzero(alive):
+        np.testing.assert_array_equal(final[i], got[int(rows[i, 6])])
+
+
+def test_mixed_ring_gradients():
+    Po = configs.p_central_from_mev(2000.0)
+    lat = mix_ring()
+    spec = adjoint.ParamSpec.strengths(lat) + adjoint.ParamSpec.offsets(lat)
+    rows = configs.fodo_beam(24, seed=8)
+    rows[:, :4] *= 2.0
+    W = np.random.default_rng(1).normal(size=(24, 6)) * WEIGHTS
+    final, alive, gp, gz, err = host_gradient(lat, spec, Po, rows, 5, W)
+    assert alive.all()
+    tp, tz, tout = torch_gradient(lat, spec, Po, rows, 5, W)
+    assert rel(final, tout) < 1e-9
+    assert rel(gp, tp) < 1e-8, np.c_[gp, tp]
+    assert rel(gz, tz) < 1e-8
+    assert err < 1e-13
+
+
+def test_mba_gradients_with_rf():
+    Po = configs.p_central_from_mev(3000.0)
+    lat = configs.mba_ring(Po)
+    spec = adjoint.ParamSpec.strengths(lat) + adjoint.ParamSpec.offsets(lat)
+    rows = configs.mba_beam(6, seed=9, scale=25.0)
+    oracle.do_tracking(rows, lat, 1, Po)  # RF fiducial
+    W = np.random.default_rng(2).normal(size=(6, 6)) * WEIGHTS
+    final, alive, gp, gz, err = host_gradient(lat, spec, Po, rows, 2, W)
+    assert alive.all()
+    tp, tz, tout = torch_gradient(lat, spec, Po, rows, 2, W)
+    assert rel(final, tout) < 1e-8
+    assert rel(gp, tp) < 1e-7
+    assert rel(gz, tz) < 1e-7
+    assert err < 1e-11

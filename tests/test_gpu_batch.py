@@ -34,3 +34,29 @@ def test_batch_variants_vs_oracle(tune):
         got = batch.download(v)
         assert_rows_identical(got, ref, batch.n_alive(v), nref, "variant %d" % v)
     assert n == total_ref
+
+
+def test_batch_repeated_calls_reuse_tables():
+    """a second track call on unchanged beamlines reuses the device tables and
+    continues exactly like a second do_tracking call of the reference"""
+    _say = print
+    _say("batch repeated calls", flush=True)
+    Po = configs.p_central_from_mev(3000.0)
+    nvar = 3
+    lats = configs.mba_variants(nvar, seed=11)
+    refl = configs.mba_variants(nvar, seed=11)
+    counts = [200, 70, 130]
+    beams = [configs.mba_beam(c, seed=10 + v, scale=400.0) for v, c in enumerate(counts)]
+    batch = tracking.Batch(beams)
+    batch.track(lats, 2, Po)
+    batch.track(lats, 1, Po)
+    n, _ = batch.track(lats, 1, Po, use_graph=False)
+    total = 0
+    for v in range(nvar):
+        r1, n1, _, _ = oracle.do_tracking(beams[v], refl[v], 2, Po)
+        r2, n2, _, _ = oracle.do_tracking(r1[:n1], refl[v], 1, Po)
+        r3, n3, _, _ = oracle.do_tracking(r2[:n2], refl[v], 1, Po)
+        expect = np.concatenate([r3, r2[n2:], r1[n1:]])
+        assert_rows_identical(batch.download(v), expect, batch.n_alive(v), n3, "variant %d" % v)
+        total += n3
+    assert n == total
