@@ -82,6 +82,7 @@ typedef struct {
   double dxi, dyi, dzi, dxf, dyf, dzf;
   double tan_e1, tan_e2, e1, e2, psi1, psi2;
   int32_t csb_fx0, csb_fy0; /* offsets of this element's field terms in the term pool */
+  int32_t csb_maxj;         /* largest power of y in the field terms */
 
   /* ---- RFCA ---- */
   int32_t rf_matrix, rf_nkicks, rf_same, rf_standing, rf_end1, rf_end2, rf_zero_volt;
@@ -191,31 +192,47 @@ EH_HD bool eh_ap_ok(double xIn, double yIn, double xmax, double ymax, int rtilt,
   return true;
 }
 
-/* apply_canonical_multipole_kicks_ret/_noret, src/multipole.h:391-445, with the
- * (x+iy)^n/n! coefficient table of expansion_coefficients (src/multipole.h:76-145) */
+/* powers x^0..x^3 and y^0..y^3 of fillPowerArray (src/multipole.h:41-46), kept
+ * in registers */
+struct eh_pow4 {
+  double x0, x1, x2, x3, y0, y1, y2, y3;
+};
+
+template <int N> EH_HD double eh_px(const eh_pow4 &w) { return N == 0 ? w.x0 : N == 1 ? w.x1 : N == 2 ? w.x2 : w.x3; }
+template <int N> EH_HD double eh_py(const eh_pow4 &w) { return N == 0 ? w.y0 : N == 1 ? w.y1 : N == 2 ? w.y2 : w.y3; }
+
+/* coefficient (-1)^floor(i/2) / (i! (n-i)!) of expansion_coefficients
+ * (src/multipole.h:76-145) */
+template <int ORDER, int I> EH_HD double eh_mcoef() {
+  if (ORDER <= 1)
+    return 1.0;
+  if (ORDER == 2)
+    return I == 0 ? 1. / 2 : (I == 1 ? 1.0 : -1. / 2);
+  return (((I / 2) & 1) ? -1.0 : 1.0) /
+         ((I == 0 ? 1.0 : (I == 1 ? 1.0 : (I == 2 ? 2.0 : 6.0))) *
+          ((ORDER - I) == 0 ? 1.0 : ((ORDER - I) == 1 ? 1.0 : ((ORDER - I) == 2 ? 2.0 : 6.0))));
+}
+
+template <int ORDER, int I> EH_HD void eh_mterm(const eh_pow4 &w, double &sum_Fx, double &sum_Fy) {
+  double f = eh_mcoef<ORDER, I>() * eh_px<ORDER - I>(w) * eh_py<I>(w);
+  if (I & 1)
+    sum_Fx += f;
+  else
+    sum_Fy += f;
+}
+
+/* apply_canonical_multipole_kicks_ret/_noret, src/multipole.h:391-445: the terms
+ * summed in index order as (coef*x^(n-i))*y^i */
 template <int ORDER>
-EH_HD void eh_mkick(double &qx, double &qy, const double *xpow, const double *ypow, double K,
-                                         int skew) {
+EH_HD void eh_mkick(double &qx, double &qy, const eh_pow4 &w, double K, int skew) {
   double sum_Fx = 0, sum_Fy = 0;
-#pragma unroll
-  for (int i = 0; i <= ORDER; i++) {
-    double coef;
-    if (ORDER == 0)
-      coef = 1.0;
-    else if (ORDER == 1)
-      coef = 1.0;
-    else if (ORDER == 2)
-      coef = (i == 0) ? 1. / 2 : (i == 1 ? 1.0 : -1. / 2);
-    else
-      coef = (((i / 2) & 1) ? -1.0 : 1.0) /
-             ((i == 0 ? 1.0 : (i == 1 ? 1.0 : (i == 2 ? 2.0 : 6.0))) *
-              ((ORDER - i) == 0 ? 1.0 : ((ORDER - i) == 1 ? 1.0 : ((ORDER - i) == 2 ? 2.0 : 6.0))));
-    double f = coef * xpow[ORDER - i] * ypow[i];
-    if (i & 1)
-      sum_Fx += f;
-    else
-      sum_Fy += f;
-  }
+  eh_mterm<ORDER, 0>(w, sum_Fx, sum_Fy);
+  if (ORDER >= 1)
+    eh_mterm<ORDER, (ORDER >= 1 ? 1 : 0)>(w, sum_Fx, sum_Fy);
+  if (ORDER >= 2)
+    eh_mterm<ORDER, (ORDER >= 2 ? 2 : 0)>(w, sum_Fx, sum_Fy);
+  if (ORDER >= 3)
+    eh_mterm<ORDER, (ORDER >= 3 ? 3 : 0)>(w, sum_Fx, sum_Fy);
   if (skew) {
     double t = sum_Fx;
     sum_Fx = sum_Fy;
@@ -226,20 +243,19 @@ EH_HD void eh_mkick(double &qx, double &qy, const double *xpow, const double *yp
   qy += K * sum_Fx;
 }
 
-EH_HD void eh_mkick_n(int order, double &qx, double &qy, const double *xpow, const double *ypow,
-                                           double K, int skew) {
+EH_HD void eh_mkick_n(int order, double &qx, double &qy, const eh_pow4 &w, double K, int skew) {
   switch (order) {
   case 0:
-    eh_mkick<0>(qx, qy, xpow, ypow, K, skew);
+    eh_mkick<0>(qx, qy, w, K, skew);
     break;
   case 1:
-    eh_mkick<1>(qx, qy, xpow, ypow, K, skew);
+    eh_mkick<1>(qx, qy, w, K, skew);
     break;
   case 2:
-    eh_mkick<2>(qx, qy, xpow, ypow, K, skew);
+    eh_mkick<2>(qx, qy, w, K, skew);
     break;
   default:
-    eh_mkick<3>(qx, qy, xpow, ypow, K, skew);
+    eh_mkick<3>(qx, qy, w, K, skew);
     break;
   }
 }
@@ -300,7 +316,7 @@ EH_HD bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
     if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
       lost = true;
     } else {
-      double xpow[4], ypow[4];
+      eh_pow4 w;
       for (int ik = 0; ik < d.nslices && !lost; ik++) {
         if (!eh_ap_ok(x + d.mdx, y + d.mdy, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1)) {
           lost = true;
@@ -320,21 +336,21 @@ EH_HD bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
           }
           if (step == d.nsub - 1)
             break;
-          xpow[0] = 1;
-          ypow[0] = 1;
-          xpow[1] = xpow[0] * x;
-          ypow[1] = ypow[0] * y;
-          xpow[2] = xpow[1] * x;
-          ypow[2] = ypow[1] * y;
-          xpow[3] = xpow[2] * x;
-          ypow[3] = ypow[2] * y;
+          w.x0 = 1;
+          w.y0 = 1;
+          w.x1 = w.x0 * x;
+          w.y1 = w.y0 * y;
+          w.x2 = w.x1 * x;
+          w.y2 = w.y1 * y;
+          w.x3 = w.x2 * x;
+          w.y3 = w.y2 * y;
           for (int io = 0; io < 3; io++)
             if (d.mp_order[io])
-              eh_mkick_n(d.mp_order[io], qx, qy, xpow, ypow, d.kick[step][io], d.mp_skew[io]);
+              eh_mkick_n(d.mp_order[io], qx, qy, w, d.kick[step][io], d.mp_skew[io]);
           if (d.has_xkick)
-            eh_mkick<0>(qx, qy, xpow, ypow, d.xk[step], 0);
+            eh_mkick<0>(qx, qy, w, d.xk[step], 0);
           if (d.has_ykick)
-            eh_mkick<0>(qx, qy, xpow, ypow, d.yk[step], 1);
+            eh_mkick<0>(qx, qy, w, d.yk[step], 1);
           if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
             lost = true;
             break;
@@ -390,42 +406,82 @@ EH_HD bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
 }
 
 /* ------------------------------------------------------------------------- */
-/* computeCSBENDFields, src/csbend.c:99-172, over the non-zero terms in the
- * reference's summation order (i outer, j inner) */
-EH_HD void eh_csb_fields(const eh_desc &d, const eh_fterm *__restrict__ terms, double x,
-                                              double y, double &Fx, double &Fy) {
+/* y^0 .. y^10 as computeCSBENDFields builds them (yp[j] = yp[j-1]*y), in
+ * registers; only the powers up to maxj are formed */
+struct eh_ypow {
+  double p[11];
+};
+
+EH_HD void eh_ypow_fill(eh_ypow &w, double y, int maxj) {
+  w.p[0] = 1;
+#pragma unroll
+  for (int j = 1; j <= 10; j++)
+    if (j <= maxj)
+      w.p[j] = w.p[j - 1] * y;
+}
+
+/* the power for a wave-uniform exponent: a scalar branch, no indexed register */
+EH_HD double eh_ypick(const eh_ypow &w, int j) {
+  switch (j) {
+  case 0:
+    return w.p[0];
+  case 1:
+    return w.p[1];
+  case 2:
+    return w.p[2];
+  case 3:
+    return w.p[3];
+  case 4:
+    return w.p[4];
+  case 5:
+    return w.p[5];
+  case 6:
+    return w.p[6];
+  case 7:
+    return w.p[7];
+  case 8:
+    return w.p[8];
+  case 9:
+    return w.p[9];
+  default:
+    return w.p[10];
+  }
+}
+
+/* computeCSBENDFields, src/csbend.c:99-172.  The reference adds
+ * coef * x^i * y^j over all (i, j) in i-outer, j-inner order; terms with a zero
+ * coefficient are skipped here, which leaves every finite sum unchanged (adding
+ * a signed zero to a sum that starts at +0 never changes it). */
+EH_HD void eh_csb_fields(const eh_desc &d, const eh_fterm *__restrict__ terms, double x, double y, double &Fx,
+                         double &Fy) {
   if (!d.csb_field) {
     Fx = 0;
     Fy = 1;
     return;
   }
+  eh_ypow w;
+  eh_ypow_fill(w, y, d.csb_maxj);
   double sumFx = 0, sumFy = 0;
   double xt = 1;
   int ic = 0;
   const eh_fterm *fx = terms + d.csb_fx0, *fy = terms + d.csb_fy0;
   for (int t = 0; t < d.csb_nfx; t++) {
-    int ti = fx[t].i, tj = fx[t].j;
+    int ti = fx[t].i;
     while (ic < ti) {
       xt *= x;
       ic++;
     }
-    double yj = 1;
-    for (int k = 0; k < tj; k++)
-      yj = yj * y;
-    sumFx += fx[t].coef * xt * yj;
+    sumFx += fx[t].coef * xt * eh_ypick(w, fx[t].j);
   }
   xt = 1;
   ic = 0;
   for (int t = 0; t < d.csb_nfy; t++) {
-    int ti = fy[t].i, tj = fy[t].j;
+    int ti = fy[t].i;
     while (ic < ti) {
       xt *= x;
       ic++;
     }
-    double yj = 1;
-    for (int k = 0; k < tj; k++)
-      yj = yj * y;
-    sumFy += fy[t].coef * xt * yj;
+    sumFy += fy[t].coef * xt * eh_ypick(w, fy[t].j);
   }
   Fx = sumFx;
   Fy = sumFy;

@@ -404,6 +404,11 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
     }
   d->csb_nfx = (int32_t)fx.size();
   d->csb_nfy = (int32_t)fy.size();
+  d->csb_maxj = 0;
+  for (const eh_fterm &t : fx)
+    d->csb_maxj = std::max(d->csb_maxj, t.j);
+  for (const eh_fterm &t : fy)
+    d->csb_maxj = std::max(d->csb_maxj, t.j);
   d->csb_fx0 = (int32_t)pool.size();
   pool.insert(pool.end(), fx.begin(), fx.end());
   d->csb_fy0 = (int32_t)pool.size();
@@ -700,7 +705,7 @@ extern "C" long eh_beam_upload(eh_beam *b, double **coord, long n_part) {
 /* kernels */
 #define EH_BLOCK 256
 
-__device__ __forceinline__ eh_p6 load_p(const eh_soa &a, long i) {
+__device__ __forceinline__ eh_p6 load_p(const eh_soa a, long i) {
   eh_p6 p;
   p.x = a.x[i];
   p.xp = a.xp[i];
@@ -711,7 +716,7 @@ __device__ __forceinline__ eh_p6 load_p(const eh_soa &a, long i) {
   return p;
 }
 
-__device__ __forceinline__ void store_p(const eh_soa &a, long i, const eh_p6 &p) {
+__device__ __forceinline__ void store_p(const eh_soa a, long i, const eh_p6 &p) {
   a.x[i] = p.x;
   a.xp[i] = p.xp;
   a.y[i] = p.y;
@@ -859,8 +864,8 @@ __device__ __forceinline__ void track_span(eh_p6 &p, bool &alive, const eh_desc 
 /* write a wavefront's particles after tracking: survivors to out at
  * out_base + atomicAdd(out_count), lost ones to the graveyard */
 __device__ __forceinline__ void wave_store(const eh_p6 &p, bool in, bool alive, int32_t row0, int32_t lpass,
-                                           int32_t levent, const eh_soa &out, long out_base,
-                                           unsigned long long *out_count, const eh_soa &g,
+                                           int32_t levent, const eh_soa out, long out_base,
+                                           unsigned long long *out_count, const eh_soa g,
                                            unsigned long long *grave_count) {
   bool lost = in && !alive;
   unsigned na, nl;

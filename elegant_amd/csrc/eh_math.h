@@ -73,9 +73,18 @@ EH_FN double eh_sqrt(double x) { return __builtin_sqrt(x); }
  * rounding errors of products */
 EH_FN double eh_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-/* Taylor polynomial of sin(r) for |r| <= pi/4 */
+/* Taylor polynomial of sin(r) for |r| <= pi/4.  Below 2^-4 the series is cut
+ * after the r^11 term: the next one is below 2^-80 of the result. */
 EH_FN double eh_sin_poly(double r) {
   double z = r * r;
+  if (eh_fabs(r) < 0x1p-4) {
+    double q = EH_SIN_5;
+    q = EH_SIN_4 + z * q;
+    q = EH_SIN_3 + z * q;
+    q = EH_SIN_2 + z * q;
+    q = EH_SIN_1 + z * q;
+    return r + (r * z) * q;
+  }
   double p = EH_SIN_10;
   p = EH_SIN_9 + z * p;
   p = EH_SIN_8 + z * p;
@@ -89,18 +98,28 @@ EH_FN double eh_sin_poly(double r) {
   return r + (r * z) * p;
 }
 
-/* Taylor polynomial of cos(r) for |r| <= pi/4 */
+/* Taylor polynomial of cos(r) for |r| <= pi/4 (below 2^-4 cut after r^12; the
+ * next term is below 2^-76 of the result) */
 EH_FN double eh_cos_poly(double r) {
   double z = r * r;
-  double p = EH_COS_10;
-  p = EH_COS_9 + z * p;
-  p = EH_COS_8 + z * p;
-  p = EH_COS_7 + z * p;
-  p = EH_COS_6 + z * p;
-  p = EH_COS_5 + z * p;
-  p = EH_COS_4 + z * p;
-  p = EH_COS_3 + z * p;
-  p = EH_COS_2 + z * p;
+  double p;
+  if (eh_fabs(r) < 0x1p-4) {
+    p = EH_COS_6;
+    p = EH_COS_5 + z * p;
+    p = EH_COS_4 + z * p;
+    p = EH_COS_3 + z * p;
+    p = EH_COS_2 + z * p;
+  } else {
+    p = EH_COS_10;
+    p = EH_COS_9 + z * p;
+    p = EH_COS_8 + z * p;
+    p = EH_COS_7 + z * p;
+    p = EH_COS_6 + z * p;
+    p = EH_COS_5 + z * p;
+    p = EH_COS_4 + z * p;
+    p = EH_COS_3 + z * p;
+    p = EH_COS_2 + z * p;
+  }
   /* 1 - z/2 + z^2 p.  1 - z/2 is formed with its exact rounding error (Fast2Sum,
    * valid since z/2 <= 1) and the exact error of z = r*r, so that the final
    * addition is the only rounding of the leading terms: the result is then
@@ -203,9 +222,29 @@ EH_FN double eh_tan(double x) {
   return s / c;
 }
 
-/* asin Taylor series; z = x*x, valid for |x| <= 1/8 */
+/* asin Taylor series; z = x*x, valid for |x| <= 1/8.  Shorter below 2^-8
+ * (4 terms) and 2^-4 (7 terms): the first omitted term is below 2^-80 resp.
+ * 2^-64 of the result. */
 EH_FN double eh_asin_small(double x) {
   double z = x * x;
+  double ax = eh_fabs(x);
+  if (ax < 0x1p-8) {
+    double q = EH_ASIN_4;
+    q = EH_ASIN_3 + z * q;
+    q = EH_ASIN_2 + z * q;
+    q = EH_ASIN_1 + z * q;
+    return x + (x * z) * q;
+  }
+  if (ax < 0x1p-4) {
+    double q = EH_ASIN_7;
+    q = EH_ASIN_6 + z * q;
+    q = EH_ASIN_5 + z * q;
+    q = EH_ASIN_4 + z * q;
+    q = EH_ASIN_3 + z * q;
+    q = EH_ASIN_2 + z * q;
+    q = EH_ASIN_1 + z * q;
+    return x + (x * z) * q;
+  }
   double p = EH_ASIN_10;
   p = EH_ASIN_9 + z * p;
   p = EH_ASIN_8 + z * p;
@@ -229,6 +268,7 @@ EH_FN double eh_asin_mid(double x) {
                         EH_ASIN_36, EH_ASIN_37, EH_ASIN_38, EH_ASIN_39, EH_ASIN_40};
   double z = x * x;
   double p = a[39];
+#pragma unroll
   for (int n = 38; n >= 0; n--)
     p = a[n] + z * p;
   return x + (x * z) * p;
