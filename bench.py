@@ -219,6 +219,18 @@ def leg_adjoint(args):
     t_bwd = time.perf_counter() - t0
     st = adj.stats
     adj.free()
+    # the same gradient with every anchor kept by the forward pass (no re-tracking):
+    # 48 B per particle per anchor per turn, affordable in HBM at this size
+    adj1 = adjoint.AdjointTracker(lat, spec, Po, n_turns=turns, checkpoint_every=1)
+    t0 = time.perf_counter()
+    adj1.forward(rows)
+    gp1, _ = adj1.backward(G)
+    t_all1 = time.perf_counter() - t0
+    st1 = adj1.stats
+    adj1.free()
+    log("adjoint, anchors kept (checkpoint every turn): %.2f s forward+backward, %.1f MB of anchors; "
+        "gradient agrees to %.2e" % (t_all1, st1["anchor_bytes"] / 1e6,
+                                     np.linalg.norm(gp1 - gp) / max(np.linalg.norm(gp), 1e-300)))
     log("adjoint: %d turns, %d particles, %d survivors; forward %.2f s, recorded forward %.2f s, backward %.2f s "
         "(GPU: %.2f fwd, %.2f re-track, %.2f reverse); recon err %.2e" % (
             turns, len(rows), n, t_fwd, t_rec, t_bwd, st["forward_ms"] / 1e3, st["retrack_ms"] / 1e3,
@@ -229,6 +241,8 @@ def leg_adjoint(args):
             "gpu_reverse_s": round(st["backward_ms"] / 1e3, 3), "turns": turns, "particles": len(rows),
             "survivors": int(n), "parameters": len(spec), "checkpoint_every": args.adj_ckpt,
             "max_recon_err": st["max_recon_err"], "grad_norm": float(np.linalg.norm(gp)),
+            "ratio_anchors_kept": round(t_all1 / t_fwd, 3), "anchors_kept_bytes": st1["anchor_bytes"],
+            "grad_rel_diff_anchors_kept": float(np.linalg.norm(gp1 - gp) / max(np.linalg.norm(gp), 1e-300)),
             "workload": "MBA ring 3 GeV, 6D Gaussian beam (emittances x%g of 5 nm / 50 pm), L = mean(x^2/bx + "
                         "y^2/by) after the turns, d/d(720 K1/K2/K3)" % args.adj_scale}
 

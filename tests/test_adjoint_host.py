@@ -13,7 +13,7 @@ import torch
 from elegant_amd import adjoint, configs
 from elegant_amd.lattice import Element, Lattice
 from oracle import oracle
-from tests.torch_model import TorchRing
+from tests.torch_model import TorchRing, forward_mode_projection
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "adjoint_host")
 _h = None
@@ -122,4 +122,29 @@ def test_mba_gradients_with_rf():
     assert rel(final, tout) < 1e-8
     assert rel(gp, tp) < 1e-7
     assert rel(gz, tz) < 1e-7
+    assert err < 1e-11
+
+
+def test_forward_mode_ten_turns_host():
+    """10 turns of the MBA ring: the adjoint's gradient projected on a random
+    direction equals the complex-step (forward-mode) derivative along it"""
+    from elegant_amd import optics
+    Po = configs.p_central_from_mev(3000.0)
+    lat = configs.mba_ring(Po)
+    spec = adjoint.ParamSpec.strengths(lat)
+    rows = configs.mba_beam(4, seed=10, scale=25.0)
+    oracle.do_tracking(rows, lat, 1, Po)  # RF fiducial
+    tw = optics.periodic_twiss(configs.mba_period())
+    bx, by = tw["betax"], tw["betay"]
+    turns = 10
+    W0 = np.zeros((4, 6))
+    final, alive, _, _, _ = host_gradient(lat, spec, Po, rows, turns, W0)
+    assert alive.all()
+    G = np.zeros((4, 6))
+    G[:, 0] = 2 * final[:, 0] / bx / 4
+    G[:, 2] = 2 * final[:, 2] / by / 4
+    _, _, gp, _, err = host_gradient(lat, spec, Po, rows, turns, G)
+    v = np.random.default_rng(3).normal(size=len(spec)) * np.abs(spec.values(lat))
+    dL = forward_mode_projection(lat, spec, Po, rows, turns, bx, by, v)
+    assert abs(float(np.dot(gp, v)) - dL) <= 1e-7 * abs(dL), (float(np.dot(gp, v)), dL)
     assert err < 1e-11

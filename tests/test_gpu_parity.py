@@ -163,3 +163,19 @@ def test_mba_da_grid(mode, ehlib, oracle_mod):
     got, n, ref, nref, stats, eff = _track_both(lat, rows, 3, Po, mode, tpl=2)
     assert_rows_identical(got, ref, n, nref, "mba " + mode)
     assert stats["particle_element_turns"] == eff
+
+
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+def test_golden_fixtures(mode):
+    """the GPU reproduces the frozen oracle fixtures of tests/golden bit for bit"""
+    import os
+    from tests.golden.make_golden import cases
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for name, lat, rows, turns, Po in cases():
+        _say("golden fixture %s, %s" % (name, mode))
+        f = np.load(os.path.join(here, "%s.npz" % name))
+        beam = tracking.Beam(rows)
+        n, _, _ = tracking.track(beam, lat, turns, Po, mode=mode, turns_per_launch=2)
+        got = beam.download()
+        beam.free()
+        assert_rows_identical(got, f["out"], n, int(f["n"]), "%s %s" % (name, mode))

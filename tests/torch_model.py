@@ -4,8 +4,9 @@ surviving particles, used to check the HIP adjoint.
 TEST INFRASTRUCTURE ONLY.  It follows the same reference routines as the
 oracle (src/multipole.c:660-1456, src/csbend.c:654-1720, src/simple_rfca.c:824-967,
 src/matrix.c order-2 DRIF, src/csbend.c:3792-3816 EDRIFT), written with torch
-operations so that autograd (reverse mode) and torch.func.jvp (forward mode)
-give exact derivatives.  Losses and apertures are not modelled: callers use
+operations so that autograd (reverse mode) and complex-step evaluation (forward
+mode: every map is analytic, so Im f(z + i h v) / h is the exact directional
+derivative, without subtractive cancellation) give exact derivatives.  Losses and apertures are not modelled: callers use
 particles that survive.  Element constants are taken from the lattice's element
 structs (the same parameter values the library reads); CSBEND field
 coefficients come from the oracle's field table.
@@ -48,8 +49,17 @@ def _s2m(xp, yp, dp):
 
 
 def _m2s(qx, qy, dp):
-    den = torch.sqrt((1 + dp) ** 2 - qx * qx - qy * qy)
+    den = torch.sqrt((1 + dp) * (1 + dp) - qx * qx - qy * qy)
     return qx / den, qy / den
+
+
+def _powers(v, n):
+    """[v^0, ..., v^n] by repeated multiplication (as the reference's power
+    arrays; also keeps complex-step evaluation free of branch cuts)"""
+    out = [torch.ones_like(v)]
+    for _ in range(n):
+        out.append(out[-1] * v)
+    return out
 
 
 def _rot(x, y, xp, yp, c, s):
@@ -196,8 +206,10 @@ class TorchRing:
     def _fields(cs, X, Y):
         if not cs["field"]:
             return torch.zeros_like(X), torch.ones_like(X)
-        Fx = sum(c * X ** i * Y ** j for (i, j, c) in cs["tx"]) if cs["tx"] else torch.zeros_like(X)
-        Fy = sum(c * X ** i * Y ** j for (i, j, c) in cs["ty"]) if cs["ty"] else torch.zeros_like(X)
+        px = _powers(X, 10)
+        py = _powers(Y, 10)
+        Fx = sum(c * px[i] * py[j] for (i, j, c) in cs["tx"]) if cs["tx"] else torch.zeros_like(X)
+        Fy = sum(c * px[i] * py[j] for (i, j, c) in cs["ty"]) if cs["ty"] else torch.zeros_like(X)
         return Fx, Fy
 
     def _csbend(self, z, i, st):
@@ -217,7 +229,7 @@ class TorchRing:
         for _ in range(st.nSlices):
             for j in range(nsub):
                 th = sl * dfr[j] / rho0
-                f = torch.sqrt((1 + dp) ** 2 - QY * QY)
+                f = torch.sqrt((1 + dp) * (1 + dp) - QY * QY)
                 phi = torch.asin(QX / f)
                 psi = th + phi
                 QXn = f * torch.sin(psi)
@@ -307,9 +319,25 @@ class TorchRing:
                 z = self._rfca(z, st)
         return z
 
-    def track(self, z0, theta, n_turns):
-        """z0: (n, 6) tensor; theta: (n_params,) tensor; returns (n, 6)"""
+    def track(self, z0, theta, n_turns, progress=None):
+        """z0: (n, 6) tensor; theta: (n_params,) tensor; returns (n, 6).  Real
+        tensors give autograd derivatives; complex ones a complex-step
+        (forward-mode) derivative: every map here is analytic in its inputs."""
         z = [z0[:, k] for k in range(6)]
-        for _ in range(n_turns):
+        for t in range(n_turns):
             z = self.turn(z, theta)
+            if progress:
+                progress(t)
         return torch.stack(z, dim=1)
+
+
+def forward_mode_projection(lat, spec, Po, rows, turns, bx, by, v, progress=None):
+    """d/dh of L(theta + h v), L = mean(x^2/bx + y^2/by) after `turns`, by complex
+    step (h = 1e-30) through this restatement; spec is an adjoint.ParamSpec"""
+    ring = TorchRing(lat, Po, spec.slot_array(lat), len(spec))
+    h = 1e-30
+    theta = torch.tensor(spec.values(lat), dtype=torch.complex128) + 1j * h * torch.tensor(v, dtype=torch.complex128)
+    z0 = torch.tensor(rows[:, :6], dtype=torch.complex128)
+    out = ring.track(z0, theta, turns, progress=progress)
+    L = (out[:, 0] * out[:, 0] / bx + out[:, 2] * out[:, 2] / by).mean()
+    return float(L.imag) / h
