@@ -247,6 +247,44 @@ def leg_adjoint(args):
                         "y^2/by) after the turns, d/d(720 K1/K2/K3)" % args.adj_scale}
 
 
+def probe_elements(particles=1000000, turns=4):
+    """time k_fused on rings of 100 copies of one element type (per-type cost)"""
+    import math
+    from elegant_amd import configs, tracking
+    from elegant_amd.lattice import Element, Lattice
+    Po = configs.p_central_from_mev(3000.0)
+    ang = 2 * math.pi / 200
+    cases = {
+        "CSBEND K1 N_KICKS=16": Element("B", "CSBEND", L=0.8, ANGLE=ang, K1=-0.5, N_KICKS=16, EDGE1_EFFECTS=0,
+                                        EDGE2_EFFECTS=0),
+        "CSBEND N_KICKS=16": Element("B", "CSBEND", L=0.8, ANGLE=ang, N_KICKS=16, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0),
+        "KQUAD N_KICKS=8": Element("Q", "KQUAD", L=0.25, K1=0.8, N_KICKS=8),
+        "KSEXT N_KICKS=4": Element("S", "KSEXT", L=0.2, K2=20.0, N_KICKS=4),
+        "KOCT": Element("O", "KOCT", L=0.2, K3=800.0),
+        "DRIF": Element("D", "DRIF", L=0.2),
+        "EDRIFT": Element("E", "EDRIFT", L=0.2),
+    }
+    side = int(round(math.sqrt(particles)))
+    rows = configs.da_grid(side, side, xmax=1e-3, ymax=1e-3)
+    out = {}
+    for name, el in cases.items():
+        els = []
+        for i in range(100):
+            prm = dict(el.params)
+            if "K1" in prm:
+                prm["K1"] = prm["K1"] * (1 if i % 2 else -1)  # alternating gradients keep the beam
+            els.append(Element(el.name + str(i), el.type, **prm))
+        beam = tracking.Beam(rows)
+        lat = Lattice(els)
+        tracking.track(beam, lat, 1, Po, mode="fused", turns_per_launch=1)
+        n, st, _ = tracking.track(beam, lat, turns, Po, mode="fused", turns_per_launch=turns)
+        beam.free()
+        ns = st["main_ms"] * 1e6 / max(1.0, st["main_units"])
+        out[name] = round(ns, 4)
+        log("probe %-22s %8d alive  %.4f ns per particle-element" % (name, n, ns))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +307,8 @@ def main():
     ap.add_argument("--adj-particles", type=int, default=100000)
     ap.add_argument("--adj-ckpt", type=int, default=50)
     ap.add_argument("--adj-scale", type=float, default=400.0, help="beam emittance scale (amplitude ~ DA)")
+    ap.add_argument("--probe-elements", action="store_true",
+                    help="also time rings of one element type each (per-type cost of the fused kernel)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="first run a plain copy kernel over N doubles (known HBM traffic, for counter scaling)")
     args = ap.parse_args()
@@ -340,6 +380,8 @@ def main():
         out["plain"] = leg_plain(args)
         if "batched" in out:
             out["batched_over_plain"] = round(out["batched"]["value"] / out["plain"]["value"], 2)
+    if args.probe_elements:
+        out["element_probe_ns_per_unit"] = probe_elements()
     if not args.no_adjoint:
         out["adjoint"] = leg_adjoint(args)
         out["adjoint_over_forward"] = out["adjoint"]["ratio"]

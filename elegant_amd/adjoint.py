@@ -1,6 +1,6 @@
 """Gradients of multi-turn tracking with respect to element strengths and offsets.
 
-    spec = ParamSpec.strengths(lattice)             # every KQUAD K1, KSEXT K2, KOCT K3
+    spec = ParamSpec.strengths(lattice)             # every KQUAD K1, KSEXT K2, KOCT K3, KQUSE K1
     adj = AdjointTracker(lattice, spec, p_central, n_turns=1000, checkpoint_every=50)
     theta = torch.tensor(spec.values(lattice), dtype=torch.float64, requires_grad=True)
     final, alive = track_turns(theta, z0, adj)      # z0: (n, 6) float64 tensor
@@ -21,9 +21,9 @@ from . import _lib
 from . import structs as S
 from . import tracking
 
-KIND_FIELD = {(S.T_KQUAD, "K"): "k1", (S.T_KSEXT, "K"): "k2", (S.T_KOCT, "K"): "k3"}
+KIND_FIELD = {(S.T_KQUAD, "K"): "k1", (S.T_KSEXT, "K"): "k2", (S.T_KOCT, "K"): "k3", (S.T_KQUSE, "K"): "k1"}
 KINDS = ("K", "DX", "DY")
-MULTIPOLES = (S.T_KQUAD, S.T_KSEXT, S.T_KOCT)
+MULTIPOLES = (S.T_KQUAD, S.T_KSEXT, S.T_KOCT, S.T_KQUSE)
 
 
 class ParamSpec:
@@ -75,7 +75,7 @@ class ParamSpec:
         for q, occ in enumerate(self.slots):
             for (i, kind) in occ:
                 if lat.elements[i].type not in MULTIPOLES:
-                    raise ValueError("element %d (%s) is not a KQUAD/KSEXT/KOCT" % (i, lat.elements[i].name))
+                    raise ValueError("element %d (%s) is not a KQUAD/KSEXT/KOCT/KQUSE" % (i, lat.elements[i].name))
                 k = KINDS.index(kind)
                 if arr[i, k] >= 0:
                     raise ValueError("element %d %s appears in two slots" % (i, kind))
@@ -102,7 +102,7 @@ class ParamSpec:
         for q, occ in enumerate(self.slots):
             for (i, kind) in occ:
                 e = lat.elements[i]
-                name = param_name[kind] or {S.T_KQUAD: "K1", S.T_KSEXT: "K2", S.T_KOCT: "K3"}[e.type]
+                name = param_name[kind] or {S.T_KQUAD: "K1", S.T_KSEXT: "K2", S.T_KOCT: "K3", S.T_KQUSE: "K1"}[e.type]
                 lat.set_param(i, name, float(theta[q]))
 
 

@@ -27,7 +27,7 @@ from . import tracking
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "elegant_amd", "autotune.json")
 
 BLOCKS = (64, 128, 256)
-WAVES_PER_CU = (4, 8, 16)
+WAVES_PER_CU = (4, 8, 16, 24)
 SPAN_FACTORS = (0.25, 0.5, 1, 2, 4)   # elements per launch, in units of the lattice length
 
 

@@ -34,7 +34,7 @@ def multipole(etype, p):
     ns = int(_get(p, "N_SLICES", default=4 if etype == S.T_KOCT else 1))
     if nk > 0:
         ns = max(1, math.ceil(nk / integ)) if integ > 2 else nk
-    orders = {S.T_KQUAD: [1], S.T_KSEXT: [2], S.T_KOCT: [3]}[etype][:]
+    orders = {S.T_KQUAD: [1], S.T_KSEXT: [2], S.T_KOCT: [3], S.T_KQUSE: [1, 2]}[etype][:]
     if etype == S.T_KSEXT:
         if _get(p, "K1"):
             orders.append(1)
@@ -105,7 +105,7 @@ def per_unit(el):
         return 11, 0  # x, y and the second-order path length of the drift map
     if t == S.T_EDRIFT:
         return 11, 0
-    if t in (S.T_KQUAD, S.T_KSEXT, S.T_KOCT):
+    if t in (S.T_KQUAD, S.T_KSEXT, S.T_KOCT, S.T_KQUSE):
         return multipole(t, p)
     if t == S.T_CSBEND:
         return csbend(p)

@@ -76,6 +76,8 @@ typedef struct {
   int32_t csb_nfx, csb_nfy;
   int32_t csb_edge1, csb_edge2;
   double rho0, rho_actual;
+  int32_t csb_recip;        /* 1: divide by rho0 and rho_actual through the reciprocals below */
+  double rho0_rh, rho0_rl, rhoa_rh, rhoa_rl; /* RN(1/rho) and RN(1/rho - RN(1/rho)) */
   double csb_dsh[EH_MAXSUB], csb_dshr[EH_MAXSUB], csb_ds[EH_MAXSUB];
   double csb_dzl; /* n * s */
   double cos_tt, sin_tt;
@@ -83,6 +85,9 @@ typedef struct {
   double tan_e1, tan_e2, e1, e2, psi1, psi2;
   int32_t csb_fx0, csb_fy0; /* offsets of this element's field terms in the term pool */
   int32_t csb_maxj;         /* largest power of y in the field terms */
+  int32_t csb_dense;        /* EO1 of the dense blocks at csb_dx0/csb_dy0 (5 or 6), else 0 */
+  int32_t csb_dx0, csb_dy0;
+  uint32_t csb_live;        /* dense blocks: bit t set if the t-th term (i-outer, j-inner) is non-zero */
 
   /* ---- RFCA ---- */
   int32_t rf_matrix, rf_nkicks, rf_same, rf_standing, rf_end1, rf_end2, rf_zero_volt;
@@ -448,6 +453,51 @@ EH_HD double eh_ypick(const eh_ypow &w, int j) {
   }
 }
 
+/* computeCSBENDFields for expansion order EO1-1 without skew terms, as the
+ * reference evaluates it (src/csbend.c:141-166): every (i, j) with i + j < EO1,
+ * Fx taking odd j and Fy even j, in i-outer j-inner order, each term
+ * (coef * x^i) * y^j.  Fully unrolled: no indexed registers, no data-dependent
+ * branches.  cx/cy hold the dense EO1 x EO1 coefficient blocks (row i, column
+ * j).  A term whose coefficient is zero adds a signed zero to a sum that starts
+ * at +0, which never changes a finite sum, so it is skipped: bit t of `live`
+ * says whether term t is non-zero.  The test is the same for every lane (a
+ * scalar branch); the empty volatile asm keeps the compiler from turning it
+ * into per-lane selects that would evaluate every term anyway. */
+template <int EO1>
+EH_HD void eh_csb_fields_dense(const eh_fterm *__restrict__ cx, const eh_fterm *__restrict__ cy, uint32_t live,
+                               double x, double y, double &Fx, double &Fy) {
+  double yp[EO1];
+  yp[0] = 1;
+#pragma unroll
+  for (int j = 1; j < EO1; j++)
+    yp[j] = yp[j - 1] * y;
+  double sumFx = 0, sumFy = 0, xt = 1;
+#pragma unroll
+  for (int i = 0; i < EO1; i++) {
+    /* the row's coefficients are loaded together, ahead of the branches */
+    double c[EO1];
+#pragma unroll
+    for (int j = 0; j < EO1 - i; j++)
+      c[j] = (j & 1) ? cx[i * EO1 + j].coef : cy[i * EO1 + j].coef;
+#pragma unroll
+    for (int j = 0; j < EO1 - i; j++) {
+      const int t = i * EO1 - i * (i - 1) / 2 + j;
+      if (!((live >> t) & 1u))
+        continue;
+#if defined(__HIP_DEVICE_COMPILE__)
+      __asm__ volatile("");
+#endif
+      if (j & 1)
+        sumFx += c[j] * xt * yp[j];
+      else
+        sumFy += c[j] * xt * yp[j];
+    }
+    xt *= x;
+  }
+  Fx = sumFx;
+  Fy = sumFy;
+}
+
 /* computeCSBENDFields, src/csbend.c:99-172.  The reference adds
  * coef * x^i * y^j over all (i, j) in i-outer, j-inner order; terms with a zero
  * coefficient are skipped here, which leaves every finite sum unchanged (adding
@@ -457,6 +507,14 @@ EH_HD void eh_csb_fields(const eh_desc &d, const eh_fterm *__restrict__ terms, d
   if (!d.csb_field) {
     Fx = 0;
     Fy = 1;
+    return;
+  }
+  if (d.csb_dense == 5) {
+    eh_csb_fields_dense<5>(terms + d.csb_dx0, terms + d.csb_dy0, d.csb_live, x, y, Fx, Fy);
+    return;
+  }
+  if (d.csb_dense == 6) {
+    eh_csb_fields_dense<6>(terms + d.csb_dx0, terms + d.csb_dy0, d.csb_live, x, y, Fx, Fy);
     return;
   }
   eh_ypow w;
@@ -485,6 +543,28 @@ EH_HD void eh_csb_fields(const eh_desc &d, const eh_fterm *__restrict__ terms, d
   }
   Fx = sumFx;
   Fy = sumFy;
+}
+
+/* a / b, correctly rounded, for a divisor known ahead through rh = RN(1/b) and
+ * rl = RN(1/b - rh).  q0 = RN(a*rh + RN(a*rl)) is within one ulp of a/b (the
+ * pair carries 1/b to ~2^-105), the remainder r = a - b*q0 is then exact (one
+ * fma), and RN(q0 + r*rh) is the correctly rounded quotient: Markstein's
+ * theorem (P. Markstein, IBM J. Res. Develop. 34 (1990) 111-119) for a
+ * reciprocal within half an ulp and a quotient within one ulp, without
+ * underflow or overflow.  Numerators outside [2^-960, 2^960] in magnitude
+ * (zeros, whose sign the formula would not keep, non-finite values, extremes)
+ * take the ordinary division; the empty volatile asm keeps that division a
+ * branch taken only by such lanes rather than work every lane does. */
+EH_HD double eh_div_recip(double a, double b, double rh, double rl) {
+  if (eh_fabs(a) >= 0x1p-960 && eh_fabs(a) <= 0x1p960) {
+    double q0 = eh_fma(a, rh, a * rl);
+    double r = eh_fma(-q0, b, a);
+    return eh_fma(r, rh, q0);
+  }
+#if defined(__HIP_DEVICE_COMPILE__)
+  __asm__ volatile("");
+#endif
+  return a / b;
 }
 
 /* CSBEND: track_through_csbend (src/csbend.c:1088-1403) for one particle, with
@@ -557,8 +637,14 @@ EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restr
       double ds = d.csb_ds[j];
       double Fx, Fy;
       eh_csb_fields(d, terms, X, Y, Fx, Fy);
-      QX += -ds * (1 + X / rho0) * Fy / rho_actual;
-      QY += ds * (1 + X / rho0) * Fx / rho_actual;
+      if (d.csb_recip) {
+        const double xr = eh_div_recip(X, rho0, d.rho0_rh, d.rho0_rl);
+        QX += eh_div_recip(-ds * (1 + xr) * Fy, rho_actual, d.rhoa_rh, d.rhoa_rl);
+        QY += eh_div_recip(ds * (1 + xr) * Fx, rho_actual, d.rhoa_rh, d.rhoa_rl);
+      } else {
+        QX += -ds * (1 + X / rho0) * Fy / rho_actual;
+        QY += ds * (1 + X / rho0) * Fx / rho_actual;
+      }
     }
   }
   if (!lost) {
@@ -759,6 +845,7 @@ EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pi,
   case 3:
   case 4:
   case 5:
+  case 9: /* KQUSE */
     lost = eh_multipole(p, d, pi);
     break;
   case 6:

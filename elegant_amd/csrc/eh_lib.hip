@@ -231,6 +231,24 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
     integ_order = q->integration_order;
     break;
   }
+  case EH_T_KQUSE: {
+    /* src/multipole.c:923-949 */
+    const EH_KQUSE *q = (const EH_KQUSE *)e->p_elem;
+    nSlices = q->nSlices;
+    n_kicks = q->n_kicks;
+    expand = q->expandHamiltonian;
+    order[0] = 1;
+    KnL[0] = q->k1 * q->length * (1 + q->fse1);
+    drift = q->length;
+    tilt = q->tilt;
+    dx = q->dx;
+    dy = q->dy;
+    dz = q->dz;
+    integ_order = q->integration_order;
+    KnL[1] = q->k2 * q->length * (1 + q->fse2);
+    order[1] = 2;
+    break;
+  }
   default:
     return fail(EH_ERR_UNSUPPORTED, "multipole_tracking2: element type not supported");
   }
@@ -265,6 +283,7 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
   d->adj_order0 = (int32_t)order[0];
   const double kfac = drift * (1 + (e->type == EH_T_KQUAD   ? ((const EH_KQUAD *)e->p_elem)->fse
                                     : e->type == EH_T_KSEXT ? ((const EH_KSEXT *)e->p_elem)->fse
+                                    : e->type == EH_T_KQUSE ? ((const EH_KQUSE *)e->p_elem)->fse1
                                                             : ((const EH_KOCT *)e->p_elem)->fse)) /
                       nSlices;
   for (int s = 0; s < nsub; s++) {
@@ -409,6 +428,22 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
     d->csb_maxj = std::max(d->csb_maxj, t.j);
   for (const eh_fterm &t : fy)
     d->csb_maxj = std::max(d->csb_maxj, t.j);
+  /* dense blocks for the unrolled evaluation of the common orders */
+  d->csb_dense = 0;
+  d->csb_live = 0;
+  if (!hs && (eo1 == 5 || eo1 == 6)) {
+    d->csb_dense = (int32_t)eo1;
+    for (int i = 0, t = 0; i < eo1; i++)
+      for (int j = 0; j < eo1 - i; j++, t++)
+        if (((j & 1) ? Fx[i * 11 + j] : Fy[i * 11 + j]) != 0)
+          d->csb_live |= 1u << t;
+    for (int c = 0; c < 2; c++) {
+      (c ? d->csb_dy0 : d->csb_dx0) = (int32_t)pool.size();
+      for (int i = 0; i < eo1; i++)
+        for (int j = 0; j < eo1; j++)
+          pool.push_back({i, j, (i + j < eo1) ? (c ? Fy[i * 11 + j] : Fx[i * 11 + j]) : 0.0});
+    }
+  }
   d->csb_fx0 = (int32_t)pool.size();
   pool.insert(pool.end(), fx.begin(), fx.end());
   d->csb_fy0 = (int32_t)pool.size();
@@ -440,6 +475,17 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
   d->dyf = cb->dy;
   d->rho0 = rho0;
   d->rho_actual = rho_actual;
+  /* reciprocal pairs for the kick's divisions (see eh_div_recip): valid for
+   * divisors well inside the normal range */
+  d->csb_recip = 0;
+  if (std::fabs(rho0) >= 0x1p-500 && std::fabs(rho0) <= 0x1p500 && std::fabs(rho_actual) >= 0x1p-500 &&
+      std::fabs(rho_actual) <= 0x1p500) {
+    d->csb_recip = 1;
+    d->rho0_rh = 1.0 / rho0;
+    d->rho0_rl = std::fma(-d->rho0_rh, rho0, 1.0) / rho0;
+    d->rhoa_rh = 1.0 / rho_actual;
+    d->rhoa_rl = std::fma(-d->rhoa_rh, rho_actual, 1.0) / rho_actual;
+  }
   d->csb_nslices = cb->nSlices;
   d->nsub = nsub;
   double s = cb->length / cb->nSlices;
@@ -543,6 +589,7 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
   case EH_T_KQUAD:
   case EH_T_KSEXT:
   case EH_T_KOCT:
+  case EH_T_KQUSE:
     return build_multipole(d, e, c);
   case EH_T_CSBEND:
     return build_csbend(d, (const EH_CSBEND *)e->p_elem, c, pool);
@@ -704,6 +751,12 @@ extern "C" long eh_beam_upload(eh_beam *b, double **coord, long n_part) {
 /* ------------------------------------------------------------------------- */
 /* kernels */
 #define EH_BLOCK 256
+/* Occupancy target for the tracking kernels: six wavefronts per SIMD caps
+ * them at 80 VGPRs (512 / 6, in granules of 8).  The compiler's own choice
+ * (~90 VGPRs, five wavefronts) leaves the FP64 pipe idle while wavefronts wait
+ * on their loads and dependent FP64 chains; the few registers it then spills
+ * cost less than the sixth wavefront gains (measured on the MBA ring). */
+#define EH_TRACK_WAVES __attribute__((amdgpu_waves_per_eu(6)))
 
 __device__ __forceinline__ eh_p6 load_p(const eh_soa a, long i) {
   eh_p6 p;
@@ -745,7 +798,7 @@ __device__ __forceinline__ unsigned wave_prefix(bool pred, unsigned *total) {
 /* one element routine (op 0), one limit_amplitudes sweep (op 1) or one
  * removeInvalidParticles sweep (op 2) over the alive particles, in place; losses
  * are marked with (pass, event) */
-__global__ void __launch_bounds__(EH_BLOCK) k_plain(eh_soa a, long n, const eh_desc *__restrict__ desc,
+__global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_plain(eh_soa a, long n, const eh_desc *__restrict__ desc,
                                                      const eh_fterm *__restrict__ terms, int op,
                                                      double xmax, double ymax, double z, double Po, int extrap,
                                                      int32_t pass, int32_t event, unsigned long long *cnt) {
@@ -765,6 +818,7 @@ __global__ void __launch_bounds__(EH_BLOCK) k_plain(eh_soa a, long n, const eh_d
       case EH_T_KQUAD:
       case EH_T_KSEXT:
       case EH_T_KOCT:
+      case EH_T_KQUSE:
         lost = eh_multipole(p, d, 0);
         break;
       case EH_T_CSBEND:
@@ -897,7 +951,7 @@ __device__ __forceinline__ void wave_store(const eh_p6 &p, bool in, bool alive, 
 
 /* fused tracking of one lattice: each thread takes one particle through the span,
  * then survivors are compacted into b and lost particles appended to the graveyard */
-__global__ void __launch_bounds__(EH_BLOCK) k_fused(eh_soa a, long n, eh_soa b, eh_soa g,
+__global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_fused(eh_soa a, long n, eh_soa b, eh_soa g,
                                                      const eh_desc *__restrict__ descs,
                                                      const eh_fterm *__restrict__ terms, int nelem, int p0, int e0,
                                                      int p1, int e1, int32_t event_base, unsigned long long *cnt) {
@@ -953,7 +1007,7 @@ __global__ void k_batch_prep(const unsigned long long *alive_prev, int32_t *aliv
   }
 }
 
-__global__ void __launch_bounds__(EH_BLOCK) k_batch(eh_soa a, eh_soa b, eh_soa g, eh_batch_dev bd,
+__global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_batch(eh_soa a, eh_soa b, eh_soa g, eh_batch_dev bd,
                                                      const eh_desc *__restrict__ pool,
                                                      const eh_fterm *__restrict__ terms, int nelem, int p0, int e0,
                                                      int p1, int e1, int32_t event_base, unsigned long long *cnt) {
@@ -964,7 +1018,10 @@ __global__ void __launch_bounds__(EH_BLOCK) k_batch(eh_soa a, eh_soa b, eh_soa g
     int32_t w = 0;
     if (lane == 0)
       w = atomicAdd(bd.queue, 1);
-    w = __shfl(w, 0, 64);
+    /* the work item is wavefront-uniform: keep it (and all that follows from
+     * it: variant, descriptor indices) in scalar registers so that descriptor
+     * fields are read with scalar loads */
+    w = __builtin_amdgcn_readfirstlane(__shfl(w, 0, 64));
     if (w >= total)
       break;
     int lo = 0, hi = bd.nvar; /* wave_start[lo] <= w < wave_start[hi] */
@@ -993,7 +1050,7 @@ __global__ void __launch_bounds__(EH_BLOCK) k_batch(eh_soa a, eh_soa b, eh_soa g
     while (pp < p1 || (pp == p1 && ee < e1)) {
       if (__ballot(alive) == 0)
         break;
-      const eh_desc *d = pool + idx[ee];
+      const eh_desc *d = pool + __builtin_amdgcn_readfirstlane(idx[ee]);
       int pi = pp > 0 ? 1 : 0;
       if (alive) {
         units += d->counted;
@@ -1499,6 +1556,7 @@ static long track_plain(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       case EH_T_KQUAD:
       case EH_T_KSEXT:
       case EH_T_KOCT:
+      case EH_T_KQUSE:
         nLeft = eh_multipole_tracking2(b, nToTrack, e, 0.0, *Pc, last_z, maxamp, -1);
         break;
       case EH_T_CSBEND:
@@ -1846,6 +1904,8 @@ static size_t elem_struct_size(int32_t type) {
     return sizeof(EH_KSEXT);
   case EH_T_KOCT:
     return sizeof(EH_KOCT);
+  case EH_T_KQUSE:
+    return sizeof(EH_KQUSE);
   case EH_T_CSBEND:
     return sizeof(EH_CSBEND);
   case EH_T_RFCA:
@@ -2139,36 +2199,17 @@ static long batch_sync_counts(eh_batch *b) {
   return alive;
 }
 
-extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, long ne, long n_passes,
-                               double *P_central, const eh_batch_tune *tune, eh_track_stats *stats) {
-  if (!b || !beamlines || ne < 1 || n_passes < 0 || !P_central)
-    return fail(EH_ERR_ARG, "eh_batch_track: bad arguments");
-  eh_batch_tune t = {256, (int32_t)ne, 8, 1};
-  if (tune)
-    t = *tune;
-  if (t.block != 64 && t.block != 128 && t.block != 256)
-    return fail(EH_ERR_ARG, "eh_batch_track: block must be 64, 128 or 256");
-  if (t.span_elements < 1)
-    t.span_elements = (int32_t)ne;
-  if (t.waves_per_cu < 1)
-    t.waves_per_cu = 8;
-  for (long v = 1; v < b->nvar; v++)
-    for (long e = 0; e < ne; e++)
-      if (beamlines[v][e].type != beamlines[0][e].type)
-        return fail(EH_ERR_ARG, "variant %ld element %ld: types differ from variant 0", v, e);
-  double Pc = *P_central;
-  /* descriptors, deduplicated across variants */
-  std::vector<eh_desc> pool;
-  std::vector<eh_fterm> terms;
-  std::vector<int32_t> idx((size_t)b->nvar * ne);
+/* the descriptor pool of all variants, de-duplicated by content (descriptor
+ * bytes and field terms), with idx[v*ne + e] the descriptor of element e of
+ * variant v.  RF cavities whose fiducial is still unknown get a MARK placeholder
+ * (never shared); their descriptors are built once the fiducial is known. */
+static long batch_tables(EH_ELEMENT_LIST *const *beamlines, long nvar, long ne, double Pc,
+                         std::vector<eh_desc> &pool, std::vector<eh_fterm> &terms, std::vector<int32_t> &idx) {
   std::unordered_map<std::string, int32_t> seen;
-  std::vector<long> rf_elems;
-  for (long e = 0; e < ne; e++)
-    if (beamlines[0][e].type == EH_T_RFCA)
-      rf_elems.push_back(e);
-  const uint64_t hash = beamlines_hash(beamlines, b->nvar, ne, Pc);
-  const bool reuse = b->d_pool && b->tables_ne == ne && b->tables_hash == hash;
-  for (long v = 0; v < b->nvar && !reuse; v++) {
+  pool.clear();
+  terms.clear();
+  idx.assign((size_t)nvar * ne, 0);
+  for (long v = 0; v < nvar; v++) {
     const EH_ELEMENT_LIST *bl = beamlines[v];
     std::vector<double> zs(ne), ze(ne);
     const EH_MAXAMP *last_amp = nullptr;
@@ -2200,8 +2241,9 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
       std::vector<eh_fterm> ft;
       long r;
       if (bl[e].type == EH_T_RFCA && !((EH_RFCA *)bl[e].p_elem)->fiducial_seen) {
-        std::memset(&d, 0, sizeof d); /* placeholder until the fiducial is known */
+        std::memset(&d, 0, sizeof d);
         d.type = EH_T_MARK;
+        d.adj_pk = d.adj_pdx = d.adj_pdy = d.adj_anchor = -1;
         r = EH_OK;
       } else
         r = build_desc(&d, bl + e, c, ft);
@@ -2214,9 +2256,14 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
         idx[(size_t)v * ne + e] = it->second;
       } else {
         if (!ft.empty()) {
-          int32_t shift = (int32_t)terms.size() - d.csb_fx0;
+          /* this element's term offsets are relative to its own list */
+          int32_t shift = (int32_t)terms.size();
           d.csb_fx0 += shift;
           d.csb_fy0 += shift;
+          if (d.csb_dense) {
+            d.csb_dx0 += shift;
+            d.csb_dy0 += shift;
+          }
           terms.insert(terms.end(), ft.begin(), ft.end());
         }
         int32_t id = (int32_t)pool.size();
@@ -2225,6 +2272,42 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
         idx[(size_t)v * ne + e] = id;
       }
     }
+  }
+  return EH_OK;
+}
+
+extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, long ne, long n_passes,
+                               double *P_central, const eh_batch_tune *tune, eh_track_stats *stats) {
+  if (!b || !beamlines || ne < 1 || n_passes < 0 || !P_central)
+    return fail(EH_ERR_ARG, "eh_batch_track: bad arguments");
+  eh_batch_tune t = {256, (int32_t)ne, 8, 1};
+  if (tune)
+    t = *tune;
+  if (t.block != 64 && t.block != 128 && t.block != 256)
+    return fail(EH_ERR_ARG, "eh_batch_track: block must be 64, 128 or 256");
+  if (t.span_elements < 1)
+    t.span_elements = (int32_t)ne;
+  if (t.waves_per_cu < 1)
+    t.waves_per_cu = 8;
+  for (long v = 1; v < b->nvar; v++)
+    for (long e = 0; e < ne; e++)
+      if (beamlines[v][e].type != beamlines[0][e].type)
+        return fail(EH_ERR_ARG, "variant %ld element %ld: types differ from variant 0", v, e);
+  double Pc = *P_central;
+  /* descriptors, deduplicated across variants */
+  std::vector<eh_desc> pool;
+  std::vector<eh_fterm> terms;
+  std::vector<int32_t> idx((size_t)b->nvar * ne);
+  std::vector<long> rf_elems;
+  for (long e = 0; e < ne; e++)
+    if (beamlines[0][e].type == EH_T_RFCA)
+      rf_elems.push_back(e);
+  const uint64_t hash = beamlines_hash(beamlines, b->nvar, ne, Pc);
+  const bool reuse = b->d_pool && b->tables_ne == ne && b->tables_hash == hash;
+  if (!reuse) {
+    long r = batch_tables(beamlines, b->nvar, ne, Pc, pool, terms, idx);
+    if (r < 0)
+      return r;
   }
   auto upload_tables = [&]() -> long {
     b->tables_hash = 0; /* valid again only once this call completes */
@@ -2445,7 +2528,7 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
  * given) receive the entry state of element 0 (slot 0) and of each anchor
  * element (slot d.adj_anchor) for every turn, at index
  * ((t - t0) * n_anchor + slot) * 6 * a_ld + c * a_ld + i */
-__global__ void __launch_bounds__(EH_BLOCK) k_adj_fwd(double *z, long ld, int32_t *alive, long n,
+__global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_adj_fwd(double *z, long ld, int32_t *alive, long n,
                                                        const eh_desc *__restrict__ descs,
                                                        const eh_fterm *__restrict__ terms, int ne, int t0, int t1,
                                                        double *anchors, int n_anchor, long a_ld,
@@ -2555,7 +2638,8 @@ __global__ void __launch_bounds__(EH_ADJ_BLOCK) k_adj_bwd(double *zb, double *la
         break;
       case EH_T_KQUAD:
       case EH_T_KSEXT:
-      case EH_T_KOCT: {
+      case EH_T_KOCT:
+      case EH_T_KQUSE: {
         eh_pgrad g = {0, 0, 0};
         eh_adj_multipole(z, l, d, g);
         if (d.adj_pk >= 0) {
@@ -2778,14 +2862,14 @@ extern "C" long eh_adjoint_forward(eh_adjoint *a, const double *coord, long n, E
       if (e > 0)
         d.adj_anchor = n_anchor++;
     }
-    if ((t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT) && d.expand)
+    if ((t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE) && d.expand)
       return fail(EH_ERR_UNSUPPORTED, "element %ld: adjoint needs EXPAND_HAMILTONIAN=0", e);
     for (int k = 0; k < 3; k++) {
       int32_t slot = param_slots ? param_slots[3 * e + k] : -1;
       if (slot < -1 || slot >= n_params)
         return fail(EH_ERR_ARG, "element %ld: parameter slot %d out of range", e, slot);
-      if (slot >= 0 && !(t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT))
-        return fail(EH_ERR_UNSUPPORTED, "element %ld: parameters only for KQUAD/KSEXT/KOCT", e);
+      if (slot >= 0 && !(t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE))
+        return fail(EH_ERR_UNSUPPORTED, "element %ld: parameters only for KQUAD/KSEXT/KOCT/KQUSE", e);
       (k == 0 ? d.adj_pk : k == 1 ? d.adj_pdx : d.adj_pdy) = slot;
     }
   }

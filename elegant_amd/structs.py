@@ -2,7 +2,7 @@
 
 Field names follow elegant's PARAMETER tables (src/track_data.c), so a lattice
 written with elegant's parameter names maps one-to-one.  Defaults are the
-reference defaults (src/track_data.c:1125, :1169, :1227, :1898, :579, :994).
+reference defaults (src/track_data.c:1125, :1169, :1227, :1898, :579, :994, :3270).
 """
 import ctypes as C
 
@@ -10,9 +10,9 @@ NPROP = 10
 ID_INDEX = 6
 LOSS_PASS_INDEX = 7
 
-T_MARK, T_DRIF, T_EDRIFT, T_KQUAD, T_KSEXT, T_KOCT, T_CSBEND, T_RFCA, T_MAXAMP = range(9)
+T_MARK, T_DRIF, T_EDRIFT, T_KQUAD, T_KSEXT, T_KOCT, T_CSBEND, T_RFCA, T_MAXAMP, T_KQUSE = range(10)
 TYPE_NAMES = {T_MARK: "MARK", T_DRIF: "DRIF", T_EDRIFT: "EDRIFT", T_KQUAD: "KQUAD", T_KSEXT: "KSEXT",
-              T_KOCT: "KOCT", T_CSBEND: "CSBEND", T_RFCA: "RFCA", T_MAXAMP: "MAXAMP"}
+              T_KOCT: "KOCT", T_CSBEND: "CSBEND", T_RFCA: "RFCA", T_MAXAMP: "MAXAMP", T_KQUSE: "KQUSE"}
 TYPE_CODES = {v: k for k, v in TYPE_NAMES.items()}
 
 BEND_EDGE1_EFFECTS = 1
@@ -48,6 +48,11 @@ class EH_KSEXT(C.Structure):
 
 class EH_KOCT(C.Structure):
     _fields_ = [(n, dbl) for n in ("length", "k3", "tilt", "dx", "dy", "dz", "fse")] + \
+               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian")]
+
+
+class EH_KQUSE(C.Structure):
+    _fields_ = [(n, dbl) for n in ("length", "k1", "k2", "tilt", "dx", "dy", "dz", "fse1", "fse2")] + \
                [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian")]
 
 
@@ -87,7 +92,7 @@ class eh_batch_tune(C.Structure):
 
 
 STRUCT_OF = {T_DRIF: EH_DRIFT, T_EDRIFT: EH_EDRIFT, T_KQUAD: EH_KQUAD, T_KSEXT: EH_KSEXT, T_KOCT: EH_KOCT,
-             T_CSBEND: EH_CSBEND, T_RFCA: EH_RFCA, T_MAXAMP: EH_MAXAMP}
+             T_CSBEND: EH_CSBEND, T_RFCA: EH_RFCA, T_MAXAMP: EH_MAXAMP, T_KQUSE: EH_KQUSE}
 
 # reference defaults (src/track_data.c)
 DEFAULTS = {
@@ -95,6 +100,7 @@ DEFAULTS = {
     T_KQUAD: dict(xKickCalibration=1.0, yKickCalibration=1.0, n_kicks=0, nSlices=1, integration_order=4),
     T_KSEXT: dict(xKickCalibration=1.0, yKickCalibration=1.0, n_kicks=0, nSlices=1, integration_order=4),
     T_KOCT: dict(n_kicks=4, nSlices=4, integration_order=4),
+    T_KQUSE: dict(n_kicks=0, nSlices=1, integration_order=4),
     T_CSBEND: dict(fint=0.5, nSlices=4, nonlinear=1, edge1_effects=1, edge2_effects=1, edge_order=1,
                    integration_order=4, etiltSign=1),
     T_RFCA: dict(freq=500.0e6, tReference=-1.0),
@@ -111,6 +117,8 @@ ALIASES = {
               "INTEGRATION_ORDER": "integration_order", "EXPAND_HAMILTONIAN": "expandHamiltonian"},
     T_KOCT: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "INTEGRATION_ORDER": "integration_order",
              "EXPAND_HAMILTONIAN": "expandHamiltonian"},
+    T_KQUSE: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "INTEGRATION_ORDER": "integration_order",
+              "EXPAND_HAMILTONIAN": "expandHamiltonian"},
     T_CSBEND: {"N_KICKS": "nSlices", "N_SLICES": "nSlices", "XKICK": "xKick", "YKICK": "yKick",
                "FSE_DIPOLE": "fseDipole", "FSE_QUADRUPOLE": "fseQuadrupole",
                "EDGE1_EFFECTS": "edge1_effects", "EDGE2_EFFECTS": "edge2_effects", "EDGE_ORDER": "edge_order",

@@ -56,7 +56,8 @@ enum eh_type {
   EH_T_KOCT = 5,
   EH_T_CSBEND = 6,
   EH_T_RFCA = 7,
-  EH_T_MAXAMP = 8
+  EH_T_MAXAMP = 8,
+  EH_T_KQUSE = 9
 };
 
 /* DRIF: src/track.h:1364-1368 (length, order; order 0 means run_setup default_order=2) */
@@ -96,6 +97,15 @@ typedef struct {
   int32_t integration_order;
   int32_t expandHamiltonian;
 } EH_KOCT;
+
+/* KQUSE (quadrupole with a sextupole as secondary multipole): src/track.h:3498-3503,
+ * parameters src/track_data.c:3270-3288, tracking src/multipole.c:923-949 */
+typedef struct {
+  double length, k1, k2, tilt, dx, dy, dz, fse1, fse2;
+  int32_t n_kicks, nSlices;
+  int32_t integration_order;
+  int32_t expandHamiltonian;
+} EH_KQUSE;
 
 /* CSBEND: src/track.h:2462-2502, defaults src/track_data.c:1898.
  * N_KICKS is an alias of N_SLICES (src/track_data.c:1943-1944). */
@@ -194,7 +204,7 @@ void eh_beam_set_pass(eh_beam *beam, long pass);
 /* ---------------- per-element entry points (the plain drop-in path) -------- */
 /* Each returns the number of surviving particles, like the CPU routine. */
 
-/* replaces multipole_tracking2, src/multipole.c:660-677 (KQUAD/KSEXT/KOCT) */
+/* replaces multipole_tracking2, src/multipole.c:660-677 (KQUAD/KSEXT/KOCT/KQUSE) */
 long eh_multipole_tracking2(eh_beam *beam, long n_part, const EH_ELEMENT_LIST *elem, double p_error, double Po,
                             double z_start, const EH_MAXAMP *maxamp, long iSlice);
 
@@ -299,7 +309,7 @@ void eh_adjoint_free(eh_adjoint *adj);
  * beamline, exactly as eh_do_tracking does for the survivors, keeping
  * checkpoints every checkpoint_every turns (1: keep every anchor, no
  * re-tracking).  param_slots[3*e + k] is the gradient slot of element e's
- * strength (k=0: K1/K2/K3 of KQUAD/KSEXT/KOCT), DX (k=1) or DY (k=2), or -1.
+ * strength (k=0: K1/K2/K3/K1 of KQUAD/KSEXT/KOCT/KQUSE), DX (k=1) or DY (k=2), or -1.
  * RF cavity fiducials must already be fixed (track one pass first).  Writes the
  * final coordinates (n x 6) and survival flags; returns the number of survivors. */
 long eh_adjoint_forward(eh_adjoint *adj, const double *coord, long n, EH_ELEMENT_LIST *beamline, long n_elements,

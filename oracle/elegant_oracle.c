@@ -556,6 +556,24 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
     integ_order = q->integration_order;
     break;
   }
+  case EH_T_KQUSE: {
+    /* src/multipole.c:923-949: quadrupole with the sextupole as second multipole */
+    const EH_KQUSE *q = (const EH_KQUSE *)elem->p_elem;
+    n_kicks = q->n_kicks;
+    nSlices = q->nSlices;
+    expand = q->expandHamiltonian;
+    order[0] = 1;
+    KnL[0] = q->k1 * q->length * (1 + q->fse1);
+    drift = q->length;
+    tilt = q->tilt;
+    dx = q->dx;
+    dy = q->dy;
+    dz = q->dz;
+    integ_order = q->integration_order;
+    KnL[1] = q->k2 * q->length * (1 + q->fse2);
+    order[1] = 2;
+    break;
+  }
   default:
     return or_fail("multipole_tracking2: element type not supported");
   }
@@ -1276,6 +1294,7 @@ long or_do_tracking(double **coord, long np, const EH_ELEMENT_LIST *beamline, lo
       case EH_T_KQUAD:
       case EH_T_KSEXT:
       case EH_T_KOCT:
+      case EH_T_KQUSE:
         nLeft = or_multipole_tracking2(coord, nToTrack, eptr, *P_central, last_z, maxamp);
         break;
       case EH_T_CSBEND:
@@ -1335,6 +1354,7 @@ long or_element_flat(double *flat, long np, const EH_ELEMENT_LIST *eptr, double 
   case EH_T_KQUAD:
   case EH_T_KSEXT:
   case EH_T_KOCT:
+  case EH_T_KQUSE:
     r = or_multipole_tracking2(rows, np, eptr, *P_central, z_start, maxamp);
     break;
   case EH_T_CSBEND:

@@ -62,7 +62,8 @@ extern "C" long adjh_gradient(const double *coord, long n, EH_ELEMENT_LIST *bl, 
           break;
         case EH_T_KQUAD:
         case EH_T_KSEXT:
-        case EH_T_KOCT: {
+        case EH_T_KOCT:
+        case EH_T_KQUSE: {
           eh_pgrad pg = {0, 0, 0};
           eh_adj_multipole(z, l, d, pg);
           if (d.adj_pk >= 0)
@@ -98,4 +99,90 @@ extern "C" long adjh_gradient(const double *coord, long n, EH_ELEMENT_LIST *bl, 
     grad_z0[6 * i + 5] = l.dp;
   }
   return n;
+}
+
+/* the de-duplicated descriptor tables of the batched path describe every
+ * variant's element exactly as the single-lattice path does: same fields (term
+ * offsets aside) and the same field values at sample points.  Returns the
+ * number of mismatching (variant, element) pairs. */
+extern "C" long adjh_batch_tables_check(EH_ELEMENT_LIST *const *beamlines, long nvar, long ne, double Pc) {
+  std::vector<eh_desc> pool;
+  std::vector<eh_fterm> terms;
+  std::vector<int32_t> idx;
+  long r = batch_tables(beamlines, nvar, ne, Pc, pool, terms, idx);
+  if (r < 0)
+    return r;
+  long bad = 0;
+  const double pts[4][2] = {{0, 0}, {1e-3, -2e-3}, {-4e-3, 1e-3}, {2.5e-3, 3e-3}};
+  for (long v = 0; v < nvar; v++) {
+    std::vector<eh_desc> hd;
+    std::vector<eh_fterm> own;
+    r = build_line(beamlines[v], ne, Pc, hd, own);
+    if (r < 0)
+      return r;
+    for (long e = 0; e < ne; e++) {
+      eh_desc a = pool[idx[(size_t)v * ne + e]], b = hd[e];
+      if (beamlines[v][e].type == EH_T_RFCA)
+        continue; /* placeholder until the fiducial is known */
+      bool ok = true;
+      if (a.type == EH_T_CSBEND && a.csb_field) {
+        for (int k = 0; k < 4; k++) {
+          double fx1, fy1, fx2, fy2;
+          eh_csb_fields(a, terms.data(), pts[k][0], pts[k][1], fx1, fy1);
+          eh_csb_fields(b, own.data(), pts[k][0], pts[k][1], fx2, fy2);
+          if (!(fx1 == fx2 && fy1 == fy2))
+            ok = false;
+        }
+      }
+      a.csb_fx0 = b.csb_fx0 = a.csb_fy0 = b.csb_fy0 = a.csb_dx0 = b.csb_dx0 = a.csb_dy0 = b.csb_dy0 = 0;
+      if (std::memcmp(&a, &b, sizeof a) != 0)
+        ok = false;
+      bad += !ok;
+    }
+  }
+  return bad;
+}
+
+/* eh_div_recip against IEEE division on n random (a, b) pairs: half with random
+ * operands over wide exponent ranges (significands near 1 and near 2 included),
+ * half with quotients placed next to a rounding midpoint, the hardest case for
+ * a division.  Returns the number of results that differ in any bit. */
+extern "C" long adjh_div_recip_check(long n, uint64_t seed) {
+  uint64_t st = seed | 1;
+  auto rnd = [&]() {
+    st ^= st << 13;
+    st ^= st >> 7;
+    st ^= st << 17;
+    return st;
+  };
+  auto mk = [](uint64_t u, int e, uint64_t m) {
+    return eh_from_bits(((uint64_t)(1023 + e) << 52) | (m & 0xfffffffffffffULL)) * ((u >> 63) ? -1.0 : 1.0);
+  };
+  long bad = 0;
+  for (long k = 0; k < n; k++) {
+    uint64_t ub = rnd(), ua = rnd();
+    uint64_t mb = ub;
+    if (((ub >> 56) & 3) == 1)
+      mb &= 0xff;
+    else if (((ub >> 56) & 3) == 2)
+      mb = ~(mb & 0xff);
+    double b = mk(ub, (int)((ub >> 40) % 81) - 40, mb);
+    double a;
+    if (k & 1) {
+      double q = mk(0, (int)((ua >> 40) % 41) - 20, ua);
+      double mid = q + std::ldexp((ua >> 63) ? 1.0 : -1.0, std::ilogb(q) - 53);
+      a = b * mid;
+      if ((ua & 7) == 0)
+        a = std::nextafter(a, ((ua >> 3) & 1) ? INFINITY : -INFINITY);
+    } else {
+      a = mk(ua, (int)((ua >> 40) % 1801) - 900, ua);
+    }
+    if ((ua & 0xff) == 3)
+      a = (ua & 0x100) ? 0.0 : -0.0;
+    double rh = 1.0 / b, rl = std::fma(-rh, b, 1.0) / b;
+    double q1 = eh_div_recip(a, b, rh, rl), q2 = a / b;
+    if (eh_bits(q1) != eh_bits(q2))
+      bad++;
+  }
+  return bad;
 }

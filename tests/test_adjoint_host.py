@@ -70,6 +70,8 @@ def mix_ring():
         Element("B", "CSBEND", L=0.9, ANGLE=0.06, K1=-0.3, K2=3.0, N_KICKS=8, E1=0.02, E2=0.03, HGAP=0.012,
                 FINT=0.6),
         Element("E", "EDRIFT", L=0.4),
+        Element("QS", "KQUSE", L=0.2, K1=0.4, K2=-12.0, FSE1=1e-3, FSE2=-2e-3, N_KICKS=8, DX=3e-5, DY=1e-5,
+                TILT=0.01),
         Element("O", "KOCT", L=0.2, K3=2000.0, TILT=-0.1),
         Element("QD", "KQUAD", L=0.3, K1=-1.0, N_KICKS=8),
         Element("D2", "DRIF", L=0.3, ORDER=1),
@@ -79,14 +81,23 @@ def mix_ring():
 WEIGHTS = np.array([1e3, 1e4, 1e3, 1e4, 1.0, 1e3])
 
 
-def test_host_forward_equals_oracle():
+@pytest.mark.parametrize("ring", ["mixed", "mba"])
+def test_host_forward_equals_oracle(ring):
     """the maps compiled for the host reproduce the oracle bit for bit"""
-    Po = configs.p_central_from_mev(2000.0)
-    lat = mix_ring()
+    if ring == "mixed":
+        Po = configs.p_central_from_mev(2000.0)
+        lat = mix_ring()
+        rows = configs.fodo_beam(50, seed=8)
+        turns = 4
+    else:
+        Po = configs.p_central_from_mev(3000.0)
+        lat = configs.mba_ring(Po)
+        rows = configs.mba_beam(40, seed=12, scale=25.0)
+        oracle.do_tracking(rows, lat, 1, Po)  # RF fiducial
+        turns = 2
     spec = adjoint.ParamSpec.strengths(lat)
-    rows = configs.fodo_beam(50, seed=8)
-    final, alive, _, _, _ = host_gradient(lat, spec, Po, rows, 4, np.zeros((50, 6)))
-    ref, n, _, _ = oracle.do_tracking(rows, lat, 4, Po)
+    final, alive, _, _, _ = host_gradient(lat, spec, Po, rows, turns, np.zeros((len(rows), 6)))
+    ref, n, _, _ = oracle.do_tracking(rows, lat, turns, Po)
     got = {int(r[6]): r[:6] for r in ref[:n]}
     assert alive.sum() == n
     for i in np.flatnonzero(alive):
@@ -148,3 +159,27 @@ def test_forward_mode_ten_turns_host():
     dL = forward_mode_projection(lat, spec, Po, rows, turns, bx, by, v)
     assert abs(float(np.dot(gp, v)) - dL) <= 1e-7 * abs(dL), (float(np.dot(gp, v)), dL)
     assert err < 1e-11
+
+
+def test_batch_tables_match_single_lattice():
+    """the batched path's de-duplicated descriptor tables (host part of
+    eh_batch_track) give every variant's elements exactly the single-lattice
+    descriptors and field values"""
+    Po = configs.p_central_from_mev(3000.0)
+    lats = configs.mba_variants(3, seed=5)
+    arrs = [l.c_beamline() for l in lats]
+    ptrs = (C.c_void_p * 3)(*[C.cast(a, C.c_void_p) for a, _ in arrs])
+    h = harness()
+    h.adjh_batch_tables_check.argtypes = [C.c_void_p, C.c_long, C.c_long, C.c_double]
+    h.adjh_batch_tables_check.restype = C.c_long
+    assert h.adjh_batch_tables_check(C.cast(ptrs, C.c_void_p), 3, len(lats[0]), Po) == 0
+
+
+def test_division_through_reciprocal_pair():
+    """eh_div_recip (the CSBEND kick's divisions by rho0 and rho_actual) equals
+    IEEE division bit for bit: 4e6 random pairs, half of them with quotients
+    next to a rounding midpoint, plus signed zeros"""
+    h = harness()
+    h.adjh_div_recip_check.argtypes = [C.c_long, C.c_uint64]
+    h.adjh_div_recip_check.restype = C.c_long
+    assert h.adjh_div_recip_check(4000000, 12345) == 0

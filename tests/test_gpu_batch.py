@@ -27,6 +27,7 @@ def test_batch_variants_vs_oracle(tune):
         beams.append(b)
     batch = tracking.Batch(beams)
     n, st = batch.track(lats, 3, Po, block=tune["block"], span_elements=tune["span"], use_graph=tune["graph"])
+    print("survivors per variant:", [batch.n_alive(v) for v in range(nvar)], "total", n, flush=True)
     total_ref = 0
     for v in range(nvar):
         ref, nref, eff, _ = oracle.do_tracking(beams[v], configs.mba_variants(nvar, seed=7)[v], 3, Po)

@@ -71,6 +71,7 @@ ELEMENT_CASES = [
     ("KSEXT", dict(L=0.2, K2=15.0, N_KICKS=4)),
     ("KSEXT", dict(L=0.2, K2=-40.0, K1=0.05, J1=0.02, N_KICKS=4, DX=3e-5, DY=-3e-5)),
     ("KOCT", dict(L=0.2, K3=1000.0)),
+    ("KQUSE", dict(L=0.25, K1=1.4, K2=-35.0, FSE1=2e-3, FSE2=-5e-3, N_KICKS=8, DX=2e-5, DY=-1e-5, TILT=0.03)),
     ("KOCT", dict(L=0.2, K3=-3e4, TILT=math.pi / 2)),
     ("CSBEND", dict(L=1.0, ANGLE=2 * math.pi / 20, N_KICKS=8, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0)),
     ("CSBEND", dict(L=0.8, ANGLE=0.0314, K1=-0.5, N_KICKS=16, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0)),
@@ -98,7 +99,7 @@ def test_single_element(case, ehlib, oracle_mod):
     arr, keep = lat.c_beamline()
     s = lat.structs()[0]
     code = S.TYPE_CODES[etype]
-    if code in (S.T_KQUAD, S.T_KSEXT, S.T_KOCT):
+    if code in (S.T_KQUAD, S.T_KSEXT, S.T_KOCT, S.T_KQUSE):
         n = ehlib.eh_multipole_tracking2(beam.handle, len(rows), arr, 0.0, Po, 1.0, C.byref(amp), -1)
     elif code == S.T_CSBEND:
         n = ehlib.eh_track_through_csbend(beam.handle, len(rows), C.byref(s), 0.0, Po, 1.0, C.byref(amp), -1)

@@ -116,6 +116,51 @@ def test_multipole_vs_textbook(oracle_mod, etype, key, order, K, integ, nk):
         assert got[5] == pts[i, 5]  # no radiation: delta untouched bit for bit
 
 
+def textbook_multi(x, xp, y, yp, delta, terms, L, nslices):
+    """textbook_multipole at 4th order with several (KnL, order) terms summed in one kick"""
+    b = 2 ** (1 / 3)
+    d = [0.5 / (2 - b), (1 - b) / (2 - b) / 2, (1 - b) / (2 - b) / 2, 0.5 / (2 - b)]
+    k = [1 / (2 - b), -b / (2 - b), 1 / (2 - b)]
+    h = L / nslices
+    den = math.sqrt(1 + xp * xp + yp * yp)
+    qx, qy = (1 + delta) * xp / den, (1 + delta) * yp / den
+    for _ in range(nslices):
+        for st in range(4):
+            pz = math.sqrt((1 + delta) ** 2 - qx * qx - qy * qy)
+            x += qx / pz * h * d[st]
+            y += qy / pz * h * d[st]
+            if st < 3:
+                for KnL, order in terms:
+                    z = complex(x, y) ** order / math.factorial(order)
+                    qx -= KnL / nslices * k[st] * z.real
+                    qy += KnL / nslices * k[st] * z.imag
+    pz = math.sqrt((1 + delta) ** 2 - qx * qx - qy * qy)
+    return x, qx / pz, y, qy / pz
+
+
+def test_kquse_vs_textbook(oracle_mod):
+    """KQUSE (src/multipole.c:923-949): quadrupole K1*L*(1+FSE1) plus sextupole
+    K2*L*(1+FSE2) in one integrator; a sign or FSE slip on either term fails here"""
+    rng = np.random.default_rng(77)
+    pts = np.column_stack([rng.normal(0, 3e-3, 40), rng.normal(0, 3e-4, 40), rng.normal(0, 2e-3, 40),
+                           rng.normal(0, 2e-4, 40), np.zeros(40), rng.normal(0, 1e-3, 40)])
+    L, K1, K2, f1, f2 = 0.25, 1.4, -35.0, 2e-3, -5e-3
+    r, n = one(oracle_mod, "KQUSE", dict(L=L, K1=K1, K2=K2, FSE1=f1, FSE2=f2, N_KICKS=8), pts)
+    assert n == 40
+    terms = [(K1 * L * (1 + f1), 1), (K2 * L * (1 + f2), 2)]
+    for i in range(40):
+        x, xp, y, yp = textbook_multi(*pts[i, [0, 1, 2, 3, 5]], terms, L, 2)
+        got = r[r[:, 6] == i + 1][0]
+        assert got[0] == pytest.approx(x, rel=1e-12, abs=1e-16)
+        assert got[1] == pytest.approx(xp, rel=1e-11, abs=1e-16)
+        assert got[2] == pytest.approx(y, rel=1e-12, abs=1e-16)
+        assert got[3] == pytest.approx(yp, rel=1e-11, abs=1e-16)
+    # K2 = 0 reduces to KQUAD with the same K1 and FSE bit for bit
+    r1, _ = one(oracle_mod, "KQUSE", dict(L=L, K1=K1, FSE1=f1, N_KICKS=8), pts)
+    r2, _ = one(oracle_mod, "KQUAD", dict(L=L, K1=K1, FSE=f1, N_KICKS=8), pts)
+    np.testing.assert_allclose(r1, r2, rtol=1e-15, atol=1e-19)
+
+
 def _jacobian(f, z0, eps=1e-6):
     # central differences; eps well above the bend's rounding floor rho0 * 1e-16
     J = np.zeros((4, 4))

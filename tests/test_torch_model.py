@@ -59,6 +59,8 @@ def test_edges_offsets_tilts_steering():
         Element("B", "CSBEND", L=0.9, ANGLE=0.06, K1=-0.3, K2=3.0, N_KICKS=8, E1=0.02, E2=0.03, HGAP=0.012,
                 FINT=0.6),
         Element("E", "EDRIFT", L=0.4),
+        Element("QS", "KQUSE", L=0.2, K1=0.4, K2=-12.0, FSE1=1e-3, FSE2=-2e-3, N_KICKS=8, DX=3e-5, DY=1e-5,
+                TILT=0.01),
         Element("O", "KOCT", L=0.2, K3=2000.0, TILT=-0.1),
         Element("D2", "DRIF", L=0.3, ORDER=1),
     ])

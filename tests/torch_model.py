@@ -99,7 +99,7 @@ class TorchRing:
             return "dx"
         if k == 2:
             return "dy"
-        return {S.T_KQUAD: "k1", S.T_KSEXT: "k2", S.T_KOCT: "k3"}[etype]
+        return {S.T_KQUAD: "k1", S.T_KSEXT: "k2", S.T_KOCT: "k3", S.T_KQUSE: "k1"}[etype]
 
     def _csbend_setup(self, cb):
         if cb.tilt or cb.etilt or cb.dx or cb.dy or cb.dz or cb.use_bn or cb.xKick or cb.yKick:
@@ -140,8 +140,11 @@ class TorchRing:
         K = self._param(theta, i, 0, getattr(st, self._field(etype, 0)))
         dx = self._param(theta, i, 1, st.dx)
         dy = self._param(theta, i, 2, st.dy)
-        order = {S.T_KQUAD: 1, S.T_KSEXT: 2, S.T_KOCT: 3}[etype]
-        terms = [(order, 0, K * st.length * (1 + st.fse))]
+        order = {S.T_KQUAD: 1, S.T_KSEXT: 2, S.T_KOCT: 3, S.T_KQUSE: 1}[etype]
+        fse = st.fse1 if etype == S.T_KQUSE else st.fse
+        terms = [(order, 0, K * st.length * (1 + fse))]
+        if etype == S.T_KQUSE:
+            terms.append((2, 0, st.k2 * st.length * (1 + st.fse2)))
         if etype == S.T_KSEXT:
             if st.k1:
                 terms.append((1, 0, st.k1 * st.length))
@@ -306,7 +309,7 @@ class TorchRing:
                 x, xp, y, yp, s, dp = z
                 L = st.length
                 z = [x + xp * L, xp, y + yp * L, yp, s + L * torch.sqrt(1 + xp * xp + yp * yp), dp]
-            elif t in (S.T_KQUAD, S.T_KSEXT, S.T_KOCT):
+            elif t in (S.T_KQUAD, S.T_KSEXT, S.T_KOCT, S.T_KQUSE):
                 z = self._multipole(z, i, st, t, theta)
             elif t == S.T_CSBEND:
                 if st.angle == 0:
