@@ -28,6 +28,8 @@ CASES = {
     "csbend": [Element("B", "CSBEND", L=0.8, ANGLE=0.04, K1=0.3, K2=2.0, N_KICKS=8, E1=0.02, E2=0.01,
                        HGAP=0.01, DX=1e-4, DZ=2e-4, FSE=1e-3)],
     "drifts": [Element("D", "DRIF", L=1.6), Element("E", "EDRIFT", L=0.7), Element("D1", "DRIF", L=0.4, ORDER=1)],
+    "kquse": [Element("QS", "KQUSE", L=0.25, K1=1.4, K2=-35.0, FSE1=2e-3, FSE2=-5e-3, N_KICKS=8, DX=2e-5,
+                      TILT=0.03)],
 }
 
 

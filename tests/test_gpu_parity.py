@@ -166,6 +166,27 @@ def test_mba_da_grid(mode, ehlib, oracle_mod):
     assert stats["particle_element_turns"] == eff
 
 
+@pytest.mark.skip(reason="the segment-major work-queue variant of k_fused stalled on the GPU (a fused FODO run "
+                         "did not finish in 120 s) and was taken out; the cause is not shown (see DESIGN.md)")
+@pytest.mark.parametrize("seg", [7, 700, 100000])
+def test_fused_work_queue_segments(seg, ehlib, oracle_mod):
+    """fused launches of several turns run as a segment-major work queue; any
+    segment length (here shorter than a turn, not dividing it, and longer than
+    the whole launch) gives the oracle's result bit for bit, losses included"""
+    _say("fused work queue, segment %d" % seg)
+    Po = configs.p_central_from_mev(3000.0)
+    lat = configs.mba_ring(Po)
+    rows = configs.da_grid(48, 40, xmax=0.03, ymax=0.012)
+    tracking.set_fused_segment(seg)
+    try:
+        got, n, ref, nref, stats, eff = _track_both(lat, rows, 3, Po, "fused", tpl=3)
+    finally:
+        tracking.set_fused_segment(0)
+    assert 0 < nref < len(rows)
+    assert_rows_identical(got, ref, n, nref, "segment %d" % seg)
+    assert stats["particle_element_turns"] == eff
+
+
 @pytest.mark.parametrize("mode", ["plain", "fused"])
 def test_golden_fixtures(mode):
     """the GPU reproduces the frozen oracle fixtures of tests/golden bit for bit"""
