@@ -885,8 +885,7 @@ __global__ void __launch_bounds__(EH_BLOCK) k_compact(eh_soa a, long n, eh_soa b
 }
 
 /* one particle through the element sequence from (p0, e0) up to (p1, e1)
- * exclusive, in registers; `units` accumulates the wavefront's particle-element
- * count (the same value in every lane).  Events are event_base + ((p - p0) * nelem + e) * 3 +
+ * exclusive, in registers; `units` counts the lane's particle-element units.  Events are event_base + ((p - p0) * nelem + e) * 3 +
  * stage, in the order the reference compacts.  The loop is wavefront-uniform (the
  * element index is the same in every lane), so descriptors are read with scalar
  * loads; the wavefront leaves early once all its lanes are lost. */
@@ -896,13 +895,12 @@ __device__ __forceinline__ void track_span(eh_p6 &p, bool &alive, const eh_desc 
                                            unsigned long long &units) {
   int pp = p0, ee = e0;
   while (pp < p1 || (pp == p1 && ee < e1)) {
-    const unsigned long long live = __ballot(alive);
-    if (live == 0)
+    if (__ballot(alive) == 0)
       break;
     const eh_desc *d = descs + ee;
     int pi = pp > 0 ? 1 : 0;
-    units += (unsigned long long)__popcll(live) * (unsigned)d->counted;
     if (alive) {
+      units += d->counted;
       int st = eh_apply_element(p, *d, pi, terms);
       if (st >= 0) {
         alive = false;
@@ -969,8 +967,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_fused(eh_soa a, lon
   int32_t lpass = 0, levent = -1;
   unsigned long long units = 0;
   track_span(p, alive, descs, terms, nelem, p0, e0, p1, e1, event_base, lpass, levent, units);
-  if (__lane_id() == 0 && units)
-    atomicAdd(&cnt[3], units);
+  wave_add(&cnt[3], units);
   wave_store(p, in, alive, row0, lpass, levent, b, 0, &cnt[0], g, &cnt[1]);
 }
 
