@@ -22,7 +22,7 @@ EXPORTS = [
     "eh_limit_amplitudes", "eh_removeInvalidParticles", "eh_do_tracking", "eh_csbend_field_table",
     "eh_replay_order", "eh_selftest_math", "eh_batch_create", "eh_batch_free", "eh_batch_upload",
     "eh_batch_download", "eh_batch_n_alive", "eh_batch_track", "eh_diag_copy", "eh_adjoint_create",
-    "eh_adjoint_free", "eh_adjoint_forward", "eh_adjoint_backward",
+    "eh_adjoint_free", "eh_adjoint_forward", "eh_adjoint_backward", "eh_adjoint_forward_batch",
 ]
 
 

@@ -124,8 +124,19 @@ def _lib_adj():
         L.eh_adjoint_forward.restype = Lg
         L.eh_adjoint_backward.argtypes = [vp, vp, vp, vp, C.POINTER(AdjointStats)]
         L.eh_adjoint_backward.restype = Lg
+        L.eh_adjoint_forward_batch.argtypes = [vp, vp, vp, Lg, vp, Lg, vp, Lg, Lg, dp, Lg, vp, vp]
+        L.eh_adjoint_forward_batch.restype = Lg
         L._adj_types = True
     return L
+
+
+def fix_rf(lat, coords, p_central):
+    """first-encounter RF fiducials, as tracking sets them: one pass on a copy"""
+    needs = any(isinstance(s, S.EH_RFCA) and not s.fiducial_seen and s.volt != 0 for s in lat.structs())
+    if needs and len(coords):
+        b = tracking.Beam(np.asarray(coords, dtype=np.float64))
+        tracking.track(b, lat, 1, p_central, mode="fused", turns_per_launch=1)
+        b.free()
 
 
 class AdjointTracker:
@@ -146,12 +157,7 @@ class AdjointTracker:
         self.n = 0
 
     def fix_rf(self, coords):
-        """first-encounter RF fiducials, as tracking sets them: one pass on a copy"""
-        needs = any(isinstance(s, S.EH_RFCA) and not s.fiducial_seen and s.volt != 0 for s in self.lat.structs())
-        if needs:
-            b = tracking.Beam(np.asarray(coords, dtype=np.float64))
-            tracking.track(b, self.lat, 1, self.Po, mode="fused", turns_per_launch=1)
-            b.free()
+        fix_rf(self.lat, coords, self.Po)
 
     def forward(self, coords, theta=None):
         """coords (n, 10) or (n, 6); returns final (n, 6) and alive (n,) bool"""
@@ -187,6 +193,108 @@ class AdjointTracker:
         _lib.check(r)
         self.stats = {k: getattr(st, k) for k, _ in AdjointStats._fields_}
         return gp[:len(self.spec)], gz
+
+    def free(self):
+        if self._h:
+            _lib_adj().eh_adjoint_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _rows10(coords):
+    coords = np.asarray(coords, dtype=np.float64)
+    if coords.shape[1] == 6:
+        full = np.zeros((coords.shape[0], S.NPROP))
+        full[:, :6] = coords
+        full[:, 6] = np.arange(1, coords.shape[0] + 1)
+        coords = full
+    return np.ascontiguousarray(coords)
+
+
+class BatchAdjointTracker:
+    """AdjointTracker for several variants of one lattice structure (error seeds,
+    say) that share one ParamSpec: all variants run in one launch per
+    checkpoint segment (eh_adjoint_forward_batch), and the parameter gradient is
+    summed over every variant's particles, i.e. it is the gradient of the sum of
+    the per-variant objectives."""
+
+    def __init__(self, lattices, spec, p_central, n_turns, checkpoint_every=50):
+        self.lats = list(lattices)
+        if not self.lats:
+            raise ValueError("no lattices")
+        types = [e.type for e in self.lats[0].elements]
+        for lat in self.lats[1:]:
+            if [e.type for e in lat.elements] != types:
+                raise ValueError("variants must share the element sequence")
+        self.spec = spec
+        self.Po = p_central
+        self.n_turns = int(n_turns)
+        self.C = int(checkpoint_every)
+        self.slots = spec.slot_array(self.lats[0])
+        L = _lib_adj()
+        self._h = L.eh_adjoint_create()
+        if not self._h:
+            raise _lib.EHError("eh_adjoint_create failed: %s" % L.eh_last_error().decode())
+        self.stats = None
+        self.counts = []
+
+    def forward(self, coords_list, theta=None):
+        """coords_list: one (n_v, 10) or (n_v, 6) array per variant; returns a
+        list of (final (n_v, 6), alive (n_v,) bool)"""
+        rows = [_rows10(c) for c in coords_list]
+        if len(rows) != len(self.lats):
+            raise ValueError("one particle array per variant")
+        if theta is not None:
+            for lat in self.lats:
+                self.spec.apply(lat, theta)
+        for lat, r in zip(self.lats, rows):
+            fix_rf(lat, r, self.Po)
+        self.counts = [len(r) for r in rows]
+        allrows = np.ascontiguousarray(np.concatenate(rows))
+        n = len(allrows)
+        final = np.empty((n, 6))
+        alive = np.empty(n, dtype=np.int32)
+        arrays, keep = [], []
+        for lat in self.lats:
+            a, k = lat.c_beamline()
+            arrays.append(a)
+            keep.append(k)
+        ptrs = (C.c_void_p * len(arrays))(*[C.cast(a, C.c_void_p) for a in arrays])
+        counts = np.array(self.counts, dtype=np.int64)
+        r = _lib_adj().eh_adjoint_forward_batch(self._h, allrows.ctypes.data, counts.ctypes.data, len(self.lats),
+                                               ptrs, len(self.lats[0]), self.slots.ctypes.data, len(self.spec),
+                                               self.n_turns, self.Po, self.C, final.ctypes.data, alive.ctypes.data)
+        _lib.check(r)
+        del keep
+        out, k = [], 0
+        for c in self.counts:
+            out.append((final[k:k + c], alive[k:k + c].astype(bool)))
+            k += c
+        return out
+
+    def backward(self, grad_list):
+        """grad_list: dL/d(final) per variant, (n_v, 6) each; returns (dL/dtheta,
+        [dL/d(initial) per variant])"""
+        g = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 6) for x in grad_list]))
+        n = sum(self.counts)
+        if g.shape[0] != n:
+            raise ValueError("gradient rows do not match the forward pass")
+        gp = np.zeros(max(1, len(self.spec)))
+        gz = np.zeros((n, 6))
+        st = AdjointStats()
+        r = _lib_adj().eh_adjoint_backward(self._h, g.ctypes.data, gp.ctypes.data, gz.ctypes.data, C.byref(st))
+        _lib.check(r)
+        self.stats = {k: getattr(st, k) for k, _ in AdjointStats._fields_}
+        out, k = [], 0
+        for c in self.counts:
+            out.append(gz[k:k + c])
+            k += c
+        return gp[:len(self.spec)], out
 
     def free(self):
         if self._h:

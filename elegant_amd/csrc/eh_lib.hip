@@ -2534,9 +2534,16 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_adj_fwd(double *z, 
                                                        const eh_desc *__restrict__ descs,
                                                        const eh_fterm *__restrict__ terms, int ne, int t0, int t1,
                                                        double *anchors, int n_anchor, long a_ld,
-                                                       unsigned long long *units) {
+                                                       unsigned long long *units, const int32_t *cvar) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   bool in = i < n;
+  /* lattice variants: rows come in 64-row chunks of one variant each (cvar),
+   * whose descriptors are the variant's block of ne; the chunk is the wavefront */
+  if (cvar) {
+    const long chunk = i >> 6;
+    const int v = __builtin_amdgcn_readfirstlane(chunk < (n + 63) / 64 ? cvar[chunk] : 0);
+    descs += (long)v * ne;
+  }
   bool live = in && alive[i];
   eh_p6 p = {0, 0, 0, 0, 0, 0};
   if (live)
@@ -2612,9 +2619,12 @@ __global__ void __launch_bounds__(EH_ADJ_BLOCK) k_adj_bwd(double *zb, double *la
                                                            long a_ld, const eh_desc *__restrict__ descs,
                                                            const eh_fterm *__restrict__ terms, int ne, int t0,
                                                            int t1, int ta0, int n_params, double *gpart,
-                                                           double *errpart) {
+                                                           double *errpart, const int32_t *cvar) {
   extern __shared__ double g_lds[];
   const int lane = threadIdx.x;
+  /* lattice variants: the block (one wavefront) is one 64-entry chunk of one variant */
+  if (cvar)
+    descs += (long)cvar[blockIdx.x] * ne;
   for (int q = lane; q < n_params; q += EH_ADJ_BLOCK)
     g_lds[q] = 0;
   __syncthreads();
@@ -2745,10 +2755,18 @@ struct eh_adjoint {
   double units = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  /* lattice variants (eh_adjoint_forward_batch): variant v's particles occupy
+   * rows [var_first[v], var_first[v] + var_count[v]) of a layout padded to 64
+   * rows per variant; cvar[chunk] is the variant of each 64-row chunk, and the
+   * caller's k-th particle sits at row row_of[k].  Empty for one lattice. */
+  long nvar = 1, n_caller = 0;
+  std::vector<long> var_first, var_count, row_of;
+  std::vector<int32_t> cvar;
+  int32_t *d_cvar = nullptr;
 };
 
 static void adj_release(eh_adjoint *a) {
-  void *ptrs[] = {a->d_z, a->d_alive, a->d_ckpt, a->d_anchor, a->d_desc, a->d_terms, a->d_cnt};
+  void *ptrs[] = {a->d_z, a->d_alive, a->d_ckpt, a->d_anchor, a->d_desc, a->d_terms, a->d_cnt, a->d_cvar};
   for (void *p : ptrs)
     if (p)
       (void)hipFree(p);
@@ -2759,6 +2777,7 @@ static void adj_release(eh_adjoint *a) {
   a->d_desc = nullptr;
   a->d_terms = nullptr;
   a->d_cnt = nullptr;
+  a->d_cvar = nullptr;
   a->anchor_bytes = 0;
   a->have_forward = false;
 }
@@ -2838,79 +2857,96 @@ static long build_line(EH_ELEMENT_LIST *bl, long ne, double Pc, std::vector<eh_d
   return EH_OK;
 }
 
-extern "C" long eh_adjoint_forward(eh_adjoint *a, const double *coord, long n, EH_ELEMENT_LIST *beamline, long ne,
-                                   const int32_t *param_slots, long n_params, long n_turns, double P_central,
-                                   long checkpoint_every, double *final_coord, int32_t *alive_out) {
-  if (!a || !coord || n < 1 || !beamline || ne < 1 || n_turns < 1 || n_params < 0 || checkpoint_every < 1)
-    return fail(EH_ERR_ARG, "eh_adjoint_forward: bad arguments");
-  if (n > 2147483647L)
-    return fail(EH_ERR_ARG, "eh_adjoint_forward: too many particles");
-  if (n_params > 8000)
-    return fail(EH_ERR_ARG, "eh_adjoint_forward: at most 8000 parameters (one LDS slot each)");
-  adj_release(a);
-  long r = build_line(beamline, ne, P_central, a->hd, a->pool);
-  if (r < 0)
-    return r;
+/* descriptors of nvar variants of one beamline structure, variant-major, with
+ * the adjoint's fields set (parameter slots, RF anchor slots); every variant's
+ * field-term offsets are shifted into one shared pool */
+static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long nvar, long ne,
+                        const int32_t *param_slots, long n_params, double Pc, const long *counts) {
+  a->hd.clear();
+  a->pool.clear();
   int n_anchor = 1;
-  for (long e = 0; e < ne; e++) {
-    eh_desc &d = a->hd[e];
-    const int t = beamline[e].type;
-    if (t == EH_T_RFCA) {
-      const EH_RFCA *rf = (const EH_RFCA *)beamline[e].p_elem;
-      if (!rf->fiducial_seen && rf->volt != 0)
-        return fail(EH_ERR_ARG, "element %ld: RFCA fiducial unknown; track one pass first to fix it", e);
-      if (!d.rf_zero_volt && !d.rf_matrix)
-        return fail(EH_ERR_UNSUPPORTED, "element %ld: adjoint supports the RFCA matrix method only", e);
-      if (e > 0)
-        d.adj_anchor = n_anchor++;
+  for (long v = 0; v < nvar; v++) {
+    std::vector<eh_desc> hd;
+    std::vector<eh_fterm> pool;
+    long r = build_line(beamlines[v], ne, Pc, hd, pool);
+    if (r < 0)
+      return r;
+    int na = 1;
+    for (long e = 0; e < ne; e++) {
+      eh_desc &d = hd[e];
+      const int t = beamlines[v][e].type;
+      if (t != beamlines[0][e].type)
+        return fail(EH_ERR_ARG, "variant %ld element %ld: types differ from variant 0", v, e);
+      if (t == EH_T_RFCA) {
+        const EH_RFCA *rf = (const EH_RFCA *)beamlines[v][e].p_elem;
+        /* a variant without particles never reads its descriptors */
+        if (!rf->fiducial_seen && rf->volt != 0 && !(counts && counts[v] == 0))
+          return fail(EH_ERR_ARG, "element %ld: RFCA fiducial unknown; track one pass first to fix it", e);
+        if (!d.rf_zero_volt && !d.rf_matrix)
+          return fail(EH_ERR_UNSUPPORTED, "element %ld: adjoint supports the RFCA matrix method only", e);
+        if (e > 0)
+          d.adj_anchor = na++;
+      }
+      if ((t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE) && d.expand)
+        return fail(EH_ERR_UNSUPPORTED, "element %ld: adjoint needs EXPAND_HAMILTONIAN=0", e);
+      for (int k = 0; k < 3; k++) {
+        int32_t slot = param_slots ? param_slots[3 * e + k] : -1;
+        if (slot < -1 || slot >= n_params)
+          return fail(EH_ERR_ARG, "element %ld: parameter slot %d out of range", e, slot);
+        if (slot >= 0 && !(t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE))
+          return fail(EH_ERR_UNSUPPORTED, "element %ld: parameters only for KQUAD/KSEXT/KOCT/KQUSE", e);
+        (k == 0 ? d.adj_pk : k == 1 ? d.adj_pdx : d.adj_pdy) = slot;
+      }
+      if (t == EH_T_CSBEND && !pool.empty()) {
+        const int32_t shift = (int32_t)a->pool.size();
+        d.csb_fx0 += shift;
+        d.csb_fy0 += shift;
+        if (d.csb_dense) {
+          d.csb_dx0 += shift;
+          d.csb_dy0 += shift;
+        }
+      }
     }
-    if ((t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE) && d.expand)
-      return fail(EH_ERR_UNSUPPORTED, "element %ld: adjoint needs EXPAND_HAMILTONIAN=0", e);
-    for (int k = 0; k < 3; k++) {
-      int32_t slot = param_slots ? param_slots[3 * e + k] : -1;
-      if (slot < -1 || slot >= n_params)
-        return fail(EH_ERR_ARG, "element %ld: parameter slot %d out of range", e, slot);
-      if (slot >= 0 && !(t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE))
-        return fail(EH_ERR_UNSUPPORTED, "element %ld: parameters only for KQUAD/KSEXT/KOCT/KQUSE", e);
-      (k == 0 ? d.adj_pk : k == 1 ? d.adj_pdx : d.adj_pdy) = slot;
-    }
+    n_anchor = na;
+    a->hd.insert(a->hd.end(), hd.begin(), hd.end());
+    a->pool.insert(a->pool.end(), pool.begin(), pool.end());
   }
+  a->n_anchor = n_anchor;
+  return EH_OK;
+}
+
+/* the forward pass over n rows already laid out (6 x n, row-major by
+ * coordinate in h); alive0 marks rows to track; a->cvar (if any) the variants */
+static long adj_forward_rows(eh_adjoint *a, const std::vector<double> &h, const std::vector<int32_t> &alive0, long n,
+                             long ne, long n_params, long n_turns, long checkpoint_every) {
   a->n = n;
   a->ne = ne;
   a->n_params = n_params;
   a->n_turns = n_turns;
   a->C = checkpoint_every;
-  a->n_anchor = n_anchor;
   a->store_all = checkpoint_every == 1;
+  const int n_anchor = a->n_anchor;
   const long n_ckpt = (n_turns + a->C - 1) / a->C;
   HIPCHK(hipMalloc(&a->d_z, sizeof(double) * 6 * n));
   HIPCHK(hipMalloc(&a->d_alive, sizeof(int32_t) * n));
   HIPCHK(hipMalloc(&a->d_cnt, sizeof(unsigned long long) * 4));
-  HIPCHK(hipMalloc(&a->d_desc, sizeof(eh_desc) * ne));
+  HIPCHK(hipMalloc(&a->d_desc, sizeof(eh_desc) * a->hd.size()));
   HIPCHK(hipMalloc(&a->d_terms, sizeof(eh_fterm) * (a->pool.size() + 1)));
+  if (!a->cvar.empty()) {
+    HIPCHK(hipMalloc(&a->d_cvar, sizeof(int32_t) * a->cvar.size()));
+    HIPCHK(hipMemcpy(a->d_cvar, a->cvar.data(), sizeof(int32_t) * a->cvar.size(), hipMemcpyHostToDevice));
+  }
   if (a->store_all) {
     a->anchor_bytes = sizeof(double) * 6 * (size_t)n * n_anchor * n_turns;
     HIPCHK(hipMalloc(&a->d_anchor, a->anchor_bytes));
   } else {
     HIPCHK(hipMalloc(&a->d_ckpt, sizeof(double) * 6 * (size_t)n * n_ckpt));
   }
-  HIPCHK(hipMemcpy(a->d_desc, a->hd.data(), sizeof(eh_desc) * ne, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(a->d_desc, a->hd.data(), sizeof(eh_desc) * a->hd.size(), hipMemcpyHostToDevice));
   if (!a->pool.empty())
     HIPCHK(hipMemcpy(a->d_terms, a->pool.data(), sizeof(eh_fterm) * a->pool.size(), hipMemcpyHostToDevice));
-  {
-    std::vector<double> h(6 * (size_t)n);
-    for (long i = 0; i < n; i++)
-      for (int c = 0; c < 6; c++)
-        h[c * (size_t)n + i] = coord[10 * i + c];
-    HIPCHK(hipMemcpy(a->d_z, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
-    std::vector<int32_t> al(n, 1);
-    /* the initial NaN sweep of do_tracking (src/do_tracking.c:478-483) */
-    for (long i = 0; i < n; i++)
-      for (int c = 0; c < 6; c++)
-        if (std::isnan(h[c * (size_t)n + i]) || std::isinf(h[c * (size_t)n + i]))
-          al[i] = 0;
-    HIPCHK(hipMemcpy(a->d_alive, al.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-  }
+  HIPCHK(hipMemcpy(a->d_z, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(a->d_alive, alive0.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(a->d_cnt, 0, sizeof(unsigned long long) * 4));
   a->fwd_ms = 0;
   HIPCHK(hipEventRecord(a->ev0, a->stream));
@@ -2923,10 +2959,10 @@ extern "C" long eh_adjoint_forward(eh_adjoint *a, const double *coord, long n, E
       HIPCHK(hipMemcpyAsync(a->d_ckpt + (size_t)k * 6 * n, a->d_z, sizeof(double) * 6 * n, hipMemcpyDeviceToDevice,
                             a->stream));
     hipLaunchKernelGGL(k_adj_fwd, dim3(grid_for(n)), dim3(EH_BLOCK), 0, a->stream, a->d_z, n, a->d_alive, n,
-                       a->d_desc, a->d_terms, (int)ne, t0, t1, anc, n_anchor, n, a->d_cnt);
+                       a->d_desc, a->d_terms, (int)ne, t0, t1, anc, n_anchor, n, a->d_cnt, a->d_cvar);
     HIPCHK(hipGetLastError());
   }
-  r = adj_timed(a, &a->fwd_ms);
+  long r = adj_timed(a, &a->fwd_ms);
   if (r < 0)
     return r;
   unsigned long long cnt[4];
@@ -2936,6 +2972,50 @@ extern "C" long eh_adjoint_forward(eh_adjoint *a, const double *coord, long n, E
   a->alive.resize(n);
   HIPCHK(hipMemcpy(a->z_final.data(), a->d_z, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(a->alive.data(), a->d_alive, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  a->have_forward = true;
+  return EH_OK;
+}
+
+/* rows of 10 doubles -> 6 x n by coordinate at the given rows; the initial NaN
+ * sweep of do_tracking (src/do_tracking.c:478-483) clears alive */
+static void adj_scatter(const double *coord, long k, long row, long n, std::vector<double> &h,
+                        std::vector<int32_t> &al) {
+  al[row] = 1;
+  for (int c = 0; c < 6; c++) {
+    double x = coord[10 * k + c];
+    h[c * (size_t)n + row] = x;
+    if (std::isnan(x) || std::isinf(x))
+      al[row] = 0;
+  }
+}
+
+extern "C" long eh_adjoint_forward(eh_adjoint *a, const double *coord, long n, EH_ELEMENT_LIST *beamline, long ne,
+                                   const int32_t *param_slots, long n_params, long n_turns, double P_central,
+                                   long checkpoint_every, double *final_coord, int32_t *alive_out) {
+  if (!a || !coord || n < 1 || !beamline || ne < 1 || n_turns < 1 || n_params < 0 || checkpoint_every < 1)
+    return fail(EH_ERR_ARG, "eh_adjoint_forward: bad arguments");
+  if (n > 2147483647L)
+    return fail(EH_ERR_ARG, "eh_adjoint_forward: too many particles");
+  if (n_params > 8000)
+    return fail(EH_ERR_ARG, "eh_adjoint_forward: at most 8000 parameters (one LDS slot each)");
+  adj_release(a);
+  a->nvar = 1;
+  a->n_caller = n;
+  a->var_first.clear();
+  a->var_count.clear();
+  a->row_of.clear();
+  a->cvar.clear();
+  EH_ELEMENT_LIST *const lines[1] = {beamline};
+  long r = adj_prepare(a, lines, 1, ne, param_slots, n_params, P_central, nullptr);
+  if (r < 0)
+    return r;
+  std::vector<double> h(6 * (size_t)n);
+  std::vector<int32_t> al(n, 0);
+  for (long i = 0; i < n; i++)
+    adj_scatter(coord, i, i, n, h, al);
+  r = adj_forward_rows(a, h, al, n, ne, n_params, n_turns, checkpoint_every);
+  if (r < 0)
+    return r;
   long n_alive = 0;
   for (long i = 0; i < n; i++) {
     n_alive += a->alive[i] != 0;
@@ -2945,7 +3025,62 @@ extern "C" long eh_adjoint_forward(eh_adjoint *a, const double *coord, long n, E
     if (alive_out)
       alive_out[i] = a->alive[i];
   }
-  a->have_forward = true;
+  return n_alive;
+}
+
+extern "C" long eh_adjoint_forward_batch(eh_adjoint *a, const double *coord, const long *n_per_variant, long nvar,
+                                         EH_ELEMENT_LIST *const *beamlines, long ne, const int32_t *param_slots,
+                                         long n_params, long n_turns, double P_central, long checkpoint_every,
+                                         double *final_coord, int32_t *alive_out) {
+  if (!a || !coord || !n_per_variant || nvar < 1 || !beamlines || ne < 1 || n_turns < 1 || n_params < 0 ||
+      checkpoint_every < 1)
+    return fail(EH_ERR_ARG, "eh_adjoint_forward_batch: bad arguments");
+  if (n_params > 8000)
+    return fail(EH_ERR_ARG, "eh_adjoint_forward_batch: at most 8000 parameters (one LDS slot each)");
+  adj_release(a);
+  a->nvar = nvar;
+  a->var_first.assign(nvar, 0);
+  a->var_count.assign(nvar, 0);
+  a->cvar.clear();
+  long rows = 0, total = 0;
+  for (long v = 0; v < nvar; v++) {
+    if (n_per_variant[v] < 0)
+      return fail(EH_ERR_ARG, "eh_adjoint_forward_batch: negative particle count");
+    a->var_first[v] = rows;
+    a->var_count[v] = n_per_variant[v];
+    long chunks = (n_per_variant[v] + 63) / 64;
+    for (long c = 0; c < chunks; c++)
+      a->cvar.push_back((int32_t)v);
+    rows += chunks * 64;
+    total += n_per_variant[v];
+  }
+  if (total < 1 || rows > 2147483647L)
+    return fail(EH_ERR_ARG, "eh_adjoint_forward_batch: particle count out of range");
+  a->n_caller = total;
+  long r = adj_prepare(a, beamlines, nvar, ne, param_slots, n_params, P_central, n_per_variant);
+  if (r < 0)
+    return r;
+  a->row_of.assign(total, 0);
+  std::vector<double> h(6 * (size_t)rows, 0.0);
+  std::vector<int32_t> al(rows, 0);
+  for (long v = 0, k = 0; v < nvar; v++)
+    for (long j = 0; j < n_per_variant[v]; j++, k++) {
+      a->row_of[k] = a->var_first[v] + j;
+      adj_scatter(coord, k, a->row_of[k], rows, h, al);
+    }
+  r = adj_forward_rows(a, h, al, rows, ne, n_params, n_turns, checkpoint_every);
+  if (r < 0)
+    return r;
+  long n_alive = 0;
+  for (long k = 0; k < total; k++) {
+    const long i = a->row_of[k];
+    n_alive += a->alive[i] != 0;
+    if (final_coord)
+      for (int c = 0; c < 6; c++)
+        final_coord[6 * k + c] = a->z_final[c * (size_t)rows + i];
+    if (alive_out)
+      alive_out[k] = a->alive[i];
+  }
   return n_alive;
 }
 
@@ -2956,20 +3091,59 @@ extern "C" long eh_adjoint_backward(eh_adjoint *a, const double *grad_final, dou
   if (!a->have_forward)
     return fail(EH_ERR_STATE, "eh_adjoint_backward: no forward pass recorded");
   const long n = a->n, ne = a->ne;
-  std::vector<int32_t> surv;
-  for (long i = 0; i < n; i++)
-    if (a->alive[i])
-      surv.push_back((int32_t)i);
+  /* survivors in row order; with lattice variants, each variant's survivors
+   * padded to whole 64-entry chunks with copies of its first survivor that
+   * carry a zero adjoint (they contribute exactly nothing), so that every
+   * wavefront of the reverse sweep reads one variant's descriptors */
+  std::vector<int32_t> surv, cvar_b;
+  std::vector<char> real;
+  long n_real = 0;
+  if (a->cvar.empty()) {
+    for (long i = 0; i < n; i++)
+      if (a->alive[i])
+        surv.push_back((int32_t)i);
+    real.assign(surv.size(), 1);
+    n_real = (long)surv.size();
+  } else {
+    for (long v = 0; v < a->nvar; v++) {
+      const size_t start = surv.size();
+      for (long j = 0; j < a->var_count[v]; j++) {
+        const long i = a->var_first[v] + j;
+        if (a->alive[i]) {
+          surv.push_back((int32_t)i);
+          real.push_back(1);
+        }
+      }
+      const size_t got = surv.size() - start;
+      if (!got)
+        continue;
+      n_real += (long)got;
+      while ((surv.size() - start) % 64) {
+        surv.push_back(surv[start]);
+        real.push_back(0);
+      }
+      for (size_t c = 0; c < (surv.size() - start) / 64; c++)
+        cvar_b.push_back((int32_t)v);
+    }
+  }
   const long m = (long)surv.size();
   const int np = (int)a->n_params;
   if (stats) {
     std::memset(stats, 0, sizeof *stats);
-    stats->n_survivors = m;
+    stats->n_survivors = n_real;
     stats->forward_ms = a->fwd_ms;
     stats->forward_units = a->units;
   }
   if (grad_initial)
-    std::fill(grad_initial, grad_initial + 6 * n, 0.0);
+    std::fill(grad_initial, grad_initial + 6 * a->n_caller, 0.0);
+  /* the caller's index of each row (identity for one lattice) */
+  std::vector<long> caller_of;
+  if (!a->row_of.empty()) {
+    caller_of.assign(n, -1);
+    for (long k = 0; k < a->n_caller; k++)
+      caller_of[a->row_of[k]] = k;
+  }
+  auto caller = [&](long row) { return caller_of.empty() ? row : caller_of[row]; };
   if (m == 0) {
     if (np)
       std::fill(grad_params, grad_params + np, 0.0);
@@ -2979,9 +3153,9 @@ extern "C" long eh_adjoint_backward(eh_adjoint *a, const double *grad_final, dou
   const int npl = np > 0 ? np : 1;
   double *d_zb = nullptr, *d_lam = nullptr, *d_gpart = nullptr, *d_err = nullptr, *d_grad = nullptr,
          *d_seg = nullptr;
-  int32_t *d_surv = nullptr, *d_ident = nullptr;
+  int32_t *d_surv = nullptr, *d_ident = nullptr, *d_cvar_b = nullptr;
   auto cleanup = [&]() {
-    void *ptrs[] = {d_zb, d_lam, d_gpart, d_err, d_grad, d_seg, d_surv, d_ident};
+    void *ptrs[] = {d_zb, d_lam, d_gpart, d_err, d_grad, d_seg, d_surv, d_ident, d_cvar_b};
     for (void *p : ptrs)
       if (p)
         (void)hipFree(p);
@@ -3005,7 +3179,14 @@ extern "C" long eh_adjoint_backward(eh_adjoint *a, const double *grad_final, dou
     }
     for (long j = 0; j < m; j++)
       for (int c = 0; c < 6; c++)
-        h[c * (size_t)m + j] = grad_final[6 * (size_t)surv[j] + c];
+        h[c * (size_t)m + j] = real[j] ? grad_final[6 * (size_t)caller(surv[j]) + c] : 0.0;
+    if (!cvar_b.empty()) {
+      if (hipMalloc(&d_cvar_b, sizeof(int32_t) * cvar_b.size()) != hipSuccess ||
+          hipMemcpy(d_cvar_b, cvar_b.data(), sizeof(int32_t) * cvar_b.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        r = fail(EH_ERR_HIP, "eh_adjoint_backward: variant table upload failed");
+        break;
+      }
+    }
     std::vector<int32_t> ident(m);
     for (long j = 0; j < m; j++)
       ident[j] = (int32_t)j;
@@ -3062,7 +3243,8 @@ extern "C" long eh_adjoint_backward(eh_adjoint *a, const double *grad_final, dou
           hipLaunchKernelGGL(k_adj_gather, dim3(grid_for(m)), dim3(EH_BLOCK), 0, a->stream,
                              a->d_ckpt + (size_t)k * 6 * n, n, d_surv, m, ztmp);
           hipLaunchKernelGGL(k_adj_fwd, dim3(grid_for(m)), dim3(EH_BLOCK), 0, a->stream, ztmp, m, al1, m,
-                             a->d_desc, a->d_terms, (int)ne, t0, t1, d_seg, a->n_anchor, m, a->d_cnt + 1);
+                             a->d_desc, a->d_terms, (int)ne, t0, t1, d_seg, a->n_anchor, m, a->d_cnt + 1,
+                             d_cvar_b);
           e1 = hipGetLastError();
         }
         if (e1 == hipSuccess)
@@ -3083,7 +3265,8 @@ extern "C" long eh_adjoint_backward(eh_adjoint *a, const double *grad_final, dou
       }
       HIPCHK(hipEventRecord(a->ev0, a->stream));
       hipLaunchKernelGGL(k_adj_bwd, dim3((unsigned)nb), dim3(EH_ADJ_BLOCK), lds, a->stream, d_zb, d_lam, m, arow, anc,
-                         a->n_anchor, a_ld, a->d_desc, a->d_terms, (int)ne, t0, t1, ta0, np, d_gpart, d_err);
+                         a->n_anchor, a_ld, a->d_desc, a->d_terms, (int)ne, t0, t1, ta0, np, d_gpart, d_err,
+                         d_cvar_b);
       hipError_t e2 = hipGetLastError();
       if (e2 != hipSuccess) {
         r = fail(EH_ERR_HIP, "k_adj_bwd launch: %s", hipGetErrorString(e2));
@@ -3110,8 +3293,9 @@ extern "C" long eh_adjoint_backward(eh_adjoint *a, const double *grad_final, dou
     }
     if (grad_initial)
       for (long j = 0; j < m; j++)
-        for (int c = 0; c < 6; c++)
-          grad_initial[6 * (size_t)surv[j] + c] = h[c * (size_t)m + j];
+        if (real[j])
+          for (int c = 0; c < 6; c++)
+            grad_initial[6 * (size_t)caller(surv[j]) + c] = h[c * (size_t)m + j];
     if (stats) {
       stats->backward_ms = bwd_ms;
       stats->retrack_ms = rec_ms;
@@ -3124,5 +3308,5 @@ extern "C" long eh_adjoint_backward(eh_adjoint *a, const double *grad_final, dou
     }
   } while (0);
   cleanup();
-  return r < 0 ? r : m;
+  return r < 0 ? r : n_real;
 }

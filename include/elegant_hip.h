@@ -315,6 +315,19 @@ void eh_adjoint_free(eh_adjoint *adj);
 long eh_adjoint_forward(eh_adjoint *adj, const double *coord, long n, EH_ELEMENT_LIST *beamline, long n_elements,
                         const int32_t *param_slots, long n_params, long n_turns, double P_central,
                         long checkpoint_every, double *final_coord, int32_t *alive);
+/* The same for nvar variants of one beamline structure (e.g. error seeds):
+ * coord holds the variants' particles one after another (n_per_variant[v] rows
+ * each), beamlines[v] variant v's elements, param_slots the slots shared by all
+ * variants, so the gradient is summed over every variant's particles (the
+ * gradient of a mean over seeds, up to the caller's weights).  All variants run
+ * in one launch per checkpoint segment, each wavefront on one variant.  Final
+ * coordinates and alive flags come back in the order of coord; backward takes
+ * and returns per-particle arrays in that order too.  No reference
+ * counterpart (a batched form of the entry point above). */
+long eh_adjoint_forward_batch(eh_adjoint *adj, const double *coord, const long *n_per_variant, long nvar,
+                              EH_ELEMENT_LIST *const *beamlines, long n_elements, const int32_t *param_slots,
+                              long n_params, long n_turns, double P_central, long checkpoint_every,
+                              double *final_coord, int32_t *alive);
 /* Given dL/d(final coordinates) (n x 6; rows of lost particles are ignored),
  * writes dL/d(parameters) and, if grad_initial is not NULL, dL/d(initial
  * coordinates) (n x 6, zero for lost particles).  Returns the survivor count. */
