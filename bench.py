@@ -285,6 +285,46 @@ def probe_elements(particles=1000000, turns=4):
     return out
 
 
+def leg_study(args):
+    """error-seed sextupole study (batched adjoint): --study-seeds seeds of the
+    MBA ring (FSE/DX/DY errors), --study-particles each, aperture penalty
+    softplus((r - r_ap)/1e-4) after --study-turns turns, twelve sextupole
+    families (six names split over even and odd cells), checkpoints every
+    --study-ckpt turns, --study-steps gradient steps; every evaluation is one
+    batched forward + backward over all seeds"""
+    from elegant_amd import adjoint, configs, studies
+    Po = configs.p_central_from_mev(3000.0)
+    lats = configs.mba_variants(args.study_seeds, seed=17, p_central=Po)
+    names = ["S0", "S1", "S2", "S3", "S4", "S5"]
+    spec = adjoint.ParamSpec.split_families(lats[0], names, groups=2)
+    beams = [configs.mba_beam(args.study_particles, seed=500 + v, scale=400.0) for v in range(args.study_seeds)]
+    obj = studies.aperture_penalty(r_ap=4e-3, width=1e-4)
+    trk = adjoint.BatchAdjointTracker(lats, spec, Po, n_turns=args.study_turns, checkpoint_every=args.study_ckpt)
+    theta0 = spec.values(lats[0])
+    trk.forward(beams, theta0)  # RF fiducials
+
+    def log_step(h):
+        log("study step %d: objective %.6g, |grad| %.3g, step %.3g" % h)
+
+    t0 = time.perf_counter()
+    theta, hist = studies.gradient_tuning(trk, beams, obj, theta0, steps=args.study_steps, step_size=0.5,
+                                          log=log_step)
+    wall = time.perf_counter() - t0
+    n_eval = args.study_steps + 1
+    # forward particle-element units of one evaluation (every element but the MAXAMP marker)
+    units = args.study_seeds * args.study_particles * args.study_turns * float(len(lats[0]) - 1)
+    per_eval = wall / n_eval
+    trk.free()
+    log("study: %d seeds x %d particles x %d turns, %d evaluations in %.2f s (%.2f s each)" %
+        (args.study_seeds, args.study_particles, args.study_turns, n_eval, wall, per_eval))
+    return {"seconds_per_evaluation": round(per_eval, 3), "evaluations": n_eval, "wall_s": round(wall, 3),
+            "forward_backward_units_per_s": units / per_eval, "objective_first": hist[0][1],
+            "objective_last": hist[-1][1], "seeds": args.study_seeds, "particles_per_seed": args.study_particles,
+            "turns": args.study_turns, "checkpoint_every": args.study_ckpt, "parameters": len(spec),
+            "workload": "MBA ring 3 GeV, error seeds (FSE 1e-3, DX/DY 30 um), 6D Gaussian beams (emittances x400), "
+                        "L = mean softplus((r - 4 mm)/0.1 mm), 12 sextupole families, batched adjoint"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,6 +347,13 @@ def main():
     ap.add_argument("--adj-particles", type=int, default=100000)
     ap.add_argument("--adj-ckpt", type=int, default=50)
     ap.add_argument("--adj-scale", type=float, default=400.0, help="beam emittance scale (amplitude ~ DA)")
+    ap.add_argument("--study", action="store_true",
+                    help="also run the error-seed sextupole study leg (batched adjoint)")
+    ap.add_argument("--study-seeds", type=int, default=64)
+    ap.add_argument("--study-particles", type=int, default=20000)
+    ap.add_argument("--study-turns", type=int, default=20)
+    ap.add_argument("--study-ckpt", type=int, default=10)
+    ap.add_argument("--study-steps", type=int, default=2)
     ap.add_argument("--probe-elements", action="store_true",
                     help="also time rings of one element type each (per-type cost of the fused kernel)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
@@ -385,6 +432,8 @@ def main():
     if not args.no_adjoint:
         out["adjoint"] = leg_adjoint(args)
         out["adjoint_over_forward"] = out["adjoint"]["ratio"]
+    if args.study:
+        out["study"] = leg_study(args)
     print(json.dumps(out))
 
 

@@ -67,6 +67,23 @@ class ParamSpec:
             slots.append(occ)
         return cls(slots, ["%s.%s" % (n, kind) for n in names])
 
+    @classmethod
+    def split_families(cls, lat, names, groups=2, kind="K"):
+        """each named family split into `groups` families by occurrence
+        (occurrence k goes to group k mod groups), e.g. the six sextupole names
+        of a ring split over even and odd cells: twelve knobs"""
+        slots, labels = [], []
+        for nm in names:
+            occ = [(i, kind) for i, e in enumerate(lat.elements) if e.name == nm and e.type in MULTIPOLES]
+            if not occ:
+                raise KeyError("no multipole named %s" % nm)
+            for g in range(groups):
+                part = occ[g::groups]
+                if part:
+                    slots.append(part)
+                    labels.append("%s/%d.%s" % (nm, g, kind))
+        return cls(slots, labels)
+
     def __add__(self, other):
         return ParamSpec(self.slots + other.slots, self.names + other.names)
 

@@ -9,8 +9,16 @@ point in between, and the area is the sum of the surviving cells' areas.
 
 gradient_tuning: an optimize-style loop (src/optimize.c:912, :2002) that steps
 element strengths along the adjoint gradient of a tracking objective instead of
-simplex moves, judging each step with the plain forward tracker.
+simplex moves, judging each step with the plain forward tracker.  It takes
+either a list of AdjointTrackers (one per error seed) or one
+BatchAdjointTracker that runs all seeds in one launch per segment.
+
+aperture_penalty: a smooth loss-of-aperture objective, the mean over particles
+of softplus((r - r_ap) / width) with r = sqrt(x^2 + y^2) after the turns; a
+lost particle counts as a fixed penalty and carries no gradient.
 """
+import math
+
 import numpy as np
 
 from . import configs, tracking
@@ -47,17 +55,54 @@ def dynamic_aperture_grid(lattice, p_central, nx, ny, xmax, ymax, turns, turns_p
             "survivors": int(n), "stats": st}
 
 
+def aperture_penalty(r_ap, width=1e-4, lost=None):
+    """objective(final, alive) -> (L, dL/dfinal) with L the mean over all
+    particles of softplus((r - r_ap) / width) for survivors and `lost` (default
+    softplus(10)) for lost particles"""
+    lost_value = math.log1p(math.exp(10.0)) if lost is None else float(lost)
+
+    def objective(final, alive):
+        n = len(final)
+        G = np.zeros_like(final)
+        if n == 0:
+            return 0.0, G
+        x, y = final[:, 0], final[:, 2]
+        r = np.sqrt(x * x + y * y)
+        u = (r - r_ap) / width
+        sp = np.maximum(u, 0) + np.log1p(np.exp(-np.abs(u)))
+        sig = 0.5 * (1 + np.tanh(0.5 * u))
+        L = (np.sum(sp[alive]) + lost_value * np.count_nonzero(~alive)) / n
+        w = np.where(alive & (r > 0), sig / (width * np.where(r > 0, r, 1.0)) / n, 0.0)
+        G[:, 0] = w * x
+        G[:, 2] = w * y
+        return float(L), G
+
+    return objective
+
+
 def gradient_tuning(trackers, beams, objective, theta0, steps, step_size, judge=None, log=None):
     """Gradient descent on the mean of objective(final, alive) over several
-    AdjointTrackers (e.g. error seeds) sharing one ParamSpec.
+    error seeds sharing one ParamSpec: `trackers` is a list of AdjointTrackers
+    (one per seed, beams a list of particle arrays) or one BatchAdjointTracker
+    (all seeds in one launch per segment).
 
     objective(final, alive) -> (L, dL/dfinal); judge(theta) -> float, the value
     used to accept a step (default: the mean objective of the forward passes).
     Steps that do not improve the main value halve the step size."""
     theta = np.array(theta0, dtype=np.float64)
+    batched = hasattr(trackers, "lats")
 
     def evaluate(th):
         tot, grad = 0.0, np.zeros_like(th)
+        if batched:
+            res = trackers.forward(beams, th)
+            Gs = []
+            for final, alive in res:
+                L, G = objective(final, alive)
+                tot += L
+                Gs.append(G)
+            gp, _ = trackers.backward(Gs)
+            return tot / len(res), gp / len(res)
         for adj, rows in zip(trackers, beams):
             final, alive = adj.forward(rows, th)
             L, G = objective(final, alive)

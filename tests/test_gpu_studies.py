@@ -48,3 +48,35 @@ def test_gradient_tuning_descends():
     vals = [h[1] for h in hist]
     assert all(b <= a for a, b in zip(vals, vals[1:]))
     assert vals[-1] < vals[0]
+
+
+def test_batched_seed_tuning_matches_per_seed_trackers():
+    """the sextupole-family study on error seeds (aperture penalty, twelve
+    split families) gives the same descent with one BatchAdjointTracker as with
+    one AdjointTracker per seed"""
+    print("batched seed tuning", flush=True)
+    Po = configs.p_central_from_mev(3000.0)
+    nseed, npart, turns = 4, 256, 6
+    names = ["S0", "S1", "S2", "S3", "S4", "S5"]
+    beams = [configs.mba_beam(npart, seed=60 + v, scale=400.0) for v in range(nseed)]
+    obj = studies.aperture_penalty(r_ap=4e-3, width=1e-4)
+
+    def run(batched):
+        lats = configs.mba_variants(nseed, seed=9, p_central=Po)
+        spec = adjoint.ParamSpec.split_families(lats[0], names, groups=2)
+        theta0 = spec.values(lats[0])
+        if batched:
+            trk = adjoint.BatchAdjointTracker(lats, spec, Po, n_turns=turns, checkpoint_every=3)
+        else:
+            trk = [adjoint.AdjointTracker(l, spec, Po, n_turns=turns, checkpoint_every=3) for l in lats]
+        theta, hist = studies.gradient_tuning(trk, beams, obj, theta0, steps=3, step_size=0.5)
+        return theta, hist
+
+    tb, hb = run(True)
+    ts, hs = run(False)
+    vb = np.array([h[1] for h in hb])
+    vs = np.array([h[1] for h in hs])
+    print("batched", vb, "per seed", vs, flush=True)
+    assert np.allclose(vb, vs, rtol=1e-10, atol=0)
+    assert np.allclose(tb, ts, rtol=1e-10, atol=1e-14)
+    assert vb[-1] < vb[0]
