@@ -245,12 +245,62 @@ EH_HD void eh_adj_multipole(eh_p6 &z, eh_p6 &l, const eh_desc &d, eh_pgrad &g) {
 /* ---- CSBEND ------------------------------------------------------------------- */
 
 /* field expansion (computeCSBENDFields) with partial derivatives */
+/* the dense-block form for expansion orders 4 and 5 (the blocks and live-term
+ * mask the forward evaluation uses): powers once, then each live term; the
+ * adjoint needs accuracy, not the reference's summation order */
+template <int EO1>
+EH_HD void eh_csb_fields_d_dense(const eh_fterm *__restrict__ cx, const eh_fterm *__restrict__ cy, uint32_t live,
+                                 double x, double y, double &Fx, double &Fy, double &Fx_x, double &Fx_y,
+                                 double &Fy_x, double &Fy_y) {
+  double xp[EO1], yp[EO1];
+  xp[0] = yp[0] = 1;
+#pragma unroll
+  for (int k = 1; k < EO1; k++) {
+    xp[k] = xp[k - 1] * x;
+    yp[k] = yp[k - 1] * y;
+  }
+  double F[2] = {0, 0}, F_x[2] = {0, 0}, F_y[2] = {0, 0};
+#pragma unroll
+  for (int i = 0; i < EO1; i++) {
+#pragma unroll
+    for (int j = 0; j < EO1 - i; j++) {
+      const int t = i * EO1 - i * (i - 1) / 2 + j;
+      if (!((live >> t) & 1u))
+        continue;
+#if defined(__HIP_DEVICE_COMPILE__)
+      __asm__ volatile("");
+#endif
+      const int c = (j & 1) ? 0 : 1;
+      const double k = ((j & 1) ? cx : cy)[i * EO1 + j].coef;
+      F[c] += k * xp[i] * yp[j];
+      if (i > 0)
+        F_x[c] += (k * i) * xp[i - 1] * yp[j];
+      if (j > 0)
+        F_y[c] += (k * j) * xp[i] * yp[j - 1];
+    }
+  }
+  Fx = F[0];
+  Fx_x = F_x[0];
+  Fx_y = F_y[0];
+  Fy = F[1];
+  Fy_x = F_x[1];
+  Fy_y = F_y[1];
+}
+
 EH_HD void eh_csb_fields_d(const eh_desc &d, const eh_fterm *__restrict__ terms, double x,
                                                 double y, double &Fx, double &Fy, double &Fx_x, double &Fx_y,
                                                 double &Fy_x, double &Fy_y) {
   Fx = Fy = Fx_x = Fx_y = Fy_x = Fy_y = 0;
   if (!d.csb_field) {
     Fy = 1;
+    return;
+  }
+  if (d.csb_dense == 5) {
+    eh_csb_fields_d_dense<5>(terms + d.csb_dx0, terms + d.csb_dy0, d.csb_live, x, y, Fx, Fy, Fx_x, Fx_y, Fy_x, Fy_y);
+    return;
+  }
+  if (d.csb_dense == 6) {
+    eh_csb_fields_d_dense<6>(terms + d.csb_dx0, terms + d.csb_dy0, d.csb_live, x, y, Fx, Fy, Fx_x, Fx_y, Fy_x, Fy_y);
     return;
   }
   for (int c = 0; c < 2; c++) {
@@ -324,6 +374,10 @@ EH_HD void eh_adj_csbend(eh_p6 &z, eh_p6 &l, const eh_desc &d,
   double lQX = 0, lQY = 0;
   eh_adj_m2s(QX, QY, DP, lxp, lyp, lQX, lQY, lDP);
   double s = z.s;
+  /* The reverse sweep needs accuracy, not the forward map's roundings (the
+   * recovered states are re-anchored every turn), so divisions become
+   * multiplications by reciprocals formed once per sub-step. */
+  const double irho0 = 1 / rho0, irho_a = 1 / rho_a;
   for (int i = d.csb_nslices - 1; i >= 0; i--) {
     for (int j = d.nsub - 1; j >= 0; j--) {
       if (j < d.nsub - 1) {
@@ -331,37 +385,39 @@ EH_HD void eh_adj_csbend(eh_p6 &z, eh_p6 &l, const eh_desc &d,
         double ds = d.csb_ds[j];
         double Fx, Fy, Fx_x, Fx_y, Fy_x, Fy_y;
         eh_csb_fields_d(d, terms, X, Y, Fx, Fy, Fx_x, Fx_y, Fy_x, Fy_y);
-        double h1 = 1 + X / rho0;
-        double c = ds / rho_a;
-        lX += c * (-lQX * (Fy / rho0 + h1 * Fy_x) + lQY * (Fx / rho0 + h1 * Fx_x));
+        double h1 = 1 + X * irho0;
+        double c = ds * irho_a;
+        lX += c * (-lQX * (Fy * irho0 + h1 * Fy_x) + lQY * (Fx * irho0 + h1 * Fx_x));
         lY += c * (-lQX * h1 * Fy_y + lQY * h1 * Fx_y);
-        QX -= -ds * (1 + X / rho0) * Fy / rho_a;
-        QY -= ds * (1 + X / rho0) * Fx / rho_a;
+        QX += c * h1 * Fy;
+        QY -= c * h1 * Fx;
       }
       /* exact drift through angle th: undo (the same flow through -th), then pull back */
       const double th = d.csb_dshr[j];
       const double sth = d.adj_sin_dshr[j];
       double f = eh_sqrt(eh_sqr(1 + DP) - eh_sqr(QY));
-      double sps = QX / f;
+      const double rf = 1 / f;
+      double sps = QX * rf;
       double psi = eh_asin(sps);
       double cps = eh_cos(psi);
       double sph, cph;
       eh_sincos(psi - th, &sph, &cph);
-      double R = (rho0 + X) * cps / cph; /* rho0 + X at the sub-step input */
-      double F = R * sth / (f * cps);    /* Y and path increment per unit QY, (1+DP) */
+      const double rcps = 1 / cps, rcph = 1 / cph;
+      double R = (rho0 + X) * cps * rcph; /* rho0 + X at the sub-step input */
+      double F = R * sth * rf * rcps;     /* Y and path increment per unit QY, (1+DP) */
       double QXin = f * sph;
       X = R - rho0;
       Y = Y - QY * F;
       s = s - (1 + DP) * F;
       /* adjoint at the input (X, QXin, QY, DP) */
       double lF = QY * lY + (1 + DP) * ls;
-      double lR = lX * (cph / cps) + lF * sth / (f * cps);
-      double tps = sps / cps, tph = sph / cph;
-      double lphi = lQX * f * cps + lX * R * sth / (cps * cps) + lF * F * tps;
-      double lf = lQX * sps - lF * F / f - lphi * tph / f;
-      double nlQX = lphi / (f * cph);
-      lQY = lQY + lY * F - lf * QY / f;
-      lDP = lDP + ls * F + lf * (1 + DP) / f;
+      double lR = lX * (cph * rcps) + lF * sth * rf * rcps;
+      double tps = sps * rcps, tph = sph * rcph;
+      double lphi = lQX * f * cps + lX * R * sth * rcps * rcps + lF * F * tps;
+      double lf = lQX * sps - lF * F * rf - lphi * tph * rf;
+      double nlQX = lphi * rf * rcph;
+      lQY = lQY + lY * F - lf * QY * rf;
+      lDP = lDP + ls * F + lf * (1 + DP) * rf;
       lQX = nlQX;
       lX = lR;
       QX = QXin;
