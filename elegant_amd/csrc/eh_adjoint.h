@@ -542,16 +542,24 @@ EH_HD void eh_rfca_dual(eh_dual v[6], const eh_desc &d) {
 EH_HD void eh_adj_rfca(eh_p6 &z, eh_p6 &l, const eh_p6 &zin, const eh_desc &d) {
   const double in[6] = {zin.x, zin.xp, zin.y, zin.yp, zin.s, zin.dp};
   const double lo[6] = {l.x, l.xp, l.y, l.yp, l.s, l.dp};
-  double li[6];
+  double li[6] = {0, 0, 0, 0, 0, 0};
+  /* one column of the Jacobian at a time, not unrolled: the cavity is one
+   * element per turn, and six inlined copies of the dual-number map would set
+   * the register budget of the whole reverse-sweep kernel */
+#pragma unroll 1
   for (int j = 0; j < 6; j++) {
     eh_dual v[6];
+#pragma unroll
     for (int k = 0; k < 6; k++)
       v[k] = {in[k], k == j ? 1.0 : 0.0};
     eh_rfca_dual(v, d);
     double acc = 0;
+#pragma unroll
     for (int k = 0; k < 6; k++)
       acc += v[k].d * lo[k];
-    li[j] = acc;
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+      li[k] = k == j ? acc : li[k];
   }
   l = {li[0], li[1], li[2], li[3], li[4], li[5]};
   z = zin;
