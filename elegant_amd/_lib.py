@@ -23,6 +23,7 @@ EXPORTS = [
     "eh_replay_order", "eh_selftest_math", "eh_batch_create", "eh_batch_free", "eh_batch_upload",
     "eh_batch_download", "eh_batch_n_alive", "eh_batch_track", "eh_diag_copy", "eh_adjoint_create",
     "eh_adjoint_free", "eh_adjoint_forward", "eh_adjoint_backward", "eh_adjoint_forward_batch",
+    "eh_multipole_tracking", "eh_offset_beam", "eh_rotate_beam",
 ]
 
 
@@ -68,6 +69,12 @@ def load():
     lib.eh_track_through_csbend.restype = L
     lib.eh_simple_rf_cavity.argtypes = [vp, L, C.POINTER(S.EH_RFCA), dp, D]
     lib.eh_simple_rf_cavity.restype = L
+    lib.eh_multipole_tracking.argtypes = [vp, L, C.POINTER(S.EH_MULT), D, D, D]
+    lib.eh_multipole_tracking.restype = L
+    lib.eh_offset_beam.argtypes = [vp, L, C.POINTER(S.EH_MALIGN), D]
+    lib.eh_offset_beam.restype = L
+    lib.eh_rotate_beam.argtypes = [vp, L, C.POINTER(S.EH_ROTATE)]
+    lib.eh_rotate_beam.restype = L
     lib.eh_exactDrift.argtypes = [vp, L, D]
     lib.eh_exactDrift.restype = L
     lib.eh_drift_beam.argtypes = [vp, L, D, L]

@@ -95,6 +95,30 @@ def rfca(p):
     return nk * 60 + 22, nk
 
 
+def mult(p):
+    """multipole_tracking (src/multipole.c:439-658), no BORE, no radiation"""
+    if not (_get(p, "KNL") and _get(p, "FACTOR", default=1.0)):
+        return (11, 0) if _get(p, "L") else (0, 0)
+    order = int(_get(p, "ORDER", default=1))
+    ns = int(_get(p, "N_SLICES", "N_KICKS", default=4))
+    drift = 14 if _get(p, "L") else 0  # x, y and the path length with its square root
+    kick = 2 * order + 3 * (order + 1) + 4 + 9  # powers, terms, qx/qy, momenta->slopes
+    f = 8 + ns * (drift + kick) + drift + 1  # slopes->momenta, slices, final drift, s
+    if _get(p, "DX") or _get(p, "DY") or _get(p, "DZ"):
+        f += 2 * 13
+    if _get(p, "TILT"):
+        f += 2 * 12
+    return f, 0
+
+
+def malign(p):
+    """offset_beam (src/do_tracking.c:3311-3372)"""
+    f = 8 + (6 if _get(p, "DZ") else 0)
+    if _get(p, "DT") or _get(p, "DP") or _get(p, "DE"):
+        f += 12 + (6 if _get(p, "DP") else 0) + (7 if _get(p, "DE") else 0)
+    return f, 0
+
+
 def per_unit(el):
     """(flops, transcendentals) for one particle through one element"""
     t = el.type
@@ -111,6 +135,12 @@ def per_unit(el):
         return csbend(p)
     if t == S.T_RFCA:
         return rfca(p)
+    if t == S.T_MULT:
+        return mult(p)
+    if t == S.T_ROTATE:
+        return 72, 0  # the 6x6 order-1 matrix loop of track_particles
+    if t == S.T_MALIGN:
+        return malign(p)
     raise ValueError(t)
 
 

@@ -25,6 +25,7 @@
 
 #define EH_MAXSUB 8     /* integrator sub-steps (order 6) */
 #define EH_MAXFTERMS 66 /* field terms per component (expansion order 10) */
+#define EH_MULT_MAXORDER 8 /* highest MULT order */
 
 #define EH_DBL_MAX 1.7976931348623157e308
 
@@ -94,6 +95,12 @@ typedef struct {
   double rf_len;  /* length / nKicks */
   double rf_flen; /* full element length (volt == 0 drift) */
   double rf_volt, rf_omega, rf_phase, rf_dtLight, rf_t0, rf_dx, rf_dy, rf_dgamma_same;
+
+  /* ---- MULT, ROTATE, MALIGN ---- */
+  int32_t on_pass;                      /* ROTATE/MALIGN: -1 every pass, -2 none, else the one pass acted on */
+  double mult_coef[EH_MULT_MAXORDER + 1]; /* MULT: expansion coefficients of its order */
+  double mal_dxp, mal_dyp, mal_dt, mal_dp, mal_de; /* MALIGN (offsets in mdx, mdy, mdz) */
+  int32_t mal_energy;                   /* MALIGN: dt, dp or de non-zero */
 
   /* ---- adjoint (reverse-mode) extras ---- */
   int32_t adj_order0;              /* order of the main multipole term, also when its strength is 0 */
@@ -286,8 +293,9 @@ EH_HD bool eh_m2s(double &xp, double &yp, double qx, double qy, double dp, int e
  * src/malign_mat.c:257-267, :580-590, tilt_matrices.c:156-187) and the loss
  * assignments of :1032-1047.  integrate_kick_multipole_ordn, src/multipole.c:1099-1456.
  * Returns true if lost. */
-EH_HD bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
-  /* face 1 */
+/* entry face of a misaligned multipole: offsetBeamCoordinatesForMisalignment
+ * (dx, dy, dz) then rotateBeamCoordinatesForMisalignment (tilt) */
+EH_HD void eh_mface1(eh_p6 &p, const eh_desc &d) {
   if (d.has_offset) {
     p.s += d.mdz * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
     p.x = p.x - d.mdx + d.mdz * p.xp;
@@ -300,6 +308,29 @@ EH_HD bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
     p.xp = xp * d.ncos + yp * d.nsin;
     p.yp = -xp * d.nsin + yp * d.ncos;
   }
+}
+
+/* exit face: rotation by -tilt, then the offset by (-dx, -dy, -dz); every row,
+ * lost ones included */
+EH_HD void eh_mface2(eh_p6 &p, const eh_desc &d) {
+  if (d.has_offset || d.has_tilt) {
+    if (d.has_tilt) {
+      double x0 = p.x, xp0 = p.xp, y0 = p.y, yp0 = p.yp;
+      p.x = x0 * d.xcos + y0 * d.xsin;
+      p.y = -x0 * d.xsin + y0 * d.xcos;
+      p.xp = xp0 * d.xcos + yp0 * d.xsin;
+      p.yp = -xp0 * d.xsin + yp0 * d.xcos;
+    }
+    if (d.has_offset) {
+      p.s += (-d.mdz) * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+      p.x = p.x - (-d.mdx) + (-d.mdz) * p.xp;
+      p.y = p.y - (-d.mdy) + (-d.mdz) * p.yp;
+    }
+  }
+}
+
+EH_HD bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
+  eh_mface1(p, d);
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
   double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, dp = p.dp, qx, qy;
   double dzLoss = 0;
@@ -392,22 +423,198 @@ EH_HD bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
     p.s = d.z_start + dzLoss;
     p.dp = d.Po * (1 + p.dp);
   }
-  /* face 2: applied to every row, lost ones included */
-  if (d.has_offset || d.has_tilt) {
-    if (d.has_tilt) {
-      double x0 = p.x, xp0 = p.xp, y0 = p.y, yp0 = p.yp;
-      p.x = x0 * d.xcos + y0 * d.xsin;
-      p.y = -x0 * d.xsin + y0 * d.xcos;
-      p.xp = xp0 * d.xcos + yp0 * d.xsin;
-      p.yp = -xp0 * d.xsin + yp0 * d.xcos;
-    }
-    if (d.has_offset) {
-      p.s += (-d.mdz) * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
-      p.x = p.x - (-d.mdx) + (-d.mdz) * p.xp;
-      p.y = p.y - (-d.mdy) + (-d.mdz) * p.yp;
-    }
-  }
+  eh_mface2(p, d);
   return lost;
+}
+
+/* ------------------------------------------------------------------------- */
+/* one MULT kick of order N: fillPowerArray for x and y, then the terms
+ * (coef[i]*x^(N-i))*y^i summed in index order, odd i into Fx, even i into Fy,
+ * and qx -= K*Fy, qy += K*Fx (src/multipole.c:578-590) */
+template <int N>
+EH_HD void eh_mult_kick(double &qx, double &qy, double x, double y, const double *coef, double K) {
+  double xp[N + 1], yp[N + 1];
+  xp[0] = 1;
+  yp[0] = 1;
+#pragma unroll
+  for (int i = 1; i <= N; i++) {
+    xp[i] = xp[i - 1] * x;
+    yp[i] = yp[i - 1] * y;
+  }
+  double sum_Fx = 0, sum_Fy = 0;
+#pragma unroll
+  for (int i = 0; i <= N; i++) {
+    if (i & 1)
+      sum_Fx += coef[i] * xp[N - i] * yp[i];
+    else
+      sum_Fy += coef[i] * xp[N - i] * yp[i];
+  }
+  qx -= K * sum_Fy;
+  qy += K * sum_Fx;
+}
+
+EH_HD void eh_mult_kick_n(int order, double &qx, double &qy, double x, double y, const double *coef, double K) {
+  switch (order) {
+  case 0:
+    eh_mult_kick<0>(qx, qy, x, y, coef, K);
+    break;
+  case 1:
+    eh_mult_kick<1>(qx, qy, x, y, coef, K);
+    break;
+  case 2:
+    eh_mult_kick<2>(qx, qy, x, y, coef, K);
+    break;
+  case 3:
+    eh_mult_kick<3>(qx, qy, x, y, coef, K);
+    break;
+  case 4:
+    eh_mult_kick<4>(qx, qy, x, y, coef, K);
+    break;
+  case 5:
+    eh_mult_kick<5>(qx, qy, x, y, coef, K);
+    break;
+  case 6:
+    eh_mult_kick<6>(qx, qy, x, y, coef, K);
+    break;
+  case 7:
+    eh_mult_kick<7>(qx, qy, x, y, coef, K);
+    break;
+  default:
+    eh_mult_kick<8>(qx, qy, x, y, coef, K);
+    break;
+  }
+}
+
+/* the coordinate and slope limits of multipole_tracking (coordLimit 10 m,
+ * slopeLimit 1), NaN included */
+EH_HD bool eh_mult_out(double x, double xp, double y, double yp) {
+  return eh_isnan(x) || eh_isnan(xp) || eh_isnan(y) || eh_isnan(yp) || eh_fabs(x) > 10.0 || eh_fabs(y) > 10.0 ||
+         eh_fabs(xp) > 1.0 || eh_fabs(yp) > 1.0;
+}
+
+/* MULT: multipole_tracking (src/multipole.c:439-658) for one particle, without
+ * BORE/BTIPL and without radiation.  n_slices kicks of strength kick[0][0]
+ * (KnL*FACTOR/N_SLICES) between drifts of mdrift (L/N_SLICES/2, doubled between
+ * kicks).  A particle outside the limits on entry keeps its coordinates; one
+ * lost in the kicks keeps the values reached; both get s = z_start and
+ * delta = Po*(1+delta).  Zero strength is an exact drift, set up by the host.
+ * Returns true if lost. */
+EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d) {
+  eh_mface1(p, d);
+  const double drift = d.mdrift;
+  double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, dp = p.dp, qx, qy;
+  bool lost;
+  if (eh_mult_out(x, xp, y, yp)) {
+    lost = true;
+  } else {
+    if (d.expand) {
+      qx = (1 + dp) * xp;
+      qy = (1 + dp) * yp;
+    } else {
+      double denom = eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
+      qx = (1 + dp) * xp / denom;
+      qy = (1 + dp) * yp / denom;
+    }
+    bool is_lost = false;
+    for (int ik = 0; ik < d.nslices; ik++) {
+      if (drift) {
+        x += xp * drift * (ik ? 2 : 1);
+        y += yp * drift * (ik ? 2 : 1);
+        if (d.expand)
+          s += (ik ? 2 : 1) * drift * (1 + (eh_sqr(xp) + eh_sqr(yp)) / 2);
+        else
+          s += (ik ? 2 : 1) * drift * eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
+      }
+      eh_mult_kick_n(d.mp_order[0], qx, qy, x, y, d.mult_coef, d.kick[0][0]);
+      if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
+        is_lost = true;
+        break;
+      }
+    }
+    if (drift && !is_lost) {
+      x += xp * drift;
+      y += yp * drift;
+      if (d.expand)
+        s += drift * (1 + (eh_sqr(xp) + eh_sqr(yp)) / 2);
+      else
+        s += drift * eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
+    }
+    p.x = x;
+    p.xp = xp;
+    p.y = y;
+    p.yp = yp;
+    p.dp = dp;
+    p.s += s;
+    lost = is_lost || eh_mult_out(x, xp, y, yp);
+  }
+  if (lost) {
+    p.s = d.z_start;
+    p.dp = d.Po * (1 + p.dp);
+  }
+  eh_mface2(p, d);
+  return lost;
+}
+
+/* one row of the order-1 loop of track_particles (src/matrix.c:424-460):
+ * C[i] (zero here) plus R[i][j]*ini[j] for j = 5 down to 0, zero entries
+ * included, so that a non-finite coordinate makes every output NaN as there */
+EH_HD double eh_mrow(double r0, double r1, double r2, double r3, double r4, double r5, const eh_p6 &p) {
+  double sum = 0.0;
+  sum += r5 * p.dp;
+  sum += r4 * p.s;
+  sum += r3 * p.yp;
+  sum += r2 * p.y;
+  sum += r1 * p.xp;
+  sum += r0 * p.x;
+  return sum;
+}
+
+/* ROTATE: rotation_matrix(tilt) (src/tilt_matrices.c:81-117) applied through
+ * track_particles; c and s are its cos and sin entries */
+EH_HD void eh_rotate(eh_p6 &p, double c, double s) {
+  const double z = 0.0, ms = -s;
+  eh_p6 q;
+  q.dp = eh_mrow(z, z, z, z, z, 1.0, p);
+  q.s = eh_mrow(z, z, z, z, 1.0, z, p);
+  q.yp = eh_mrow(z, ms, z, c, z, z, p);
+  q.y = eh_mrow(ms, z, c, z, z, z, p);
+  q.xp = eh_mrow(z, c, z, s, z, z, p);
+  q.x = eh_mrow(c, z, s, z, z, z, p);
+  p = q;
+}
+
+/* MALIGN: offset_beam (src/do_tracking.c:3311-3372) for one particle */
+EH_HD void eh_offset(eh_p6 &p, const eh_desc &d) {
+  const double c_mks = 2.99792458e8;
+  const double Pc = d.Po;
+  double ds;
+  if (d.mdz)
+    ds = d.mdz * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+  else
+    ds = 0;
+  p.x += d.mdx + d.mdz * p.xp;
+  p.xp += d.mal_dxp;
+  p.y += d.mdy + d.mdz * p.yp;
+  p.yp += d.mal_dyp;
+  p.s += ds;
+  if (d.mal_energy) {
+    double pc = Pc * (1 + p.dp);
+    double gamma = eh_sqrt(1 + pc * pc);
+    double beta = pc / gamma;
+    double t = p.s / (beta * c_mks) + d.mal_dt;
+    if (d.mal_dp) {
+      p.dp += d.mal_dp;
+      pc = Pc * (1 + p.dp);
+      beta = pc / eh_sqrt(1 + pc * pc);
+    }
+    if (d.mal_de) {
+      gamma += d.mal_de * gamma;
+      pc = eh_sqrt(gamma * gamma - 1);
+      beta = pc / gamma;
+      p.dp = (pc - Pc) / Pc;
+    }
+    p.s = t * beta * c_mks;
+  }
 }
 
 /* ------------------------------------------------------------------------- */
@@ -832,8 +1039,8 @@ EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d) {
 /* one element followed by do_tracking's post-element sweeps
  * (src/do_tracking.c:2506-2524, :2675-2684).  Returns -1 if the particle
  * survives, else the loss stage. */
-EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pi,
-                                                const eh_fterm *__restrict__ terms) {
+EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms) {
+  const int pi = pass > 0 ? 1 : 0;
   bool lost = false;
   switch (d.type) {
   case 1: /* DRIF */
@@ -853,6 +1060,18 @@ EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pi,
     break;
   case 7:
     lost = eh_rfca(p, d);
+    break;
+  case 10: /* MULT (no slices: zero strength and length, nothing to do) */
+    if (d.nslices)
+      lost = eh_mult(p, d);
+    break;
+  case 11: /* ROTATE, src/do_tracking.c:1836-1852 */
+    if (d.on_pass == -1 || d.on_pass == pass)
+      eh_rotate(p, d.ncos, d.nsin);
+    break;
+  case 12: /* MALIGN, src/do_tracking.c:1853-1857 */
+    if (d.on_pass == -1 || d.on_pass == pass)
+      eh_offset(p, d);
     break;
   default: /* MARK, MAXAMP */
     break;

@@ -98,6 +98,10 @@ static void rot_cs(double angle, double *c, double *s) {
   }
 }
 
+/* rotateBeamCoordinatesForMisalignment leaves the beam alone for angle 0 and
+ * for |angle| within 1e-12 of 2*pi (src/malign_mat.c) */
+static int tilt_on(double angle) { return angle != 0 && !(std::fabs(std::fabs(angle) - 2 * EH_PI) < 1e-12); }
+
 /* ------------------------------------------------------------------------- */
 /* CSBEND field table: computeCSBENDFieldCoefficients (src/csbend.c:175-652),
  * evaluated term by term from the generated Maxwell-recursion table */
@@ -298,7 +302,7 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
   d->mdy = dy;
   d->mdz = dz;
   d->has_offset = (dx || dy || dz);
-  d->has_tilt = tilt != 0;
+  d->has_tilt = tilt_on(tilt);
   rot_cs(tilt + 0.0, &d->ncos, &d->nsin);
   rot_cs(-(tilt + 0.0), &d->xcos, &d->xsin);
   double rt = -tilt;
@@ -548,8 +552,94 @@ static long build_rfca(eh_desc *d, const EH_RFCA *rf) {
   return EH_OK;
 }
 
+/* MULT, src/multipole.c:439-658: the slice strength, the half drift, the
+ * misalignment faces and the expansion coefficients of its order
+ * (expansion_coefficients, src/multipole.h:76-145); a zero strength is the
+ * exact drift the reference takes instead */
+static long build_mult(eh_desc *d, const EH_MULT *m) {
+  if (m->nSlices <= 0)
+    return fail(EH_ERR_UNSUPPORTED, "N_SLICES (or N_KICKS) <=0  in multipole_tracking()");
+  if (m->order < 0)
+    return fail(EH_ERR_UNSUPPORTED, "order < 0 in multipole_tracking()");
+  if (m->order > EH_MULT_MAXORDER)
+    return fail(EH_ERR_UNSUPPORTED, "MULT order above %d not supported", EH_MULT_MAXORDER);
+  if (m->bore)
+    return fail(EH_ERR_UNSUPPORTED, "MULT with BORE/BTIPL not supported");
+  if (m->synch_rad)
+    return fail(EH_ERR_UNSUPPORTED, "MULT with SYNCH_RAD not supported");
+  const long nSlices = m->nSlices;
+  const double drift = m->length / nSlices / 2;
+  const double KnL = m->KnL * m->factor / nSlices;
+  d->length = m->length;
+  if (KnL == 0) {
+    /* exactDrift over the length; with no length the routine returns at once */
+    if (m->length != 0)
+      d->type = EH_T_EDRIFT;
+    return EH_OK;
+  }
+  d->nslices = (int32_t)nSlices;
+  d->expand = m->expandHamiltonian;
+  d->mdrift = drift;
+  d->kick[0][0] = KnL;
+  d->mp_order[0] = m->order;
+  const int n = m->order;
+  for (int i = 0; i <= n; i++) {
+    double fi = 1, fn = 1;
+    for (long k = i; k > 1; k--)
+      fi *= k;
+    for (long k = n - i; k > 1; k--)
+      fn *= k;
+    d->mult_coef[i] = n == 0   ? 1.0
+                      : n == 1 ? 1.0
+                      : n == 2 ? (i == 0 ? 1. / 2 : i == 1 ? 1. : -1. / 2)
+                               : (((i / 2) & 1) ? -1.0 : 1.0) / (fi * fn);
+  }
+  d->mdx = m->dx;
+  d->mdy = m->dy;
+  d->mdz = m->dz;
+  d->has_offset = (m->dx || m->dy || m->dz);
+  d->has_tilt = tilt_on(m->tilt);
+  rot_cs(m->tilt, &d->ncos, &d->nsin);
+  rot_cs(-m->tilt, &d->xcos, &d->xsin);
+  return EH_OK;
+}
+
+/* ROTATE, rotation_matrix (src/tilt_matrices.c:81-117): its cos and sin entries */
+static long build_rotate(eh_desc *d, const EH_ROTATE *r) {
+  const double tilt = r->tilt;
+  d->on_pass = r->on_pass < 0 ? -1 : r->on_pass;
+  if (std::fabs(tilt - EH_PI / 2) < 1e-12) {
+    d->nsin = 1;
+    d->ncos = 0;
+  } else if (std::fabs(tilt - EH_PI) < 1e-12) {
+    d->nsin = 0;
+    d->ncos = -1;
+  } else {
+    d->nsin = eh_sin(tilt);
+    d->ncos = eh_cos(tilt);
+  }
+  return EH_OK;
+}
+
+/* MALIGN, offset_beam (src/do_tracking.c:3311-3372), all particles */
+static long build_malign(eh_desc *d, const EH_MALIGN *m) {
+  if (m->startPID != -1 || m->endPID != -1)
+    return fail(EH_ERR_UNSUPPORTED, "MALIGN particle-ID ranges not supported");
+  d->on_pass = m->on_pass == -1 ? -1 : m->on_pass < -1 ? -2 : m->on_pass;
+  d->mdx = m->dx;
+  d->mdy = m->dy;
+  d->mdz = m->dz;
+  d->mal_dxp = m->dxp;
+  d->mal_dyp = m->dyp;
+  d->mal_dt = m->dt;
+  d->mal_dp = m->dp;
+  d->mal_de = m->de;
+  d->mal_energy = (m->dt || m->dp || m->de);
+  return EH_OK;
+}
+
 static double elem_length(const EH_ELEMENT_LIST *e) {
-  if (e->type == EH_T_MARK || e->type == EH_T_MAXAMP)
+  if (e->type == EH_T_MARK || e->type == EH_T_MAXAMP || e->type == EH_T_ROTATE || e->type == EH_T_MALIGN)
     return 0;
   return *(const double *)e->p_elem;
 }
@@ -596,6 +686,12 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
   case EH_T_RFCA:
     d->length = ((const EH_RFCA *)e->p_elem)->length;
     return build_rfca(d, (const EH_RFCA *)e->p_elem);
+  case EH_T_MULT:
+    return build_mult(d, (const EH_MULT *)e->p_elem);
+  case EH_T_ROTATE:
+    return build_rotate(d, (const EH_ROTATE *)e->p_elem);
+  case EH_T_MALIGN:
+    return build_malign(d, (const EH_MALIGN *)e->p_elem);
   }
   return fail(EH_ERR_UNSUPPORTED, "element type %d not supported", e->type);
 }
@@ -827,6 +923,16 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_plain(eh_soa a, lon
       case EH_T_RFCA:
         lost = eh_rfca(p, d);
         break;
+      case EH_T_MULT:
+        if (d.nslices)
+          lost = eh_mult(p, d);
+        break;
+      case EH_T_ROTATE:
+        eh_rotate(p, d.ncos, d.nsin);
+        break;
+      case EH_T_MALIGN:
+        eh_offset(p, d);
+        break;
       default:
         break;
       }
@@ -898,10 +1004,9 @@ __device__ __forceinline__ void track_span(eh_p6 &p, bool &alive, const eh_desc 
     if (__ballot(alive) == 0)
       break;
     const eh_desc *d = descs + ee;
-    int pi = pp > 0 ? 1 : 0;
     if (alive) {
       units += d->counted;
-      int st = eh_apply_element(p, *d, pi, terms);
+      int st = eh_apply_element(p, *d, pp, terms);
       if (st >= 0) {
         alive = false;
         lpass = pp;
@@ -1052,10 +1157,9 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_batch(eh_soa a, eh_
       if (live == 0)
         break;
       const eh_desc *d = pool + __builtin_amdgcn_readfirstlane(idx[ee]);
-      int pi = pp > 0 ? 1 : 0;
       units += (unsigned long long)__popcll(live) * (unsigned)d->counted;
       if (alive) {
-        int st = eh_apply_element(p, *d, pi, terms);
+        int st = eh_apply_element(p, *d, pp, terms);
         if (st >= 0) {
           alive = false;
           lpass = pp;
@@ -1236,6 +1340,59 @@ extern "C" long eh_track_through_csbend(eh_beam *b, long n_part, const EH_CSBEND
   if (r < 0)
     return r;
   return run_plain(b, &d, &pool, 0, 0, 0, 0, Po, 0);
+}
+
+extern "C" long eh_multipole_tracking(eh_beam *b, long n_part, const EH_MULT *mult, double p_error, double Po,
+                                      double z_start) {
+  (void)p_error;
+  long r = check_np(b, n_part);
+  if (r < 0)
+    return r;
+  if (!mult)
+    return fail(EH_ERR_ARG, "null MULT pointer (multipole_tracking)");
+  eh_desc d;
+  std::memset(&d, 0, sizeof d);
+  d.type = EH_T_MULT;
+  d.Po = Po;
+  d.z_start = z_start;
+  d.z_end = z_start + mult->length;
+  r = build_mult(&d, mult);
+  if (r < 0)
+    return r;
+  if (d.type == EH_T_MULT && d.nslices == 0)
+    return n_part;
+  return run_plain(b, &d, nullptr, 0, 0, 0, 0, Po, 0);
+}
+
+extern "C" long eh_offset_beam(eh_beam *b, long n_part, const EH_MALIGN *offset, double P_central) {
+  long r = check_np(b, n_part);
+  if (r < 0)
+    return r;
+  if (!offset)
+    return fail(EH_ERR_ARG, "null MALIGN pointer (offset_beam)");
+  eh_desc d;
+  std::memset(&d, 0, sizeof d);
+  d.type = EH_T_MALIGN;
+  d.Po = P_central;
+  r = build_malign(&d, offset);
+  if (r < 0)
+    return r;
+  return run_plain(b, &d, nullptr, 0, 0, 0, 0, P_central, 0);
+}
+
+extern "C" long eh_rotate_beam(eh_beam *b, long n_part, const EH_ROTATE *rotate) {
+  long r = check_np(b, n_part);
+  if (r < 0)
+    return r;
+  if (!rotate)
+    return fail(EH_ERR_ARG, "null ROTATE pointer");
+  eh_desc d;
+  std::memset(&d, 0, sizeof d);
+  d.type = EH_T_ROTATE;
+  r = build_rotate(&d, rotate);
+  if (r < 0)
+    return r;
+  return run_plain(b, &d, nullptr, 0, 0, 0, 0, 0, 0);
 }
 
 extern "C" long eh_exactDrift(eh_beam *b, long np, double length) {
@@ -1567,6 +1724,21 @@ static long track_plain(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       case EH_T_RFCA:
         nLeft = eh_simple_rf_cavity(b, nToTrack, (EH_RFCA *)e->p_elem, Pc, z_travel);
         break;
+      case EH_T_MULT:
+        nLeft = eh_multipole_tracking(b, nToTrack, (const EH_MULT *)e->p_elem, 0.0, *Pc, last_z);
+        break;
+      case EH_T_ROTATE: {
+        const EH_ROTATE *r = (const EH_ROTATE *)e->p_elem;
+        if (r->on_pass < 0 || r->on_pass == pass)
+          nLeft = eh_rotate_beam(b, nToTrack, r);
+        break;
+      }
+      case EH_T_MALIGN: {
+        const EH_MALIGN *m = (const EH_MALIGN *)e->p_elem;
+        if (m->on_pass == -1 || m->on_pass == pass)
+          nLeft = eh_offset_beam(b, nToTrack, m, *Pc);
+        break;
+      }
       default:
         return fail(EH_ERR_UNSUPPORTED, "element type %d not supported", e->type);
       }
@@ -1914,6 +2086,12 @@ static size_t elem_struct_size(int32_t type) {
     return sizeof(EH_RFCA);
   case EH_T_MAXAMP:
     return sizeof(EH_MAXAMP);
+  case EH_T_MULT:
+    return sizeof(EH_MULT);
+  case EH_T_ROTATE:
+    return sizeof(EH_ROTATE);
+  case EH_T_MALIGN:
+    return sizeof(EH_MALIGN);
   default:
     return 0;
   }
@@ -2552,7 +2730,6 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_adj_fwd(double *z, 
   for (int t = t0; t < t1; t++) {
     if (__ballot(live) == 0)
       break;
-    const int pi = t > 0 ? 1 : 0;
     for (int e = 0; e < ne; e++) {
       const eh_desc *d = descs + e;
       if (live) {
@@ -2567,7 +2744,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_adj_fwd(double *z, 
           a[5 * a_ld] = p.dp;
         }
         u += d->counted;
-        if (eh_apply_element(p, *d, pi, terms) >= 0)
+        if (eh_apply_element(p, *d, t, terms) >= 0)
           live = false;
       }
     }
@@ -2887,6 +3064,8 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
         if (e > 0)
           d.adj_anchor = na++;
       }
+      if (t == EH_T_MULT || t == EH_T_ROTATE || t == EH_T_MALIGN)
+        return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support MULT, ROTATE or MALIGN", e);
       if ((t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE) && d.expand)
         return fail(EH_ERR_UNSUPPORTED, "element %ld: adjoint needs EXPAND_HAMILTONIAN=0", e);
       for (int k = 0; k < 3; k++) {

@@ -2,7 +2,8 @@
 
 Field names follow elegant's PARAMETER tables (src/track_data.c), so a lattice
 written with elegant's parameter names maps one-to-one.  Defaults are the
-reference defaults (src/track_data.c:1125, :1169, :1227, :1898, :579, :994, :3270).
+reference defaults (src/track_data.c:1125, :1169, :1227, :1898, :579, :994, :3270, :486-502,
+:929-944, :1005-1010).
 """
 import ctypes as C
 
@@ -10,9 +11,11 @@ NPROP = 10
 ID_INDEX = 6
 LOSS_PASS_INDEX = 7
 
-T_MARK, T_DRIF, T_EDRIFT, T_KQUAD, T_KSEXT, T_KOCT, T_CSBEND, T_RFCA, T_MAXAMP, T_KQUSE = range(10)
+(T_MARK, T_DRIF, T_EDRIFT, T_KQUAD, T_KSEXT, T_KOCT, T_CSBEND, T_RFCA, T_MAXAMP, T_KQUSE, T_MULT, T_ROTATE,
+ T_MALIGN) = range(13)
 TYPE_NAMES = {T_MARK: "MARK", T_DRIF: "DRIF", T_EDRIFT: "EDRIFT", T_KQUAD: "KQUAD", T_KSEXT: "KSEXT",
-              T_KOCT: "KOCT", T_CSBEND: "CSBEND", T_RFCA: "RFCA", T_MAXAMP: "MAXAMP", T_KQUSE: "KQUSE"}
+              T_KOCT: "KOCT", T_CSBEND: "CSBEND", T_RFCA: "RFCA", T_MAXAMP: "MAXAMP", T_KQUSE: "KQUSE",
+              T_MULT: "MULT", T_ROTATE: "ROTATE", T_MALIGN: "MALIGN"}
 TYPE_CODES = {v: k for k, v in TYPE_NAMES.items()}
 
 BEND_EDGE1_EFFECTS = 1
@@ -74,6 +77,20 @@ class EH_RFCA(C.Structure):
                [("phase_fiducial", dbl), ("t_fiducial", dbl)]
 
 
+class EH_MULT(C.Structure):
+    _fields_ = [(n, dbl) for n in ("length", "KnL", "tilt", "bore", "BTipL", "dx", "dy", "dz", "factor")] + \
+               [(n, i32) for n in ("order", "synch_rad", "expandHamiltonian", "nSlices")]
+
+
+class EH_ROTATE(C.Structure):
+    _fields_ = [("tilt", dbl)] + [(n, i32) for n in ("on_pass", "exclude_floor", "exclude_optics")]
+
+
+class EH_MALIGN(C.Structure):
+    _fields_ = [(n, dbl) for n in ("dxp", "dyp", "dx", "dy", "dz", "dt", "dp", "de")] + \
+               [(n, i32) for n in ("on_pass", "startPID", "endPID")]
+
+
 class EH_MAXAMP(C.Structure):
     _fields_ = [("x_max", dbl), ("y_max", dbl), ("elliptical", i32), ("exponent", i32)]
 
@@ -92,7 +109,8 @@ class eh_batch_tune(C.Structure):
 
 
 STRUCT_OF = {T_DRIF: EH_DRIFT, T_EDRIFT: EH_EDRIFT, T_KQUAD: EH_KQUAD, T_KSEXT: EH_KSEXT, T_KOCT: EH_KOCT,
-             T_CSBEND: EH_CSBEND, T_RFCA: EH_RFCA, T_MAXAMP: EH_MAXAMP, T_KQUSE: EH_KQUSE}
+             T_CSBEND: EH_CSBEND, T_RFCA: EH_RFCA, T_MAXAMP: EH_MAXAMP, T_KQUSE: EH_KQUSE, T_MULT: EH_MULT,
+             T_ROTATE: EH_ROTATE, T_MALIGN: EH_MALIGN}
 
 # reference defaults (src/track_data.c)
 DEFAULTS = {
@@ -105,6 +123,9 @@ DEFAULTS = {
                    integration_order=4, etiltSign=1),
     T_RFCA: dict(freq=500.0e6, tReference=-1.0),
     T_MAXAMP: dict(exponent=2),
+    T_MULT: dict(factor=1.0, order=1, nSlices=4),
+    T_ROTATE: dict(on_pass=-1),
+    T_MALIGN: dict(on_pass=-1, startPID=-1, endPID=-1),
 }
 
 # elegant parameter names -> struct fields where they differ
@@ -128,6 +149,10 @@ ALIASES = {
     T_RFCA: {"N_KICKS": "nKicks", "CHANGE_P0": "change_p0", "CHANGE_T": "change_t", "END1_FOCUS": "end1Focus",
              "END2_FOCUS": "end2Focus", "STANDING_WAVE": "standingWave", "LINEARIZE": "linearize",
              "LOCK_PHASE": "lockPhase", "T_REFERENCE": "tReference", "VOLT": "volt"},
+    T_MULT: {"N_SLICES": "nSlices", "N_KICKS": "nSlices", "KNL": "KnL", "BTIPL": "BTipL",
+             "SYNCH_RAD": "synch_rad", "EXPAND_HAMILTONIAN": "expandHamiltonian"},
+    T_ROTATE: {"ON_PASS": "on_pass", "EXCLUDE_FLOOR": "exclude_floor", "EXCLUDE_OPTICS": "exclude_optics"},
+    T_MALIGN: {"ON_PASS": "on_pass", "START_PID": "startPID", "END_PID": "endPID"},
     T_MAXAMP: {},
     T_DRIF: {},
     T_EDRIFT: {},

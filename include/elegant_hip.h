@@ -57,7 +57,10 @@ enum eh_type {
   EH_T_CSBEND = 6,
   EH_T_RFCA = 7,
   EH_T_MAXAMP = 8,
-  EH_T_KQUSE = 9
+  EH_T_KQUSE = 9,
+  EH_T_MULT = 10,
+  EH_T_ROTATE = 11,
+  EH_T_MALIGN = 12
 };
 
 /* DRIF: src/track.h:1364-1368 (length, order; order 0 means run_setup default_order=2) */
@@ -106,6 +109,32 @@ typedef struct {
   int32_t integration_order;
   int32_t expandHamiltonian;
 } EH_KQUSE;
+
+/* MULT (thin multipole kicks between drifts): src/track.h:1407-1412, parameters
+ * src/track_data.c:486-502, tracking src/multipole.c:439-658.  BORE/BTIPL and
+ * SYNCH_RAD are not supported (EH_ERR_UNSUPPORTED). */
+typedef struct {
+  double length, KnL, tilt, bore, BTipL, dx, dy, dz, factor;
+  int32_t order, synch_rad, expandHamiltonian;
+  int32_t nSlices;
+} EH_MULT;
+
+/* ROTATE (rotation of the coordinates about s): parameters
+ * src/track_data.c:1005-1010, tracking src/do_tracking.c:1836-1852 through
+ * rotation_matrix (src/tilt_matrices.c:81-117) and the order-1 matrix loop
+ * (src/matrix.c:424-460).  on_pass < 0: every pass. */
+typedef struct {
+  double tilt;
+  int32_t on_pass, exclude_floor, exclude_optics;
+} EH_ROTATE;
+
+/* MALIGN (beam offset): parameters src/track_data.c:929-944, tracking
+ * offset_beam (src/do_tracking.c:3311-3366).  Only the default particle range
+ * (START_PID = END_PID = -1) is supported.  on_pass < 0: every pass. */
+typedef struct {
+  double dxp, dyp, dx, dy, dz, dt, dp, de;
+  int32_t on_pass, startPID, endPID;
+} EH_MALIGN;
 
 /* CSBEND: src/track.h:2462-2502, defaults src/track_data.c:1898.
  * N_KICKS is an alias of N_SLICES (src/track_data.c:1943-1944). */
@@ -203,6 +232,18 @@ void eh_beam_set_pass(eh_beam *beam, long pass);
 
 /* ---------------- per-element entry points (the plain drop-in path) -------- */
 /* Each returns the number of surviving particles, like the CPU routine. */
+
+/* replaces multipole_tracking, src/multipole.c:439-446 (MULT) */
+long eh_multipole_tracking(eh_beam *beam, long n_part, const EH_MULT *mult, double p_error, double Po,
+                           double z_start);
+
+/* replaces offset_beam, src/do_tracking.c:3311-3315 (MALIGN, for the pass given
+ * by eh_beam_set_pass) */
+long eh_offset_beam(eh_beam *beam, long n_part, const EH_MALIGN *offset, double P_central);
+
+/* replaces track_particles with rotation_matrix(tilt) for ROTATE,
+ * src/do_tracking.c:1836-1852 (for the pass given by eh_beam_set_pass) */
+long eh_rotate_beam(eh_beam *beam, long n_part, const EH_ROTATE *rotate);
 
 /* replaces multipole_tracking2, src/multipole.c:660-677 (KQUAD/KSEXT/KOCT/KQUSE) */
 long eh_multipole_tracking2(eh_beam *beam, long n_part, const EH_ELEMENT_LIST *elem, double p_error, double Po,

@@ -626,6 +626,184 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
 }
 
 /* ------------------------------------------------------------------------- */
+/* MULT: multipole_tracking, src/multipole.c:439-658, without BORE/BTIPL and
+ * without radiation: N_SLICES thin kicks KnL*FACTOR/N_SLICES, each between
+ * drifts of L/N_SLICES (half drifts at the ends); the NaN and coordinate/slope
+ * limit losses record coord[4] = z_start; the entry offset/rotation act on the
+ * rows passed in and the exit ones on all of them, lost rows included. */
+long or_multipole_tracking(double **particle, long n_part, const EH_MULT *multipole, double Po, double z_start) {
+  long order = multipole->order, nSlices = multipole->nSlices, i_part, i_kick, i_top, is_lost;
+  double KnL, drift, qx, qy, x, xp, y, yp, s, dp, xpow[16], ypow[16];
+  const long expand = multipole->expandHamiltonian;
+  if (nSlices <= 0)
+    return or_fail("N_SLICES (or N_KICKS) <=0  in multipole_tracking()");
+  if (order < 0)
+    return or_fail("order < 0 in multipole_tracking()");
+  if (order > 8)
+    return or_fail("MULT order above 8 not supported");
+  if (multipole->bore)
+    return or_fail("MULT with BORE/BTIPL not supported");
+  if (multipole->synch_rad)
+    return or_fail("MULT with SYNCH_RAD not supported");
+  drift = multipole->length / nSlices / 2;
+  KnL = multipole->KnL * multipole->factor / nSlices;
+  if (KnL == 0) {
+    if (multipole->length == 0)
+      return n_part;
+    or_exactDrift(particle, n_part, multipole->length);
+    return n_part;
+  }
+  if (multipole->dx || multipole->dy || multipole->dz)
+    offsetBeamCoordinatesForMisalignment(particle, n_part, multipole->dx, multipole->dy, multipole->dz);
+  if (multipole->tilt)
+    rotateBeamCoordinatesForMisalignment(particle, n_part, multipole->tilt);
+  i_top = n_part - 1;
+  for (i_part = 0; i_part <= i_top; i_part++) {
+    double *coord = particle[i_part];
+    x = coord[0];
+    xp = coord[1];
+    y = coord[2];
+    yp = coord[3];
+    s = 0;
+    dp = coord[5];
+    if (isnan(x) || isnan(xp) || isnan(y) || isnan(yp) || fabs(x) > 10.0 || fabs(y) > 10.0 || fabs(xp) > 1.0 ||
+        fabs(yp) > 1.0) {
+      swapParticles(particle[i_part], particle[i_top]);
+      particle[i_top][4] = z_start;
+      particle[i_top][5] = Po * (1 + particle[i_top][5]);
+      i_top--;
+      i_part--;
+      continue;
+    }
+    slopes_to_momenta(&qx, &qy, xp, yp, dp, expand);
+    is_lost = 0;
+    for (i_kick = 0; i_kick < nSlices; i_kick++) {
+      if (drift) {
+        x += xp * drift * (i_kick ? 2 : 1);
+        y += yp * drift * (i_kick ? 2 : 1);
+        if (expand)
+          s += (i_kick ? 2 : 1) * drift * (1 + (sqr(xp) + sqr(yp)) / 2);
+        else
+          s += (i_kick ? 2 : 1) * drift * sqrt(1 + sqr(xp) + sqr(yp));
+      }
+      fillPowerArray(x, xpow, order);
+      fillPowerArray(y, ypow, order);
+      or_kick(&qx, &qy, xpow, ypow, order, KnL, 0);
+      if (!momenta_to_slopes(&xp, &yp, qx, qy, dp, expand)) {
+        is_lost = 1;
+        break;
+      }
+    }
+    if (drift && !is_lost) {
+      x += xp * drift;
+      y += yp * drift;
+      if (expand)
+        s += drift * (1 + (sqr(xp) + sqr(yp)) / 2);
+      else
+        s += drift * sqrt(1 + sqr(xp) + sqr(yp));
+    }
+    coord[0] = x;
+    coord[1] = xp;
+    coord[2] = y;
+    coord[3] = yp;
+    coord[5] = dp;
+    coord[4] += s;
+    if (isnan(x) || isnan(xp) || isnan(y) || isnan(yp) || fabs(x) > 10.0 || fabs(y) > 10.0 || fabs(xp) > 1.0 ||
+        fabs(yp) > 1.0 || is_lost) {
+      swapParticles(particle[i_part], particle[i_top]);
+      particle[i_top][4] = z_start;
+      particle[i_top][5] = Po * (1 + particle[i_top][5]);
+      i_top--;
+      i_part--;
+      continue;
+    }
+  }
+  if (multipole->tilt)
+    rotateBeamCoordinatesForMisalignment(particle, n_part, -multipole->tilt);
+  if (multipole->dx || multipole->dy || multipole->dz)
+    offsetBeamCoordinatesForMisalignment(particle, n_part, -multipole->dx, -multipole->dy, -multipole->dz);
+  return i_top + 1;
+}
+
+/* ROTATE: rotation_matrix(tilt) (src/tilt_matrices.c:81-117) applied by the
+ * order-1 loop of track_particles (src/matrix.c:424-460): every output is
+ * C[i] plus R[i][j]*ini[j] for j = 5 down to 0, zero entries included */
+void or_rotate(double **coord, long np, const EH_ROTATE *rot) {
+  double R[6][6], sin_tilt, cos_tilt, temp[6];
+  long i, j, ip;
+  const double tilt = rot->tilt;
+  if (fabs(tilt - PI / 2) < 1e-12) {
+    sin_tilt = 1;
+    cos_tilt = 0;
+  } else if (fabs(tilt - PI) < 1e-12) {
+    sin_tilt = 0;
+    cos_tilt = -1;
+  } else {
+    sin_tilt = OR_SIN(tilt);
+    cos_tilt = OR_COS(tilt);
+  }
+  memset(R, 0, sizeof R);
+  R[0][0] = cos_tilt;
+  R[0][2] = sin_tilt;
+  R[2][0] = -sin_tilt;
+  R[2][2] = cos_tilt;
+  R[1][1] = cos_tilt;
+  R[1][3] = sin_tilt;
+  R[3][1] = -sin_tilt;
+  R[3][3] = cos_tilt;
+  R[4][4] = R[5][5] = 1;
+  for (ip = np - 1; ip >= 0; ip--) {
+    double *ini = coord[ip];
+    for (i = 5; i >= 0; i--) {
+      double sum = 0;
+      for (j = 5; j >= 0; j--)
+        sum += R[i][j] * ini[j];
+      temp[i] = sum;
+    }
+    for (i = 5; i >= 0; i--)
+      ini[i] = temp[i];
+  }
+}
+
+/* MALIGN: offset_beam, src/do_tracking.c:3311-3366, every particle */
+long or_offset_beam(double **coord, long nToTrack, const EH_MALIGN *offset, double P_central) {
+  long i_part;
+  double *part, pc, beta, gamma, t, ds;
+  if (offset->startPID != -1 || offset->endPID != -1)
+    return or_fail("MALIGN particle ranges not supported");
+  for (i_part = nToTrack - 1; i_part >= 0; i_part--) {
+    part = coord[i_part];
+    if (offset->dz)
+      ds = offset->dz * sqrt(1 + sqr(part[1]) + sqr(part[3]));
+    else
+      ds = 0;
+    part[0] += offset->dx + offset->dz * part[1];
+    part[1] += offset->dxp;
+    part[2] += offset->dy + offset->dz * part[3];
+    part[3] += offset->dyp;
+    part[4] += ds;
+    if (offset->dt || offset->dp || offset->de) {
+      pc = P_central * (1 + part[5]);
+      beta = pc / (gamma = sqrt(1 + pc * pc));
+      t = part[4] / (beta * c_mks) + offset->dt;
+      if (offset->dp) {
+        part[5] += offset->dp;
+        pc = P_central * (1 + part[5]);
+        beta = pc / sqrt(1 + pc * pc);
+      }
+      if (offset->de) {
+        gamma += offset->de * gamma;
+        pc = sqrt(gamma * gamma - 1);
+        beta = pc / gamma;
+        part[5] = (pc - P_central) / P_central;
+      }
+      part[4] = t * beta * c_mks;
+    }
+  }
+  return nToTrack;
+}
+
+/* ------------------------------------------------------------------------- */
 /* CSBEND field expansion (computeCSBENDFieldCoefficients, src/csbend.c:175-652):
  * coefficients evaluated term by term from the generated table, whose entries
  * equal the reference's formulas */
@@ -1237,6 +1415,8 @@ static double or_elem_length(const EH_ELEMENT_LIST *e) {
   switch (e->type) {
   case EH_T_MARK:
   case EH_T_MAXAMP:
+  case EH_T_ROTATE:
+  case EH_T_MALIGN:
     return 0;
   default:
     return *(const double *)e->p_elem; /* length is the first member, as in the reference */
@@ -1300,6 +1480,21 @@ long or_do_tracking(double **coord, long np, const EH_ELEMENT_LIST *beamline, lo
       case EH_T_CSBEND:
         nLeft = or_track_through_csbend(coord, nToTrack, (const EH_CSBEND *)eptr->p_elem, *P_central, last_z, maxamp);
         break;
+      case EH_T_MULT:
+        nLeft = or_multipole_tracking(coord, nToTrack, (const EH_MULT *)eptr->p_elem, *P_central, last_z);
+        break;
+      case EH_T_ROTATE: {
+        const EH_ROTATE *r = (const EH_ROTATE *)eptr->p_elem;
+        if (r->on_pass < 0 || r->on_pass == i_pass)
+          or_rotate(coord, nToTrack, r);
+        break;
+      }
+      case EH_T_MALIGN: {
+        const EH_MALIGN *m = (const EH_MALIGN *)eptr->p_elem;
+        if (m->on_pass == -1 || m->on_pass == i_pass)
+          nLeft = or_offset_beam(coord, nToTrack, m, *P_central);
+        break;
+      }
       case EH_T_RFCA:
         nLeft = or_simple_rf_cavity(coord, nToTrack, (EH_RFCA *)eptr->p_elem, P_central, z_travel);
         break;
@@ -1359,6 +1554,15 @@ long or_element_flat(double *flat, long np, const EH_ELEMENT_LIST *eptr, double 
     break;
   case EH_T_CSBEND:
     r = or_track_through_csbend(rows, np, (const EH_CSBEND *)eptr->p_elem, *P_central, z_start, maxamp);
+    break;
+  case EH_T_MULT:
+    r = or_multipole_tracking(rows, np, (const EH_MULT *)eptr->p_elem, *P_central, z_start);
+    break;
+  case EH_T_ROTATE:
+    or_rotate(rows, np, (const EH_ROTATE *)eptr->p_elem);
+    break;
+  case EH_T_MALIGN:
+    r = or_offset_beam(rows, np, (const EH_MALIGN *)eptr->p_elem, *P_central);
     break;
   case EH_T_RFCA:
     r = or_simple_rf_cavity(rows, np, (EH_RFCA *)eptr->p_elem, P_central, z_end);

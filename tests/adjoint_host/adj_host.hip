@@ -33,7 +33,7 @@ extern "C" long adjh_gradient(const double *coord, long n, EH_ELEMENT_LIST *bl, 
         int slot = e == 0 ? 0 : hd[e].adj_anchor;
         if (slot >= 0)
           anchors[t * n_anchor + slot] = p;
-        if (eh_apply_element(p, hd[e], t > 0 ? 1 : 0, pool.data()) >= 0)
+        if (eh_apply_element(p, hd[e], t, pool.data()) >= 0)
           live = false;
       }
     alive_out[i] = live;

@@ -30,6 +30,13 @@ CASES = {
     "drifts": [Element("D", "DRIF", L=1.6), Element("E", "EDRIFT", L=0.7), Element("D1", "DRIF", L=0.4, ORDER=1)],
     "kquse": [Element("QS", "KQUSE", L=0.25, K1=1.4, K2=-35.0, FSE1=2e-3, FSE2=-5e-3, N_KICKS=8, DX=2e-5,
                       TILT=0.03)],
+    "mult": [Element("M3", "MULT", L=0.2, KNL=400.0, ORDER=3, N_SLICES=4, TILT=0.02, DX=1e-4, DZ=1e-3),
+             Element("M0", "MULT", L=0.0, KNL=-2e-4, ORDER=0, N_SLICES=1),
+             Element("M5", "MULT", L=0.1, KNL=2e7, ORDER=5, N_SLICES=2)],
+    "rotate_malign": [Element("R", "ROTATE", TILT=0.3), Element("MA", "MALIGN", DX=1e-4, DXP=2e-5, DZ=1e-3,
+                                                                 DP=1e-3, ON_PASS=1),
+                      Element("D", "DRIF", L=0.5), Element("R0", "ROTATE", TILT=-0.3, ON_PASS=0),
+                      Element("ME", "MALIGN", DE=1e-4, DT=1e-12)],
 }
 
 

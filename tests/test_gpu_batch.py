@@ -61,3 +61,43 @@ def test_batch_repeated_calls_reuse_tables():
         assert_rows_identical(batch.download(v), expect, batch.n_alive(v), n3, "variant %d" % v)
         total += n3
     assert n == total
+
+
+def test_batch_more_elements_on_pass():
+    """variants of a ring with MULT, ROTATE and MALIGN: the batched kernel applies
+    ON_PASS elements on the absolute pass, as each variant tracked alone does"""
+    from elegant_amd.lattice import Element, Lattice
+    print("batch more elements", flush=True)
+    sys.stdout.flush()
+    Po = configs.p_central_from_mev(1500.0)
+
+    def ring(v):
+        return Lattice([
+            Element("AP", "MAXAMP", X_MAX=0.03, Y_MAX=0.02),
+            Element("QF", "KQUAD", L=0.3, K1=1.1 + 0.01 * v, N_KICKS=8),
+            Element("M3", "MULT", L=0.1, KNL=300.0 * (1 + v), ORDER=3, N_SLICES=2, TILT=0.01 * v),
+            Element("D1", "DRIF", L=1.0),
+            Element("R1", "ROTATE", TILT=0.2, ON_PASS=1),
+            Element("QD", "KQUAD", L=0.3, K1=-1.1, N_KICKS=8),
+            Element("R2", "ROTATE", TILT=-0.2, ON_PASS=1),
+            Element("MA", "MALIGN", DXP=2e-5 * v, DX=1e-5, ON_PASS=2),
+            Element("D2", "EDRIFT", L=1.0),
+        ])
+
+    nvar = 3
+    rng = np.random.default_rng(4)
+    beams = []
+    for v in range(nvar):
+        b = np.zeros((400 + 50 * v, 10))
+        b[:, :4] = rng.normal(0, [6e-3, 6e-4, 4e-3, 4e-4], (len(b), 4))
+        b[:, 5] = rng.normal(0, 1e-3, len(b))
+        b[:, 6] = np.arange(1, len(b) + 1)
+        beams.append(b)
+    batch = tracking.Batch(beams)
+    n, _ = batch.track([ring(v) for v in range(nvar)], 4, Po)
+    total = 0
+    for v in range(nvar):
+        ref, nref, _, _ = oracle.do_tracking(beams[v], ring(v), 4, Po)
+        total += nref
+        assert_rows_identical(batch.download(v), ref, batch.n_alive(v), nref, "variant %d" % v)
+    assert n == total

@@ -81,6 +81,15 @@ ELEMENT_CASES = [
     ("EDRIFT", dict(L=0.7)),
     ("DRIF", dict(L=1.6)),
     ("DRIF", dict(L=1.6, ORDER=1)),
+    ("MULT", dict(L=0.2, KNL=400.0, ORDER=3, N_SLICES=4, TILT=0.02, DX=1e-4, DY=-5e-5, DZ=1e-3)),
+    ("MULT", dict(L=0.0, KNL=-2e-4, ORDER=0, N_SLICES=1)),
+    ("MULT", dict(L=0.1, KNL=2e7, ORDER=5, FACTOR=1.5, N_SLICES=2, EXPAND_HAMILTONIAN=1)),
+    ("MULT", dict(L=0.3, KNL=-40.0, ORDER=1, N_SLICES=6, TILT=math.pi / 2)),
+    ("MULT", dict(L=0.4, KNL=0.0, ORDER=2)),
+    ("ROTATE", dict(TILT=0.3)),
+    ("ROTATE", dict(TILT=math.pi)),
+    ("MALIGN", dict(DX=1e-4, DY=-2e-4, DZ=3e-3, DXP=1e-5, DYP=-2e-5)),
+    ("MALIGN", dict(DZ=-1e-3, DP=1e-3, DE=-2e-4, DT=3e-12)),
 ]
 
 
@@ -103,6 +112,12 @@ def test_single_element(case, ehlib, oracle_mod):
         n = ehlib.eh_multipole_tracking2(beam.handle, len(rows), arr, 0.0, Po, 1.0, C.byref(amp), -1)
     elif code == S.T_CSBEND:
         n = ehlib.eh_track_through_csbend(beam.handle, len(rows), C.byref(s), 0.0, Po, 1.0, C.byref(amp), -1)
+    elif code == S.T_MULT:
+        n = ehlib.eh_multipole_tracking(beam.handle, len(rows), C.byref(s), 0.0, Po, 1.0)
+    elif code == S.T_ROTATE:
+        n = ehlib.eh_rotate_beam(beam.handle, len(rows), C.byref(s))
+    elif code == S.T_MALIGN:
+        n = ehlib.eh_offset_beam(beam.handle, len(rows), C.byref(s), Po)
     elif code == S.T_EDRIFT:
         n = ehlib.eh_exactDrift(beam.handle, len(rows), L)
     else:
@@ -163,6 +178,42 @@ def test_mba_da_grid(mode, ehlib, oracle_mod):
     rows = configs.da_grid(33, 17, xmax=0.03, ymax=0.012)
     got, n, ref, nref, stats, eff = _track_both(lat, rows, 3, Po, mode, tpl=2)
     assert_rows_identical(got, ref, n, nref, "mba " + mode)
+    assert stats["particle_element_turns"] == eff
+
+
+def _more_elements_ring():
+    """a FODO-like ring with MULT error multipoles, a pass-dependent ROTATE pair
+    and MALIGN kicks, behind a MAXAMP"""
+    return Lattice([
+        Element("AP", "MAXAMP", X_MAX=0.03, Y_MAX=0.02),
+        Element("M3", "MULT", L=0.1, KNL=300.0, ORDER=3, N_SLICES=2, TILT=0.01, DX=5e-5),
+        Element("QF", "KQUAD", L=0.3, K1=1.1, N_KICKS=8),
+        Element("D1", "DRIF", L=1.0),
+        Element("R1", "ROTATE", TILT=0.2, ON_PASS=1),
+        Element("QD", "KQUAD", L=0.3, K1=-1.1, N_KICKS=8),
+        Element("R2", "ROTATE", TILT=-0.2, ON_PASS=1),
+        Element("MA", "MALIGN", DXP=2e-5, DYP=-1e-5, DX=1e-5),
+        Element("M0", "MULT", L=0.0, KNL=1e-5, ORDER=0),
+        Element("D2", "EDRIFT", L=1.0),
+        Element("MD", "MALIGN", DP=1e-4, DZ=1e-4, ON_PASS=2),
+        Element("M2", "MULT", L=0.2, KNL=-6.0, ORDER=2, N_SLICES=3, EXPAND_HAMILTONIAN=1),
+        Element("D3", "DRIF", L=0.5),
+    ])
+
+
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+def test_more_elements_ring(mode, ehlib, oracle_mod):
+    """MULT, ROTATE (ON_PASS) and MALIGN (every pass and ON_PASS) inside do_tracking,
+    with MAXAMP and NaN sweeps after each, bit for bit with the oracle, losses included"""
+    _say("more-elements ring %s" % mode)
+    lat = _more_elements_ring()
+    rows = _beam_rows(3000, seed=21, scale=2.0)
+    rows[:30, 0] = np.linspace(-0.04, 0.04, 30)
+    rows[30:40, 1] = np.linspace(-2.0, 2.0, 10)  # slopes beyond MULT's limit, lost at its entry
+    Po = configs.p_central_from_mev(1500.0)
+    got, n, ref, nref, stats, eff = _track_both(lat, rows, 5, Po, mode, tpl=2)
+    assert 0 < nref < len(rows)
+    assert_rows_identical(got, ref, n, nref, "more elements " + mode)
     assert stats["particle_element_turns"] == eff
 
 

@@ -401,3 +401,167 @@ def test_libm_vs_eh_math_drift(oracle_mod):
     # rho0 * 1.1e-16 = 3.5e-16 m per sub-step
     for name, (mx, rms) in st.items():
         assert mx <= 5e-11, (name, mx)
+
+
+# ---------------------------------------------------------------- MULT
+def _thin_mult(x, xp, y, yp, delta, KnL, order, tilt=0.0, dx=0.0, dy=0.0):
+    """a zero-length multipole of strength KnL: in the element frame (shifted by
+    DX/DY, rotated by TILT) the canonical momenta change by
+    dqx + i dqy = -KnL conj((x+iy)^n)/n!, i.e. B_y + i B_x from the complex potential"""
+    rot = complex(math.cos(tilt), -math.sin(tilt))
+    z = complex(x - dx, y - dy) * rot
+    w = complex(xp, yp) * rot
+    q = (1 + delta) * w / math.sqrt(1 + abs(w) ** 2)
+    q += -KnL * (z ** order).conjugate() / math.factorial(order)
+    w = q / math.sqrt((1 + delta) ** 2 - abs(q) ** 2)
+    z, w = z / rot + complex(dx, dy), w / rot
+    return z.real, w.real, z.imag, w.imag
+
+
+@pytest.mark.parametrize("order,KnL", [(0, 2e-4), (1, 0.8), (2, -30.0), (3, 900.0), (5, 4e6), (8, -3e10)])
+@pytest.mark.parametrize("nsl", [1, 3])
+def test_mult_thin_kick_closed_form(oracle_mod, order, KnL, nsl):
+    """multipole_tracking (src/multipole.c:439-658) at zero length: N_SLICES kicks of
+    KnL*FACTOR/N_SLICES at one point add up to the complex-potential kick; TILT
+    and DX/DY act as a frame change.  A dropped slice, a factor applied twice, a
+    wrong sign or a transposed x/y power fails here"""
+    rng = np.random.default_rng(order * 7 + nsl)
+    pts = np.column_stack([rng.normal(0, 3e-3, 30), rng.normal(0, 3e-4, 30), rng.normal(0, 2e-3, 30),
+                           rng.normal(0, 2e-4, 30), rng.normal(0, 1e-3, 30), rng.normal(0, 1e-3, 30)])
+    tilt, dx, dy = 0.07, 2e-4, -1e-4
+    p = dict(L=0.0, KNL=KnL / 2, FACTOR=2.0, ORDER=order, N_SLICES=nsl, TILT=tilt, DX=dx, DY=dy)
+    r, n = one(oracle_mod, "MULT", p, pts)
+    assert n == 30
+    for i in range(30):
+        x, xp, y, yp = _thin_mult(*pts[i, [0, 1, 2, 3, 5]], KnL, order, tilt, dx, dy)
+        got = r[r[:, 6] == i + 1][0]
+        scale = max(abs(x), abs(y), 1e-6)
+        assert abs(got[0] - x) < 1e-15 * scale and abs(got[2] - y) < 1e-15 * scale
+        kick = max(abs(xp), abs(yp), abs(pts[i, 1]), abs(pts[i, 3]))
+        assert abs(got[1] - xp) < 1e-13 * kick and abs(got[3] - yp) < 1e-13 * kick
+        assert got[4] == pts[i, 4] and got[5] == pts[i, 5]
+
+
+def test_mult_thick_quadrupole_limit(oracle_mod):
+    """MULT ORDER=1 of length L in many slices converges to the thick-quadrupole
+    matrix (half drifts at the ends, full drifts between kicks of K1 L/N)"""
+    L, K = 0.3, 1.2
+    p = dict(L=L, KNL=K * L, ORDER=1, N_SLICES=400)
+    r, _ = one(oracle_mod, "MULT", p, [[1e-7, 0, 1e-7, 0, 0, 0], [0, 1e-7, 0, 1e-7, 0, 0]])
+    k = math.sqrt(K)
+    Mx = np.array([[math.cos(k * L), math.sin(k * L) / k], [-k * math.sin(k * L), math.cos(k * L)]])
+    My = np.array([[math.cosh(k * L), math.sinh(k * L) / k], [k * math.sinh(k * L), math.cosh(k * L)]])
+    assert np.allclose(r[0, [0, 1]] / 1e-7, Mx[:, 0], rtol=2e-6)
+    assert np.allclose(r[1, [0, 1]] / 1e-7, Mx[:, 1], rtol=2e-6)
+    assert np.allclose(r[0, [2, 3]] / 1e-7, My[:, 0], rtol=2e-6)
+    assert np.allclose(r[1, [2, 3]] / 1e-7, My[:, 1], rtol=2e-6)
+
+
+@pytest.mark.parametrize("expand", [0, 1])
+def test_mult_reversible(oracle_mod, expand):
+    """drift-kick-drift with half drifts at both ends is time-reversible: track,
+    flip the slopes, track again, flip; the path length is the same both ways"""
+    p = dict(L=0.3, KNL=12.0, ORDER=2, N_SLICES=5, EXPAND_HAMILTONIAN=expand)
+    start = [3e-3, 2e-4, -2e-3, 1e-4, 0.0, 1e-3]
+    r, _ = one(oracle_mod, "MULT", p, [start])
+    back = [r[0, 0], -r[0, 1], r[0, 2], -r[0, 3], 0.0, r[0, 5]]
+    r2, _ = one(oracle_mod, "MULT", p, [back])
+    assert abs(r2[0, 0] - start[0]) < 1e-17 and abs(-r2[0, 1] - start[1]) < 1e-17
+    assert abs(r2[0, 2] - start[2]) < 1e-17 and abs(-r2[0, 3] - start[3]) < 1e-17
+    assert r2[0, 4] == pytest.approx(r[0, 4], rel=1e-14)
+    assert r[0, 4] > 0.3
+
+
+def test_mult_zero_strength_is_exact_drift(oracle_mod):
+    """KnL*FACTOR = 0: exactDrift over L (src/multipole.c:493-498), offsets and
+    tilt ignored; 3-4-5 slopes make it exact.  At zero length nothing changes."""
+    r, _ = one(oracle_mod, "MULT", dict(L=2.0, KNL=5.0, FACTOR=0.0, DX=1.0, TILT=0.3),
+               [[0.5, 0.75, -0.25, 0.0, 1.0, 0.0]])
+    assert list(r[0, :6]) == [0.5 + 1.5, 0.75, -0.25, 0.0, 1.0 + 2.0 * 1.25, 0.0]
+    r, _ = one(oracle_mod, "MULT", dict(L=0.0, KNL=0.0), [[-0.0, np.inf, 0.0, 0.0, 1.0, 0.0]])
+    assert math.copysign(1, r[0, 0]) < 0 and r[0, 1] == np.inf
+
+
+def test_mult_loss_bookkeeping(oracle_mod):
+    """hand-worked losses of multipole_tracking: |x| > 10 m on entry keeps the
+    coordinates; a kick that leaves no longitudinal momentum keeps the values
+    reached (x after the first half drift, the last good slopes); both record
+    s = z_start and coord[5] = Po (1 + delta), swapped to the top"""
+    Po = 1000.0
+    pts = [[11.0, 0.0, 0.0, 0.0, 7.0, 0.5], [0.25, 0.5, 0.0, 0.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0, 3.0, 0.0]]
+    r, n = one(oracle_mod, "MULT", dict(L=0.5, KNL=100.0, ORDER=1, N_SLICES=1), pts, Po=Po, z0=5.0)
+    assert n == 1
+    assert r[0, 6] == 3 and r[0, 4] == 3.0 + 0.5  # the survivor, on axis
+    lost = {int(row[6]): row for row in r[1:]}
+    assert list(lost[1][:6]) == [11.0, 0.0, 0.0, 0.0, 5.0, Po * 1.5]
+    assert list(lost[2][:6]) == [0.25 + 0.5 * 0.25, 0.5, 0.0, 0.0, 5.0, Po]
+
+
+# ---------------------------------------------------------------- ROTATE, MALIGN
+def test_rotate_closed_form(oracle_mod):
+    """ROTATE through rotation_matrix (src/tilt_matrices.c:81-117): (x, y) and
+    (x', y') turn by -TILT in the transverse plane; pi/2 and pi are exact"""
+    rng = np.random.default_rng(5)
+    pts = rng.normal(0, 1e-3, (20, 6))
+    t = 0.37
+    r, n = one(oracle_mod, "ROTATE", dict(TILT=t), pts)
+    assert n == 20
+    for i in range(20):
+        got = r[r[:, 6] == i + 1][0]
+        z = complex(pts[i, 0], pts[i, 2]) * complex(math.cos(t), -math.sin(t))
+        w = complex(pts[i, 1], pts[i, 3]) * complex(math.cos(t), -math.sin(t))
+        assert got[[0, 2]] == pytest.approx([z.real, z.imag], rel=1e-15, abs=1e-19)
+        assert got[[1, 3]] == pytest.approx([w.real, w.imag], rel=1e-15, abs=1e-19)
+        assert got[4] == pts[i, 4] and got[5] == pts[i, 5]
+    r, _ = one(oracle_mod, "ROTATE", dict(TILT=math.pi / 2), pts[:3])
+    assert (r[:, 0] == pts[:3, 2]).all() and (r[:, 2] == -pts[:3, 0]).all()
+    assert (r[:, 1] == pts[:3, 3]).all() and (r[:, 3] == -pts[:3, 1]).all()
+    r, _ = one(oracle_mod, "ROTATE", dict(TILT=math.pi), pts[:3])
+    assert (r[:, :4] == -pts[:3, :4]).all()
+    # the order-1 matrix loop multiplies every coordinate, zero entries included:
+    # an infinite s turns every other output into NaN (0 * inf) and stays inf
+    r, _ = one(oracle_mod, "ROTATE", dict(TILT=0.1), [[0, 0, 0, 0, np.inf, 0]])
+    assert np.isnan(r[0, [0, 1, 2, 3, 5]]).all() and r[0, 4] == np.inf
+
+
+def test_rotate_on_pass(oracle_mod):
+    """ON_PASS (src/do_tracking.c:1836-1852): the rotation acts on that pass only"""
+    lat = Lattice([Element("R", "ROTATE", TILT=math.pi / 2, ON_PASS=1), Element("D", "EDRIFT", L=1.0)])
+    pts = rows_of([[1e-3, 0, 0, 0, 0, 0]])
+    r, n, _, _ = oracle_mod.do_tracking(pts, lat, 3, 1000.0)
+    assert n == 1 and r[0, 0] == 0.0 and r[0, 2] == -1e-3
+
+
+def _beta(Po, delta):
+    p = Po * (1 + delta)
+    return p / math.sqrt(1 + p * p)
+
+
+def test_malign_closed_form(oracle_mod):
+    """offset_beam (src/do_tracking.c:3311-3372): transverse offsets, a DZ drift,
+    and DP/DE/DT changes that keep the arrival time: s' = (s/(beta c) + DT) beta' c"""
+    c = 2.99792458e8
+    Po = 2000.0
+    start = [1e-3, 2e-4, -5e-4, 1e-4, 0.25, 1e-3]
+    dx, dy, dz, dxp, dyp = 1e-4, -2e-4, 3e-3, 1e-5, -2e-5
+    r, _ = one(oracle_mod, "MALIGN", dict(DX=dx, DY=dy, DZ=dz, DXP=dxp, DYP=dyp), [start], Po=Po)
+    x, xp, y, yp, s, d = start
+    assert r[0, 0] == pytest.approx(x + dx + dz * xp, rel=1e-15)
+    assert r[0, 2] == pytest.approx(y + dy + dz * yp, rel=1e-15)
+    assert r[0, 1] == pytest.approx(xp + dxp, rel=1e-15) and r[0, 3] == pytest.approx(yp + dyp, rel=1e-15)
+    assert r[0, 4] == pytest.approx(s + dz * math.sqrt(1 + xp * xp + yp * yp), rel=1e-15)
+    assert r[0, 5] == d
+    # DP: delta' = delta + DP, same arrival time
+    r, _ = one(oracle_mod, "MALIGN", dict(DP=2e-3), [start], Po=Po)
+    assert r[0, 5] == pytest.approx(d + 2e-3, rel=1e-15)
+    assert r[0, 4] == pytest.approx(s * _beta(Po, d + 2e-3) / _beta(Po, d), rel=1e-14)
+    # DE: gamma' = gamma (1 + DE)
+    r, _ = one(oracle_mod, "MALIGN", dict(DE=1e-3), [start], Po=Po)
+    g = math.sqrt(1 + (Po * (1 + d)) ** 2) * (1 + 1e-3)
+    d1 = math.sqrt(g * g - 1) / Po - 1
+    assert r[0, 5] == pytest.approx(d1, rel=1e-12)
+    assert r[0, 4] == pytest.approx(s * _beta(Po, d1) / _beta(Po, d), rel=1e-14)
+    # DT: a time shift
+    r, _ = one(oracle_mod, "MALIGN", dict(DT=1e-11), [start], Po=Po)
+    assert r[0, 4] == pytest.approx(s + 1e-11 * _beta(Po, d) * c, rel=1e-14)
+    assert r[0, 5] == d
