@@ -23,7 +23,7 @@ EXPORTS = [
     "eh_replay_order", "eh_selftest_math", "eh_batch_create", "eh_batch_free", "eh_batch_upload",
     "eh_batch_download", "eh_batch_n_alive", "eh_batch_track", "eh_diag_copy", "eh_adjoint_create",
     "eh_adjoint_free", "eh_adjoint_forward", "eh_adjoint_backward", "eh_adjoint_forward_batch",
-    "eh_multipole_tracking", "eh_offset_beam", "eh_rotate_beam",
+    "eh_multipole_tracking", "eh_offset_beam", "eh_rotate_beam", "eh_replay_sequence",
 ]
 
 
@@ -89,6 +89,8 @@ def load():
     lib.eh_csbend_field_table.restype = L
     lib.eh_replay_order.argtypes = [L, vp, vp, L, vp]
     lib.eh_replay_order.restype = L
+    lib.eh_replay_sequence.argtypes = [L, vp, vp, L, vp, vp]
+    lib.eh_replay_sequence.restype = L
     lib.eh_selftest_math.argtypes = [vp, vp, L, vp]
     lib.eh_selftest_math.restype = L
     lib.eh_diag_copy.argtypes = [L, C.c_int]

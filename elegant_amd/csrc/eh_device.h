@@ -33,6 +33,11 @@
 #define EH_STAGE_ELEMENT 0 /* swap-to-top inside the element routine */
 #define EH_STAGE_MAXAMP 1  /* limit_amplitudes with MAXAMP limits after the element */
 #define EH_STAGE_NAN 2     /* limit_amplitudes(DBL_MAX) NaN sweep */
+/* flag on EH_STAGE_ELEMENT: a kick multipole lost the particle before its first
+ * step; the reference records z_start plus the dzLoss its previous particle left
+ * behind (integrate_kick_multipole_ordn resets dzLoss only after the entry
+ * checks, src/multipole.c:1160-1232), which the host resolves in row order */
+#define EH_STAGE_INHERIT 4
 
 typedef struct {
   int32_t i, j;
@@ -288,11 +293,6 @@ EH_HD bool eh_m2s(double &xp, double &yp, double qx, double qy, double dp, int e
   return true;
 }
 
-/* KQUAD/KSEXT/KOCT: multipole_tracking2 (src/multipole.c:660-1084) for one particle,
- * including the misalignment passes around the particle loop (method 0,
- * src/malign_mat.c:257-267, :580-590, tilt_matrices.c:156-187) and the loss
- * assignments of :1032-1047.  integrate_kick_multipole_ordn, src/multipole.c:1099-1456.
- * Returns true if lost. */
 /* entry face of a misaligned multipole: offsetBeamCoordinatesForMisalignment
  * (dx, dy, dz) then rotateBeamCoordinatesForMisalignment (tilt) */
 EH_HD void eh_mface1(eh_p6 &p, const eh_desc &d) {
@@ -311,8 +311,9 @@ EH_HD void eh_mface1(eh_p6 &p, const eh_desc &d) {
 }
 
 /* exit face: rotation by -tilt, then the offset by (-dx, -dy, -dz); every row,
- * lost ones included */
-EH_HD void eh_mface2(eh_p6 &p, const eh_desc &d) {
+ * lost ones included.  With s_too false the path-length term of the offset is
+ * left out (the host adds it to the loss position it settles) */
+EH_HD void eh_mface2(eh_p6 &p, const eh_desc &d, bool s_too) {
   if (d.has_offset || d.has_tilt) {
     if (d.has_tilt) {
       double x0 = p.x, xp0 = p.xp, y0 = p.y, yp0 = p.yp;
@@ -322,23 +323,32 @@ EH_HD void eh_mface2(eh_p6 &p, const eh_desc &d) {
       p.yp = -xp0 * d.xsin + yp0 * d.xcos;
     }
     if (d.has_offset) {
-      p.s += (-d.mdz) * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
+      if (s_too)
+        p.s += (-d.mdz) * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
       p.x = p.x - (-d.mdx) + (-d.mdz) * p.xp;
       p.y = p.y - (-d.mdy) + (-d.mdz) * p.yp;
     }
   }
 }
 
-EH_HD bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
+/* KQUAD/KSEXT/KOCT/KQUSE: multipole_tracking2 (src/multipole.c:660-1084) for one
+ * particle, including the misalignment passes around the particle loop (method 0,
+ * src/malign_mat.c:257-267, :580-590, tilt_matrices.c:156-187) and the loss
+ * assignments of :1032-1047.  integrate_kick_multipole_ordn, src/multipole.c:1099-1456.
+ * Returns 0 if the particle survives, 1 if lost, 2 if lost before the first
+ * step (its loss position is z_start plus the previous particle's dzLoss).  A
+ * lost particle keeps s = z_start + dzLoss without the exit offset's path-length
+ * term. */
+EH_HD int eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
   eh_mface1(p, d);
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
   double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, dp = p.dp, qx, qy;
   double dzLoss = 0;
-  bool lost = false;
+  bool lost = false, entry = false;
   int how = 0; /* 0: coord untouched, 1: x,y written, 2: all written */
   if (eh_isnan(x) || eh_isnan(xp) || eh_isnan(y) || eh_isnan(yp) || eh_fabs(x) > 10.0 || eh_fabs(y) > 10.0 ||
       eh_fabs(xp) > 1.0 || eh_fabs(yp) > 1.0) {
-    lost = true;
+    lost = entry = true;
   } else {
     /* convertSlopesToMomenta, src/multipole.h:448-464 */
     if (d.expand) {
@@ -350,7 +360,7 @@ EH_HD bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
       qy = (1 + dp) * yp / denom;
     }
     if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
-      lost = true;
+      lost = entry = true;
     } else {
       eh_pow4 w;
       for (int ik = 0; ik < d.nslices && !lost; ik++) {
@@ -423,8 +433,8 @@ EH_HD bool eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
     p.s = d.z_start + dzLoss;
     p.dp = d.Po * (1 + p.dp);
   }
-  eh_mface2(p, d);
-  return lost;
+  eh_mface2(p, d, !lost);
+  return lost ? (entry ? 2 : 1) : 0;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -551,7 +561,7 @@ EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d) {
     p.s = d.z_start;
     p.dp = d.Po * (1 + p.dp);
   }
-  eh_mface2(p, d);
+  eh_mface2(p, d, true);
   return lost;
 }
 
@@ -1038,10 +1048,12 @@ EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d) {
 
 /* one element followed by do_tracking's post-element sweeps
  * (src/do_tracking.c:2506-2524, :2675-2684).  Returns -1 if the particle
- * survives, else the loss stage. */
+ * survives, else the loss stage (EH_STAGE_ELEMENT possibly with
+ * EH_STAGE_INHERIT). */
 EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms) {
   const int pi = pass > 0 ? 1 : 0;
   bool lost = false;
+  int flag = 0;
   switch (d.type) {
   case 1: /* DRIF */
     eh_drif(p, d.length, d.T11);
@@ -1052,9 +1064,12 @@ EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm 
   case 3:
   case 4:
   case 5:
-  case 9: /* KQUSE */
-    lost = eh_multipole(p, d, pi);
+  case 9: { /* KQUSE */
+    const int m = eh_multipole(p, d, pi);
+    lost = m != 0;
+    flag = m == 2 ? EH_STAGE_INHERIT : 0;
     break;
+  }
   case 6:
     lost = eh_csbend(p, d, pi, terms);
     break;
@@ -1077,7 +1092,7 @@ EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm 
     break;
   }
   if (lost)
-    return EH_STAGE_ELEMENT;
+    return EH_STAGE_ELEMENT | flag;
   if (d.type == 0)
     return -1; /* MARK: NO_APERTURE, and nothing can be NaN-lost here that survived */
   if (d.post_ap[pi] && eh_limit_one(p, d.post_xmax[pi], d.post_ymax[pi], d.z_end, d.Po, d.extrap))

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "elegant_hip.h"
@@ -697,6 +698,124 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
 }
 
 /* ------------------------------------------------------------------------- */
+/* loss positions settled on the host.  multipole_tracking2 records a lost
+ * particle at z_start + dzLoss, then its exit offset adds a path-length term
+ * to every row (src/multipole.c:1043, :1069-1070); a particle lost before the
+ * integrator's first step keeps the dzLoss the routine's previous particle left
+ * behind (dzLoss is reset only after the entry checks, :1160-1232).  The kernels
+ * store z_start + the particle's own dzLoss without the offset term and flag the
+ * entry losses; the download replays each routine call's particle sequence and
+ * settles both, so lost rows match the reference bit for bit. */
+/* loss pass word: the pass, with this bit set for a loss that inherits its
+ * position (EH_STAGE_INHERIT); cleared before the word reaches the caller */
+#define EH_LPASS_INHERIT 0x40000000
+
+struct eh_kmp {
+  int32_t kick;       /* KQUAD/KSEXT/KOCT/KQUSE */
+  int32_t has_offset; /* exit offset (dx, dy or dz) */
+  double z_start;
+  double zfull; /* z_start + the dzLoss of a particle that ran the whole element */
+  double mdz;
+};
+
+static eh_kmp kmp_of(const eh_desc &d) {
+  eh_kmp k = {};
+  k.kick = d.type == EH_T_KQUAD || d.type == EH_T_KSEXT || d.type == EH_T_KOCT || d.type == EH_T_KQUSE;
+  if (!k.kick)
+    return k;
+  /* the survivor's dzLoss: every drift sub-step of every slice, in order */
+  double dz = 0;
+  for (int ik = 0; ik < d.nslices; ik++)
+    for (int st = 0; st < d.nsub; st++)
+      if (d.mdrift != 0)
+        dz += d.dsh[st];
+  k.has_offset = d.has_offset;
+  k.z_start = d.z_start;
+  k.zfull = d.z_start + dz;
+  k.mdz = d.mdz;
+  return k;
+}
+
+typedef std::shared_ptr<const std::vector<eh_kmp>> eh_kmp_tab;
+typedef std::shared_ptr<const std::vector<int32_t>> eh_kmp_idx;
+
+static eh_kmp_tab kmp_table(const std::vector<eh_desc> &hd) {
+  auto t = std::make_shared<std::vector<eh_kmp>>(hd.size());
+  for (size_t i = 0; i < hd.size(); i++)
+    (*t)[i] = kmp_of(hd[i]);
+  return t;
+}
+
+/* the events [base, base + span) of one launch: (event - base) = ((pass - p0) *
+ * nelem + element) * 3 + stage; a plain routine call is one event (span 1) */
+struct eh_evrec {
+  int32_t base, span, p0, nelem;
+  eh_kmp_tab tab; /* per element, or per descriptor-pool entry when idx is set */
+  eh_kmp_idx idx; /* batches: variant * nelem + element -> tab entry */
+};
+
+/* the kick-multipole record of an element-stage loss event, or null */
+static const eh_kmp *kmp_at(const std::vector<eh_evrec> &log, int32_t ev, long variant) {
+  auto it = std::upper_bound(log.begin(), log.end(), ev,
+                             [](int32_t e, const eh_evrec &r) { return e < r.base; });
+  if (it == log.begin())
+    return nullptr;
+  --it;
+  const long rel = (long)ev - it->base;
+  if (rel < 0 || rel >= it->span || rel % 3 != 0)
+    return nullptr;
+  const long el = (rel / 3) % it->nelem;
+  long k = el;
+  if (it->idx) {
+    const size_t j = (size_t)variant * it->nelem + el;
+    if (variant < 0 || j >= it->idx->size())
+      return nullptr;
+    k = (*it->idx)[j];
+  }
+  if (k < 0 || (size_t)k >= it->tab->size() || !(*it->tab)[k].kick)
+    return nullptr;
+  return &(*it->tab)[k];
+}
+
+/* order: lost identities in the sequence the reference's routines processed
+ * them; pred[id]: -2 the first particle of its routine call, -1 processed after
+ * a survivor, else the lost identity processed just before.  G holds the
+ * graveyard's six columns (6 x nl), gk[id] the slot of identity id, gev and
+ * gpass its event and pass word (whose flag is cleared here). */
+static void settle_losses(const std::vector<eh_evrec> &log, long variant, const std::vector<int32_t> &order,
+                          const std::vector<int32_t> &pred, const std::vector<int32_t> &gk,
+                          const std::vector<int32_t> &gev, std::vector<double> &G, long nl,
+                          std::vector<int32_t> &gpass) {
+  if (!log.empty()) {
+    std::vector<double> base(pred.size(), 0.0);
+    for (int32_t id : order) {
+      const long k = gk[id];
+      if (k < 0)
+        continue;
+      const eh_kmp *m = kmp_at(log, gev[k], variant);
+      if (!m)
+        continue;
+      double b;
+      if (gpass[k] & EH_LPASS_INHERIT) {
+        const int32_t q = pred[id];
+        b = q == -2 ? m->z_start + 0.0 : q == -1 ? m->zfull : base[q];
+      } else {
+        b = G[4 * (size_t)nl + k];
+      }
+      base[id] = b;
+      double sv = b;
+      if (m->has_offset) {
+        const double xp = G[1 * (size_t)nl + k], yp = G[3 * (size_t)nl + k];
+        sv += (-m->mdz) * std::sqrt(1 + xp * xp + yp * yp);
+      }
+      G[4 * (size_t)nl + k] = sv;
+    }
+  }
+  for (long k = 0; k < nl; k++)
+    gpass[k] &= ~EH_LPASS_INHERIT;
+}
+
+/* ------------------------------------------------------------------------- */
 /* device storage */
 struct eh_soa {
   double *x, *xp, *y, *yp, *s, *dp;
@@ -726,6 +845,7 @@ struct eh_beam {
   double last_ms = 0; /* the most recent timed launch */
   double main_ms = 0, main_units = 0;
   long main_launches = 0;
+  std::vector<eh_evrec> evlog; /* launches whose losses need settling */
 };
 
 static int g_device = -1;
@@ -838,6 +958,7 @@ extern "C" long eh_beam_upload(eh_beam *b, double **coord, long n_part) {
   b->n_alive = n_part;
   b->n_lost = 0;
   b->event = 0;
+  b->evlog.clear();
   b->pass = 0;
   b->kernel_ms = 0;
   b->launches = 0;
@@ -899,7 +1020,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_plain(eh_soa a, lon
                                                      double xmax, double ymax, double z, double Po, int extrap,
                                                      int32_t pass, int32_t event, unsigned long long *cnt) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  bool lost = false;
+  bool lost = false, inherit = false;
   if (i < n) {
     eh_p6 p = load_p(a, i);
     if (op == 0) {
@@ -914,9 +1035,12 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_plain(eh_soa a, lon
       case EH_T_KQUAD:
       case EH_T_KSEXT:
       case EH_T_KOCT:
-      case EH_T_KQUSE:
-        lost = eh_multipole(p, d, 0);
+      case EH_T_KQUSE: {
+        const int m = eh_multipole(p, d, 0);
+        lost = m != 0;
+        inherit = m == 2;
         break;
+      }
       case EH_T_CSBEND:
         lost = eh_csbend(p, d, 0, terms);
         break;
@@ -949,7 +1073,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_plain(eh_soa a, lon
     }
     store_p(a, i, p);
     if (lost) {
-      a.lpass[i] = pass;
+      a.lpass[i] = pass | (inherit ? EH_LPASS_INHERIT : 0);
       a.levent[i] = event;
     }
   }
@@ -1009,8 +1133,8 @@ __device__ __forceinline__ void track_span(eh_p6 &p, bool &alive, const eh_desc 
       int st = eh_apply_element(p, *d, pp, terms);
       if (st >= 0) {
         alive = false;
-        lpass = pp;
-        levent = event_base + ((pp - p0) * nelem + ee) * 3 + st;
+        lpass = pp | (st & EH_STAGE_INHERIT ? EH_LPASS_INHERIT : 0);
+        levent = event_base + ((pp - p0) * nelem + ee) * 3 + (st & 3);
       }
     }
     if (++ee == nelem) {
@@ -1162,8 +1286,8 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_batch(eh_soa a, eh_
         int st = eh_apply_element(p, *d, pp, terms);
         if (st >= 0) {
           alive = false;
-          lpass = pp;
-          levent = event_base + ((pp - p0) * nelem + ee) * 3 + st;
+          lpass = pp | (st & EH_STAGE_INHERIT ? EH_LPASS_INHERIT : 0);
+          levent = event_base + ((pp - p0) * nelem + ee) * 3 + (st & 3);
         }
       }
       if (++ee == nelem) {
@@ -1279,6 +1403,13 @@ static long run_plain(eh_beam *b, const eh_desc *hd, const std::vector<eh_fterm>
   }
   unsigned long long lost = 0;
   HIPCHK(hipMemcpy(&lost, b->d_cnt + 2, sizeof lost, hipMemcpyDeviceToHost));
+  if (op == 0 && lost && hd) {
+    const eh_kmp k = kmp_of(*hd);
+    if (k.kick) {
+      eh_evrec rec = {ev, 1, 0, 1, std::make_shared<const std::vector<eh_kmp>>(1, k), nullptr};
+      b->evlog.push_back(rec);
+    }
+  }
   long before_lost = b->n_lost;
   long na = compact(b, (long)lost);
   if (na < 0)
@@ -1437,7 +1568,8 @@ extern "C" long eh_limit_amplitudes(eh_beam *b, double xmax, double ymax, long n
 /* Kahan-compensated TMEAN fiducial over the alive particles in reference row
  * order (findFiducialTime, src/simple_rfca.c:179-254; KahanPlus, src/kahan.c:27-35) */
 static long reference_order(eh_beam *b, std::vector<int32_t> &slot_at_row, std::vector<int32_t> &alive_row0,
-                            std::vector<int32_t> &dev_index_alive);
+                            std::vector<int32_t> &dev_index_alive, std::vector<int32_t> *pred = nullptr,
+                            std::vector<int32_t> *order = nullptr);
 static long fiducial_tmean(eh_beam *b, double sOffset, double p0, double *tFid) {
   std::vector<int32_t> slot_at_row, alive_row0, idx;
   long r = reference_order(b, slot_at_row, alive_row0, idx);
@@ -1533,7 +1665,12 @@ extern "C" long eh_removeInvalidParticles(eh_beam *b, long np, double z, double 
  * :294-327).  Within one compaction event the reference scans rows upward and
  * swaps every lost row with the current top row, re-examining the row it just
  * filled; the chain below reproduces that sequence touching only O(lost) rows. */
-static void replay_order(long N, std::vector<std::pair<int32_t, int32_t>> &lost, std::vector<int32_t> &slot_at) {
+static void replay_order(long N, std::vector<std::pair<int32_t, int32_t>> &lost, std::vector<int32_t> &slot_at,
+                         std::vector<int32_t> *pred = nullptr, std::vector<int32_t> *order = nullptr) {
+  if (pred) {
+    pred->assign(N, -1);
+    order->clear();
+  }
   slot_at.resize(N);
   std::vector<int32_t> pos(N);
   for (long r = 0; r < N; r++)
@@ -1562,7 +1699,14 @@ static void replay_order(long N, std::vector<std::pair<int32_t, int32_t>> &lost,
       long i = rows[k];
       if (i > top)
         break;
+      int32_t prev = i == 0 ? -2 : -1; /* rows below i ended with survivors */
       for (;;) {
+        if (pred) {
+          const int32_t cur = slot_at[i];
+          (*pred)[cur] = prev;
+          order->push_back(cur);
+          prev = cur;
+        }
         if (i == top) {
           top--;
           break;
@@ -1589,7 +1733,7 @@ static void replay_order(long N, std::vector<std::pair<int32_t, int32_t>> &lost,
 /* slot_at_row[r] = identity (upload row) of the particle in row r of the
  * reference's arrangement; idx[identity] = index in the alive buffer or -1 */
 static long reference_order(eh_beam *b, std::vector<int32_t> &slot_at_row, std::vector<int32_t> &alive_row0,
-                            std::vector<int32_t> &idx) {
+                            std::vector<int32_t> &idx, std::vector<int32_t> *pred, std::vector<int32_t> *order) {
   long N = b->n_alive + b->n_lost;
   alive_row0.resize(b->n_alive);
   std::vector<int32_t> grow0(b->n_lost), gev(b->n_lost);
@@ -1612,7 +1756,7 @@ static long reference_order(eh_beam *b, std::vector<int32_t> &slot_at_row, std::
       return fail(EH_ERR_STATE, "corrupt particle identity");
     lost[k] = std::make_pair(gev[k], grow0[k]);
   }
-  replay_order(N, lost, slot_at_row);
+  replay_order(N, lost, slot_at_row, pred, order);
   return EH_OK;
 }
 
@@ -1622,8 +1766,8 @@ extern "C" long eh_beam_download(eh_beam *b, double **coord, long n_part) {
   long N = b->n_alive + b->n_lost;
   if (n_part != N)
     return fail(EH_ERR_ARG, "eh_beam_download: n_part=%ld but the beam holds %ld particles", n_part, N);
-  std::vector<int32_t> slot_at_row, alive_row0, idx;
-  long r = reference_order(b, slot_at_row, alive_row0, idx);
+  std::vector<int32_t> slot_at_row, alive_row0, idx, pred, order;
+  long r = reference_order(b, slot_at_row, alive_row0, idx, &pred, &order);
   if (r < 0)
     return r;
   long na = b->n_alive, nl = b->n_lost;
@@ -1641,9 +1785,12 @@ extern "C" long eh_beam_download(eh_beam *b, double **coord, long n_part) {
     HIPCHK(hipMemcpy(grow0.data(), g.row0, sizeof(int32_t) * nl, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(gpass.data(), g.lpass, sizeof(int32_t) * nl, hipMemcpyDeviceToHost));
   }
-  std::vector<int32_t> gidx(N, -1);
+  std::vector<int32_t> gidx(N, -1), gev(nl);
   for (long k = 0; k < nl; k++)
     gidx[grow0[k]] = (int32_t)k;
+  if (nl)
+    HIPCHK(hipMemcpy(gev.data(), g.levent, sizeof(int32_t) * nl, hipMemcpyDeviceToHost));
+  settle_losses(b->evlog, -1, order, pred, gidx, gev, G, nl, gpass);
   for (long row = 0; row < N; row++) {
     int32_t id = slot_at_row[row];
     double *c = coord[row];
@@ -1758,13 +1905,18 @@ static long track_plain(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
   return nToTrack;
 }
 
-static long launch_fused(eh_beam *b, long ne, int p0, int e0, int p1, int e1, double *effort) {
+static long launch_fused(eh_beam *b, long ne, int p0, int e0, int p1, int e1, double *effort,
+                         const eh_kmp_tab &tab) {
   long n = b->n_alive;
   long span = ((long)(p1 - p0) * ne + ne) * 3 + 3;
   if ((long)b->event + span >= 2147483647L)
     return fail(EH_ERR_STATE, "loss event counter overflow");
   int32_t base = b->event;
   b->event += (int32_t)span;
+  if (n > 0 && tab) {
+    eh_evrec rec = {base, (int32_t)span, p0, (int32_t)ne, tab, nullptr};
+    b->evlog.push_back(rec);
+  }
   if (n == 0 || (p0 == p1 && e0 >= e1))
     return b->n_alive;
   unsigned long long init[4] = {0, (unsigned long long)b->n_lost, 0, 0};
@@ -1857,6 +2009,14 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
   r = upload_terms(b, pool);
   if (r < 0)
     return r;
+  /* kick-multipole records for settling loss positions (RF cavities built later
+   * are not kick multipoles) */
+  eh_kmp_tab tab = kmp_table(hd);
+  bool any_kick = false;
+  for (const eh_kmp &k : *tab)
+    any_kick = any_kick || k.kick;
+  if (!any_kick)
+    tab = nullptr;
   int p = 0, e = 0;
   if (n_passes > 0) {
     /* pass 0: stop before each RFCA whose fiducial is still unknown */
@@ -1866,7 +2026,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       EH_RFCA *rf = (EH_RFCA *)bl[ie].p_elem;
       if (rf->fiducial_seen)
         continue;
-      r = launch_fused(b, ne, p, e, 0, (int)ie, effort);
+      r = launch_fused(b, ne, p, e, 0, (int)ie, effort, tab);
       if (r < 0)
         return r;
       r = rfca_fiducial(b, rf, *Pc, ze[ie]);
@@ -1894,7 +2054,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       ee = (int)ne;
       p1 = p + 1;
     }
-    r = launch_fused(b, ne, p, e, pe, ee, effort);
+    r = launch_fused(b, ne, p, e, pe, ee, effort, tab);
     if (r < 0)
       return r;
     p = p1;
@@ -1967,6 +2127,25 @@ extern "C" long eh_replay_order(long n, const int32_t *lost_event, const int32_t
   std::vector<int32_t> s;
   replay_order(n, lost, s);
   std::memcpy(slot_at_row, s.data(), sizeof(int32_t) * n);
+  return EH_OK;
+}
+
+extern "C" long eh_replay_sequence(long n, const int32_t *lost_event, const int32_t *lost_id, long n_lost,
+                                   int32_t *pred, int32_t *order) {
+  if (n < 0 || n_lost < 0 || n_lost > n || (n_lost && (!lost_event || !lost_id)) || !pred || !order)
+    return fail(EH_ERR_ARG, "eh_replay_sequence: bad arguments");
+  std::vector<std::pair<int32_t, int32_t>> lost(n_lost);
+  for (long k = 0; k < n_lost; k++) {
+    if (lost_id[k] < 0 || lost_id[k] >= n)
+      return fail(EH_ERR_ARG, "eh_replay_sequence: identity out of range");
+    lost[k] = std::make_pair(lost_event[k], lost_id[k]);
+  }
+  std::vector<int32_t> s, pr, od;
+  replay_order(n, lost, s, &pr, &od);
+  if ((long)od.size() != n_lost)
+    return fail(EH_ERR_STATE, "eh_replay_sequence: %zu of %ld lost particles sequenced", od.size(), n_lost);
+  std::memcpy(pred, pr.data(), sizeof(int32_t) * n);
+  std::memcpy(order, od.data(), sizeof(int32_t) * n_lost);
   return EH_OK;
 }
 
@@ -2064,6 +2243,9 @@ struct eh_batch {
   /* descriptor tables of the last call, reused while the beamlines are unchanged */
   uint64_t tables_hash = 0;
   long tables_ne = 0;
+  eh_kmp_tab kmp_pool; /* kick-multipole records of the descriptor pool */
+  eh_kmp_idx kmp_idx;  /* variant * ne + element -> pool entry */
+  std::vector<eh_evrec> evlog;
 };
 
 static size_t elem_struct_size(int32_t type) {
@@ -2229,7 +2411,8 @@ extern "C" long eh_batch_n_alive(const eh_batch *b, long v) {
 /* reference row order of variant v (local identities 0..n_v-1); idx[local id] =
  * index in the alive region of v or -1; the graveyard slots of v go to gidx */
 static long batch_order(eh_batch *b, long v, std::vector<int32_t> &slot_at_row, std::vector<int32_t> &idx,
-                        std::vector<int32_t> &gidx) {
+                        std::vector<int32_t> &gidx, std::vector<int32_t> *pred = nullptr,
+                        std::vector<int32_t> *order = nullptr) {
   long nv = b->n_v[v], na = b->alive_v[v], off = b->id_off[v];
   std::vector<int32_t> arow(na > 0 ? na : 1), grow0(b->n_lost > 0 ? b->n_lost : 1),
       gev(b->n_lost > 0 ? b->n_lost : 1);
@@ -2259,15 +2442,15 @@ static long batch_order(eh_batch *b, long v, std::vector<int32_t> &slot_at_row, 
   }
   if ((long)lost.size() + na != nv)
     return fail(EH_ERR_STATE, "variant %ld: %ld alive + %zu lost != %ld", v, na, lost.size(), nv);
-  replay_order(nv, lost, slot_at_row);
+  replay_order(nv, lost, slot_at_row, pred, order);
   return EH_OK;
 }
 
 extern "C" long eh_batch_download(eh_batch *b, long v, double *flat, long n) {
   if (!b || v < 0 || v >= b->nvar || !flat || n != b->n_v[v])
     return fail(EH_ERR_ARG, "eh_batch_download: bad arguments");
-  std::vector<int32_t> slot, idx, gidx;
-  long r = batch_order(b, v, slot, idx, gidx);
+  std::vector<int32_t> slot, idx, gidx, pred, order;
+  long r = batch_order(b, v, slot, idx, gidx, &pred, &order);
   if (r < 0)
     return r;
   long na = b->alive_v[v], nl = b->n_lost;
@@ -2282,8 +2465,12 @@ extern "C" long eh_batch_download(eh_batch *b, long v, double *flat, long n) {
     if (nl)
       HIPCHK(hipMemcpy(G.data() + k * (size_t)nl, gs[k], sizeof(double) * nl, hipMemcpyDeviceToHost));
   }
-  if (nl)
+  std::vector<int32_t> gev(nl > 0 ? nl : 1);
+  if (nl) {
     HIPCHK(hipMemcpy(gpass.data(), g.lpass, sizeof(int32_t) * nl, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(gev.data(), g.levent, sizeof(int32_t) * nl, hipMemcpyDeviceToHost));
+  }
+  settle_losses(b->evlog, v, order, pred, gidx, gev, G, nl, gpass);
   for (long row = 0; row < n; row++) {
     int32_t id = slot[row];
     double *c = flat + row * EH_NPROP;
@@ -2342,6 +2529,10 @@ static long batch_enqueue_span(eh_batch *b, long ne, int p0, int e0, int p1, int
     return fail(EH_ERR_STATE, "loss event counter overflow");
   int32_t base = b->event;
   b->event += (int32_t)span;
+  if (b->kmp_pool) {
+    eh_evrec rec = {base, (int32_t)span, p0, (int32_t)ne, b->kmp_pool, b->kmp_idx};
+    b->evlog.push_back(rec);
+  }
   hipLaunchKernelGGL(k_batch_prep, dim3(1), dim3(64), 0, b->stream, b->d_alive_out, b->d_alive_in, b->d_wave_start,
                      b->d_alive_out, b->d_queue, (int)b->nvar, first);
   eh_batch_dev bd;
@@ -2488,6 +2679,12 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
     long r = batch_tables(beamlines, b->nvar, ne, Pc, pool, terms, idx);
     if (r < 0)
       return r;
+    eh_kmp_tab tab = kmp_table(pool);
+    bool any_kick = false;
+    for (const eh_kmp &k : *tab)
+      any_kick = any_kick || k.kick;
+    b->kmp_pool = any_kick ? tab : nullptr;
+    b->kmp_idx = any_kick ? std::make_shared<const std::vector<int32_t>>(idx) : nullptr;
   }
   auto upload_tables = [&]() -> long {
     b->tables_hash = 0; /* valid again only once this call completes */

@@ -381,6 +381,16 @@ long eh_adjoint_backward(eh_adjoint *adj, const double *grad_final, double *grad
  * e.g. src/limit_amplitudes.c:262-283, for testing) */
 long eh_replay_order(long n, const int32_t *lost_event, const int32_t *lost_id, long n_lost, int32_t *slot_at_row);
 
+/* the sequence in which the reference's element routines process the particles
+ * they lose, for the same loss events (no reference counterpart, for testing):
+ * order[k] = the k-th lost identity processed; pred[id] = -2 if it was the
+ * first particle of its routine call, -1 if a surviving particle was processed
+ * just before it, else the lost identity processed just before it.  It settles
+ * the loss position a kick multipole gives a particle lost at its entry
+ * (src/multipole.c:1160-1232). */
+long eh_replay_sequence(long n, const int32_t *lost_event, const int32_t *lost_id, long n_lost, int32_t *pred,
+                        int32_t *order);
+
 /* GPU evaluation of sin, cos, asin, log, sqrt, division and tan for checking
  * bit-identity with the host (no reference counterpart) */
 long eh_selftest_math(const double *x, const double *y, long n, double *out);

@@ -101,3 +101,21 @@ def test_batch_more_elements_on_pass():
         total += nref
         assert_rows_identical(batch.download(v), ref, batch.n_alive(v), nref, "variant %d" % v)
     assert n == total
+
+
+def test_batch_kick_multipole_entry_losses():
+    """inherited loss positions at kick-multipole entries, settled per variant"""
+    from tests.test_gpu_parity import entry_loss_beam, entry_loss_ring
+    print("batch entry losses", flush=True)
+    sys.stdout.flush()
+    Po = configs.p_central_from_mev(2000.0)
+    lats = [entry_loss_ring(v % 2 == 1) for v in range(3)]
+    beams = [entry_loss_beam(300 + 40 * v, seed=40 + v) for v in range(3)]
+    batch = tracking.Batch(beams)
+    n, _ = batch.track(lats, 3, Po)
+    total = 0
+    for v in range(3):
+        ref, nref, _, _ = oracle.do_tracking(beams[v], entry_loss_ring(v % 2 == 1), 3, Po)
+        total += nref
+        assert_rows_identical(batch.download(v), ref, batch.n_alive(v), nref, "variant %d" % v)
+    assert n == total

@@ -217,6 +217,57 @@ def test_more_elements_ring(mode, ehlib, oracle_mod):
     assert stats["particle_element_turns"] == eff
 
 
+def entry_loss_ring(offset):
+    """a kick multipole first (misaligned or not), so that steep or far particles
+    are lost at its entry"""
+    q = dict(L=0.3, K1=1.2, N_KICKS=8)
+    if offset:
+        q.update(DX=1e-4, DY=-5e-5, DZ=2e-3, TILT=0.01)
+    return Lattice([Element("AP", "MAXAMP", X_MAX=0.02, Y_MAX=0.02), Element("QF", "KQUAD", **q),
+                    Element("D1", "DRIF", L=0.5), Element("SX", "KSEXT", L=0.2, K2=20.0, N_KICKS=4),
+                    Element("D2", "EDRIFT", L=0.5)])
+
+
+def entry_loss_beam(n, seed):
+    """particles lost at the entry of the first kick multipole: row 0, runs of
+    neighbours, the last rows (so that swaps chain through them), one far out
+    (|x| > 10 m), and particles lost inside the element by the aperture"""
+    rows = _beam_rows(n, seed, scale=2.0)
+    rows[0, 1] = 1.5
+    rows[10:14, 1] = [-1.2, 2.0, 1.01, -3.0]
+    rows[20:24, 0] = 0.0195
+    rows[20:24, 1] = 0.01
+    rows[24, 3] = 1.7
+    rows[30, 0] = 11.0
+    rows[n - 6:, 1] = np.linspace(-4.0, 4.0, 6)
+    return rows
+
+
+@pytest.mark.parametrize("offset", [False, True])
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+def test_kick_multipole_entry_losses(mode, offset, ehlib, oracle_mod):
+    """a particle lost at a kick multipole's entry is recorded at z_start plus the
+    dzLoss of the particle processed before it (src/multipole.c:1160-1232), and
+    every loss there gets the exit offset's path-length term; the download
+    settles both in the reference's processing order, bit for bit"""
+    _say("entry losses %s offset=%s" % (mode, offset))
+    lat = entry_loss_ring(offset)
+    rows = entry_loss_beam(600, seed=31)
+    Po = configs.p_central_from_mev(2000.0)
+    got, n, ref, nref, stats, eff = _track_both(lat, rows, 3, Po, mode, tpl=2)
+    assert nref <= len(rows) - 12
+    assert_rows_identical(got, ref, n, nref, "entry losses %s offset=%s" % (mode, offset))
+    # the element entry point alone (plain routine call)
+    lat1 = Lattice([entry_loss_ring(offset).elements[1]])
+    amp = S.EH_MAXAMP(0.02, 0.02, 0, 2)
+    ref1, nref1 = oracle_mod.element(rows, lat1, 0, Po, 1.0, 1.3, maxamp=amp)
+    beam = tracking.Beam(rows)
+    arr, keep = lat1.c_beamline()
+    n1 = ehlib.eh_multipole_tracking2(beam.handle, len(rows), arr, 0.0, Po, 1.0, C.byref(amp), -1)
+    assert n1 >= 0, ehlib.eh_last_error()
+    assert_rows_identical(beam.download(), ref1, n1, nref1, "entry losses, one KQUAD call")
+
+
 @pytest.mark.skip(reason="the segment-major work-queue variant of k_fused stalled on the GPU (a fused FODO run "
                          "did not finish in 120 s) and was taken out; the cause is not shown (see DESIGN.md)")
 @pytest.mark.parametrize("seg", [7, 700, 100000])

@@ -565,3 +565,25 @@ def test_malign_closed_form(oracle_mod):
     r, _ = one(oracle_mod, "MALIGN", dict(DT=1e-11), [start], Po=Po)
     assert r[0, 4] == pytest.approx(s + 1e-11 * _beta(Po, d) * c, rel=1e-14)
     assert r[0, 5] == d
+
+
+def test_kick_multipole_entry_loss_inherits_dzloss(oracle_mod):
+    """integrate_kick_multipole_ordn resets dzLoss only after its entry checks
+    (src/multipole.c:1160-1232), so a particle lost at entry (|x'| > 1 here) is
+    recorded at z_start plus whatever dzLoss the previous particle of the same
+    call left (src/multipole.c:1043): the full length after a survivor, the
+    partial length after a particle lost inside, 0 for the first particle.
+    Hand-worked with L = 0.5, two slices of order-2 drifts (0.125 each)."""
+    q = dict(L=0.5, K1=1.0, N_KICKS=2, INTEGRATION_ORDER=2)
+    amp = S.EH_MAXAMP(0.01, 0.01, 0, 2)
+    S_, E_ = [1e-3, 0, 0, 0, 0, 0], [1e-3, 2.0, 0, 0, 0, 0]
+    M_ = [0.009, 0.01, 0, 0, 0, 0]  # crosses the 10 mm aperture in the first slice
+
+    def s_of(coords):
+        r, n = one(oracle_mod, "KQUAD", q, coords, z0=2.0, maxamp=amp)
+        return {int(row[6]): row[4] for row in r[n:]}
+
+    assert s_of([S_, E_, S_]) == {2: 2.5}            # after a survivor: z_start + L
+    assert s_of([E_, S_]) == {1: 2.0}                # first particle: z_start + 0
+    assert s_of([S_, E_, E_]) == {2: 2.5, 3: 2.5}    # a chain keeps the survivor's value
+    assert s_of([S_, M_, E_]) == {2: 2.25, 3: 2.25}  # after a particle lost at 0.25 m
