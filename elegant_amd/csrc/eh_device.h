@@ -293,6 +293,41 @@ EH_HD bool eh_m2s(double &xp, double &yp, double qx, double qy, double dp, int e
   return true;
 }
 
+/* the reciprocal pair of b for eh_round_trip, formed per particle: rh = RN(1/b)
+ * by a correctly rounded division and rl = RN(e*rh) with e = 1 - b*rh exact (one
+ * fma).  rh + rl is then within ~2^-104 of 1/b relative, enough for the first
+ * quotient of Markstein's method to be within one ulp.  False (use a division)
+ * unless b lies in [2^-500, 2^500]. */
+EH_HD bool eh_recip_pair(double b, double &rh, double &rl) {
+  const double ab = eh_fabs(b);
+  if (!(ab >= 0x1p-500 && ab <= 0x1p500))
+    return false;
+  rh = 1.0 / b;
+  rl = eh_fma(-b, rh, 1.0) * rh;
+  return true;
+}
+
+/* RN(RN(a / b) * b): what the reference's radiation block leaves on a canonical
+ * momentum when no radiation is on (it divides by 1 + delta and multiplies back,
+ * src/multipole.c:1362-1385, src/csbend.c:1645-1658).  The quotient is correctly
+ * rounded through the pair (Markstein, as eh_div_recip); a zero numerator comes
+ * back unchanged, as a/b*b does for finite non-zero b; other numerators outside
+ * [2^-960, 2^960], or no pair, take the division. */
+EH_HD double eh_round_trip(double a, double b, double rh, double rl, bool ok) {
+  const double aa = eh_fabs(a);
+  if (ok && aa >= 0x1p-960 && aa <= 0x1p960) {
+    const double q0 = eh_fma(a, rh, a * rl);
+    const double r = eh_fma(-q0, b, a);
+    return eh_fma(r, rh, q0) * b;
+  }
+  if (a == 0)
+    return a;
+#if defined(__HIP_DEVICE_COMPILE__)
+  __asm__ volatile("");
+#endif
+  return a / b * b;
+}
+
 /* entry face of a misaligned multipole: offsetBeamCoordinatesForMisalignment
  * (dx, dy, dz) then rotateBeamCoordinatesForMisalignment (tilt) */
 EH_HD void eh_mface1(eh_p6 &p, const eh_desc &d) {
@@ -359,6 +394,10 @@ EH_HD int eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
       qx = (1 + dp) * xp / denom;
       qy = (1 + dp) * yp / denom;
     }
+    /* 1 + delta and its reciprocal pair for the radiation block's round trip */
+    const double b1 = 1 + dp;
+    double rh = 0, rl = 0;
+    const bool rok = d.mdrift != 0 && eh_recip_pair(b1, rh, rl);
     if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
       lost = entry = true;
     } else {
@@ -397,6 +436,19 @@ EH_HD int eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
             eh_mkick<0>(qx, qy, w, d.xk[step], 0);
           if (d.has_ykick)
             eh_mkick<0>(qx, qy, w, d.yk[step], 1);
+          if (d.mdrift != 0) {
+            /* convertMomentaToSlopes, whose slopes the radiation block's second
+             * conversion replaces: only its failure test remains; then the
+             * block's round trip of the momenta (src/multipole.c:1356-1390; it
+             * runs for any element with length: isr_coef is non-zero even
+             * without ISR, :757-760) */
+            if (!d.expand && eh_sqr(1 + dp) - eh_sqr(qx) - eh_sqr(qy) <= 0) {
+              lost = true;
+              break;
+            }
+            qx = eh_round_trip(qx, b1, rh, rl, rok);
+            qy = eh_round_trip(qy, b1, rh, rl, rok);
+          }
           if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
             lost = true;
             break;
@@ -800,6 +852,7 @@ EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restr
   yp = -p.xp * d.sin_tt + p.yp * d.cos_tt;
   s = p.s;
   dp = p.dp;
+  const double dp0 = dp;
   if (d.csb_edge1) {
     double rho = (1 + dp) * rho_actual;
     double delta_xp = d.tan_e1 / rho * x;
@@ -813,7 +866,11 @@ EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restr
     QX = xp * f;
     QY = yp * f;
   }
-  /* integrate_csbend_ordn */
+  /* integrate_csbend_ordn; 1 + DPoP and its reciprocal pair for the radiation
+   * block's round trip after each kick */
+  const double b1 = 1 + DP;
+  double rh = 0, rl = 0;
+  const bool rok = eh_recip_pair(b1, rh, rl);
   double dist = 0, dz_lost = 0;
   bool lost = false;
   for (int i = 0; i < d.csb_nslices && !lost; i++) {
@@ -862,6 +919,10 @@ EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restr
         QX += -ds * (1 + X / rho0) * Fy / rho_actual;
         QY += ds * (1 + X / rho0) * Fx / rho_actual;
       }
+      /* the radiation block runs for every bend (src/csbend.c:1627-1660;
+       * isrConstant is non-zero even without ISR, :961-964) */
+      QX = eh_round_trip(QX, b1, rh, rl, rok);
+      QY = eh_round_trip(QY, b1, rh, rl, rok);
     }
   }
   if (!lost) {
@@ -887,7 +948,13 @@ EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restr
     p.dp = d.Po * (1 + DP);
     return true;
   }
-  s += S4;
+  {
+    /* time-of-flight rescaling (src/csbend.c:1285-1296, taken for every bend):
+     * without radiation delta is unchanged, so beta1 = beta0 */
+    const double P0 = d.Po * (1 + dp0);
+    const double beta = P0 / eh_sqrt(eh_sqr(P0) + 1);
+    s = beta * s / beta + S4;
+  }
   x = X;
   xp = QX;
   y = Y;

@@ -15,6 +15,13 @@
  *
  * Parameter subset: see include/elegant_hip.h.  Unsupported values set an error
  * and make the routine return a negative count.
+ *
+ * Parity unpinned: the round trips the reference's radiation blocks leave when
+ * no radiation is on -- q / (1 + delta) * (1 + delta) after every kick of a
+ * kick multipole or bend and beta * s / beta at a bend's exit -- are the
+ * identity in exact arithmetic, so no closed form or invariant can check them;
+ * they are restated from the reference's text (cited where they are applied)
+ * and change results only in the last bit.
  */
 #include <float.h>
 #include <math.h>
@@ -463,6 +470,18 @@ static int or_integrate_kick_multipole(double *coord, double dx, double dy, doub
         or_kick(&qx, &qy, xpow, ypow, 0, -ykick * kickFrac[step], 1);
       if (!momenta_to_slopes(&xp, &yp, qx, qy, dp, expand))
         return 0;
+      /* the radiation block runs whenever the element has length
+       * (src/multipole.c:1360-1390): isr_coef is non-zero even without ISR
+       * (negative, :757-760); without radiation it leaves qx, qy divided by
+       * and multiplied by (1 + dp), and converts them to slopes again */
+      if (drift) {
+        qx /= (1 + dp);
+        qy /= (1 + dp);
+        qx *= (1 + dp);
+        qy *= (1 + dp);
+        if (!momenta_to_slopes(&xp, &yp, qx, qy, dp, expand))
+          return 0;
+      }
     }
   }
   if (!or_check_ap(x + dx, y + dy, apData)) {
@@ -960,6 +979,13 @@ static long or_integrate_csbend(double *Qf, const double *Qi, double s, long n, 
       or_csbend_fields_eval(F, &Fx, &Fy, x, y);
       QX += -ds * (1 + X / rho0) * Fy / rho_actual;
       QY += ds * (1 + X / rho0) * Fx / rho_actual;
+      /* the radiation block runs for every bend (src/csbend.c:1627-1660):
+       * isrConstant is non-zero even without ISR (negative, :961-964); without
+       * radiation it leaves QX, QY divided by and multiplied by (1 + DPoP) */
+      QX /= (1 + DPoP);
+      QY /= (1 + DPoP);
+      QX *= (1 + DPoP);
+      QY *= (1 + DPoP);
     }
   }
   *dz_lost = n * s;
@@ -993,7 +1019,7 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
                              const EH_MAXAMP *maxamp) {
   double b[9], c[9], h, rho0, rho_actual, fse, angle, e1, e2, tilt, etilt, ttilt, cos_ttilt, sin_ttilt;
   double psi1, psi2, Kg1, Kg2, dxi, dyi, dzi, dxf, dyf, dzf, dz_lost, rho, delta_xp;
-  double x, xp, y, yp, s, dp, Qi[6], Qf[6], *coord;
+  double x, xp, y, yp, s, dp, dp0, Qi[6], Qf[6], *coord;
   long i_part, i_top, j, particle_lost;
   or_csbend_fields F;
   or_apdata ap;
@@ -1117,7 +1143,7 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
     xp = coord[1] * cos_ttilt + coord[3] * sin_ttilt;
     yp = -coord[1] * sin_ttilt + coord[3] * cos_ttilt;
     s = coord[4];
-    dp = coord[5];
+    dp = dp0 = coord[5];
 
     if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) && csbend->edge1_effects == 1) {
       rho = (1 + dp) * rho_actual;
@@ -1150,7 +1176,14 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
       i_part--;
       continue;
     }
-    s += Qf[4];
+    {
+      /* the time-of-flight rescaling of src/csbend.c:1285-1296 runs for every
+       * bend (rad_coef || isrConstant, isrConstant non-zero): without radiation
+       * beta1 = beta0, leaving the round trip beta0 * s / beta0 */
+      double p0 = Po * (1 + dp0), beta0 = p0 / sqrt(sqr(p0) + 1);
+      double p1 = Po * (1 + Qf[5]), beta1 = p1 / sqrt(sqr(p1) + 1);
+      s = beta1 * s / beta0 + Qf[4];
+    }
     x = Qf[0];
     xp = Qf[1];
     y = Qf[2];
