@@ -5,8 +5,9 @@ executes for one surviving particle counts as one flop; negation, fabs and
 comparisons count zero; each call of sin, cos, asin or log counts as one
 transcendental (T).  The "algorithmic" figure used for the roofline adds T as
 one flop each, so it is a lower bound on the work the hardware has to do.
-Counts follow the reference code paths cited per function (no radiation, no
-error multipoles, no fringe, misalignment method 0).
+Counts follow the reference code paths cited per function (radiation off, no
+error multipoles, no fringe, misalignment method 0); the radiation blocks the
+reference runs with radiation off are counted.
 """
 import math
 
@@ -51,6 +52,12 @@ def multipole(etype, p):
     if _get(p, "VKICK") or _get(p, "ykick"):
         kick += 1 + 2 + 4
     kick += 9  # momenta->slopes
+    if _get(p, "L"):
+        # the radiation block, run for any element with length even without
+        # radiation (src/multipole.c:1360-1390): the delta_q sums of the kicks,
+        # the round trip of qx, qy through 1 + dp, F2, the path-length factors
+        # and a second momenta->slopes conversion
+        kick += 4 * len(orders) + 39
     f += ns * (4 + nsub * drift + (nsub - 1) * kick) + 4 + 9 + 1
     if _get(p, "DX") or _get(p, "DY") or _get(p, "DZ"):
         f += 2 * 13
@@ -77,9 +84,11 @@ def csbend(p):
     else:
         field = 0
     drift = 31
-    kick = 1 + field + 12
+    # the kick, then the radiation block every bend runs (src/csbend.c:1627-1660):
+    # slopes and path-length factor, F2, the round trip of QX, QY through 1 + DPoP
+    kick = 1 + field + 12 + 36
     per_slice = 2 + nsub * drift + (nsub - 1) * kick
-    f = ns * per_slice + 80
+    f = ns * per_slice + 80 + 15  # + the time-of-flight rescaling (:1285-1296)
     t = ns * nsub * 4
     return f, t
 
