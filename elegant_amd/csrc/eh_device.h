@@ -107,6 +107,14 @@ typedef struct {
   double mal_dxp, mal_dyp, mal_dt, mal_dp, mal_de; /* MALIGN (offsets in mdx, mdy, mdz) */
   int32_t mal_energy;                   /* MALIGN: dt, dp or de non-zero */
 
+  /* ---- classical radiation (SYNCH_RAD) ---- */
+  int32_t rad;                 /* rad_coef != 0: the radiating instantiation of the map */
+  double rad_coef;
+  double kfrac[EH_MAXSUB];     /* multipoles: the integrator's kick fractions */
+  double rad_xk, rad_yk;       /* multipoles: xkick, ykick / n_parts */
+  double csb_irho0;            /* bends: 1./rho0 as the radiation block forms it */
+  int32_t csb_e1r, csb_e2r;    /* bends: the entry/exit delta adjustments apply */
+
   /* ---- adjoint (reverse-mode) extras ---- */
   int32_t adj_order0;              /* order of the main multipole term, also when its strength is 0 */
   int32_t adj_pk, adj_pdx, adj_pdy; /* gradient slots of strength, DX, DY (-1: not a parameter) */
@@ -260,6 +268,48 @@ EH_HD void eh_mkick(double &qx, double &qy, const eh_pow4 &w, double K, int skew
   qy += K * sum_Fx;
 }
 
+/* the kick with its increments accumulated into dqx, dqy
+ * (apply_canonical_multipole_kicks_ret, src/multipole.h:391-420) */
+template <int ORDER>
+EH_HD void eh_mkick_ret(double &qx, double &qy, double &dqx, double &dqy, const eh_pow4 &w, double K, int skew) {
+  double sum_Fx = 0, sum_Fy = 0;
+  eh_mterm<ORDER, 0>(w, sum_Fx, sum_Fy);
+  if (ORDER >= 1)
+    eh_mterm<ORDER, (ORDER >= 1 ? 1 : 0)>(w, sum_Fx, sum_Fy);
+  if (ORDER >= 2)
+    eh_mterm<ORDER, (ORDER >= 2 ? 2 : 0)>(w, sum_Fx, sum_Fy);
+  if (ORDER >= 3)
+    eh_mterm<ORDER, (ORDER >= 3 ? 3 : 0)>(w, sum_Fx, sum_Fy);
+  if (skew) {
+    double t = sum_Fx;
+    sum_Fx = sum_Fy;
+    sum_Fy = t;
+    sum_Fx = -sum_Fx;
+  }
+  qx -= K * sum_Fy;
+  qy += K * sum_Fx;
+  dqx -= K * sum_Fy;
+  dqy += K * sum_Fx;
+}
+
+EH_HD void eh_mkick_ret_n(int order, double &qx, double &qy, double &dqx, double &dqy, const eh_pow4 &w, double K,
+                          int skew) {
+  switch (order) {
+  case 0:
+    eh_mkick_ret<0>(qx, qy, dqx, dqy, w, K, skew);
+    break;
+  case 1:
+    eh_mkick_ret<1>(qx, qy, dqx, dqy, w, K, skew);
+    break;
+  case 2:
+    eh_mkick_ret<2>(qx, qy, dqx, dqy, w, K, skew);
+    break;
+  default:
+    eh_mkick_ret<3>(qx, qy, dqx, dqy, w, K, skew);
+    break;
+  }
+}
+
 EH_HD void eh_mkick_n(int order, double &qx, double &qy, const eh_pow4 &w, double K, int skew) {
   switch (order) {
   case 0:
@@ -374,10 +424,15 @@ EH_HD void eh_mface2(eh_p6 &p, const eh_desc &d, bool s_too) {
  * step (its loss position is z_start plus the previous particle's dzLoss).  A
  * lost particle keeps s = z_start + dzLoss without the exit offset's path-length
  * term. */
-EH_HD int eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
+template <bool RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh_desc &d, int pi) {
   eh_mface1(p, d);
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
   double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, dp = p.dp, qx, qy;
+  double beta0 = 0;
+  if constexpr (RAD) {
+    const double P = d.Po * (1 + dp);
+    beta0 = P / eh_sqrt(eh_sqr(P) + 1);
+  }
   double dzLoss = 0;
   bool lost = false, entry = false;
   int how = 0; /* 0: coord untouched, 1: x,y written, 2: all written */
@@ -429,14 +484,39 @@ EH_HD int eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
           w.y2 = w.y1 * y;
           w.x3 = w.x2 * x;
           w.y3 = w.y2 * y;
+          double dqx = 0, dqy = 0;
           for (int io = 0; io < 3; io++)
-            if (d.mp_order[io])
-              eh_mkick_n(d.mp_order[io], qx, qy, w, d.kick[step][io], d.mp_skew[io]);
+            if (d.mp_order[io]) {
+              if constexpr (RAD)
+                eh_mkick_ret_n(d.mp_order[io], qx, qy, dqx, dqy, w, d.kick[step][io], d.mp_skew[io]);
+              else
+                eh_mkick_n(d.mp_order[io], qx, qy, w, d.kick[step][io], d.mp_skew[io]);
+            }
           if (d.has_xkick)
             eh_mkick<0>(qx, qy, w, d.xk[step], 0);
           if (d.has_ykick)
             eh_mkick<0>(qx, qy, w, d.yk[step], 1);
-          if (d.mdrift != 0) {
+          if constexpr (RAD) {
+            /* convertMomentaToSlopes, then the radiation block with rad_coef
+             * (src/multipole.c:1356-1390): the kick increments without
+             * kickFrac give the field, the energy loss follows */
+            if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
+              lost = true;
+              break;
+            }
+            const double drift = d.mdrift, kf = d.kfrac[step];
+            qx /= (1 + dp);
+            qy /= (1 + dp);
+            const double deltaFactor = eh_sqr(1 + dp);
+            dqx /= kf;
+            dqy /= kf;
+            const double F2 = eh_sqr(dqx / drift - d.rad_xk / drift) + eh_sqr(dqy / drift + d.rad_yk / drift);
+            double dsFactor = eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
+            dsFactor *= drift * kf;
+            dp -= d.rad_coef * deltaFactor * F2 * dsFactor;
+            qx *= (1 + dp);
+            qy *= (1 + dp);
+          } else if (d.mdrift != 0) {
             /* convertMomentaToSlopes, whose slopes the radiation block's second
              * conversion replaces: only its failure test remains; then the
              * block's round trip of the momenta (src/multipole.c:1356-1390; it
@@ -478,7 +558,14 @@ EH_HD int eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
     p.xp = xp;
     p.y = y;
     p.yp = yp;
-    p.s += s;
+    if constexpr (RAD) {
+      /* src/multipole.c:1436-1440 */
+      const double P = d.Po * (1 + dp);
+      const double beta1 = P / eh_sqrt(eh_sqr(P) + 1);
+      p.s = beta1 * (p.s / beta0 + 2 * s / (beta0 + beta1));
+    } else {
+      p.s += s;
+    }
     p.dp = dp;
   }
   if (lost) {
@@ -489,12 +576,16 @@ EH_HD int eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
   return lost ? (entry ? 2 : 1) : 0;
 }
 
+EH_HD int eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
+  return d.rad ? eh_multipole_t<true>(p, d, pi) : eh_multipole_t<false>(p, d, pi);
+}
+
 /* ------------------------------------------------------------------------- */
-/* one MULT kick of order N: fillPowerArray for x and y, then the terms
- * (coef[i]*x^(N-i))*y^i summed in index order, odd i into Fx, even i into Fy,
- * and qx -= K*Fy, qy += K*Fx (src/multipole.c:578-590) */
+/* the field sums of one MULT kick of order N: fillPowerArray for x and y, then
+ * the terms (coef[i]*x^(N-i))*y^i summed in index order, odd i into Fx, even i
+ * into Fy (src/multipole.c:578-586); the kick is qx -= K*Fy, qy += K*Fx */
 template <int N>
-EH_HD void eh_mult_kick(double &qx, double &qy, double x, double y, const double *coef, double K) {
+EH_HD void eh_mult_sums(double x, double y, const double *coef, double &sum_Fx, double &sum_Fy) {
   double xp[N + 1], yp[N + 1];
   xp[0] = 1;
   yp[0] = 1;
@@ -503,7 +594,8 @@ EH_HD void eh_mult_kick(double &qx, double &qy, double x, double y, const double
     xp[i] = xp[i - 1] * x;
     yp[i] = yp[i - 1] * y;
   }
-  double sum_Fx = 0, sum_Fy = 0;
+  sum_Fx = 0;
+  sum_Fy = 0;
 #pragma unroll
   for (int i = 0; i <= N; i++) {
     if (i & 1)
@@ -511,38 +603,36 @@ EH_HD void eh_mult_kick(double &qx, double &qy, double x, double y, const double
     else
       sum_Fy += coef[i] * xp[N - i] * yp[i];
   }
-  qx -= K * sum_Fy;
-  qy += K * sum_Fx;
 }
 
-EH_HD void eh_mult_kick_n(int order, double &qx, double &qy, double x, double y, const double *coef, double K) {
+EH_HD void eh_mult_sums_n(int order, double x, double y, const double *coef, double &Fx, double &Fy) {
   switch (order) {
   case 0:
-    eh_mult_kick<0>(qx, qy, x, y, coef, K);
+    eh_mult_sums<0>(x, y, coef, Fx, Fy);
     break;
   case 1:
-    eh_mult_kick<1>(qx, qy, x, y, coef, K);
+    eh_mult_sums<1>(x, y, coef, Fx, Fy);
     break;
   case 2:
-    eh_mult_kick<2>(qx, qy, x, y, coef, K);
+    eh_mult_sums<2>(x, y, coef, Fx, Fy);
     break;
   case 3:
-    eh_mult_kick<3>(qx, qy, x, y, coef, K);
+    eh_mult_sums<3>(x, y, coef, Fx, Fy);
     break;
   case 4:
-    eh_mult_kick<4>(qx, qy, x, y, coef, K);
+    eh_mult_sums<4>(x, y, coef, Fx, Fy);
     break;
   case 5:
-    eh_mult_kick<5>(qx, qy, x, y, coef, K);
+    eh_mult_sums<5>(x, y, coef, Fx, Fy);
     break;
   case 6:
-    eh_mult_kick<6>(qx, qy, x, y, coef, K);
+    eh_mult_sums<6>(x, y, coef, Fx, Fy);
     break;
   case 7:
-    eh_mult_kick<7>(qx, qy, x, y, coef, K);
+    eh_mult_sums<7>(x, y, coef, Fx, Fy);
     break;
   default:
-    eh_mult_kick<8>(qx, qy, x, y, coef, K);
+    eh_mult_sums<8>(x, y, coef, Fx, Fy);
     break;
   }
 }
@@ -555,7 +645,7 @@ EH_HD bool eh_mult_out(double x, double xp, double y, double yp) {
 }
 
 /* MULT: multipole_tracking (src/multipole.c:439-658) for one particle, without
- * BORE/BTIPL and without radiation.  n_slices kicks of strength kick[0][0]
+ * BORE/BTIPL, with classical radiation.  n_slices kicks of strength kick[0][0]
  * (KnL*FACTOR/N_SLICES) between drifts of mdrift (L/N_SLICES/2, doubled between
  * kicks).  A particle outside the limits on entry keeps its coordinates; one
  * lost in the kicks keeps the values reached; both get s = z_start and
@@ -565,6 +655,8 @@ EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d) {
   eh_mface1(p, d);
   const double drift = d.mdrift;
   double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, dp = p.dp, qx, qy;
+  const double P0 = d.Po * (1 + dp);
+  const double beta0 = d.rad ? P0 / eh_sqrt(eh_sqr(P0) + 1) : 0.0;
   bool lost;
   if (eh_mult_out(x, xp, y, yp)) {
     lost = true;
@@ -587,10 +679,27 @@ EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d) {
         else
           s += (ik ? 2 : 1) * drift * eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
       }
-      eh_mult_kick_n(d.mp_order[0], qx, qy, x, y, d.mult_coef, d.kick[0][0]);
+      double sum_Fx, sum_Fy;
+      const double K = d.kick[0][0];
+      eh_mult_sums_n(d.mp_order[0], x, y, d.mult_coef, sum_Fx, sum_Fy);
+      qx -= K * sum_Fy;
+      qy += K * sum_Fx;
       if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
         is_lost = true;
         break;
+      }
+      if (d.rad && drift) {
+        /* classical radiation, src/multipole.c:599-611 */
+        qx /= (1 + dp);
+        qy /= (1 + dp);
+        dp -= d.rad_coef * eh_sqr(K * (1 + dp)) * (eh_sqr(sum_Fy) + eh_sqr(sum_Fx)) *
+              eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp)) / (2 * drift);
+        qx *= (1 + dp);
+        qy *= (1 + dp);
+        if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
+          is_lost = true;
+          break;
+        }
       }
     }
     if (drift && !is_lost) {
@@ -606,7 +715,14 @@ EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d) {
     p.y = y;
     p.yp = yp;
     p.dp = dp;
-    p.s += s;
+    if (d.rad) {
+      /* src/multipole.c:623-628 */
+      const double P1 = d.Po * (1 + dp);
+      const double beta1 = P1 / eh_sqrt(eh_sqr(P1) + 1);
+      p.s = beta1 * (p.s / beta0 + 2 * s / (beta0 + beta1));
+    } else {
+      p.s += s;
+    }
     lost = is_lost || eh_mult_out(x, xp, y, yp);
   }
   if (lost) {
@@ -839,7 +955,8 @@ EH_HD double eh_div_recip(double a, double b, double rh, double rl) {
 /* CSBEND: track_through_csbend (src/csbend.c:1088-1403) for one particle, with
  * integrate_csbend_ordn (src/csbend.c:1459-1720) and the canonical conversions
  * (:1436-1454).  Returns true if lost. */
-EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms) {
+template <bool RAD>
+EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms) {
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
   const double rho0 = d.rho0, rho_actual = d.rho_actual;
   double x, xp, y, yp, s, dp;
@@ -859,8 +976,19 @@ EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restr
     xp += delta_xp;
     yp -= eh_tan(d.e1 - d.psi1 / (1 + dp)) / rho * y;
   }
+  double dpi = dp;
+  if constexpr (RAD) {
+    if (d.csb_e1r) {
+      /* pre-adjust delta for the sector integration, src/csbend.c:1214-1222 */
+      double Fx, Fy;
+      eh_csb_fields(d, terms, x, y, Fx, Fy);
+      const double dp_prime = -d.rad_coef * (eh_sqr(Fx) + eh_sqr(Fy)) * eh_sqr(1 + dp) *
+                              eh_sqrt(eh_sqr(1 + x / rho0) + eh_sqr(xp) + eh_sqr(yp));
+      dpi -= dp_prime * x * d.tan_e1;
+    }
+  }
   /* convertToDipoleCanonicalCoordinates */
-  double X = x, QX, Y = y, QY, DP = dp, S4 = 0;
+  double X = x, QX, Y = y, QY, DP = dpi, S4 = 0;
   {
     double f = (1 + DP) / eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
     QX = xp * f;
@@ -870,7 +998,7 @@ EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restr
    * block's round trip after each kick */
   const double b1 = 1 + DP;
   double rh = 0, rl = 0;
-  const bool rok = eh_recip_pair(b1, rh, rl);
+  const bool rok = !RAD && eh_recip_pair(b1, rh, rl);
   double dist = 0, dz_lost = 0;
   bool lost = false;
   for (int i = 0; i < d.csb_nslices && !lost; i++) {
@@ -921,8 +1049,23 @@ EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restr
       }
       /* the radiation block runs for every bend (src/csbend.c:1627-1660;
        * isrConstant is non-zero even without ISR, :961-964) */
-      QX = eh_round_trip(QX, b1, rh, rl, rok);
-      QY = eh_round_trip(QY, b1, rh, rl, rok);
+      if constexpr (RAD) {
+        /* with rad_coef: the energy loss of the sub-step (src/csbend.c:1627-1651) */
+        const double h1 = 1 + X * d.csb_irho0;
+        const double fr = h1 / eh_sqrt(eh_sqr(1 + DP) - eh_sqr(QX) - eh_sqr(QY));
+        const double xpr = QX * fr, ypr = QY * fr;
+        const double dsFactor = eh_sqrt(eh_sqr(1 + X * d.csb_irho0) + eh_sqr(xpr) + eh_sqr(ypr));
+        const double F2 = eh_sqr(Fx) + eh_sqr(Fy);
+        const double deltaFactor = eh_sqr(1 + DP);
+        QX /= (1 + DP);
+        QY /= (1 + DP);
+        DP -= d.rad_coef * deltaFactor * F2 * ds * dsFactor;
+        QX *= (1 + DP);
+        QY *= (1 + DP);
+      } else {
+        QX = eh_round_trip(QX, b1, rh, rl, rok);
+        QY = eh_round_trip(QY, b1, rh, rl, rok);
+      }
     }
   }
   if (!lost) {
@@ -948,7 +1091,20 @@ EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restr
     p.dp = d.Po * (1 + DP);
     return true;
   }
-  {
+  if constexpr (RAD) {
+    if (d.csb_e2r) {
+      /* post-adjust delta, src/csbend.c:1270-1282 */
+      double Fx, Fy;
+      eh_csb_fields(d, terms, X, Y, Fx, Fy);
+      const double dp_prime = -d.rad_coef * (eh_sqr(Fx) + eh_sqr(Fy)) * eh_sqr(1 + DP) *
+                              eh_sqrt(eh_sqr(1 + X / rho0) + eh_sqr(QX) + eh_sqr(QY));
+      DP -= dp_prime * X * d.tan_e2;
+    }
+    /* time-of-flight rescaling, src/csbend.c:1285-1296 */
+    const double P0 = d.Po * (1 + dp0), P1 = d.Po * (1 + DP);
+    const double beta0 = P0 / eh_sqrt(eh_sqr(P0) + 1), beta1 = P1 / eh_sqrt(eh_sqr(P1) + 1);
+    s = beta1 * s / beta0 + S4;
+  } else {
     /* time-of-flight rescaling (src/csbend.c:1285-1296, taken for every bend):
      * without radiation delta is unchanged, so beta1 = beta0 */
     const double P0 = d.Po * (1 + dp0);
@@ -976,6 +1132,10 @@ EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restr
   p.y += d.dyf + d.dzf * p.yp;
   p.s += d.dzf * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
   return false;
+}
+
+EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms) {
+  return d.rad ? eh_csbend_t<true>(p, d, pi, terms) : eh_csbend_t<false>(p, d, pi, terms);
 }
 
 /* ------------------------------------------------------------------------- */

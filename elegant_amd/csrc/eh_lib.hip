@@ -99,6 +99,16 @@ static void rot_cs(double angle, double *c, double *s) {
   }
 }
 
+static double hsqr(double x) { return x * x; }
+static double hpow3(double x) { return x * x * x; }
+
+/* rad_coef of multipole_tracking and multipole_tracking2 (src/multipole.c:501,
+ * :756), the bend's with sqr(1 + fse)/sqr(rho0) (src/csbend.c:957); the same
+ * expression, constant for constant, as the oracle */
+static double rad_coef_multipole(double Po) {
+  return hsqr(EH_E_MKS) * hpow3(Po) / (6 * EH_PI * EH_EPSILON_O * hsqr(EH_C_MKS) * EH_ME_MKS);
+}
+
 /* rotateBeamCoordinatesForMisalignment leaves the beam alone for angle 0 and
  * for |angle| within 1e-12 of 2*pi (src/malign_mat.c) */
 static int tilt_on(double angle) { return angle != 0 && !(std::fabs(std::fabs(angle) - 2 * EH_PI) < 1e-12); }
@@ -176,10 +186,11 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
   int skew[3] = {0, 0, 0};
   double dx = 0, dy = 0, dz = 0, tilt = 0, xkick = 0, ykick = 0, drift = 0;
   long nSlices = 0, n_kicks = 0, integ_order = 0;
-  int expand = 0;
+  int expand = 0, synch_rad = 0;
   switch (e->type) {
   case EH_T_KQUAD: {
     const EH_KQUAD *q = (const EH_KQUAD *)e->p_elem;
+    synch_rad = q->synch_rad;
     nSlices = q->nSlices;
     n_kicks = q->n_kicks;
     expand = q->expandHamiltonian;
@@ -197,6 +208,7 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
   }
   case EH_T_KSEXT: {
     const EH_KSEXT *q = (const EH_KSEXT *)e->p_elem;
+    synch_rad = q->synch_rad;
     nSlices = q->nSlices;
     n_kicks = q->n_kicks;
     expand = q->expandHamiltonian;
@@ -223,6 +235,7 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
   }
   case EH_T_KOCT: {
     const EH_KOCT *q = (const EH_KOCT *)e->p_elem;
+    synch_rad = q->synch_rad;
     nSlices = q->nSlices;
     n_kicks = q->n_kicks;
     expand = q->expandHamiltonian;
@@ -239,6 +252,7 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
   case EH_T_KQUSE: {
     /* src/multipole.c:923-949 */
     const EH_KQUSE *q = (const EH_KQUSE *)e->p_elem;
+    synch_rad = q->synch_rad;
     nSlices = q->nSlices;
     n_kicks = q->n_kicks;
     expand = q->expandHamiltonian;
@@ -291,7 +305,13 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
                                     : e->type == EH_T_KQUSE ? ((const EH_KQUSE *)e->p_elem)->fse1
                                                             : ((const EH_KOCT *)e->p_elem)->fse)) /
                       nSlices;
+  /* classical radiation: none for elements shorter than 1e-6 m (src/multipole.c:761-763) */
+  d->rad_coef = synch_rad && drift >= 1e-6 ? rad_coef_multipole(c.Po) : 0.0;
+  d->rad = d->rad_coef != 0;
+  d->rad_xk = xk;
+  d->rad_yk = yk;
   for (int s = 0; s < nsub; s++) {
+    d->kfrac[s] = kickFrac[s];
     d->adj_kfac[s] = kfac * kickFrac[s];
     d->dsh[s] = mdrift * driftFrac[s];
     for (int i = 0; i < 3; i++)
@@ -392,6 +412,12 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
   if (std::fabs(fse + 1) < 1e-12)
     fse = -1 + 1e-12;
   double rho_actual = 1 / ((1 + fse) * h);
+  /* classical radiation, src/csbend.c:956-959 (the oracle's expression) */
+  d->rad_coef = cb->synch_rad ? hsqr(EH_E_MKS) * hpow3(c.Po) * hsqr(1 + fse) /
+                                    (6 * EH_PI * EH_EPSILON_O * hsqr(EH_C_MKS) * EH_ME_MKS * hsqr(rho0))
+                              : 0.0;
+  d->rad = d->rad_coef != 0;
+  d->csb_irho0 = 1. / rho0;
   double sgn = rho0 < 0 ? -1.0 : (rho0 > 0 ? 1.0 : 0.0);
   double Kg1 = 2 * cb->hgap * cb->fint * sgn;
   double psi1 = Kg1 / std::fabs(rho_actual) / eh_cos(e1) * (1 + eh_sqr(eh_sin(e1)));
@@ -503,6 +529,8 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
   d->csb_dzl = cb->nSlices * s;
   d->csb_edge1 = (cb->edgeFlags & EH_BEND_EDGE1_EFFECTS) && cb->edge1_effects == 1;
   d->csb_edge2 = (cb->edgeFlags & EH_BEND_EDGE2_EFFECTS) && cb->edge2_effects == 1;
+  d->csb_e1r = (cb->edgeFlags & EH_BEND_EDGE1_EFFECTS) && e1 != 0;
+  d->csb_e2r = (cb->edgeFlags & EH_BEND_EDGE2_EFFECTS) && e2 != 0;
   d->e1 = e1;
   d->e2 = e2;
   d->tan_e1 = eh_tan(e1);
@@ -557,7 +585,7 @@ static long build_rfca(eh_desc *d, const EH_RFCA *rf) {
  * misalignment faces and the expansion coefficients of its order
  * (expansion_coefficients, src/multipole.h:76-145); a zero strength is the
  * exact drift the reference takes instead */
-static long build_mult(eh_desc *d, const EH_MULT *m) {
+static long build_mult(eh_desc *d, const EH_MULT *m, double Po) {
   if (m->nSlices <= 0)
     return fail(EH_ERR_UNSUPPORTED, "N_SLICES (or N_KICKS) <=0  in multipole_tracking()");
   if (m->order < 0)
@@ -566,8 +594,6 @@ static long build_mult(eh_desc *d, const EH_MULT *m) {
     return fail(EH_ERR_UNSUPPORTED, "MULT order above %d not supported", EH_MULT_MAXORDER);
   if (m->bore)
     return fail(EH_ERR_UNSUPPORTED, "MULT with BORE/BTIPL not supported");
-  if (m->synch_rad)
-    return fail(EH_ERR_UNSUPPORTED, "MULT with SYNCH_RAD not supported");
   const long nSlices = m->nSlices;
   const double drift = m->length / nSlices / 2;
   const double KnL = m->KnL * m->factor / nSlices;
@@ -581,6 +607,8 @@ static long build_mult(eh_desc *d, const EH_MULT *m) {
   d->nslices = (int32_t)nSlices;
   d->expand = m->expandHamiltonian;
   d->mdrift = drift;
+  d->rad_coef = m->synch_rad ? rad_coef_multipole(Po) : 0.0;
+  d->rad = d->rad_coef != 0;
   d->kick[0][0] = KnL;
   d->mp_order[0] = m->order;
   const int n = m->order;
@@ -688,7 +716,7 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
     d->length = ((const EH_RFCA *)e->p_elem)->length;
     return build_rfca(d, (const EH_RFCA *)e->p_elem);
   case EH_T_MULT:
-    return build_mult(d, (const EH_MULT *)e->p_elem);
+    return build_mult(d, (const EH_MULT *)e->p_elem, c.Po);
   case EH_T_ROTATE:
     return build_rotate(d, (const EH_ROTATE *)e->p_elem);
   case EH_T_MALIGN:
@@ -1487,7 +1515,7 @@ extern "C" long eh_multipole_tracking(eh_beam *b, long n_part, const EH_MULT *mu
   d.Po = Po;
   d.z_start = z_start;
   d.z_end = z_start + mult->length;
-  r = build_mult(&d, mult);
+  r = build_mult(&d, mult, Po);
   if (r < 0)
     return r;
   if (d.type == EH_T_MULT && d.nslices == 0)
@@ -3263,6 +3291,8 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
       }
       if (t == EH_T_MULT || t == EH_T_ROTATE || t == EH_T_MALIGN)
         return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support MULT, ROTATE or MALIGN", e);
+      if (d.rad)
+        return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support SYNCH_RAD", e);
       if ((t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE) && d.expand)
         return fail(EH_ERR_UNSUPPORTED, "element %ld: adjoint needs EXPAND_HAMILTONIAN=0", e);
       for (int k = 0; k < 3; k++) {

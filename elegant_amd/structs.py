@@ -40,23 +40,23 @@ class EH_EDRIFT(C.Structure):
 class EH_KQUAD(C.Structure):
     _fields_ = [(n, dbl) for n in ("length", "k1", "tilt", "dx", "dy", "dz", "fse", "xkick", "ykick",
                                    "xKickCalibration", "yKickCalibration")] + \
-               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian")]
+               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian", "synch_rad")]
 
 
 class EH_KSEXT(C.Structure):
     _fields_ = [(n, dbl) for n in ("length", "k2", "k1", "j1", "tilt", "dx", "dy", "dz", "fse", "xkick", "ykick",
                                    "xKickCalibration", "yKickCalibration")] + \
-               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian")]
+               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian", "synch_rad")]
 
 
 class EH_KOCT(C.Structure):
     _fields_ = [(n, dbl) for n in ("length", "k3", "tilt", "dx", "dy", "dz", "fse")] + \
-               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian")]
+               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian", "synch_rad")]
 
 
 class EH_KQUSE(C.Structure):
     _fields_ = [(n, dbl) for n in ("length", "k1", "k2", "tilt", "dx", "dy", "dz", "fse1", "fse2")] + \
-               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian")]
+               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian", "synch_rad")]
 
 
 class EH_CSBEND(C.Structure):
@@ -67,7 +67,7 @@ class EH_CSBEND(C.Structure):
                                    "b1", "b2", "b3", "b4", "b5", "b6", "b7", "b8")] + \
                [(n, i32) for n in ("nSlices", "nonlinear", "edge1_effects", "edge2_effects", "edge_order",
                                    "integration_order", "expandHamiltonian", "use_bn", "expansionOrder",
-                                   "etiltSign", "malignMethod", "edgeFlags")]
+                                   "etiltSign", "malignMethod", "edgeFlags", "synch_rad")]
 
 
 class EH_RFCA(C.Structure):
@@ -130,22 +130,22 @@ DEFAULTS = {
 
 # elegant parameter names -> struct fields where they differ
 ALIASES = {
-    T_KQUAD: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "HKICK": "xkick", "VKICK": "ykick",
+    T_KQUAD: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "HKICK": "xkick", "VKICK": "ykick", "SYNCH_RAD": "synch_rad",
               "HCALIBRATION": "xKickCalibration", "VCALIBRATION": "yKickCalibration",
               "INTEGRATION_ORDER": "integration_order", "EXPAND_HAMILTONIAN": "expandHamiltonian"},
-    T_KSEXT: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "HKICK": "xkick", "VKICK": "ykick",
+    T_KSEXT: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "HKICK": "xkick", "VKICK": "ykick", "SYNCH_RAD": "synch_rad",
               "HCALIBRATION": "xKickCalibration", "VCALIBRATION": "yKickCalibration",
               "INTEGRATION_ORDER": "integration_order", "EXPAND_HAMILTONIAN": "expandHamiltonian"},
-    T_KOCT: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "INTEGRATION_ORDER": "integration_order",
+    T_KOCT: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "INTEGRATION_ORDER": "integration_order", "SYNCH_RAD": "synch_rad",
              "EXPAND_HAMILTONIAN": "expandHamiltonian"},
-    T_KQUSE: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "INTEGRATION_ORDER": "integration_order",
+    T_KQUSE: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "INTEGRATION_ORDER": "integration_order", "SYNCH_RAD": "synch_rad",
               "EXPAND_HAMILTONIAN": "expandHamiltonian"},
     T_CSBEND: {"N_KICKS": "nSlices", "N_SLICES": "nSlices", "XKICK": "xKick", "YKICK": "yKick",
                "FSE_DIPOLE": "fseDipole", "FSE_QUADRUPOLE": "fseQuadrupole",
                "EDGE1_EFFECTS": "edge1_effects", "EDGE2_EFFECTS": "edge2_effects", "EDGE_ORDER": "edge_order",
                "INTEGRATION_ORDER": "integration_order", "EXPAND_HAMILTONIAN": "expandHamiltonian",
                "USE_BN": "use_bn", "EXPANSION_ORDER": "expansionOrder", "ETILT_SIGN": "etiltSign",
-               "MALIGN_METHOD": "malignMethod", "NONLINEAR": "nonlinear"},
+               "MALIGN_METHOD": "malignMethod", "NONLINEAR": "nonlinear", "SYNCH_RAD": "synch_rad"},
     T_RFCA: {"N_KICKS": "nKicks", "CHANGE_P0": "change_p0", "CHANGE_T": "change_t", "END1_FOCUS": "end1Focus",
              "END2_FOCUS": "end2Focus", "STANDING_WAVE": "standingWave", "LINEARIZE": "linearize",
              "LOCK_PHASE": "lockPhase", "T_REFERENCE": "tReference", "VOLT": "volt"},

@@ -82,6 +82,7 @@ typedef struct {
   int32_t n_kicks, nSlices;                               /* N_KICKS (deprecated alias path), N_SLICES */
   int32_t integration_order;                              /* 2, 4 or 6 */
   int32_t expandHamiltonian;
+  int32_t synch_rad; /* SYNCH_RAD: classical radiation (rad_coef, src/multipole.c:755-763) */
 } EH_KQUAD;
 
 /* KSEXT: src/track.h:2149-2166, defaults src/track_data.c:1125 */
@@ -91,6 +92,7 @@ typedef struct {
   int32_t n_kicks, nSlices;
   int32_t integration_order;
   int32_t expandHamiltonian;
+  int32_t synch_rad; /* SYNCH_RAD: classical radiation (rad_coef, src/multipole.c:755-763) */
 } EH_KSEXT;
 
 /* KOCT: src/track.h:2171-2182, defaults src/track_data.c:1169 (N_KICKS=4) */
@@ -99,6 +101,7 @@ typedef struct {
   int32_t n_kicks, nSlices;
   int32_t integration_order;
   int32_t expandHamiltonian;
+  int32_t synch_rad; /* SYNCH_RAD: classical radiation (rad_coef, src/multipole.c:755-763) */
 } EH_KOCT;
 
 /* KQUSE (quadrupole with a sextupole as secondary multipole): src/track.h:3498-3503,
@@ -108,11 +111,12 @@ typedef struct {
   int32_t n_kicks, nSlices;
   int32_t integration_order;
   int32_t expandHamiltonian;
+  int32_t synch_rad; /* SYNCH_RAD: classical radiation (rad_coef, src/multipole.c:755-763) */
 } EH_KQUSE;
 
 /* MULT (thin multipole kicks between drifts): src/track.h:1407-1412, parameters
- * src/track_data.c:486-502, tracking src/multipole.c:439-658.  BORE/BTIPL and
- * SYNCH_RAD are not supported (EH_ERR_UNSUPPORTED). */
+ * src/track_data.c:486-502, tracking src/multipole.c:439-658, SYNCH_RAD
+ * included.  BORE/BTIPL and ORDER > 8 are not supported (EH_ERR_UNSUPPORTED). */
 typedef struct {
   double length, KnL, tilt, bore, BTipL, dx, dy, dz, factor;
   int32_t order, synch_rad, expandHamiltonian;
@@ -157,6 +161,7 @@ typedef struct {
   int32_t etiltSign;          /* default 1 */
   int32_t malignMethod;       /* 0 supported */
   int32_t edgeFlags;          /* EH_BEND_EDGE1/2 as determined for the beamline */
+  int32_t synch_rad;          /* SYNCH_RAD: classical radiation (src/csbend.c:956-959) */
 } EH_CSBEND;
 
 #define EH_BEND_EDGE1_EFFECTS 1
@@ -196,6 +201,9 @@ typedef struct {
 /* particle charge/mass constants used on the path (electrons, CODATA 2018) */
 #define EH_C_MKS 2.99792458e8
 #define EH_ME_MEV 0.51099895000
+#define EH_E_MKS 1.602176634e-19      /* particleCharge */
+#define EH_ME_MKS 9.1093837015e-31    /* particleMass */
+#define EH_EPSILON_O 8.8541878128e-12 /* epsilon_o */
 
 const char *eh_last_error(void);
 

@@ -58,6 +58,8 @@
 #define SLOPE_LIMIT 1.0  /* src/track.h:1250, src/track_data.c:51 */
 
 static double sqr(double x) { return x * x; }
+/* SDDS's pow3 is not vendored: (x*x)*x here and in the library (unpinned) */
+static double pow3(double x) { return x * x * x; }
 
 static char or_err[256];
 const char *or_last_error(void) { return or_err; }
@@ -313,7 +315,17 @@ static void or_coefs(long n, double *coef) {
   }
 }
 
+static void or_kick_ret(double *qx, double *qy, double *dqx, double *dqy, const double *xpow, const double *ypow,
+                        long order, double KnL, long skew);
+
 static void or_kick(double *qx, double *qy, const double *xpow, const double *ypow, long order, double KnL, long skew) {
+  or_kick_ret(qx, qy, NULL, NULL, xpow, ypow, order, KnL, skew);
+}
+
+/* the kick with the increments accumulated into dqx, dqy when given
+ * (apply_canonical_multipole_kicks_ret, src/multipole.h:391-420) */
+static void or_kick_ret(double *qx, double *qy, double *dqx, double *dqy, const double *xpow, const double *ypow,
+                        long order, double KnL, long skew) {
   double coef[16], sum_Fx = 0, sum_Fy = 0, tmp;
   long i;
   or_coefs(order, coef);
@@ -332,6 +344,10 @@ static void or_kick(double *qx, double *qy, const double *xpow, const double *yp
   }
   *qx -= KnL * sum_Fy;
   *qy += KnL * sum_Fx;
+  if (dqx) {
+    *dqx -= KnL * sum_Fy;
+    *dqy += KnL * sum_Fx;
+  }
 }
 
 static void fillPowerArray(double x, double *xpow, long order) {
@@ -405,10 +421,11 @@ static int or_integrator_tables(long order, const double **drift, const double *
 /* integrate_kick_multipole_ordn, src/multipole.c:1099-1456, for i_part = -1, no
  * radiation, no error/edge/steering multipoles, non-radial */
 static int or_integrate_kick_multipole(double *coord, double dx, double dy, double xkick, double ykick, double Po,
+                                       double rad_coef,
                                        const long *order, const double *KnL, const short *skew, long n_parts,
                                        double drift, long integration_order, const or_apdata *apData, double *dzLoss,
                                        int expand) {
-  double qx, qy, s, dp;
+  double qx, qy, s, dp, p, beta0, beta1, dqx = 0, dqy = 0;
   double x, y, xp, yp, xpow[16], ypow[16], KnLp[3];
   long i_kick, step, iOrder, maxOrder, nSubsteps, i;
   double dsh;
@@ -425,6 +442,8 @@ static int or_integrate_kick_multipole(double *coord, double dx, double dy, doub
   yp = coord[3];
   s = 0;
   dp = coord[5];
+  p = Po * (1 + dp);
+  beta0 = p / sqrt(sqr(p) + 1);
 
   if (isnan(x) || isnan(xp) || isnan(y) || isnan(yp))
     return 0;
@@ -461,9 +480,10 @@ static int or_integrate_kick_multipole(double *coord, double dx, double dy, doub
         break;
       fillPowerArray(x, xpow, maxOrder);
       fillPowerArray(y, ypow, maxOrder);
+      dqx = dqy = 0;
       for (iOrder = 0; iOrder < 3; iOrder++)
         if (KnL[iOrder])
-          or_kick(&qx, &qy, xpow, ypow, order[iOrder], KnLp[iOrder] * kickFrac[step], skew[iOrder]);
+          or_kick_ret(&qx, &qy, &dqx, &dqy, xpow, ypow, order[iOrder], KnLp[iOrder] * kickFrac[step], skew[iOrder]);
       if (xkick)
         or_kick(&qx, &qy, xpow, ypow, 0, -xkick * kickFrac[step], 0);
       if (ykick)
@@ -475,8 +495,20 @@ static int or_integrate_kick_multipole(double *coord, double dx, double dy, doub
        * (negative, :757-760); without radiation it leaves qx, qy divided by
        * and multiplied by (1 + dp), and converts them to slopes again */
       if (drift) {
+        double deltaFactor, F2, dsFactor;
         qx /= (1 + dp);
         qy /= (1 + dp);
+        deltaFactor = sqr(1 + dp);
+        /* the kick increments carry kickFrac[step]; remove it */
+        dqx /= kickFrac[step];
+        dqy /= kickFrac[step];
+        F2 = sqr(dqx / drift - xkick / drift) + sqr(dqy / drift + ykick / drift);
+        dqx = 0;
+        dqy = 0;
+        dsFactor = sqrt(1 + sqr(xp) + sqr(yp));
+        dsFactor *= drift * kickFrac[step];
+        if (rad_coef)
+          dp -= rad_coef * deltaFactor * F2 * dsFactor;
         qx *= (1 + dp);
         qy *= (1 + dp);
         if (!momenta_to_slopes(&xp, &yp, qx, qy, dp, expand))
@@ -495,7 +527,12 @@ static int or_integrate_kick_multipole(double *coord, double dx, double dy, doub
   coord[1] = xp;
   coord[2] = y;
   coord[3] = yp;
-  coord[4] += s;
+  if (rad_coef) {
+    p = Po * (1 + dp);
+    beta1 = p / sqrt(sqr(p) + 1);
+    coord[4] = beta1 * (coord[4] / beta0 + 2 * s / (beta0 + beta1));
+  } else
+    coord[4] += s;
   coord[5] = dp;
   if (isnan(x) || isnan(xp) || isnan(y) || isnan(yp))
     return 0;
@@ -511,8 +548,8 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
   double KnL[3] = {0, 0, 0};
   long order[3] = {0, 0, 0};
   short skew[3] = {0, 0, 0};
-  double dx = 0, dy = 0, dz = 0, tilt = 0, xkick = 0, ykick = 0, drift = 0, dzLoss = 0;
-  long nSlices = 0, n_kicks = 0, integ_order = 0, i_part, i_top;
+  double dx = 0, dy = 0, dz = 0, tilt = 0, xkick = 0, ykick = 0, drift = 0, dzLoss = 0, rad_coef = 0;
+  long nSlices = 0, n_kicks = 0, integ_order = 0, i_part, i_top, synch_rad = 0;
   int expand = 0;
   or_apdata ap;
 
@@ -532,6 +569,7 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
     xkick = q->xkick * q->xKickCalibration;
     ykick = q->ykick * q->yKickCalibration;
     integ_order = q->integration_order;
+    synch_rad = q->synch_rad;
     break;
   }
   case EH_T_KSEXT: {
@@ -558,6 +596,7 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
       order[2] = 1;
       skew[2] = 1;
     }
+    synch_rad = q->synch_rad;
     break;
   }
   case EH_T_KOCT: {
@@ -573,6 +612,7 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
     dy = q->dy;
     dz = q->dz;
     integ_order = q->integration_order;
+    synch_rad = q->synch_rad;
     break;
   }
   case EH_T_KQUSE: {
@@ -591,6 +631,7 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
     integ_order = q->integration_order;
     KnL[1] = q->k2 * q->length * (1 + q->fse2);
     order[1] = 2;
+    synch_rad = q->synch_rad;
     break;
   }
   default:
@@ -600,6 +641,10 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
     return or_fail("multipole integration_order must be 2, 4, or 6");
   if (maxamp && maxamp->elliptical)
     return or_fail("elliptical MAXAMP not supported");
+  /* classical radiation, src/multipole.c:755-763 (and :827-836, :888-897,
+   * :938-947): none for elements shorter than 1e-6 m */
+  if (synch_rad && drift >= 1e-6)
+    rad_coef = sqr(EH_E_MKS) * pow3(Po) / (6 * PI * EH_EPSILON_O * sqr(c_mks) * EH_ME_MKS);
 
   i_top = n_part - 1;
   if (n_kicks <= 0) {
@@ -623,7 +668,8 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
     rotateBeamCoordinatesForMisalignment(particle, n_part, tilt + 0.0);
 
   for (i_part = 0; i_part <= i_top; i_part++) {
-    if (!or_integrate_kick_multipole(particle[i_part], dx, dy, xkick, ykick, Po, order, KnL, skew, nSlices, drift,
+    if (!or_integrate_kick_multipole(particle[i_part], dx, dy, xkick, ykick, Po, rad_coef, order, KnL, skew, nSlices,
+                                     drift,
                                      integ_order, &ap, &dzLoss, expand)) {
       swapParticles(particle[i_part], particle[i_top]);
       particle[i_top][4] = z_start + dzLoss;
@@ -652,7 +698,7 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
  * rows passed in and the exit ones on all of them, lost rows included. */
 long or_multipole_tracking(double **particle, long n_part, const EH_MULT *multipole, double Po, double z_start) {
   long order = multipole->order, nSlices = multipole->nSlices, i_part, i_kick, i_top, is_lost;
-  double KnL, drift, qx, qy, x, xp, y, yp, s, dp, xpow[16], ypow[16];
+  double KnL, drift, qx, qy, x, xp, y, yp, s, dp, xpow[16], ypow[16], rad_coef, p, beta0, beta1;
   const long expand = multipole->expandHamiltonian;
   if (nSlices <= 0)
     return or_fail("N_SLICES (or N_KICKS) <=0  in multipole_tracking()");
@@ -662,8 +708,6 @@ long or_multipole_tracking(double **particle, long n_part, const EH_MULT *multip
     return or_fail("MULT order above 8 not supported");
   if (multipole->bore)
     return or_fail("MULT with BORE/BTIPL not supported");
-  if (multipole->synch_rad)
-    return or_fail("MULT with SYNCH_RAD not supported");
   drift = multipole->length / nSlices / 2;
   KnL = multipole->KnL * multipole->factor / nSlices;
   if (KnL == 0) {
@@ -672,6 +716,10 @@ long or_multipole_tracking(double **particle, long n_part, const EH_MULT *multip
     or_exactDrift(particle, n_part, multipole->length);
     return n_part;
   }
+  if (multipole->synch_rad)
+    rad_coef = sqr(EH_E_MKS) * pow3(Po) / (6 * PI * EH_EPSILON_O * sqr(c_mks) * EH_ME_MKS);
+  else
+    rad_coef = 0;
   if (multipole->dx || multipole->dy || multipole->dz)
     offsetBeamCoordinatesForMisalignment(particle, n_part, multipole->dx, multipole->dy, multipole->dz);
   if (multipole->tilt)
@@ -685,6 +733,8 @@ long or_multipole_tracking(double **particle, long n_part, const EH_MULT *multip
     yp = coord[3];
     s = 0;
     dp = coord[5];
+    p = Po * (1 + dp);
+    beta0 = p / sqrt(sqr(p) + 1);
     if (isnan(x) || isnan(xp) || isnan(y) || isnan(yp) || fabs(x) > 10.0 || fabs(y) > 10.0 || fabs(xp) > 1.0 ||
         fabs(yp) > 1.0) {
       swapParticles(particle[i_part], particle[i_top]);
@@ -712,6 +762,28 @@ long or_multipole_tracking(double **particle, long n_part, const EH_MULT *multip
         is_lost = 1;
         break;
       }
+      if (rad_coef && drift) {
+        /* src/multipole.c:599-611, with the kick's field sums */
+        double coef[16], sum_Fx = 0, sum_Fy = 0;
+        long i;
+        or_coefs(order, coef);
+        for (i = 0; i <= order; i++) {
+          if (i & 1)
+            sum_Fx += coef[i] * xpow[order - i] * ypow[i];
+          else
+            sum_Fy += coef[i] * xpow[order - i] * ypow[i];
+        }
+        qx /= (1 + dp);
+        qy /= (1 + dp);
+        dp -= rad_coef * sqr(KnL * (1 + dp)) * (sqr(sum_Fy) + sqr(sum_Fx)) * sqrt(1 + sqr(xp) + sqr(yp)) /
+              (2 * drift);
+        qx *= (1 + dp);
+        qy *= (1 + dp);
+        if (!momenta_to_slopes(&xp, &yp, qx, qy, dp, expand)) {
+          is_lost = 1;
+          break;
+        }
+      }
     }
     if (drift && !is_lost) {
       x += xp * drift;
@@ -726,7 +798,12 @@ long or_multipole_tracking(double **particle, long n_part, const EH_MULT *multip
     coord[2] = y;
     coord[3] = yp;
     coord[5] = dp;
-    coord[4] += s;
+    if (rad_coef) {
+      p = Po * (1 + dp);
+      beta1 = p / sqrt(sqr(p) + 1);
+      coord[4] = beta1 * (coord[4] / beta0 + 2 * s / (beta0 + beta1));
+    } else
+      coord[4] += s;
     if (isnan(x) || isnan(xp) || isnan(y) || isnan(yp) || fabs(x) > 10.0 || fabs(y) > 10.0 || fabs(xp) > 1.0 ||
         fabs(yp) > 1.0 || is_lost) {
       swapParticles(particle[i_part], particle[i_top]);
@@ -931,7 +1008,7 @@ long or_csbend_field_table(const double *b, const double *c, double h, int nonli
  * trajectory correction, iSlice = -1) */
 static long or_integrate_csbend(double *Qf, const double *Qi, double s, long n, double rho0, double rho_actual,
                                 double *dz_lost, const or_apdata *apData, long integration_order,
-                                const or_csbend_fields *F) {
+                                const or_csbend_fields *F, double rad_coef) {
   long i, j, nSubsteps;
   double factor, f, phi, ds, dsh, dist, Fx, Fy, x, y, sine, cosi, tang, sin_phi, cos_phi;
   const double *driftFrac, *kickFrac;
@@ -982,10 +1059,19 @@ static long or_integrate_csbend(double *Qf, const double *Qi, double s, long n, 
       /* the radiation block runs for every bend (src/csbend.c:1627-1660):
        * isrConstant is non-zero even without ISR (negative, :961-964); without
        * radiation it leaves QX, QY divided by and multiplied by (1 + DPoP) */
-      QX /= (1 + DPoP);
-      QY /= (1 + DPoP);
-      QX *= (1 + DPoP);
-      QY *= (1 + DPoP);
+      {
+        double fr = (1 + X * (1. / rho0)) / sqrt(sqr(1 + DPoP) - sqr(QX) - sqr(QY));
+        double xpr = QX * fr, ypr = QY * fr;
+        double dsFactor = sqrt(sqr(1 + X * (1. / rho0)) + sqr(xpr) + sqr(ypr));
+        double F2 = sqr(Fx) + sqr(Fy);
+        double deltaFactor = sqr(1 + DPoP);
+        QX /= (1 + DPoP);
+        QY /= (1 + DPoP);
+        if (rad_coef)
+          DPoP -= rad_coef * deltaFactor * F2 * ds * dsFactor;
+        QX *= (1 + DPoP);
+        QY *= (1 + DPoP);
+      }
     }
   }
   *dz_lost = n * s;
@@ -1019,7 +1105,7 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
                              const EH_MAXAMP *maxamp) {
   double b[9], c[9], h, rho0, rho_actual, fse, angle, e1, e2, tilt, etilt, ttilt, cos_ttilt, sin_ttilt;
   double psi1, psi2, Kg1, Kg2, dxi, dyi, dzi, dxf, dyf, dzf, dz_lost, rho, delta_xp;
-  double x, xp, y, yp, s, dp, dp0, Qi[6], Qf[6], *coord;
+  double x, xp, y, yp, s, dp, dp0, Qi[6], Qf[6], *coord, rad_coef;
   long i_part, i_top, j, particle_lost;
   or_csbend_fields F;
   or_apdata ap;
@@ -1096,6 +1182,11 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
   if (fabs(fse + 1) < 1e-12)
     fse = -1 + 1e-12;
   rho_actual = 1 / ((1 + fse) * h);
+  /* classical radiation, src/csbend.c:956-959 */
+  if (csbend->synch_rad)
+    rad_coef = sqr(EH_E_MKS) * pow3(Po) * sqr(1 + fse) / (6 * PI * EH_EPSILON_O * sqr(c_mks) * EH_ME_MKS * sqr(rho0));
+  else
+    rad_coef = 0;
   Kg1 = 2 * csbend->hgap * csbend->fint * SIGN(rho0);
   psi1 = Kg1 / fabs(rho_actual) / OR_COS(e1) * (1 + sqr(OR_SIN(e1)));
   Kg2 = 2 * csbend->hgap * csbend->fint * SIGN(rho0);
@@ -1157,9 +1248,16 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
     Qi[3] = yp;
     Qi[4] = 0;
     Qi[5] = dp;
+    if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) && e1 != 0 && rad_coef) {
+      /* pre-adjust delta for the sector integration, src/csbend.c:1214-1222 */
+      double Fx, Fy, dp_prime;
+      or_csbend_fields_eval(&F, &Fx, &Fy, x, y);
+      dp_prime = -rad_coef * (sqr(Fx) + sqr(Fy)) * sqr(1 + dp) * sqrt(sqr(1 + x / rho0) + sqr(xp) + sqr(yp));
+      Qi[5] -= dp_prime * x * OR_TAN(e1);
+    }
     toDipoleCanonical(Qi);
     particle_lost = !or_integrate_csbend(Qf, Qi, csbend->length, csbend->nSlices, rho0, rho_actual, &dz_lost, &ap,
-                                         csbend->integration_order, &F);
+                                         csbend->integration_order, &F, rad_coef);
     fromDipoleCanonical(Qf);
     if (particle_lost) {
       double X0 = Qf[0], XP0 = Qf[1], Y0 = Qf[2], YP0 = Qf[3];
@@ -1175,6 +1273,14 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
       i_top--;
       i_part--;
       continue;
+    }
+    if ((csbend->edgeFlags & EH_BEND_EDGE2_EFFECTS) && e2 != 0 && rad_coef) {
+      /* post-adjust delta, src/csbend.c:1270-1282 */
+      double Fx, Fy, dp_prime;
+      or_csbend_fields_eval(&F, &Fx, &Fy, Qf[0], Qf[2]);
+      dp_prime = -rad_coef * (sqr(Fx) + sqr(Fy)) * sqr(1 + Qf[5]) *
+                 sqrt(sqr(1 + Qf[0] / rho0) + sqr(Qf[1]) + sqr(Qf[3]));
+      Qf[5] -= dp_prime * Qf[0] * OR_TAN(e2);
     }
     {
       /* the time-of-flight rescaling of src/csbend.c:1285-1296 runs for every

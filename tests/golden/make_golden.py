@@ -37,6 +37,9 @@ CASES = {
                                                                  DP=1e-3, ON_PASS=1),
                       Element("D", "DRIF", L=0.5), Element("R0", "ROTATE", TILT=-0.3, ON_PASS=0),
                       Element("ME", "MALIGN", DE=1e-4, DT=1e-12)],
+    "radiation": [Element("Q", "KQUAD", L=0.3, K1=1.2, N_KICKS=8, SYNCH_RAD=1),
+                  Element("B", "CSBEND", L=0.8, ANGLE=0.04, K1=0.3, N_KICKS=8, E1=0.02, E2=0.01, SYNCH_RAD=1),
+                  Element("M", "MULT", L=0.1, KNL=30.0, ORDER=2, N_SLICES=2, SYNCH_RAD=1)],
 }
 
 

@@ -119,3 +119,20 @@ def test_batch_kick_multipole_entry_losses():
         total += nref
         assert_rows_identical(batch.download(v), ref, batch.n_alive(v), nref, "variant %d" % v)
     assert n == total
+
+
+def test_batch_radiation():
+    """radiating variants (different SYNCH_RAD rings) through the batched path"""
+    from tests.test_gpu_parity import _beam_rows, radiation_ring
+    print("batch radiation", flush=True)
+    sys.stdout.flush()
+    Po = configs.p_central_from_mev(1000.0)
+    beams = [_beam_rows(300 + 100 * v, seed=50 + v) for v in range(2)]
+    batch = tracking.Batch(beams)
+    n, _ = batch.track([radiation_ring(), radiation_ring()], 3, Po)
+    total = 0
+    for v in range(2):
+        ref, nref, _, _ = oracle.do_tracking(beams[v], radiation_ring(), 3, Po)
+        total += nref
+        assert_rows_identical(batch.download(v), ref, batch.n_alive(v), nref, "variant %d" % v)
+    assert n == total

@@ -90,6 +90,13 @@ ELEMENT_CASES = [
     ("ROTATE", dict(TILT=math.pi)),
     ("MALIGN", dict(DX=1e-4, DY=-2e-4, DZ=3e-3, DXP=1e-5, DYP=-2e-5)),
     ("MALIGN", dict(DZ=-1e-3, DP=1e-3, DE=-2e-4, DT=3e-12)),
+    ("KQUAD", dict(L=0.3, K1=1.2, N_KICKS=8, SYNCH_RAD=1, DX=1e-4, TILT=0.01, HKICK=1e-5, VKICK=-2e-5)),
+    ("KSEXT", dict(L=0.2, K2=-40.0, K1=0.05, J1=0.02, N_KICKS=4, SYNCH_RAD=1)),
+    ("KQUSE", dict(L=0.25, K1=1.4, K2=-35.0, N_KICKS=8, SYNCH_RAD=1, EXPAND_HAMILTONIAN=1)),
+    ("CSBEND", dict(L=0.8, ANGLE=-0.05, K1=0.3, K2=2.0, N_KICKS=8, E1=0.02, E2=0.01, HGAP=0.01, DX=1e-4,
+                    DZ=2e-4, FSE=1e-3, SYNCH_RAD=1)),
+    ("CSBEND", dict(L=1.0, ANGLE=2 * math.pi / 20, N_KICKS=8, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0, SYNCH_RAD=1)),
+    ("MULT", dict(L=0.2, KNL=400.0, ORDER=3, N_SLICES=4, TILT=0.02, DX=1e-4, SYNCH_RAD=1)),
 ]
 
 
@@ -266,6 +273,37 @@ def test_kick_multipole_entry_losses(mode, offset, ehlib, oracle_mod):
     n1 = ehlib.eh_multipole_tracking2(beam.handle, len(rows), arr, 0.0, Po, 1.0, C.byref(amp), -1)
     assert n1 >= 0, ehlib.eh_last_error()
     assert_rows_identical(beam.download(), ref1, n1, nref1, "entry losses, one KQUAD call")
+
+
+def radiation_ring():
+    """a FODO ring with bends, all radiating (SYNCH_RAD=1), and an RF cavity"""
+    from elegant_amd.lattice import Element as E
+    els = [E("AP", "MAXAMP", X_MAX=0.02, Y_MAX=0.02)]
+    for c in range(4):
+        els += [E("QF", "KQUAD", L=0.3, K1=1.1, N_KICKS=8, SYNCH_RAD=1),
+                E("D", "DRIF", L=0.4),
+                E("B", "CSBEND", L=1.0, ANGLE=2 * math.pi / 8, K1=-0.1, N_KICKS=8, E1=0.05, E2=0.05, SYNCH_RAD=1),
+                E("S", "KSEXT", L=0.2, K2=5.0, N_KICKS=4, SYNCH_RAD=1),
+                E("QD", "KQUAD", L=0.3, K1=-1.1, N_KICKS=8, SYNCH_RAD=1),
+                E("B", "CSBEND", L=1.0, ANGLE=2 * math.pi / 8, N_KICKS=8, SYNCH_RAD=1),
+                E("M", "MULT", L=0.1, KNL=50.0, ORDER=2, N_SLICES=2, SYNCH_RAD=1)]
+    return Lattice(els)
+
+
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+def test_radiation_ring(mode, ehlib, oracle_mod):
+    """classical radiation in KQUAD, KSEXT, CSBEND (with its edge delta
+    adjustments) and MULT over several turns, bit for bit with the oracle"""
+    _say("radiation ring %s" % mode)
+    lat = radiation_ring()
+    rows = _beam_rows(2000, seed=17, scale=2.0)
+    rows[:20, 0] = np.linspace(-0.03, 0.03, 20)
+    Po = configs.p_central_from_mev(1000.0)
+    got, n, ref, nref, stats, eff = _track_both(lat, rows, 6, Po, mode, tpl=2)
+    assert 0 < nref < len(rows)
+    assert ref[:nref, 5].mean() < -1e-4  # the beam has lost energy
+    assert_rows_identical(got, ref, n, nref, "radiation " + mode)
+    assert stats["particle_element_turns"] == eff
 
 
 @pytest.mark.skip(reason="the segment-major work-queue variant of k_fused stalled on the GPU (a fused FODO run "

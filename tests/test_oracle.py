@@ -587,3 +587,50 @@ def test_kick_multipole_entry_loss_inherits_dzloss(oracle_mod):
     assert s_of([E_, S_]) == {1: 2.0}                # first particle: z_start + 0
     assert s_of([S_, E_, E_]) == {2: 2.5, 3: 2.5}    # a chain keeps the survivor's value
     assert s_of([S_, M_, E_]) == {2: 2.25, 3: 2.25}  # after a particle lost at 0.25 m
+
+
+# ---------------------------------------------------------------- classical radiation
+R_E = 2.8179403262e-15  # classical electron radius, CODATA 2018
+
+
+def test_csbend_radiation_energy_loss(oracle_mod):
+    """SYNCH_RAD in a sector bend (src/csbend.c:956-959, :1627-1660): an on-axis
+    particle loses dE/E = (2/3) r_e gamma^3 theta / rho (the textbook C_gamma
+    E^4 / rho loss over the bend); without SYNCH_RAD delta is untouched"""
+    Po = configs.p_central_from_mev(3000.0)
+    L, ang = 1.0, 0.04
+    p = dict(L=L, ANGLE=ang, N_KICKS=16, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0, SYNCH_RAD=1)
+    r, n = one(oracle_mod, "CSBEND", p, [[0, 0, 0, 0, 0.5, 0]], Po=Po)
+    expect = -(2.0 / 3.0) * R_E * Po ** 3 * ang / (L / ang)
+    assert n == 1 and r[0, 5] == pytest.approx(expect, rel=1e-4)
+    # time of flight kept: s_out - path = s_in * beta(delta_out) / beta(0); the
+    # path stays L to the orbit's shift by the lost energy (~1e-11 m here)
+    b0 = Po / math.sqrt(Po * Po + 1)
+    p1 = Po * (1 + r[0, 5])
+    assert r[0, 4] - 0.5 * (p1 / math.sqrt(p1 * p1 + 1)) / b0 == pytest.approx(L, rel=1e-9)
+    p.pop("SYNCH_RAD")
+    r0, _ = one(oracle_mod, "CSBEND", p, [[0, 0, 0, 0, 0.5, 0]], Po=Po)
+    assert r0[0, 5] == 0.0
+
+
+def _quad_loss(Po, K1, L, x0):
+    """-(2/3) r_e Po^3 K1^2 integral of x(s)^2 over the quadrupole, x = x0 cos(sqrt(K1) s)"""
+    k = math.sqrt(K1)
+    return -(2.0 / 3.0) * R_E * Po ** 3 * K1 ** 2 * x0 ** 2 * (L / 2 + math.sin(2 * k * L) / (4 * k))
+
+
+def test_quadrupole_radiation_energy_loss(oracle_mod):
+    """SYNCH_RAD in KQUAD (src/multipole.c:755-763, :1360-1390) and in a MULT
+    of order 1 (src/multipole.c:599-611): the loss follows the field seen along
+    the orbit, B ~ K1 x"""
+    Po = configs.p_central_from_mev(3000.0)
+    K1, L, x0 = 1.0, 0.3, 2e-3
+    expect = _quad_loss(Po, K1, L, x0)
+    r, _ = one(oracle_mod, "KQUAD", dict(L=L, K1=K1, N_KICKS=40, SYNCH_RAD=1), [[x0, 0, 0, 0, 0, 0]], Po=Po)
+    assert r[0, 5] == pytest.approx(expect, rel=2e-3)
+    r, _ = one(oracle_mod, "MULT", dict(L=L, KNL=K1 * L, ORDER=1, N_SLICES=200, SYNCH_RAD=1),
+               [[x0, 0, 0, 0, 0, 0]], Po=Po)
+    assert r[0, 5] == pytest.approx(expect, rel=2e-3)
+    # no radiation for an element shorter than 1e-6 m (src/multipole.c:761-763)
+    r, _ = one(oracle_mod, "KQUAD", dict(L=5e-7, K1=K1, N_KICKS=4, SYNCH_RAD=1), [[x0, 0, 0, 0, 0, 0]], Po=Po)
+    assert r[0, 5] == 0.0
