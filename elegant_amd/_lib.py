@@ -23,7 +23,7 @@ EXPORTS = [
     "eh_replay_order", "eh_selftest_math", "eh_batch_create", "eh_batch_free", "eh_batch_upload",
     "eh_batch_download", "eh_batch_n_alive", "eh_batch_track", "eh_diag_copy", "eh_adjoint_create",
     "eh_adjoint_free", "eh_adjoint_forward", "eh_adjoint_backward", "eh_adjoint_forward_batch",
-    "eh_multipole_tracking", "eh_offset_beam", "eh_rotate_beam", "eh_replay_sequence",
+    "eh_multipole_tracking", "eh_offset_beam", "eh_rotate_beam", "eh_replay_sequence", "eh_set_central_momentum",
 ]
 
 
@@ -75,6 +75,8 @@ def load():
     lib.eh_offset_beam.restype = L
     lib.eh_rotate_beam.argtypes = [vp, L, C.POINTER(S.EH_ROTATE)]
     lib.eh_rotate_beam.restype = L
+    lib.eh_set_central_momentum.argtypes = [vp, L, D, dp]
+    lib.eh_set_central_momentum.restype = L
     lib.eh_exactDrift.argtypes = [vp, L, D]
     lib.eh_exactDrift.restype = L
     lib.eh_drift_beam.argtypes = [vp, L, D, L]

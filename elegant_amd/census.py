@@ -150,6 +150,8 @@ def per_unit(el):
         return 72, 0  # the 6x6 order-1 matrix loop of track_particles
     if t == S.T_MALIGN:
         return malign(p)
+    if t == S.T_ENERGY:
+        return 3, 0  # set_central_momentum: ((1 + delta) P - P_new) / P_new
     raise ValueError(t)
 
 

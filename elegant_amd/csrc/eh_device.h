@@ -106,6 +106,8 @@ typedef struct {
   double mult_coef[EH_MULT_MAXORDER + 1]; /* MULT: expansion coefficients of its order */
   double mal_dxp, mal_dyp, mal_dt, mal_dp, mal_de; /* MALIGN (offsets in mdx, mdy, mdz) */
   int32_t mal_energy;                   /* MALIGN: dt, dp or de non-zero */
+  int32_t en_apply;                     /* ENERGY: the reference momentum changes here */
+  double en_pold, en_pnew;              /* ENERGY: P_central before and after */
 
   /* ---- classical radiation (SYNCH_RAD) ---- */
   int32_t rad;                 /* rad_coef != 0: the radiating instantiation of the map */
@@ -1314,6 +1316,10 @@ EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm 
   case 12: /* MALIGN, src/do_tracking.c:1853-1857 */
     if (d.on_pass == -1 || d.on_pass == pass)
       eh_offset(p, d);
+    break;
+  case 13: /* ENERGY: set_central_momentum, src/do_tracking.c:3591-3595 */
+    if (d.en_apply)
+      p.dp = ((1 + p.dp) * d.en_pold - d.en_pnew) / d.en_pnew;
     break;
   default: /* MARK, MAXAMP */
     break;

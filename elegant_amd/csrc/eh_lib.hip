@@ -667,8 +667,25 @@ static long build_malign(eh_desc *d, const EH_MALIGN *m) {
   return EH_OK;
 }
 
+/* the reference momentum an ENERGY element sets (src/do_tracking.c:1861-1881):
+ * 1 with *P_new, 0 if it sets none, negative if a matching mode is asked for */
+static long energy_target(const EH_ENERGY *en, double *P_new) {
+  if (en->match_beamline || en->match_particles)
+    return fail(EH_ERR_UNSUPPORTED, "ENERGY with MATCH_BEAMLINE or MATCH_PARTICLES not supported");
+  if (en->central_energy) {
+    *P_new = std::sqrt(hsqr(en->central_energy + 1) - 1);
+    return 1;
+  }
+  if (en->central_momentum) {
+    *P_new = en->central_momentum;
+    return 1;
+  }
+  return 0;
+}
+
 static double elem_length(const EH_ELEMENT_LIST *e) {
-  if (e->type == EH_T_MARK || e->type == EH_T_MAXAMP || e->type == EH_T_ROTATE || e->type == EH_T_MALIGN)
+  if (e->type == EH_T_MARK || e->type == EH_T_MAXAMP || e->type == EH_T_ROTATE || e->type == EH_T_MALIGN ||
+      e->type == EH_T_ENERGY)
     return 0;
   return *(const double *)e->p_elem;
 }
@@ -721,6 +738,16 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
     return build_rotate(d, (const EH_ROTATE *)e->p_elem);
   case EH_T_MALIGN:
     return build_malign(d, (const EH_MALIGN *)e->p_elem);
+  case EH_T_ENERGY: {
+    double Pn = c.Po;
+    long r = energy_target((const EH_ENERGY *)e->p_elem, &Pn);
+    if (r < 0)
+      return r;
+    d->en_pold = c.Po;
+    d->en_pnew = Pn;
+    d->en_apply = r > 0 && c.Po != Pn;
+    return EH_OK;
+  }
   }
   return fail(EH_ERR_UNSUPPORTED, "element type %d not supported", e->type);
 }
@@ -1084,6 +1111,10 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_plain(eh_soa a, lon
         break;
       case EH_T_MALIGN:
         eh_offset(p, d);
+        break;
+      case EH_T_ENERGY:
+        if (d.en_apply)
+          p.dp = ((1 + p.dp) * d.en_pold - d.en_pnew) / d.en_pnew;
         break;
       default:
         break;
@@ -1539,6 +1570,28 @@ extern "C" long eh_offset_beam(eh_beam *b, long n_part, const EH_MALIGN *offset,
   return run_plain(b, &d, nullptr, 0, 0, 0, 0, P_central, 0);
 }
 
+extern "C" long eh_set_central_momentum(eh_beam *b, long np, double P_new, double *P_central) {
+  long r = check_np(b, np);
+  if (r < 0)
+    return r;
+  if (!P_central)
+    return fail(EH_ERR_ARG, "null P_central (set_central_momentum)");
+  if (np && *P_central != P_new) {
+    eh_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.type = EH_T_ENERGY;
+    d.en_apply = 1;
+    d.en_pold = *P_central;
+    d.en_pnew = P_new;
+    r = run_plain(b, &d, nullptr, 0, 0, 0, 0, P_new, 0);
+    if (r < 0)
+      return r;
+  }
+  if (!np || *P_central != P_new)
+    *P_central = P_new;
+  return np;
+}
+
 extern "C" long eh_rotate_beam(eh_beam *b, long n_part, const EH_ROTATE *rotate) {
   long r = check_np(b, n_part);
   if (r < 0)
@@ -1914,6 +1967,15 @@ static long track_plain(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
           nLeft = eh_offset_beam(b, nToTrack, m, *Pc);
         break;
       }
+      case EH_T_ENERGY: {
+        double Pn = *Pc;
+        long t = energy_target((const EH_ENERGY *)e->p_elem, &Pn);
+        if (t < 0)
+          return t;
+        if (t > 0)
+          nLeft = eh_set_central_momentum(b, nToTrack, Pn, Pc);
+        break;
+      }
       default:
         return fail(EH_ERR_UNSUPPORTED, "element type %d not supported", e->type);
       }
@@ -2004,9 +2066,34 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       amp1[ie] = cur1;
     }
   }
+  /* the reference momentum in force at each element on pass 0 (Pe0) and on
+   * later passes (Pe1): ENERGY elements change it for the elements after them */
+  std::vector<double> Pe0(ne), Pe1(ne);
+  double P_end = *Pc;
+  bool two_phase = false;
+  for (int phase = 0; phase < 2; phase++) {
+    double P = phase == 0 ? *Pc : P_end;
+    std::vector<double> &Pe = phase == 0 ? Pe0 : Pe1;
+    for (long ie = 0; ie < ne; ie++) {
+      Pe[ie] = P;
+      if (bl[ie].type == EH_T_ENERGY) {
+        double Pn = P;
+        r = energy_target((const EH_ENERGY *)bl[ie].p_elem, &Pn);
+        if (r < 0)
+          return r;
+        if (r > 0)
+          P = Pn;
+      }
+    }
+    if (phase == 0)
+      P_end = P;
+  }
+  for (long ie = 0; ie < ne; ie++)
+    two_phase = two_phase || Pe0[ie] != Pe1[ie];
+  const std::vector<double> *Pe = &Pe0;
   auto build = [&](long ie) -> long {
     eh_ctx c = {};
-    c.Po = *Pc;
+    c.Po = (*Pe)[ie];
     c.z_start = zs[ie];
     c.z_end = ze[ie];
     c.ap[0] = amp0[ie];
@@ -2057,7 +2144,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       r = launch_fused(b, ne, p, e, 0, (int)ie, effort, tab);
       if (r < 0)
         return r;
-      r = rfca_fiducial(b, rf, *Pc, ze[ie]);
+      r = rfca_fiducial(b, rf, Pe0[ie], ze[ie]);
       if (r < 0)
         return r;
       if (rf->volt != 0 && rf->freq != 0)
@@ -2076,6 +2163,8 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
     int p1 = p + tpl;
     if (p1 > n_passes)
       p1 = (int)n_passes;
+    if (two_phase && p == 0)
+      p1 = 1; /* pass 0 alone: later passes see other reference momenta */
     int pe = p1, ee = 0;
     if (e != 0) { /* finish the partial pass first */
       pe = p;
@@ -2085,9 +2174,25 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
     r = launch_fused(b, ne, p, e, pe, ee, effort, tab);
     if (r < 0)
       return r;
+    if (two_phase && p == 0 && p1 < n_passes) {
+      /* the descriptors of the later passes */
+      Pe = &Pe1;
+      pool.clear();
+      for (long ie = 0; ie < ne; ie++) {
+        r = build(ie);
+        if (r < 0)
+          return r;
+      }
+      HIPCHK(hipMemcpy(b->d_desc, hd.data(), sizeof(eh_desc) * ne, hipMemcpyHostToDevice));
+      r = upload_terms(b, pool);
+      if (r < 0)
+        return r;
+    }
     p = p1;
     e = 0;
   }
+  if (n_passes > 0)
+    *Pc = P_end;
   b->pass = n_passes > 0 ? n_passes - 1 : 0;
   return b->n_alive;
 }
@@ -2302,6 +2407,8 @@ static size_t elem_struct_size(int32_t type) {
     return sizeof(EH_ROTATE);
   case EH_T_MALIGN:
     return sizeof(EH_MALIGN);
+  case EH_T_ENERGY:
+    return sizeof(EH_ENERGY);
   default:
     return 0;
   }
@@ -2692,6 +2799,9 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
     for (long e = 0; e < ne; e++)
       if (beamlines[v][e].type != beamlines[0][e].type)
         return fail(EH_ERR_ARG, "variant %ld element %ld: types differ from variant 0", v, e);
+  for (long e = 0; e < ne; e++)
+    if (beamlines[0][e].type == EH_T_ENERGY)
+      return fail(EH_ERR_UNSUPPORTED, "element %ld: ENERGY is not supported in batched tracking", e);
   double Pc = *P_central;
   /* descriptors, deduplicated across variants */
   std::vector<eh_desc> pool;
@@ -3289,8 +3399,9 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
         if (e > 0)
           d.adj_anchor = na++;
       }
-      if (t == EH_T_MULT || t == EH_T_ROTATE || t == EH_T_MALIGN)
-        return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support MULT, ROTATE or MALIGN", e);
+      if (t == EH_T_MULT || t == EH_T_ROTATE || t == EH_T_MALIGN || t == EH_T_ENERGY)
+        return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support MULT, ROTATE, MALIGN or ENERGY",
+                    e);
       if (d.rad)
         return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support SYNCH_RAD", e);
       if ((t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE) && d.expand)

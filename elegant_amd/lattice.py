@@ -20,7 +20,7 @@ class Element:
         self.params = dict(params)
 
     def length(self):
-        if self.type in (S.T_MARK, S.T_MAXAMP):
+        if self.type in (S.T_MARK, S.T_MAXAMP, S.T_ROTATE, S.T_MALIGN, S.T_ENERGY):
             return 0.0
         for k in ("L", "length"):
             if k in self.params:

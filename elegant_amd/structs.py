@@ -12,10 +12,10 @@ ID_INDEX = 6
 LOSS_PASS_INDEX = 7
 
 (T_MARK, T_DRIF, T_EDRIFT, T_KQUAD, T_KSEXT, T_KOCT, T_CSBEND, T_RFCA, T_MAXAMP, T_KQUSE, T_MULT, T_ROTATE,
- T_MALIGN) = range(13)
+ T_MALIGN, T_ENERGY) = range(14)
 TYPE_NAMES = {T_MARK: "MARK", T_DRIF: "DRIF", T_EDRIFT: "EDRIFT", T_KQUAD: "KQUAD", T_KSEXT: "KSEXT",
               T_KOCT: "KOCT", T_CSBEND: "CSBEND", T_RFCA: "RFCA", T_MAXAMP: "MAXAMP", T_KQUSE: "KQUSE",
-              T_MULT: "MULT", T_ROTATE: "ROTATE", T_MALIGN: "MALIGN"}
+              T_MULT: "MULT", T_ROTATE: "ROTATE", T_MALIGN: "MALIGN", T_ENERGY: "ENERGY"}
 TYPE_CODES = {v: k for k, v in TYPE_NAMES.items()}
 
 BEND_EDGE1_EFFECTS = 1
@@ -91,6 +91,11 @@ class EH_MALIGN(C.Structure):
                [(n, i32) for n in ("on_pass", "startPID", "endPID")]
 
 
+class EH_ENERGY(C.Structure):
+    _fields_ = [("central_energy", dbl), ("central_momentum", dbl), ("match_beamline", i32),
+                ("match_particles", i32)]
+
+
 class EH_MAXAMP(C.Structure):
     _fields_ = [("x_max", dbl), ("y_max", dbl), ("elliptical", i32), ("exponent", i32)]
 
@@ -110,7 +115,7 @@ class eh_batch_tune(C.Structure):
 
 STRUCT_OF = {T_DRIF: EH_DRIFT, T_EDRIFT: EH_EDRIFT, T_KQUAD: EH_KQUAD, T_KSEXT: EH_KSEXT, T_KOCT: EH_KOCT,
              T_CSBEND: EH_CSBEND, T_RFCA: EH_RFCA, T_MAXAMP: EH_MAXAMP, T_KQUSE: EH_KQUSE, T_MULT: EH_MULT,
-             T_ROTATE: EH_ROTATE, T_MALIGN: EH_MALIGN}
+             T_ROTATE: EH_ROTATE, T_MALIGN: EH_MALIGN, T_ENERGY: EH_ENERGY}
 
 # reference defaults (src/track_data.c)
 DEFAULTS = {
@@ -153,6 +158,8 @@ ALIASES = {
              "SYNCH_RAD": "synch_rad", "EXPAND_HAMILTONIAN": "expandHamiltonian"},
     T_ROTATE: {"ON_PASS": "on_pass", "EXCLUDE_FLOOR": "exclude_floor", "EXCLUDE_OPTICS": "exclude_optics"},
     T_MALIGN: {"ON_PASS": "on_pass", "START_PID": "startPID", "END_PID": "endPID"},
+    T_ENERGY: {"CENTRAL_ENERGY": "central_energy", "CENTRAL_MOMENTUM": "central_momentum",
+               "MATCH_BEAMLINE": "match_beamline", "MATCH_PARTICLES": "match_particles"},
     T_MAXAMP: {},
     T_DRIF: {},
     T_EDRIFT: {},

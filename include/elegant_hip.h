@@ -60,7 +60,8 @@ enum eh_type {
   EH_T_KQUSE = 9,
   EH_T_MULT = 10,
   EH_T_ROTATE = 11,
-  EH_T_MALIGN = 12
+  EH_T_MALIGN = 12,
+  EH_T_ENERGY = 13
 };
 
 /* DRIF: src/track.h:1364-1368 (length, order; order 0 means run_setup default_order=2) */
@@ -139,6 +140,15 @@ typedef struct {
   double dxp, dyp, dx, dy, dz, dt, dp, de;
   int32_t on_pass, startPID, endPID;
 } EH_MALIGN;
+
+/* ENERGY (reference momentum change): src/track.h:1998-2004, parameters
+ * src/track_data.c:985-990, tracking src/do_tracking.c:1861-1881 through
+ * set_central_momentum (:3563-3596).  MATCH_BEAMLINE and MATCH_PARTICLES are
+ * not supported (EH_ERR_UNSUPPORTED). */
+typedef struct {
+  double central_energy, central_momentum;
+  int32_t match_beamline, match_particles;
+} EH_ENERGY;
 
 /* CSBEND: src/track.h:2462-2502, defaults src/track_data.c:1898.
  * N_KICKS is an alias of N_SLICES (src/track_data.c:1943-1944). */
@@ -248,6 +258,10 @@ long eh_multipole_tracking(eh_beam *beam, long n_part, const EH_MULT *mult, doub
 /* replaces offset_beam, src/do_tracking.c:3311-3315 (MALIGN, for the pass given
  * by eh_beam_set_pass) */
 long eh_offset_beam(eh_beam *beam, long n_part, const EH_MALIGN *offset, double P_central);
+
+/* replaces set_central_momentum, src/do_tracking.c:3563-3596: every resident
+ * particle's delta is re-referenced to P_new and *P_central becomes P_new */
+long eh_set_central_momentum(eh_beam *beam, long np, double P_new, double *P_central);
 
 /* replaces track_particles with rotation_matrix(tilt) for ROTATE,
  * src/do_tracking.c:1836-1852 (for the pass given by eh_beam_set_pass) */

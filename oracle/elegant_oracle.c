@@ -861,6 +861,31 @@ void or_rotate(double **coord, long np, const EH_ROTATE *rot) {
   }
 }
 
+/* set_central_momentum, src/do_tracking.c:3563-3596 (no MPI) */
+void or_set_central_momentum(double **coord, long np, double P_new, double *P_central) {
+  long ip;
+  if (!np) {
+    *P_central = P_new;
+    return;
+  }
+  if (*P_central != P_new) {
+    for (ip = 0; ip < np; ip++)
+      coord[ip][5] = ((1 + coord[ip][5]) * (*P_central) - P_new) / P_new;
+    *P_central = P_new;
+  }
+}
+
+/* ENERGY, src/do_tracking.c:1861-1881, without the matching modes */
+long or_energy(double **coord, long np, const EH_ENERGY *energy, double *P_central) {
+  if (energy->match_beamline || energy->match_particles)
+    return or_fail("ENERGY with MATCH_BEAMLINE or MATCH_PARTICLES not supported");
+  if (energy->central_energy)
+    or_set_central_momentum(coord, np, sqrt(sqr(energy->central_energy + 1) - 1), P_central);
+  else if (energy->central_momentum)
+    or_set_central_momentum(coord, np, energy->central_momentum, P_central);
+  return np;
+}
+
 /* MALIGN: offset_beam, src/do_tracking.c:3311-3366, every particle */
 long or_offset_beam(double **coord, long nToTrack, const EH_MALIGN *offset, double P_central) {
   long i_part;
@@ -1556,6 +1581,7 @@ static double or_elem_length(const EH_ELEMENT_LIST *e) {
   case EH_T_MAXAMP:
   case EH_T_ROTATE:
   case EH_T_MALIGN:
+  case EH_T_ENERGY:
     return 0;
   default:
     return *(const double *)e->p_elem; /* length is the first member, as in the reference */
@@ -1634,6 +1660,9 @@ long or_do_tracking(double **coord, long np, const EH_ELEMENT_LIST *beamline, lo
           nLeft = or_offset_beam(coord, nToTrack, m, *P_central);
         break;
       }
+      case EH_T_ENERGY:
+        nLeft = or_energy(coord, nToTrack, (const EH_ENERGY *)eptr->p_elem, P_central);
+        break;
       case EH_T_RFCA:
         nLeft = or_simple_rf_cavity(coord, nToTrack, (EH_RFCA *)eptr->p_elem, P_central, z_travel);
         break;
@@ -1702,6 +1731,9 @@ long or_element_flat(double *flat, long np, const EH_ELEMENT_LIST *eptr, double 
     break;
   case EH_T_MALIGN:
     r = or_offset_beam(rows, np, (const EH_MALIGN *)eptr->p_elem, *P_central);
+    break;
+  case EH_T_ENERGY:
+    r = or_energy(rows, np, (const EH_ENERGY *)eptr->p_elem, P_central);
     break;
   case EH_T_RFCA:
     r = or_simple_rf_cavity(rows, np, (EH_RFCA *)eptr->p_elem, P_central, z_end);

@@ -275,6 +275,50 @@ def test_kick_multipole_entry_losses(mode, offset, ehlib, oracle_mod):
     assert_rows_identical(beam.download(), ref1, n1, nref1, "entry losses, one KQUAD call")
 
 
+def test_set_central_momentum(ehlib, oracle_mod):
+    """the ENERGY entry point (set_central_momentum): deltas re-referenced bit for
+    bit, P_central updated"""
+    _say("set_central_momentum")
+    rows = _beam_rows(3000, seed=23)
+    Po, Pn = configs.p_central_from_mev(2000.0), configs.p_central_from_mev(2100.0)
+    lat = Lattice([Element("E", "ENERGY", CENTRAL_MOMENTUM=Pn)])
+    ref, nref = oracle_mod.element(rows, lat, 0, Po, 0.0, 0.0)
+    beam = tracking.Beam(rows)
+    pc = C.c_double(Po)
+    n = ehlib.eh_set_central_momentum(beam.handle, len(rows), Pn, C.byref(pc))
+    assert n == len(rows), ehlib.eh_last_error()
+    assert pc.value == Pn
+    assert_rows_identical(beam.download(), ref, n, nref, "set_central_momentum")
+
+
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+def test_energy_ring(mode, ehlib, oracle_mod):
+    """an ENERGY element changes the reference momentum mid-line: on pass 0 the
+    elements before it see the initial momentum, on later passes the new one;
+    tracking, losses and the returned P_central match the oracle"""
+    _say("energy ring %s" % mode)
+    Po = configs.p_central_from_mev(1000.0)
+    lat = Lattice([Element("AP", "MAXAMP", X_MAX=0.02, Y_MAX=0.02),
+                   Element("QF", "KQUAD", L=0.3, K1=1.1, N_KICKS=8),
+                   Element("D1", "DRIF", L=1.0),
+                   Element("B", "CSBEND", L=1.0, ANGLE=0.1, K1=0.1, N_KICKS=8),
+                   Element("E", "ENERGY", CENTRAL_MOMENTUM=Po * 1.002),
+                   Element("QD", "KQUAD", L=0.3, K1=-1.1, N_KICKS=8),
+                   Element("M", "MULT", L=0.1, KNL=20.0, ORDER=2, N_SLICES=2),
+                   Element("D2", "EDRIFT", L=1.0)])
+    rows = _beam_rows(2000, seed=29, scale=2.0)
+    rows[:20, 0] = np.linspace(-0.03, 0.03, 20)
+    ref_lat = lat.copy()
+    from oracle import oracle as om
+    ref, nref, eff, pc_ref = om.do_tracking(rows, ref_lat, 4, Po)
+    beam = tracking.Beam(rows)
+    n, stats, pc = tracking.track(beam, lat.copy(), 4, Po, mode=mode, turns_per_launch=2)
+    assert pc == pc_ref == Po * 1.002
+    assert 0 < nref < len(rows)
+    assert_rows_identical(beam.download(), ref, n, nref, "energy " + mode)
+    assert stats["particle_element_turns"] == eff
+
+
 def radiation_ring():
     """a FODO ring with bends, all radiating (SYNCH_RAD=1), and an RF cavity"""
     from elegant_amd.lattice import Element as E

@@ -634,3 +634,25 @@ def test_quadrupole_radiation_energy_loss(oracle_mod):
     # no radiation for an element shorter than 1e-6 m (src/multipole.c:761-763)
     r, _ = one(oracle_mod, "KQUAD", dict(L=5e-7, K1=K1, N_KICKS=4, SYNCH_RAD=1), [[x0, 0, 0, 0, 0, 0]], Po=Po)
     assert r[0, 5] == 0.0
+
+
+def test_energy_keeps_momentum(oracle_mod):
+    """ENERGY through set_central_momentum (src/do_tracking.c:3563-3596): each
+    particle's momentum P (1 + delta) is kept while the reference changes;
+    CENTRAL_ENERGY gives P_new = sqrt((E + 1)^2 - 1); do_tracking carries the new
+    reference to the following elements and passes"""
+    rng = np.random.default_rng(3)
+    pts = np.column_stack([rng.normal(0, 1e-3, (20, 4)), np.zeros(20), rng.normal(0, 1e-3, 20)])
+    Po, Pn = 2000.0, 2100.0
+    lat = Lattice([Element("E", "ENERGY", CENTRAL_MOMENTUM=Pn)])
+    r, n = oracle_mod.element(rows_of(pts), lat, 0, Po, 0.0, 0.0)
+    for i in range(20):
+        got = r[r[:, 6] == i + 1][0]
+        assert Pn * (1 + got[5]) == pytest.approx(Po * (1 + pts[i, 5]), rel=2e-16)
+        assert list(got[:5]) == list(pts[i, :5])
+    E = 4000.0
+    lat = Lattice([Element("E", "ENERGY", CENTRAL_ENERGY=E), Element("D", "EDRIFT", L=1.0)])
+    r, n, _, pc = oracle_mod.do_tracking(rows_of(pts), lat, 2, Po)
+    assert pc == math.sqrt((E + 1) ** 2 - 1)
+    got = r[r[:, 6] == 1][0]
+    assert pc * (1 + got[5]) == pytest.approx(Po * (1 + pts[0, 5]), rel=4e-16)
