@@ -653,12 +653,12 @@ EH_HD bool eh_mult_out(double x, double xp, double y, double yp) {
  * lost in the kicks keeps the values reached; both get s = z_start and
  * delta = Po*(1+delta).  Zero strength is an exact drift, set up by the host.
  * Returns true if lost. */
-EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d) {
+template <bool RAD> EH_HD bool eh_mult_t(eh_p6 &p, const eh_desc &d) {
   eh_mface1(p, d);
   const double drift = d.mdrift;
   double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, dp = p.dp, qx, qy;
   const double P0 = d.Po * (1 + dp);
-  const double beta0 = d.rad ? P0 / eh_sqrt(eh_sqr(P0) + 1) : 0.0;
+  const double beta0 = RAD && d.rad ? P0 / eh_sqrt(eh_sqr(P0) + 1) : 0.0;
   bool lost;
   if (eh_mult_out(x, xp, y, yp)) {
     lost = true;
@@ -690,7 +690,7 @@ EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d) {
         is_lost = true;
         break;
       }
-      if (d.rad && drift) {
+      if (RAD && d.rad && drift) {
         /* classical radiation, src/multipole.c:599-611 */
         qx /= (1 + dp);
         qy /= (1 + dp);
@@ -717,7 +717,7 @@ EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d) {
     p.y = y;
     p.yp = yp;
     p.dp = dp;
-    if (d.rad) {
+    if (RAD && d.rad) {
       /* src/multipole.c:623-628 */
       const double P1 = d.Po * (1 + dp);
       const double beta1 = P1 / eh_sqrt(eh_sqr(P1) + 1);
@@ -734,6 +734,8 @@ EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d) {
   eh_mface2(p, d, true);
   return lost;
 }
+
+EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d) { return eh_mult_t<true>(p, d); }
 
 /* one row of the order-1 loop of track_particles (src/matrix.c:424-460):
  * C[i] (zero here) plus R[i][j]*ini[j] for j = 5 down to 0, zero entries
@@ -1279,7 +1281,8 @@ EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d) {
  * (src/do_tracking.c:2506-2524, :2675-2684).  Returns -1 if the particle
  * survives, else the loss stage (EH_STAGE_ELEMENT possibly with
  * EH_STAGE_INHERIT). */
-EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms) {
+template <bool RAD>
+EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms) {
   const int pi = pass > 0 ? 1 : 0;
   bool lost = false;
   int flag = 0;
@@ -1294,20 +1297,20 @@ EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm 
   case 4:
   case 5:
   case 9: { /* KQUSE */
-    const int m = eh_multipole(p, d, pi);
+    const int m = RAD ? eh_multipole(p, d, pi) : eh_multipole_t<false>(p, d, pi);
     lost = m != 0;
     flag = m == 2 ? EH_STAGE_INHERIT : 0;
     break;
   }
   case 6:
-    lost = eh_csbend(p, d, pi, terms);
+    lost = RAD ? eh_csbend(p, d, pi, terms) : eh_csbend_t<false>(p, d, pi, terms);
     break;
   case 7:
     lost = eh_rfca(p, d);
     break;
   case 10: /* MULT (no slices: zero strength and length, nothing to do) */
     if (d.nslices)
-      lost = eh_mult(p, d);
+      lost = eh_mult_t<RAD>(p, d);
     break;
   case 11: /* ROTATE, src/do_tracking.c:1836-1852 */
     if (d.on_pass == -1 || d.on_pass == pass)
@@ -1333,6 +1336,12 @@ EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm 
   if (eh_limit_one(p, EH_DBL_MAX, EH_DBL_MAX, d.z_end, d.Po, 0))
     return EH_STAGE_NAN;
   return -1;
+}
+
+/* RAD = false: a kernel for lines without radiating elements carries none of
+ * the radiating instantiations (and none of their registers) */
+EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms) {
+  return eh_apply_element_t<true>(p, d, pass, terms);
 }
 
 #endif

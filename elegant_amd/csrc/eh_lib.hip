@@ -1178,6 +1178,7 @@ __global__ void __launch_bounds__(EH_BLOCK) k_compact(eh_soa a, long n, eh_soa b
  * stage, in the order the reference compacts.  The loop is wavefront-uniform (the
  * element index is the same in every lane), so descriptors are read with scalar
  * loads; the wavefront leaves early once all its lanes are lost. */
+template <bool RAD>
 __device__ __forceinline__ void track_span(eh_p6 &p, bool &alive, const eh_desc *__restrict__ descs,
                                            const eh_fterm *__restrict__ terms, int nelem, int p0, int e0, int p1,
                                            int e1, int32_t event_base, int32_t &lpass, int32_t &levent,
@@ -1189,7 +1190,7 @@ __device__ __forceinline__ void track_span(eh_p6 &p, bool &alive, const eh_desc 
     const eh_desc *d = descs + ee;
     if (alive) {
       units += d->counted;
-      int st = eh_apply_element(p, *d, pp, terms);
+      int st = eh_apply_element_t<RAD>(p, *d, pp, terms);
       if (st >= 0) {
         alive = false;
         lpass = pp | (st & EH_STAGE_INHERIT ? EH_LPASS_INHERIT : 0);
@@ -1239,6 +1240,7 @@ __device__ __forceinline__ void wave_store(const eh_p6 &p, bool in, bool alive, 
 
 /* fused tracking of one lattice: each thread takes one particle through the span,
  * then survivors are compacted into b and lost particles appended to the graveyard */
+template <bool RAD>
 __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_fused(eh_soa a, long n, eh_soa b, eh_soa g,
                                                      const eh_desc *__restrict__ descs,
                                                      const eh_fterm *__restrict__ terms, int nelem, int p0, int e0,
@@ -1254,7 +1256,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_fused(eh_soa a, lon
   }
   int32_t lpass = 0, levent = -1;
   unsigned long long units = 0;
-  track_span(p, alive, descs, terms, nelem, p0, e0, p1, e1, event_base, lpass, levent, units);
+  track_span<RAD>(p, alive, descs, terms, nelem, p0, e0, p1, e1, event_base, lpass, levent, units);
   wave_add(&cnt[3], units);
   wave_store(p, in, alive, row0, lpass, levent, b, 0, &cnt[0], g, &cnt[1]);
 }
@@ -1295,6 +1297,7 @@ __global__ void k_batch_prep(const unsigned long long *alive_prev, int32_t *aliv
   }
 }
 
+template <bool RAD>
 __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_batch(eh_soa a, eh_soa b, eh_soa g, eh_batch_dev bd,
                                                      const eh_desc *__restrict__ pool,
                                                      const eh_fterm *__restrict__ terms, int nelem, int p0, int e0,
@@ -1342,7 +1345,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_batch(eh_soa a, eh_
       const eh_desc *d = pool + __builtin_amdgcn_readfirstlane(idx[ee]);
       units += (unsigned long long)__popcll(live) * (unsigned)d->counted;
       if (alive) {
-        int st = eh_apply_element(p, *d, pp, terms);
+        int st = eh_apply_element_t<RAD>(p, *d, pp, terms);
         if (st >= 0) {
           alive = false;
           lpass = pp | (st & EH_STAGE_INHERIT ? EH_LPASS_INHERIT : 0);
@@ -1996,7 +1999,7 @@ static long track_plain(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
 }
 
 static long launch_fused(eh_beam *b, long ne, int p0, int e0, int p1, int e1, double *effort,
-                         const eh_kmp_tab &tab) {
+                         const eh_kmp_tab &tab, bool rad) {
   long n = b->n_alive;
   long span = ((long)(p1 - p0) * ne + ne) * 3 + 3;
   if ((long)b->event + span >= 2147483647L)
@@ -2014,8 +2017,12 @@ static long launch_fused(eh_beam *b, long ne, int p0, int e0, int p1, int e1, do
   long r = timed_begin(b);
   if (r < 0)
     return r;
-  hipLaunchKernelGGL(k_fused, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, b->buf[b->cur], n, b->buf[1 - b->cur],
-                     b->grave, b->d_desc, b->d_terms, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
+  if (rad)
+    hipLaunchKernelGGL(k_fused<true>, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, b->buf[b->cur], n,
+                       b->buf[1 - b->cur], b->grave, b->d_desc, b->d_terms, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
+  else
+    hipLaunchKernelGGL(k_fused<false>, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, b->buf[b->cur], n,
+                       b->buf[1 - b->cur], b->grave, b->d_desc, b->d_terms, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
   HIPCHK(hipGetLastError());
   r = timed_end(b);
   if (r < 0)
@@ -2132,6 +2139,10 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
     any_kick = any_kick || k.kick;
   if (!any_kick)
     tab = nullptr;
+  /* the radiating kernel only for lines that radiate */
+  bool rad = false;
+  for (long ie = 0; ie < ne; ie++)
+    rad = rad || hd[ie].rad;
   int p = 0, e = 0;
   if (n_passes > 0) {
     /* pass 0: stop before each RFCA whose fiducial is still unknown */
@@ -2141,7 +2152,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       EH_RFCA *rf = (EH_RFCA *)bl[ie].p_elem;
       if (rf->fiducial_seen)
         continue;
-      r = launch_fused(b, ne, p, e, 0, (int)ie, effort, tab);
+      r = launch_fused(b, ne, p, e, 0, (int)ie, effort, tab, rad);
       if (r < 0)
         return r;
       r = rfca_fiducial(b, rf, Pe0[ie], ze[ie]);
@@ -2171,7 +2182,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       ee = (int)ne;
       p1 = p + 1;
     }
-    r = launch_fused(b, ne, p, e, pe, ee, effort, tab);
+    r = launch_fused(b, ne, p, e, pe, ee, effort, tab, rad);
     if (r < 0)
       return r;
     if (two_phase && p == 0 && p1 < n_passes) {
@@ -2376,6 +2387,7 @@ struct eh_batch {
   /* descriptor tables of the last call, reused while the beamlines are unchanged */
   uint64_t tables_hash = 0;
   long tables_ne = 0;
+  bool rad = false;    /* some descriptor of the pool radiates */
   eh_kmp_tab kmp_pool; /* kick-multipole records of the descriptor pool */
   eh_kmp_idx kmp_idx;  /* variant * ne + element -> pool entry */
   std::vector<eh_evrec> evlog;
@@ -2680,8 +2692,12 @@ static long batch_enqueue_span(eh_batch *b, long ne, int p0, int e0, int p1, int
   bd.nvar = (int)b->nvar;
   long threads = (long)b->n_cu * waves_per_cu * 64;
   unsigned grid = (unsigned)((threads + block - 1) / block);
-  hipLaunchKernelGGL(k_batch, dim3(grid), dim3(block), 0, b->stream, b->buf[b->cur], b->buf[1 - b->cur], b->grave, bd,
-                     b->d_pool, b->d_terms, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
+  if (b->rad)
+    hipLaunchKernelGGL(k_batch<true>, dim3(grid), dim3(block), 0, b->stream, b->buf[b->cur], b->buf[1 - b->cur],
+                       b->grave, bd, b->d_pool, b->d_terms, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
+  else
+    hipLaunchKernelGGL(k_batch<false>, dim3(grid), dim3(block), 0, b->stream, b->buf[b->cur], b->buf[1 - b->cur],
+                       b->grave, bd, b->d_pool, b->d_terms, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
   b->cur = 1 - b->cur;
   b->launches++;
   return EH_OK;
@@ -2822,6 +2838,9 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
     for (const eh_kmp &k : *tab)
       any_kick = any_kick || k.kick;
     b->kmp_pool = any_kick ? tab : nullptr;
+    b->rad = false;
+    for (const eh_desc &d : pool)
+      b->rad = b->rad || d.rad;
     b->kmp_idx = any_kick ? std::make_shared<const std::vector<int32_t>>(idx) : nullptr;
   }
   auto upload_tables = [&]() -> long {
@@ -3079,7 +3098,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_adj_fwd(double *z, 
           a[5 * a_ld] = p.dp;
         }
         u += d->counted;
-        if (eh_apply_element(p, *d, t, terms) >= 0)
+        if (eh_apply_element_t<false>(p, *d, t, terms) >= 0)
           live = false;
       }
     }
