@@ -89,6 +89,8 @@ typedef struct {
   double cos_tt, sin_tt;
   double dxi, dyi, dzi, dxf, dyf, dzf;
   double tan_e1, tan_e2, e1, e2, psi1, psi2;
+  int32_t csb_eorder2;      /* EDGE_ORDER >= 2: apply_edge_effects at the edges */
+  double csb_n, csb_sec1, csb_sec2, csb_he1, csb_he2; /* its n, 1/cos(e), pole-face curvatures */
   int32_t csb_fx0, csb_fy0; /* offsets of this element's field terms in the term pool */
   int32_t csb_maxj;         /* largest power of y in the field terms */
   int32_t csb_dense;        /* EO1 of the dense blocks at csb_dx0/csb_dy0 (5 or 6), else 0 */
@@ -956,6 +958,39 @@ EH_HD double eh_div_recip(double a, double b, double rh, double rl) {
   return a / b;
 }
 
+/* apply_edge_effects, src/csbend.c:4291-4332: K. L. Brown's second-order edge
+ * map; which = -1 at the entrance, 1 at the exit; tan_beta and sec_beta are
+ * tan(beta) and 1./cos(beta) as the routine forms them */
+EH_HD void eh_edge2(double &x, double &xp, double &y, double &yp, double rho, double n, double beta,
+                    double tan_beta, double sec_beta, double he, double psi, int which) {
+  const double h = 1 / rho;
+  const double R21 = h * tan_beta;
+  const double R43 = -h * eh_tan(beta - psi);
+  const double h2 = eh_sqr(h);
+  const double tan2_beta = eh_sqr(tan_beta);
+  const double w = (double)which, mw = (double)(-which);
+  const double T111 = w * h / 2 * tan2_beta;
+  const double sec2_beta = eh_sqr(sec_beta);
+  const double T133 = mw * h / 2 * sec2_beta;
+  double T211 = which == -1 ? -n * h2 * tan_beta : -h2 * (n + tan2_beta / 2) * tan_beta;
+  const double T221 = mw * h * tan2_beta, T331 = T221, T441 = -T221;
+  double T233 = which == -1 ? h2 * (n + .5 + tan2_beta) * tan_beta : h2 * (n - tan2_beta / 2) * tan_beta;
+  const double T243 = w * h * tan2_beta;
+  double T431 = h2 * (2 * n + (which == 1 ? sec2_beta : 0)) * tan_beta;
+  const double T432 = w * h * sec2_beta;
+  if (he != 0) {
+    const double term = h / 2 * he * sec2_beta * sec_beta;
+    T211 += term;
+    T233 -= term;
+    T431 -= 2 * term;
+  }
+  const double x0 = x, xp0 = xp, y0 = y, yp0 = yp;
+  x = x0 + T111 * eh_sqr(x0) + T133 * eh_sqr(y0);
+  xp = xp0 + R21 * x0 + T211 * eh_sqr(x0) + T221 * x0 * xp0 + T233 * eh_sqr(y0) + T243 * y0 * yp0;
+  y = y0 + T331 * x0 * y0;
+  yp = yp0 + R43 * y0 + T441 * yp0 * x0 + T431 * x0 * y0 + T432 * xp0 * y0;
+}
+
 /* CSBEND: track_through_csbend (src/csbend.c:1088-1403) for one particle, with
  * integrate_csbend_ordn (src/csbend.c:1459-1720) and the canonical conversions
  * (:1436-1454).  Returns true if lost. */
@@ -976,9 +1011,13 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   const double dp0 = dp;
   if (d.csb_edge1) {
     double rho = (1 + dp) * rho_actual;
-    double delta_xp = d.tan_e1 / rho * x;
-    xp += delta_xp;
-    yp -= eh_tan(d.e1 - d.psi1 / (1 + dp)) / rho * y;
+    if (!d.csb_eorder2) {
+      double delta_xp = d.tan_e1 / rho * x;
+      xp += delta_xp;
+      yp -= eh_tan(d.e1 - d.psi1 / (1 + dp)) / rho * y;
+    } else {
+      eh_edge2(x, xp, y, yp, rho, d.csb_n, d.e1, d.tan_e1, d.csb_sec1, d.csb_he1, d.psi1 * (1 + dp), -1);
+    }
   }
   double dpi = dp;
   if constexpr (RAD) {
@@ -1122,9 +1161,13 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   dp = DP;
   if (d.csb_edge2) {
     double rho = (1 + dp) * rho_actual;
-    double delta_xp = d.tan_e2 / rho * x;
-    xp += delta_xp;
-    yp -= eh_tan(d.e2 - d.psi2 / (1 + dp)) / rho * y;
+    if (!d.csb_eorder2) {
+      double delta_xp = d.tan_e2 / rho * x;
+      xp += delta_xp;
+      yp -= eh_tan(d.e2 - d.psi2 / (1 + dp)) / rho * y;
+    } else {
+      eh_edge2(x, xp, y, yp, rho, d.csb_n, d.e2, d.tan_e2, d.csb_sec2, d.csb_he2, d.psi2 * (1 + dp), 1);
+    }
   }
   p.x = x * d.cos_tt - y * d.sin_tt + 0.0;
   p.y = x * d.sin_tt + y * d.cos_tt + 0.0;

@@ -344,7 +344,7 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
   int nsub;
   if (!integ_tables(cb->integration_order, &driftFrac, &kickFrac, &nsub))
     return fail(EH_ERR_UNSUPPORTED, "CSBEND integration_order is invalid");
-  if (cb->malignMethod != 0 || cb->etilt != 0 || cb->expandHamiltonian || cb->edge_order > 1)
+  if (cb->malignMethod != 0 || cb->etilt != 0 || cb->expandHamiltonian)
     return fail(EH_ERR_UNSUPPORTED, "CSBEND option not supported");
   if ((cb->edge1_effects != 0 && cb->edge1_effects != 1) || (cb->edge2_effects != 0 && cb->edge2_effects != 1))
     return fail(EH_ERR_UNSUPPORTED, "CSBEND edge method not supported");
@@ -535,6 +535,13 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
   d->e2 = e2;
   d->tan_e1 = eh_tan(e1);
   d->tan_e2 = eh_tan(e2);
+  /* EDGE_ORDER >= 2: apply_edge_effects (src/csbend.c:1135-1138, :1316-1318) */
+  d->csb_eorder2 = cb->edge_order >= 2;
+  d->csb_n = -b[1] / h;
+  d->csb_sec1 = 1. / eh_cos(e1);
+  d->csb_sec2 = 1. / eh_cos(e2);
+  d->csb_he1 = cb->h1;
+  d->csb_he2 = cb->h2;
   d->psi1 = psi1;
   d->psi2 = psi2;
   d->length = cb->length;

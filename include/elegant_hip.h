@@ -163,7 +163,7 @@ typedef struct {
   int32_t nonlinear;          /* default 1 */
   int32_t edge1_effects;      /* default 1; 0 or 1 supported */
   int32_t edge2_effects;
-  int32_t edge_order;         /* default 1; <=1 supported */
+  int32_t edge_order;         /* default 1; >= 2: second-order edge map (apply_edge_effects) */
   int32_t integration_order;  /* 2, 4, 6 */
   int32_t expandHamiltonian;  /* 0 supported */
   int32_t use_bn;

@@ -1126,11 +1126,49 @@ static void fromDipoleCanonical(double *Qi) {
 
 /* track_through_csbend, src/csbend.c:654-1434 (malignMethod 0, ETILT 0, no
  * radiation, EDGE*_EFFECTS 0 or 1 with EDGE_ORDER 1, no reference correction) */
+/* apply_edge_effects, src/csbend.c:4291-4332: K. L. Brown's second-order edge
+ * map (EDGE_ORDER >= 2); which_edge -1 at the entrance, 1 at the exit */
+static void or_apply_edge_effects(double *x, double *xp, double *y, double *yp, double rho, double n, double beta,
+                                  double he, double psi, long which_edge) {
+  double h, tan_beta, tan2_beta, sec_beta, sec2_beta, h2;
+  double R21, R43;
+  double T111, T133, T211, T441, T331, T221, T233, T243, T431, T432;
+  double x0, xp0, y0, yp0;
+  h = 1 / rho;
+  R21 = h * (tan_beta = OR_TAN(beta));
+  R43 = -h * OR_TAN(beta - psi);
+  h2 = sqr(h);
+  T111 = which_edge * h / 2 * (tan2_beta = sqr(tan_beta));
+  sec_beta = 1. / OR_COS(beta);
+  sec2_beta = sqr(sec_beta);
+  T133 = -which_edge * h / 2 * sec2_beta;
+  T211 = which_edge == -1 ? -n * h2 * tan_beta : -h2 * (n + tan2_beta / 2) * tan_beta;
+  T441 = -(T331 = T221 = -which_edge * h * tan2_beta);
+  T233 = which_edge == -1 ? h2 * (n + .5 + tan2_beta) * tan_beta : h2 * (n - tan2_beta / 2) * tan_beta;
+  T243 = which_edge * h * tan2_beta;
+  T431 = h2 * (2 * n + (which_edge == 1 ? sec2_beta : 0)) * tan_beta;
+  T432 = which_edge * h * sec2_beta;
+  if (he != 0) {
+    double term = h / 2 * he * sec2_beta * sec_beta;
+    T211 += term;
+    T233 -= term;
+    T431 -= 2 * term;
+  }
+  x0 = *x;
+  xp0 = *xp;
+  y0 = *y;
+  yp0 = *yp;
+  *x = x0 + T111 * sqr(x0) + T133 * sqr(y0);
+  *xp = xp0 + R21 * x0 + T211 * sqr(x0) + T221 * x0 * xp0 + T233 * sqr(y0) + T243 * y0 * yp0;
+  *y = y0 + T331 * x0 * y0;
+  *yp = yp0 + R43 * y0 + T441 * yp0 * x0 + T431 * x0 * y0 + T432 * xp0 * y0;
+}
+
 long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend, double Po, double z_start,
                              const EH_MAXAMP *maxamp) {
   double b[9], c[9], h, rho0, rho_actual, fse, angle, e1, e2, tilt, etilt, ttilt, cos_ttilt, sin_ttilt;
   double psi1, psi2, Kg1, Kg2, dxi, dyi, dzi, dxf, dyf, dzf, dz_lost, rho, delta_xp;
-  double x, xp, y, yp, s, dp, dp0, Qi[6], Qf[6], *coord, rad_coef;
+  double x, xp, y, yp, s, dp, dp0, Qi[6], Qf[6], *coord, rad_coef, n;
   long i_part, i_top, j, particle_lost;
   or_csbend_fields F;
   or_apdata ap;
@@ -1141,7 +1179,7 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
   }
   if (csbend->integration_order != 2 && csbend->integration_order != 4 && csbend->integration_order != 6)
     return or_fail("CSBEND integration_order is invalid");
-  if (csbend->malignMethod != 0 || csbend->etilt != 0 || csbend->expandHamiltonian || csbend->edge_order > 1)
+  if (csbend->malignMethod != 0 || csbend->etilt != 0 || csbend->expandHamiltonian)
     return or_fail("CSBEND option not supported");
   if ((csbend->edge1_effects != 0 && csbend->edge1_effects != 1) ||
       (csbend->edge2_effects != 0 && csbend->edge2_effects != 1))
@@ -1204,6 +1242,7 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
 
   fse = csbend->fse + csbend->fseDipole;
   h = 1 / rho0;
+  n = -b[1] / h;
   if (fabs(fse + 1) < 1e-12)
     fse = -1 + 1e-12;
   rho_actual = 1 / ((1 + fse) * h);
@@ -1263,9 +1302,12 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
 
     if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) && csbend->edge1_effects == 1) {
       rho = (1 + dp) * rho_actual;
-      delta_xp = OR_TAN(e1) / rho * x;
-      xp += delta_xp;
-      yp -= OR_TAN(e1 - psi1 / (1 + dp)) / rho * y;
+      if (csbend->edge_order <= 1) {
+        delta_xp = OR_TAN(e1) / rho * x;
+        xp += delta_xp;
+        yp -= OR_TAN(e1 - psi1 / (1 + dp)) / rho * y;
+      } else
+        or_apply_edge_effects(&x, &xp, &y, &yp, rho, n, e1, csbend->h1, psi1 * (1 + dp), -1);
     }
     Qi[0] = x;
     Qi[1] = xp;
@@ -1322,9 +1364,12 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
     dp = Qf[5];
     if ((csbend->edgeFlags & EH_BEND_EDGE2_EFFECTS) && csbend->edge2_effects == 1) {
       rho = (1 + dp) * rho_actual;
-      delta_xp = OR_TAN(e2) / rho * x;
-      xp += delta_xp;
-      yp -= OR_TAN(e2 - psi2 / (1 + dp)) / rho * y;
+      if (csbend->edge_order <= 1) {
+        delta_xp = OR_TAN(e2) / rho * x;
+        xp += delta_xp;
+        yp -= OR_TAN(e2 - psi2 / (1 + dp)) / rho * y;
+      } else
+        or_apply_edge_effects(&x, &xp, &y, &yp, rho, n, e2, csbend->h2, psi2 * (1 + dp), 1);
     }
     coord[0] = x * cos_ttilt - y * sin_ttilt + 0.0;
     coord[2] = x * sin_ttilt + y * cos_ttilt + 0.0;

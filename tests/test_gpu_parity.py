@@ -97,6 +97,10 @@ ELEMENT_CASES = [
                     DZ=2e-4, FSE=1e-3, SYNCH_RAD=1)),
     ("CSBEND", dict(L=1.0, ANGLE=2 * math.pi / 20, N_KICKS=8, EDGE1_EFFECTS=0, EDGE2_EFFECTS=0, SYNCH_RAD=1)),
     ("MULT", dict(L=0.2, KNL=400.0, ORDER=3, N_SLICES=4, TILT=0.02, DX=1e-4, SYNCH_RAD=1)),
+    ("CSBEND", dict(L=0.8, ANGLE=0.1, K1=0.4, N_KICKS=8, E1=0.05, E2=0.03, HGAP=0.01, FINT=0.5, H1=0.2, H2=-0.1,
+                    EDGE_ORDER=2)),
+    ("CSBEND", dict(L=0.8, ANGLE=-0.05, K1=0.3, N_KICKS=8, E1=0.02, E2=0.01, HGAP=0.01, EDGE_ORDER=2,
+                    SYNCH_RAD=1)),
 ]
 
 

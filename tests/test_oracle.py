@@ -656,3 +656,20 @@ def test_energy_keeps_momentum(oracle_mod):
     assert pc == math.sqrt((E + 1) ** 2 - 1)
     got = r[r[:, 6] == 1][0]
     assert pc * (1 + got[5]) == pytest.approx(Po * (1 + pts[0, 5]), rel=4e-16)
+
+
+def test_csbend_second_order_edges(oracle_mod):
+    """EDGE_ORDER >= 2 (apply_edge_effects, src/csbend.c:4291-4332): at small
+    amplitude the map reduces to the first-order edge focusing (R21 = tan(e)/rho,
+    R43 = -tan(e - psi)/rho), and the difference grows as the square of the
+    amplitude (the T terms)"""
+    p = dict(L=0.8, ANGLE=0.1, K1=0.4, N_KICKS=8, E1=0.05, E2=0.03, HGAP=0.01, FINT=0.5, H1=0.2, H2=-0.1)
+    d = []
+    for a in (1e-5, 2e-5):
+        pt = [[a, a, a, -a, 0, 0.0]]  # on momentum: psi (1 + delta) = psi / (1 + delta)
+        r2, _ = one(oracle_mod, "CSBEND", dict(p, EDGE_ORDER=2), pt)
+        r1, _ = one(oracle_mod, "CSBEND", dict(p, EDGE_ORDER=1), pt)
+        d.append(np.abs(r2[0, :4] - r1[0, :4]))
+        assert np.allclose(r2[0, :4], r1[0, :4], rtol=1e-4, atol=1e-14)
+    ratio = d[1][1] / d[0][1]
+    assert 3.5 < ratio < 4.5  # quadratic in the amplitude
