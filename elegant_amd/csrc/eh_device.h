@@ -1351,21 +1351,31 @@ EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fter
   case 7:
     lost = eh_rfca(p, d);
     break;
+  /* the full instantiation (RAD) also carries the element types below; the
+   * host runs the lean one only on lines without them */
   case 10: /* MULT (no slices: zero strength and length, nothing to do) */
-    if (d.nslices)
-      lost = eh_mult_t<RAD>(p, d);
+    if constexpr (RAD) {
+      if (d.nslices)
+        lost = eh_mult_t<RAD>(p, d);
+    }
     break;
   case 11: /* ROTATE, src/do_tracking.c:1836-1852 */
-    if (d.on_pass == -1 || d.on_pass == pass)
-      eh_rotate(p, d.ncos, d.nsin);
+    if constexpr (RAD) {
+      if (d.on_pass == -1 || d.on_pass == pass)
+        eh_rotate(p, d.ncos, d.nsin);
+    }
     break;
   case 12: /* MALIGN, src/do_tracking.c:1853-1857 */
-    if (d.on_pass == -1 || d.on_pass == pass)
-      eh_offset(p, d);
+    if constexpr (RAD) {
+      if (d.on_pass == -1 || d.on_pass == pass)
+        eh_offset(p, d);
+    }
     break;
   case 13: /* ENERGY: set_central_momentum, src/do_tracking.c:3591-3595 */
-    if (d.en_apply)
-      p.dp = ((1 + p.dp) * d.en_pold - d.en_pnew) / d.en_pnew;
+    if constexpr (RAD) {
+      if (d.en_apply)
+        p.dp = ((1 + p.dp) * d.en_pold - d.en_pnew) / d.en_pnew;
+    }
     break;
   default: /* MARK, MAXAMP */
     break;
@@ -1381,8 +1391,9 @@ EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fter
   return -1;
 }
 
-/* RAD = false: a kernel for lines without radiating elements carries none of
- * the radiating instantiations (and none of their registers) */
+/* RAD = false: a kernel for lines without radiating elements and without
+ * MULT, ROTATE, MALIGN or ENERGY carries none of that code (and none of its
+ * registers) */
 EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms) {
   return eh_apply_element_t<true>(p, d, pass, terms);
 }

@@ -2146,10 +2146,11 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
     any_kick = any_kick || k.kick;
   if (!any_kick)
     tab = nullptr;
-  /* the radiating kernel only for lines that radiate */
+  /* the full kernel only for lines that radiate or hold MULT, ROTATE, MALIGN
+   * or ENERGY */
   bool rad = false;
   for (long ie = 0; ie < ne; ie++)
-    rad = rad || hd[ie].rad;
+    rad = rad || hd[ie].rad || hd[ie].type >= EH_T_MULT || bl[ie].type >= EH_T_MULT;
   int p = 0, e = 0;
   if (n_passes > 0) {
     /* pass 0: stop before each RFCA whose fiducial is still unknown */
@@ -2847,7 +2848,7 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
     b->kmp_pool = any_kick ? tab : nullptr;
     b->rad = false;
     for (const eh_desc &d : pool)
-      b->rad = b->rad || d.rad;
+      b->rad = b->rad || d.rad || d.type >= EH_T_MULT;
     b->kmp_idx = any_kick ? std::make_shared<const std::vector<int32_t>>(idx) : nullptr;
   }
   auto upload_tables = [&]() -> long {
