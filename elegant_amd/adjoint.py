@@ -172,6 +172,7 @@ class AdjointTracker:
             raise _lib.EHError("eh_adjoint_create failed: %s" % L.eh_last_error().decode())
         self.stats = None
         self.n = 0
+        self.generation = 0  # forward passes so far; the device holds the last one's checkpoints
 
     def fix_rf(self, coords):
         fix_rf(self.lat, coords, self.Po)
@@ -194,6 +195,7 @@ class AdjointTracker:
         alive = np.empty(n, dtype=np.int32)
         arr, keep = self.lat.c_beamline()
         L = _lib_adj()
+        self.generation += 1
         r = L.eh_adjoint_forward(self._h, coords.ctypes.data, n, C.cast(arr, C.c_void_p), len(self.lat),
                                  self.slots.ctypes.data, len(self.spec), self.n_turns, self.Po, self.C,
                                  final.ctypes.data, alive.ctypes.data)
@@ -259,6 +261,7 @@ class BatchAdjointTracker:
             raise _lib.EHError("eh_adjoint_create failed: %s" % L.eh_last_error().decode())
         self.stats = None
         self.counts = []
+        self.generation = 0  # forward passes so far; the device holds the last one's checkpoints
 
     def forward(self, coords_list, theta=None):
         """coords_list: one (n_v, 10) or (n_v, 6) array per variant; returns a
@@ -283,6 +286,7 @@ class BatchAdjointTracker:
             keep.append(k)
         ptrs = (C.c_void_p * len(arrays))(*[C.cast(a, C.c_void_p) for a in arrays])
         counts = np.array(self.counts, dtype=np.int64)
+        self.generation += 1
         r = _lib_adj().eh_adjoint_forward_batch(self._h, allrows.ctypes.data, counts.ctypes.data, len(self.lats),
                                                ptrs, len(self.lats[0]), self.slots.ctypes.data, len(self.spec),
                                                self.n_turns, self.Po, self.C, final.ctypes.data, alive.ctypes.data)
@@ -335,7 +339,10 @@ def _make_function():
 
     class TrackTurns(torch.autograd.Function):
         """(theta, z0) -> (final coordinates, survival mask) through the HIP tracker;
-        backward through the HIP adjoint."""
+        backward through the HIP adjoint.  The tracker keeps the checkpoints of
+        its last forward pass only (on the device), so a graph node can be
+        differentiated only while its forward pass is the tracker's last one;
+        forward also writes theta into the tracker's lattice."""
 
         @staticmethod
         def forward(ctx, theta, z0, tracker):
@@ -343,6 +350,7 @@ def _make_function():
             z = z0.detach().cpu().numpy().astype(np.float64)
             final, alive = tracker.forward(z, th)
             ctx.tracker = tracker
+            ctx.generation = tracker.generation
             ctx.theta_dev = theta.device
             ctx.z_dev = z0.device
             out = torch.from_numpy(final).to(z0.device)
@@ -352,6 +360,9 @@ def _make_function():
 
         @staticmethod
         def backward(ctx, grad_final, grad_mask):
+            if ctx.tracker.generation != ctx.generation:
+                raise RuntimeError("track_turns: the tracker has run another forward pass since this one; "
+                                   "call backward before reusing the tracker (or use one tracker per graph node)")
             g = grad_final.detach().cpu().numpy().astype(np.float64)
             gp, gz = ctx.tracker.backward(g)
             return torch.from_numpy(gp).to(ctx.theta_dev), torch.from_numpy(gz).to(ctx.z_dev), None

@@ -38,6 +38,10 @@
  * behind (integrate_kick_multipole_ordn resets dzLoss only after the entry
  * checks, src/multipole.c:1160-1232), which the host resolves in row order */
 #define EH_STAGE_INHERIT 4
+/* returned by the lean map (RAD = false) for an element type it does not carry
+ * (MULT, ROTATE, MALIGN, ENERGY); the host never launches a lean kernel on a
+ * line that holds one (eh_lean_capable) */
+#define EH_STAGE_REFUSED 8
 
 typedef struct {
   int32_t i, j;
@@ -1352,30 +1356,30 @@ EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fter
     lost = eh_rfca(p, d);
     break;
   /* the full instantiation (RAD) also carries the element types below; the
-   * host runs the lean one only on lines without them */
+   * lean one refuses them (the host runs it only on lines without them) */
   case 10: /* MULT (no slices: zero strength and length, nothing to do) */
-    if constexpr (RAD) {
-      if (d.nslices)
-        lost = eh_mult_t<RAD>(p, d);
-    }
+    if constexpr (!RAD)
+      return EH_STAGE_REFUSED;
+    if (d.nslices)
+      lost = eh_mult_t<RAD>(p, d);
     break;
   case 11: /* ROTATE, src/do_tracking.c:1836-1852 */
-    if constexpr (RAD) {
-      if (d.on_pass == -1 || d.on_pass == pass)
-        eh_rotate(p, d.ncos, d.nsin);
-    }
+    if constexpr (!RAD)
+      return EH_STAGE_REFUSED;
+    if (d.on_pass == -1 || d.on_pass == pass)
+      eh_rotate(p, d.ncos, d.nsin);
     break;
   case 12: /* MALIGN, src/do_tracking.c:1853-1857 */
-    if constexpr (RAD) {
-      if (d.on_pass == -1 || d.on_pass == pass)
-        eh_offset(p, d);
-    }
+    if constexpr (!RAD)
+      return EH_STAGE_REFUSED;
+    if (d.on_pass == -1 || d.on_pass == pass)
+      eh_offset(p, d);
     break;
   case 13: /* ENERGY: set_central_momentum, src/do_tracking.c:3591-3595 */
-    if constexpr (RAD) {
-      if (d.en_apply)
-        p.dp = ((1 + p.dp) * d.en_pold - d.en_pnew) / d.en_pnew;
-    }
+    if constexpr (!RAD)
+      return EH_STAGE_REFUSED;
+    if (d.en_apply)
+      p.dp = ((1 + p.dp) * d.en_pold - d.en_pnew) / d.en_pnew;
     break;
   default: /* MARK, MAXAMP */
     break;
@@ -1391,9 +1395,12 @@ EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fter
   return -1;
 }
 
-/* RAD = false: a kernel for lines without radiating elements and without
- * MULT, ROTATE, MALIGN or ENERGY carries none of that code (and none of its
- * registers) */
+/* whether a descriptor can run through the lean map (RAD = false): kernels for
+ * lines without radiating elements and without MULT, ROTATE, MALIGN or ENERGY
+ * carry none of that code (and none of its registers) */
+EH_HD bool eh_lean_capable(const eh_desc &d) { return !d.rad && d.type < 10; }
+
+
 EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms) {
   return eh_apply_element_t<true>(p, d, pass, terms);
 }

@@ -1286,20 +1286,44 @@ struct eh_batch_dev {
   int nvar;
 };
 
-/* work list for the next span: wave_start = prefix of ceil(alive/64); clears the
- * output counters and the queue (a kernel, not a memset, so it can live in a graph) */
+/* the batched kernel's work list: variant v's alive particles form ceil(alive/64)
+ * one-wavefront items numbered from wave_start[v]; wave_start[nvar] is the total.
+ * Host and device code, so that the work list can be checked on the CPU. */
+EH_HD int32_t eh_batch_wave_starts(const int32_t *alive, int nvar, int32_t *wave_start) {
+  int32_t acc = 0;
+  for (int v = 0; v < nvar; v++) {
+    wave_start[v] = acc;
+    acc += (alive[v] + 63) / 64;
+  }
+  wave_start[nvar] = acc;
+  return acc;
+}
+
+/* the variant of work item w (0 <= w < wave_start[nvar]): the v with
+ * wave_start[v] <= w < wave_start[v + 1]; variants without particles own no item */
+EH_HD int eh_batch_item_variant(const int32_t *wave_start, int nvar, int32_t w) {
+  int lo = 0, hi = nvar; /* wave_start[lo] <= w < wave_start[hi] */
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (wave_start[mid] <= w)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+/* work list for the next span from the previous span's survivor counts (or, on
+ * the first span of a call, from alive_in as uploaded); clears the output
+ * counters and the queue (a kernel, not a memset, so it can live in a graph) */
 __global__ void k_batch_prep(const unsigned long long *alive_prev, int32_t *alive_in, int32_t *wave_start,
                              unsigned long long *alive_out, int32_t *queue, int nvar, int first) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    int32_t acc = 0;
     for (int v = 0; v < nvar; v++) {
-      int32_t a = first ? alive_in[v] : (int32_t)alive_prev[v];
-      alive_in[v] = a;
-      wave_start[v] = acc;
-      acc += (a + 63) / 64;
+      alive_in[v] = first ? alive_in[v] : (int32_t)alive_prev[v];
       alive_out[v] = 0;
     }
-    wave_start[nvar] = acc;
+    eh_batch_wave_starts(alive_in, nvar, wave_start);
     *queue = 0;
   }
 }
@@ -1322,15 +1346,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_batch(eh_soa a, eh_
     w = __builtin_amdgcn_readfirstlane(__shfl(w, 0, 64));
     if (w >= total)
       break;
-    int lo = 0, hi = bd.nvar; /* wave_start[lo] <= w < wave_start[hi] */
-    while (hi - lo > 1) {
-      int mid = (lo + hi) >> 1;
-      if (bd.wave_start[mid] <= w)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    const int v = lo;
+    const int v = eh_batch_item_variant(bd.wave_start, bd.nvar, w);
     const long local = (long)(w - bd.wave_start[v]) * 64 + lane;
     const bool in = local < bd.alive_in[v];
     const long i = bd.region_off[v] + local;
@@ -1693,28 +1709,47 @@ static long fiducial_tmean(eh_beam *b, double sOffset, double p0, double *tFid) 
   return EH_OK;
 }
 
-/* first-encounter fiducial, src/simple_rfca.c:414-463 */
-static long rfca_fiducial(eh_beam *b, EH_RFCA *rf, double Pc, double zEnd) {
-  if (rf->fiducial_seen || rf->volt == 0)
-    return EH_OK;
+/* an RF cavity whose phase fiducial is still to be taken from the beam on its
+ * first pass (src/simple_rfca.c:414-463; a cavity without voltage never takes one) */
+static bool rf_pending(const EH_ELEMENT_LIST *e) {
+  if (e->type != EH_T_RFCA)
+    return false;
+  const EH_RFCA *rf = (const EH_RFCA *)e->p_elem;
+  return !rf->fiducial_seen && rf->volt != 0;
+}
+
+/* fiducial time of a cavity's first pass: TMEAN from the beam (tmean_of), LIGHT
+ * from its position, or T_REFERENCE (src/simple_rfca.c:414-463) */
+template <class TMean>
+static long rf_take_fiducial(EH_RFCA *rf, double zEnd, TMean tmean_of) {
   double length = rf->length;
-  long nKicks = length ? rf->nKicks : 1;
-  if (nKicks <= 0)
-    nKicks = 1;
-  length /= nKicks;
-  double omega = 2 * EH_PI * rf->freq, t0;
+  long nk = length ? rf->nKicks : 1;
+  if (nk <= 0)
+    nk = 1;
+  length /= nk;
+  double t0;
   if (rf->tReference != -1)
     t0 = rf->tReference;
   else if (rf->fiducial_mode == EH_FID_LIGHT)
     t0 = ((zEnd - rf->length) + length / 2) / EH_C_MKS;
   else {
-    long r = fiducial_tmean(b, length / 2, Pc, &t0);
+    long r = tmean_of(length / 2, &t0);
     if (r < 0)
       return r;
   }
-  rf->phase_fiducial = -omega * t0;
+  rf->phase_fiducial = -(2 * EH_PI * rf->freq) * t0;
   rf->fiducial_seen = 1;
+  if (rf->freq != 0)
+    rf->t_fiducial = -rf->phase_fiducial / (2 * EH_PI * rf->freq);
   return EH_OK;
+}
+
+/* first-encounter fiducial, src/simple_rfca.c:414-463 */
+static long rfca_fiducial(eh_beam *b, EH_RFCA *rf, double Pc, double zEnd) {
+  EH_ELEMENT_LIST e = {EH_T_RFCA, 0, "RFCA", (void *)rf};
+  if (!rf_pending(&e))
+    return EH_OK;
+  return rf_take_fiducial(rf, zEnd, [&](double sOffset, double *t0) { return fiducial_tmean(b, sOffset, Pc, t0); });
 }
 
 extern "C" long eh_simple_rf_cavity(eh_beam *b, long np, EH_RFCA *rfca, double *P_central, double zEnd) {
@@ -2123,7 +2158,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
     return build_desc(&hd[ie], bl + ie, c, pool);
   };
   for (long ie = 0; ie < ne; ie++) {
-    if (bl[ie].type == EH_T_RFCA && !((EH_RFCA *)bl[ie].p_elem)->fiducial_seen)
+    if (rf_pending(bl + ie))
       continue; /* built once its fiducial is known */
     r = build(ie);
     if (r < 0)
@@ -2147,27 +2182,23 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
   if (!any_kick)
     tab = nullptr;
   /* the full kernel only for lines that radiate or hold MULT, ROTATE, MALIGN
-   * or ENERGY */
+   * or ENERGY (RF cavities still pending are built later; they are lean) */
   bool rad = false;
   for (long ie = 0; ie < ne; ie++)
-    rad = rad || hd[ie].rad || hd[ie].type >= EH_T_MULT || bl[ie].type >= EH_T_MULT;
+    rad = rad || !eh_lean_capable(hd[ie]) || bl[ie].type >= EH_T_MULT;
   int p = 0, e = 0;
   if (n_passes > 0) {
     /* pass 0: stop before each RFCA whose fiducial is still unknown */
     for (long ie = 0; ie < ne; ie++) {
-      if (bl[ie].type != EH_T_RFCA)
+      if (!rf_pending(bl + ie))
         continue;
       EH_RFCA *rf = (EH_RFCA *)bl[ie].p_elem;
-      if (rf->fiducial_seen)
-        continue;
       r = launch_fused(b, ne, p, e, 0, (int)ie, effort, tab, rad);
       if (r < 0)
         return r;
       r = rfca_fiducial(b, rf, Pe0[ie], ze[ie]);
       if (r < 0)
         return r;
-      if (rf->volt != 0 && rf->freq != 0)
-        rf->t_fiducial = -rf->phase_fiducial / (2 * EH_PI * rf->freq);
       for (long je = 0; je < ne; je++)
         if (bl[je].p_elem == bl[ie].p_elem) {
           r = build(je);
@@ -2729,81 +2760,133 @@ static long batch_sync_counts(eh_batch *b) {
   return alive;
 }
 
+/* the context of every element of one beamline: its s range from the line start,
+ * the MAXAMP in force on pass 0 and on later passes (a MAXAMP stays in force
+ * across the end of a pass), and the limits of the sweep that follows it */
+static void line_contexts(const EH_ELEMENT_LIST *bl, long ne, double Pc, std::vector<eh_ctx> &ctx) {
+  ctx.assign(ne, eh_ctx{});
+  const EH_MAXAMP *last_amp = nullptr;
+  double z = 0;
+  for (long e = 0; e < ne; e++) {
+    ctx[e].Po = Pc;
+    ctx[e].z_start = z;
+    z += elem_length(bl + e);
+    ctx[e].z_end = z;
+    if (bl[e].type == EH_T_MAXAMP)
+      last_amp = (const EH_MAXAMP *)bl[e].p_elem;
+  }
+  const EH_MAXAMP *cur0 = nullptr, *cur1 = last_amp;
+  for (long e = 0; e < ne; e++) {
+    if (bl[e].type == EH_T_MAXAMP)
+      cur0 = cur1 = (const EH_MAXAMP *)bl[e].p_elem;
+    eh_ctx &c = ctx[e];
+    c.ap[0] = cur0;
+    c.ap[1] = cur1;
+    for (int pi = 0; pi < 2; pi++) {
+      const EH_MAXAMP *m = pi ? cur1 : cur0;
+      c.post_x[pi] = m ? m->x_max : 0;
+      c.post_y[pi] = m ? m->y_max : 0;
+      c.post_on[pi] = (bl[e].type != EH_T_MARK) && m && (m->x_max || m->y_max);
+    }
+  }
+}
+
+/* append one descriptor (and its field terms) to the pool; returns its index */
+static int32_t pool_push(eh_desc d, const std::vector<eh_fterm> &ft, std::vector<eh_desc> &pool,
+                         std::vector<eh_fterm> &terms) {
+  if (!ft.empty()) {
+    /* this element's term offsets are relative to its own list */
+    int32_t shift = (int32_t)terms.size();
+    d.csb_fx0 += shift;
+    d.csb_fy0 += shift;
+    if (d.csb_dense) {
+      d.csb_dx0 += shift;
+      d.csb_dy0 += shift;
+    }
+    terms.insert(terms.end(), ft.begin(), ft.end());
+  }
+  pool.push_back(d);
+  return (int32_t)pool.size() - 1;
+}
+
 /* the descriptor pool of all variants, de-duplicated by content (descriptor
  * bytes and field terms), with idx[v*ne + e] the descriptor of element e of
- * variant v.  RF cavities whose fiducial is still unknown get a MARK placeholder
- * (never shared); their descriptors are built once the fiducial is known. */
+ * variant v.  RF cavities whose fiducial is still pending get a MARK placeholder
+ * (never shared); batch_rf_update replaces it once the fiducial is known.  Host
+ * code only (no HIP calls), so that tests can run it on the CPU. */
 static long batch_tables(EH_ELEMENT_LIST *const *beamlines, long nvar, long ne, double Pc,
                          std::vector<eh_desc> &pool, std::vector<eh_fterm> &terms, std::vector<int32_t> &idx) {
   std::unordered_map<std::string, int32_t> seen;
   pool.clear();
   terms.clear();
   idx.assign((size_t)nvar * ne, 0);
+  std::vector<eh_ctx> ctx;
   for (long v = 0; v < nvar; v++) {
     const EH_ELEMENT_LIST *bl = beamlines[v];
-    std::vector<double> zs(ne), ze(ne);
-    const EH_MAXAMP *last_amp = nullptr;
-    double z = 0;
+    line_contexts(bl, ne, Pc, ctx);
     for (long e = 0; e < ne; e++) {
-      zs[e] = z;
-      z += elem_length(bl + e);
-      ze[e] = z;
-      if (bl[e].type == EH_T_MAXAMP)
-        last_amp = (const EH_MAXAMP *)bl[e].p_elem;
-    }
-    const EH_MAXAMP *cur0 = nullptr, *cur1 = last_amp;
-    for (long e = 0; e < ne; e++) {
-      if (bl[e].type == EH_T_MAXAMP)
-        cur0 = cur1 = (const EH_MAXAMP *)bl[e].p_elem;
-      eh_ctx c = {};
-      c.Po = Pc;
-      c.z_start = zs[e];
-      c.z_end = ze[e];
-      c.ap[0] = cur0;
-      c.ap[1] = cur1;
-      for (int pi = 0; pi < 2; pi++) {
-        const EH_MAXAMP *m = pi ? cur1 : cur0;
-        c.post_x[pi] = m ? m->x_max : 0;
-        c.post_y[pi] = m ? m->y_max : 0;
-        c.post_on[pi] = (bl[e].type != EH_T_MARK) && m && (m->x_max || m->y_max);
-      }
       eh_desc d;
       std::vector<eh_fterm> ft;
       long r;
-      if (bl[e].type == EH_T_RFCA && !((EH_RFCA *)bl[e].p_elem)->fiducial_seen) {
+      if (rf_pending(bl + e)) {
         std::memset(&d, 0, sizeof d);
         d.type = EH_T_MARK;
         d.adj_pk = d.adj_pdx = d.adj_pdy = d.adj_anchor = -1;
         r = EH_OK;
       } else
-        r = build_desc(&d, bl + e, c, ft);
+        r = build_desc(&d, bl + e, ctx[e], ft);
       if (r < 0)
         return r;
       std::string key((const char *)&d, sizeof d);
       key.append((const char *)ft.data(), sizeof(eh_fterm) * ft.size());
       auto it = seen.find(key);
-      if (it != seen.end() && !(bl[e].type == EH_T_RFCA)) {
+      if (it != seen.end() && bl[e].type != EH_T_RFCA) {
         idx[(size_t)v * ne + e] = it->second;
       } else {
-        if (!ft.empty()) {
-          /* this element's term offsets are relative to its own list */
-          int32_t shift = (int32_t)terms.size();
-          d.csb_fx0 += shift;
-          d.csb_fy0 += shift;
-          if (d.csb_dense) {
-            d.csb_dx0 += shift;
-            d.csb_dy0 += shift;
-          }
-          terms.insert(terms.end(), ft.begin(), ft.end());
-        }
-        int32_t id = (int32_t)pool.size();
-        pool.push_back(d);
+        int32_t id = pool_push(d, ft, pool, terms);
         seen.emplace(key, id);
         idx[(size_t)v * ne + e] = id;
       }
     }
   }
   return EH_OK;
+}
+
+/* once the RF cavity at position er has its fiducial in every variant: a fresh
+ * descriptor for every position of each variant's line that holds the same
+ * cavity (the same element struct, as track_fused does).  Host code only. */
+static long batch_rf_update(EH_ELEMENT_LIST *const *beamlines, long nvar, long ne, long er, double Pc,
+                            std::vector<eh_desc> &pool, std::vector<eh_fterm> &terms, std::vector<int32_t> &idx) {
+  std::vector<eh_ctx> ctx;
+  for (long v = 0; v < nvar; v++) {
+    const EH_ELEMENT_LIST *bl = beamlines[v];
+    line_contexts(bl, ne, Pc, ctx);
+    for (long je = 0; je < ne; je++) {
+      if (bl[je].p_elem != bl[er].p_elem || bl[je].type != EH_T_RFCA)
+        continue;
+      eh_desc d;
+      std::vector<eh_fterm> ft;
+      long r = build_desc(&d, bl + je, ctx[je], ft);
+      if (r < 0)
+        return r;
+      idx[(size_t)v * ne + je] = pool_push(d, ft, pool, terms);
+    }
+  }
+  return EH_OK;
+}
+
+/* the device tables of the last call can serve this one only if the beamlines
+ * are byte-for-byte unchanged and no RF cavity still waits for its fiducial
+ * (whose placeholder the pass-0 stop has to replace in the host copy) */
+static bool batch_tables_reusable(bool have_tables, long tables_ne, uint64_t tables_hash,
+                                  EH_ELEMENT_LIST *const *beamlines, long nvar, long ne, uint64_t hash) {
+  if (!have_tables || tables_ne != ne || tables_hash != hash)
+    return false;
+  for (long v = 0; v < nvar; v++)
+    for (long e = 0; e < ne; e++)
+      if (rf_pending(beamlines[v] + e))
+        return false;
+  return true;
 }
 
 extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, long ne, long n_passes,
@@ -2836,7 +2919,8 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
     if (beamlines[0][e].type == EH_T_RFCA)
       rf_elems.push_back(e);
   const uint64_t hash = beamlines_hash(beamlines, b->nvar, ne, Pc);
-  const bool reuse = b->d_pool && b->tables_ne == ne && b->tables_hash == hash;
+  const bool reuse =
+      batch_tables_reusable(b->d_pool != nullptr, b->tables_ne, b->tables_hash, beamlines, b->nvar, ne, hash);
   if (!reuse) {
     long r = batch_tables(beamlines, b->nvar, ne, Pc, pool, terms, idx);
     if (r < 0)
@@ -2848,7 +2932,7 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
     b->kmp_pool = any_kick ? tab : nullptr;
     b->rad = false;
     for (const eh_desc &d : pool)
-      b->rad = b->rad || d.rad || d.type >= EH_T_MULT;
+      b->rad = b->rad || !eh_lean_capable(d);
     b->kmp_idx = any_kick ? std::make_shared<const std::vector<int32_t>>(idx) : nullptr;
   }
   auto upload_tables = [&]() -> long {
@@ -2898,12 +2982,13 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
   };
   if (n_passes > 0) {
     for (long er : rf_elems) {
-      bool unseen = false;
+      bool pending = false;
       for (long v = 0; v < b->nvar; v++)
-        if (!((EH_RFCA *)beamlines[v][er].p_elem)->fiducial_seen)
-          unseen = true;
-      if (!unseen)
+        pending = pending || rf_pending(beamlines[v] + er);
+      if (!pending)
         continue;
+      if (reuse)
+        return fail(EH_ERR_STATE, "eh_batch_track: reused tables with a pending RF fiducial");
       if (e < er) {
         HIPCHK(hipEventRecord(b->ev0, b->stream));
         r = batch_enqueue_span(b, ne, p, e, 0, (int)er, first, t.block, t.waves_per_cu);
@@ -2918,63 +3003,19 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
           return r;
       }
       for (long v = 0; v < b->nvar; v++) {
-        EH_RFCA *rf = (EH_RFCA *)beamlines[v][er].p_elem;
-        if (!rf->fiducial_seen && rf->volt != 0) {
-          double length = rf->length;
-          long nk = length ? rf->nKicks : 1;
-          if (nk <= 0)
-            nk = 1;
-          length /= nk;
-          double z = 0, t0;
-          for (long k = 0; k <= er; k++)
-            z += elem_length(beamlines[v] + k);
-          if (rf->tReference != -1)
-            t0 = rf->tReference;
-          else if (rf->fiducial_mode == EH_FID_LIGHT)
-            t0 = ((z - rf->length) + length / 2) / EH_C_MKS;
-          else {
-            r = batch_fiducial(b, v, length / 2, Pc, &t0);
-            if (r < 0)
-              return r;
-          }
-          rf->phase_fiducial = -(2 * EH_PI * rf->freq) * t0;
-          rf->fiducial_seen = 1;
-          if (rf->freq != 0)
-            rf->t_fiducial = -rf->phase_fiducial / (2 * EH_PI * rf->freq);
-        }
-        double z = 0, zs = 0;
-        for (long k = 0; k <= er; k++) {
-          zs = z;
-          z += elem_length(beamlines[v] + k);
-        }
-        eh_ctx c = {};
-        c.Po = Pc;
-        c.z_start = zs;
-        c.z_end = z;
-        eh_desc d;
-        std::vector<eh_fterm> ft;
-        r = build_desc(&d, beamlines[v] + er, c, ft);
+        if (!rf_pending(beamlines[v] + er))
+          continue;
+        double zEnd = 0;
+        for (long k = 0; k <= er; k++)
+          zEnd += elem_length(beamlines[v] + k);
+        r = rf_take_fiducial((EH_RFCA *)beamlines[v][er].p_elem, zEnd,
+                             [&](double sOffset, double *t0) { return batch_fiducial(b, v, sOffset, Pc, t0); });
         if (r < 0)
           return r;
-        /* the RFCA post sweeps follow the MAXAMP in force at that point */
-        const eh_desc &old = pool[idx[(size_t)v * ne + er]];
-        (void)old;
-        const EH_MAXAMP *cur0 = nullptr, *cur1 = nullptr;
-        for (long k = 0; k < ne; k++)
-          if (beamlines[v][k].type == EH_T_MAXAMP)
-            cur1 = (const EH_MAXAMP *)beamlines[v][k].p_elem;
-        for (long k = 0; k <= er; k++)
-          if (beamlines[v][k].type == EH_T_MAXAMP)
-            cur0 = cur1 = (const EH_MAXAMP *)beamlines[v][k].p_elem;
-        for (int pi = 0; pi < 2; pi++) {
-          const EH_MAXAMP *m = pi ? cur1 : cur0;
-          d.post_xmax[pi] = m ? m->x_max : 0;
-          d.post_ymax[pi] = m ? m->y_max : 0;
-          d.post_ap[pi] = m && (m->x_max || m->y_max);
-        }
-        idx[(size_t)v * ne + er] = (int32_t)pool.size();
-        pool.push_back(d);
       }
+      r = batch_rf_update(beamlines, b->nvar, ne, er, Pc, pool, terms, idx);
+      if (r < 0)
+        return r;
       r = upload_tables();
       if (r < 0)
         return r;
@@ -3426,11 +3467,14 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
         if (e > 0)
           d.adj_anchor = na++;
       }
+      /* k_adj_fwd runs the lean map */
       if (t == EH_T_MULT || t == EH_T_ROTATE || t == EH_T_MALIGN || t == EH_T_ENERGY)
         return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support MULT, ROTATE, MALIGN or ENERGY",
                     e);
       if (d.rad)
         return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support SYNCH_RAD", e);
+      if (!eh_lean_capable(d))
+        return fail(EH_ERR_STATE, "element %ld: not trackable by the adjoint's map", e);
       if ((t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE) && d.expand)
         return fail(EH_ERR_UNSUPPORTED, "element %ld: adjoint needs EXPAND_HAMILTONIAN=0", e);
       for (int k = 0; k < 3; k++) {

@@ -241,3 +241,37 @@ def test_batched_variants_equal_individual(ckpt):
     assert _rel(gp, total) < 1e-12, (gp, total)
     assert bt.stats["n_survivors"] == sum(int(r[1].sum()) for r in res)
     bt.free()
+
+
+def test_track_turns_autograd_function():
+    """loss.backward() through track_turns gives the gradients of
+    AdjointTracker.backward and of PyTorch autograd through the independent
+    restatement; a second forward on the same tracker before backward is refused"""
+    _say("track_turns torch.autograd.Function")
+    Po = configs.p_central_from_mev(2000.0)
+    lat = _mix_ring()
+    spec = adjoint.ParamSpec.strengths(lat) + adjoint.ParamSpec.offsets(lat)
+    rows = configs.fodo_beam(64, seed=12)
+    rows[:, :4] *= 2.0
+    turns = 4
+    W = np.random.default_rng(6).normal(size=(len(rows), 6)) * np.array([1e3, 1e4, 1e3, 1e4, 1.0, 1e3])
+    adj = adjoint.AdjointTracker(lat, spec, Po, n_turns=turns, checkpoint_every=2)
+    theta = torch.tensor(spec.values(lat), dtype=torch.float64, requires_grad=True)
+    z0 = torch.tensor(rows[:, :6], dtype=torch.float64, requires_grad=True)
+    final, alive = adjoint.track_turns(theta, z0, adj)
+    assert bool(alive.all())
+    (final * torch.tensor(W)).sum().backward()
+    gp, gz = adj.backward(W)  # the same forward pass, directly
+    np.testing.assert_array_equal(theta.grad.numpy(), gp)
+    np.testing.assert_array_equal(z0.grad.numpy(), gz)
+    tp, tz, tout = _torch_grads(_mix_ring(), spec, Po, rows, turns, W)
+    assert _rel(final.detach().numpy(), tout) < 1e-9
+    assert _rel(theta.grad.numpy(), tp) < 1e-7
+    assert _rel(z0.grad.numpy(), tz) < 1e-7
+    # two graph nodes on one tracker: the first node's checkpoints are gone
+    th2 = theta.detach().clone().requires_grad_(True)
+    f1, _ = adjoint.track_turns(th2, z0.detach(), adj)
+    f2, _ = adjoint.track_turns(th2 * 1.0, z0.detach(), adj)
+    with pytest.raises(RuntimeError, match="another forward pass"):
+        (f1 * torch.tensor(W)).sum().backward()
+    adj.free()

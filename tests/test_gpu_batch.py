@@ -136,3 +136,70 @@ def test_batch_radiation():
         total += nref
         assert_rows_identical(batch.download(v), ref, batch.n_alive(v), nref, "variant %d" % v)
     assert n == total
+
+
+def _rf_variants(nvar, volt, shared=False):
+    from tests.test_batch_host import _rf_line
+    out = []
+    for v in range(nvar):
+        lat, Po = _rf_line(volt=volt, shared=shared, two=shared)
+        lat.set_param(1, "K1", 0.8 + 0.02 * v)
+        out.append(lat)
+    return out, Po
+
+
+def _small_beams(counts, seed):
+    rng = np.random.default_rng(seed)
+    beams = []
+    for c in counts:
+        b = np.zeros((c, 10))
+        b[:, :4] = rng.normal(0, [5e-3, 5e-4, 4e-3, 4e-4], (c, 4))
+        b[:, 5] = rng.normal(0, 2e-3, c)
+        b[:, 6] = np.arange(1, c + 1)
+        beams.append(b)
+    return beams
+
+
+@pytest.mark.parametrize("volt", [0.0, 2e6])
+def test_batch_two_calls_and_zero_pass_call(volt):
+    """a call with n_passes=0, then 2 passes, then 1 more, on unchanged
+    beamlines (device tables kept between calls unless an RF fiducial is still
+    pending), each variant equal to the oracle's consecutive do_tracking calls"""
+    print("batch two calls volt", volt, flush=True)
+    nvar = 3
+    lats, Po = _rf_variants(nvar, volt)
+    refl, _ = _rf_variants(nvar, volt)
+    counts = [150, 64, 97]
+    beams = _small_beams(counts, 21)
+    batch = tracking.Batch(beams)
+    n0, _ = batch.track(lats, 0, Po)
+    assert n0 == sum(counts)
+    batch.track(lats, 2, Po)
+    n, _ = batch.track(lats, 1, Po)
+    total = 0
+    for v in range(nvar):
+        r1, n1, _, _ = oracle.do_tracking(beams[v], refl[v], 2, Po)
+        r2, n2, _, _ = oracle.do_tracking(r1[:n1], refl[v], 1, Po)
+        expect = np.concatenate([r2, r1[n1:]])
+        assert_rows_identical(batch.download(v), expect, batch.n_alive(v), n2, "variant %d" % v)
+        total += n2
+    assert n == total
+
+
+def test_batch_shared_cavity_struct():
+    """one RF cavity struct at two positions of each line (and a second cavity):
+    every position gets the fiducial the first one took"""
+    print("batch shared cavity", flush=True)
+    nvar = 3
+    lats, Po = _rf_variants(nvar, 2e6, shared=True)
+    refl, _ = _rf_variants(nvar, 2e6, shared=True)
+    counts = [130, 1, 70]
+    beams = _small_beams(counts, 22)
+    batch = tracking.Batch(beams)
+    n, _ = batch.track(lats, 3, Po, span_elements=4)
+    total = 0
+    for v in range(nvar):
+        ref, nref, _, _ = oracle.do_tracking(beams[v], refl[v], 3, Po)
+        total += nref
+        assert_rows_identical(batch.download(v), ref, batch.n_alive(v), nref, "variant %d" % v)
+    assert n == total

@@ -354,13 +354,16 @@ def test_radiation_ring(mode, ehlib, oracle_mod):
     assert stats["particle_element_turns"] == eff
 
 
-@pytest.mark.skip(reason="the segment-major work-queue variant of k_fused stalled on the GPU (a fused FODO run "
-                         "did not finish in 120 s) and was taken out; the cause is not shown (see DESIGN.md)")
+@pytest.mark.skip(reason="the segment-major work-queue variant of k_fused (entry point "
+                         "tracking.set_fused_segment, now removed) stalled on the GPU (a fused FODO run did not "
+                         "finish in 120 s) and was taken out; the cause is not shown")
 @pytest.mark.parametrize("seg", [7, 700, 100000])
 def test_fused_work_queue_segments(seg, ehlib, oracle_mod):
     """fused launches of several turns run as a segment-major work queue; any
     segment length (here shorter than a turn, not dividing it, and longer than
     the whole launch) gives the oracle's result bit for bit, losses included"""
+    if not hasattr(tracking, "set_fused_segment"):
+        pytest.fail("the segment-major work-queue entry point was removed; this test cannot run without it")
     _say("fused work queue, segment %d" % seg)
     Po = configs.p_central_from_mev(3000.0)
     lat = configs.mba_ring(Po)

@@ -80,3 +80,37 @@ def test_batched_seed_tuning_matches_per_seed_trackers():
     assert np.allclose(vb, vs, rtol=1e-10, atol=0)
     assert np.allclose(tb, ts, rtol=1e-10, atol=1e-14)
     assert vb[-1] < vb[0]
+
+
+def test_batched_seed_gradient_vs_autograd_fixture():
+    """the batched HIP adjoint's gradient of the error-seed sextupole-tuning
+    objective (error-seeded MBA rings, softplus aperture penalty averaged over
+    seeds, twelve split sextupole families, a checkpoint inside the run) equals
+    PyTorch autograd through the independent restatement of the maps, frozen in
+    tests/golden/cfg4_gradient.npz by tests/golden/make_cfg4_gradient.py"""
+    import os
+    import sys
+    print("batched seed gradient vs autograd fixture", flush=True)
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    import make_cfg4_gradient as mk
+    ref = np.load(os.path.join(here, "cfg4_gradient.npz"))
+    Po, lats, spec, beams = mk.workload()
+    np.testing.assert_array_equal(spec.values(lats[0]), ref["theta"])
+    bt = adjoint.BatchAdjointTracker(lats, spec, Po, n_turns=mk.TURNS, checkpoint_every=2)
+    res = bt.forward(beams, ref["theta"])
+    for v in range(mk.NSEED):
+        np.testing.assert_array_equal(mk.rf_fiducials(lats[v]), ref["fiducials"][v])  # TMEAN, bit for bit
+        assert res[v][1].all()
+        fin, tf = res[v][0], ref["finals"][v]
+        assert np.linalg.norm(fin - tf) <= 1e-8 * np.linalg.norm(tf)  # the restatement is pinned to ~1e-10
+    obj = studies.aperture_penalty(r_ap=float(ref["r_ap"]), width=float(ref["width"]))
+    Ls, Gs = zip(*[obj(f, a) for f, a in res])
+    gp, _ = bt.backward(list(Gs))
+    gp = gp / mk.NSEED
+    L = float(np.mean(Ls))
+    print("L", L, ref["L"], "grad", gp, ref["grad"], flush=True)
+    assert abs(L - float(ref["L"])) <= 1e-12 * abs(float(ref["L"]))
+    assert np.linalg.norm(gp - ref["grad"]) <= 1e-7 * np.linalg.norm(ref["grad"])
+    assert np.all(np.abs(gp - ref["grad"]) <= 1e-6 * np.abs(ref["grad"]) + 1e-7 * np.linalg.norm(ref["grad"]))
+    bt.free()
