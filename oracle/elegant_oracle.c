@@ -1417,6 +1417,14 @@ double or_findFiducialTime(double **part, long np, double s0, double sOffset, do
   return tFid;
 }
 
+/* OR_PLANT (tests only: the RFCA checks build the oracle with a planted mistake
+ * and show that they reject it): 1 R12 with the momentum ratio inverted, 2 R22
+ * growing with the energy, 3 exit path-length rescaling inverted, 4 the kick
+ * method's slope ratio inverted.  0 (the default) is the reference's code. */
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
+
 /* add_to_particle_energy, src/simple_rfca.c:987-1006 */
 static void add_to_particle_energy(double *coord, double timeOfFlight, double Po, double dgamma) {
   double gamma, gamma1, PRatio, P, P1, Pz1, Pz;
@@ -1429,7 +1437,7 @@ static void add_to_particle_energy(double *coord, double timeOfFlight, double Po
   coord[4] = timeOfFlight * c_mks * P1 / gamma1;
   Pz = P / sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
   Pz1 = sqrt(Pz * Pz + gamma1 * gamma1 - gamma * gamma);
-  PRatio = Pz / Pz1;
+  PRatio = OR_PLANT == 4 ? Pz1 / Pz : Pz / Pz1;
   coord[1] *= PRatio;
   coord[3] *= PRatio;
 }
@@ -1579,9 +1587,9 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
           coord[3] -= coord[2] * inverseF;
         }
         dP = sqrt(sqr(gamma + dgamma) - 1) - P;
-        R22 = 1 / (1 + dP / P);
+        R22 = OR_PLANT == 2 ? 1 + dP / P : 1 / (1 + dP / P);
         if (fabs(dP / P) > 1e-14)
-          R12 = length * (P / dP * OR_LOG(1 + dP / P));
+          R12 = OR_PLANT == 1 ? length * (dP / P * OR_LOG(1 + dP / P)) : length * (P / dP * OR_LOG(1 + dP / P));
         else
           R12 = length;
         coord[4] += ds1;
@@ -1602,7 +1610,7 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
           coord[1] -= coord[0] * inverseF;
           coord[3] -= coord[2] * inverseF;
         }
-        coord[4] = (P + dP) / gamma * coord[4] / beta_i;
+        coord[4] = OR_PLANT == 3 ? beta_i * coord[4] / ((P + dP) / gamma) : (P + dP) / gamma * coord[4] / beta_i;
       }
     }
     (void)dgammaMax;

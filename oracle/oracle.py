@@ -24,6 +24,21 @@ def load(libm=False):
     path = os.path.join(HERE, name)
     if not os.path.exists(path):
         build()
+    _libs[name] = load_path(path)
+    return _libs[name]
+
+
+def build_variant(out_path, defines):
+    """compile the checker with extra -D definitions (e.g. a planted mistake,
+    OR_PLANT=n) into out_path; returns the loaded library"""
+    cmd = ["gcc", "-m64", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I" + os.path.join(HERE, "..", "include"),
+           "-I" + os.path.join(HERE, "..", "elegant_amd", "csrc")] + ["-D" + d for d in defines] + \
+        ["-shared", "-o", out_path, os.path.join(HERE, "elegant_oracle.c"), "-lm"]
+    subprocess.run(cmd, check=True)
+    return load_path(out_path)
+
+
+def load_path(path):
     lib = C.CDLL(path)
     dp = C.POINTER(C.c_double)
     vp = C.c_void_p
@@ -39,7 +54,6 @@ def load(libm=False):
     for f in ("or_sin", "or_cos", "or_asin", "or_log"):
         getattr(lib, f).argtypes = [C.c_double]
         getattr(lib, f).restype = C.c_double
-    _libs[name] = lib
     return lib
 
 
@@ -62,9 +76,10 @@ def do_tracking(coords, lattice, n_passes, p_central, libm=False):
     return rows, n, eff.value, pc.value
 
 
-def element(coords, lattice, index, p_central, z_start, z_end, maxamp=None, libm=False):
-    """Apply one element routine (no sweeps) to a copy of coords."""
-    lib = load(libm)
+def element(coords, lattice, index, p_central, z_start, z_end, maxamp=None, libm=False, lib=None):
+    """Apply one element routine (no sweeps) to a copy of coords (lib: a checker
+    build other than the default, e.g. from build_variant)."""
+    lib = lib or load(libm)
     rows = np.array(coords, dtype=np.float64, order="C", copy=True)
     arr, keep = lattice.c_beamline()
     pc = C.c_double(p_central)
