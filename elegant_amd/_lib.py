@@ -24,6 +24,7 @@ EXPORTS = [
     "eh_batch_download", "eh_batch_n_alive", "eh_batch_track", "eh_diag_copy", "eh_adjoint_create",
     "eh_adjoint_free", "eh_adjoint_forward", "eh_adjoint_backward", "eh_adjoint_forward_batch",
     "eh_multipole_tracking", "eh_offset_beam", "eh_rotate_beam", "eh_replay_sequence", "eh_set_central_momentum",
+    "eh_track_particles", "eh_beam_row_order",
 ]
 
 
@@ -77,6 +78,10 @@ def load():
     lib.eh_rotate_beam.restype = L
     lib.eh_set_central_momentum.argtypes = [vp, L, D, dp]
     lib.eh_set_central_momentum.restype = L
+    lib.eh_track_particles.argtypes = [vp, L, C.POINTER(S.EH_VMATRIX)]
+    lib.eh_track_particles.restype = L
+    lib.eh_beam_row_order.argtypes = [vp, vp, L]
+    lib.eh_beam_row_order.restype = L
     lib.eh_exactDrift.argtypes = [vp, L, D]
     lib.eh_exactDrift.restype = L
     lib.eh_drift_beam.argtypes = [vp, L, D, L]

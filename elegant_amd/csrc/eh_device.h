@@ -179,6 +179,40 @@ EH_HD void eh_edrift(eh_p6 &p, double L) {
 /* ------------------------------------------------------------------------- */
 /* limit_amplitudes for one particle, src/limit_amplitudes.c:226-284 (openCode 0).
  * Returns true if the particle is lost and applies the loss assignments. */
+/* track_particles' loops (src/matrix.c:373-385 for order 2, the
+ * TURBO_FASTMATTRACK=3 branch; :438-445 for order 1): zero entries included, in
+ * the reference's summation order */
+EH_HD void eh_vmatrix_apply(eh_p6 &p, const EH_VMATRIX &M) {
+  const double ini[6] = {p.x, p.xp, p.y, p.yp, p.s, p.dp};
+  double fin[6];
+  if (M.order >= 2) {
+    for (int i = 5; i >= 0; i--) {
+      double sum = M.C[i];
+      for (int j = 5; j >= 0; j--) {
+        double sum1 = M.R[i][j];
+        const double *Tij = M.T[i] + j * (j + 1) / 2;
+        for (int k = j; k >= 0; k--)
+          sum1 += Tij[k] * ini[k];
+        sum += sum1 * ini[j];
+      }
+      fin[i] = sum;
+    }
+  } else {
+    for (int i = 5; i >= 0; i--) {
+      double sum = M.C[i];
+      for (int j = 5; j >= 0; j--)
+        sum += M.R[i][j] * ini[j];
+      fin[i] = sum;
+    }
+  }
+  p.x = fin[0];
+  p.xp = fin[1];
+  p.y = fin[2];
+  p.yp = fin[3];
+  p.s = fin[4];
+  p.dp = fin[5];
+}
+
 EH_HD bool eh_limit_one(eh_p6 &p, double xmax, double ymax, double z, double Po, int extrap) {
   int is_out = 0;
   if (xmax > 0 && eh_fabs(p.x) > xmax)

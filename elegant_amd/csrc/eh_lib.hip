@@ -897,6 +897,7 @@ struct eh_beam {
   long desc_cap = 0;
   eh_fterm *d_terms = nullptr; /* CSBEND field-term pool */
   long terms_cap = 0;
+  EH_VMATRIX *d_mat = nullptr; /* transport map of eh_track_particles */
   std::vector<double> cols; /* columns 6..9 of each uploaded row, by row0 */
   long pass = 0;
   int32_t event = 0;
@@ -957,6 +958,7 @@ extern "C" eh_beam *eh_beam_create(long n_capacity) {
   soa_carve(&b->buf[1], (char *)b->mem + per, n_capacity);
   soa_carve(&b->grave, (char *)b->mem + 2 * per, n_capacity);
   if (hipMalloc(&b->d_cnt, 8 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&b->d_mat, sizeof(EH_VMATRIX)) != hipSuccess ||
       hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
       hipEventCreate(&b->ev1) != hipSuccess) {
     fail(EH_ERR_HIP, "beam setup failed");
@@ -975,6 +977,8 @@ extern "C" void eh_beam_free(eh_beam *b) {
     (void)hipStreamSynchronize(b->stream);
   (void)hipFree(b->mem);
   (void)hipFree(b->d_cnt);
+  if (b->d_mat)
+    (void)hipFree(b->d_mat);
   if (b->d_desc)
     (void)hipFree(b->d_desc);
   if (b->d_terms)
@@ -1144,6 +1148,16 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_plain(eh_soa a, lon
     }
   }
   wave_add(&cnt[2], lost ? 1ull : 0ull);
+}
+
+/* track_particles with one transport map over the alive particles, in place */
+__global__ void __launch_bounds__(EH_BLOCK) k_matrix(eh_soa a, long n, const EH_VMATRIX *__restrict__ M) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  eh_p6 p = load_p(a, i);
+  eh_vmatrix_apply(p, *M);
+  store_p(a, i, p);
 }
 
 /* move survivors of a to the front of b and lost particles to the graveyard */
@@ -1660,6 +1674,29 @@ extern "C" long eh_drift_beam(eh_beam *b, long np, double length, long order) {
   return run_plain(b, &d, nullptr, 0, 0, 0, 0, 0, 0);
 }
 
+extern "C" long eh_track_particles(eh_beam *b, long n_part, const EH_VMATRIX *M) {
+  long r = check_np(b, n_part);
+  if (r < 0)
+    return r;
+  if (!M)
+    return fail(EH_ERR_ARG, "eh_track_particles: null matrix");
+  if (M->order < 1 || M->order > 2)
+    return fail(EH_ERR_UNSUPPORTED, "eh_track_particles: order %d not supported (1 and 2 are)", M->order);
+  long n = b->n_alive;
+  if (n == 0)
+    return 0;
+  HIPCHK(hipMemcpyAsync(b->d_mat, M, sizeof *M, hipMemcpyHostToDevice, b->stream));
+  r = timed_begin(b);
+  if (r < 0)
+    return r;
+  hipLaunchKernelGGL(k_matrix, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, b->buf[b->cur], n, b->d_mat);
+  HIPCHK(hipGetLastError());
+  r = timed_end(b);
+  if (r < 0)
+    return r;
+  return n;
+}
+
 extern "C" long eh_limit_amplitudes(eh_beam *b, double xmax, double ymax, long np, double z, double Po,
                                     long extrapolate_z, long openCode) {
   long r = check_np(b, np);
@@ -1936,6 +1973,21 @@ extern "C" long eh_beam_download(eh_beam *b, double **coord, long n_part) {
     }
   }
   return na;
+}
+
+extern "C" long eh_beam_row_order(eh_beam *b, int32_t *upload_row, long n_part) {
+  if (!b || !upload_row)
+    return fail(EH_ERR_ARG, "eh_beam_row_order: null argument");
+  long N = b->n_alive + b->n_lost;
+  if (n_part != N)
+    return fail(EH_ERR_ARG, "eh_beam_row_order: n_part=%ld but the beam holds %ld particles", n_part, N);
+  std::vector<int32_t> slot_at_row, alive_row0, idx;
+  long r = reference_order(b, slot_at_row, alive_row0, idx);
+  if (r < 0)
+    return r;
+  for (long row = 0; row < N; row++)
+    upload_row[row] = slot_at_row[row];
+  return b->n_alive;
 }
 
 /* ------------------------------------------------------------------------- */

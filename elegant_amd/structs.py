@@ -82,6 +82,26 @@ class EH_MULT(C.Structure):
                [(n, i32) for n in ("order", "synch_rad", "expandHamiltonian", "nSlices")]
 
 
+class EH_VMATRIX(C.Structure):
+    """first- or second-order transport map; T packed by rows, T[i][j*(j+1)/2 + k], k <= j"""
+    _fields_ = [("order", i32), ("pad_", i32), ("C", dbl * 6), ("R", (dbl * 6) * 6), ("T", (dbl * 21) * 6)]
+
+
+def vmatrix(C0, R, T=None):
+    """EH_VMATRIX from C (6), R (6, 6) and optionally T (6, 6, 6), of which the
+    entries T[i, j, k] with k <= j are used (elegant's storage)"""
+    m = EH_VMATRIX()
+    m.order = 2 if T is not None else 1
+    for i in range(6):
+        m.C[i] = float(C0[i])
+        for j in range(6):
+            m.R[i][j] = float(R[i][j])
+            if T is not None:
+                for k in range(j + 1):
+                    m.T[i][j * (j + 1) // 2 + k] = float(T[i][j][k])
+    return m
+
+
 class EH_ROTATE(C.Structure):
     _fields_ = [("tilt", dbl)] + [(n, i32) for n in ("on_pass", "exclude_floor", "exclude_optics")]
 

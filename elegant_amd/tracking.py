@@ -50,6 +50,17 @@ class Beam:
         _lib.check(lib().eh_beam_download_flat(self._h, out.ctypes.data, self.n))
         return out
 
+    def row_order(self):
+        """upload row of the particle a download puts in each row (how an
+        `accepted` array moves with the particles)"""
+        out = np.empty(self.n, dtype=np.int32)
+        _lib.check(lib().eh_beam_row_order(self._h, out.ctypes.data, self.n))
+        return out
+
+    def track_particles(self, vmatrix):
+        """apply a transport map (S.EH_VMATRIX) to the surviving particles"""
+        return _lib.check(lib().eh_track_particles(self._h, self.n_alive(), C.byref(vmatrix)))
+
     def free(self):
         if self._h:
             lib().eh_beam_free(self._h)

@@ -200,6 +200,19 @@ typedef struct {
   int32_t elliptical, exponent;
 } EH_MAXAMP;
 
+/* a first- or second-order transport map as track_particles applies it
+ * (VMATRIX, src/track.h:205-221): final_i = C[i] + sum_j (R[i][j] + sum_{k<=j}
+ * T[i][j][k] x_k) x_j, summed j = 5 down to 0 and k = j down to 0
+ * (src/matrix.c:373-385; order 1: C[i] + sum_j R[i][j] x_j, src/matrix.c:438-445).
+ * T is packed by rows: T[i][j*(j+1)/2 + k] holds elegant's T[i][j][k], k <= j. */
+typedef struct {
+  int32_t order; /* 1 or 2 */
+  int32_t pad_;
+  double C[6];
+  double R[6][6];
+  double T[6][21];
+} EH_VMATRIX;
+
 /* one beamline entry; mirrors ELEMENT_LIST's (name, type, p_elem) */
 typedef struct {
   int32_t type; /* eh_type */
@@ -240,6 +253,14 @@ long eh_beam_download(eh_beam *beam, double **coord, long n_part);
  * czarray_2d gives the reference's particle arrays, e.g. src/elegant.c:3183) */
 long eh_beam_upload_flat(eh_beam *beam, const double *flat, long n_part);
 long eh_beam_download_flat(eh_beam *beam, double *flat, long n_part);
+
+/* the row arrangement a download produces: upload_row[r] = the row (of the
+ * last upload) whose particle a download writes to row r.  The reference's
+ * element routines move the rows of a caller's `accepted` array in lockstep
+ * with the particle rows (swapParticles, src/multipole.c:1045-1046,
+ * src/limit_amplitudes.c:270-271); a caller permutes its copy with this.
+ * n_part = eh_beam_n_total. */
+long eh_beam_row_order(eh_beam *beam, int32_t *upload_row, long n_part);
 
 /* survivors, and all particles held (survivors + lost), i.e. the n_part of a download */
 long eh_beam_n_alive(const eh_beam *beam);
@@ -284,6 +305,12 @@ long eh_exactDrift(eh_beam *beam, long np, double length);
 /* replaces drift_beam / track_particles with drift_matrix, src/do_tracking.c:3754,
  * src/matrix.c:212, src/compute_matrices.c:391 */
 long eh_drift_beam(eh_beam *beam, long np, double length, long order);
+
+/* replaces track_particles, src/matrix.c:212-470, for first- and second-order
+ * maps (order 3 returns EH_ERR_UNSUPPORTED): every matrix-tracked element
+ * (src/do_tracking.c:1311), DRIF through drift_matrix, ROTATE through
+ * rotation_matrix (src/do_tracking.c:1836-1852).  No particle is lost. */
+long eh_track_particles(eh_beam *beam, long n_part, const EH_VMATRIX *M);
 
 /* replaces limit_amplitudes, src/limit_amplitudes.c:197-199 (openCode must be 0) */
 long eh_limit_amplitudes(eh_beam *beam, double xmax, double ymax, long np, double z, double Po, long extrapolate_z,

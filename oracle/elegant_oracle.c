@@ -204,6 +204,37 @@ static void or_track_particles2(double **part, const or_vmatrix2 *M, long n_part
   }
 }
 
+/* track_particles for a packed first- or second-order map (EH_VMATRIX), rows
+ * from the top down as the reference loops (src/matrix.c:334-381 order 2,
+ * TURBO_FASTMATTRACK=3; :424-460 order 1) */
+long or_track_particles_flat(double *flat, long n_part, const EH_VMATRIX *M) {
+  double temp[6], sum, sum1;
+  long i, j, k, i_part;
+  if (M->order < 1 || M->order > 2)
+    return or_fail("track_particles: only orders 1 and 2 are supported");
+  for (i_part = n_part - 1; i_part >= 0; i_part--) {
+    double *ini = flat + i_part * EH_NPROP;
+    for (i = 5; i >= 0; i--) {
+      sum = M->C[i];
+      if (M->order == 2) {
+        for (j = 5; j >= 0; j--) {
+          sum1 = M->R[i][j];
+          for (k = j; k >= 0; k--)
+            sum1 += M->T[i][j * (j + 1) / 2 + k] * ini[k];
+          sum += sum1 * ini[j];
+        }
+      } else {
+        for (j = 5; j >= 0; j--)
+          sum += M->R[i][j] * ini[j];
+      }
+      temp[i] = sum;
+    }
+    for (i = 5; i >= 0; i--)
+      ini[i] = temp[i];
+  }
+  return n_part;
+}
+
 /* drift_beam, src/do_tracking.c:3754-3767; also the DRIF element path
  * (matrix tracking, src/do_tracking.c:1239-1311) with order 0 -> default_order 2 */
 void or_drift_beam(double **part, long np, double length, long order) {

@@ -51,6 +51,8 @@ def load_path(path):
     lib.or_csbend_field_table.argtypes = [dp, dp, C.c_double, C.c_int, C.c_int, dp, dp]
     lib.or_csbend_field_table.restype = C.c_long
     lib.or_last_error.restype = C.c_char_p
+    lib.or_track_particles_flat.argtypes = [vp, C.c_long, vp]
+    lib.or_track_particles_flat.restype = C.c_long
     for f in ("or_sin", "or_cos", "or_asin", "or_log"):
         getattr(lib, f).argtypes = [C.c_double]
         getattr(lib, f).restype = C.c_double
@@ -100,3 +102,11 @@ def field_table(b, c, h, nonlinear=1, expansion_order=0):
     r = _check(lib, lib.or_csbend_field_table(b.ctypes.data_as(dp), c.ctypes.data_as(dp), h, nonlinear,
                                               expansion_order, Fx.ctypes.data_as(dp), Fy.ctypes.data_as(dp)))
     return Fx.reshape(11, 11), Fy.reshape(11, 11), r
+
+
+def track_particles(coords, vmatrix, libm=False):
+    """track_particles with an EH_VMATRIX (ctypes struct) on a copy of coords"""
+    lib = load(libm)
+    rows = np.array(coords, dtype=np.float64, order="C", copy=True)
+    _check(lib, lib.or_track_particles_flat(rows.ctypes.data, rows.shape[0], C.cast(C.pointer(vmatrix), C.c_void_p)))
+    return rows

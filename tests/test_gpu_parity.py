@@ -13,6 +13,7 @@ import pytest
 
 from elegant_amd import configs, structs as S, tracking
 from elegant_amd.lattice import Element, Lattice
+from oracle import oracle
 from tests.helpers import assert_rows_identical, bit_equal
 
 pytestmark = pytest.mark.gpu
@@ -392,3 +393,62 @@ def test_golden_fixtures(mode):
         got = beam.download()
         beam.free()
         assert_rows_identical(got, f["out"], n, int(f["n"]), "%s %s" % (name, mode))
+
+
+def test_track_particles_generic_maps():
+    """eh_track_particles (track_particles, src/matrix.c:212-470) equals the
+    checker bit for bit for random first- and second-order maps, special values
+    included, and a drift map through it equals eh_drift_beam"""
+    from elegant_amd import structs as S
+    _say("generic transport maps")
+    rng = np.random.default_rng(31)
+    x = rng.normal(0, [1e-3, 1e-4, 1e-3, 1e-4, 1e-3, 1e-3], (3000, 6))
+    x[0] = [0.0, -0.0, 0.0, -0.0, 0.0, -0.0]
+    x[1, 2] = np.inf
+    x[2, 5] = np.nan
+    rows = np.zeros((len(x), 10))
+    rows[:, :6] = x
+    rows[:, 6] = np.arange(1, len(x) + 1)
+    for order in (1, 2):
+        for trial in range(3):
+            C0 = rng.normal(0, 1e-3, 6)
+            R = rng.normal(0, 1, (6, 6))
+            T = rng.normal(0, 10, (6, 6, 6)) if order == 2 else None
+            if trial == 2:
+                R[rng.random((6, 6)) < 0.5] = 0.0
+            M = S.vmatrix(C0, R, T)
+            ref = oracle.track_particles(rows, M)
+            beam = tracking.Beam(rows)
+            assert beam.track_particles(M) == len(rows)
+            got = beam.download()
+            beam.free()
+            assert_rows_identical(got, ref, len(rows), len(rows), "order %d trial %d" % (order, trial))
+    L = 0.8
+    R = np.eye(6)
+    R[0, 1] = R[2, 3] = L
+    C0 = np.zeros(6)
+    C0[4] = L
+    T = np.zeros((6, 6, 6))
+    T[4, 1, 1] = T[4, 3, 3] = L / 2
+    b1, b2 = tracking.Beam(rows), tracking.Beam(rows)
+    b1.track_particles(S.vmatrix(C0, R, T))
+    _lib_check = tracking._lib.check
+    _lib_check(tracking.lib().eh_drift_beam(b2.handle, len(rows), L, 2))
+    assert_rows_identical(b1.download(), b2.download(), len(rows), len(rows), "drift map")
+
+
+def test_row_order_moves_like_accepted():
+    """eh_beam_row_order names, for every row of a download, the upload row of its
+    particle: the permutation the reference applies to `accepted` in lockstep"""
+    _say("row order for accepted")
+    Po = configs.p_central_from_mev(3000.0)
+    rows = configs.da_grid(30, 20, xmax=0.03, ymax=0.012)
+    beam = tracking.Beam(rows)
+    n, _, _ = tracking.track(beam, configs.mba_ring(Po), 2, Po, mode="fused")
+    got = beam.download()
+    order = beam.row_order()
+    beam.free()
+    assert 0 < n < len(rows)
+    assert np.array_equal(got[:, 6], rows[order, 6])
+    ref, nref, _, _ = oracle.do_tracking(rows, configs.mba_ring(Po), 2, Po)
+    assert np.array_equal(ref[:, 6], got[:, 6])
