@@ -16,6 +16,26 @@ extern "C" long bh_worklist(const int32_t *alive, int nvar, int32_t *wave_start,
   return total;
 }
 
+/* Planted mistake (CPU only): when set, bh_track leaves the dense field-block
+ * offsets of every pool descriptor where build_csbend put them for the
+ * element's own term list, i.e. pool_push shifts csb_fx0/csb_fy0 but not
+ * csb_dx0/csb_dy0.  That was the state of the batched tables while the dense
+ * CSBEND blocks were being introduced (their shift came with them later); the
+ * test shows what it does to a tracked batch. */
+static int g_plant_unshifted_dense = 0;
+extern "C" void bh_plant_unshifted_dense(int on) { g_plant_unshifted_dense = on; }
+
+static void plant_unshift(std::vector<eh_desc> &pool) {
+  if (!g_plant_unshifted_dense)
+    return;
+  for (eh_desc &d : pool)
+    if (d.csb_dense && d.csb_dx0 != 0) {
+      /* build_csbend lays the dense blocks out first: dx0 = 0, dy0 = eo1^2 */
+      d.csb_dy0 -= d.csb_dx0;
+      d.csb_dx0 = 0;
+    }
+}
+
 /* the lean map's answer for a descriptor of the given type */
 extern "C" long bh_lean_stage(int type) {
   eh_desc d;
@@ -218,6 +238,7 @@ extern "C" long bh_track(const double *flat, const long *counts, long nvar, EH_E
   long r = batch_tables(beamlines, nvar, ne, Pc, pool, terms, idx);
   if (r < 0)
     return r;
+  plant_unshift(pool);
   std::mt19937 rng(seed);
   int cur = 0, p = 0, e = 0;
   long n_grave = 0;
@@ -254,6 +275,7 @@ extern "C" long bh_track(const double *flat, const long *counts, long nvar, EH_E
       r = batch_rf_update(beamlines, nvar, ne, er, Pc, pool, terms, idx);
       if (r < 0)
         return r;
+      plant_unshift(pool);
     }
   }
   while (p < n_passes) {

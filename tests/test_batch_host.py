@@ -236,3 +236,37 @@ def test_emulated_batch_shared_and_zero_volt_cavities():
     r, out6, lpass, cnt = _host_track(beams, mk(), 4, Po, 5, 11)
     total = _compare_with_oracle(beams, mk, 4, Po, out6, lpass, cnt)
     assert r == total
+
+
+def test_unshifted_dense_blocks_lose_the_batch():
+    """The batched wrong result once seen on the GPU (counts 300, 0, 1, 129, 64,
+    three passes: all of variant 0's 300 particles reported lost) is what the
+    batched tables give when the dense field blocks of a de-duplicated CSBEND
+    keep the offsets of the bend's own term list instead of being shifted to
+    where its terms start in the shared list: every bend after the first reads
+    the first bend's coefficients.  Planted here on the CPU, the emulated batch
+    loses every particle of variant 0 (and of the other variants); without the
+    plant it matches the oracle (test_emulated_batch_matches_oracle)."""
+    Po = configs.p_central_from_mev(3000.0)
+    counts = [300, 0, 1, 129, 64]
+
+    def mk():
+        lats = configs.mba_variants(len(counts), seed=7)
+        for lat in lats:
+            for i, e in enumerate(lat.elements):
+                if e.type == S.T_RFCA:
+                    lat.set_param(i, "fiducial_mode", 1)
+        return lats
+
+    beams = _beams(counts)
+    h = harness()
+    h.bh_plant_unshifted_dense.argtypes = [C.c_int]
+    h.bh_plant_unshifted_dense(1)
+    try:
+        r, out6, lpass, cnt = _host_track(beams, mk(), 3, Po, None, 1)
+    finally:
+        h.bh_plant_unshifted_dense(0)
+    assert (lpass[:300] != -1).all(), "variant 0: %d of 300 survive the planted tables" % (lpass[:300] == -1).sum()
+    assert r == 0
+    ref, nref, _, _ = oracle.do_tracking(beams[0], mk()[0], 3, Po)
+    assert nref == 300
