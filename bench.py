@@ -66,46 +66,79 @@ def workload(name, grid):
     return lat, rows, Po, desc
 
 
-def _cpu_worker(args):
-    name, grid, idx, turns = args
-    from oracle import oracle
-    lat, rows, Po, _ = workload(name, grid)
-    sub = rows[idx]
-    t0 = time.perf_counter()
-    _, n, eff, _ = oracle.do_tracking(sub, lat, turns, Po, libm=True)
-    return eff, time.perf_counter() - t0
+_CPU = {}
 
 
-def cpu_baseline(name, grid, sample, turns, repeats=3):
-    """the checker with C-library transcendentals on a strided sample of the same
-    workload: one single-threaded worker per host core (particle shards, the
-    reference's MPI decomposition), median of `repeats` runs; the per-core rate
-    is reported beside it"""
+def _cpu_init(name, grid, barrier):
+    """worker set-up outside the timed window: the checker, the lattice and the
+    grid are built once per worker before the barrier"""
     from oracle import oracle
     oracle.load(libm=True)
+    lat, rows, Po, _ = workload(name, grid)
+    _CPU.update(lat=lat, rows=rows, Po=Po, barrier=barrier)
+
+
+def _cpu_worker(args):
+    idx, turns = args
+    from oracle import oracle
+    sub = _CPU["rows"][idx]
+    lat = _CPU["lat"].copy()
+    _CPU["barrier"].wait()
+    t0 = time.time()
+    _, n, eff, _ = oracle.do_tracking(sub, lat, turns, _CPU["Po"], libm=True)
+    return eff, t0, time.time()
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(name, grid, sample, turns, repeats=2, single_sample=None):
+    """the checker with C-library transcendentals on a strided sample of the same
+    workload: one single-threaded worker per host core (particle shards, the
+    reference's MPI decomposition), timed from a barrier that every worker
+    reaches after building its lattice and grid (pool start-up excluded) to the
+    last worker's end, median of `repeats` runs; and a genuine one-core run of
+    a 1/cores share of the sample.  The pool is sized to this process's CPU
+    affinity, at most 16 workers (the host-core share one GPU is given)."""
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    lat, rows, _, _ = workload(name, grid)
+    _, rows, _, _ = workload(name, grid)
     n_total = len(rows)
     idx = np.linspace(0, n_total - 1, sample).astype(np.int64)
     shards = np.array_split(idx, cores)
-    rates, per_core = [], []
+    rates = []
+    ctx = mp.get_context("fork")
     for _ in range(repeats):
-        t0 = time.perf_counter()
-        with mp.get_context("fork").Pool(cores) as pool:
-            res = pool.map(_cpu_worker, [(name, grid, s, turns) for s in shards])
-        wall = time.perf_counter() - t0
+        barrier = ctx.Barrier(cores)
+        with ctx.Pool(cores, initializer=_cpu_init, initargs=(name, grid, barrier)) as pool:
+            res = pool.map(_cpu_worker, [(s, turns) for s in shards], chunksize=1)
         units = sum(r[0] for r in res)
+        wall = max(r[2] for r in res) - min(r[1] for r in res)
         rates.append(units / wall)
-        per_core.append(units / sum(r[1] for r in res))
+    # one core: a single worker process on its own share of the sample
+    n1 = single_sample or max(1, sample // cores)
+    idx1 = np.linspace(0, n_total - 1, n1).astype(np.int64)
+    barrier = ctx.Barrier(1)
+    with ctx.Pool(1, initializer=_cpu_init, initargs=(name, grid, barrier)) as pool:
+        eff1, t0, t1 = pool.map(_cpu_worker, [(idx1, turns)])[0]
+    one = eff1 / (t1 - t0)
     return {"value": float(np.median(rates)), "unit": UNIT, "cores": cores, "kind": "port",
-            "sample": "%d of the %d particles (strided), %d turns, median of %d runs; oracle with C-library "
-                      "sin/cos/asin/log, gcc -O3 -m64, no FMA" % (sample, n_total, turns, repeats),
-            "per_core_value": float(np.median(per_core)),
-            "cpu": platform.processor() or platform.machine()}
+            "sample": "%d of the %d particles (strided), %d turns, %d cores, median of %d runs timed from a "
+                      "barrier after worker set-up; one-core run: %d particles, %d turns; oracle with "
+                      "C-library sin/cos/asin/log, gcc -O3 -m64, no FMA" % (sample, n_total, turns, cores,
+                                                                            repeats, n1, turns),
+            "one_core_value": float(one), "one_core_particles": n1,
+            "cpu": cpu_model()}
 
 
 COUNTERS = os.path.join(ROOT, "profiles", "k_fused_mba_1e6_counters.md")
@@ -337,8 +370,8 @@ def main():
     ap.add_argument("--variants", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=4)
     ap.add_argument("--plain-steps", type=int, default=4)
-    ap.add_argument("--cpu-sample", type=int, default=4096)
-    ap.add_argument("--cpu-turns", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=10000)
+    ap.add_argument("--cpu-turns", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -397,14 +430,14 @@ def main():
         hw = {"achieved": round(hw_t, 3), "frac": round(hw_t / FP64_PEAK_TFLOPS, 4),
               "source": "hardware FP64 ops per unit from profiles/k_fused_mba_1e6_counters.md (FMA = 2) x "
                         "units per launch / live launch time"}
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+    roofline = {"bound": "fp64_valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
                 "traffic_source": "FETCH_SIZE + WRITE_SIZE scaled by a copy kernel, "
                                   "profiles/k_fused_mba_1e6_counters.md" if traffic else None,
                 "algorithmic_traffic": 112 * n0,
                 "hw_fp64": hw,
-                "pipe": "FP64 VALU, no MFMA (compute-bound; the MI355X FP64 vector peak equals its FP64 "
-                        "matrix peak)",
+                "pipe": "FP64 VALU, no MFMA (compute-bound: the vector FP64 pipe; peak is MI355X's FP64 "
+                        "vector peak)",
                 "kernel": "k_fused" if args.mode == "fused" else "k_plain",
                 "algorithmic_flops_per_unit": round(alg_per_unit, 2),
                 "units_per_launch": round(st["main_units"] / launches),

@@ -16,7 +16,7 @@
  *    Manual for the Elementary Functions", 1980) with pi/2 split into four
  *    24-bit pieces plus a tail, so k*piece is exact for |k| < 2^29; the reduced
  *    argument |r| <= ~pi/4 is fed to the Taylor series of sin (to r^21) and cos
- *    (to r^20), evaluated by Horner's rule.
+ *    (to r^20), evaluated by Horner's rule with a fused multiply-add per step.
  *  - asin: Taylor series sum_n (2n)!/(4^n (n!)^2 (2n+1)) x^(2n+1) for |x| <= 1/2
  *    (10 terms for |x| <= 1/8, 40 terms above), and asin(x) = pi/2 - 2 asin(
  *    sqrt((1-|x|)/2)) for |x| > 1/2.
@@ -69,8 +69,11 @@ EH_FN double eh_nan(void) { return eh_from_bits(0x7ff8000000000000ULL); }
 EH_FN double eh_sqrt(double x) { return __builtin_sqrt(x); }
 
 /* fused multiply-add: exactly specified by IEEE-754 (one rounding), hence
- * identical on every conforming host and on the GPU; used only to obtain exact
- * rounding errors of products */
+ * identical on every conforming host and on the GPU.  Used for the exact
+ * rounding errors of products and for every Horner step (one instruction and
+ * one rounding per coefficient instead of two).  The host build of this header
+ * must compile it to the fused instruction (gcc/clang -mfma) or to the C
+ * library's fma, which is correctly rounded too. */
 EH_FN double eh_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 /* Taylor polynomial of sin(r) for |r| <= pi/4.  Below 2^-4 the series is cut
@@ -79,23 +82,23 @@ EH_FN double eh_sin_poly(double r) {
   double z = r * r;
   if (eh_fabs(r) < 0x1p-4) {
     double q = EH_SIN_5;
-    q = EH_SIN_4 + z * q;
-    q = EH_SIN_3 + z * q;
-    q = EH_SIN_2 + z * q;
-    q = EH_SIN_1 + z * q;
-    return r + (r * z) * q;
+    q = eh_fma(z, q, EH_SIN_4);
+    q = eh_fma(z, q, EH_SIN_3);
+    q = eh_fma(z, q, EH_SIN_2);
+    q = eh_fma(z, q, EH_SIN_1);
+    return eh_fma(r * z, q, r);
   }
   double p = EH_SIN_10;
-  p = EH_SIN_9 + z * p;
-  p = EH_SIN_8 + z * p;
-  p = EH_SIN_7 + z * p;
-  p = EH_SIN_6 + z * p;
-  p = EH_SIN_5 + z * p;
-  p = EH_SIN_4 + z * p;
-  p = EH_SIN_3 + z * p;
-  p = EH_SIN_2 + z * p;
-  p = EH_SIN_1 + z * p;
-  return r + (r * z) * p;
+  p = eh_fma(z, p, EH_SIN_9);
+  p = eh_fma(z, p, EH_SIN_8);
+  p = eh_fma(z, p, EH_SIN_7);
+  p = eh_fma(z, p, EH_SIN_6);
+  p = eh_fma(z, p, EH_SIN_5);
+  p = eh_fma(z, p, EH_SIN_4);
+  p = eh_fma(z, p, EH_SIN_3);
+  p = eh_fma(z, p, EH_SIN_2);
+  p = eh_fma(z, p, EH_SIN_1);
+  return eh_fma(r * z, p, r);
 }
 
 /* Taylor polynomial of cos(r) for |r| <= pi/4 (below 2^-4 cut after r^12; the
@@ -105,20 +108,20 @@ EH_FN double eh_cos_poly(double r) {
   double p;
   if (eh_fabs(r) < 0x1p-4) {
     p = EH_COS_6;
-    p = EH_COS_5 + z * p;
-    p = EH_COS_4 + z * p;
-    p = EH_COS_3 + z * p;
-    p = EH_COS_2 + z * p;
+    p = eh_fma(z, p, EH_COS_5);
+    p = eh_fma(z, p, EH_COS_4);
+    p = eh_fma(z, p, EH_COS_3);
+    p = eh_fma(z, p, EH_COS_2);
   } else {
     p = EH_COS_10;
-    p = EH_COS_9 + z * p;
-    p = EH_COS_8 + z * p;
-    p = EH_COS_7 + z * p;
-    p = EH_COS_6 + z * p;
-    p = EH_COS_5 + z * p;
-    p = EH_COS_4 + z * p;
-    p = EH_COS_3 + z * p;
-    p = EH_COS_2 + z * p;
+    p = eh_fma(z, p, EH_COS_9);
+    p = eh_fma(z, p, EH_COS_8);
+    p = eh_fma(z, p, EH_COS_7);
+    p = eh_fma(z, p, EH_COS_6);
+    p = eh_fma(z, p, EH_COS_5);
+    p = eh_fma(z, p, EH_COS_4);
+    p = eh_fma(z, p, EH_COS_3);
+    p = eh_fma(z, p, EH_COS_2);
   }
   /* 1 - z/2 + z^2 p.  1 - z/2 is formed with its exact rounding error (Fast2Sum,
    * valid since z/2 <= 1) and the exact error of z = r*r, so that the final
@@ -128,7 +131,7 @@ EH_FN double eh_cos_poly(double r) {
   double hz = 0.5 * z;
   double w = 1.0 - hz;
   double t = (1.0 - w) - hz;
-  return w + ((t - 0.5 * zl) + (z * z) * p);
+  return w + eh_fma(z * z, p, t - 0.5 * zl);
 }
 
 /* Reduce x to r = x - k*pi/2; returns the quadrant k mod 4, or -1 when x is
@@ -143,12 +146,12 @@ EH_FN int eh_reduce_pio2(double x, double *r) {
     *r = eh_nan();
     return -1;
   }
-  double kf = (x * EH_TWO_OVER_PI + 0x1.8p52) - 0x1.8p52;
-  double t = x - kf * EH_PIO2_1;
-  t = t - kf * EH_PIO2_2;
-  t = t - kf * EH_PIO2_3;
-  t = t - kf * EH_PIO2_4;
-  t = t - kf * EH_PIO2_5;
+  double kf = eh_fma(x, EH_TWO_OVER_PI, 0x1.8p52) - 0x1.8p52;
+  double t = eh_fma(-kf, EH_PIO2_1, x);
+  t = eh_fma(-kf, EH_PIO2_2, t);
+  t = eh_fma(-kf, EH_PIO2_3, t);
+  t = eh_fma(-kf, EH_PIO2_4, t);
+  t = eh_fma(-kf, EH_PIO2_5, t);
   *r = t;
   int64_t k = (int64_t)kf;
   return (int)(k & 3);
@@ -230,32 +233,32 @@ EH_FN double eh_asin_small(double x) {
   double ax = eh_fabs(x);
   if (ax < 0x1p-8) {
     double q = EH_ASIN_4;
-    q = EH_ASIN_3 + z * q;
-    q = EH_ASIN_2 + z * q;
-    q = EH_ASIN_1 + z * q;
-    return x + (x * z) * q;
+    q = eh_fma(z, q, EH_ASIN_3);
+    q = eh_fma(z, q, EH_ASIN_2);
+    q = eh_fma(z, q, EH_ASIN_1);
+    return eh_fma(x * z, q, x);
   }
   if (ax < 0x1p-4) {
     double q = EH_ASIN_7;
-    q = EH_ASIN_6 + z * q;
-    q = EH_ASIN_5 + z * q;
-    q = EH_ASIN_4 + z * q;
-    q = EH_ASIN_3 + z * q;
-    q = EH_ASIN_2 + z * q;
-    q = EH_ASIN_1 + z * q;
-    return x + (x * z) * q;
+    q = eh_fma(z, q, EH_ASIN_6);
+    q = eh_fma(z, q, EH_ASIN_5);
+    q = eh_fma(z, q, EH_ASIN_4);
+    q = eh_fma(z, q, EH_ASIN_3);
+    q = eh_fma(z, q, EH_ASIN_2);
+    q = eh_fma(z, q, EH_ASIN_1);
+    return eh_fma(x * z, q, x);
   }
   double p = EH_ASIN_10;
-  p = EH_ASIN_9 + z * p;
-  p = EH_ASIN_8 + z * p;
-  p = EH_ASIN_7 + z * p;
-  p = EH_ASIN_6 + z * p;
-  p = EH_ASIN_5 + z * p;
-  p = EH_ASIN_4 + z * p;
-  p = EH_ASIN_3 + z * p;
-  p = EH_ASIN_2 + z * p;
-  p = EH_ASIN_1 + z * p;
-  return x + (x * z) * p;
+  p = eh_fma(z, p, EH_ASIN_9);
+  p = eh_fma(z, p, EH_ASIN_8);
+  p = eh_fma(z, p, EH_ASIN_7);
+  p = eh_fma(z, p, EH_ASIN_6);
+  p = eh_fma(z, p, EH_ASIN_5);
+  p = eh_fma(z, p, EH_ASIN_4);
+  p = eh_fma(z, p, EH_ASIN_3);
+  p = eh_fma(z, p, EH_ASIN_2);
+  p = eh_fma(z, p, EH_ASIN_1);
+  return eh_fma(x * z, p, x);
 }
 
 /* asin Taylor series valid for |x| <= 1/2 (40 terms) */
@@ -270,8 +273,8 @@ EH_FN double eh_asin_mid(double x) {
   double p = a[39];
 #pragma unroll
   for (int n = 38; n >= 0; n--)
-    p = a[n] + z * p;
-  return x + (x * z) * p;
+    p = eh_fma(z, p, a[n]);
+  return eh_fma(x * z, p, x);
 }
 
 EH_FN double eh_asin_half(double x) {
@@ -320,36 +323,36 @@ EH_FN double eh_log(double x) {
     double w = f - hh;
     double t = (f - w) - hh;
     double q = EH_LOG1P_16;
-    q = EH_LOG1P_15 + f * q;
-    q = EH_LOG1P_14 + f * q;
-    q = EH_LOG1P_13 + f * q;
-    q = EH_LOG1P_12 + f * q;
-    q = EH_LOG1P_11 + f * q;
-    q = EH_LOG1P_10 + f * q;
-    q = EH_LOG1P_9 + f * q;
-    q = EH_LOG1P_8 + f * q;
-    q = EH_LOG1P_7 + f * q;
-    q = EH_LOG1P_6 + f * q;
-    q = EH_LOG1P_5 + f * q;
-    q = EH_LOG1P_4 + f * q;
-    q = EH_LOG1P_3 + f * q;
-    return w + ((t - 0.5 * hl) + (h * f) * q);
+    q = eh_fma(f, q, EH_LOG1P_15);
+    q = eh_fma(f, q, EH_LOG1P_14);
+    q = eh_fma(f, q, EH_LOG1P_13);
+    q = eh_fma(f, q, EH_LOG1P_12);
+    q = eh_fma(f, q, EH_LOG1P_11);
+    q = eh_fma(f, q, EH_LOG1P_10);
+    q = eh_fma(f, q, EH_LOG1P_9);
+    q = eh_fma(f, q, EH_LOG1P_8);
+    q = eh_fma(f, q, EH_LOG1P_7);
+    q = eh_fma(f, q, EH_LOG1P_6);
+    q = eh_fma(f, q, EH_LOG1P_5);
+    q = eh_fma(f, q, EH_LOG1P_4);
+    q = eh_fma(f, q, EH_LOG1P_3);
+    return w + eh_fma(h * f, q, t - 0.5 * hl);
   }
   double s = f / (2.0 + f);
   double z = s * s;
   double p = EH_LOG_12;
-  p = EH_LOG_11 + z * p;
-  p = EH_LOG_10 + z * p;
-  p = EH_LOG_9 + z * p;
-  p = EH_LOG_8 + z * p;
-  p = EH_LOG_7 + z * p;
-  p = EH_LOG_6 + z * p;
-  p = EH_LOG_5 + z * p;
-  p = EH_LOG_4 + z * p;
-  p = EH_LOG_3 + z * p;
-  p = EH_LOG_2 + z * p;
-  p = EH_LOG_1 + z * p;
-  double lm = 2.0 * s + s * (z * p);
+  p = eh_fma(z, p, EH_LOG_11);
+  p = eh_fma(z, p, EH_LOG_10);
+  p = eh_fma(z, p, EH_LOG_9);
+  p = eh_fma(z, p, EH_LOG_8);
+  p = eh_fma(z, p, EH_LOG_7);
+  p = eh_fma(z, p, EH_LOG_6);
+  p = eh_fma(z, p, EH_LOG_5);
+  p = eh_fma(z, p, EH_LOG_4);
+  p = eh_fma(z, p, EH_LOG_3);
+  p = eh_fma(z, p, EH_LOG_2);
+  p = eh_fma(z, p, EH_LOG_1);
+  double lm = eh_fma(s * z, p, 2.0 * s);
   double de = (double)e;
   return de * EH_LN2_HI + (lm + de * EH_LN2_LO);
 }

@@ -48,6 +48,13 @@
 
 #include "eh_csbend_terms.h"
 
+/* the counting build (or_count.h) tallies the generic transport-matrix loops
+ * by their non-trivial terms: these mark those loops, and do nothing here */
+#ifndef OR_COUNT_MATRIX_BEGIN
+#define OR_COUNT_MATRIX_BEGIN()
+#define OR_COUNT_MATRIX_END()
+#endif
+
 #define PI 3.14159265358979323846
 #define PIx2 (2 * PI)
 #define PIo2 (PI / 2)
@@ -187,6 +194,7 @@ static void or_drift_matrix(or_vmatrix2 *M, double length, long order) {
 static void or_track_particles2(double **part, const or_vmatrix2 *M, long n_part) {
   double temp[6], sum, sum1;
   long i, j, k, i_part;
+  OR_COUNT_MATRIX_BEGIN();
   for (i_part = 0; i_part < n_part; i_part++) {
     double *fin = part[i_part], *ini = part[i_part];
     for (i = 5; i >= 0; i--) {
@@ -202,6 +210,7 @@ static void or_track_particles2(double **part, const or_vmatrix2 *M, long n_part
     for (i = 5; i >= 0; i--)
       fin[i] = temp[i];
   }
+  OR_COUNT_MATRIX_END();
 }
 
 /* track_particles for a packed first- or second-order map (EH_VMATRIX), rows
@@ -212,6 +221,7 @@ long or_track_particles_flat(double *flat, long n_part, const EH_VMATRIX *M) {
   long i, j, k, i_part;
   if (M->order < 1 || M->order > 2)
     return or_fail("track_particles: only orders 1 and 2 are supported");
+  OR_COUNT_MATRIX_BEGIN();
   for (i_part = n_part - 1; i_part >= 0; i_part--) {
     double *ini = flat + i_part * EH_NPROP;
     for (i = 5; i >= 0; i--) {
@@ -232,6 +242,7 @@ long or_track_particles_flat(double *flat, long n_part, const EH_VMATRIX *M) {
     for (i = 5; i >= 0; i--)
       ini[i] = temp[i];
   }
+  OR_COUNT_MATRIX_END();
   return n_part;
 }
 
@@ -879,6 +890,7 @@ void or_rotate(double **coord, long np, const EH_ROTATE *rot) {
   R[3][1] = -sin_tilt;
   R[3][3] = cos_tilt;
   R[4][4] = R[5][5] = 1;
+  OR_COUNT_MATRIX_BEGIN();
   for (ip = np - 1; ip >= 0; ip--) {
     double *ini = coord[ip];
     for (i = 5; i >= 0; i--) {
@@ -890,6 +902,7 @@ void or_rotate(double **coord, long np, const EH_ROTATE *rot) {
     for (i = 5; i >= 0; i--)
       ini[i] = temp[i];
   }
+  OR_COUNT_MATRIX_END();
 }
 
 /* set_central_momentum, src/do_tracking.c:3563-3596 (no MPI) */

@@ -17,8 +17,8 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def load(libm=False):
-    name = "liboracle_libm.so" if libm else "liboracle.so"
+def load(libm=False, count=False):
+    name = "liboracle_count.so" if count else ("liboracle_libm.so" if libm else "liboracle.so")
     if name in _libs:
         return _libs[name]
     path = os.path.join(HERE, name)
@@ -31,7 +31,7 @@ def load(libm=False):
 def build_variant(out_path, defines):
     """compile the checker with extra -D definitions (e.g. a planted mistake,
     OR_PLANT=n) into out_path; returns the loaded library"""
-    cmd = ["gcc", "-m64", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I" + os.path.join(HERE, "..", "include"),
+    cmd = ["gcc", "-m64", "-O3", "-mfma", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I" + os.path.join(HERE, "..", "include"),
            "-I" + os.path.join(HERE, "..", "elegant_amd", "csrc")] + ["-D" + d for d in defines] + \
         ["-shared", "-o", out_path, os.path.join(HERE, "elegant_oracle.c"), "-lm"]
     subprocess.run(cmd, check=True)
@@ -53,10 +53,38 @@ def load_path(path):
     lib.or_last_error.restype = C.c_char_p
     lib.or_track_particles_flat.argtypes = [vp, C.c_long, vp]
     lib.or_track_particles_flat.restype = C.c_long
+    if hasattr(lib, "or_count_read"):
+        lib.or_count_read.argtypes = [vp]
+        lib.or_count_reset.argtypes = []
     for f in ("or_sin", "or_cos", "or_asin", "or_log"):
         getattr(lib, f).argtypes = [C.c_double]
         getattr(lib, f).restype = C.c_double
     return lib
+
+
+OP_NAMES = ("add", "sub", "mul", "div", "sqrt", "T")
+
+
+def count_ops(coords, lattice, n_passes, p_central):
+    """Run do_tracking in the counting build; returns (counts, rows, n_alive,
+    effort).  counts[op] for op in OP_NAMES is the census tally of the + - * /
+    and sqrt on doubles and the sin, cos, tan, asin and log calls (every
+    operation executed, the generic matrix loops counted by their non-trivial
+    terms, see or_count.h); counts["all_" + op] every operation executed."""
+    lib = load(count=True)
+    lib.or_count_reset()
+    rows = np.array(coords, dtype=np.float64, order="C", copy=True)
+    arr, keep = lattice.c_beamline()
+    pc = C.c_double(p_central)
+    eff = C.c_double(0)
+    n = _check(lib, lib.or_do_tracking_flat(rows.ctypes.data, rows.shape[0], C.cast(arr, C.c_void_p), len(lattice),
+                                            n_passes, C.byref(pc), C.byref(eff)))
+    out = np.zeros(12, dtype=np.uint64)
+    lib.or_count_read(out.ctypes.data)
+    del keep
+    counts = dict(zip(OP_NAMES, (int(v) for v in out[:6])))
+    counts.update(("all_" + k, int(v)) for k, v in zip(OP_NAMES, out[6:]))
+    return counts, rows, n, eff.value
 
 
 def _check(lib, r):
