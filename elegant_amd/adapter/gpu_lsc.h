@@ -1,0 +1,3 @@
+/* gpu_lsc.h -- included by elegant under HAVE_GPU; every declaration of the
+ * adapter is in gpu_base.h */
+#include "gpu_base.h"

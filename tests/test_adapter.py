@@ -1,0 +1,199 @@
+"""The elegant adapter (elegant_amd/adapter): elegant's HAVE_GPU interface on
+the library, checked against elegant's own struct definitions.
+
+* every field of every element struct the adapter reads (src/track.h) is
+  either copied into the library's struct, refused when it has a value the
+  library does not carry, or listed as not changing tracked coordinates;
+* the adapter compiles against elegant's struct definitions and routine
+  prototypes, extracted from src/track.h into a temporary header at test time
+  (types the adapter only passes through by pointer are declared opaque);
+* its residency protocol, host fallbacks, fiducial write-back, parameter
+  refusals and matrix packing behave as intended, with elegant's routines and
+  the library's entry points replaced by recording stand-ins
+  (tests/adapter_host/adapter_host.c).
+
+These tests read the reference source and are skipped where it is absent."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADAPTER = os.path.join(ROOT, "elegant_amd", "adapter")
+TRACK_H = "/root/reference/src/track.h"
+
+pytestmark = pytest.mark.skipif(not os.path.exists(TRACK_H), reason="reference source not present")
+
+# element struct -> the adapter function that reads it and its name stem
+FILLED = {"KQUAD": "kquad", "KSEXT": "ksext", "KOCT": "koct", "KQUSE": "kquse", "CSBEND": "csbend",
+          "RFCA": "rfca", "MALIGN": "malign", "MAXAMP": "maxamp"}
+HOST_READ = {"DRIFT": "drift", "EDRIFT": "edrift", "ROTATE": "rotate", "ENERGY": "energy"}
+
+EXTRACT_TYPES = ["VMATRIX", "MULTIPOLE_DATA", "APERTURE_DATA", "ELEMENT_LIST", "ELEMENT_DESCRIPTION", "APCONTOUR",
+                 "DRIFT", "EDRIFT", "KQUAD", "KSEXT", "KOCT", "KQUSE", "CSBEND", "RFCA", "MALIGN", "MAXAMP",
+                 "ROTATE", "ENERGY"]
+OPAQUE = ["RUN", "CHARGE", "WAKE", "TRWAKE", "LSCKICK", "RFCW", "MATTER", "LSCDRIFT", "CSRDRIFT", "CSRCSBEND",
+          "RCOL", "ECOL", "SCRAPER", "MATR", "EMATRIX", "CENTER", "BEAM_SUMS", "TWISS", "SIGMA_MATRIX",
+          "SDDS_DATASET", "PARAMETER"]
+PROTOS = ["bombElegant", "multipole_tracking2", "track_through_csbend", "exactDrift", "track_particles",
+          "limit_amplitudes", "elimit_amplitudes", "removeInvalidParticles", "offset_beam", "set_central_momentum",
+          "findFiducialTime", "do_match_energy", "compute_centroids", "center_beam", "accumulate_beam_sums",
+          "imposeApertureData", "parseFiducialMode"]
+
+
+def _strip_comments(text):
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    return re.sub(r"//[^\n]*", " ", text)
+
+
+def _track_text():
+    return open(TRACK_H).read()
+
+
+def struct_block(text, name):
+    """the text `typedef struct [tag] { ... } NAME;` of track.h"""
+    m = re.search(r"\}\s*%s\s*;" % name, text)
+    assert m, name
+    depth, i = 0, m.start()
+    while True:
+        if text[i] == "}":
+            depth += 1
+        elif text[i] == "{":
+            depth -= 1
+            if depth == 0:
+                break
+        i -= 1
+    start = text.rfind("typedef", 0, i)
+    return text[start:m.end()]
+
+
+def struct_fields(text, name):
+    block = _strip_comments(struct_block(text, name))
+    body = block[block.index("{") + 1:block.rindex("}")]
+    body = re.sub(r"#\s*if\b.*?#\s*endif", " ", body, flags=re.S)  # conditional members
+    body = re.sub(r"^\s*#.*$", " ", body, flags=re.M)
+    fields = []
+    for decl in body.split(";"):
+        decl = " ".join(decl.split())
+        if not decl:
+            continue
+        m = re.match(r"^((?:unsigned |signed |const |struct )*\w+)\s+(.*)$", decl)
+        assert m, decl
+        for d in m.group(2).split(","):
+            d = re.sub(r"\[[^\]]*\]", "", d).replace("*", "").strip()
+            if d:
+                fields.append(d)
+    return fields
+
+
+def _adapter_text():
+    return open(os.path.join(ADAPTER, "gpu_adapter.c")).read()
+
+
+def _function_body(text, name):
+    m = re.search(r"\n(?:static )?\w[\w\s\*]*\b%s\s*\([^)]*\)\s*\{" % name, text)
+    assert m, name
+    i, depth = m.end() - 1, 0
+    while True:
+        if text[i] == "{":
+            depth += 1
+        elif text[i] == "}":
+            depth -= 1
+            if depth == 0:
+                return text[m.start():i + 1]
+        i += 1
+
+
+def _string_list(text, name):
+    m = re.search(r"%s\[\][^=]*=\s*\{(.*?)\};" % name, text, flags=re.S)
+    return re.findall(r'"(\w+)"', m.group(1)) if m else []
+
+
+@pytest.mark.parametrize("struct", sorted(FILLED) + sorted(HOST_READ))
+def test_every_field_copied_refused_or_listed(struct):
+    """every field of the element struct is read by the adapter's fill
+    function (copied or refused), or named as not changing tracked coordinates,
+    or read by do_tracking itself before it calls a gpu_ routine"""
+    text = _track_text()
+    fields = struct_fields(text, struct)
+    assert fields
+    src = _adapter_text()
+    if struct in FILLED:
+        stem = FILLED[struct]
+        body = _strip_comments(_function_body(src, "fill_" + stem))
+        param = re.search(r"const %s \*(\w+)" % struct, body).group(1)
+        read = set(re.findall(r"\b%s->(\w+)" % param, body))
+        listed = set(_string_list(src, stem + "_untracked")) | set(_string_list(src, stem + "_host_read"))
+    else:
+        stem = HOST_READ[struct]
+        read = set()
+        listed = set(_string_list(src, stem + "_host_read"))
+    missing = [f for f in fields if f not in read and f not in listed]
+    assert not missing, "%s fields neither copied, refused nor listed: %s" % (struct, missing)
+    unknown = sorted((read | listed) - set(fields))
+    assert not unknown, "%s: the adapter names fields track.h does not have: %s" % (struct, unknown)
+
+
+def _defines(text):
+    out = []
+    for pat in (r"^#define T_\w+\s+\d+\s*$", r"^#define N_TYPES\s+\d+\s*$", r"^#define MATRIX_TRACKING\s.*$",
+                r"^#define FID_MODE_\w+\s.*$", r"^#define IS_MONITOR\(type\).*$"):
+        out += re.findall(pat, text, flags=re.M)
+    return out
+
+
+def _prototype(text, name):
+    m = re.search(r"^[ \t]*(?:extern\s+)?(?:unsigned\s+)?(?:long|void|double|int)\s*\**\s*%s\s*\([^;{]*\)\s*;" % name,
+                  text, flags=re.M)
+    assert m, name
+    return m.group(0).strip()
+
+
+def write_track_subset(dirpath):
+    """a header named track.h with elegant's definitions the adapter uses"""
+    text = _track_text()
+    lines = ["/* extracted at test time from the reference's src/track.h (not committed) */",
+             "#include <stdint.h>", "#include <stddef.h>", "#include <stdio.h>"]
+    lines += ["typedef struct eh_opaque_%s %s;" % (t, t) for t in OPAQUE]
+    lines += _defines(text)
+    for t in EXTRACT_TYPES:
+        lines.append(struct_block(text, t))
+    lines.append("extern ELEMENT_DESCRIPTION entity_description[N_TYPES];")
+    lines.append("extern int totalPropertiesPerParticle, globalLossCoordOffset;")
+    lines += [_prototype(text, p) for p in PROTOS]
+    path = os.path.join(dirpath, "track.h")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
+def test_adapter_compiles_against_track_h(tmp_path):
+    write_track_subset(str(tmp_path))
+    cmd = ["gcc", "-std=gnu99", "-fsyntax-only", "-DHAVE_GPU", "-Wall", "-Wno-unused-parameter",
+           "-Werror=implicit-function-declaration", "-Werror=incompatible-pointer-types", "-Werror=int-conversion",
+           "-Werror=return-type", "-I" + str(tmp_path), "-I" + ADAPTER, "-I" + os.path.join(ROOT, "include"),
+           os.path.join(ADAPTER, "gpu_adapter.c")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert "warning" not in r.stderr, r.stderr
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    d = tmp_path_factory.mktemp("adapter")
+    write_track_subset(str(d))
+    so = os.path.join(str(d), "libadapter_host.so")
+    cmd = ["gcc", "-std=gnu99", "-O1", "-fPIC", "-shared", "-DHAVE_GPU", "-w", "-I" + str(d), "-I" + ADAPTER,
+           "-I" + os.path.join(ROOT, "include"), "-o", so, os.path.join(ROOT, "tests", "adapter_host", "adapter_host.c")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return C.CDLL(so)
+
+
+@pytest.mark.parametrize("scenario", ["residency", "host_fallback", "rfca", "fill", "vmatrix"])
+def test_adapter_scenarios(harness, scenario):
+    fn = getattr(harness, "ah_scenario_" + scenario)
+    fn.restype = C.c_int
+    assert fn() == 0, "scenario %s failed at check %d" % (scenario, fn())
