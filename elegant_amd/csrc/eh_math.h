@@ -15,8 +15,10 @@
  *  - sin/cos: Cody & Waite argument reduction (W. J. Cody, W. Waite, "Software
  *    Manual for the Elementary Functions", 1980) with pi/2 split into four
  *    24-bit pieces plus a tail, so k*piece is exact for |k| < 2^29; the reduced
- *    argument |r| <= ~pi/4 is fed to the Taylor series of sin (to r^21) and cos
- *    (to r^20), evaluated by Horner's rule with a fused multiply-add per step.
+ *    argument |r| <= ~pi/4 is kept as r + rl (the rounding error of r in rl)
+ *    and fed to the Taylor series of sin (to r^21) and cos (to r^20), evaluated
+ *    by Horner's rule with a fused multiply-add per step, plus the first-order
+ *    terms in rl.
  *  - asin: Taylor series sum_n (2n)!/(4^n (n!)^2 (2n+1)) x^(2n+1) for |x| <= 1/2
  *    (10 terms for |x| <= 1/8, 40 terms above), and asin(x) = pi/2 - 2 asin(
  *    sqrt((1-|x|)/2)) for |x| > 1/2.
@@ -76,53 +78,58 @@ EH_FN double eh_sqrt(double x) { return __builtin_sqrt(x); }
  * library's fma, which is correctly rounded too. */
 EH_FN double eh_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-/* Taylor polynomial of sin(r) for |r| <= pi/4.  Below 2^-4 the series is cut
- * after the r^11 term: the next one is below 2^-80 of the result. */
+/* Taylor polynomial of sin(r) for |r| < 2^-4, cut after the r^11 term (the
+ * next one is below 2^-80 of the result) */
 EH_FN double eh_sin_poly(double r) {
   double z = r * r;
-  if (eh_fabs(r) < 0x1p-4) {
-    double q = EH_SIN_5;
-    q = eh_fma(z, q, EH_SIN_4);
-    q = eh_fma(z, q, EH_SIN_3);
-    q = eh_fma(z, q, EH_SIN_2);
-    q = eh_fma(z, q, EH_SIN_1);
-    return eh_fma(r * z, q, r);
-  }
-  double p = EH_SIN_10;
-  p = eh_fma(z, p, EH_SIN_9);
-  p = eh_fma(z, p, EH_SIN_8);
-  p = eh_fma(z, p, EH_SIN_7);
-  p = eh_fma(z, p, EH_SIN_6);
-  p = eh_fma(z, p, EH_SIN_5);
-  p = eh_fma(z, p, EH_SIN_4);
-  p = eh_fma(z, p, EH_SIN_3);
-  p = eh_fma(z, p, EH_SIN_2);
-  p = eh_fma(z, p, EH_SIN_1);
-  return eh_fma(r * z, p, r);
+  double q = EH_SIN_5;
+  q = eh_fma(z, q, EH_SIN_4);
+  q = eh_fma(z, q, EH_SIN_3);
+  q = eh_fma(z, q, EH_SIN_2);
+  q = eh_fma(z, q, EH_SIN_1);
+  return eh_fma(r * z, q, r);
 }
 
-/* Taylor polynomial of cos(r) for |r| <= pi/4 (below 2^-4 cut after r^12; the
- * next term is below 2^-76 of the result) */
+/* sin(r + rl) for a reduced argument carried as r + rl (|r| <= ~pi/4,
+ * |rl| <= ulp(r)/2), for the arguments that needed a reduction.  The series is
+ * r + r^3 (S1 + S2 z + z^2 Q(z)) with z = r^2: r^2, r^3, S1, S2 and the products
+ * that carry the leading terms are kept with their exact rounding errors
+ * (double-double by fma), Q in plain doubles, so that only the final addition
+ * rounds at the level of the result; rl enters as rl cos(r) = rl (1 - z/2). */
+EH_FN double eh_sin_poly_l(double r, double rl) {
+  const double z = r * r, zl = eh_fma(r, r, -z);
+  double q = EH_SIN_10;
+  q = eh_fma(z, q, EH_SIN_9);
+  q = eh_fma(z, q, EH_SIN_8);
+  q = eh_fma(z, q, EH_SIN_7);
+  q = eh_fma(z, q, EH_SIN_6);
+  q = eh_fma(z, q, EH_SIN_5);
+  q = eh_fma(z, q, EH_SIN_4);
+  q = eh_fma(z, q, EH_SIN_3);
+  /* S = S1 + z S2 + z^2 q as sh + sl */
+  const double t2 = z * EH_SIN_2;
+  const double t2e = eh_fma(z, EH_SIN_2, -t2) + (zl * EH_SIN_2 + z * EH_SIN_2_LO);
+  const double sh = EH_SIN_1 + t2;
+  const double sl = ((EH_SIN_1 - sh) + t2) + (EH_SIN_1_LO + t2e + (z * z) * q);
+  /* r^3 = c3 + c3l */
+  const double c3 = r * z, c3l = eh_fma(r, z, -c3) + r * zl;
+  /* r^3 S = T + te */
+  const double T = c3 * sh;
+  const double te = eh_fma(c3, sh, -T) + (c3 * sl + c3l * sh);
+  const double h = r + T;
+  const double hl = (r - h) + T;
+  return h + (hl + (te + rl * (1.0 - 0.5 * z)));
+}
+
+/* Taylor polynomial of cos(r) for |r| < 2^-4, cut after the r^12 term (the
+ * next one is below 2^-76 of the result) */
 EH_FN double eh_cos_poly(double r) {
   double z = r * r;
-  double p;
-  if (eh_fabs(r) < 0x1p-4) {
-    p = EH_COS_6;
-    p = eh_fma(z, p, EH_COS_5);
-    p = eh_fma(z, p, EH_COS_4);
-    p = eh_fma(z, p, EH_COS_3);
-    p = eh_fma(z, p, EH_COS_2);
-  } else {
-    p = EH_COS_10;
-    p = eh_fma(z, p, EH_COS_9);
-    p = eh_fma(z, p, EH_COS_8);
-    p = eh_fma(z, p, EH_COS_7);
-    p = eh_fma(z, p, EH_COS_6);
-    p = eh_fma(z, p, EH_COS_5);
-    p = eh_fma(z, p, EH_COS_4);
-    p = eh_fma(z, p, EH_COS_3);
-    p = eh_fma(z, p, EH_COS_2);
-  }
+  double p = EH_COS_6;
+  p = eh_fma(z, p, EH_COS_5);
+  p = eh_fma(z, p, EH_COS_4);
+  p = eh_fma(z, p, EH_COS_3);
+  p = eh_fma(z, p, EH_COS_2);
   /* 1 - z/2 + z^2 p.  1 - z/2 is formed with its exact rounding error (Fast2Sum,
    * valid since z/2 <= 1) and the exact error of z = r*r, so that the final
    * addition is the only rounding of the leading terms: the result is then
@@ -134,37 +141,78 @@ EH_FN double eh_cos_poly(double r) {
   return w + eh_fma(z * z, p, t - 0.5 * zl);
 }
 
-/* Reduce x to r = x - k*pi/2; returns the quadrant k mod 4, or -1 when x is
- * outside the supported domain (non-finite or |x| >= 2^50). */
-EH_FN int eh_reduce_pio2(double x, double *r) {
-  if (eh_fabs(x) < 0x1.921fb54442d18p-1) {
-    /* |x| < pi/4: k = 0 and x - 0*piece is x exactly */
-    *r = x;
-    return 0;
-  }
+/* cos(r + rl) for a reduced argument carried as r + rl: 1 - z/2 + z^2 C(z),
+ * C = C2 + C3 z + z^2 Q(z), with 1 - z/2 formed with its exact error (as in
+ * eh_cos_poly), z^2 and the leading terms of C as double-double, the sum of
+ * the two leading parts formed exactly, so that only the final addition rounds
+ * at the level of the result; rl enters as -rl sin(r). */
+EH_FN double eh_cos_poly_l(double r, double rl) {
+  const double z = r * r, zl = eh_fma(r, r, -z);
+  double q = EH_COS_10;
+  q = eh_fma(z, q, EH_COS_9);
+  q = eh_fma(z, q, EH_COS_8);
+  q = eh_fma(z, q, EH_COS_7);
+  q = eh_fma(z, q, EH_COS_6);
+  q = eh_fma(z, q, EH_COS_5);
+  q = eh_fma(z, q, EH_COS_4);
+  /* C = ch + cl */
+  const double t3 = z * EH_COS_3;
+  const double t3e = eh_fma(z, EH_COS_3, -t3) + (zl * EH_COS_3 + z * EH_COS_3_LO);
+  const double ch = EH_COS_2 + t3;
+  const double cl = ((EH_COS_2 - ch) + t3) + (EH_COS_2_LO + t3e + (z * z) * q);
+  /* z^2 = z2 + z2l, z^2 C = P + pe */
+  const double z2 = z * z, z2l = eh_fma(z, z, -z2) + 2.0 * z * zl;
+  const double P = z2 * ch;
+  const double pe = eh_fma(z2, ch, -P) + (z2 * cl + z2l * ch);
+  /* 1 - z/2 = w + t exactly (up to the -zl/2 term) */
+  const double hz = 0.5 * z;
+  const double w = 1.0 - hz;
+  const double t = (1.0 - w) - hz;
+  /* w + P = h + hl exactly (|w| > |P|) */
+  const double h = w + P;
+  const double hl = (w - h) + P;
+  /* rl sin(r), with sin(r) = r (1 - z/6 + z^2/120) to the precision that matters */
+  const double rs = rl * (r * (1.0 + z * (EH_SIN_1 + z * EH_SIN_2)));
+  return h + (hl + ((t - 0.5 * zl) + pe - rs));
+}
+
+/* Reduce x (|x| >= 2^-4) to r + rl = x - k pi/2 with the rounding error of r kept
+ * in rl: the first two pieces of pi/2 are subtracted exactly (Cody-Waite), the
+ * rest of k pi/2 beyond r's precision is formed separately and subtracted with
+ * an exact two-sum.  Returns k mod 4, or -1 outside the domain (non-finite or
+ * |x| >= 2^50). */
+EH_FN int eh_reduce_pio2_l(double x, double *r, double *rl) {
   if (!(eh_fabs(x) < 0x1p50)) {
-    *r = eh_nan();
+    *r = *rl = eh_nan();
     return -1;
   }
   double kf = eh_fma(x, EH_TWO_OVER_PI, 0x1.8p52) - 0x1.8p52;
   double t = eh_fma(-kf, EH_PIO2_1, x);
   t = eh_fma(-kf, EH_PIO2_2, t);
-  t = eh_fma(-kf, EH_PIO2_3, t);
-  t = eh_fma(-kf, EH_PIO2_4, t);
-  t = eh_fma(-kf, EH_PIO2_5, t);
-  *r = t;
+  double c = eh_fma(kf, EH_PIO2_3, eh_fma(kf, EH_PIO2_4, kf * EH_PIO2_5));
+  /* h + rl = t - c exactly (Knuth's TwoSum: no ordering of |t|, |c| needed) */
+  double h = t - c;
+  double v = h - t;
+  *rl = (t - (h - v)) + (-c - v);
+  *r = h;
   int64_t k = (int64_t)kf;
   return (int)(k & 3);
 }
 
 EH_FN void eh_sincos(double x, double *s, double *c) {
-  double r;
-  int q = eh_reduce_pio2(x, &r);
+  if (eh_fabs(x) < 0x1p-4) {
+    /* |x| < 2^-4: the short series, no reduction */
+    *s = eh_sin_poly(x);
+    *c = eh_cos_poly(x);
+    return;
+  }
+  double r, rl;
+  int q = eh_reduce_pio2_l(x, &r, &rl);
   if (q < 0) {
     *s = *c = eh_nan();
     return;
   }
-  double ps = eh_sin_poly(r), pc = eh_cos_poly(r);
+  double ps = eh_sin_poly_l(r, rl), pc = eh_cos_poly_l(r, rl);
   switch (q) {
   case 0:
     *s = ps;
@@ -186,34 +234,38 @@ EH_FN void eh_sincos(double x, double *s, double *c) {
 }
 
 EH_FN double eh_sin(double x) {
-  double r;
-  int q = eh_reduce_pio2(x, &r);
+  if (eh_fabs(x) < 0x1p-4)
+    return eh_sin_poly(x);
+  double r, rl;
+  int q = eh_reduce_pio2_l(x, &r, &rl);
   switch (q) {
   case 0:
-    return eh_sin_poly(r);
+    return eh_sin_poly_l(r, rl);
   case 1:
-    return eh_cos_poly(r);
+    return eh_cos_poly_l(r, rl);
   case 2:
-    return -eh_sin_poly(r);
+    return -eh_sin_poly_l(r, rl);
   case 3:
-    return -eh_cos_poly(r);
+    return -eh_cos_poly_l(r, rl);
   default:
     return eh_nan();
   }
 }
 
 EH_FN double eh_cos(double x) {
-  double r;
-  int q = eh_reduce_pio2(x, &r);
+  if (eh_fabs(x) < 0x1p-4)
+    return eh_cos_poly(x);
+  double r, rl;
+  int q = eh_reduce_pio2_l(x, &r, &rl);
   switch (q) {
   case 0:
-    return eh_cos_poly(r);
+    return eh_cos_poly_l(r, rl);
   case 1:
-    return -eh_sin_poly(r);
+    return -eh_sin_poly_l(r, rl);
   case 2:
-    return -eh_cos_poly(r);
+    return -eh_cos_poly_l(r, rl);
   case 3:
-    return eh_sin_poly(r);
+    return eh_sin_poly_l(r, rl);
   default:
     return eh_nan();
   }

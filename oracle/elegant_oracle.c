@@ -31,7 +31,30 @@
 
 #include "elegant_hip.h"
 
-#ifdef OR_LIBM
+#if defined(OR_DUAL)
+/* every transcendental computed both ways: the C library's value is used (the
+ * trajectory is elegant's as built) and the calls where the shared math
+ * library would have returned another double are counted (test builds) */
+#include "eh_math.h"
+static long or_dual_diff;
+long or_dual_count(void) { return or_dual_diff; }
+void or_dual_reset(void) { or_dual_diff = 0; }
+static double or_dual(double mine, double libc) {
+  if (mine != libc && !(mine != mine && libc != libc))
+    or_dual_diff++;
+  return libc;
+}
+static double or_dual_sin(double x) { return or_dual(eh_sin(x), sin(x)); }
+static double or_dual_cos(double x) { return or_dual(eh_cos(x), cos(x)); }
+static double or_dual_tan(double x) { return or_dual(eh_tan(x), tan(x)); }
+static double or_dual_asin(double x) { return or_dual(eh_asin(x), asin(x)); }
+static double or_dual_log(double x) { return or_dual(eh_log(x), log(x)); }
+#define OR_SIN or_dual_sin
+#define OR_COS or_dual_cos
+#define OR_TAN or_dual_tan
+#define OR_ASIN or_dual_asin
+#define OR_LOG or_dual_log
+#elif defined(OR_LIBM)
 #define OR_SIN sin
 #define OR_COS cos
 #define OR_TAN tan

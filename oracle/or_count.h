@@ -106,16 +106,31 @@ static inline or_cd fabs(or_cd a) { return or_cd(::fabs(a.v)); }
 static inline or_cd ceil(or_cd a) { return or_cd(::ceil(a.v)); }
 static inline bool isnan(or_cd a) { return a.v != a.v; }
 static inline bool isinf(or_cd a) { return ::fabs(a.v) == HUGE_VAL; }
-#define OR_COUNT_T(name)                                                                                          \
+/* the arguments of the transcendental calls, when a log is open (kind: 0 sin,
+ * 1 cos, 2 tan, 3 asin, 4 log) */
+extern "C" {
+extern double *or_targ;
+extern int32_t *or_tkind;
+extern long or_targ_n, or_targ_cap;
+}
+static inline void or_targ_note(int kind, double x) {
+  if (or_targ_n < or_targ_cap) {
+    or_targ[or_targ_n] = x;
+    or_tkind[or_targ_n] = kind;
+    or_targ_n++;
+  }
+}
+#define OR_COUNT_T(name, kind)                                                                                    \
   static inline or_cd name(or_cd a) {                                                                              \
     or_tally(5, true);                                                                                             \
+    or_targ_note(kind, a.v);                                                                                       \
     return or_cd(::name(a.v));                                                                                     \
   }
-OR_COUNT_T(sin)
-OR_COUNT_T(cos)
-OR_COUNT_T(tan)
-OR_COUNT_T(asin)
-OR_COUNT_T(log)
+OR_COUNT_T(sin, 0)
+OR_COUNT_T(cos, 1)
+OR_COUNT_T(tan, 2)
+OR_COUNT_T(asin, 3)
+OR_COUNT_T(log, 4)
 #undef OR_COUNT_T
 
 #define double or_cd

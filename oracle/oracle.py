@@ -56,6 +56,9 @@ def load_path(path):
     if hasattr(lib, "or_count_read"):
         lib.or_count_read.argtypes = [vp]
         lib.or_count_reset.argtypes = []
+        lib.or_targ_start.argtypes = [C.c_long]
+        lib.or_targ_read.argtypes = [vp, vp, C.c_long]
+        lib.or_targ_read.restype = C.c_long
     for f in ("or_sin", "or_cos", "or_asin", "or_log"):
         getattr(lib, f).argtypes = [C.c_double]
         getattr(lib, f).restype = C.c_double
@@ -85,6 +88,24 @@ def count_ops(coords, lattice, n_passes, p_central):
     counts = dict(zip(OP_NAMES, (int(v) for v in out[:6])))
     counts.update(("all_" + k, int(v)) for k, v in zip(OP_NAMES, out[6:]))
     return counts, rows, n, eff.value
+
+
+T_KINDS = ("sin", "cos", "tan", "asin", "log")
+
+
+def transcendental_args(coords, lattice, n_passes, p_central, cap=10_000_000):
+    """the arguments of every sin, cos, tan, asin and log call do_tracking makes
+    (counting build): returns (args, kinds) with kinds indexing T_KINDS"""
+    lib = load(count=True)
+    lib.or_targ_start(cap)
+    count_ops(coords, lattice, n_passes, p_central)
+    n = lib.or_targ_read(None, None, 0)
+    n = min(n, cap)
+    args = np.zeros(n)
+    kinds = np.zeros(n, dtype=np.int32)
+    lib.or_targ_read(args.ctypes.data, kinds.ctypes.data, n)
+    lib.or_targ_start(0)
+    return args, kinds
 
 
 def _check(lib, r):
