@@ -176,6 +176,29 @@ def leg_fused(args, census):
     return desc, len(rows), n, wall, st, cen
 
 
+def leg_lossy(args):
+    """the headline ring with a grid reaching past the 20 mm MAXAMP and the
+    dynamic aperture (x in +-30 mm, y in +-25 mm): particles are lost from the
+    first element on, so every launch compacts survivors and appends lost
+    particles to the graveyard; timed from the first turn (pass-0 RF fiducial
+    included)"""
+    from elegant_amd import configs, tracking
+    Po = configs.p_central_from_mev(3000.0)
+    lat = configs.mba_ring(Po)
+    rows = configs.da_grid(args.grid, args.grid, xmax=0.030, ymax=0.025)
+    beam = tracking.Beam(rows)
+    t0 = time.perf_counter()
+    n, st, _ = tracking.track(beam, lat, args.lossy_steps, Po, mode="fused", turns_per_launch=args.turns_per_launch)
+    wall = time.perf_counter() - t0
+    beam.free()
+    units = st["particle_element_turns"]
+    log("lossy timed: %d turns, %d of %d alive at the end, %.3f s wall" % (args.lossy_steps, n, len(rows), wall))
+    return {"value": units / wall, "unit": UNIT, "turns": args.lossy_steps, "particles": len(rows), "survivors": n,
+            "lost": len(rows) - n, "wall_s": round(wall, 4), "launches": st["launches"],
+            "workload": "40-cell MBA ring 3 GeV, %dx%d grid x +-30 mm, y +-25 mm (past the 20 mm MAXAMP), fused, "
+                        "timed from the first turn" % (args.grid, args.grid)}
+
+
 def leg_batched(args):
     from elegant_amd import autotune, configs, tracking
     Po = configs.p_central_from_mev(3000.0)
@@ -370,6 +393,8 @@ def main():
     ap.add_argument("--variants", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=4)
     ap.add_argument("--plain-steps", type=int, default=4)
+    ap.add_argument("--lossy-steps", type=int, default=8)
+    ap.add_argument("--no-lossy", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=10000)
     ap.add_argument("--cpu-turns", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -454,6 +479,8 @@ def main():
            "roofline": roofline}
     if cpu is not None:
         out["cpu_baseline"] = cpu
+    if not args.no_lossy and args.config == "mba":
+        out["lossy"] = leg_lossy(args)
     if not args.no_batch:
         out["batched"] = leg_batched(args)
     if not args.no_plain:

@@ -17,8 +17,7 @@
  *    24-bit pieces plus a tail, so k*piece is exact for |k| < 2^29; the reduced
  *    argument |r| <= ~pi/4 is kept as r + rl (the rounding error of r in rl)
  *    and fed to the Taylor series of sin (to r^21) and cos (to r^20), evaluated
- *    by Horner's rule with a fused multiply-add per step, plus the first-order
- *    terms in rl.
+ *    by Horner's rule, plus the first-order terms in rl.
  *  - asin: Taylor series sum_n (2n)!/(4^n (n!)^2 (2n+1)) x^(2n+1) for |x| <= 1/2
  *    (10 terms for |x| <= 1/8, 40 terms above), and asin(x) = pi/2 - 2 asin(
  *    sqrt((1-|x|)/2)) for |x| > 1/2.
@@ -71,22 +70,39 @@ EH_FN double eh_nan(void) { return eh_from_bits(0x7ff8000000000000ULL); }
 EH_FN double eh_sqrt(double x) { return __builtin_sqrt(x); }
 
 /* fused multiply-add: exactly specified by IEEE-754 (one rounding), hence
- * identical on every conforming host and on the GPU.  Used for the exact
- * rounding errors of products and for every Horner step (one instruction and
- * one rounding per coefficient instead of two).  The host build of this header
+ * identical on every conforming host and on the GPU; used for the exact rounding
+ * errors of products.  (Horner steps stay a multiply and an add: with fused
+ * steps the compiler keeps the coefficients in vector registers across the
+ * tracking kernels' loops, which then spill.)  The host build of this header
  * must compile it to the fused instruction (gcc/clang -mfma) or to the C
  * library's fma, which is correctly rounded too. */
 EH_FN double eh_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+/* Taylor polynomial of sin(r) for 2^-4 <= |r| < pi/4 (to r^21) */
+EH_FN double eh_sin_poly10(double r) {
+  double z = r * r;
+  double p = EH_SIN_10;
+  p = EH_SIN_9 + z * p;
+  p = EH_SIN_8 + z * p;
+  p = EH_SIN_7 + z * p;
+  p = EH_SIN_6 + z * p;
+  p = EH_SIN_5 + z * p;
+  p = EH_SIN_4 + z * p;
+  p = EH_SIN_3 + z * p;
+  p = EH_SIN_2 + z * p;
+  p = EH_SIN_1 + z * p;
+  return eh_fma(r * z, p, r);
+}
 
 /* Taylor polynomial of sin(r) for |r| < 2^-4, cut after the r^11 term (the
  * next one is below 2^-80 of the result) */
 EH_FN double eh_sin_poly(double r) {
   double z = r * r;
   double q = EH_SIN_5;
-  q = eh_fma(z, q, EH_SIN_4);
-  q = eh_fma(z, q, EH_SIN_3);
-  q = eh_fma(z, q, EH_SIN_2);
-  q = eh_fma(z, q, EH_SIN_1);
+  q = EH_SIN_4 + z * q;
+  q = EH_SIN_3 + z * q;
+  q = EH_SIN_2 + z * q;
+  q = EH_SIN_1 + z * q;
   return eh_fma(r * z, q, r);
 }
 
@@ -99,13 +115,13 @@ EH_FN double eh_sin_poly(double r) {
 EH_FN double eh_sin_poly_l(double r, double rl) {
   const double z = r * r, zl = eh_fma(r, r, -z);
   double q = EH_SIN_10;
-  q = eh_fma(z, q, EH_SIN_9);
-  q = eh_fma(z, q, EH_SIN_8);
-  q = eh_fma(z, q, EH_SIN_7);
-  q = eh_fma(z, q, EH_SIN_6);
-  q = eh_fma(z, q, EH_SIN_5);
-  q = eh_fma(z, q, EH_SIN_4);
-  q = eh_fma(z, q, EH_SIN_3);
+  q = EH_SIN_9 + z * q;
+  q = EH_SIN_8 + z * q;
+  q = EH_SIN_7 + z * q;
+  q = EH_SIN_6 + z * q;
+  q = EH_SIN_5 + z * q;
+  q = EH_SIN_4 + z * q;
+  q = EH_SIN_3 + z * q;
   /* S = S1 + z S2 + z^2 q as sh + sl */
   const double t2 = z * EH_SIN_2;
   const double t2e = eh_fma(z, EH_SIN_2, -t2) + (zl * EH_SIN_2 + z * EH_SIN_2_LO);
@@ -121,15 +137,35 @@ EH_FN double eh_sin_poly_l(double r, double rl) {
   return h + (hl + (te + rl * (1.0 - 0.5 * z)));
 }
 
+/* Taylor polynomial of cos(r) for 2^-4 <= |r| < pi/4 (to r^20), leading terms
+ * as in eh_cos_poly */
+EH_FN double eh_cos_poly10(double r) {
+  double z = r * r;
+  double p = EH_COS_10;
+  p = EH_COS_9 + z * p;
+  p = EH_COS_8 + z * p;
+  p = EH_COS_7 + z * p;
+  p = EH_COS_6 + z * p;
+  p = EH_COS_5 + z * p;
+  p = EH_COS_4 + z * p;
+  p = EH_COS_3 + z * p;
+  p = EH_COS_2 + z * p;
+  double zl = eh_fma(r, r, -z);
+  double hz = 0.5 * z;
+  double w = 1.0 - hz;
+  double t = (1.0 - w) - hz;
+  return w + eh_fma(z * z, p, t - 0.5 * zl);
+}
+
 /* Taylor polynomial of cos(r) for |r| < 2^-4, cut after the r^12 term (the
  * next one is below 2^-76 of the result) */
 EH_FN double eh_cos_poly(double r) {
   double z = r * r;
   double p = EH_COS_6;
-  p = eh_fma(z, p, EH_COS_5);
-  p = eh_fma(z, p, EH_COS_4);
-  p = eh_fma(z, p, EH_COS_3);
-  p = eh_fma(z, p, EH_COS_2);
+  p = EH_COS_5 + z * p;
+  p = EH_COS_4 + z * p;
+  p = EH_COS_3 + z * p;
+  p = EH_COS_2 + z * p;
   /* 1 - z/2 + z^2 p.  1 - z/2 is formed with its exact rounding error (Fast2Sum,
    * valid since z/2 <= 1) and the exact error of z = r*r, so that the final
    * addition is the only rounding of the leading terms: the result is then
@@ -149,12 +185,12 @@ EH_FN double eh_cos_poly(double r) {
 EH_FN double eh_cos_poly_l(double r, double rl) {
   const double z = r * r, zl = eh_fma(r, r, -z);
   double q = EH_COS_10;
-  q = eh_fma(z, q, EH_COS_9);
-  q = eh_fma(z, q, EH_COS_8);
-  q = eh_fma(z, q, EH_COS_7);
-  q = eh_fma(z, q, EH_COS_6);
-  q = eh_fma(z, q, EH_COS_5);
-  q = eh_fma(z, q, EH_COS_4);
+  q = EH_COS_9 + z * q;
+  q = EH_COS_8 + z * q;
+  q = EH_COS_7 + z * q;
+  q = EH_COS_6 + z * q;
+  q = EH_COS_5 + z * q;
+  q = EH_COS_4 + z * q;
   /* C = ch + cl */
   const double t3 = z * EH_COS_3;
   const double t3e = eh_fma(z, EH_COS_3, -t3) + (zl * EH_COS_3 + z * EH_COS_3_LO);
@@ -176,7 +212,7 @@ EH_FN double eh_cos_poly_l(double r, double rl) {
   return h + (hl + ((t - 0.5 * zl) + pe - rs));
 }
 
-/* Reduce x (|x| >= 2^-4) to r + rl = x - k pi/2 with the rounding error of r kept
+/* Reduce x (|x| >= pi/4) to r + rl = x - k pi/2 with the rounding error of r kept
  * in rl: the first two pieces of pi/2 are subtracted exactly (Cody-Waite), the
  * rest of k pi/2 beyond r's precision is formed separately and subtracted with
  * an exact two-sum.  Returns k mod 4, or -1 outside the domain (non-finite or
@@ -199,20 +235,45 @@ EH_FN int eh_reduce_pio2_l(double x, double *r, double *rl) {
   return (int)(k & 3);
 }
 
+/* Reduce x to r = x - k pi/2 with k the nearest integer to x 2/pi (the rounding
+ * of r is not kept); returns k mod 4, or -1 outside the domain */
+EH_FN int eh_reduce_pio2(double x, double *r) {
+  if (!(eh_fabs(x) < 0x1p50)) {
+    *r = eh_nan();
+    return -1;
+  }
+  double kf = eh_fma(x, EH_TWO_OVER_PI, 0x1.8p52) - 0x1.8p52;
+  double t = eh_fma(-kf, EH_PIO2_1, x);
+  t = eh_fma(-kf, EH_PIO2_2, t);
+  t = eh_fma(-kf, EH_PIO2_3, t);
+  t = eh_fma(-kf, EH_PIO2_4, t);
+  t = eh_fma(-kf, EH_PIO2_5, t);
+  *r = t;
+  int64_t k = (int64_t)kf;
+  return (int)(k & 3);
+}
+
+/* sin and cos of x for the tracking maps (CSBEND's sub-steps), whose arguments
+ * lie below pi/4: the short series below 2^-4, the full series below pi/4, and
+ * a plain reduction beyond (kept small: this path sits inside the maps' inner
+ * loops, where registers are scarce) */
 EH_FN void eh_sincos(double x, double *s, double *c) {
-  if (eh_fabs(x) < 0x1p-4) {
-    /* |x| < 2^-4: the short series, no reduction */
+  const double ax = eh_fabs(x);
+  if (ax < 0x1p-4) {
     *s = eh_sin_poly(x);
     *c = eh_cos_poly(x);
     return;
   }
-  double r, rl;
-  int q = eh_reduce_pio2_l(x, &r, &rl);
-  if (q < 0) {
-    *s = *c = eh_nan();
-    return;
+  double r = x;
+  int q = 0;
+  if (!(ax < 0x1.921fb54442d18p-1)) {
+    q = eh_reduce_pio2(x, &r);
+    if (q < 0) {
+      *s = *c = eh_nan();
+      return;
+    }
   }
-  double ps = eh_sin_poly_l(r, rl), pc = eh_cos_poly_l(r, rl);
+  double ps = eh_sin_poly10(r), pc = eh_cos_poly10(r);
   switch (q) {
   case 0:
     *s = ps;
@@ -233,39 +294,48 @@ EH_FN void eh_sincos(double x, double *s, double *c) {
   }
 }
 
-EH_FN double eh_sin(double x) {
-  if (eh_fabs(x) < 0x1p-4)
-    return eh_sin_poly(x);
-  double r, rl;
-  int q = eh_reduce_pio2_l(x, &r, &rl);
-  switch (q) {
+EH_FN double eh_cos(double x) {
+  const double ax = eh_fabs(x);
+  if (ax < 0x1p-4)
+    return eh_cos_poly(x);
+  if (ax < 0x1.921fb54442d18p-1)
+    return eh_cos_poly10(x);
+  double r;
+  switch (eh_reduce_pio2(x, &r)) {
   case 0:
-    return eh_sin_poly_l(r, rl);
+    return eh_cos_poly10(r);
   case 1:
-    return eh_cos_poly_l(r, rl);
+    return -eh_sin_poly10(r);
   case 2:
-    return -eh_sin_poly_l(r, rl);
+    return -eh_cos_poly10(r);
   case 3:
-    return -eh_cos_poly_l(r, rl);
+    return eh_sin_poly10(r);
   default:
     return eh_nan();
   }
 }
 
-EH_FN double eh_cos(double x) {
-  if (eh_fabs(x) < 0x1p-4)
-    return eh_cos_poly(x);
+/* sin of any argument: below pi/4 as eh_sincos; beyond, the reduction keeps
+ * its rounding error and the series is evaluated in double-double, so that the
+ * RF cavity's phase (about 3 rad) is correctly rounded as often as the C
+ * library's */
+EH_FN double eh_sin(double x) {
+  const double ax = eh_fabs(x);
+  if (ax < 0x1p-4)
+    return eh_sin_poly(x);
+  if (ax < 0x1.921fb54442d18p-1)
+    return eh_sin_poly10(x);
   double r, rl;
   int q = eh_reduce_pio2_l(x, &r, &rl);
   switch (q) {
   case 0:
-    return eh_cos_poly_l(r, rl);
-  case 1:
-    return -eh_sin_poly_l(r, rl);
-  case 2:
-    return -eh_cos_poly_l(r, rl);
-  case 3:
     return eh_sin_poly_l(r, rl);
+  case 1:
+    return eh_cos_poly_l(r, rl);
+  case 2:
+    return -eh_sin_poly_l(r, rl);
+  case 3:
+    return -eh_cos_poly_l(r, rl);
   default:
     return eh_nan();
   }
@@ -285,31 +355,31 @@ EH_FN double eh_asin_small(double x) {
   double ax = eh_fabs(x);
   if (ax < 0x1p-8) {
     double q = EH_ASIN_4;
-    q = eh_fma(z, q, EH_ASIN_3);
-    q = eh_fma(z, q, EH_ASIN_2);
-    q = eh_fma(z, q, EH_ASIN_1);
+    q = EH_ASIN_3 + z * q;
+    q = EH_ASIN_2 + z * q;
+    q = EH_ASIN_1 + z * q;
     return eh_fma(x * z, q, x);
   }
   if (ax < 0x1p-4) {
     double q = EH_ASIN_7;
-    q = eh_fma(z, q, EH_ASIN_6);
-    q = eh_fma(z, q, EH_ASIN_5);
-    q = eh_fma(z, q, EH_ASIN_4);
-    q = eh_fma(z, q, EH_ASIN_3);
-    q = eh_fma(z, q, EH_ASIN_2);
-    q = eh_fma(z, q, EH_ASIN_1);
+    q = EH_ASIN_6 + z * q;
+    q = EH_ASIN_5 + z * q;
+    q = EH_ASIN_4 + z * q;
+    q = EH_ASIN_3 + z * q;
+    q = EH_ASIN_2 + z * q;
+    q = EH_ASIN_1 + z * q;
     return eh_fma(x * z, q, x);
   }
   double p = EH_ASIN_10;
-  p = eh_fma(z, p, EH_ASIN_9);
-  p = eh_fma(z, p, EH_ASIN_8);
-  p = eh_fma(z, p, EH_ASIN_7);
-  p = eh_fma(z, p, EH_ASIN_6);
-  p = eh_fma(z, p, EH_ASIN_5);
-  p = eh_fma(z, p, EH_ASIN_4);
-  p = eh_fma(z, p, EH_ASIN_3);
-  p = eh_fma(z, p, EH_ASIN_2);
-  p = eh_fma(z, p, EH_ASIN_1);
+  p = EH_ASIN_9 + z * p;
+  p = EH_ASIN_8 + z * p;
+  p = EH_ASIN_7 + z * p;
+  p = EH_ASIN_6 + z * p;
+  p = EH_ASIN_5 + z * p;
+  p = EH_ASIN_4 + z * p;
+  p = EH_ASIN_3 + z * p;
+  p = EH_ASIN_2 + z * p;
+  p = EH_ASIN_1 + z * p;
   return eh_fma(x * z, p, x);
 }
 
@@ -325,7 +395,7 @@ EH_FN double eh_asin_mid(double x) {
   double p = a[39];
 #pragma unroll
   for (int n = 38; n >= 0; n--)
-    p = eh_fma(z, p, a[n]);
+    p = a[n] + z * p;
   return eh_fma(x * z, p, x);
 }
 
@@ -375,35 +445,35 @@ EH_FN double eh_log(double x) {
     double w = f - hh;
     double t = (f - w) - hh;
     double q = EH_LOG1P_16;
-    q = eh_fma(f, q, EH_LOG1P_15);
-    q = eh_fma(f, q, EH_LOG1P_14);
-    q = eh_fma(f, q, EH_LOG1P_13);
-    q = eh_fma(f, q, EH_LOG1P_12);
-    q = eh_fma(f, q, EH_LOG1P_11);
-    q = eh_fma(f, q, EH_LOG1P_10);
-    q = eh_fma(f, q, EH_LOG1P_9);
-    q = eh_fma(f, q, EH_LOG1P_8);
-    q = eh_fma(f, q, EH_LOG1P_7);
-    q = eh_fma(f, q, EH_LOG1P_6);
-    q = eh_fma(f, q, EH_LOG1P_5);
-    q = eh_fma(f, q, EH_LOG1P_4);
-    q = eh_fma(f, q, EH_LOG1P_3);
+    q = EH_LOG1P_15 + f * q;
+    q = EH_LOG1P_14 + f * q;
+    q = EH_LOG1P_13 + f * q;
+    q = EH_LOG1P_12 + f * q;
+    q = EH_LOG1P_11 + f * q;
+    q = EH_LOG1P_10 + f * q;
+    q = EH_LOG1P_9 + f * q;
+    q = EH_LOG1P_8 + f * q;
+    q = EH_LOG1P_7 + f * q;
+    q = EH_LOG1P_6 + f * q;
+    q = EH_LOG1P_5 + f * q;
+    q = EH_LOG1P_4 + f * q;
+    q = EH_LOG1P_3 + f * q;
     return w + eh_fma(h * f, q, t - 0.5 * hl);
   }
   double s = f / (2.0 + f);
   double z = s * s;
   double p = EH_LOG_12;
-  p = eh_fma(z, p, EH_LOG_11);
-  p = eh_fma(z, p, EH_LOG_10);
-  p = eh_fma(z, p, EH_LOG_9);
-  p = eh_fma(z, p, EH_LOG_8);
-  p = eh_fma(z, p, EH_LOG_7);
-  p = eh_fma(z, p, EH_LOG_6);
-  p = eh_fma(z, p, EH_LOG_5);
-  p = eh_fma(z, p, EH_LOG_4);
-  p = eh_fma(z, p, EH_LOG_3);
-  p = eh_fma(z, p, EH_LOG_2);
-  p = eh_fma(z, p, EH_LOG_1);
+  p = EH_LOG_11 + z * p;
+  p = EH_LOG_10 + z * p;
+  p = EH_LOG_9 + z * p;
+  p = EH_LOG_8 + z * p;
+  p = EH_LOG_7 + z * p;
+  p = EH_LOG_6 + z * p;
+  p = EH_LOG_5 + z * p;
+  p = EH_LOG_4 + z * p;
+  p = EH_LOG_3 + z * p;
+  p = EH_LOG_2 + z * p;
+  p = EH_LOG_1 + z * p;
   double lm = eh_fma(s * z, p, 2.0 * s);
   double de = (double)e;
   return de * EH_LN2_HI + (lm + de * EH_LN2_LO);
