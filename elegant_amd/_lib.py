@@ -24,7 +24,7 @@ EXPORTS = [
     "eh_batch_download", "eh_batch_n_alive", "eh_batch_track", "eh_diag_copy", "eh_adjoint_create",
     "eh_adjoint_free", "eh_adjoint_forward", "eh_adjoint_backward", "eh_adjoint_forward_batch",
     "eh_multipole_tracking", "eh_offset_beam", "eh_rotate_beam", "eh_replay_sequence", "eh_set_central_momentum",
-    "eh_track_particles", "eh_beam_row_order",
+    "eh_track_particles", "eh_beam_row_order", "eh_elimit_amplitudes",
 ]
 
 
@@ -88,6 +88,8 @@ def load():
     lib.eh_drift_beam.restype = L
     lib.eh_limit_amplitudes.argtypes = [vp, D, D, L, D, D, L, L]
     lib.eh_limit_amplitudes.restype = L
+    lib.eh_elimit_amplitudes.argtypes = [vp, D, D, L, D, D, L, L, L, L]
+    lib.eh_elimit_amplitudes.restype = L
     lib.eh_removeInvalidParticles.argtypes = [vp, L, D, D]
     lib.eh_removeInvalidParticles.restype = L
     lib.eh_do_tracking.argtypes = [vp, elp, L, L, dp, I, I, C.POINTER(S.eh_track_stats)]

@@ -31,7 +31,7 @@
  * upload and permutes them by eh_beam_row_order on every download.
  *
  * A gpu_<routine> called with an argument the library does not carry (an
- * aperture contour or file, sigmaDelta2, a slice index, an elliptical MAXAMP,
+ * aperture contour or file, sigmaDelta2, a slice index, an open-sided MAXAMP,
  * wakes, a third-order matrix, ...) runs the CPU routine itself on the host copy
  * with elementOnGpu cleared for the duration of the call.
  *
@@ -456,16 +456,19 @@ static const char *const maxamp_untracked[] __attribute__((unused)) = {NULL};
 
 /* MAXAMP, src/track.h:2008-2013, as the in-element apertures and the
  * post-element sweeps read it (setupMultApertureData, src/multipole.c:1690-1749;
- * src/do_tracking.c:2506-2517): the library carries the rectangular form */
+ * src/do_tracking.c:2506-2517): rectangular and (super-)elliptical, no open
+ * side; exponents that do not fit the library's int32 stay on the host */
 static int fill_maxamp(const MAXAMP *m, EH_MAXAMP *d) {
-  if (m->elliptical || !no_file(m->openSide))
+  if (!no_file(m->openSide))
+    return 0;
+  if (m->exponent != (int32_t)m->exponent || m->yExponent != (int32_t)m->yExponent)
     return 0;
   memset(d, 0, sizeof *d);
   d->x_max = m->x_max;
   d->y_max = m->y_max;
-  d->elliptical = (int32_t)m->elliptical;
+  d->elliptical = (int32_t)(m->elliptical != 0);
   d->exponent = (int32_t)m->exponent;
-  (void)m->yExponent; /* read only with ELLIPTICAL */
+  d->y_exponent = (int32_t)m->yExponent;
   return 1;
 }
 
@@ -807,13 +810,28 @@ long gpu_limit_amplitudes(double xmax, double ymax, long np, double **accepted, 
   return r;
 }
 
-/* the elliptical sweep (src/limit_amplitudes.c:487-620) on the host copy */
+/* the elliptical sweep (src/limit_amplitudes.c:487-620); open sides and a
+ * foreign accepted array on the host copy */
 long gpu_elimit_amplitudes(double xmax, double ymax, long np, double **accepted, double z, double Po,
                            long extrapolate_z, long openCode, long exponent, long yexponent, ELEMENT_LIST *eptr) {
   long r;
-  double **coord = ad_host_begin();
-  r = elimit_amplitudes(coord, xmax, ymax, np, accepted, z, Po, extrapolate_z, openCode, exponent, yexponent, eptr);
-  ad_host_end(1);
+  if (openCode == 0 && (accepted == NULL || accepted == G.accepted)) {
+    ad_to_device(np);
+    r = eh_elimit_amplitudes(G.beam, xmax, ymax, np, z, Po, extrapolate_z, openCode, exponent, yexponent);
+    if (r >= 0) {
+      ad_device_changed();
+      return r;
+    }
+    if (r != EH_ERR_UNSUPPORTED && r != EH_ERR_ARG)
+      ad_fatal("elimit_amplitudes");
+  }
+  /* elegant's own routine: the exponent errors are its to report */
+  {
+    double **coord = ad_host_begin();
+    r = elimit_amplitudes(coord, xmax, ymax, np, accepted, z, Po, extrapolate_z, openCode, exponent, yexponent,
+                          eptr);
+    ad_host_end(1);
+  }
   return r;
 }
 

@@ -38,9 +38,13 @@
  * behind (integrate_kick_multipole_ordn resets dzLoss only after the entry
  * checks, src/multipole.c:1160-1232), which the host resolves in row order */
 #define EH_STAGE_INHERIT 4
+/* loss pass word: the pass, with this bit set for a loss that inherits its
+ * position (EH_STAGE_INHERIT); cleared before the word reaches the caller */
+#define EH_LPASS_INHERIT 0x40000000
 /* returned by the lean map (RAD = false) for an element type it does not carry
- * (MULT, ROTATE, MALIGN, ENERGY); the host never launches a lean kernel on a
- * line that holds one (eh_lean_capable) */
+ * (MULT, ROTATE, MALIGN, ENERGY) or a descriptor with an elliptical aperture;
+ * the host never launches a lean kernel on a line that holds one
+ * (eh_lean_capable) */
 #define EH_STAGE_REFUSED 8
 
 typedef struct {
@@ -58,6 +62,15 @@ typedef struct {
   double post_xmax[2], post_ymax[2];
   int32_t post_ap[2];
   int32_t extrap;
+  /* elliptical MAXAMP for pass 0 / later passes: the in-element check of
+   * checkMultAperture (ap_ell: ELLIPTICAL with both limits positive) and the
+   * elimit_amplitudes sweep (post_ell), with exponents xe, ye and
+   * a2 = ipow(x_max, xe), b2 = ipow(y_max, ye); ell_any if any is set (such
+   * descriptors run only through the full instantiation of the maps) */
+  int32_t ap_ell[2], post_ell[2];
+  int32_t ell_xe[2], ell_ye[2];
+  double ell_a2[2], ell_b2[2];
+  int32_t ell_any;
   double z_start, z_end; /* last_z and z of do_tracking for this element */
   double Po;
 
@@ -244,15 +257,117 @@ EH_HD bool eh_limit_one(eh_p6 &p, double xmax, double ymax, double z, double Po,
   return true;
 }
 
+/* x^p for p >= 0 as the SDDS toolkit's ipow (mdbmth/ipow.c, which elegant
+ * links for its elliptical apertures): fixed products of x^2 up to p = 8 (x^4 =
+ * x^2 x^2, x^6 = x^2 x^2 x^2, x^7 = x^3 x^3 x, x^8 = (x^2 x^2)(x^2 x^2)), larger
+ * p by halving, x^p = h h or h h x with h = x^(p/2), unrolled here from the
+ * innermost level outwards; 0^p is 0 (p > 0) or 1.  p is wavefront-uniform. */
+EH_HD double eh_ipow(double x, int p) {
+  if (x == 0)
+    return p == 0 ? 1.0 : 0.0;
+  uint32_t bits = 0; /* parities of the halving levels, the innermost in bit 0 */
+  int depth = 0;
+  while (p > 8) {
+    bits = (bits << 1) | (uint32_t)(p & 1);
+    p >>= 1;
+    depth++;
+  }
+  double h, r;
+  switch (p) {
+  case 0:
+    r = 1.0;
+    break;
+  case 1:
+    r = x;
+    break;
+  case 2:
+    r = x * x;
+    break;
+  case 3:
+    h = x * x;
+    r = h * x;
+    break;
+  case 4:
+    h = x * x;
+    r = h * h;
+    break;
+  case 5:
+    h = x * x;
+    r = h * h * x;
+    break;
+  case 6:
+    h = x * x;
+    r = h * h * h;
+    break;
+  case 7:
+    h = x * x * x;
+    r = h * h * x;
+    break;
+  default:
+    h = x * x;
+    h = h * h;
+    r = h * h;
+    break;
+  }
+  for (int i = 0; i < depth; i++) {
+    r = r * r;
+    if ((bits >> i) & 1)
+      r = r * x;
+  }
+  return r;
+}
+
+/* elimit_amplitudes for one particle, src/limit_amplitudes.c:548-616 (no open
+ * side): a non-finite x or y, or x^xe/a2 + y^ye/b2 > 1, loses the particle;
+ * with a true ellipse after a drift (extrap) the loss point is found by
+ * solving the straight line back to the boundary.  A lost particle's dp
+ * becomes sqrt((Po (1 + dp))^2 + 1) and its s the loss position. */
+EH_HD bool eh_elimit_one(eh_p6 &p, double a2, double b2, int xe, int ye, double z, double Po, int extrap) {
+  if (eh_isinf(p.x) || eh_isinf(p.y) || eh_isnan(p.x) || eh_isnan(p.y)) {
+    p.s = z;
+    p.dp = eh_sqrt(eh_sqr(Po * (1 + p.dp)) + 1);
+    return true;
+  }
+  if (!((eh_ipow(p.x, xe) / a2 + eh_ipow(p.y, ye) / b2) > 1))
+    return false;
+  double dz = 0;
+  if (extrap && xe == 2 && ye == 2) {
+    const double c0 = eh_sqr(p.x) / a2 + eh_sqr(p.y) / b2 - 1;
+    const double c1 = 2 * (p.x * p.xp / a2 + p.y * p.yp / b2);
+    const double c2 = eh_sqr(p.xp) / a2 + eh_sqr(p.yp) / b2;
+    const double det = eh_sqr(c1) - 4 * c0 * c2;
+    if (z > 0 && c2 != 0 && det >= 0) {
+      if ((dz = (-c1 + eh_sqrt(det)) / (2 * c2)) > 0)
+        dz = (-c1 - eh_sqrt(det)) / (2 * c2);
+      if ((z + dz) < 0)
+        dz = -z;
+      p.x += dz * p.xp;
+      p.y += dz * p.yp;
+    }
+  }
+  p.s = z + dz;
+  p.dp = eh_sqrt(eh_sqr(Po * (1 + p.dp)) + 1);
+  return true;
+}
+
 /* ------------------------------------------------------------------------- */
-/* rectangular in-element aperture, checkMultAperture (src/multipole.c:1751-1798) */
-EH_HD bool eh_ap_ok(double xIn, double yIn, double xmax, double ymax, int rtilt, double cs0,
-                                         double cs1) {
+/* in-element aperture, checkMultAperture (src/multipole.c:1751-1798): the
+ * rectangular limits, or with ELL and ell set the (super-)ellipse
+ * (x/xmax)^xe + (y/ymax)^ye >= 1 (ell is set only with xmax, ymax > 0).  ELL is
+ * false in the lean instantiation of the maps, which never runs a descriptor
+ * with an elliptical aperture. */
+template <bool ELL = false>
+EH_HD bool eh_ap_ok(double xIn, double yIn, double xmax, double ymax, int rtilt, double cs0, double cs1, int ell = 0,
+                    int xe = 0, int ye = 0) {
   double x = xIn - 0.0, y = yIn - 0.0;
   if (rtilt) {
     double x0 = x, y0 = y;
     x = x0 * cs0 + y0 * cs1;
     y = -x0 * cs1 + y0 * cs0;
+  }
+  if (ELL && ell) {
+    const double xa = x / xmax, yb = y / ymax;
+    return !((eh_ipow(xa, xe) + eh_ipow(yb, ye)) >= 1);
   }
   if ((xmax > 0 && eh_fabs(x) > xmax) || (ymax > 0 && eh_fabs(y) > ymax))
     return false;
@@ -465,10 +580,12 @@ EH_HD void eh_mface2(eh_p6 &p, const eh_desc &d, bool s_too) {
  * Returns 0 if the particle survives, 1 if lost, 2 if lost before the first
  * step (its loss position is z_start plus the previous particle's dzLoss).  A
  * lost particle keeps s = z_start + dzLoss without the exit offset's path-length
- * term. */
-template <bool RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh_desc &d, int pi) {
+ * term.  RAD: the element radiates (d.rad); ELL: the elliptical aperture check is
+ * compiled in (the full instantiation of the maps; the lean one has neither). */
+template <bool RAD, bool ELL = RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh_desc &d, int pi) {
   eh_mface1(p, d);
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
+  const int ell = ELL ? d.ap_ell[pi] : 0;
   double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, dp = p.dp, qx, qy;
   double beta0 = 0;
   if constexpr (RAD) {
@@ -500,7 +617,8 @@ template <bool RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh_desc &d, int pi)
     } else {
       eh_pow4 w;
       for (int ik = 0; ik < d.nslices && !lost; ik++) {
-        if (!eh_ap_ok(x + d.mdx, y + d.mdy, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1)) {
+        if (!eh_ap_ok<ELL>(x + d.mdx, y + d.mdy, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi],
+                           d.ell_ye[pi])) {
           lost = true;
           how = 1;
           break;
@@ -578,7 +696,8 @@ template <bool RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh_desc &d, int pi)
         }
       }
       if (!lost) {
-        if (!eh_ap_ok(x + d.mdx, y + d.mdy, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1)) {
+        if (!eh_ap_ok<ELL>(x + d.mdx, y + d.mdy, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi],
+                           d.ell_ye[pi])) {
           lost = true;
           how = 1;
         } else if (!eh_m2s(xp, yp, qx, qy, dp, d.expand)) {
@@ -619,7 +738,7 @@ template <bool RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh_desc &d, int pi)
 }
 
 EH_HD int eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
-  return d.rad ? eh_multipole_t<true>(p, d, pi) : eh_multipole_t<false>(p, d, pi);
+  return d.rad ? eh_multipole_t<true, true>(p, d, pi) : eh_multipole_t<false, true>(p, d, pi);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1031,10 +1150,11 @@ EH_HD void eh_edge2(double &x, double &xp, double &y, double &yp, double rho, do
 
 /* CSBEND: track_through_csbend (src/csbend.c:1088-1403) for one particle, with
  * integrate_csbend_ordn (src/csbend.c:1459-1720) and the canonical conversions
- * (:1436-1454).  Returns true if lost. */
-template <bool RAD>
+ * (:1436-1454).  Returns true if lost.  RAD and ELL as for eh_multipole_t. */
+template <bool RAD, bool ELL = RAD>
 EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms) {
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
+  const int ell = ELL ? d.ap_ell[pi] : 0;
   const double rho0 = d.rho0, rho_actual = d.rho_actual;
   double x, xp, y, yp, s, dp;
   p.s += d.dzi * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
@@ -1083,7 +1203,7 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   double dist = 0, dz_lost = 0;
   bool lost = false;
   for (int i = 0; i < d.csb_nslices && !lost; i++) {
-    if (!eh_ap_ok(X, Y, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1)) {
+    if (!eh_ap_ok<ELL>(X, Y, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi], d.ell_ye[pi])) {
       lost = true;
       break;
     }
@@ -1151,7 +1271,7 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   }
   if (!lost) {
     dz_lost = d.csb_dzl;
-    if (!eh_ap_ok(X, Y, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1))
+    if (!eh_ap_ok<ELL>(X, Y, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi], d.ell_ye[pi]))
       lost = true;
     else
       S4 += dist;
@@ -1220,7 +1340,7 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
 }
 
 EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms) {
-  return d.rad ? eh_csbend_t<true>(p, d, pi, terms) : eh_csbend_t<false>(p, d, pi, terms);
+  return d.rad ? eh_csbend_t<true, true>(p, d, pi, terms) : eh_csbend_t<false, true>(p, d, pi, terms);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1367,6 +1487,10 @@ EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fter
   const int pi = pass > 0 ? 1 : 0;
   bool lost = false;
   int flag = 0;
+  if constexpr (!RAD) {
+    if (d.ell_any)
+      return EH_STAGE_REFUSED;
+  }
   switch (d.type) {
   case 1: /* DRIF */
     eh_drif(p, d.length, d.T11);
@@ -1422,17 +1546,41 @@ EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fter
     return EH_STAGE_ELEMENT | flag;
   if (d.type == 0)
     return -1; /* MARK: NO_APERTURE, and nothing can be NaN-lost here that survived */
-  if (d.post_ap[pi] && eh_limit_one(p, d.post_xmax[pi], d.post_ymax[pi], d.z_end, d.Po, d.extrap))
-    return EH_STAGE_MAXAMP;
+  if (d.post_ap[pi]) {
+    bool out;
+    if (RAD && d.post_ell[pi])
+      out = eh_elimit_one(p, d.ell_a2[pi], d.ell_b2[pi], d.ell_xe[pi], d.ell_ye[pi], d.z_end, d.Po, d.extrap);
+    else
+      out = eh_limit_one(p, d.post_xmax[pi], d.post_ymax[pi], d.z_end, d.Po, d.extrap);
+    if (out)
+      return EH_STAGE_MAXAMP;
+  }
   if (eh_limit_one(p, EH_DBL_MAX, EH_DBL_MAX, d.z_end, d.Po, 0))
     return EH_STAGE_NAN;
   return -1;
 }
 
+/* where and how a particle was lost inside a fused span, in one word: element
+ * ee of pass p0 + dpass (nelem elements per pass) at stage st (EH_STAGE_*; the
+ * element, sweep and NaN stages and the inherit flag are kept).  The host keeps
+ * (dpass + 1) * nelem * 8 below 2^31. */
+EH_HD int32_t eh_loss_code(int dpass, int nelem, int ee, int st) {
+  return ((dpass * nelem + ee) << 3) | (st & (3 | EH_STAGE_INHERIT));
+}
+
+/* the loss pass and loss event of a loss code: events are
+ * event_base + (dpass * nelem + ee) * 3 + stage, in the order the reference
+ * compacts; the pass carries EH_LPASS_INHERIT for an inherited loss position */
+EH_HD void eh_loss_words(int32_t lcode, int nelem, int p0, int32_t event_base, int32_t &lpass, int32_t &levent) {
+  const int32_t k = lcode >> 3;
+  lpass = (p0 + k / nelem) | ((lcode & EH_STAGE_INHERIT) ? EH_LPASS_INHERIT : 0);
+  levent = event_base + k * 3 + (lcode & 3);
+}
+
 /* whether a descriptor can run through the lean map (RAD = false): kernels for
  * lines without radiating elements and without MULT, ROTATE, MALIGN or ENERGY
  * carry none of that code (and none of its registers) */
-EH_HD bool eh_lean_capable(const eh_desc &d) { return !d.rad && d.type < 10; }
+EH_HD bool eh_lean_capable(const eh_desc &d) { return !d.rad && d.type < 10 && !d.ell_any; }
 
 
 EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms) {

@@ -690,6 +690,20 @@ static long energy_target(const EH_ENERGY *en, double *P_new) {
   return 0;
 }
 
+/* the exponents of an elliptical MAXAMP, checked as elimit_amplitudes does
+ * (src/limit_amplitudes.c:521-536): even and >= 2, YEXPONENT 0 meaning EXPONENT */
+static long ell_exponents(const EH_MAXAMP *m, int *xe, int *ye) {
+  if (m->exponent < 2 || m->exponent % 2)
+    return fail(EH_ERR_ARG, "MAXAMP: EXPONENT=%d is not valid; give an even integer >= 2", (int)m->exponent);
+  *xe = *ye = m->exponent;
+  if (m->y_exponent) {
+    if (m->y_exponent < 2 || m->y_exponent % 2)
+      return fail(EH_ERR_ARG, "MAXAMP: YEXPONENT=%d is not valid; give an even integer >= 2", (int)m->y_exponent);
+    *ye = m->y_exponent;
+  }
+  return EH_OK;
+}
+
 static double elem_length(const EH_ELEMENT_LIST *e) {
   if (e->type == EH_T_MARK || e->type == EH_T_MAXAMP || e->type == EH_T_ROTATE || e->type == EH_T_MALIGN ||
       e->type == EH_T_ENERGY)
@@ -706,13 +720,28 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
   d->z_start = c.z_start;
   d->z_end = c.z_end;
   for (int pi = 0; pi < 2; pi++) {
-    if (c.ap[pi] && c.ap[pi]->elliptical)
-      return fail(EH_ERR_UNSUPPORTED, "elliptical MAXAMP not supported");
-    d->ap_xmax[pi] = c.ap[pi] ? c.ap[pi]->x_max : 0;
-    d->ap_ymax[pi] = c.ap[pi] ? c.ap[pi]->y_max : 0;
+    const EH_MAXAMP *m = c.ap[pi];
+    d->ap_xmax[pi] = m ? m->x_max : 0;
+    d->ap_ymax[pi] = m ? m->y_max : 0;
     d->post_ap[pi] = c.post_on[pi];
     d->post_xmax[pi] = c.post_x[pi];
     d->post_ymax[pi] = c.post_y[pi];
+    if (m && m->elliptical && (m->x_max || m->y_max)) {
+      int xe, ye;
+      long r = ell_exponents(m, &xe, &ye);
+      if (r < 0)
+        return r;
+      /* with a non-positive limit both checks are the rectangular ones
+       * (src/multipole.c:1775, src/limit_amplitudes.c:538-543) */
+      const int both = m->x_max > 0 && m->y_max > 0;
+      d->ap_ell[pi] = both;
+      d->post_ell[pi] = both && c.post_on[pi];
+      d->ell_xe[pi] = xe;
+      d->ell_ye[pi] = ye;
+      d->ell_a2[pi] = eh_ipow(m->x_max, xe);
+      d->ell_b2[pi] = eh_ipow(m->y_max, ye);
+      d->ell_any = d->ell_any || both;
+    }
   }
   d->extrap = e->type == EH_T_DRIF;
   switch (e->type) {
@@ -768,10 +797,6 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
  * store z_start + the particle's own dzLoss without the offset term and flag the
  * entry losses; the download replays each routine call's particle sequence and
  * settles both, so lost rows match the reference bit for bit. */
-/* loss pass word: the pass, with this bit set for a loss that inherits its
- * position (EH_STAGE_INHERIT); cleared before the word reaches the caller */
-#define EH_LPASS_INHERIT 0x40000000
-
 struct eh_kmp {
   int32_t kick;       /* KQUAD/KSEXT/KOCT/KQUSE */
   int32_t has_offset; /* exit offset (dx, dy or dz) */
@@ -912,6 +937,7 @@ struct eh_beam {
 };
 
 static int g_device = -1;
+
 
 extern "C" int eh_init(int device) {
   int n = 0;
@@ -1078,9 +1104,10 @@ __device__ __forceinline__ unsigned wave_prefix(bool pred, unsigned *total) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-/* one element routine (op 0), one limit_amplitudes sweep (op 1) or one
- * removeInvalidParticles sweep (op 2) over the alive particles, in place; losses
- * are marked with (pass, event) */
+/* one element routine (op 0), one limit_amplitudes sweep (op 1), one
+ * removeInvalidParticles sweep (op 2) or one elimit_amplitudes sweep (op 3, its
+ * exponents and a2, b2 in the descriptor) over the alive particles, in place;
+ * losses are marked with (pass, event) */
 __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_plain(eh_soa a, long n, const eh_desc *__restrict__ desc,
                                                      const eh_fterm *__restrict__ terms, int op,
                                                      double xmax, double ymax, double z, double Po, int extrap,
@@ -1132,6 +1159,9 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_plain(eh_soa a, lon
       }
     } else if (op == 1) {
       lost = eh_limit_one(p, xmax, ymax, z, Po, extrap);
+    } else if (op == 3) {
+      const eh_desc &d = *desc;
+      lost = eh_elimit_one(p, d.ell_a2[0], d.ell_b2[0], d.ell_xe[0], d.ell_ye[0], z, Po, extrap);
     } else {
       /* removeInvalidParticles, src/limit_amplitudes.c:289-330 */
       if (eh_isnan(p.x) || eh_isnan(p.xp) || eh_isnan(p.y) || eh_isnan(p.yp) || eh_isnan(p.s) || eh_isnan(p.dp) ||
@@ -1195,27 +1225,28 @@ __global__ void __launch_bounds__(EH_BLOCK) k_compact(eh_soa a, long n, eh_soa b
 }
 
 /* one particle through the element sequence from (p0, e0) up to (p1, e1)
- * exclusive, in registers; `units` counts the lane's particle-element units.  Events are event_base + ((p - p0) * nelem + e) * 3 +
- * stage, in the order the reference compacts.  The loop is wavefront-uniform (the
- * element index is the same in every lane), so descriptors are read with scalar
- * loads; the wavefront leaves early once all its lanes are lost. */
+ * exclusive, in registers.  The loop is wavefront-uniform (the element index is
+ * the same in every lane), so descriptors are read with scalar loads; the
+ * wavefront leaves early once all its lanes are lost.  `units` (wavefront-
+ * uniform) counts the wavefront's particle-element units; a lost lane keeps
+ * where and how in one word (eh_loss_code, decoded by eh_loss_words): a single
+ * loop-carried register. */
 template <bool RAD>
 __device__ __forceinline__ void track_span(eh_p6 &p, bool &alive, const eh_desc *__restrict__ descs,
                                            const eh_fterm *__restrict__ terms, int nelem, int p0, int e0, int p1,
-                                           int e1, int32_t event_base, int32_t &lpass, int32_t &levent,
-                                           unsigned long long &units) {
+                                           int e1, int32_t &lcode, unsigned long long &units) {
   int pp = p0, ee = e0;
   while (pp < p1 || (pp == p1 && ee < e1)) {
-    if (__ballot(alive) == 0)
+    const unsigned long long live = __ballot(alive);
+    if (live == 0)
       break;
     const eh_desc *d = descs + ee;
+    units += (unsigned long long)__popcll(live) * (unsigned)d->counted;
     if (alive) {
-      units += d->counted;
       int st = eh_apply_element_t<RAD>(p, *d, pp, terms);
       if (st >= 0) {
         alive = false;
-        lpass = pp | (st & EH_STAGE_INHERIT ? EH_LPASS_INHERIT : 0);
-        levent = event_base + ((pp - p0) * nelem + ee) * 3 + (st & 3);
+        lcode = eh_loss_code(pp - p0, nelem, ee, st);
       }
     }
     if (++ee == nelem) {
@@ -1270,15 +1301,20 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_fused(eh_soa a, lon
   bool in = i < n;
   bool alive = in;
   eh_p6 p = {0, 0, 0, 0, 0, 0};
-  int32_t row0 = 0;
-  if (in) {
+  if (in)
     p = load_p(a, i);
-    row0 = a.row0[i];
-  }
-  int32_t lpass = 0, levent = -1;
+  int32_t lcode = 0;
   unsigned long long units = 0;
-  track_span<RAD>(p, alive, descs, terms, nelem, p0, e0, p1, e1, event_base, lpass, levent, units);
-  wave_add(&cnt[3], units);
+  track_span<RAD>(p, alive, descs, terms, nelem, p0, e0, p1, e1, lcode, units);
+  if (__lane_id() == 0 && units)
+    atomicAdd(&cnt[3], units);
+  /* identity and loss words are read and formed only now, so that none of them
+   * is carried through the element loop */
+  int32_t row0 = 0, lpass = 0, levent = -1;
+  if (in)
+    row0 = a.row0[i];
+  if (in && !alive)
+    eh_loss_words(lcode, nelem, p0, event_base, lpass, levent);
   wave_store(p, in, alive, row0, lpass, levent, b, 0, &cnt[0], g, &cnt[1]);
 }
 
@@ -1366,12 +1402,9 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_batch(eh_soa a, eh_
     const long i = bd.region_off[v] + local;
     bool alive = in;
     eh_p6 p = {0, 0, 0, 0, 0, 0};
-    int32_t row0 = 0;
-    if (in) {
+    if (in)
       p = load_p(a, i);
-      row0 = a.row0[i];
-    }
-    int32_t lpass = 0, levent = -1;
+    int32_t lcode = 0; /* as in track_span */
     /* descriptors of variant v: pool entries listed by desc_idx (uniform) */
     const int32_t *idx = bd.desc_idx + (long)v * nelem;
     int pp = p0, ee = e0;
@@ -1385,8 +1418,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_batch(eh_soa a, eh_
         int st = eh_apply_element_t<RAD>(p, *d, pp, terms);
         if (st >= 0) {
           alive = false;
-          lpass = pp | (st & EH_STAGE_INHERIT ? EH_LPASS_INHERIT : 0);
-          levent = event_base + ((pp - p0) * nelem + ee) * 3 + (st & 3);
+          lcode = eh_loss_code(pp - p0, nelem, ee, st);
         }
       }
       if (++ee == nelem) {
@@ -1394,6 +1426,11 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_batch(eh_soa a, eh_
         pp++;
       }
     }
+    int32_t row0 = 0, lpass = 0, levent = -1;
+    if (in)
+      row0 = a.row0[i];
+    if (in && !alive)
+      eh_loss_words(lcode, nelem, p0, event_base, lpass, levent);
     wave_store(p, in, alive, row0, lpass, levent, b, bd.region_off[v], &bd.alive_out[v], g, &cnt[1]);
   }
   if (__lane_id() == 0 && units)
@@ -1473,7 +1510,7 @@ static long run_plain(eh_beam *b, const eh_desc *hd, const std::vector<eh_fterm>
   int32_t ev = next_event(b);
   if (n == 0)
     return 0;
-  if (op == 0) {
+  if (op == 0 || op == 3) {
     long r = ensure_desc(b, 1);
     if (r < 0)
       return r;
@@ -1707,6 +1744,41 @@ extern "C" long eh_limit_amplitudes(eh_beam *b, double xmax, double ymax, long n
   if (xmax < 0 && ymax < 0)
     return np;
   return run_plain(b, nullptr, nullptr, 1, xmax, ymax, z, Po, (int)extrapolate_z);
+}
+
+extern "C" long eh_elimit_amplitudes(eh_beam *b, double xmax, double ymax, long np, double z, double Po,
+                                     long extrapolate_z, long openCode, long exponent, long yexponent) {
+  long r = check_np(b, np);
+  if (r < 0)
+    return r;
+  if (openCode)
+    return fail(EH_ERR_UNSUPPORTED, "open-sided apertures not supported");
+  EH_MAXAMP m = {};
+  m.x_max = xmax;
+  m.y_max = ymax;
+  m.elliptical = 1;
+  m.exponent = (int32_t)exponent;
+  m.y_exponent = (int32_t)yexponent;
+  if (exponent != m.exponent || yexponent != m.y_exponent)
+    return fail(EH_ERR_ARG, "MAXAMP exponent out of range");
+  int xe, ye;
+  r = ell_exponents(&m, &xe, &ye);
+  if (r < 0)
+    return r;
+  if (xmax <= 0 || ymax <= 0) {
+    /* src/limit_amplitudes.c:538-543 */
+    if (xmax > 0 || ymax > 0)
+      return eh_limit_amplitudes(b, xmax, ymax, np, z, Po, extrapolate_z, 0);
+    return np;
+  }
+  eh_desc d;
+  std::memset(&d, 0, sizeof d);
+  d.type = EH_T_MAXAMP;
+  d.ell_a2[0] = eh_ipow(xmax, xe);
+  d.ell_b2[0] = eh_ipow(ymax, ye);
+  d.ell_xe[0] = xe;
+  d.ell_ye[0] = ye;
+  return run_plain(b, &d, nullptr, 3, xmax, ymax, z, Po, (int)extrapolate_z);
 }
 
 /* Kahan-compensated TMEAN fiducial over the alive particles in reference row
@@ -1996,6 +2068,7 @@ static long track_plain(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
   long nToTrack = b->n_alive, nLeft;
   const EH_MAXAMP *maxamp = nullptr;
   double x_max = 0, y_max = 0;
+  long elliptical = 0, maxampExponent = 0, maxampYExponent = 0;
   b->pass = 0;
   nLeft = eh_limit_amplitudes(b, EH_DBL_MAX, EH_DBL_MAX, nToTrack, 0.0, *Pc, 0, 0);
   if (nLeft < 0)
@@ -2018,10 +2091,11 @@ static long track_plain(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
         break;
       case EH_T_MAXAMP:
         maxamp = (const EH_MAXAMP *)e->p_elem;
-        if (maxamp->elliptical)
-          return fail(EH_ERR_UNSUPPORTED, "elliptical MAXAMP not supported");
         x_max = maxamp->x_max;
         y_max = maxamp->y_max;
+        elliptical = maxamp->elliptical;
+        maxampExponent = maxamp->exponent;
+        maxampYExponent = maxamp->y_exponent;
         break;
       case EH_T_DRIF: {
         const EH_DRIFT *d = (const EH_DRIFT *)e->p_elem;
@@ -2079,7 +2153,12 @@ static long track_plain(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       if (nLeft < 0)
         return nLeft;
       if (e->type != EH_T_MARK && (x_max || y_max)) {
-        nLeft = eh_limit_amplitudes(b, x_max, y_max, nLeft, z, *Pc, e->type == EH_T_DRIF, 0);
+        /* src/do_tracking.c:2508-2517 */
+        if (!elliptical)
+          nLeft = eh_limit_amplitudes(b, x_max, y_max, nLeft, z, *Pc, e->type == EH_T_DRIF, 0);
+        else
+          nLeft = eh_elimit_amplitudes(b, x_max, y_max, nLeft, z, *Pc, e->type == EH_T_DRIF, 0, maxampExponent,
+                                       maxampYExponent);
         if (nLeft < 0)
           return nLeft;
       }
@@ -2098,6 +2177,8 @@ static long launch_fused(eh_beam *b, long ne, int p0, int e0, int p1, int e1, do
   long span = ((long)(p1 - p0) * ne + ne) * 3 + 3;
   if ((long)b->event + span >= 2147483647L)
     return fail(EH_ERR_STATE, "loss event counter overflow");
+  if (((long)(p1 - p0) * ne + ne) * 8 >= 2147483647L)
+    return fail(EH_ERR_ARG, "span of %ld passes x %ld elements too long for one launch", (long)(p1 - p0 + 1), ne);
   int32_t base = b->event;
   b->event += (int32_t)span;
   if (n > 0 && tab) {
@@ -2204,8 +2285,6 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       c.post_x[pi] = m ? m->x_max : 0;
       c.post_y[pi] = m ? m->y_max : 0;
       c.post_on[pi] = (bl[ie].type != EH_T_MARK) && m && (m->x_max || m->y_max);
-      if (m && m->elliptical)
-        return fail(EH_ERR_UNSUPPORTED, "elliptical MAXAMP not supported");
     }
     return build_desc(&hd[ie], bl + ie, c, pool);
   };
@@ -2765,6 +2844,8 @@ static long batch_enqueue_span(eh_batch *b, long ne, int p0, int e0, int p1, int
   long span = ((long)(p1 - p0) * ne + ne) * 3 + 3;
   if ((long)b->event + span >= 2147483647L)
     return fail(EH_ERR_STATE, "loss event counter overflow");
+  if (((long)(p1 - p0) * ne + ne) * 8 >= 2147483647L)
+    return fail(EH_ERR_ARG, "span of %ld passes x %ld elements too long for one launch", (long)(p1 - p0 + 1), ne);
   int32_t base = b->event;
   b->event += (int32_t)span;
   if (b->kmp_pool) {
@@ -3479,8 +3560,8 @@ static long build_line(EH_ELEMENT_LIST *bl, long ne, double Pc, std::vector<eh_d
       c.post_x[pi] = m ? m->x_max : 0;
       c.post_y[pi] = m ? m->y_max : 0;
       c.post_on[pi] = (bl[ie].type != EH_T_MARK) && m && (m->x_max || m->y_max);
-      if (m && m->elliptical)
-        return fail(EH_ERR_UNSUPPORTED, "elliptical MAXAMP not supported");
+      if (m && m->elliptical && m->x_max > 0 && m->y_max > 0)
+        return fail(EH_ERR_UNSUPPORTED, "the adjoint does not support elliptical MAXAMP");
     }
     long r = build_desc(&hd[ie], bl + ie, c, pool);
     if (r < 0)

@@ -117,7 +117,8 @@ class EH_ENERGY(C.Structure):
 
 
 class EH_MAXAMP(C.Structure):
-    _fields_ = [("x_max", dbl), ("y_max", dbl), ("elliptical", i32), ("exponent", i32)]
+    _fields_ = [("x_max", dbl), ("y_max", dbl), ("elliptical", i32), ("exponent", i32), ("y_exponent", i32),
+                ("pad_", i32)]
 
 
 class EH_ELEMENT_LIST(C.Structure):
@@ -180,7 +181,7 @@ ALIASES = {
     T_MALIGN: {"ON_PASS": "on_pass", "START_PID": "startPID", "END_PID": "endPID"},
     T_ENERGY: {"CENTRAL_ENERGY": "central_energy", "CENTRAL_MOMENTUM": "central_momentum",
                "MATCH_BEAMLINE": "match_beamline", "MATCH_PARTICLES": "match_particles"},
-    T_MAXAMP: {},
+    T_MAXAMP: {"YEXPONENT": "y_exponent"},
     T_DRIF: {},
     T_EDRIFT: {},
 }

@@ -194,10 +194,14 @@ typedef struct {
   double phase_fiducial, t_fiducial;
 } EH_RFCA;
 
-/* MAXAMP: src/track_data.c:994 (rectangular, no open side) */
+/* MAXAMP: src/track_data.c:994 (no open side).  ELLIPTICAL selects the
+ * (super-)elliptical boundary |x/x_max|^EXPONENT + |y/y_max|^YEXPONENT = 1
+ * (src/limit_amplitudes.c:487-620, src/multipole.c:1784-1792); y_exponent 0
+ * means EXPONENT.  Exponents must be even and >= 2 (EH_ERR_ARG otherwise). */
 typedef struct {
   double x_max, y_max;
   int32_t elliptical, exponent;
+  int32_t y_exponent, pad_;
 } EH_MAXAMP;
 
 /* a first- or second-order transport map as track_particles applies it
@@ -315,6 +319,13 @@ long eh_track_particles(eh_beam *beam, long n_part, const EH_VMATRIX *M);
 /* replaces limit_amplitudes, src/limit_amplitudes.c:197-199 (openCode must be 0) */
 long eh_limit_amplitudes(eh_beam *beam, double xmax, double ymax, long np, double z, double Po, long extrapolate_z,
                          long openCode);
+
+/* replaces elimit_amplitudes, src/limit_amplitudes.c:498-500 (openCode must be
+ * 0): the (super-)elliptical sweep |x/xmax|^exponent + |y/ymax|^yexponent > 1
+ * (yexponent 0 means exponent; both even and >= 2, else EH_ERR_ARG), a lost
+ * particle's dp becoming sqrt((Po (1 + dp))^2 + 1) */
+long eh_elimit_amplitudes(eh_beam *beam, double xmax, double ymax, long np, double z, double Po, long extrapolate_z,
+                          long openCode, long exponent, long yexponent);
 
 /* replaces removeInvalidParticles, src/limit_amplitudes.c:289-291 */
 long eh_removeInvalidParticles(eh_beam *beam, long np, double z, double Po);

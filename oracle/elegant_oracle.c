@@ -158,6 +158,120 @@ long or_limit_amplitudes(double **coord, double xmax, double ymax, long np, doub
   return np;
 }
 
+/* ipow of the SDDS toolkit's mdbmth library (ipow.c), which elegant links and
+ * calls for the elliptical apertures; the library is not part of the reference
+ * sources.  Restated from its published algorithm for p >= 0: exponents up to 8
+ * by fixed products of x^2 (x^4 = x^2 x^2, x^6 = x^2 x^2 x^2, x^7 = x^3 x^3 x,
+ * x^8 = (x^2 x^2)(x^2 x^2)), larger ones by halving, x^p = h*h or h*h*x with
+ * h = x^(p/2).  x == 0 gives 1 for p == 0, else 0. */
+double or_ipow(double x, long p) {
+  double hp;
+  long n;
+  if (x == 0)
+    return p == 0 ? 1.0 : 0.0;
+  switch (p) {
+  case 0:
+    return 1.0;
+  case 1:
+    return x;
+  case 2:
+    return x * x;
+  case 3:
+    hp = x * x;
+    return hp * x;
+  case 4:
+    hp = x * x;
+    return hp * hp;
+  case 5:
+    hp = x * x;
+    return hp * hp * x;
+  case 6:
+    hp = x * x;
+    return hp * hp * hp;
+  case 7:
+    hp = x * x * x;
+    return hp * hp * x;
+  case 8:
+    hp = x * x;
+    hp = hp * hp;
+    return hp * hp;
+  default:
+    n = p / 2;
+    hp = or_ipow(x, n);
+    return (p - 2 * n) ? hp * hp * x : hp * hp;
+  }
+}
+
+/* the exponent checks of elimit_amplitudes, src/limit_amplitudes.c:521-536:
+ * 0 with the x and y exponents, or the error code */
+static long or_ell_exponents(long exponent, long yexponent, long *xe, long *ye) {
+  if ((*xe = exponent) < 2 || *xe % 2)
+    return or_fail("MAXAMP EXPONENT must be an even integer >= 2");
+  *ye = *xe;
+  if (yexponent && ((*ye = yexponent) < 2 || *ye % 2))
+    return or_fail("MAXAMP YEXPONENT must be an even integer >= 2");
+  return 0;
+}
+
+/* elimit_amplitudes, src/limit_amplitudes.c:498-617 (openCode 0, no global loss
+ * coordinates): non-finite particles, then particles with
+ * x^xe/a2 + y^ye/b2 > 1, are moved to the top; a lost particle's dp becomes
+ * sqrt((Po(1+dp))^2 + 1) and its s the loss position, extrapolated back along
+ * a straight line for drifts with a true ellipse */
+long or_elimit_amplitudes(double **coord, double xmax, double ymax, long np, double z, double Po, long extrapolate_z,
+                          long exponent, long yexponent) {
+  long ip, itop, xe, ye, r;
+  double *part;
+  double a2, b2, c1, c2, c0, dz, det;
+
+  if ((r = or_ell_exponents(exponent, yexponent, &xe, &ye)) < 0)
+    return r;
+  if (xmax <= 0 || ymax <= 0) {
+    if (xmax > 0 || ymax > 0)
+      return or_limit_amplitudes(coord, xmax, ymax, np, z, Po, extrapolate_z);
+    return np;
+  }
+  a2 = or_ipow(xmax, xe);
+  b2 = or_ipow(ymax, ye);
+  itop = np - 1;
+  for (ip = 0; ip < np; ip++) {
+    part = coord[ip];
+    if (isinf(part[0]) || isinf(part[2]) || isnan(part[0]) || isnan(part[2])) {
+      swapParticles(coord[ip], coord[itop]);
+      coord[itop][4] = z;
+      coord[itop][5] = sqrt(sqr(Po * (1 + coord[itop][5])) + 1);
+      --itop;
+      --ip;
+      np--;
+      continue;
+    }
+    if (!((or_ipow(part[0], xe) / a2 + or_ipow(part[2], ye) / b2) > 1))
+      continue;
+    dz = 0;
+    if (extrapolate_z && xe == 2 && ye == 2) {
+      c0 = sqr(part[0]) / a2 + sqr(part[2]) / b2 - 1;
+      c1 = 2 * (part[0] * part[1] / a2 + part[2] * part[3] / b2);
+      c2 = sqr(part[1]) / a2 + sqr(part[3]) / b2;
+      det = sqr(c1) - 4 * c0 * c2;
+      if (z > 0 && c2 && det >= 0) {
+        if ((dz = (-c1 + sqrt(det)) / (2 * c2)) > 0)
+          dz = (-c1 - sqrt(det)) / (2 * c2);
+        if ((z + dz) < 0)
+          dz = -z;
+        part[0] += dz * part[1];
+        part[2] += dz * part[3];
+      }
+    }
+    swapParticles(coord[ip], coord[itop]);
+    coord[itop][4] = z + dz;
+    coord[itop][5] = sqrt(sqr(Po * (1 + coord[itop][5])) + 1);
+    --itop;
+    --ip;
+    np--;
+  }
+  return np;
+}
+
 /* removeInvalidParticles, src/limit_amplitudes.c:289-330 */
 long or_removeInvalidParticles(double **coord, long np, double z, double Po) {
   long ip, itop, is_out, ic;
@@ -325,9 +439,10 @@ static void rotateBeamCoordinatesForMisalignment(double **part, long np, double 
 
 /* ------------------------------------------------------------------------- */
 /* aperture data for in-element checks: setupMultApertureData / checkMultAperture,
- * src/multipole.c:1690-1798, rectangular MAXAMP only */
+ * src/multipole.c:1690-1798, MAXAMP without open sides, contours or files */
 typedef struct {
   double xMax, yMax, reverseTilt, cs0, cs1;
+  long elliptical, xExponent, yExponent;
 } or_apdata;
 
 static void or_setup_ap(or_apdata *ap, double reverseTilt, const EH_MAXAMP *maxamp) {
@@ -335,9 +450,14 @@ static void or_setup_ap(or_apdata *ap, double reverseTilt, const EH_MAXAMP *maxa
   ap->cs0 = OR_COS(reverseTilt);
   ap->cs1 = OR_SIN(reverseTilt);
   ap->xMax = ap->yMax = 0;
+  ap->elliptical = ap->xExponent = ap->yExponent = 0;
   if (maxamp) {
     ap->xMax = maxamp->x_max;
     ap->yMax = maxamp->y_max;
+    ap->elliptical = maxamp->elliptical;
+    ap->xExponent = ap->yExponent = maxamp->exponent;
+    if (maxamp->y_exponent)
+      ap->yExponent = maxamp->y_exponent;
   }
 }
 
@@ -348,8 +468,14 @@ static long or_check_ap(double xInput, double yInput, const or_apdata *ap) {
     x = x0 * ap->cs0 + y0 * ap->cs1;
     y = -x0 * ap->cs1 + y0 * ap->cs0;
   }
-  if ((ap->xMax > 0 && fabs(x) > ap->xMax) || (ap->yMax > 0 && fabs(y) > ap->yMax))
-    return 0;
+  if (ap->elliptical == 0 || ap->xMax <= 0 || ap->yMax <= 0) {
+    if ((ap->xMax > 0 && fabs(x) > ap->xMax) || (ap->yMax > 0 && fabs(y) > ap->yMax))
+      return 0;
+  } else {
+    double xa = x / ap->xMax, yb = y / ap->yMax;
+    if ((or_ipow(xa, ap->xExponent) + or_ipow(yb, ap->yExponent)) >= 1)
+      return 0;
+  }
   return 1;
 }
 
@@ -704,8 +830,6 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
   }
   if (integ_order != 2 && integ_order != 4 && integ_order != 6)
     return or_fail("multipole integration_order must be 2, 4, or 6");
-  if (maxamp && maxamp->elliptical)
-    return or_fail("elliptical MAXAMP not supported");
   /* classical radiation, src/multipole.c:755-763 (and :827-836, :888-897,
    * :938-947): none for elements shorter than 1e-6 m */
   if (synch_rad && drift >= 1e-6)
@@ -1251,8 +1375,6 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
   if ((csbend->edge1_effects != 0 && csbend->edge1_effects != 1) ||
       (csbend->edge2_effects != 0 && csbend->edge2_effects != 1))
     return or_fail("CSBEND edge method not supported");
-  if (maxamp && maxamp->elliptical)
-    return or_fail("elliptical MAXAMP not supported");
 
   rho0 = csbend->length / csbend->angle;
   b[0] = 0;
@@ -1719,6 +1841,7 @@ long or_do_tracking(double **coord, long np, const EH_ELEMENT_LIST *beamline, lo
   long nToTrack = np, nLeft, i_pass, ie;
   const EH_MAXAMP *maxamp = NULL;
   double x_max = 0, y_max = 0, z, last_z, z_travel;
+  long elliptical = 0, maxampExponent = 0, maxampYExponent = 0;
 
   if (effort)
     *effort = 0;
@@ -1743,10 +1866,11 @@ long or_do_tracking(double **coord, long np, const EH_ELEMENT_LIST *beamline, lo
         break;
       case EH_T_MAXAMP:
         maxamp = (const EH_MAXAMP *)eptr->p_elem;
-        if (maxamp->elliptical)
-          return or_fail("elliptical MAXAMP not supported");
         x_max = maxamp->x_max;
         y_max = maxamp->y_max;
+        elliptical = maxamp->elliptical;
+        maxampExponent = maxamp->exponent;
+        maxampYExponent = maxamp->y_exponent;
         break;
       case EH_T_DRIF: {
         const EH_DRIFT *d = (const EH_DRIFT *)eptr->p_elem;
@@ -1792,8 +1916,15 @@ long or_do_tracking(double **coord, long np, const EH_ELEMENT_LIST *beamline, lo
       if (nLeft < 0)
         return nLeft;
       if (eptr->type != EH_T_MARK) {
-        if (x_max || y_max)
-          nLeft = or_limit_amplitudes(coord, x_max, y_max, nLeft, z, *P_central, eptr->type == EH_T_DRIF);
+        if (x_max || y_max) {
+          if (!elliptical)
+            nLeft = or_limit_amplitudes(coord, x_max, y_max, nLeft, z, *P_central, eptr->type == EH_T_DRIF);
+          else
+            nLeft = or_elimit_amplitudes(coord, x_max, y_max, nLeft, z, *P_central, eptr->type == EH_T_DRIF,
+                                         maxampExponent, maxampYExponent);
+          if (nLeft < 0)
+            return nLeft;
+        }
       }
       nLeft = or_limit_amplitudes(coord, DBL_MAX, DBL_MAX, nLeft, z, *P_central, 0);
       if (nLeft != nToTrack)
@@ -1868,6 +1999,14 @@ long or_element_flat(double *flat, long np, const EH_ELEMENT_LIST *eptr, double 
 long or_limit_amplitudes_flat(double *flat, double xmax, double ymax, long np, double z, double Po, long extrapolate_z) {
   double **rows = or_rows(flat, np);
   long r = or_limit_amplitudes(rows, xmax, ymax, np, z, Po, extrapolate_z);
+  free(rows);
+  return r;
+}
+
+long or_elimit_amplitudes_flat(double *flat, double xmax, double ymax, long np, double z, double Po,
+                               long extrapolate_z, long exponent, long yexponent) {
+  double **rows = or_rows(flat, np);
+  long r = or_elimit_amplitudes(rows, xmax, ymax, np, z, Po, extrapolate_z, exponent, yexponent);
   free(rows);
   return r;
 }

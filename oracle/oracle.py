@@ -48,6 +48,11 @@ def load_path(path):
     lib.or_element_flat.restype = C.c_long
     lib.or_limit_amplitudes_flat.argtypes = [vp, C.c_double, C.c_double, C.c_long, C.c_double, C.c_double, C.c_long]
     lib.or_limit_amplitudes_flat.restype = C.c_long
+    lib.or_elimit_amplitudes_flat.argtypes = [vp, C.c_double, C.c_double, C.c_long, C.c_double, C.c_double, C.c_long,
+                                              C.c_long, C.c_long]
+    lib.or_elimit_amplitudes_flat.restype = C.c_long
+    lib.or_ipow.argtypes = [C.c_double, C.c_long]
+    lib.or_ipow.restype = C.c_double
     lib.or_csbend_field_table.argtypes = [dp, dp, C.c_double, C.c_int, C.c_int, dp, dp]
     lib.or_csbend_field_table.restype = C.c_long
     lib.or_last_error.restype = C.c_char_p

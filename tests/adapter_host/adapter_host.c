@@ -13,8 +13,9 @@ struct eh_beam {
   double rows[64][EH_NPROP];
 };
 static struct {
-  long uploads, downloads, mp2, csb, rf, drift, tp, lim, rip, off, scm, pass_set;
+  long uploads, downloads, mp2, csb, rf, drift, tp, lim, elim, rip, off, scm, pass_set;
   long last_pass;
+  EH_MAXAMP last_amp; /* the aperture of the last element call */
   int perm[64]; /* download row r <- uploaded row perm[r] */
   long lose;    /* particles each tracking call loses */
 } M;
@@ -71,8 +72,10 @@ static long track_stand_in(eh_beam *b, long n) {
 }
 long eh_multipole_tracking2(eh_beam *b, long n, const EH_ELEMENT_LIST *e, double pe, double Po, double z,
                             const EH_MAXAMP *a, long iSlice) {
-  (void)e, (void)pe, (void)Po, (void)z, (void)a, (void)iSlice;
+  (void)e, (void)pe, (void)Po, (void)z, (void)iSlice;
   M.mp2++;
+  if (a)
+    M.last_amp = *a;
   return track_stand_in(b, n);
 }
 long eh_track_through_csbend(eh_beam *b, long n, const EH_CSBEND *c, double pe, double Po, double z,
@@ -104,6 +107,16 @@ long eh_limit_amplitudes(eh_beam *b, double x, double y, long n, double z, doubl
   if (oc)
     return EH_ERR_UNSUPPORTED;
   M.lim++;
+  return track_stand_in(b, n);
+}
+long eh_elimit_amplitudes(eh_beam *b, double x, double y, long n, double z, double Po, long ex, long oc, long xe,
+                          long ye) {
+  (void)x, (void)y, (void)z, (void)Po, (void)ex, (void)ye;
+  if (oc)
+    return EH_ERR_UNSUPPORTED;
+  if (xe < 2 || xe % 2)
+    return EH_ERR_ARG;
+  M.elim++;
   return track_stand_in(b, n);
 }
 long eh_removeInvalidParticles(eh_beam *b, long n, double z, double Po) {
@@ -342,15 +355,32 @@ int ah_scenario_host_fallback(void) {
     memset(&amp, 0, sizeof amp);
     amp.x_max = amp.y_max = 0.02;
     amp.elliptical = 1;
+    amp.exponent = 4;
+    amp.yExponent = 2;
+    n = gpu_multipole_tracking2(8, &el_kq, 0.0, 5000.0, accp_h, 0.0, &amp, NULL, NULL, NULL, -1);
+    if (M.mp2 != 3 || E.mp2 != 2 || M.last_amp.elliptical != 1 || M.last_amp.exponent != 4 ||
+        M.last_amp.y_exponent != 2)
+      return 5;
+    /* an open side stays on the host */
+    amp.openSide = (char *)"+x";
     n = gpu_multipole_tracking2(8, &el_kq, 0.0, 5000.0, accp_h, 0.0, &amp, NULL, NULL, NULL, -1);
     if (E.mp2 != 3)
-      return 5;
+      return 51;
+    amp.openSide = NULL;
+    /* the elliptical sweep runs on the device; open sides and exponents the
+     * library rejects reach elegant's routine */
     n = gpu_elimit_amplitudes(0.02, 0.02, 8, accp_h, 0.0, 5000.0, 0, 0, 2, 2, NULL);
-    if (E.elim != 1)
+    if (M.elim != 1 || E.elim != 0)
       return 6;
+    n = gpu_elimit_amplitudes(0.02, 0.02, 8, accp_h, 0.0, 5000.0, 0, 1, 2, 2, NULL);
+    if (E.elim != 1)
+      return 61;
+    n = gpu_elimit_amplitudes(0.02, 0.02, 8, accp_h, 0.0, 5000.0, 0, 0, 3, 0, NULL);
+    if (E.elim != 2 || M.elim != 1)
+      return 62;
     amp.elliptical = 0;
     n = gpu_multipole_tracking2(8, &el_kq, 0.0, 5000.0, accp_h, 0.0, &amp, NULL, NULL, NULL, -1);
-    if (M.mp2 != 3)
+    if (M.mp2 != 4)
       return 7;
   }
   /* a third-order matrix stays on the host; order 2 runs on the device */

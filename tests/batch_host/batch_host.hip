@@ -36,14 +36,28 @@ static void plant_unshift(std::vector<eh_desc> &pool) {
     }
 }
 
-/* the lean map's answer for a descriptor of the given type */
-extern "C" long bh_lean_stage(int type) {
+/* the lean map's answer for a descriptor of the given type (with an elliptical
+ * aperture if ell), and whether the host would send it to the lean map */
+extern "C" long bh_lean_stage(int type, int ell, int *capable) {
   eh_desc d;
   std::memset(&d, 0, sizeof d);
   d.type = type;
   d.adj_pk = d.adj_pdx = d.adj_pdy = d.adj_anchor = -1;
+  if (ell) {
+    d.ap_ell[0] = d.ap_ell[1] = d.post_ell[0] = d.post_ell[1] = d.ell_any = 1;
+    d.ap_xmax[0] = d.ap_xmax[1] = d.ap_ymax[0] = d.ap_ymax[1] = 1.0;
+    d.ell_xe[0] = d.ell_xe[1] = d.ell_ye[0] = d.ell_ye[1] = 2;
+  }
+  *capable = eh_lean_capable(d);
   eh_p6 p = {1e-3, 0, 0, 0, 0, 0};
   return eh_apply_element_t<false>(p, d, 0, nullptr);
+}
+
+/* the loss code of track_span and k_batch decoded back (host check of the
+ * encoding) */
+extern "C" void bh_loss_words(int dpass, int nelem, int ee, int st, int p0, int32_t event_base, int32_t *lpass,
+                              int32_t *levent) {
+  eh_loss_words(eh_loss_code(dpass, nelem, ee, st), nelem, p0, event_base, *lpass, *levent);
 }
 
 /* a pool descriptor equals a descriptor built alone for the same position:
@@ -171,15 +185,15 @@ static void bh_span(const bh_soa &a, bh_soa &out, bh_soa &grave, long &n_grave, 
       const long i = region_off[v] + local;
       eh_p6 p = {a.c[0][i], a.c[1][i], a.c[2][i], a.c[3][i], a.c[4][i], a.c[5][i]};
       bool live = true;
-      int32_t lpass = 0, levent = -1;
+      int32_t lpass = 0, levent = -1, lcode = 0;
       int pp = p0, ee = e0;
       while (live && (pp < p1 || (pp == p1 && ee < e1))) {
         const eh_desc &d = pool[idx[(size_t)v * nelem + ee]];
         int st = eh_apply_element_t<true>(p, d, pp, terms.data());
         if (st >= 0) {
           live = false;
-          lpass = pp | (st & EH_STAGE_INHERIT ? EH_LPASS_INHERIT : 0);
-          levent = event_base + ((pp - p0) * nelem + ee) * 3 + (st & 3);
+          lcode = eh_loss_code(pp - p0, nelem, ee, st);
+          eh_loss_words(lcode, nelem, p0, event_base, lpass, levent);
         }
         if (++ee == nelem) {
           ee = 0;

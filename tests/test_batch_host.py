@@ -31,8 +31,9 @@ def harness():
         vp, L = C.c_void_p, C.c_long
         _h.bh_worklist.argtypes = [vp, C.c_int, vp, vp]
         _h.bh_worklist.restype = L
-        _h.bh_lean_stage.argtypes = [C.c_int]
+        _h.bh_lean_stage.argtypes = [C.c_int, C.c_int, vp]
         _h.bh_lean_stage.restype = L
+        _h.bh_loss_words.argtypes = [C.c_int] * 5 + [C.c_int32, vp, vp]
         _h.bh_tables.argtypes = [vp, L, L, C.c_double, vp, vp, vp, vp]
         _h.bh_tables.restype = L
         _h.bh_track.argtypes = [vp, vp, L, vp, L, L, C.c_double, L, C.c_uint, vp, vp]
@@ -74,12 +75,43 @@ def test_worklist_covers_every_particle_once(counts):
         assert (seen[v] == 1).all(), "variant %d: particles not covered exactly once" % v
 
 
+def _lean(t, ell=0):
+    cap = C.c_int(-1)
+    st = harness().bh_lean_stage(t, ell, C.byref(cap))
+    return st, cap.value
+
+
 def test_lean_map_refuses_full_only_types():
-    h = harness()
     for t in (S.T_MULT, S.T_ROTATE, S.T_MALIGN, S.T_ENERGY):
-        assert h.bh_lean_stage(t) == 8, S.TYPE_NAMES[t]
+        assert _lean(t) == (8, 0), S.TYPE_NAMES[t]
     for t in (S.T_MARK, S.T_MAXAMP):
-        assert h.bh_lean_stage(t) == -1
+        assert _lean(t) == (-1, 1)
+    # a descriptor with an elliptical aperture: refused, and never sent there
+    for t in (S.T_DRIF, S.T_KQUAD, S.T_CSBEND, S.T_MARK):
+        assert _lean(t, 1) == (8, 0), S.TYPE_NAMES[t]
+
+
+def test_loss_code_round_trip():
+    """the single loss word the fused kernels carry decodes to the pass and
+    event the reference order needs: event_base + (dpass * nelem + e) * 3 + stage"""
+    lp, le = C.c_int32(), C.c_int32()
+    for dpass, nelem, ee, st, p0, base in ((0, 2082, 0, 0, 0, 0), (3, 2082, 2081, 2, 17, 1000),
+                                          (15, 7, 6, 1 | 4, 3, 5), (1, 1, 0, 8, 0, 0)):
+        harness().bh_loss_words(dpass, nelem, ee, st, p0, base, C.byref(lp), C.byref(le))
+        assert lp.value == (p0 + dpass) | (0x40000000 if st & 4 else 0)
+        assert le.value == base + (dpass * nelem + ee) * 3 + (st & 3)
+
+
+def test_emulated_batch_elliptical_apertures():
+    """variants behind elliptical MAXAMPs (exponents 2, 4 and 2 with YEXPONENT
+    4): the full map's elliptical checks and sweeps against the oracle"""
+    from tests.test_gpu_elliptical import _beam, elliptical_ring
+    exps = ((2, 0), (4, 0), (2, 4))
+    Po = configs.p_central_from_mev(1500.0)
+    beams = [_beam(300 + 50 * v, seed=60 + v, xs=0.016, ys=0.009) for v in range(3)]
+    r, out6, lpass, cnt = _host_track(beams, [elliptical_ring(*e) for e in exps], 3, Po, None, 1)
+    total = _compare_with_oracle(beams, lambda: [elliptical_ring(*e) for e in exps], 3, Po, out6, lpass, cnt)
+    assert r == total and 0 < total < sum(len(b) for b in beams)
 
 
 def _rf_line(volt=2.0e6, shared=False, two=False, mode=0):
