@@ -24,7 +24,7 @@ EXPORTS = [
     "eh_batch_download", "eh_batch_n_alive", "eh_batch_track", "eh_diag_copy", "eh_adjoint_create",
     "eh_adjoint_free", "eh_adjoint_forward", "eh_adjoint_backward", "eh_adjoint_forward_batch",
     "eh_multipole_tracking", "eh_offset_beam", "eh_rotate_beam", "eh_replay_sequence", "eh_set_central_momentum",
-    "eh_track_particles", "eh_beam_row_order", "eh_elimit_amplitudes",
+    "eh_track_particles", "eh_beam_row_order", "eh_do_tracking_noalloc", "eh_elimit_amplitudes",
 ]
 
 
@@ -94,6 +94,8 @@ def load():
     lib.eh_removeInvalidParticles.restype = L
     lib.eh_do_tracking.argtypes = [vp, elp, L, L, dp, I, I, C.POINTER(S.eh_track_stats)]
     lib.eh_do_tracking.restype = L
+    lib.eh_do_tracking_noalloc.argtypes = [vp, elp, L, L, dp, I, C.POINTER(S.eh_track_stats)]
+    lib.eh_do_tracking_noalloc.restype = L
     lib.eh_csbend_field_table.argtypes = [dp, dp, D, I, I, dp, dp, C.POINTER(I), C.POINTER(I)]
     lib.eh_csbend_field_table.restype = L
     lib.eh_replay_order.argtypes = [L, vp, vp, L, vp]

@@ -934,10 +934,26 @@ struct eh_beam {
   double main_ms = 0, main_units = 0;
   long main_launches = 0;
   std::vector<eh_evrec> evlog; /* launches whose losses need settling */
+  bool noalloc = false;         /* descriptors, terms, counters in the static scratch */
 };
 
 static int g_device = -1;
 
+/* The static scratch of eh_do_tracking_noalloc: per GPU, 8 MiB holding the
+ * launch counters (first 256 B), up to EH_SCRATCH_NDESC element descriptors and
+ * up to EH_SCRATCH_NTERMS CSBEND field terms.  In host code `scratch` names a
+ * host-side stand-in, not device memory: its device address on each GPU is
+ * taken once with hipGetSymbolAddress (eh_init) and kept in g_scratch. */
+#define EH_SCRATCH_BYTES (8l << 20)
+#define EH_SCRATCH_NDESC 4096l
+#define EH_SCRATCH_DESC_OFF 256l
+#define EH_SCRATCH_TERMS_OFF (EH_SCRATCH_DESC_OFF + EH_SCRATCH_NDESC * (long)sizeof(eh_desc))
+#define EH_SCRATCH_NTERMS ((EH_SCRATCH_BYTES - EH_SCRATCH_TERMS_OFF) / (long)sizeof(eh_fterm))
+/* field terms of one CSBEND at most: both sparse lists and both dense blocks */
+#define EH_MAX_TERMS_PER_BEND (2l * EH_MAXFTERMS + 2l * 36l)
+static __device__ __align__(256) unsigned char scratch[EH_SCRATCH_BYTES];
+#define EH_MAX_DEVICES 64
+static void *g_scratch[EH_MAX_DEVICES];
 
 extern "C" int eh_init(int device) {
   int n = 0;
@@ -947,6 +963,12 @@ extern "C" int eh_init(int device) {
     return (int)fail(EH_ERR_ARG, "device %d out of range (%d devices)", device, n);
   if (hipSetDevice(device) != hipSuccess)
     return (int)fail(EH_ERR_HIP, "hipSetDevice(%d) failed", device);
+  if (device < EH_MAX_DEVICES && !g_scratch[device]) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+      return (int)fail(EH_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d", device);
+    g_scratch[device] = ptr;
+  }
   g_device = device;
   return EH_OK;
 }
@@ -1442,6 +1464,8 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_batch(eh_soa a, eh_
 static long ensure_desc(eh_beam *b, long n) {
   if (b->desc_cap >= n)
     return EH_OK;
+  if (b->noalloc)
+    return fail(EH_ERR_ARG, "%ld element descriptors exceed the static scratch (%ld)", n, b->desc_cap);
   if (b->d_desc)
     HIPCHK(hipFree(b->d_desc));
   b->d_desc = nullptr;
@@ -1454,6 +1478,8 @@ static long upload_terms(eh_beam *b, const std::vector<eh_fterm> &pool) {
   if (pool.empty())
     return EH_OK;
   if (b->terms_cap < (long)pool.size()) {
+    if (b->noalloc)
+      return fail(EH_ERR_ARG, "%zu field terms exceed the static scratch (%ld)", pool.size(), b->terms_cap);
     if (b->d_terms)
       HIPCHK(hipFree(b->d_terms));
     b->d_terms = nullptr;
@@ -2401,6 +2427,47 @@ extern "C" long eh_do_tracking(eh_beam *b, EH_ELEMENT_LIST *beamline, long n_ele
     stats->main_launches = b->main_launches - ml0;
     stats->main_units = b->main_units - mu0;
   }
+  return r;
+}
+
+/* eh_do_tracking in fused mode with every device buffer beyond the beam (the
+ * element descriptors, the CSBEND field terms, the launch counters) in the
+ * static scratch of the current GPU: no allocation.  Lines larger than the
+ * scratch are refused before anything runs.  Calls that overlap on one GPU
+ * share the scratch and race. */
+extern "C" long eh_do_tracking_noalloc(eh_beam *b, EH_ELEMENT_LIST *beamline, long n_elements, long n_passes,
+                                       double *P_central, int turns_per_launch, eh_track_stats *stats) {
+  if (!b || !beamline || !P_central || n_elements < 0 || n_passes < 0)
+    return fail(EH_ERR_ARG, "eh_do_tracking_noalloc: bad arguments");
+  if (g_device < 0 || g_device >= EH_MAX_DEVICES || !g_scratch[g_device])
+    return fail(EH_ERR_STATE, "eh_do_tracking_noalloc: no scratch on this device (call eh_init first)");
+  if (n_elements > EH_SCRATCH_NDESC)
+    return fail(EH_ERR_ARG, "eh_do_tracking_noalloc: %ld elements, the static scratch holds %ld", n_elements,
+                EH_SCRATCH_NDESC);
+  long bends = 0;
+  for (long e = 0; e < n_elements; e++)
+    bends += beamline[e].type == EH_T_CSBEND;
+  if (bends * EH_MAX_TERMS_PER_BEND > EH_SCRATCH_NTERMS)
+    return fail(EH_ERR_ARG, "eh_do_tracking_noalloc: %ld CSBENDs may need more field terms than the static "
+                            "scratch holds (%ld)", bends, EH_SCRATCH_NTERMS);
+  char *base = (char *)g_scratch[g_device];
+  unsigned long long *cnt0 = b->d_cnt;
+  eh_desc *desc0 = b->d_desc;
+  eh_fterm *terms0 = b->d_terms;
+  long dcap0 = b->desc_cap, tcap0 = b->terms_cap;
+  b->d_cnt = (unsigned long long *)base;
+  b->d_desc = (eh_desc *)(base + EH_SCRATCH_DESC_OFF);
+  b->desc_cap = EH_SCRATCH_NDESC;
+  b->d_terms = (eh_fterm *)(base + EH_SCRATCH_TERMS_OFF);
+  b->terms_cap = EH_SCRATCH_NTERMS;
+  b->noalloc = true;
+  long r = eh_do_tracking(b, beamline, n_elements, n_passes, P_central, EH_MODE_FUSED, turns_per_launch, stats);
+  b->d_cnt = cnt0;
+  b->d_desc = desc0;
+  b->d_terms = terms0;
+  b->desc_cap = dcap0;
+  b->terms_cap = tcap0;
+  b->noalloc = false;
   return r;
 }
 

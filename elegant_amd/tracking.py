@@ -85,6 +85,19 @@ def track(beam, lattice, n_passes, p_central, mode="fused", turns_per_launch=16)
     return n, stats_dict(st), pc.value
 
 
+def track_noalloc(beam, lattice, n_passes, p_central, turns_per_launch=16):
+    """track() in fused mode through eh_do_tracking_noalloc (no device
+    allocation: descriptors, field terms and counters in the static scratch)"""
+    arr, keep = lattice.c_beamline()
+    pc = C.c_double(p_central)
+    st = S.eh_track_stats()
+    n = lib().eh_do_tracking_noalloc(beam.handle, arr, len(lattice), n_passes, C.byref(pc), int(turns_per_launch),
+                                     C.byref(st))
+    _lib.check(n)
+    del keep
+    return n, stats_dict(st), pc.value
+
+
 def stats_dict(st):
     return {k: getattr(st, k) for k, _ in S.eh_track_stats._fields_}
 

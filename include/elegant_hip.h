@@ -354,6 +354,19 @@ typedef struct {
 long eh_do_tracking(eh_beam *beam, EH_ELEMENT_LIST *beamline, long n_elements, long n_passes, double *P_central,
                     int mode, int turns_per_launch, eh_track_stats *stats);
 
+/* No counterpart in the reference: eh_do_tracking in EH_MODE_FUSED for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The element descriptors,
+ * CSBEND field terms and launch counters live in a static 8 MiB scratch array
+ * per GPU instead of per-beam allocations; the caller still creates the beam
+ * (its particles are the inputs and outputs).  Limits: at most 4096 elements, and
+ * as many CSBENDs as the field terms left in the scratch allow at the worst
+ * case per bend (421 with the current descriptor size); larger lines return
+ * EH_ERR_ARG before anything runs.  eh_init must have run on the device.  Calls
+ * that overlap on one GPU share the scratch and race. */
+long eh_do_tracking_noalloc(eh_beam *beam, EH_ELEMENT_LIST *beamline, long n_elements, long n_passes,
+                            double *P_central, int turns_per_launch, eh_track_stats *stats);
+
 /* host utility: CSBEND field-expansion table (computeCSBENDFieldCoefficients,
  * src/csbend.c:175-652) for b[9], c[9] (index 0 = dipole term) and curvature h.
  * Writes 11x11 row-major Fx, Fy; returns expansionOrder+1 or a negative code. */
