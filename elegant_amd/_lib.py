@@ -25,6 +25,7 @@ EXPORTS = [
     "eh_adjoint_free", "eh_adjoint_forward", "eh_adjoint_backward", "eh_adjoint_forward_batch",
     "eh_multipole_tracking", "eh_offset_beam", "eh_rotate_beam", "eh_replay_sequence", "eh_set_central_momentum",
     "eh_track_particles", "eh_beam_row_order", "eh_do_tracking_noalloc", "eh_elimit_amplitudes",
+    "eh_get_phase_reference", "eh_set_phase_reference", "eh_delete_phase_references",
 ]
 
 
@@ -90,6 +91,12 @@ def load():
     lib.eh_limit_amplitudes.restype = L
     lib.eh_elimit_amplitudes.argtypes = [vp, D, D, L, D, D, L, L, L, L]
     lib.eh_elimit_amplitudes.restype = L
+    lib.eh_get_phase_reference.argtypes = [C.c_int64, dp]
+    lib.eh_get_phase_reference.restype = L
+    lib.eh_set_phase_reference.argtypes = [C.c_int64, D]
+    lib.eh_set_phase_reference.restype = L
+    lib.eh_delete_phase_references.argtypes = []
+    lib.eh_delete_phase_references.restype = L
     lib.eh_removeInvalidParticles.argtypes = [vp, L, D, D]
     lib.eh_removeInvalidParticles.restype = L
     lib.eh_do_tracking.argtypes = [vp, elp, L, L, dp, I, I, C.POINTER(S.eh_track_stats)]

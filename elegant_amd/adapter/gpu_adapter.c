@@ -387,17 +387,27 @@ static const char *const rfca_untracked[] __attribute__((unused)) = {"backtrack"
 /* RFCA, src/track.h:1494-1508, as trackRfCavityWithWakes reads it
  * (src/simple_rfca.c:295-985).  The fiducial state (fiducial_seen,
  * phase_fiducial, t_fiducial) is copied in and written back after the call,
- * as the reference mutates the element.  backtrack is set only by elegant's
- * backtracking of RF cavities, which the library is not called for. */
+ * as the reference mutates the element; PHASE_REFERENCE groups are kept in
+ * elegant's own table (gpu_trackRfCavityWithWakes).  backtrack is set only by
+ * elegant's backtracking of RF cavities, which the library is not called for. */
 static int fill_rfca(const RFCA *rf, EH_RFCA *d) {
   unsigned long mode;
-  if (rf->phase_reference || rf->change_p0 || rf->change_t || rf->linearize || rf->lockPhase || rf->Q != 0)
+  int32_t fid;
+  if (rf->change_p0 || rf->change_t || rf->linearize || rf->lockPhase || rf->Q != 0)
     return 0;
   if (rf->bodyFocusModel && rf->bodyFocusModel[0] && strcmp(rf->bodyFocusModel, "none") != 0)
     return 0;
   mode = parseFiducialMode(rf->fiducial);
-  if (mode != FID_MODE_TMEAN && mode != FID_MODE_LIGHT)
-    return 0;
+  if (mode == FID_MODE_TMEAN)
+    fid = EH_FID_TMEAN;
+  else if (mode == FID_MODE_LIGHT)
+    fid = EH_FID_LIGHT;
+  else if (mode == FID_MODE_FIRST)
+    fid = EH_FID_FIRST;
+  else if (mode == FID_MODE_PMAX)
+    fid = EH_FID_PMAX;
+  else
+    return 0; /* an invalid mode: elegant's routine reports it */
   memset(d, 0, sizeof *d);
   d->length = rf->length;
   d->volt = rf->volt;
@@ -415,7 +425,8 @@ static int fill_rfca(const RFCA *rf, EH_RFCA *d) {
   d->standingWave = rf->standingWave;
   d->linearize = rf->linearize;
   d->lockPhase = rf->lockPhase;
-  d->fiducial_mode = mode == FID_MODE_LIGHT ? EH_FID_LIGHT : EH_FID_TMEAN;
+  d->fiducial_mode = fid;
+  d->phase_reference = (int64_t)rf->phase_reference;
   d->fiducial_seen = rf->fiducial_seen;
   d->phase_fiducial = rf->phase_fiducial;
   d->t_fiducial = rf->t_fiducial;
@@ -749,14 +760,27 @@ long gpu_trackRfCavityWithWakes(long np, RFCA *rfca, double **accepted, double *
   long r;
   if (fill_rfca(rfca, &r_) && charge == NULL && wake == NULL && trwake == NULL && LSCKick == NULL &&
       (accepted == NULL || accepted == G.accepted)) {
+    double ph;
+    /* the cavity's phase-reference group as elegant's table holds it
+     * (src/phase_reference.c:28-110), mirrored into the library's for the call
+     * and back after it */
+    if (r_.phase_reference) {
+      eh_delete_phase_references();
+      if (get_phase_reference(&ph, rfca->phase_reference) == REF_PHASE_RETURNED)
+        eh_set_phase_reference(r_.phase_reference, ph);
+    }
     ad_to_device(np);
     r = eh_simple_rf_cavity(G.beam, np, &r_, P_central, zEnd);
     if (r >= 0) {
       rfca_write_back(rfca, &r_);
+      if (r_.phase_reference && eh_get_phase_reference(r_.phase_reference, &ph) == EH_REF_PHASE_RETURNED)
+        set_phase_reference(rfca->phase_reference, ph);
       ad_device_changed();
       return r;
     }
-    if (r != EH_ERR_UNSUPPORTED)
+    /* options the library does not carry, and fiducial failures, which
+     * elegant's routine reports itself */
+    if (r != EH_ERR_UNSUPPORTED && r != EH_ERR_ARG)
       ad_fatal("trackRfCavityWithWakes");
   }
   {

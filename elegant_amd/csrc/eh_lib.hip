@@ -178,6 +178,10 @@ struct eh_ctx {
   const EH_MAXAMP *ap[2];
   double post_x[2], post_y[2];
   int post_on[2];
+  /* an RFCA in a phase-reference group set by another cavity: the group's
+   * phase replaces the cavity's own phase_fiducial (src/simple_rfca.c:419-421) */
+  int rf_linked;
+  double rf_ref_phase;
 };
 
 static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c) {
@@ -549,7 +553,7 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
 }
 
 /* RFCA constants, src/simple_rfca.c:378-470, once the fiducial phase is known */
-static long build_rfca(eh_desc *d, const EH_RFCA *rf) {
+static long build_rfca(eh_desc *d, const EH_RFCA *rf, const eh_ctx &c) {
   if (rf->Q != 0 || rf->change_t || rf->change_p0 || rf->linearize || rf->lockPhase)
     return fail(EH_ERR_UNSUPPORTED, "RFCA option not supported");
   d->rf_flen = rf->length;
@@ -574,7 +578,7 @@ static long build_rfca(eh_desc *d, const EH_RFCA *rf) {
   d->rf_volt = volt;
   d->rf_omega = omega;
   d->rf_dtLight = length / EH_C_MKS;
-  double phase = rf->phase_fiducial;
+  double phase = c.rf_linked ? c.rf_ref_phase : rf->phase_fiducial;
   d->rf_t0 = omega ? -rf->phase_fiducial / omega : 0;
   phase += rf->phase * EH_PI / 180.0;
   d->rf_phase = phase;
@@ -767,7 +771,7 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
     return build_csbend(d, (const EH_CSBEND *)e->p_elem, c, pool);
   case EH_T_RFCA:
     d->length = ((const EH_RFCA *)e->p_elem)->length;
-    return build_rfca(d, (const EH_RFCA *)e->p_elem);
+    return build_rfca(d, (const EH_RFCA *)e->p_elem, c);
   case EH_T_MULT:
     return build_mult(d, (const EH_MULT *)e->p_elem, c.Po);
   case EH_T_ROTATE:
@@ -1807,12 +1811,18 @@ extern "C" long eh_elimit_amplitudes(eh_beam *b, double xmax, double ymax, long 
   return run_plain(b, &d, nullptr, 3, xmax, ymax, z, Po, (int)extrapolate_z);
 }
 
-/* Kahan-compensated TMEAN fiducial over the alive particles in reference row
- * order (findFiducialTime, src/simple_rfca.c:179-254; KahanPlus, src/kahan.c:27-35) */
+/* the beam's fiducial time at a cavity (findFiducialTime, src/simple_rfca.c:62-262,
+ * serial elegant), over the alive particles in the reference's row order: TMEAN
+ * as a Kahan-compensated mean (KahanPlus, src/kahan.c:27-35); PMAXIMUM from the
+ * first particle whose momentum exceeds every earlier one, starting from row 0
+ * (an error if row 0 has the largest); FIRST always fails, as the reference's
+ * loop, which never stops early, does (src/simple_rfca.c:101-111) */
 static long reference_order(eh_beam *b, std::vector<int32_t> &slot_at_row, std::vector<int32_t> &alive_row0,
                             std::vector<int32_t> &dev_index_alive, std::vector<int32_t> *pred = nullptr,
                             std::vector<int32_t> *order = nullptr);
-static long fiducial_tmean(eh_beam *b, double sOffset, double p0, double *tFid) {
+static long fiducial_from_beam(eh_beam *b, int mode, double sOffset, double p0, double *tFid) {
+  if (mode == EH_FID_FIRST)
+    return fail(EH_ERR_ARG, "No available particle for the FID_MODE_FIRST mode in findFiducialTime");
   std::vector<int32_t> slot_at_row, alive_row0, idx;
   long r = reference_order(b, slot_at_row, alive_row0, idx);
   if (r < 0)
@@ -1823,18 +1833,35 @@ static long fiducial_tmean(eh_beam *b, double sOffset, double p0, double *tFid) 
   HIPCHK(hipMemcpy(s.data(), a.s, sizeof(double) * n, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(dp.data(), a.dp, sizeof(double) * n, hipMemcpyDeviceToHost));
   /* idx[row0] = device index of the alive particle with that identity */
-  double tsum = 0, error = 0.0;
-  long nsum = 0;
+  std::vector<long> dev(n);
   for (long row = 0; row < n; row++) {
-    int32_t id = slot_at_row[row];
-    long k = idx[id];
-    if (k < 0)
+    dev[row] = idx[slot_at_row[row]];
+    if (dev[row] < 0)
       return fail(EH_ERR_STATE, "replay produced a lost particle among survivors");
+  }
+  auto t_of = [&](long k) {
     double p = p0;
     p *= 1 + dp[k];
     double beta = p / std::sqrt(p * p + 1);
-    double term = (s[k] + sOffset) / (EH_C_MKS * beta);
-    double Y = term + error;
+    return (s[k] + sOffset) / (EH_C_MKS * beta);
+  };
+  if (mode == EH_FID_PMAX) {
+    long ibest = -1;
+    double best = n > 0 ? dp[dev[0]] : 0;
+    for (long row = 0; row < n; row++)
+      if (best < dp[dev[row]]) {
+        best = dp[dev[row]];
+        ibest = row;
+      }
+    if (ibest < 0)
+      return fail(EH_ERR_ARG, "No available particle for RF cavity fiducialization");
+    *tFid = t_of(dev[ibest]);
+    return EH_OK;
+  }
+  double tsum = 0, error = 0.0;
+  long nsum = 0;
+  for (long row = 0; row < n; row++) {
+    double Y = t_of(dev[row]) + error;
     double sum = tsum + Y;
     error = Y - (sum - tsum);
     tsum = sum;
@@ -1844,19 +1871,72 @@ static long fiducial_tmean(eh_beam *b, double sOffset, double p0, double *tFid) 
   return EH_OK;
 }
 
-/* an RF cavity whose phase fiducial is still to be taken from the beam on its
- * first pass (src/simple_rfca.c:414-463; a cavity without voltage never takes one) */
+/* phase references, src/phase_reference.c:28-143: numbered groups of
+ * time-dependent elements sharing one reference phase, set by the first cavity
+ * of the group tracked with particles and kept until eh_delete_phase_references.
+ * A cavity with PHASE_REFERENCE 0 is its own group (the reference gives it an
+ * unused number; the phase it then tracks with is always its own fiducial). */
+struct eh_phase_ref {
+  int64_t ref;
+  int set;
+  double phase;
+};
+static std::vector<eh_phase_ref> g_phase_refs;
+
+extern "C" long eh_get_phase_reference(int64_t ref, double *phase) {
+  if (ref == 0 || !phase)
+    return 0;
+  for (const eh_phase_ref &g : g_phase_refs)
+    if (g.ref == ref) {
+      if (!g.set)
+        return EH_REF_PHASE_NOT_SET;
+      *phase = g.phase;
+      return EH_REF_PHASE_RETURNED;
+    }
+  return EH_REF_PHASE_NONEXISTENT;
+}
+
+extern "C" long eh_set_phase_reference(int64_t ref, double phase) {
+  if (ref == 0)
+    return 0;
+  if (ref > INT64_MAX / 2 || ref < 0)
+    return fail(EH_ERR_ARG, "please use a small integer for the phase_reference number");
+  for (eh_phase_ref &g : g_phase_refs)
+    if (g.ref == ref) {
+      g.phase = phase;
+      g.set = 1;
+      return 1;
+    }
+  g_phase_refs.push_back({ref, 1, phase});
+  return 1;
+}
+
+extern "C" long eh_delete_phase_references(void) {
+  for (eh_phase_ref &g : g_phase_refs)
+    g.set = 0;
+  return EH_OK;
+}
+
+/* an RF cavity that may need the beam for its phase on its first pass: it
+ * takes a fiducial there unless it has one or its group's phase is set by
+ * then (src/simple_rfca.c:414-463; a cavity without voltage takes none) */
 static bool rf_pending(const EH_ELEMENT_LIST *e) {
   if (e->type != EH_T_RFCA)
     return false;
   const EH_RFCA *rf = (const EH_RFCA *)e->p_elem;
+  double ph;
+  if (rf->phase_reference && eh_get_phase_reference(rf->phase_reference, &ph) == EH_REF_PHASE_RETURNED)
+    return false;
   return !rf->fiducial_seen && rf->volt != 0;
 }
 
-/* fiducial time of a cavity's first pass: TMEAN from the beam (tmean_of), LIGHT
- * from its position, or T_REFERENCE (src/simple_rfca.c:414-463) */
-template <class TMean>
-static long rf_take_fiducial(EH_RFCA *rf, double zEnd, TMean tmean_of) {
+/* fiducial time of a cavity's first pass: from the beam (fid_of, for TMEAN,
+ * PMAXIMUM and FIRST), LIGHT from its position, or T_REFERENCE
+ * (src/simple_rfca.c:414-463) */
+template <class BeamFid>
+static long rf_take_fiducial(EH_RFCA *rf, double zEnd, BeamFid fid_of) {
+  if (rf->fiducial_mode < EH_FID_TMEAN || rf->fiducial_mode > EH_FID_PMAX)
+    return fail(EH_ERR_ARG, "invalid fiducial mode for RFCA element");
   double length = rf->length;
   long nk = length ? rf->nKicks : 1;
   if (nk <= 0)
@@ -1868,7 +1948,7 @@ static long rf_take_fiducial(EH_RFCA *rf, double zEnd, TMean tmean_of) {
   else if (rf->fiducial_mode == EH_FID_LIGHT)
     t0 = ((zEnd - rf->length) + length / 2) / EH_C_MKS;
   else {
-    long r = tmean_of(length / 2, &t0);
+    long r = fid_of(rf->fiducial_mode, length / 2, &t0);
     if (r < 0)
       return r;
   }
@@ -1879,12 +1959,31 @@ static long rf_take_fiducial(EH_RFCA *rf, double zEnd, TMean tmean_of) {
   return EH_OK;
 }
 
-/* first-encounter fiducial, src/simple_rfca.c:414-463 */
-static long rfca_fiducial(eh_beam *b, EH_RFCA *rf, double Pc, double zEnd) {
-  EH_ELEMENT_LIST e = {EH_T_RFCA, 0, "RFCA", (void *)rf};
-  if (!rf_pending(&e))
+/* the phase a cavity with voltage tracks with at a call with particles
+ * (src/simple_rfca.c:419-466): its group's if set, else its own fiducial (taken
+ * from the beam b if it has none), which then sets the group.  c gets the
+ * group's phase for build_desc. */
+static long rf_resolve(eh_beam *b, EH_RFCA *rf, double Pc, double zEnd, eh_ctx &c) {
+  double ph = 0;
+  c.rf_linked = 0;
+  if (rf->phase_reference && eh_get_phase_reference(rf->phase_reference, &ph) == EH_REF_PHASE_RETURNED) {
+    c.rf_linked = 1;
+    c.rf_ref_phase = ph;
     return EH_OK;
-  return rf_take_fiducial(rf, zEnd, [&](double sOffset, double *t0) { return fiducial_tmean(b, sOffset, Pc, t0); });
+  }
+  if (!rf->fiducial_seen) {
+    long r = rf_take_fiducial(rf, zEnd, [&](int mode, double sOffset, double *t0) {
+      return fiducial_from_beam(b, mode, sOffset, Pc, t0);
+    });
+    if (r < 0)
+      return r;
+  }
+  if (rf->phase_reference) {
+    long r = eh_set_phase_reference(rf->phase_reference, rf->phase_fiducial);
+    if (r < 0)
+      return r;
+  }
+  return EH_OK;
 }
 
 extern "C" long eh_simple_rf_cavity(eh_beam *b, long np, EH_RFCA *rfca, double *P_central, double zEnd) {
@@ -1895,15 +1994,17 @@ extern "C" long eh_simple_rf_cavity(eh_beam *b, long np, EH_RFCA *rfca, double *
     return np;
   if (rfca->Q != 0 || rfca->change_t || rfca->change_p0 || rfca->linearize || rfca->lockPhase)
     return fail(EH_ERR_UNSUPPORTED, "RFCA option not supported");
-  r = rfca_fiducial(b, rfca, *P_central, zEnd);
-  if (r < 0)
-    return r;
-  if (rfca->volt != 0 && rfca->freq != 0)
-    rfca->t_fiducial = -rfca->phase_fiducial / (2 * EH_PI * rfca->freq);
+  eh_ctx c = {};
+  if (rfca->volt != 0) {
+    r = rf_resolve(b, rfca, *P_central, zEnd, c);
+    if (r < 0)
+      return r;
+    if (rfca->freq != 0)
+      rfca->t_fiducial = -rfca->phase_fiducial / (2 * EH_PI * rfca->freq);
+  }
   EH_ELEMENT_LIST e = {EH_T_RFCA, 0, "RFCA", (void *)rfca};
   eh_desc d;
   std::vector<eh_fterm> pool;
-  eh_ctx c = {};
   c.Po = *P_central;
   c.z_start = zEnd - rfca->length;
   c.z_end = zEnd;
@@ -2299,8 +2400,14 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
   for (long ie = 0; ie < ne; ie++)
     two_phase = two_phase || Pe0[ie] != Pe1[ie];
   const std::vector<double> *Pe = &Pe0;
+  /* RF cavities whose phase needs the beam at their first pass (pend), and
+   * those tracking with another cavity's group phase (link, lph) */
+  std::vector<char> pend(ne, 0), link(ne, 0);
+  std::vector<double> lph(ne, 0.0);
   auto build = [&](long ie) -> long {
     eh_ctx c = {};
+    c.rf_linked = link[ie];
+    c.rf_ref_phase = lph[ie];
     c.Po = (*Pe)[ie];
     c.z_start = zs[ie];
     c.z_end = ze[ie];
@@ -2314,18 +2421,51 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
     }
     return build_desc(&hd[ie], bl + ie, c, pool);
   };
-  for (long ie = 0; ie < ne; ie++) {
-    if (rf_pending(bl + ie))
-      continue; /* built once its fiducial is known */
-    r = build(ie);
-    if (r < 0)
-      return r;
-  }
   /* initial NaN sweep, src/do_tracking.c:478-483 */
   b->pass = 0;
   r = eh_limit_amplitudes(b, EH_DBL_MAX, EH_DBL_MAX, b->n_alive, 0.0, *Pc, 0, 0);
   if (r < 0)
     return r;
+  /* the cavities' phases in pass-0 order (src/simple_rfca.c:419-466): a group
+   * phase already set; a cavity with a fiducial sets its group; one without
+   * waits for the beam, and so do the later cavities of its group.  Groups are
+   * set only when the pass is tracked with particles, as in the reference. */
+  const bool tracks = n_passes > 0 && b->n_alive > 0;
+  {
+    std::vector<int64_t> claimed;
+    for (long ie = 0; ie < ne; ie++) {
+      if (bl[ie].type != EH_T_RFCA)
+        continue;
+      EH_RFCA *rf = (EH_RFCA *)bl[ie].p_elem;
+      const int64_t ref = rf->phase_reference;
+      double ph;
+      if (rf->volt == 0)
+        continue;
+      if (ref && std::find(claimed.begin(), claimed.end(), ref) != claimed.end()) {
+        pend[ie] = 1;
+      } else if (ref && eh_get_phase_reference(ref, &ph) == EH_REF_PHASE_RETURNED) {
+        link[ie] = 1;
+        lph[ie] = ph;
+      } else if (rf->fiducial_seen) {
+        if (ref && tracks) {
+          r = eh_set_phase_reference(ref, rf->phase_fiducial);
+          if (r < 0)
+            return r;
+        }
+      } else {
+        pend[ie] = 1;
+        if (ref)
+          claimed.push_back(ref);
+      }
+    }
+  }
+  for (long ie = 0; ie < ne; ie++) {
+    if (pend[ie])
+      continue; /* built once its phase is known */
+    r = build(ie);
+    if (r < 0)
+      return r;
+  }
   HIPCHK(hipMemcpy(b->d_desc, hd.data(), sizeof(eh_desc) * ne, hipMemcpyHostToDevice));
   r = upload_terms(b, pool);
   if (r < 0)
@@ -2345,25 +2485,43 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
     rad = rad || !eh_lean_capable(hd[ie]) || bl[ie].type >= EH_T_MULT;
   int p = 0, e = 0;
   if (n_passes > 0) {
-    /* pass 0: stop before each RFCA whose fiducial is still unknown */
+    /* pass 0: stop before each RFCA whose phase waits for the beam */
     for (long ie = 0; ie < ne; ie++) {
-      if (!rf_pending(bl + ie))
+      if (!pend[ie])
         continue;
       EH_RFCA *rf = (EH_RFCA *)bl[ie].p_elem;
       r = launch_fused(b, ne, p, e, 0, (int)ie, effort, tab, rad);
       if (r < 0)
         return r;
-      r = rfca_fiducial(b, rf, Pe0[ie], ze[ie]);
+      e = (int)ie;
+      if (b->n_alive == 0)
+        break; /* no particle reaches it: the reference takes no fiducial */
+      eh_ctx rc = {};
+      r = rf_resolve(b, rf, Pe0[ie], ze[ie], rc);
       if (r < 0)
         return r;
-      for (long je = 0; je < ne; je++)
+      /* this cavity, the other positions of the same cavity, and the later
+       * cavities of its group now have their phase */
+      for (long je = ie; je < ne; je++) {
+        if (!pend[je])
+          continue;
+        const EH_RFCA *rj = (const EH_RFCA *)bl[je].p_elem;
+        double ph;
         if (bl[je].p_elem == bl[ie].p_elem) {
-          r = build(je);
-          if (r < 0)
-            return r;
-          HIPCHK(hipMemcpy(b->d_desc + je, &hd[je], sizeof(eh_desc), hipMemcpyHostToDevice));
-        }
-      e = (int)ie;
+          link[je] = (char)rc.rf_linked;
+          lph[je] = rc.rf_ref_phase;
+        } else if (rj->phase_reference &&
+                   eh_get_phase_reference(rj->phase_reference, &ph) == EH_REF_PHASE_RETURNED) {
+          link[je] = 1;
+          lph[je] = ph;
+        } else
+          continue;
+        pend[je] = 0;
+        r = build(je);
+        if (r < 0)
+          return r;
+        HIPCHK(hipMemcpy(b->d_desc + je, &hd[je], sizeof(eh_desc), hipMemcpyHostToDevice));
+      }
     }
   }
   while (p < n_passes) {
@@ -3109,6 +3267,17 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
   for (long e = 0; e < ne; e++)
     if (beamlines[0][e].type == EH_T_ENERGY)
       return fail(EH_ERR_UNSUPPORTED, "element %ld: ENERGY is not supported in batched tracking", e);
+  /* every variant is a run of its own: no phase-reference groups across them,
+   * and the batch takes TMEAN and LIGHT fiducials */
+  for (long v = 0; v < b->nvar; v++)
+    for (long e = 0; e < ne; e++)
+      if (beamlines[v][e].type == EH_T_RFCA) {
+        const EH_RFCA *rf = (const EH_RFCA *)beamlines[v][e].p_elem;
+        if (rf->phase_reference)
+          return fail(EH_ERR_UNSUPPORTED, "element %ld: PHASE_REFERENCE is not supported in batched tracking", e);
+        if (rf->fiducial_mode != EH_FID_TMEAN && rf->fiducial_mode != EH_FID_LIGHT && !rf->fiducial_seen)
+          return fail(EH_ERR_UNSUPPORTED, "element %ld: batched tracking takes TMEAN and LIGHT fiducials only", e);
+      }
   double Pc = *P_central;
   /* descriptors, deduplicated across variants */
   std::vector<eh_desc> pool;
@@ -3208,8 +3377,9 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
         double zEnd = 0;
         for (long k = 0; k <= er; k++)
           zEnd += elem_length(beamlines[v] + k);
-        r = rf_take_fiducial((EH_RFCA *)beamlines[v][er].p_elem, zEnd,
-                             [&](double sOffset, double *t0) { return batch_fiducial(b, v, sOffset, Pc, t0); });
+        r = rf_take_fiducial((EH_RFCA *)beamlines[v][er].p_elem, zEnd, [&](int, double sOffset, double *t0) {
+          return batch_fiducial(b, v, sOffset, Pc, t0);
+        });
         if (r < 0)
           return r;
       }
@@ -3659,6 +3829,8 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
         return fail(EH_ERR_ARG, "variant %ld element %ld: types differ from variant 0", v, e);
       if (t == EH_T_RFCA) {
         const EH_RFCA *rf = (const EH_RFCA *)beamlines[v][e].p_elem;
+        if (rf->phase_reference)
+          return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support PHASE_REFERENCE", e);
         /* a variant without particles never reads its descriptors */
         if (!rf->fiducial_seen && rf->volt != 0 && !(counts && counts[v] == 0))
           return fail(EH_ERR_ARG, "element %ld: RFCA fiducial unknown; track one pass first to fix it", e);

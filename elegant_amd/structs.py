@@ -20,7 +20,7 @@ TYPE_CODES = {v: k for k, v in TYPE_NAMES.items()}
 
 BEND_EDGE1_EFFECTS = 1
 BEND_EDGE2_EFFECTS = 2
-FID_TMEAN, FID_LIGHT = 0, 1
+FID_TMEAN, FID_LIGHT, FID_FIRST, FID_PMAX = 0, 1, 2, 3
 
 C_MKS = 2.99792458e8
 ME_MEV = 0.51099895000
@@ -73,8 +73,9 @@ class EH_CSBEND(C.Structure):
 class EH_RFCA(C.Structure):
     _fields_ = [(n, dbl) for n in ("length", "volt", "phase", "freq", "Q", "dx", "dy", "tReference")] + \
                [(n, i32) for n in ("nKicks", "change_p0", "change_t", "end1Focus", "end2Focus", "standingWave",
-                                   "linearize", "lockPhase", "fiducial_mode", "fiducial_seen")] + \
-               [("phase_fiducial", dbl), ("t_fiducial", dbl)]
+                                   "linearize", "lockPhase", "fiducial_mode")] + \
+               [("phase_reference", C.c_int64), ("fiducial_seen", i32), ("pad_", i32),
+                ("phase_fiducial", dbl), ("t_fiducial", dbl)]
 
 
 class EH_MULT(C.Structure):
@@ -173,6 +174,7 @@ ALIASES = {
                "USE_BN": "use_bn", "EXPANSION_ORDER": "expansionOrder", "ETILT_SIGN": "etiltSign",
                "MALIGN_METHOD": "malignMethod", "NONLINEAR": "nonlinear", "SYNCH_RAD": "synch_rad"},
     T_RFCA: {"N_KICKS": "nKicks", "CHANGE_P0": "change_p0", "CHANGE_T": "change_t", "END1_FOCUS": "end1Focus",
+             "PHASE_REFERENCE": "phase_reference",
              "END2_FOCUS": "end2Focus", "STANDING_WAVE": "standingWave", "LINEARIZE": "linearize",
              "LOCK_PHASE": "lockPhase", "T_REFERENCE": "tReference", "VOLT": "volt"},
     T_MULT: {"N_SLICES": "nSlices", "N_KICKS": "nSlices", "KNL": "KnL", "BTIPL": "BTipL",

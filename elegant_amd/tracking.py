@@ -98,6 +98,19 @@ def track_noalloc(beam, lattice, n_passes, p_central, turns_per_launch=16):
     return n, stats_dict(st), pc.value
 
 
+def delete_phase_references():
+    """forget every phase-reference group's phase in the library's table
+    (delete_phase_references, src/phase_reference.c:115-127)"""
+    _lib.check(lib().eh_delete_phase_references())
+
+
+def get_phase_reference(ref):
+    """(status, phase) of a phase-reference group: status 1 set, 2 not set, 3 unknown"""
+    ph = C.c_double(0)
+    st = lib().eh_get_phase_reference(ref, C.byref(ph))
+    return st, ph.value
+
+
 def stats_dict(st):
     return {k: getattr(st, k) for k, _ in S.eh_track_stats._fields_}
 

@@ -177,8 +177,10 @@ typedef struct {
 #define EH_BEND_EDGE1_EFFECTS 1
 #define EH_BEND_EDGE2_EFFECTS 2
 
-/* fiducial modes, src/simple_rfca.c:35-48 */
-enum eh_fid_mode { EH_FID_TMEAN = 0, EH_FID_LIGHT = 1 };
+/* fiducial modes, src/simple_rfca.c:23-48 (FIDUCIAL "tmean", "light", "first",
+ * "pmaximum"; as in serial elegant, FIRST always fails and PMAXIMUM fails when
+ * the first particle has the largest momentum, src/simple_rfca.c:101-146) */
+enum eh_fid_mode { EH_FID_TMEAN = 0, EH_FID_LIGHT = 1, EH_FID_FIRST = 2, EH_FID_PMAX = 3 };
 
 /* RFCA: src/track.h:1494-1508, defaults src/track_data.c:579 */
 typedef struct {
@@ -189,8 +191,13 @@ typedef struct {
   int32_t change_p0, change_t, end1Focus, end2Focus, standingWave;
   int32_t linearize, lockPhase;
   int32_t fiducial_mode; /* eh_fid_mode */
+  /* PHASE_REFERENCE: cavities with the same non-zero number share the phase
+   * fiducial of the first of them tracked (src/simple_rfca.c:412-466); 0: the
+   * cavity's own (the library does not write a number back, as the reference
+   * does with unused_phase_reference) */
+  int64_t phase_reference;
   /* state mutated by tracking, like the reference element (src/simple_rfca.c:460-475) */
-  int32_t fiducial_seen;
+  int32_t fiducial_seen, pad_;
   double phase_fiducial, t_fiducial;
 } EH_RFCA;
 
@@ -326,6 +333,18 @@ long eh_limit_amplitudes(eh_beam *beam, double xmax, double ymax, long np, doubl
  * particle's dp becoming sqrt((Po (1 + dp))^2 + 1) */
 long eh_elimit_amplitudes(eh_beam *beam, double xmax, double ymax, long np, double z, double Po, long extrapolate_z,
                           long openCode, long exponent, long yexponent);
+
+/* phase references (src/phase_reference.c:28-143): the library's table of
+ * PHASE_REFERENCE groups, which eh_simple_rf_cavity and eh_do_tracking read and
+ * set.  eh_get_phase_reference replaces get_phase_reference (:28) and returns
+ * EH_REF_PHASE_RETURNED with *phase, EH_REF_PHASE_NOT_SET or
+ * EH_REF_PHASE_NONEXISTENT (0 for group 0); eh_set_phase_reference replaces
+ * set_phase_reference (:73); eh_delete_phase_references replaces
+ * delete_phase_references (:115), which elegant calls between steps. */
+enum { EH_REF_PHASE_RETURNED = 1, EH_REF_PHASE_NOT_SET = 2, EH_REF_PHASE_NONEXISTENT = 3 };
+long eh_get_phase_reference(int64_t ref, double *phase);
+long eh_set_phase_reference(int64_t ref, double phase);
+long eh_delete_phase_references(void);
 
 /* replaces removeInvalidParticles, src/limit_amplitudes.c:289-291 */
 long eh_removeInvalidParticles(eh_beam *beam, long np, double z, double Po);

@@ -24,6 +24,7 @@
  * and change results only in the last bit.
  */
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -1588,11 +1589,33 @@ static double beta_from_delta(double p, double delta) { /* src/do_tracking.c:95-
   return (p / sqrt(p * p + 1));
 }
 
-/* findFiducialTime, src/simple_rfca.c:62-262 (TMEAN with Kahan, LIGHT) */
-double or_findFiducialTime(double **part, long np, double s0, double sOffset, double p0, int mode) {
-  double tFid = 0.0;
+/* findFiducialTime, src/simple_rfca.c:62-262 (serial elegant, the whole beam):
+ * LIGHT from the position; FIRST, whose loop never stops early and so always
+ * ends in the reference's error; PMAXIMUM from the first particle whose
+ * momentum exceeds every earlier one, starting from particle 0 (an error when
+ * particle 0 has the largest); TMEAN as a Kahan sum.  0, or the error code. */
+long or_findFiducialTime(double **part, long np, double s0, double sOffset, double p0, int mode, double *tFid) {
+  *tFid = 0.0;
   if (mode == EH_FID_LIGHT) {
-    tFid = (s0 + sOffset) / c_mks;
+    *tFid = (s0 + sOffset) / c_mks;
+  } else if (mode == EH_FID_FIRST) {
+    long i;
+    for (i = 0; i < np; i++)
+      *tFid = (part[i][4] + sOffset) / (c_mks * beta_from_delta(p0, part[i][5]));
+    if (i == np)
+      return or_fail("No available particle for the FID_MODE_FIRST mode in findFiducialTime");
+  } else if (mode == EH_FID_PMAX) {
+    long i, ibest = -1;
+    double best = part[0][5];
+    for (i = 0; i < np; i++)
+      if (best < part[i][5]) {
+        best = part[i][5];
+        ibest = i;
+      }
+    if (ibest >= 0 && ibest < np)
+      *tFid = (part[ibest][4] + sOffset) / (c_mks * beta_from_delta(p0, part[ibest][5]));
+    else
+      return or_fail("No available particle for RF cavity fiducialization");
   } else {
     double tsum = 0, error = 0.0;
     long ip, nsum = 0;
@@ -1601,10 +1624,67 @@ double or_findFiducialTime(double **part, long np, double s0, double sOffset, do
       nsum++;
     }
     if (nsum > 0)
-      tFid = tsum / nsum;
+      *tFid = tsum / nsum;
   }
-  return tFid;
+  return 0;
 }
+
+/* phase references, src/phase_reference.c:28-143: numbered groups of
+ * time-dependent elements that share one reference phase, set by the first
+ * element of the group tracked (with particles) and kept until deleted.  The
+ * reference gives an RFCA without a number an unused one (unused_phase_reference,
+ * :129-141), a group only that cavity can set, so it always tracks with its own
+ * fiducial; the checker keeps no entry for it and writes no number back. */
+#define OR_REF_PHASE_RETURNED 1
+#define OR_REF_PHASE_NOT_SET 2
+#define OR_REF_PHASE_NONEXISTENT 3
+#define OR_MAX_PHASE_REFS 4096
+static struct {
+  long ref_number, set;
+  double phase;
+} or_refs[OR_MAX_PHASE_REFS];
+static long or_n_refs;
+
+long or_get_phase_reference(double *phase, long ref) {
+  long i;
+  if (ref == 0)
+    return 0;
+  for (i = 0; i < or_n_refs; i++)
+    if (or_refs[i].ref_number == ref) {
+      if (!or_refs[i].set)
+        return OR_REF_PHASE_NOT_SET;
+      *phase = or_refs[i].phase;
+      return OR_REF_PHASE_RETURNED;
+    }
+  return OR_REF_PHASE_NONEXISTENT;
+}
+
+long or_set_phase_reference(long ref, double phase) {
+  long i;
+  if (ref == 0)
+    return 0;
+  for (i = 0; i < or_n_refs; i++)
+    if (or_refs[i].ref_number == ref)
+      break;
+  if (i == or_n_refs) {
+    if (ref > LONG_MAX / 2)
+      return or_fail("please use a small integer for the phase_reference number");
+    if (or_n_refs == OR_MAX_PHASE_REFS)
+      return or_fail("too many phase references");
+    or_n_refs++;
+  }
+  or_refs[i].ref_number = ref;
+  or_refs[i].phase = phase;
+  or_refs[i].set = 1;
+  return 1;
+}
+
+void or_delete_phase_references(void) {
+  long i;
+  for (i = 0; i < or_n_refs; i++)
+    or_refs[i].set = 0;
+}
+
 
 /* OR_PLANT (tests only: the RFCA checks build the oracle with a planted mistake
  * and show that they reject it): 1 R12 with the momentum ratio inverted, 2 R22
@@ -1668,15 +1748,27 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
   volt /= nKicks;
   dtLight = length / c_mks;
 
-  if (!rfca->fiducial_seen) {
-    if (rfca->tReference != -1)
-      t0 = rfca->tReference;
-    else
-      t0 = or_findFiducialTime(part, np, zEnd - rfca->length, length / 2, *P_central, rfca->fiducial_mode);
-    rfca->phase_fiducial = -omega * t0;
-    rfca->fiducial_seen = 1;
+  /* src/simple_rfca.c:412-466 */
+  phase = 0;
+  if (!rfca->phase_reference || or_get_phase_reference(&phase, rfca->phase_reference) != OR_REF_PHASE_RETURNED) {
+    if (!rfca->fiducial_seen) {
+      if (rfca->fiducial_mode < EH_FID_TMEAN || rfca->fiducial_mode > EH_FID_PMAX)
+        return or_fail("invalid fiducial mode for RFCA element");
+      if (rfca->tReference != -1)
+        t0 = rfca->tReference;
+      else {
+        long r = or_findFiducialTime(part, np, zEnd - rfca->length, length / 2, *P_central, rfca->fiducial_mode,
+                                     &t0);
+        if (r < 0)
+          return r;
+      }
+      rfca->phase_fiducial = -omega * t0;
+      rfca->fiducial_seen = 1;
+    }
+    long r = or_set_phase_reference(rfca->phase_reference, phase = rfca->phase_fiducial);
+    if (r < 0)
+      return r;
   }
-  phase = rfca->phase_fiducial;
   if (omega) {
     t0 = -rfca->phase_fiducial / omega;
     rfca->t_fiducial = t0;

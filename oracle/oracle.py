@@ -53,6 +53,11 @@ def load_path(path):
     lib.or_elimit_amplitudes_flat.restype = C.c_long
     lib.or_ipow.argtypes = [C.c_double, C.c_long]
     lib.or_ipow.restype = C.c_double
+    lib.or_get_phase_reference.argtypes = [dp, C.c_long]
+    lib.or_get_phase_reference.restype = C.c_long
+    lib.or_set_phase_reference.argtypes = [C.c_long, C.c_double]
+    lib.or_set_phase_reference.restype = C.c_long
+    lib.or_delete_phase_references.argtypes = []
     lib.or_csbend_field_table.argtypes = [dp, dp, C.c_double, C.c_int, C.c_int, dp, dp]
     lib.or_csbend_field_table.restype = C.c_long
     lib.or_last_error.restype = C.c_char_p
@@ -130,6 +135,19 @@ def do_tracking(coords, lattice, n_passes, p_central, libm=False):
                                             n_passes, C.byref(pc), C.byref(eff)))
     del keep
     return rows, n, eff.value, pc.value
+
+
+def delete_phase_references(libm=False):
+    """forget every phase-reference group's phase (delete_phase_references,
+    src/phase_reference.c:115-127), as elegant does between steps"""
+    load(libm).or_delete_phase_references()
+
+
+def get_phase_reference(ref, libm=False):
+    """(status, phase) of a phase-reference group: status 1 set, 2 not set, 3 unknown"""
+    ph = C.c_double(0)
+    st = load(libm).or_get_phase_reference(C.byref(ph), ref)
+    return st, ph.value
 
 
 def element(coords, lattice, index, p_central, z_start, z_end, maxamp=None, libm=False, lib=None):

@@ -16,6 +16,8 @@ static struct {
   long uploads, downloads, mp2, csb, rf, drift, tp, lim, elim, rip, off, scm, pass_set;
   long last_pass;
   EH_MAXAMP last_amp; /* the aperture of the last element call */
+  long linked;        /* cavity calls that found their group's phase set */
+  int32_t last_rf_mode;
   int perm[64]; /* download row r <- uploaded row perm[r] */
   long lose;    /* particles each tracking call loses */
 } M;
@@ -84,12 +86,48 @@ long eh_track_through_csbend(eh_beam *b, long n, const EH_CSBEND *c, double pe, 
   M.csb++;
   return track_stand_in(b, n);
 }
+/* the library's phase-reference table: one group */
+static struct {
+  int64_t ref;
+  int set;
+  double phase;
+} LT;
+long eh_get_phase_reference(int64_t ref, double *phase) {
+  if (!ref)
+    return 0;
+  if (ref != LT.ref)
+    return EH_REF_PHASE_NONEXISTENT;
+  if (!LT.set)
+    return EH_REF_PHASE_NOT_SET;
+  *phase = LT.phase;
+  return EH_REF_PHASE_RETURNED;
+}
+long eh_set_phase_reference(int64_t ref, double phase) {
+  LT.ref = ref;
+  LT.set = 1;
+  LT.phase = phase;
+  return 1;
+}
+long eh_delete_phase_references(void) {
+  LT.set = 0;
+  return EH_OK;
+}
 long eh_simple_rf_cavity(eh_beam *b, long n, EH_RFCA *rf, double *P, double zEnd) {
+  double ph;
   (void)P, (void)zEnd;
+  if (rf->fiducial_mode == EH_FID_FIRST)
+    return EH_ERR_ARG; /* as the library: FIRST never finds its particle */
   M.rf++;
+  M.last_rf_mode = rf->fiducial_mode;
+  if (rf->phase_reference && eh_get_phase_reference(rf->phase_reference, &ph) == EH_REF_PHASE_RETURNED) {
+    M.linked++;
+    return track_stand_in(b, n);
+  }
   rf->fiducial_seen = 1;
   rf->phase_fiducial = 1.25;
   rf->t_fiducial = 2.5;
+  if (rf->phase_reference)
+    eh_set_phase_reference(rf->phase_reference, rf->phase_fiducial);
   return track_stand_in(b, n);
 }
 long eh_exactDrift(eh_beam *b, long n, double L) {
@@ -234,7 +272,30 @@ unsigned long parseFiducialMode(char *mode) {
     return FID_MODE_TMEAN;
   if (!strcmp(mode, "first"))
     return FID_MODE_FIRST;
+  if (!strcmp(mode, "pmaximum"))
+    return FID_MODE_PMAX;
   return 0;
+}
+/* elegant's phase-reference table: one group */
+static struct {
+  long ref, set;
+  double phase;
+} ET;
+long get_phase_reference(double *phase, long ref) {
+  if (!ref)
+    return 0;
+  if (ref != ET.ref)
+    return REF_PHASE_NONEXISTENT;
+  if (!ET.set)
+    return REF_PHASE_NOT_SET;
+  *phase = ET.phase;
+  return REF_PHASE_RETURNED;
+}
+long set_phase_reference(long ref, double phase) {
+  ET.ref = ref;
+  ET.set = 1;
+  ET.phase = phase;
+  return 1;
 }
 
 /* ---- fixtures ---- */
@@ -245,6 +306,8 @@ static void reset(long n) {
   long k, q;
   memset(&M, 0, sizeof M);
   memset(&E, 0, sizeof E);
+  memset(&LT, 0, sizeof LT);
+  memset(&ET, 0, sizeof ET);
   for (k = 0; k < 64; k++)
     M.perm[k] = (int)k;
   for (k = 0; k < 16; k++) {
@@ -426,16 +489,40 @@ int ah_scenario_rfca(void) {
   n = gpu_trackRfCavityWithWakes(4, &rf, accp_h, &P, 1.0, 0, NULL, NULL, NULL, NULL, NULL, 0);
   if (n != 4 || M.rf != 1 || rf.fiducial_seen != 1 || rf.phase_fiducial != 1.25 || rf.t_fiducial != 2.5)
     return 2;
-  rf.phase_reference = 3;
-  if (element_on_gpu(&e))
-    return 3;
+  /* PHASE_REFERENCE 3: the first cavity of the group sets elegant's table,
+   * a second one tracks with that phase and keeps its own fiducial unset */
+  {
+    RFCA rf2 = rf;
+    rf.phase_reference = rf2.phase_reference = 3;
+    rf.fiducial_seen = 0;
+    rf2.fiducial_seen = 0;
+    rf2.phase_fiducial = 0;
+    if (!element_on_gpu(&e))
+      return 3;
+    n = gpu_trackRfCavityWithWakes(4, &rf, accp_h, &P, 1.0, 0, NULL, NULL, NULL, NULL, NULL, 0);
+    if (n != 4 || M.rf != 2 || E.rf != 0 || ET.ref != 3 || !ET.set || ET.phase != 1.25)
+      return 4;
+    n = gpu_trackRfCavityWithWakes(4, &rf2, accp_h, &P, 1.0, 0, NULL, NULL, NULL, NULL, NULL, 0);
+    if (M.rf != 3 || M.linked != 1 || rf2.fiducial_seen != 0)
+      return 41;
+    /* elegant deleted its phase references (a new step): the library's copy
+     * is not used */
+    ET.set = 0;
+    rf2.phase_fiducial = 0;
+    n = gpu_trackRfCavityWithWakes(4, &rf2, accp_h, &P, 1.0, 0, NULL, NULL, NULL, NULL, NULL, 0);
+    if (M.linked != 1 || rf2.fiducial_seen != 1 || !ET.set)
+      return 42;
+  }
+  rf.phase_reference = 0;
+  /* FIRST: the library fails as elegant's own routine does, which then reports it */
+  rf.fiducial = "first";
   n = gpu_trackRfCavityWithWakes(4, &rf, accp_h, &P, 1.0, 0, NULL, NULL, NULL, NULL, NULL, 0);
   if (E.rf != 1)
-    return 4;
-  rf.phase_reference = 0;
-  rf.fiducial = "first";
-  if (element_on_gpu(&e))
     return 5;
+  rf.fiducial = "pmaximum";
+  n = gpu_trackRfCavityWithWakes(4, &rf, accp_h, &P, 1.0, 0, NULL, NULL, NULL, NULL, NULL, 0);
+  if (M.last_rf_mode != EH_FID_PMAX || E.rf != 1)
+    return 51;
   rf.fiducial = "light";
   if (!element_on_gpu(&e))
     return 6;

@@ -112,7 +112,7 @@ extern "C" long bh_tables(EH_ELEMENT_LIST *const *beamlines, long nvar, long ne,
       if (!rf_pending(beamlines[v] + er))
         continue;
       EH_RFCA *rf = (EH_RFCA *)beamlines[v][er].p_elem;
-      r = rf_take_fiducial(rf, 0.0, [&](double, double *t0) {
+      r = rf_take_fiducial(rf, 0.0, [&](int, double, double *t0) {
         *t0 = t_set[v];
         return (long)EH_OK;
       });
@@ -280,7 +280,7 @@ extern "C" long bh_track(const double *flat, const long *counts, long nvar, EH_E
         double zEnd = 0;
         for (long k = 0; k <= er; k++)
           zEnd += elem_length(beamlines[v] + k);
-        r = rf_take_fiducial((EH_RFCA *)beamlines[v][er].p_elem, zEnd, [&](double, double *) {
+        r = rf_take_fiducial((EH_RFCA *)beamlines[v][er].p_elem, zEnd, [&](int, double, double *) {
           return fail(EH_ERR_UNSUPPORTED, "bh_track: TMEAN fiducial needs the device path");
         });
         if (r < 0)

@@ -40,7 +40,7 @@ OPAQUE = ["RUN", "CHARGE", "WAKE", "TRWAKE", "LSCKICK", "RFCW", "MATTER", "LSCDR
 PROTOS = ["bombElegant", "multipole_tracking2", "track_through_csbend", "exactDrift", "track_particles",
           "limit_amplitudes", "elimit_amplitudes", "removeInvalidParticles", "offset_beam", "set_central_momentum",
           "findFiducialTime", "do_match_energy", "compute_centroids", "center_beam", "accumulate_beam_sums",
-          "imposeApertureData", "parseFiducialMode"]
+          "imposeApertureData", "parseFiducialMode", "get_phase_reference", "set_phase_reference"]
 
 
 def _strip_comments(text):
@@ -139,7 +139,7 @@ def test_every_field_copied_refused_or_listed(struct):
 def _defines(text):
     out = []
     for pat in (r"^#define T_\w+\s+\d+\s*$", r"^#define N_TYPES\s+\d+\s*$", r"^#define MATRIX_TRACKING\s.*$",
-                r"^#define FID_MODE_\w+\s.*$", r"^#define IS_MONITOR\(type\).*$"):
+                r"^#define FID_MODE_\w+\s.*$", r"^#define IS_MONITOR\(type\).*$", r"^#define REF_PHASE_\w+\s.*$"):
         out += re.findall(pat, text, flags=re.M)
     return out
 
