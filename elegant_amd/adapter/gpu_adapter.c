@@ -302,13 +302,13 @@ static const char *const csbend_untracked[] __attribute__((unused)) = {
     "fseCorrectionValue", "fseCorrectionPathError", "refTrajectoryChangeSet", "refLength", "refAngle",
     "refTrajectoryChange", "refSlices", "photonFileActive", "SDDSphotons", "expansionOrder_ref", "nonlinear_ref",
     "expansionOrder1_ref", "hasSkew_ref", "hasNormal_ref", "h_ref", "b_ref", "c_ref", "Fx_xy_ref", "Fy_xy_ref",
-    "isr1Particle", "edge_kick_limit", "fringeInt", NULL};
+    "isr1Particle", "fringeInt", NULL};
 
 /* CSBEND, src/track.h:2462-2502, as track_through_csbend reads it
  * (src/csbend.c:654-1000).  e[], h[], edge_effects[], fint[] are read through
- * e1Index/e2Index (a reversed element swaps them, :859-876); the edge kick limit
- * and Lindberg's fringe integrals only act with edge methods the library does
- * not carry (EDGE*_EFFECTS > 1), which are refused; the b/c arrays and the
+ * e1Index/e2Index (a reversed element swaps them, :859-876); Lindberg's fringe
+ * integrals only act with edge methods the library does not carry
+ * (EDGE*_EFFECTS 2, 4, 5), which are refused; the b/c arrays and the
  * *_ref fields are caches of the strengths above; the reference-trajectory
  * fields act only with REFERENCE_CORRECTION and the FSE correction value only
  * with FSE_CORRECTION, both refused */
@@ -319,9 +319,10 @@ static int fill_csbend(const CSBEND *cb, EH_CSBEND *d) {
     return 0;
   if (cb->epitch != 0 || cb->eyaw != 0 || cb->isr || cb->distributionBasedRadiation || cb->includeOpeningAngle ||
       !no_file(cb->photonOutputFile) || cb->referenceCorrection || cb->trackingMatrix || cb->fseCorrection ||
-      cb->kick_limit_scaling || cb->edgeFlip)
+      cb->edgeFlip)
     return 0;
-  if (cb->edge_effects[i1] > 1 || cb->edge_effects[i2] > 1 || cb->edge_effects[i1] < 0 || cb->edge_effects[i2] < 0)
+  if (cb->edge_effects[i1] < 0 || cb->edge_effects[i2] < 0 || cb->edge_effects[i1] == 2 ||
+      cb->edge_effects[i2] == 2 || cb->edge_effects[i1] > 3 || cb->edge_effects[i2] > 3)
     return 0;
   if (cb->xReference != 0 || cb->f1 || cb->f2 || cb->f3 || cb->f4 || cb->f5 || cb->f6 || cb->f7 || cb->f8 || cb->g1 ||
       cb->g2 || cb->g3 || cb->g4 || cb->g5 || cb->g6 || cb->g7 || cb->g8)
@@ -379,6 +380,9 @@ static int fill_csbend(const CSBEND *cb, EH_CSBEND *d) {
   d->malignMethod = cb->malignMethod;
   d->edgeFlags = cb->edgeFlags;
   d->synch_rad = cb->synch_rad;
+  d->kick_limit_scaling = (int32_t)(cb->kick_limit_scaling != 0);
+  d->edge1_kick_limit = cb->edge_kick_limit[i1];
+  d->edge2_kick_limit = cb->edge_kick_limit[i2];
   return 1;
 }
 

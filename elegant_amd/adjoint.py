@@ -21,9 +21,14 @@ from . import _lib
 from . import structs as S
 from . import tracking
 
-KIND_FIELD = {(S.T_KQUAD, "K"): "k1", (S.T_KSEXT, "K"): "k2", (S.T_KOCT, "K"): "k3", (S.T_KQUSE, "K"): "k1"}
-KINDS = ("K", "DX", "DY")
+KIND_FIELD = {(S.T_KQUAD, "K"): "k1", (S.T_KSEXT, "K"): "k2", (S.T_KOCT, "K"): "k3", (S.T_KQUSE, "K"): "k1",
+              (S.T_CSBEND, "K"): "k1"}
+# gradient slots per element, in the order of the library's param_slots rows
+KINDS = ("K", "DX", "DY", "FSE")
 MULTIPOLES = (S.T_KQUAD, S.T_KSEXT, S.T_KOCT, S.T_KQUSE)
+# the kinds each element type can be differentiated with respect to (a bend's
+# K is its K1)
+ALLOWED = {**{t: ("K", "DX", "DY") for t in MULTIPOLES}, S.T_CSBEND: ("K", "FSE")}
 
 
 class ParamSpec:
@@ -84,15 +89,27 @@ class ParamSpec:
                     labels.append("%s/%d.%s" % (nm, g, kind))
         return cls(slots, labels)
 
+    @classmethod
+    def bends(cls, lat, kinds=("K", "FSE")):
+        """K1 and FSE of every CSBEND with an angle"""
+        slots, names = [], []
+        for i, e in enumerate(lat.elements):
+            if e.type == S.T_CSBEND and e.params.get("ANGLE", 0):
+                for k in kinds:
+                    slots.append([(i, k)])
+                    names.append("%s#%d.%s" % (e.name, i, "K1" if k == "K" else k))
+        return cls(slots, names)
+
     def __add__(self, other):
         return ParamSpec(self.slots + other.slots, self.names + other.names)
 
     def slot_array(self, lat):
-        arr = np.full((len(lat), 3), -1, dtype=np.int32)
+        arr = np.full((len(lat), len(KINDS)), -1, dtype=np.int32)
         for q, occ in enumerate(self.slots):
             for (i, kind) in occ:
-                if lat.elements[i].type not in MULTIPOLES:
-                    raise ValueError("element %d (%s) is not a KQUAD/KSEXT/KOCT/KQUSE" % (i, lat.elements[i].name))
+                if kind not in ALLOWED.get(lat.elements[i].type, ()):
+                    raise ValueError("element %d (%s): no %s gradient for %s" % (
+                        i, lat.elements[i].name, kind, S.TYPE_NAMES[lat.elements[i].type]))
                 k = KINDS.index(kind)
                 if arr[i, k] >= 0:
                     raise ValueError("element %d %s appears in two slots" % (i, kind))
@@ -115,11 +132,12 @@ class ParamSpec:
     def apply(self, lat, theta):
         """write values into the lattice (element params and structs)"""
         theta = np.asarray(theta, dtype=np.float64)
-        param_name = {"K": None, "DX": "DX", "DY": "DY"}
+        param_name = {"K": None, "DX": "DX", "DY": "DY", "FSE": "FSE"}
+        strength = {S.T_KQUAD: "K1", S.T_KSEXT: "K2", S.T_KOCT: "K3", S.T_KQUSE: "K1", S.T_CSBEND: "K1"}
         for q, occ in enumerate(self.slots):
             for (i, kind) in occ:
                 e = lat.elements[i]
-                name = param_name[kind] or {S.T_KQUAD: "K1", S.T_KSEXT: "K2", S.T_KOCT: "K3", S.T_KQUSE: "K1"}[e.type]
+                name = param_name[kind] or strength[e.type]
                 lat.set_param(i, name, float(theta[q]))
 
 

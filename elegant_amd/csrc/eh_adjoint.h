@@ -22,9 +22,10 @@
 
 #include "eh_device.h"
 
-/* parameter gradients of one element: d/d(strength), d/dDX, d/dDY */
+/* parameter gradients of one element: d/d(strength), d/dDX, d/dDY, d/dFSE
+ * (the strength of a bend is its K1) */
 struct eh_pgrad {
-  double k, dx, dy;
+  double k, dx, dy, fse;
 };
 
 /* ---- slopes <-> momenta --------------------------------------------------- */
@@ -335,10 +336,12 @@ EH_HD void eh_csb_fields_d(const eh_desc &d, const eh_fterm *__restrict__ terms,
 }
 
 /* edge kick xp += a(dp) x, yp -= b(dp) y with a = tan(e)/rho, b = tan(e - psi/(1+dp))/rho,
- * rho = (1+dp) rho_actual (src/csbend.c:1126-1134, :1355-1363); undo and pull back */
+ * rho = (1+dp) rho_actual (src/csbend.c:1126-1134, :1355-1363); undo and pull back.
+ * With fse1 = 1/(1 + FSE + FSE_DIPOLE), 1/rho_actual and psi both change by
+ * the factor fse1 per unit FSE; their effect is added to gfse. */
 EH_HD void eh_adj_edge(double &x, double &xp, double &y, double &yp, double dp, double &lx,
                                             double lxp, double &ly, double lyp, double &ldp, double tan_e, double e,
-                                            double psi, double rho_act) {
+                                            double psi, double rho_act, double fse1 = 0, double *gfse = nullptr) {
   double op = 1 + dp;
   double rho = op * rho_act;
   double a = tan_e / rho;
@@ -352,11 +355,32 @@ EH_HD void eh_adj_edge(double &x, double &xp, double &y, double &yp, double dp, 
   lx += a * lxp;
   ly += -b * lyp;
   ldp += lxp * x * a_dp - lyp * y * b_dp;
+  if (gfse) {
+    const double a_f = a * fse1;
+    const double b_f = (1 + tu * tu) * (-psi * fse1 / op) / rho + b * fse1;
+    *gfse += lxp * x * a_f - lyp * y * b_f;
+  }
 }
 
-EH_HD void eh_adj_csbend(eh_p6 &z, eh_p6 &l, const eh_desc &d,
-                                              const eh_fterm *__restrict__ terms) {
+/* a field component from a term list: sum of coef x^i y^j */
+EH_HD double eh_terms_eval(const eh_fterm *__restrict__ t, int n, double x, double y) {
+  double F = 0;
+  for (int k = 0; k < n; k++) {
+    double v = t[k].coef;
+    for (int m = 0; m < t[k].i; m++)
+      v *= x;
+    for (int m = 0; m < t[k].j; m++)
+      v *= y;
+    F += v;
+  }
+  return F;
+}
+
+EH_HD void eh_adj_csbend(eh_p6 &z, eh_p6 &l, const eh_desc &d, const eh_fterm *__restrict__ terms,
+                          eh_pgrad &g) {
   const double rho0 = d.rho0, rho_a = d.rho_actual;
+  const bool want_k = d.adj_pk >= 0, want_f = d.adj_pfse >= 0;
+  double *gf = want_f ? &g.fse : nullptr;
   /* exit offset, then rotation back into the bend frame */
   eh_adj_offset(z, l, d.dxf, d.dyf, d.dzf);
   {
@@ -367,7 +391,7 @@ EH_HD void eh_adj_csbend(eh_p6 &z, eh_p6 &l, const eh_desc &d,
   double lX = l.x, lY = l.y, lxp = l.xp, lyp = l.yp, lDP = l.dp;
   const double ls = l.s;
   if (d.csb_edge2)
-    eh_adj_edge(X, xp, Y, yp, DP, lX, lxp, lY, lyp, lDP, d.tan_e2, d.e2, d.psi2, rho_a);
+    eh_adj_edge(X, xp, Y, yp, DP, lX, lxp, lY, lyp, lDP, d.tan_e2, d.e2, d.psi2, rho_a, d.adj_fse1, gf);
   /* slopes from the canonical momenta at the exit */
   double QX, QY;
   eh_s2m(xp, yp, DP, QX, QY);
@@ -389,6 +413,18 @@ EH_HD void eh_adj_csbend(eh_p6 &z, eh_p6 &l, const eh_desc &d,
         double c = ds * irho_a;
         lX += c * (-lQX * (Fy * irho0 + h1 * Fy_x) + lQY * (Fx * irho0 + h1 * Fx_x));
         lY += c * (-lQX * h1 * Fy_y + lQY * h1 * Fx_y);
+        if (want_k) {
+          /* the field per unit K1 */
+          const double Fxk = eh_terms_eval(terms + d.adj_dk_fx0, d.adj_dk_nfx, X, Y);
+          const double Fyk = eh_terms_eval(terms + d.adj_dk_fy0, d.adj_dk_nfy, X, Y);
+          g.k += c * h1 * (-lQX * Fyk + lQY * Fxk);
+        }
+        if (want_f) {
+          /* the field per unit FSE, and 1/rho_actual growing by the factor adj_fse1 */
+          const double Fxf = eh_terms_eval(terms + d.adj_df_fx0, d.adj_df_nfx, X, Y);
+          const double Fyf = eh_terms_eval(terms + d.adj_df_fy0, d.adj_df_nfy, X, Y);
+          g.fse += c * h1 * (-lQX * (Fyf + Fy * d.adj_fse1) + lQY * (Fxf + Fx * d.adj_fse1));
+        }
         QX += c * h1 * Fy;
         QY -= c * h1 * Fx;
       }
@@ -429,7 +465,7 @@ EH_HD void eh_adj_csbend(eh_p6 &z, eh_p6 &l, const eh_desc &d,
   lyp = 0;
   eh_adj_s2m(xp, yp, DP, lQX, lQY, lxp, lyp, lDP);
   if (d.csb_edge1)
-    eh_adj_edge(X, xp, Y, yp, DP, lX, lxp, lY, lyp, lDP, d.tan_e1, d.e1, d.psi1, rho_a);
+    eh_adj_edge(X, xp, Y, yp, DP, lX, lxp, lY, lyp, lDP, d.tan_e1, d.e1, d.psi1, rho_a, d.adj_fse1, gf);
   z.x = X;
   z.y = Y;
   z.xp = xp;

@@ -71,6 +71,9 @@ typedef struct {
   int32_t ell_xe[2], ell_ye[2];
   double ell_a2[2], ell_b2[2];
   int32_t ell_any;
+  /* the descriptor needs the full instantiation of the maps: an elliptical
+   * aperture, a bend edge method 3 or an edge kick limit */
+  int32_t full_only;
   double z_start, z_end; /* last_z and z of do_tracking for this element */
   double Po;
 
@@ -97,7 +100,9 @@ typedef struct {
   int32_t csb_nslices;
   int32_t csb_field; /* hasSkew || hasNormal */
   int32_t csb_nfx, csb_nfy;
-  int32_t csb_edge1, csb_edge2;
+  int32_t csb_edge1, csb_edge2;  /* the edge applies (EDGE*_EFFECTS 1 or 3) */
+  int32_t csb_esimple1, csb_esimple2; /* EDGE*_EFFECTS 3: applySimpleDipoleEdgeKick */
+  double csb_klim1, csb_klim2;   /* edge x' kick limits (<= 0: none) */
   double rho0, rho_actual;
   int32_t csb_recip;        /* 1: divide by rho0 and rho_actual through the reciprocals below */
   double rho0_rh, rho0_rl, rhoa_rh, rhoa_rl; /* RN(1/rho) and RN(1/rho - RN(1/rho)) */
@@ -142,6 +147,14 @@ typedef struct {
   int32_t adj_anchor;              /* slot of the stored entry state (RF cavity), -1 if none */
   double adj_kfac[EH_MAXSUB];      /* d kick[step][0] / d strength */
   double adj_sin_dshr[EH_MAXSUB];  /* sin of the CSBEND sub-step angles */
+  /* bends as parameters: adj_pk is the slot of K1, adj_pfse that of FSE; the
+   * field per unit K1 and per unit FSE as term lists (i, j, coefficient) in
+   * the term pool; adj_fse1 = 1 / (1 + FSE + FSE_DIPOLE), the relative change
+   * of 1/rho_actual and of the edge angles psi per unit FSE */
+  int32_t adj_pfse;
+  int32_t adj_dk_fx0, adj_dk_nfx, adj_dk_fy0, adj_dk_nfy;
+  int32_t adj_df_fx0, adj_df_nfx, adj_df_fy0, adj_df_nfy;
+  double adj_fse1;
 } eh_desc;
 
 typedef struct {
@@ -580,12 +593,13 @@ EH_HD void eh_mface2(eh_p6 &p, const eh_desc &d, bool s_too) {
  * Returns 0 if the particle survives, 1 if lost, 2 if lost before the first
  * step (its loss position is z_start plus the previous particle's dzLoss).  A
  * lost particle keeps s = z_start + dzLoss without the exit offset's path-length
- * term.  RAD: the element radiates (d.rad); ELL: the elliptical aperture check is
- * compiled in (the full instantiation of the maps; the lean one has neither). */
-template <bool RAD, bool ELL = RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh_desc &d, int pi) {
+ * term.  RAD: the element radiates (d.rad); FULL: the options only the full
+ * instantiation of the maps carries (elliptical apertures; for bends also edge
+ * method 3 and edge kick limits) are compiled in; the lean one has neither. */
+template <bool RAD, bool FULL = RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh_desc &d, int pi) {
   eh_mface1(p, d);
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
-  const int ell = ELL ? d.ap_ell[pi] : 0;
+  const int ell = FULL ? d.ap_ell[pi] : 0;
   double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, dp = p.dp, qx, qy;
   double beta0 = 0;
   if constexpr (RAD) {
@@ -617,7 +631,7 @@ template <bool RAD, bool ELL = RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh_
     } else {
       eh_pow4 w;
       for (int ik = 0; ik < d.nslices && !lost; ik++) {
-        if (!eh_ap_ok<ELL>(x + d.mdx, y + d.mdy, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi],
+        if (!eh_ap_ok<FULL>(x + d.mdx, y + d.mdy, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi],
                            d.ell_ye[pi])) {
           lost = true;
           how = 1;
@@ -696,7 +710,7 @@ template <bool RAD, bool ELL = RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh_
         }
       }
       if (!lost) {
-        if (!eh_ap_ok<ELL>(x + d.mdx, y + d.mdy, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi],
+        if (!eh_ap_ok<FULL>(x + d.mdx, y + d.mdy, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi],
                            d.ell_ye[pi])) {
           lost = true;
           how = 1;
@@ -1148,13 +1162,32 @@ EH_HD void eh_edge2(double &x, double &xp, double &y, double &yp, double rho, do
   yp = yp0 + R43 * y0 + T441 * yp0 * x0 + T431 * x0 * y0 + T432 * xp0 * y0;
 }
 
+/* EDGE_EFFECTS 3, applySimpleDipoleEdgeKick (src/csbend.c:5664-5690): the
+ * first-order edge kick applied to the canonical momenta (dq_x = tan(e)/rho x,
+ * limited to klim if positive; dq_y = -tan(e - psi/(1+dp))/rho y), with rho the
+ * bend's rho_actual, not scaled by 1 + dp */
+EH_HD void eh_edge_simple(double &xp, double &yp, double x, double y, double dp, double rho, double tan_e, double e,
+                          double psi, double klim) {
+  double f = (1 + dp) / eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
+  double qx = xp * f, qy = yp * f;
+  double dqx = tan_e / rho * x;
+  if (klim > 0 && eh_fabs(dqx) > klim)
+    dqx = (dqx < 0 ? -1.0 : 1.0) * klim;
+  double dqy = -eh_tan(e - psi / (1 + dp)) / rho * y;
+  qx += dqx;
+  qy += dqy;
+  f = 1 / eh_sqrt(eh_sqr(1 + dp) - eh_sqr(qx) - eh_sqr(qy));
+  xp = qx * f;
+  yp = qy * f;
+}
+
 /* CSBEND: track_through_csbend (src/csbend.c:1088-1403) for one particle, with
  * integrate_csbend_ordn (src/csbend.c:1459-1720) and the canonical conversions
- * (:1436-1454).  Returns true if lost.  RAD and ELL as for eh_multipole_t. */
-template <bool RAD, bool ELL = RAD>
+ * (:1436-1454).  Returns true if lost.  RAD and FULL as for eh_multipole_t. */
+template <bool RAD, bool FULL = RAD>
 EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms) {
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
-  const int ell = ELL ? d.ap_ell[pi] : 0;
+  const int ell = FULL ? d.ap_ell[pi] : 0;
   const double rho0 = d.rho0, rho_actual = d.rho_actual;
   double x, xp, y, yp, s, dp;
   p.s += d.dzi * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
@@ -1169,8 +1202,12 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   const double dp0 = dp;
   if (d.csb_edge1) {
     double rho = (1 + dp) * rho_actual;
-    if (!d.csb_eorder2) {
+    if (FULL && d.csb_esimple1) {
+      eh_edge_simple(xp, yp, x, y, dp, rho_actual, d.tan_e1, d.e1, d.psi1, d.csb_klim1);
+    } else if (!d.csb_eorder2) {
       double delta_xp = d.tan_e1 / rho * x;
+      if (FULL && d.csb_klim1 > 0 && eh_fabs(delta_xp) > d.csb_klim1)
+        delta_xp = (delta_xp < 0 ? -1.0 : 1.0) * d.csb_klim1;
       xp += delta_xp;
       yp -= eh_tan(d.e1 - d.psi1 / (1 + dp)) / rho * y;
     } else {
@@ -1203,7 +1240,7 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   double dist = 0, dz_lost = 0;
   bool lost = false;
   for (int i = 0; i < d.csb_nslices && !lost; i++) {
-    if (!eh_ap_ok<ELL>(X, Y, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi], d.ell_ye[pi])) {
+    if (!eh_ap_ok<FULL>(X, Y, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi], d.ell_ye[pi])) {
       lost = true;
       break;
     }
@@ -1271,7 +1308,7 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   }
   if (!lost) {
     dz_lost = d.csb_dzl;
-    if (!eh_ap_ok<ELL>(X, Y, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi], d.ell_ye[pi]))
+    if (!eh_ap_ok<FULL>(X, Y, apx, apy, d.ap_rtilt, d.ap_cs0, d.ap_cs1, ell, d.ell_xe[pi], d.ell_ye[pi]))
       lost = true;
     else
       S4 += dist;
@@ -1319,8 +1356,12 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   dp = DP;
   if (d.csb_edge2) {
     double rho = (1 + dp) * rho_actual;
-    if (!d.csb_eorder2) {
+    if (FULL && d.csb_esimple2) {
+      eh_edge_simple(xp, yp, x, y, dp, rho_actual, d.tan_e2, d.e2, d.psi2, d.csb_klim2);
+    } else if (!d.csb_eorder2) {
       double delta_xp = d.tan_e2 / rho * x;
+      if (FULL && d.csb_klim2 > 0 && eh_fabs(delta_xp) > d.csb_klim2)
+        delta_xp = (delta_xp < 0 ? -1.0 : 1.0) * d.csb_klim2;
       xp += delta_xp;
       yp -= eh_tan(d.e2 - d.psi2 / (1 + dp)) / rho * y;
     } else {
@@ -1488,7 +1529,7 @@ EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fter
   bool lost = false;
   int flag = 0;
   if constexpr (!RAD) {
-    if (d.ell_any)
+    if (d.full_only)
       return EH_STAGE_REFUSED;
   }
   switch (d.type) {
@@ -1580,7 +1621,7 @@ EH_HD void eh_loss_words(int32_t lcode, int nelem, int p0, int32_t event_base, i
 /* whether a descriptor can run through the lean map (RAD = false): kernels for
  * lines without radiating elements and without MULT, ROTATE, MALIGN or ENERGY
  * carry none of that code (and none of its registers) */
-EH_HD bool eh_lean_capable(const eh_desc &d) { return !d.rad && d.type < 10 && !d.ell_any; }
+EH_HD bool eh_lean_capable(const eh_desc &d) { return !d.rad && d.type < 10 && !d.full_only; }
 
 
 EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms) {

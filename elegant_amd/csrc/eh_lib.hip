@@ -338,23 +338,10 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
   return EH_OK;
 }
 
-static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::vector<eh_fterm> &pool) {
-  if (cb->angle == 0) {
-    d->type = EH_T_EDRIFT;
-    d->length = cb->length;
-    return EH_OK;
-  }
-  const double *driftFrac, *kickFrac;
-  int nsub;
-  if (!integ_tables(cb->integration_order, &driftFrac, &kickFrac, &nsub))
-    return fail(EH_ERR_UNSUPPORTED, "CSBEND integration_order is invalid");
-  if (cb->malignMethod != 0 || cb->etilt != 0 || cb->expandHamiltonian)
-    return fail(EH_ERR_UNSUPPORTED, "CSBEND option not supported");
-  if ((cb->edge1_effects != 0 && cb->edge1_effects != 1) || (cb->edge2_effects != 0 && cb->edge2_effects != 1))
-    return fail(EH_ERR_UNSUPPORTED, "CSBEND edge method not supported");
-  if (cb->nSlices < 1)
-    return fail(EH_ERR_UNSUPPORTED, "invalid number of steps (integrate_csbend_ordn)");
-  double b[9], cc[9];
+/* the bend's b[] and c[] coefficients before the negative-angle flip
+ * (src/csbend.c:800-857: K1..K8 times rho0 or B1..B8, the FSE adjustments of
+ * the quadrupole and sextupole terms, the steering kicks) */
+static void csbend_coefficients(const EH_CSBEND *cb, double b[9], double cc[9]) {
   double rho0 = cb->length / cb->angle;
   b[0] = 0;
   if (cb->use_bn) {
@@ -384,6 +371,28 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
   cc[2] *= (1 + cb->fse) / (1 + cb->fse + cb->fseDipole);
   b[0] = cb->xKick / cb->angle;
   cc[0] = cb->yKick / cb->angle;
+}
+
+static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::vector<eh_fterm> &pool) {
+  if (cb->angle == 0) {
+    d->type = EH_T_EDRIFT;
+    d->length = cb->length;
+    return EH_OK;
+  }
+  const double *driftFrac, *kickFrac;
+  int nsub;
+  if (!integ_tables(cb->integration_order, &driftFrac, &kickFrac, &nsub))
+    return fail(EH_ERR_UNSUPPORTED, "CSBEND integration_order is invalid");
+  if (cb->malignMethod != 0 || cb->etilt != 0 || cb->expandHamiltonian)
+    return fail(EH_ERR_UNSUPPORTED, "CSBEND option not supported");
+  if ((cb->edge1_effects != 0 && cb->edge1_effects != 1 && cb->edge1_effects != 3) ||
+      (cb->edge2_effects != 0 && cb->edge2_effects != 1 && cb->edge2_effects != 3))
+    return fail(EH_ERR_UNSUPPORTED, "CSBEND edge method not supported");
+  if (cb->nSlices < 1)
+    return fail(EH_ERR_UNSUPPORTED, "invalid number of steps (integrate_csbend_ordn)");
+  double b[9], cc[9];
+  double rho0 = cb->length / cb->angle;
+  csbend_coefficients(cb, b, cc);
   double angle, e1, e2, tilt, etilt;
   if (cb->angle < 0) {
     angle = -cb->angle;
@@ -531,8 +540,21 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
     d->csb_ds[j] = s * kickFrac[j];
   }
   d->csb_dzl = cb->nSlices * s;
-  d->csb_edge1 = (cb->edgeFlags & EH_BEND_EDGE1_EFFECTS) && cb->edge1_effects == 1;
-  d->csb_edge2 = (cb->edgeFlags & EH_BEND_EDGE2_EFFECTS) && cb->edge2_effects == 1;
+  d->csb_edge1 = (cb->edgeFlags & EH_BEND_EDGE1_EFFECTS) && (cb->edge1_effects == 1 || cb->edge1_effects == 3);
+  d->csb_edge2 = (cb->edgeFlags & EH_BEND_EDGE2_EFFECTS) && (cb->edge2_effects == 1 || cb->edge2_effects == 3);
+  d->csb_esimple1 = d->csb_edge1 && cb->edge1_effects == 3;
+  d->csb_esimple2 = d->csb_edge2 && cb->edge2_effects == 3;
+  /* edge kick limits, src/csbend.c:932-937 (they act in methods 1 with
+   * EDGE_ORDER 1, and 3) */
+  d->csb_klim1 = cb->edge1_kick_limit;
+  d->csb_klim2 = cb->edge2_kick_limit;
+  if (cb->kick_limit_scaling) {
+    d->csb_klim1 *= rho0 / rho_actual;
+    d->csb_klim2 *= rho0 / rho_actual;
+  }
+  d->full_only = d->full_only || d->csb_esimple1 || d->csb_esimple2 ||
+                 (d->csb_edge1 && d->csb_klim1 > 0 && (d->csb_esimple1 || cb->edge_order <= 1)) ||
+                 (d->csb_edge2 && d->csb_klim2 > 0 && (d->csb_esimple2 || cb->edge_order <= 1));
   d->csb_e1r = (cb->edgeFlags & EH_BEND_EDGE1_EFFECTS) && e1 != 0;
   d->csb_e2r = (cb->edgeFlags & EH_BEND_EDGE2_EFFECTS) && e2 != 0;
   d->e1 = e1;
@@ -719,7 +741,7 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
   std::memset(d, 0, sizeof *d);
   d->type = e->type;
   d->counted = !(e->type == EH_T_MARK || e->type == EH_T_MAXAMP);
-  d->adj_pk = d->adj_pdx = d->adj_pdy = d->adj_anchor = -1;
+  d->adj_pk = d->adj_pdx = d->adj_pdy = d->adj_pfse = d->adj_anchor = -1;
   d->Po = c.Po;
   d->z_start = c.z_start;
   d->z_end = c.z_end;
@@ -747,6 +769,7 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
       d->ell_any = d->ell_any || both;
     }
   }
+  d->full_only = d->ell_any;
   d->extrap = e->type == EH_T_DRIF;
   switch (e->type) {
   case EH_T_MARK:
@@ -3189,7 +3212,7 @@ static long batch_tables(EH_ELEMENT_LIST *const *beamlines, long nvar, long ne, 
       if (rf_pending(bl + e)) {
         std::memset(&d, 0, sizeof d);
         d.type = EH_T_MARK;
-        d.adj_pk = d.adj_pdx = d.adj_pdy = d.adj_anchor = -1;
+        d.adj_pk = d.adj_pdx = d.adj_pdy = d.adj_pfse = d.adj_anchor = -1;
         r = EH_OK;
       } else
         r = build_desc(&d, bl + e, ctx[e], ft);
@@ -3564,7 +3587,10 @@ __device__ __forceinline__ eh_p6 load_anchor(const double *anchors, long idx, lo
  * zb/lam hold the state and adjoint at the end of turn t1-1 on entry and at
  * the start of turn t0 on exit.  Survivor i's anchors sit at column arow[i]
  * of turn (t - ta0). */
-__global__ void __launch_bounds__(EH_ADJ_BLOCK) k_adj_bwd(double *zb, double *lam, long m, const int32_t *arow,
+/* three wavefronts per SIMD: at most 168 VGPRs (the RF cavity's dual-number
+ * Jacobian and the bend's parameter gradients sit at that limit) */
+__global__ void __launch_bounds__(EH_ADJ_BLOCK) __attribute__((amdgpu_waves_per_eu(3)))
+k_adj_bwd(double *zb, double *lam, long m, const int32_t *arow,
                                                            const double *__restrict__ anchors, int n_anchor,
                                                            long a_ld, const eh_desc *__restrict__ descs,
                                                            const eh_fterm *__restrict__ terms, int ne, int t0,
@@ -3602,7 +3628,7 @@ __global__ void __launch_bounds__(EH_ADJ_BLOCK) k_adj_bwd(double *zb, double *la
       case EH_T_KSEXT:
       case EH_T_KOCT:
       case EH_T_KQUSE: {
-        eh_pgrad g = {0, 0, 0};
+        eh_pgrad g = {0, 0, 0, 0};
         eh_adj_multipole(z, l, d, g);
         if (d.adj_pk >= 0) {
           double v = wave_sum(g.k);
@@ -3621,9 +3647,21 @@ __global__ void __launch_bounds__(EH_ADJ_BLOCK) k_adj_bwd(double *zb, double *la
         }
         break;
       }
-      case EH_T_CSBEND:
-        eh_adj_csbend(z, l, d, terms);
+      case EH_T_CSBEND: {
+        eh_pgrad g = {0, 0, 0, 0};
+        eh_adj_csbend(z, l, d, terms, g);
+        if (d.adj_pk >= 0) {
+          double v = wave_sum(g.k);
+          if (lane == 0)
+            g_lds[d.adj_pk] += v;
+        }
+        if (d.adj_pfse >= 0) {
+          double v = wave_sum(g.fse);
+          if (lane == 0)
+            g_lds[d.adj_pfse] += v;
+        }
         break;
+      }
       case EH_T_RFCA: {
         eh_p6 zin = load_anchor(anchors, tb + (e == 0 ? 0 : d.adj_anchor), a_ld, col);
         eh_adj_rfca(z, l, zin, d);
@@ -3768,6 +3806,71 @@ static long adj_timed(eh_adjoint *a, double *acc) {
 }
 
 /* descriptors of a beamline as one do_tracking call sees them */
+/* gradient slots per element: strength (K of the kick multipoles, K1 of a
+ * bend), DX, DY (kick multipoles), FSE (bends) */
+#define EH_ADJ_KINDS 4
+
+/* the field of a bend per unit K1 and per unit FSE, for the adjoint's
+ * parameter gradients: the field table is linear in the b[] coefficients, so
+ * each is the table of the coefficients' derivatives (src/csbend.c:800-857:
+ * b1 = K1 rho0 (1 + FSE + FSE_QUADRUPOLE) / (1 + FSE + FSE_DIPOLE), b2 = K2 rho0
+ * (1 + FSE) / (1 + FSE + FSE_DIPOLE), odd terms negated for a negative angle),
+ * at the bend's own expansion order, without the constant dipole term.  Appends
+ * the term lists to pool and points the descriptor at them. */
+static long adj_bend_params(eh_desc &d, const EH_CSBEND *cb, std::vector<eh_fterm> &pool) {
+  d.adj_dk_nfx = d.adj_dk_nfy = d.adj_df_nfx = d.adj_df_nfy = 0;
+  d.adj_dk_fx0 = d.adj_dk_fy0 = d.adj_df_fx0 = d.adj_df_fy0 = 0;
+  if (d.adj_pk < 0 && d.adj_pfse < 0)
+    return EH_OK;
+  if (d.type != EH_T_CSBEND)
+    return fail(EH_ERR_UNSUPPORTED, "a CSBEND without angle has no K1 or FSE gradient");
+  if (d.adj_pk >= 0 && cb->use_bn)
+    return fail(EH_ERR_UNSUPPORTED, "CSBEND with USE_BN: K1 is not a parameter");
+  const double rho0s = cb->length / cb->angle, flip = cb->angle < 0 ? -1.0 : 1.0;
+  const double F = cb->fse, fd = cb->fseDipole, fq = cb->fseQuadrupole, den = 1 + F + fd;
+  double b[9], cc[9], bk[9], bf[9], zero[9];
+  csbend_coefficients(cb, b, cc);
+  for (int j = 0; j < 9; j++)
+    bk[j] = bf[j] = zero[j] = 0;
+  if (cb->angle < 0)
+    for (int j = 1; j < 9; j += 2) {
+      b[j] *= -1;
+      cc[j] *= -1;
+    }
+  bk[1] = rho0s * ((1 + F + fq) / den) * flip;
+  const double s1 = cb->use_bn ? cb->b1 : cb->k1 * rho0s, s2 = cb->use_bn ? cb->b2 : cb->k2 * rho0s;
+  bf[1] = s1 * ((fd - fq) / (den * den)) * flip;
+  bf[2] = s2 * (fd / (den * den));
+  const double h = 1 / (cb->length / std::fabs(cb->angle)); /* as build_csbend */
+  double Fx[121], Fy[121];
+  long eo1 = eh_csbend_field_table(b, cc, h, cb->nonlinear, cb->expansionOrder, Fx, Fy, nullptr, nullptr);
+  if (eo1 < 0)
+    return eo1;
+  for (int which = 0; which < 2; which++) {
+    if ((which ? d.adj_pfse : d.adj_pk) < 0)
+      continue;
+    long r = eh_csbend_field_table(which ? bf : bk, zero, h, cb->nonlinear, (int)eo1 - 1, Fx, Fy, nullptr, nullptr);
+    if (r < 0)
+      return r;
+    Fy[0] -= 1; /* the dipole's own field, 1 - b[0] with b[0] = 0 */
+    for (int c = 0; c < 2; c++) {
+      (which ? (c ? d.adj_df_fy0 : d.adj_df_fx0) : (c ? d.adj_dk_fy0 : d.adj_dk_fx0)) = (int32_t)pool.size();
+      int32_t n = 0;
+      for (int i = 0; i < eo1; i++)
+        for (int j = 0; j < eo1 - i; j++) {
+          const double v = c ? Fy[i * 11 + j] : Fx[i * 11 + j];
+          if (v != 0) {
+            pool.push_back({i, j, v});
+            n++;
+          }
+        }
+      (which ? (c ? d.adj_df_nfy : d.adj_df_nfx) : (c ? d.adj_dk_nfy : d.adj_dk_nfx)) = n;
+    }
+  }
+  d.adj_fse1 = 1 / den;
+  return EH_OK;
+}
+
 static long build_line(EH_ELEMENT_LIST *bl, long ne, double Pc, std::vector<eh_desc> &hd,
                        std::vector<eh_fterm> &pool) {
   hd.assign(ne, eh_desc());
@@ -3845,17 +3948,28 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
                     e);
       if (d.rad)
         return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support SYNCH_RAD", e);
+      if (d.full_only)
+        return fail(EH_ERR_UNSUPPORTED,
+                    "element %ld: the adjoint does not support elliptical MAXAMP, CSBEND edge method 3 or edge kick "
+                    "limits", e);
       if (!eh_lean_capable(d))
         return fail(EH_ERR_STATE, "element %ld: not trackable by the adjoint's map", e);
       if ((t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE) && d.expand)
         return fail(EH_ERR_UNSUPPORTED, "element %ld: adjoint needs EXPAND_HAMILTONIAN=0", e);
-      for (int k = 0; k < 3; k++) {
-        int32_t slot = param_slots ? param_slots[3 * e + k] : -1;
+      const bool kick = t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE;
+      for (int k = 0; k < EH_ADJ_KINDS; k++) {
+        int32_t slot = param_slots ? param_slots[EH_ADJ_KINDS * e + k] : -1;
         if (slot < -1 || slot >= n_params)
           return fail(EH_ERR_ARG, "element %ld: parameter slot %d out of range", e, slot);
-        if (slot >= 0 && !(t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE))
-          return fail(EH_ERR_UNSUPPORTED, "element %ld: parameters only for KQUAD/KSEXT/KOCT/KQUSE", e);
-        (k == 0 ? d.adj_pk : k == 1 ? d.adj_pdx : d.adj_pdy) = slot;
+        if (slot >= 0 && !(kick && k <= 2) && !(t == EH_T_CSBEND && (k == 0 || k == 3)))
+          return fail(EH_ERR_UNSUPPORTED,
+                      "element %ld: parameters are K, DX, DY of KQUAD/KSEXT/KOCT/KQUSE and K1, FSE of CSBEND", e);
+        (k == 0 ? d.adj_pk : k == 1 ? d.adj_pdx : k == 2 ? d.adj_pdy : d.adj_pfse) = slot;
+      }
+      if (t == EH_T_CSBEND) {
+        long r2 = adj_bend_params(d, (const EH_CSBEND *)beamlines[v][e].p_elem, pool);
+        if (r2 < 0)
+          return r2;
       }
       if (t == EH_T_CSBEND && !pool.empty()) {
         const int32_t shift = (int32_t)a->pool.size();
@@ -3865,6 +3979,10 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
           d.csb_dx0 += shift;
           d.csb_dy0 += shift;
         }
+        d.adj_dk_fx0 += shift;
+        d.adj_dk_fy0 += shift;
+        d.adj_df_fx0 += shift;
+        d.adj_df_fy0 += shift;
       }
     }
     n_anchor = na;

@@ -67,7 +67,8 @@ class EH_CSBEND(C.Structure):
                                    "b1", "b2", "b3", "b4", "b5", "b6", "b7", "b8")] + \
                [(n, i32) for n in ("nSlices", "nonlinear", "edge1_effects", "edge2_effects", "edge_order",
                                    "integration_order", "expandHamiltonian", "use_bn", "expansionOrder",
-                                   "etiltSign", "malignMethod", "edgeFlags", "synch_rad")]
+                                   "etiltSign", "malignMethod", "edgeFlags", "synch_rad", "kick_limit_scaling")] + \
+               [("edge1_kick_limit", dbl), ("edge2_kick_limit", dbl)]
 
 
 class EH_RFCA(C.Structure):
@@ -172,7 +173,9 @@ ALIASES = {
                "EDGE1_EFFECTS": "edge1_effects", "EDGE2_EFFECTS": "edge2_effects", "EDGE_ORDER": "edge_order",
                "INTEGRATION_ORDER": "integration_order", "EXPAND_HAMILTONIAN": "expandHamiltonian",
                "USE_BN": "use_bn", "EXPANSION_ORDER": "expansionOrder", "ETILT_SIGN": "etiltSign",
-               "MALIGN_METHOD": "malignMethod", "NONLINEAR": "nonlinear", "SYNCH_RAD": "synch_rad"},
+               "MALIGN_METHOD": "malignMethod", "NONLINEAR": "nonlinear", "SYNCH_RAD": "synch_rad",
+               "EDGE1_KICK_LIMIT": "edge1_kick_limit", "EDGE2_KICK_LIMIT": "edge2_kick_limit",
+               "KICK_LIMIT_SCALING": "kick_limit_scaling"},
     T_RFCA: {"N_KICKS": "nKicks", "CHANGE_P0": "change_p0", "CHANGE_T": "change_t", "END1_FOCUS": "end1Focus",
              "PHASE_REFERENCE": "phase_reference",
              "END2_FOCUS": "end2Focus", "STANDING_WAVE": "standingWave", "LINEARIZE": "linearize",

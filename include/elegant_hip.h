@@ -161,7 +161,7 @@ typedef struct {
   double b1, b2, b3, b4, b5, b6, b7, b8; /* used when use_bn */
   int32_t nSlices;
   int32_t nonlinear;          /* default 1 */
-  int32_t edge1_effects;      /* default 1; 0 or 1 supported */
+  int32_t edge1_effects;      /* default 1; 0, 1, or 3 (the symplectic kick on the canonical momenta) */
   int32_t edge2_effects;
   int32_t edge_order;         /* default 1; >= 2: second-order edge map (apply_edge_effects) */
   int32_t integration_order;  /* 2, 4, 6 */
@@ -172,6 +172,8 @@ typedef struct {
   int32_t malignMethod;       /* 0 supported */
   int32_t edgeFlags;          /* EH_BEND_EDGE1/2 as determined for the beamline */
   int32_t synch_rad;          /* SYNCH_RAD: classical radiation (src/csbend.c:956-959) */
+  int32_t kick_limit_scaling; /* KICK_LIMIT_SCALING: the limits below times rho0/rho_actual */
+  double edge1_kick_limit, edge2_kick_limit; /* EDGE*_KICK_LIMIT on the edges' x' kick (<= 0: none) */
 } EH_CSBEND;
 
 #define EH_BEND_EDGE1_EFFECTS 1
@@ -441,8 +443,9 @@ void eh_adjoint_free(eh_adjoint *adj);
 /* Track n particles (rows of 10 doubles, elegant layout) n_turns through the
  * beamline, exactly as eh_do_tracking does for the survivors, keeping
  * checkpoints every checkpoint_every turns (1: keep every anchor, no
- * re-tracking).  param_slots[3*e + k] is the gradient slot of element e's
- * strength (k=0: K1/K2/K3/K1 of KQUAD/KSEXT/KOCT/KQUSE), DX (k=1) or DY (k=2), or -1.
+ * re-tracking).  param_slots[4*e + k] is the gradient slot of element e's
+ * strength (k=0: K1/K2/K3/K1 of KQUAD/KSEXT/KOCT/KQUSE, K1 of CSBEND), DX (k=1)
+ * or DY (k=2) of the kick multipoles, or FSE (k=3) of CSBEND, or -1.
  * RF cavity fiducials must already be fixed (track one pass first).  Writes the
  * final coordinates (n x 6) and survival flags; returns the number of survivors. */
 long eh_adjoint_forward(eh_adjoint *adj, const double *coord, long n, EH_ELEMENT_LIST *beamline, long n_elements,

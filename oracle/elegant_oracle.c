@@ -1316,8 +1316,31 @@ static void fromDipoleCanonical(double *Qi) {
   Qi[3] *= f;
 }
 
-/* track_through_csbend, src/csbend.c:654-1434 (malignMethod 0, ETILT 0, no
- * radiation, EDGE*_EFFECTS 0 or 1 with EDGE_ORDER 1, no reference correction) */
+/* applySimpleDipoleEdgeKick, src/csbend.c:5664-5690 (EDGE_EFFECTS 3, the
+ * Hamiltonian not expanded) */
+static void or_simple_edge_kick(double *xp, double *yp, double x, double y, double delta, double rho, double ea,
+                                double psi, double kickLimit) {
+  double Qi[6], dqx, dqy;
+  Qi[0] = x;
+  Qi[1] = *xp;
+  Qi[2] = y;
+  Qi[3] = *yp;
+  Qi[4] = 0;
+  Qi[5] = delta;
+  toDipoleCanonical(Qi);
+  dqx = OR_TAN(ea) / rho * x;
+  if (kickLimit > 0 && fabs(dqx) > kickLimit)
+    dqx = SIGN(dqx) * kickLimit;
+  dqy = -OR_TAN(ea - psi / (1 + delta)) / rho * y;
+  Qi[1] += dqx;
+  Qi[3] += dqy;
+  fromDipoleCanonical(Qi);
+  *xp = Qi[1];
+  *yp = Qi[3];
+}
+
+/* track_through_csbend, src/csbend.c:654-1434 (malignMethod 0, ETILT 0,
+ * EDGE*_EFFECTS 0, 1 or 3, no reference correction) */
 /* apply_edge_effects, src/csbend.c:4291-4332: K. L. Brown's second-order edge
  * map (EDGE_ORDER >= 2); which_edge -1 at the entrance, 1 at the exit */
 static void or_apply_edge_effects(double *x, double *xp, double *y, double *yp, double rho, double n, double beta,
@@ -1359,7 +1382,7 @@ static void or_apply_edge_effects(double *x, double *xp, double *y, double *yp, 
 long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend, double Po, double z_start,
                              const EH_MAXAMP *maxamp) {
   double b[9], c[9], h, rho0, rho_actual, fse, angle, e1, e2, tilt, etilt, ttilt, cos_ttilt, sin_ttilt;
-  double psi1, psi2, Kg1, Kg2, dxi, dyi, dzi, dxf, dyf, dzf, dz_lost, rho, delta_xp;
+  double psi1, psi2, Kg1, Kg2, dxi, dyi, dzi, dxf, dyf, dzf, dz_lost, rho, delta_xp, e1_kick_limit, e2_kick_limit;
   double x, xp, y, yp, s, dp, dp0, Qi[6], Qf[6], *coord, rad_coef, n;
   long i_part, i_top, j, particle_lost;
   or_csbend_fields F;
@@ -1373,8 +1396,8 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
     return or_fail("CSBEND integration_order is invalid");
   if (csbend->malignMethod != 0 || csbend->etilt != 0 || csbend->expandHamiltonian)
     return or_fail("CSBEND option not supported");
-  if ((csbend->edge1_effects != 0 && csbend->edge1_effects != 1) ||
-      (csbend->edge2_effects != 0 && csbend->edge2_effects != 1))
+  if ((csbend->edge1_effects != 0 && csbend->edge1_effects != 1 && csbend->edge1_effects != 3) ||
+      (csbend->edge2_effects != 0 && csbend->edge2_effects != 1 && csbend->edge2_effects != 3))
     return or_fail("CSBEND edge method not supported");
 
   rho0 = csbend->length / csbend->angle;
@@ -1441,6 +1464,13 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
     rad_coef = sqr(EH_E_MKS) * pow3(Po) * sqr(1 + fse) / (6 * PI * EH_EPSILON_O * sqr(c_mks) * EH_ME_MKS * sqr(rho0));
   else
     rad_coef = 0;
+  /* edge kick limits, src/csbend.c:932-937 */
+  e1_kick_limit = csbend->edge1_kick_limit;
+  e2_kick_limit = csbend->edge2_kick_limit;
+  if (csbend->kick_limit_scaling) {
+    e1_kick_limit *= rho0 / rho_actual;
+    e2_kick_limit *= rho0 / rho_actual;
+  }
   Kg1 = 2 * csbend->hgap * csbend->fint * SIGN(rho0);
   psi1 = Kg1 / fabs(rho_actual) / OR_COS(e1) * (1 + sqr(OR_SIN(e1)));
   Kg2 = 2 * csbend->hgap * csbend->fint * SIGN(rho0);
@@ -1490,15 +1520,19 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
     s = coord[4];
     dp = dp0 = coord[5];
 
+    /* src/csbend.c:1125-1200 */
     if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) && csbend->edge1_effects == 1) {
       rho = (1 + dp) * rho_actual;
       if (csbend->edge_order <= 1) {
         delta_xp = OR_TAN(e1) / rho * x;
+        if (e1_kick_limit > 0 && fabs(delta_xp) > e1_kick_limit)
+          delta_xp = SIGN(delta_xp) * e1_kick_limit;
         xp += delta_xp;
         yp -= OR_TAN(e1 - psi1 / (1 + dp)) / rho * y;
       } else
         or_apply_edge_effects(&x, &xp, &y, &yp, rho, n, e1, csbend->h1, psi1 * (1 + dp), -1);
-    }
+    } else if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) && csbend->edge1_effects == 3)
+      or_simple_edge_kick(&xp, &yp, x, y, dp, rho_actual, e1, psi1, e1_kick_limit);
     Qi[0] = x;
     Qi[1] = xp;
     Qi[2] = y;
@@ -1552,15 +1586,19 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
     y = Qf[2];
     yp = Qf[3];
     dp = Qf[5];
+    /* src/csbend.c:1306-1340 */
     if ((csbend->edgeFlags & EH_BEND_EDGE2_EFFECTS) && csbend->edge2_effects == 1) {
       rho = (1 + dp) * rho_actual;
       if (csbend->edge_order <= 1) {
         delta_xp = OR_TAN(e2) / rho * x;
+        if (e2_kick_limit > 0 && fabs(delta_xp) > e2_kick_limit)
+          delta_xp = SIGN(delta_xp) * e2_kick_limit;
         xp += delta_xp;
         yp -= OR_TAN(e2 - psi2 / (1 + dp)) / rho * y;
       } else
         or_apply_edge_effects(&x, &xp, &y, &yp, rho, n, e2, csbend->h2, psi2 * (1 + dp), 1);
-    }
+    } else if ((csbend->edgeFlags & EH_BEND_EDGE2_EFFECTS) && csbend->edge2_effects == 3)
+      or_simple_edge_kick(&xp, &yp, x, y, dp, rho_actual, e2, psi2, e2_kick_limit);
     coord[0] = x * cos_ttilt - y * sin_ttilt + 0.0;
     coord[2] = x * sin_ttilt + y * cos_ttilt + 0.0;
     coord[1] = xp * cos_ttilt - yp * sin_ttilt + 0.0;

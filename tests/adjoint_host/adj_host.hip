@@ -17,9 +17,15 @@ extern "C" long adjh_gradient(const double *coord, long n, EH_ELEMENT_LIST *bl, 
   for (long e = 0; e < ne; e++) {
     if (bl[e].type == EH_T_RFCA && e > 0)
       hd[e].adj_anchor = n_anchor++;
-    hd[e].adj_pk = slots[3 * e];
-    hd[e].adj_pdx = slots[3 * e + 1];
-    hd[e].adj_pdy = slots[3 * e + 2];
+    hd[e].adj_pk = slots[EH_ADJ_KINDS * e];
+    hd[e].adj_pdx = slots[EH_ADJ_KINDS * e + 1];
+    hd[e].adj_pdy = slots[EH_ADJ_KINDS * e + 2];
+    hd[e].adj_pfse = slots[EH_ADJ_KINDS * e + 3];
+    if (bl[e].type == EH_T_CSBEND) {
+      r = adj_bend_params(hd[e], (const EH_CSBEND *)bl[e].p_elem, pool);
+      if (r < 0)
+        return r;
+    }
   }
   std::fill(grad_params, grad_params + np, 0.0);
   *max_err = 0;
@@ -64,7 +70,7 @@ extern "C" long adjh_gradient(const double *coord, long n, EH_ELEMENT_LIST *bl, 
         case EH_T_KSEXT:
         case EH_T_KOCT:
         case EH_T_KQUSE: {
-          eh_pgrad pg = {0, 0, 0};
+          eh_pgrad pg = {0, 0, 0, 0};
           eh_adj_multipole(z, l, d, pg);
           if (d.adj_pk >= 0)
             grad_params[d.adj_pk] += pg.k;
@@ -74,9 +80,15 @@ extern "C" long adjh_gradient(const double *coord, long n, EH_ELEMENT_LIST *bl, 
             grad_params[d.adj_pdy] += pg.dy;
           break;
         }
-        case EH_T_CSBEND:
-          eh_adj_csbend(z, l, d, pool.data());
+        case EH_T_CSBEND: {
+          eh_pgrad pg = {0, 0, 0, 0};
+          eh_adj_csbend(z, l, d, pool.data(), pg);
+          if (d.adj_pk >= 0)
+            grad_params[d.adj_pk] += pg.k;
+          if (d.adj_pfse >= 0)
+            grad_params[d.adj_pfse] += pg.fse;
           break;
+        }
         case EH_T_RFCA:
           eh_adj_rfca(z, l, anchors[t * n_anchor + (e == 0 ? 0 : d.adj_anchor)], d);
           break;

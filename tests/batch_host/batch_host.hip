@@ -42,9 +42,9 @@ extern "C" long bh_lean_stage(int type, int ell, int *capable) {
   eh_desc d;
   std::memset(&d, 0, sizeof d);
   d.type = type;
-  d.adj_pk = d.adj_pdx = d.adj_pdy = d.adj_anchor = -1;
+  d.adj_pk = d.adj_pdx = d.adj_pdy = d.adj_pfse = d.adj_anchor = -1;
   if (ell) {
-    d.ap_ell[0] = d.ap_ell[1] = d.post_ell[0] = d.post_ell[1] = d.ell_any = 1;
+    d.ap_ell[0] = d.ap_ell[1] = d.post_ell[0] = d.post_ell[1] = d.ell_any = d.full_only = 1;
     d.ap_xmax[0] = d.ap_xmax[1] = d.ap_ymax[0] = d.ap_ymax[1] = 1.0;
     d.ell_xe[0] = d.ell_xe[1] = d.ell_ye[0] = d.ell_ye[1] = 2;
   }

@@ -120,6 +120,30 @@ def test_mixed_ring_gradients():
     assert err < 1e-13
 
 
+def _elementwise(gp, tp, tol):
+    scale = np.maximum(np.abs(tp), 1e-3 * np.max(np.abs(tp)))
+    return np.max(np.abs(gp - tp) / scale) < tol
+
+
+def test_bend_k1_and_fse_gradients():
+    """a bend's K1 and FSE as parameters (field per unit K1 and FSE, 1/rho_actual
+    and the edge angles growing with FSE) against autograd through the torch
+    restatement, parameter by parameter"""
+    Po = configs.p_central_from_mev(2000.0)
+    lat = mix_ring()
+    spec = adjoint.ParamSpec.bends(lat) + adjoint.ParamSpec.strengths(lat)
+    assert spec.names[:2] == ["B#3.K1", "B#3.FSE"]
+    rows = configs.fodo_beam(24, seed=8)
+    rows[:, :4] *= 2.0
+    W = np.random.default_rng(3).normal(size=(24, 6)) * WEIGHTS
+    final, alive, gp, gz, err = host_gradient(lat, spec, Po, rows, 5, W)
+    assert alive.all()
+    tp, tz, tout = torch_gradient(lat, spec, Po, rows, 5, W)
+    assert rel(final, tout) < 1e-9
+    assert _elementwise(gp, tp, 1e-7), np.c_[gp, tp]
+    assert rel(gz, tz) < 1e-8
+
+
 def test_mba_gradients_with_rf():
     Po = configs.p_central_from_mev(3000.0)
     lat = configs.mba_ring(Po)

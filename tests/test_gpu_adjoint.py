@@ -106,6 +106,30 @@ def test_gradients_vs_autograd_mixed_ring(ckpt):
 
 
 @pytest.mark.parametrize("ckpt", [1, 2])
+def test_bend_gradients_vs_autograd(ckpt):
+    """K1 and FSE of the MBA ring's bends (and the quadrupoles' K), parameter by
+    parameter against autograd through the torch restatement"""
+    _say("adjoint vs autograd, MBA bends K1 and FSE, checkpoint every %d" % ckpt)
+    Po = configs.p_central_from_mev(3000.0)
+    lat = configs.mba_ring(Po)
+    spec = adjoint.ParamSpec.bends(lat) + adjoint.ParamSpec.strengths(lat)
+    rows = configs.mba_beam(32, seed=10, scale=25.0)
+    turns = 2
+    W = np.random.default_rng(5).normal(size=(len(rows), 6)) * np.array([1e3, 1e4, 1e3, 1e4, 1.0, 1e3])
+    adj = adjoint.AdjointTracker(lat, spec, Po, n_turns=turns, checkpoint_every=ckpt)
+    final, alive = adj.forward(rows)
+    assert alive.all()
+    gp, gz = adj.backward(W)
+    tp, tz, tout = _torch_grads(lat, spec, Po, rows, turns, W)
+    assert _rel(final, tout) < 1e-8
+    nb = len(adjoint.ParamSpec.bends(lat))
+    scale = np.maximum(np.abs(tp[:nb]), 1e-3 * np.max(np.abs(tp[:nb])))
+    assert np.max(np.abs(gp[:nb] - tp[:nb]) / scale) < 1e-6, np.c_[gp[:nb], tp[:nb]][:8]
+    assert _rel(gp, tp) < 1e-7
+    assert _rel(gz, tz) < 1e-7
+
+
+@pytest.mark.parametrize("ckpt", [1, 2])
 def test_gradients_vs_autograd_mba(ckpt):
     _say("adjoint vs autograd, MBA ring with RF, checkpoint every %d" % ckpt)
     Po = configs.p_central_from_mev(3000.0)

@@ -102,6 +102,12 @@ ELEMENT_CASES = [
                     EDGE_ORDER=2)),
     ("CSBEND", dict(L=0.8, ANGLE=-0.05, K1=0.3, N_KICKS=8, E1=0.02, E2=0.01, HGAP=0.01, EDGE_ORDER=2,
                     SYNCH_RAD=1)),
+    ("CSBEND", dict(L=0.8, ANGLE=0.1, K1=0.4, N_KICKS=8, E1=0.05, E2=0.03, HGAP=0.01, FINT=0.5, EDGE1_EFFECTS=3,
+                    EDGE2_EFFECTS=3)),
+    ("CSBEND", dict(L=0.8, ANGLE=-0.06, K1=0.2, N_KICKS=8, E1=0.08, E2=0.06, HGAP=0.01, EDGE1_EFFECTS=3,
+                    EDGE2_EFFECTS=1, EDGE1_KICK_LIMIT=1e-4, EDGE2_KICK_LIMIT=5e-5, KICK_LIMIT_SCALING=1,
+                    FSE=1e-2)),
+    ("CSBEND", dict(L=0.8, ANGLE=0.1, N_KICKS=8, E1=0.1, E2=0.1, EDGE1_KICK_LIMIT=2e-4, EDGE2_KICK_LIMIT=2e-4)),
 ]
 
 
@@ -226,6 +232,31 @@ def test_more_elements_ring(mode, ehlib, oracle_mod):
     got, n, ref, nref, stats, eff = _track_both(lat, rows, 5, Po, mode, tpl=2)
     assert 0 < nref < len(rows)
     assert_rows_identical(got, ref, n, nref, "more elements " + mode)
+    assert stats["particle_element_turns"] == eff
+
+
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+def test_edge_method3_ring(mode, ehlib, oracle_mod):
+    """bends with EDGE_EFFECTS 3 and edge kick limits (the full instantiation of
+    the maps) in a ring, bit for bit with the oracle, losses included"""
+    _say("edge method 3 ring %s" % mode)
+    lat = Lattice([
+        Element("AP", "MAXAMP", X_MAX=0.02, Y_MAX=0.015),
+        Element("QF", "KQUAD", L=0.3, K1=1.2, N_KICKS=8),
+        Element("D1", "DRIF", L=0.6),
+        Element("B1", "CSBEND", L=0.9, ANGLE=0.12, K1=-0.2, N_KICKS=8, E1=0.06, E2=0.06, HGAP=0.012, FINT=0.5,
+                EDGE1_EFFECTS=3, EDGE2_EFFECTS=3),
+        Element("D2", "EDRIFT", L=0.4),
+        Element("QD", "KQUAD", L=0.3, K1=-1.2, N_KICKS=8),
+        Element("B2", "CSBEND", L=0.9, ANGLE=0.12, N_KICKS=8, E1=0.06, E2=0.06, EDGE1_KICK_LIMIT=1e-4,
+                EDGE2_KICK_LIMIT=1e-4),
+        Element("D3", "DRIF", L=0.6),
+    ])
+    rows = _beam_rows(3000, seed=41, scale=4.0)
+    Po = configs.p_central_from_mev(2000.0)
+    got, n, ref, nref, stats, eff = _track_both(lat, rows, 4, Po, mode, tpl=2)
+    assert 0 < nref < len(rows)
+    assert_rows_identical(got, ref, n, nref, "edge method 3 ring " + mode)
     assert stats["particle_element_turns"] == eff
 
 

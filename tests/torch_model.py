@@ -76,7 +76,7 @@ class TorchRing:
         self.Po = p_central
         self.structs = lattice.structs()
         ne = len(lattice)
-        self.slots = np.full((ne, 3), -1, dtype=np.int64) if param_slots is None else np.asarray(param_slots)
+        self.slots = np.full((ne, 4), -1, dtype=np.int64) if param_slots is None else np.asarray(param_slots)
         self.n_params = n_params
         self.csb = {}
         for i, (el, st) in enumerate(zip(lattice.elements, self.structs)):
@@ -87,7 +87,7 @@ class TorchRing:
         """current parameter values in slot order"""
         th = np.zeros(self.n_params)
         for i, st in enumerate(self.structs):
-            for k in range(3):
+            for k in range(4):
                 q = self.slots[i, k]
                 if q >= 0:
                     th[q] = getattr(st, self._field(self.lat.elements[i].type, k))
@@ -99,7 +99,9 @@ class TorchRing:
             return "dx"
         if k == 2:
             return "dy"
-        return {S.T_KQUAD: "k1", S.T_KSEXT: "k2", S.T_KOCT: "k3", S.T_KQUSE: "k1"}[etype]
+        if k == 3:
+            return "fse"
+        return {S.T_KQUAD: "k1", S.T_KSEXT: "k2", S.T_KOCT: "k3", S.T_KQUSE: "k1", S.T_CSBEND: "k1"}[etype]
 
     def _csbend_setup(self, cb):
         if cb.tilt or cb.etilt or cb.dx or cb.dy or cb.dz or cb.use_bn or cb.xKick or cb.yKick:
@@ -127,8 +129,47 @@ class TorchRing:
         field = any(b[1:] != 0)
         edge1 = bool(cb.edgeFlags & 1) and cb.edge1_effects == 1
         edge2 = bool(cb.edgeFlags & 2) and cb.edge2_effects == 1
+        # for K1 and FSE as parameters: the table is linear in b[], so it is the
+        # table without b1 and b2 plus b1 and b2 times their unit tables (the
+        # dipole's constant 1 - b0 only in the first)
+        rest = b.copy()
+        rest[1] = rest[2] = 0
+        F0x, F0y, _ = oracle.field_table(rest, c, h, cb.nonlinear, eo1 - 1)
+        unit = []
+        for n in (1, 2):
+            e = np.zeros(9)
+            e[n] = 1.0
+            ux, uy, _ = oracle.field_table(e, np.zeros(9), h, cb.nonlinear, eo1 - 1)
+            uy[0, 0] -= 1
+            unit.append((ux, uy))
         return dict(rho0=rho0, rho_a=rho_actual, tx=terms_x, ty=terms_y, field=field, e1=e1, e2=e2, psi1=psi1,
-                    psi2=psi2, edge1=edge1, edge2=edge2)
+                    psi2=psi2, edge1=edge1, edge2=edge2, eo1=eo1, F0=(F0x, F0y), unit=unit, h=h,
+                    k1=cb.k1, k2=cb.k2, fse=cb.fse, fd=cb.fseDipole, fq=cb.fseQuadrupole, Kg=Kg)
+
+    def _bend_params(self, i, theta):
+        """the bend's rho_actual, edge angles and field terms as functions of its
+        K1 and FSE (src/csbend.c:800-857, :940-1000) when either is a parameter"""
+        cs = self.csb[i]
+        if self.slots[i, 0] < 0 and self.slots[i, 3] < 0:
+            return cs["rho_a"], cs["psi1"], cs["psi2"], cs["tx"], cs["ty"], cs["field"]
+        k1 = self._param(theta, i, 0, cs["k1"])
+        F = self._param(theta, i, 3, cs["fse"])
+        rho0, fd, fq = cs["rho0"], cs["fd"], cs["fq"]
+        b1 = k1 * rho0 * (1 + F + fq) / (1 + F + fd)
+        b2 = cs["k2"] * rho0 * (1 + F) / (1 + F + fd)
+        rho_a = 1 / ((1 + F + fd) * cs["h"])
+        psi1 = cs["Kg"] / rho_a / math.cos(cs["e1"]) * (1 + math.sin(cs["e1"]) ** 2)
+        psi2 = cs["Kg"] / rho_a / math.cos(cs["e2"]) * (1 + math.sin(cs["e2"]) ** 2)
+        (F0x, F0y), (u1x, u1y), (u2x, u2y) = cs["F0"], cs["unit"][0], cs["unit"][1]
+        eo1 = cs["eo1"]
+        tx, ty = [], []
+        for a in range(eo1):
+            for j in range(eo1 - a):
+                if F0x[a, j] or u1x[a, j] or u2x[a, j]:
+                    tx.append((a, j, F0x[a, j] + b1 * u1x[a, j] + b2 * u2x[a, j]))
+                if F0y[a, j] or u1y[a, j] or u2y[a, j]:
+                    ty.append((a, j, F0y[a, j] + b1 * u1y[a, j] + b2 * u2y[a, j]))
+        return rho_a, psi1, psi2, tx, ty, True
 
     # ---- element maps -----------------------------------------------------
     def _param(self, theta, i, k, default):
@@ -206,23 +247,24 @@ class TorchRing:
         return [x, xp, y, yp, s, dp]
 
     @staticmethod
-    def _fields(cs, X, Y):
-        if not cs["field"]:
+    def _fields(field, tx, ty, X, Y):
+        if not field:
             return torch.zeros_like(X), torch.ones_like(X)
         px = _powers(X, 10)
         py = _powers(Y, 10)
-        Fx = sum(c * px[i] * py[j] for (i, j, c) in cs["tx"]) if cs["tx"] else torch.zeros_like(X)
-        Fy = sum(c * px[i] * py[j] for (i, j, c) in cs["ty"]) if cs["ty"] else torch.zeros_like(X)
+        Fx = sum(c * px[i] * py[j] for (i, j, c) in tx) if tx else torch.zeros_like(X)
+        Fy = sum(c * px[i] * py[j] for (i, j, c) in ty) if ty else torch.zeros_like(X)
         return Fx, Fy
 
-    def _csbend(self, z, i, st):
+    def _csbend(self, z, i, st, theta):
         x, xp, y, yp, s, dp = z
         cs = self.csb[i]
-        rho0, rho_a = cs["rho0"], cs["rho_a"]
+        rho0 = cs["rho0"]
+        rho_a, psi1, psi2, tx, ty, field = self._bend_params(i, theta)
         if cs["edge1"]:
             rho = (1 + dp) * rho_a
             xp = xp + math.tan(cs["e1"]) / rho * x
-            yp = yp - torch.tan(cs["e1"] - cs["psi1"] / (1 + dp)) / rho * y
+            yp = yp - torch.tan(cs["e1"] - psi1 / (1 + dp)) / rho * y
         QX, QY = _s2m(xp, yp, dp)
         X, Y = x, y
         dfr, kfr = _tables(st.integration_order)
@@ -245,7 +287,7 @@ class TorchRing:
                 if j == nsub - 1:
                     break
                 ds = sl * kfr[j]
-                Fx, Fy = self._fields(cs, X, Y)
+                Fx, Fy = self._fields(field, tx, ty, X, Y)
                 QX = QX - ds * (1 + X / rho0) * Fy / rho_a
                 QY = QY + ds * (1 + X / rho0) * Fx / rho_a
         s = s + dist
@@ -254,7 +296,7 @@ class TorchRing:
         if cs["edge2"]:
             rho = (1 + dp) * rho_a
             xp = xp + math.tan(cs["e2"]) / rho * x
-            yp = yp - torch.tan(cs["e2"] - cs["psi2"] / (1 + dp)) / rho * y
+            yp = yp - torch.tan(cs["e2"] - psi2 / (1 + dp)) / rho * y
         return [x, xp, y, yp, s, dp]
 
     def _rfca(self, z, st):
@@ -317,7 +359,7 @@ class TorchRing:
                     L = st.length
                     z = [x + xp * L, xp, y + yp * L, yp, s + L * torch.sqrt(1 + xp * xp + yp * yp), dp]
                 else:
-                    z = self._csbend(z, i, st)
+                    z = self._csbend(z, i, st, theta)
             elif t == S.T_RFCA:
                 z = self._rfca(z, st)
         return z
