@@ -25,7 +25,7 @@ EXPORTS = [
     "eh_adjoint_free", "eh_adjoint_forward", "eh_adjoint_backward", "eh_adjoint_forward_batch",
     "eh_multipole_tracking", "eh_offset_beam", "eh_rotate_beam", "eh_replay_sequence", "eh_set_central_momentum",
     "eh_track_particles", "eh_beam_row_order", "eh_do_tracking_noalloc", "eh_elimit_amplitudes",
-    "eh_get_phase_reference", "eh_set_phase_reference", "eh_delete_phase_references",
+    "eh_get_phase_reference", "eh_set_phase_reference", "eh_delete_phase_references", "eh_do_match_energy",
 ]
 
 
@@ -79,6 +79,8 @@ def load():
     lib.eh_rotate_beam.restype = L
     lib.eh_set_central_momentum.argtypes = [vp, L, D, dp]
     lib.eh_set_central_momentum.restype = L
+    lib.eh_do_match_energy.argtypes = [vp, L, dp, L]
+    lib.eh_do_match_energy.restype = L
     lib.eh_track_particles.argtypes = [vp, L, C.POINTER(S.EH_VMATRIX)]
     lib.eh_track_particles.restype = L
     lib.eh_beam_row_order.argtypes = [vp, vp, L]

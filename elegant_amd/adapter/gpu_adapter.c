@@ -397,7 +397,7 @@ static const char *const rfca_untracked[] __attribute__((unused)) = {"backtrack"
 static int fill_rfca(const RFCA *rf, EH_RFCA *d) {
   unsigned long mode;
   int32_t fid;
-  if (rf->change_p0 || rf->change_t || rf->linearize || rf->lockPhase || rf->Q != 0)
+  if (rf->linearize || rf->lockPhase || rf->Q != 0)
     return 0;
   if (rf->bodyFocusModel && rf->bodyFocusModel[0] && strcmp(rf->bodyFocusModel, "none") != 0)
     return 0;
@@ -912,10 +912,12 @@ double gpu_findFiducialTime(long np, double s0, double sOffset, double p0, unsig
   return t;
 }
 
+/* do_match_energy on the device copy (src/do_tracking.c:3398-3411: elegant
+ * calls it for RFCA CHANGE_P0, ENERGY matching and ALWAYS_CHANGE_P0) */
 void gpu_do_match_energy(long np, double *P_central, long change_beam) {
-  double **coord = ad_host_begin();
-  do_match_energy(coord, np, P_central, change_beam);
-  ad_host_end(1);
+  ad_to_device(np);
+  ad_check(eh_do_match_energy(G.beam, np, P_central, change_beam), "do_match_energy");
+  ad_device_changed();
 }
 
 void gpu_compute_centroids(double *centroid, long n_part) {

@@ -124,6 +124,7 @@ typedef struct {
   double rf_len;  /* length / nKicks */
   double rf_flen; /* full element length (volt == 0 drift) */
   double rf_volt, rf_omega, rf_phase, rf_dtLight, rf_t0, rf_dx, rf_dy, rf_dgamma_same;
+  double rf_toff; /* CHANGE_T: whole RF periods taken off the arrival times at this visit (0: none) */
 
   /* ---- MULT, ROTATE, MALIGN ---- */
   int32_t on_pass;                      /* ROTATE/MALIGN: -1 every pass, -2 none, else the one pass acted on */
@@ -1412,7 +1413,9 @@ EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d) {
         double P = Pc * (1 + p.dp);
         double gamma = eh_sqrt(eh_sqr(P) + 1);
         double beta_i = P / gamma;
-        double t = (p.s + dc4) / (c_mks * beta_i) - 0.0;
+        double t = (p.s + dc4) / (c_mks * beta_i) - d.rf_toff;
+        if (ik == 0 && d.rf_toff != 0)
+          p.s = t * c_mks * beta_i - dc4;
         double dgamma = d.rf_dgamma_same;
         if (!d.rf_same) {
           if (d.rf_standing)
@@ -1470,7 +1473,9 @@ EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d) {
     double gamma = eh_sqrt(eh_sqr(P) + 1);
     double beta_i = P / gamma;
     double ds1 = length / 2 * eh_sqrt(1 + eh_sqr(p.xp) + eh_sqr(p.yp));
-    double t = (p.s + ds1) / (c_mks * beta_i) - 0.0;
+    double t = (p.s + ds1) / (c_mks * beta_i) - d.rf_toff;
+    if (d.rf_toff != 0)
+      p.s = t * c_mks * beta_i - ds1;
     double dgamma = d.rf_dgamma_same;
     if (!d.rf_same) {
       double sin_phase = eh_sin(omega * (t - 0.0 - 0 * d.rf_dtLight) + phase);

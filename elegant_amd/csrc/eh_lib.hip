@@ -182,6 +182,8 @@ struct eh_ctx {
    * phase replaces the cavity's own phase_fiducial (src/simple_rfca.c:419-421) */
   int rf_linked;
   double rf_ref_phase;
+  /* CHANGE_T: the time offset found at this visit (src/simple_rfca.c:488-498) */
+  double rf_toff;
 };
 
 static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c) {
@@ -576,7 +578,7 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
 
 /* RFCA constants, src/simple_rfca.c:378-470, once the fiducial phase is known */
 static long build_rfca(eh_desc *d, const EH_RFCA *rf, const eh_ctx &c) {
-  if (rf->Q != 0 || rf->change_t || rf->change_p0 || rf->linearize || rf->lockPhase)
+  if (rf->Q != 0 || rf->linearize || rf->lockPhase)
     return fail(EH_ERR_UNSUPPORTED, "RFCA option not supported");
   d->rf_flen = rf->length;
   if (rf->volt == 0) {
@@ -611,6 +613,7 @@ static long build_rfca(eh_desc *d, const EH_RFCA *rf, const eh_ctx &c) {
   d->rf_end2 = rf->end2Focus;
   d->rf_dx = rf->dx;
   d->rf_dy = rf->dy;
+  d->rf_toff = rf->change_t ? c.rf_toff : 0.0;
   return EH_OK;
 }
 
@@ -701,10 +704,17 @@ static long build_malign(eh_desc *d, const EH_MALIGN *m) {
 }
 
 /* the reference momentum an ENERGY element sets (src/do_tracking.c:1861-1881):
- * 1 with *P_new, 0 if it sets none, negative if a matching mode is asked for */
+ * 1 with *P_new, 0 if it sets none; EH_MATCH_BEAMLINE / EH_MATCH_PARTICLES for
+ * the matching modes (do_match_energy with change_beam 0 / 1; no fiducial beam
+ * flags, so MATCH_BEAMLINE always matches the reference momentum to the beam) */
+enum { EH_MATCH_BEAMLINE = 2, EH_MATCH_PARTICLES = 3 };
 static long energy_target(const EH_ENERGY *en, double *P_new) {
-  if (en->match_beamline || en->match_particles)
-    return fail(EH_ERR_UNSUPPORTED, "ENERGY with MATCH_BEAMLINE or MATCH_PARTICLES not supported");
+  if (en->match_beamline && en->match_particles)
+    return fail(EH_ERR_ARG, "can't match_beamline AND match_particles for ENERGY element");
+  if (en->match_beamline)
+    return EH_MATCH_BEAMLINE;
+  if (en->match_particles)
+    return EH_MATCH_PARTICLES;
   if (en->central_energy) {
     *P_new = std::sqrt(hsqr(en->central_energy + 1) - 1);
     return 1;
@@ -808,7 +818,7 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
       return r;
     d->en_pold = c.Po;
     d->en_pnew = Pn;
-    d->en_apply = r > 0 && c.Po != Pn;
+    d->en_apply = r == 1 && c.Po != Pn; /* the matching modes run between launches */
     return EH_OK;
   }
   }
@@ -1227,6 +1237,30 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_plain(eh_soa a, lon
     }
   }
   wave_add(&cnt[2], lost ? 1ull : 0ull);
+}
+
+/* do_match_energy's particle step (src/do_tracking.c:3474-3484, :3503-3512):
+ * mode 0 re-expresses delta against the new reference momentum (dp = c1 + dp*c2),
+ * mode 1 moves every momentum by c1 keeping the particle's time of flight;
+ * cnt[2] counts non-finite time coordinates */
+__global__ void __launch_bounds__(EH_BLOCK) k_match(eh_soa a, long n, int mode, double c1, double c2, double Pc,
+                                                    unsigned long long *cnt) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const double dp = a.dp[i];
+  if (mode == 0) {
+    a.dp[i] = c1 + dp * c2;
+  } else {
+    double P = (1 + dp) * Pc;
+    const double t = a.s[i] / (P / eh_sqrt(P * P + 1));
+    P += c1;
+    a.dp[i] = (P - Pc) / Pc;
+    const double s = t * (P / eh_sqrt(P * P + 1));
+    a.s[i] = s;
+    if (isnan(s) || isinf(s))
+      atomicAdd(cnt + 2, 1ull);
+  }
 }
 
 /* track_particles with one transport map over the alive particles, in place */
@@ -1894,6 +1928,104 @@ static long fiducial_from_beam(eh_beam *b, int mode, double sOffset, double p0, 
   return EH_OK;
 }
 
+/* do_match_energy (src/do_tracking.c:3376-3549, serial elegant with USE_KAHAN):
+ * the mean of P_central*(1+delta) over the alive particles as a Kahan sum in the
+ * reference's row order; change_beam 0 moves the reference momentum to it when
+ * they differ by more than 1e-14 relative, change_beam 1 moves every particle's
+ * momentum so that the mean is the reference momentum.  A non-finite time
+ * coordinate is an error (the reference aborts). */
+static long match_energy(eh_beam *b, double *Pc, long change_beam) {
+  const long n = b->n_alive;
+  if (n == 0)
+    return 0;
+  std::vector<int32_t> slot_at_row, alive_row0, idx;
+  long r = reference_order(b, slot_at_row, alive_row0, idx);
+  if (r < 0)
+    return r;
+  std::vector<double> dp(n);
+  eh_soa &a = b->buf[b->cur];
+  HIPCHK(hipMemcpy(dp.data(), a.dp, sizeof(double) * n, hipMemcpyDeviceToHost));
+  const double P0 = *Pc;
+  double P_average = 0, error = 0.0;
+  for (long row = 0; row < n; row++) {
+    const int32_t k = idx[slot_at_row[row]];
+    if (k < 0)
+      return fail(EH_ERR_STATE, "replay produced a lost particle among survivors");
+    double Y = P0 * (1 + dp[k]) + error;
+    double sum = P_average + Y;
+    error = Y - (sum - P_average);
+    P_average = sum;
+  }
+  P_average /= n;
+  int mode;
+  double c1, c2 = 0;
+  if (!change_beam) {
+    if (!(std::fabs(P_average - P0) / P0 > 1e-14))
+      return n;
+    mode = 0;
+    c1 = (P0 - P_average) / P_average;
+    c2 = P0 / P_average;
+  } else {
+    mode = 1;
+    c1 = P0 - P_average;
+  }
+  unsigned long long zero = 0;
+  HIPCHK(hipMemcpyAsync(b->d_cnt + 2, &zero, sizeof zero, hipMemcpyHostToDevice, b->stream));
+  r = timed_begin(b);
+  if (r < 0)
+    return r;
+  hipLaunchKernelGGL(k_match, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, a, n, mode, c1, c2, P0, b->d_cnt);
+  HIPCHK(hipGetLastError());
+  r = timed_end(b);
+  if (r < 0)
+    return r;
+  if (mode == 1) {
+    unsigned long long bad = 0;
+    HIPCHK(hipMemcpy(&bad, b->d_cnt + 2, sizeof bad, hipMemcpyDeviceToHost));
+    if (bad)
+      return fail(EH_ERR_STATE, "bad time coordinate for %llu particles in do_match_energy", bad);
+  } else
+    *Pc = P_average;
+  return n;
+}
+
+extern "C" long eh_do_match_energy(eh_beam *b, long np, double *P_central, long change_beam) {
+  long r = check_np(b, np);
+  if (r < 0)
+    return r;
+  if (!P_central)
+    return fail(EH_ERR_ARG, "null P_central (do_match_energy)");
+  return match_energy(b, P_central, change_beam);
+}
+
+/* CHANGE_T's time offset at a cavity visit (src/simple_rfca.c:488-498): the
+ * whole RF periods in the arrival time of the particle in row 0 of the
+ * reference's arrangement, when that time exceeds 0.9 periods */
+static long rf_time_offset(eh_beam *b, const EH_RFCA *rf, double Pc, double *toff) {
+  *toff = 0;
+  const double omega = 2 * EH_PI * rf->freq;
+  if (!(omega && rf->change_t && rf->volt != 0 && b->n_alive > 0))
+    return EH_OK;
+  std::vector<int32_t> slot_at_row, alive_row0, idx;
+  long r = reference_order(b, slot_at_row, alive_row0, idx);
+  if (r < 0)
+    return r;
+  const int32_t k = idx[slot_at_row[0]];
+  if (k < 0)
+    return fail(EH_ERR_STATE, "replay produced a lost particle in row 0");
+  double s, dp;
+  eh_soa &a = b->buf[b->cur];
+  HIPCHK(hipMemcpy(&s, a.s + k, sizeof s, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&dp, a.dp + k, sizeof dp, hipMemcpyDeviceToHost));
+  const double To = 2 * EH_PI / omega;
+  double P = Pc * (1 + dp);
+  double beta_i = P / std::sqrt(P * P + 1);
+  double t = s / (EH_C_MKS * beta_i);
+  if (t > (0.9 * To))
+    *toff = ((long)(t / To + 0.5)) * To;
+  return EH_OK;
+}
+
 /* phase references, src/phase_reference.c:28-143: numbered groups of
  * time-dependent elements sharing one reference phase, set by the first cavity
  * of the group tracked with particles and kept until eh_delete_phase_references.
@@ -2015,7 +2147,7 @@ extern "C" long eh_simple_rf_cavity(eh_beam *b, long np, EH_RFCA *rfca, double *
     return r;
   if (np <= 0)
     return np;
-  if (rfca->Q != 0 || rfca->change_t || rfca->change_p0 || rfca->linearize || rfca->lockPhase)
+  if (rfca->Q != 0 || rfca->linearize || rfca->lockPhase)
     return fail(EH_ERR_UNSUPPORTED, "RFCA option not supported");
   eh_ctx c = {};
   if (rfca->volt != 0) {
@@ -2024,6 +2156,9 @@ extern "C" long eh_simple_rf_cavity(eh_beam *b, long np, EH_RFCA *rfca, double *
       return r;
     if (rfca->freq != 0)
       rfca->t_fiducial = -rfca->phase_fiducial / (2 * EH_PI * rfca->freq);
+    r = rf_time_offset(b, rfca, *P_central, &c.rf_toff);
+    if (r < 0)
+      return r;
   }
   EH_ELEMENT_LIST e = {EH_T_RFCA, 0, "RFCA", (void *)rfca};
   eh_desc d;
@@ -2034,7 +2169,13 @@ extern "C" long eh_simple_rf_cavity(eh_beam *b, long np, EH_RFCA *rfca, double *
   r = build_desc(&d, &e, c, pool);
   if (r < 0)
     return r;
-  return run_plain(b, &d, nullptr, 0, 0, 0, 0, *P_central, 0);
+  long nLeft = run_plain(b, &d, nullptr, 0, 0, 0, 0, *P_central, 0);
+  if (nLeft > 0 && rfca->change_p0 && rfca->volt != 0) {
+    r = match_energy(b, P_central, 0);
+    if (r < 0)
+      return r;
+  }
+  return nLeft;
 }
 
 extern "C" long eh_removeInvalidParticles(eh_beam *b, long np, double z, double Po) {
@@ -2293,7 +2434,9 @@ static long track_plain(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
         long t = energy_target((const EH_ENERGY *)e->p_elem, &Pn);
         if (t < 0)
           return t;
-        if (t > 0)
+        if (t == EH_MATCH_BEAMLINE || t == EH_MATCH_PARTICLES)
+          nLeft = match_energy(b, Pc, t == EH_MATCH_PARTICLES);
+        else if (t > 0)
           nLeft = eh_set_central_momentum(b, nToTrack, Pn, Pc);
         break;
       }
@@ -2413,7 +2556,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
         r = energy_target((const EH_ENERGY *)bl[ie].p_elem, &Pn);
         if (r < 0)
           return r;
-        if (r > 0)
+        if (r == 1)
           P = Pn;
       }
     }
@@ -2423,6 +2566,32 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
   for (long ie = 0; ie < ne; ie++)
     two_phase = two_phase || Pe0[ie] != Pe1[ie];
   const std::vector<double> *Pe = &Pe0;
+  /* elements that need the beam at every pass: an RF cavity with CHANGE_T (its
+   * time offset, judged on the particle in row 0 as the beam reaches it) and the
+   * momentum matching after an RF cavity with CHANGE_P0 or an ENERGY element in
+   * a matching mode.  Lines holding them are tracked in launches that end there. */
+  std::vector<char> stop_before(ne, 0), stop_after(ne, 0);
+  std::vector<long> match_mode(ne, 0);
+  std::vector<double> toff(ne, 0.0);
+  bool dynamic = false;
+  for (long ie = 0; ie < ne; ie++) {
+    if (bl[ie].type == EH_T_RFCA) {
+      const EH_RFCA *rf = (const EH_RFCA *)bl[ie].p_elem;
+      stop_before[ie] = rf->volt != 0 && rf->change_t && rf->freq != 0;
+      stop_after[ie] = rf->volt != 0 && rf->change_p0;
+    } else if (bl[ie].type == EH_T_ENERGY) {
+      double Pn = 0;
+      r = energy_target((const EH_ENERGY *)bl[ie].p_elem, &Pn);
+      stop_after[ie] = r == EH_MATCH_BEAMLINE || r == EH_MATCH_PARTICLES;
+      match_mode[ie] = r == EH_MATCH_PARTICLES;
+    }
+    dynamic = dynamic || stop_before[ie] || stop_after[ie];
+  }
+  std::vector<double> Pdyn;
+  if (dynamic) {
+    Pdyn = Pe0;
+    Pe = &Pdyn;
+  }
   /* RF cavities whose phase needs the beam at their first pass (pend), and
    * those tracking with another cavity's group phase (link, lph) */
   std::vector<char> pend(ne, 0), link(ne, 0);
@@ -2431,6 +2600,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
     eh_ctx c = {};
     c.rf_linked = link[ie];
     c.rf_ref_phase = lph[ie];
+    c.rf_toff = toff[ie];
     c.Po = (*Pe)[ie];
     c.z_start = zs[ie];
     c.z_end = ze[ie];
@@ -2442,7 +2612,12 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       c.post_y[pi] = m ? m->y_max : 0;
       c.post_on[pi] = (bl[ie].type != EH_T_MARK) && m && (m->x_max || m->y_max);
     }
-    return build_desc(&hd[ie], bl + ie, c, pool);
+    long r2 = build_desc(&hd[ie], bl + ie, c, pool);
+    /* an element followed by a matching: the reference sweeps its MAXAMP after
+     * the matching, so the sweep runs on the host side after it */
+    if (r2 >= 0 && stop_after[ie])
+      hd[ie].post_ap[0] = hd[ie].post_ap[1] = 0;
+    return r2;
   };
   /* initial NaN sweep, src/do_tracking.c:478-483 */
   b->pass = 0;
@@ -2506,45 +2681,161 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
   bool rad = false;
   for (long ie = 0; ie < ne; ie++)
     rad = rad || !eh_lean_capable(hd[ie]) || bl[ie].type >= EH_T_MULT;
+  /* a pending cavity's phase from the beam that reaches it on pass 0; then
+   * this cavity, the other positions of the same cavity, and the later
+   * cavities of its group have their phase */
+  auto resolve_pending = [&](long ie) -> long {
+    EH_RFCA *rf = (EH_RFCA *)bl[ie].p_elem;
+    eh_ctx rc = {};
+    long r2 = rf_resolve(b, rf, (*Pe)[ie], ze[ie], rc);
+    if (r2 < 0)
+      return r2;
+    for (long je = ie; je < ne; je++) {
+      if (!pend[je])
+        continue;
+      const EH_RFCA *rj = (const EH_RFCA *)bl[je].p_elem;
+      double ph;
+      if (bl[je].p_elem == bl[ie].p_elem) {
+        link[je] = (char)rc.rf_linked;
+        lph[je] = rc.rf_ref_phase;
+      } else if (rj->phase_reference &&
+                 eh_get_phase_reference(rj->phase_reference, &ph) == EH_REF_PHASE_RETURNED) {
+        link[je] = 1;
+        lph[je] = ph;
+      } else
+        continue;
+      pend[je] = 0;
+      r2 = build(je);
+      if (r2 < 0)
+        return r2;
+      HIPCHK(hipMemcpy(b->d_desc + je, &hd[je], sizeof(eh_desc), hipMemcpyHostToDevice));
+    }
+    return EH_OK;
+  };
   int p = 0, e = 0;
+  if (dynamic) {
+    /* the reference momentum P at the position (p, e); the momentum each
+     * element sees at its next visit (Pdyn) follows from it through the
+     * ENERGY targets, and every descriptor is rebuilt when that changes */
+    double P = *Pc;
+    auto advance = [&](long from, long to, double q) {
+      for (long ie = from; ie < to; ie++)
+        if (bl[ie].type == EH_T_ENERGY) {
+          double Pn = q;
+          if (energy_target((const EH_ENERGY *)bl[ie].p_elem, &Pn) == 1)
+            q = Pn;
+        }
+      return q;
+    };
+    auto repropagate = [&](long from, double q) -> long {
+      bool changed = false;
+      for (long k = 0; k < ne; k++) {
+        const long ie = (from + k) % ne;
+        changed = changed || Pdyn[ie] != q;
+        Pdyn[ie] = q;
+        q = advance(ie, ie + 1, q);
+      }
+      if (!changed)
+        return EH_OK;
+      pool.clear();
+      for (long ie = 0; ie < ne; ie++) {
+        if (pend[ie])
+          continue;
+        long r2 = build(ie);
+        if (r2 < 0)
+          return r2;
+      }
+      HIPCHK(hipMemcpy(b->d_desc, hd.data(), sizeof(eh_desc) * ne, hipMemcpyHostToDevice));
+      return upload_terms(b, pool);
+    };
+    while (p < n_passes) {
+      if (e == ne) {
+        p++;
+        e = 0;
+        r = repropagate(0, P);
+        if (r < 0)
+          return r;
+        continue;
+      }
+      if (b->n_alive == 0) {
+        /* matching does nothing without particles; ENERGY targets still apply */
+        P = advance(e, ne, P);
+        if (p + 1 < n_passes)
+          P = advance(0, ne, P);
+        break;
+      }
+      if (Pdyn[e] != P)
+        return fail(EH_ERR_STATE, "reference momentum bookkeeping out of step at element %d", e);
+      if (p == 0 && pend[e]) {
+        r = resolve_pending(e);
+        if (r < 0)
+          return r;
+      }
+      if (stop_before[e]) {
+        r = rf_time_offset(b, (const EH_RFCA *)bl[e].p_elem, P, &toff[e]);
+        if (r < 0)
+          return r;
+        r = build(e);
+        if (r < 0)
+          return r;
+        HIPCHK(hipMemcpy(b->d_desc + e, &hd[e], sizeof(eh_desc), hipMemcpyHostToDevice));
+      }
+      int e1 = e;
+      while (e1 < ne) {
+        if (e1 > e && ((p == 0 && pend[e1]) || stop_before[e1]))
+          break;
+        e1++;
+        if (stop_after[e1 - 1])
+          break;
+      }
+      r = launch_fused(b, ne, p, e, p, e1, effort, tab, rad);
+      if (r < 0)
+        return r;
+      P = advance(e, e1, P);
+      if (stop_after[e1 - 1]) {
+        const long je = e1 - 1;
+        r = match_energy(b, &P, match_mode[je]);
+        if (r < 0)
+          return r;
+        /* then the element's MAXAMP sweep (src/do_tracking.c:2506-2517; its NaN
+         * sweep ran in the launch and loses nothing here: the cavity removed
+         * every invalid particle and a matching keeps finite values finite) */
+        const EH_MAXAMP *m = p == 0 ? amp0[je] : amp1[je];
+        b->pass = p;
+        if (m && (m->x_max || m->y_max)) {
+          r = m->elliptical ? eh_elimit_amplitudes(b, m->x_max, m->y_max, b->n_alive, ze[je], P, 0, 0, m->exponent,
+                                                   m->y_exponent)
+                            : eh_limit_amplitudes(b, m->x_max, m->y_max, b->n_alive, ze[je], P, 0, 0);
+          if (r < 0)
+            return r;
+          if (m->elliptical) /* the per-element sweep used descriptor slot 0 */
+            HIPCHK(hipMemcpy(b->d_desc, &hd[0], sizeof(eh_desc), hipMemcpyHostToDevice));
+        }
+        r = repropagate(e1 % ne, P);
+        if (r < 0)
+          return r;
+      }
+      e = e1;
+    }
+    if (n_passes > 0)
+      *Pc = P;
+    b->pass = n_passes > 0 ? n_passes - 1 : 0;
+    return b->n_alive;
+  }
   if (n_passes > 0) {
     /* pass 0: stop before each RFCA whose phase waits for the beam */
     for (long ie = 0; ie < ne; ie++) {
       if (!pend[ie])
         continue;
-      EH_RFCA *rf = (EH_RFCA *)bl[ie].p_elem;
       r = launch_fused(b, ne, p, e, 0, (int)ie, effort, tab, rad);
       if (r < 0)
         return r;
       e = (int)ie;
       if (b->n_alive == 0)
         break; /* no particle reaches it: the reference takes no fiducial */
-      eh_ctx rc = {};
-      r = rf_resolve(b, rf, Pe0[ie], ze[ie], rc);
+      r = resolve_pending(ie);
       if (r < 0)
         return r;
-      /* this cavity, the other positions of the same cavity, and the later
-       * cavities of its group now have their phase */
-      for (long je = ie; je < ne; je++) {
-        if (!pend[je])
-          continue;
-        const EH_RFCA *rj = (const EH_RFCA *)bl[je].p_elem;
-        double ph;
-        if (bl[je].p_elem == bl[ie].p_elem) {
-          link[je] = (char)rc.rf_linked;
-          lph[je] = rc.rf_ref_phase;
-        } else if (rj->phase_reference &&
-                   eh_get_phase_reference(rj->phase_reference, &ph) == EH_REF_PHASE_RETURNED) {
-          link[je] = 1;
-          lph[je] = ph;
-        } else
-          continue;
-        pend[je] = 0;
-        r = build(je);
-        if (r < 0)
-          return r;
-        HIPCHK(hipMemcpy(b->d_desc + je, &hd[je], sizeof(eh_desc), hipMemcpyHostToDevice));
-      }
     }
   }
   while (p < n_passes) {
@@ -3298,6 +3589,9 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
         const EH_RFCA *rf = (const EH_RFCA *)beamlines[v][e].p_elem;
         if (rf->phase_reference)
           return fail(EH_ERR_UNSUPPORTED, "element %ld: PHASE_REFERENCE is not supported in batched tracking", e);
+        if (rf->change_t || rf->change_p0)
+          return fail(EH_ERR_UNSUPPORTED, "element %ld: CHANGE_T and CHANGE_P0 are not supported in batched tracking",
+                      e);
         if (rf->fiducial_mode != EH_FID_TMEAN && rf->fiducial_mode != EH_FID_LIGHT && !rf->fiducial_seen)
           return fail(EH_ERR_UNSUPPORTED, "element %ld: batched tracking takes TMEAN and LIGHT fiducials only", e);
       }
@@ -3934,6 +4228,8 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
         const EH_RFCA *rf = (const EH_RFCA *)beamlines[v][e].p_elem;
         if (rf->phase_reference)
           return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support PHASE_REFERENCE", e);
+        if (rf->change_t || rf->change_p0)
+          return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support CHANGE_T or CHANGE_P0", e);
         /* a variant without particles never reads its descriptors */
         if (!rf->fiducial_seen && rf->volt != 0 && !(counts && counts[v] == 0))
           return fail(EH_ERR_ARG, "element %ld: RFCA fiducial unknown; track one pass first to fix it", e);

@@ -61,6 +61,13 @@ class Beam:
         """apply a transport map (S.EH_VMATRIX) to the surviving particles"""
         return _lib.check(lib().eh_track_particles(self._h, self.n_alive(), C.byref(vmatrix)))
 
+    def match_energy(self, p_central, change_beam):
+        """do_match_energy on the surviving particles; returns the reference
+        momentum afterwards"""
+        pc = C.c_double(p_central)
+        _lib.check(lib().eh_do_match_energy(self._h, self.n_alive(), C.byref(pc), int(change_beam)))
+        return pc.value
+
     def free(self):
         if self._h:
             lib().eh_beam_free(self._h)

@@ -143,8 +143,11 @@ typedef struct {
 
 /* ENERGY (reference momentum change): src/track.h:1998-2004, parameters
  * src/track_data.c:985-990, tracking src/do_tracking.c:1861-1881 through
- * set_central_momentum (:3563-3596).  MATCH_BEAMLINE and MATCH_PARTICLES are
- * not supported (EH_ERR_UNSUPPORTED). */
+ * set_central_momentum (:3563-3596) and, for MATCH_BEAMLINE (the reference
+ * momentum moved to the beam's mean; no fiducial-beam flags) and
+ * MATCH_PARTICLES (the particles' momenta moved so that their mean is the
+ * reference momentum), do_match_energy (:3376-3549).  Both matching flags at
+ * once is EH_ERR_ARG, as the reference refuses it. */
 typedef struct {
   double central_energy, central_momentum;
   int32_t match_beamline, match_particles;
@@ -190,6 +193,11 @@ typedef struct {
   double dx, dy;
   double tReference; /* default -1 */
   int32_t nKicks;    /* 0 = matrix method */
+  /* CHANGE_P0: the reference momentum matched to the surviving beam after the
+   * cavity (src/simple_rfca.c:977-982); CHANGE_T: whole RF periods taken off the
+   * time coordinates, judged on the particle in row 0 of the reference's
+   * arrangement (:488-498, :586-588, :864-865).  Not in batched tracking or the
+   * adjoint (EH_ERR_UNSUPPORTED). */
   int32_t change_p0, change_t, end1Focus, end2Focus, standingWave;
   int32_t linearize, lockPhase;
   int32_t fiducial_mode; /* eh_fid_mode */
@@ -297,6 +305,15 @@ long eh_offset_beam(eh_beam *beam, long n_part, const EH_MALIGN *offset, double 
  * particle's delta is re-referenced to P_new and *P_central becomes P_new */
 long eh_set_central_momentum(eh_beam *beam, long np, double P_new, double *P_central);
 
+/* replaces do_match_energy, src/do_tracking.c:3376-3549, called by elegant
+ * through gpu_do_match_energy (:3398-3411): the mean momentum of the resident
+ * particles as a Kahan sum in the reference's row order; change_beam 0 moves
+ * *P_central to it (when they differ by more than 1e-14 relative) and
+ * re-expresses every delta, change_beam 1 moves every particle's momentum so
+ * that the mean is *P_central, keeping each time of flight (a non-finite time
+ * coordinate is EH_ERR_STATE, where the reference aborts) */
+long eh_do_match_energy(eh_beam *beam, long np, double *P_central, long change_beam);
+
 /* replaces track_particles with rotation_matrix(tilt) for ROTATE,
  * src/do_tracking.c:1836-1852 (for the pass given by eh_beam_set_pass) */
 long eh_rotate_beam(eh_beam *beam, long n_part, const EH_ROTATE *rotate);
@@ -309,7 +326,8 @@ long eh_multipole_tracking2(eh_beam *beam, long n_part, const EH_ELEMENT_LIST *e
 long eh_track_through_csbend(eh_beam *beam, long n_part, const EH_CSBEND *csbend, double p_error, double Po,
                              double z_start, const EH_MAXAMP *maxamp, long iSlice);
 
-/* replaces simple_rf_cavity, src/simple_rfca.c:264-293 (mutates rfca state) */
+/* replaces simple_rf_cavity, src/simple_rfca.c:264-293 (mutates rfca state;
+ * with CHANGE_P0 also *P_central) */
 long eh_simple_rf_cavity(eh_beam *beam, long np, EH_RFCA *rfca, double *P_central, double zEnd);
 
 /* replaces exactDrift, src/csbend.c:3792 */

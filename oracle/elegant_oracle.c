@@ -1067,11 +1067,22 @@ void or_set_central_momentum(double **coord, long np, double P_new, double *P_ce
   }
 }
 
-/* ENERGY, src/do_tracking.c:1861-1881, without the matching modes */
+long or_do_match_energy(double **coord, long np, double *P_central, long change_beam);
+
+/* ENERGY, src/do_tracking.c:1861-1881 (no fiducial beam flags: MATCH_BEAMLINE
+ * always matches the reference momentum to the beam) */
 long or_energy(double **coord, long np, const EH_ENERGY *energy, double *P_central) {
-  if (energy->match_beamline || energy->match_particles)
-    return or_fail("ENERGY with MATCH_BEAMLINE or MATCH_PARTICLES not supported");
-  if (energy->central_energy)
+  if (energy->match_beamline) {
+    long r = or_do_match_energy(coord, np, P_central, 0);
+    if (r < 0)
+      return r;
+    if (energy->match_particles)
+      return or_fail("can't match_beamline AND match_particles for ENERGY element");
+  } else if (energy->match_particles) {
+    long r = or_do_match_energy(coord, np, P_central, 1);
+    if (r < 0)
+      return r;
+  } else if (energy->central_energy)
     or_set_central_momentum(coord, np, sqrt(sqr(energy->central_energy + 1) - 1), P_central);
   else if (energy->central_momentum)
     or_set_central_momentum(coord, np, energy->central_momentum, P_central);
@@ -1622,6 +1633,44 @@ static double KahanPlus(double oldSum, double b, double *error) { /* src/kahan.c
   return sum;
 }
 
+/* do_match_energy, src/do_tracking.c:3376-3549 (serial, USE_KAHAN): the mean
+ * momentum of the np particles in row order as a Kahan sum; change_beam 0 moves
+ * the reference momentum to it (when they differ by more than 1e-14 relative)
+ * and re-expresses every delta, change_beam 1 moves every particle's momentum by
+ * the same amount so that the mean is the reference momentum, keeping its time
+ * of flight.  A non-finite time coordinate is an error (the reference aborts). */
+long or_do_match_energy(double **coord, long np, double *P_central, long change_beam) {
+  long ip;
+  double P_average, dP_centroid, P, t, dp, dr, error = 0.0;
+  if (!np)
+    return 0;
+  P_average = 0;
+  for (ip = 0; ip < np; ip++)
+    P_average = KahanPlus(P_average, (*P_central) * (1 + coord[ip][5]), &error);
+  P_average /= np;
+  if (!change_beam) {
+    if (fabs(P_average - (*P_central)) / (*P_central) > 1e-14) {
+      dp = (*P_central - P_average) / P_average;
+      dr = (*P_central) / P_average;
+      for (ip = 0; ip < np; ip++)
+        coord[ip][5] = dp + coord[ip][5] * dr;
+      *P_central = P_average;
+    }
+  } else {
+    dP_centroid = *P_central - P_average;
+    for (ip = 0; ip < np; ip++) {
+      P = (1 + coord[ip][5]) * (*P_central);
+      t = coord[ip][4] / (P / sqrt(P * P + 1));
+      P += dP_centroid;
+      coord[ip][5] = (P - *P_central) / (*P_central);
+      coord[ip][4] = t * (P / sqrt(P * P + 1));
+      if (isnan(coord[ip][4]) || isinf(coord[ip][4]))
+        return or_fail("bad time coordinate in do_match_energy");
+    }
+  }
+  return np;
+}
+
 static double beta_from_delta(double p, double delta) { /* src/do_tracking.c:95-98 */
   p *= 1 + delta;
   return (p / sqrt(p * p + 1));
@@ -1750,15 +1799,17 @@ static void add_to_particle_energy(double *coord, double timeOfFlight, double Po
 }
 
 /* simple_rf_cavity -> trackRfCavityWithWakes, src/simple_rfca.c:264-985 (no wakes,
- * no LSC, Q = 0, change_t = 0, change_p0 = 0, no linearize/lockPhase, no backtrack,
- * body focus model none) */
+ * no LSC, Q = 0, no linearize/lockPhase, no backtrack, body focus model none);
+ * CHANGE_T takes whole RF periods off the time coordinates, judged on the
+ * particle in row 0 (:488-498, :586-588, :864-865); CHANGE_P0 matches the
+ * reference momentum to the surviving beam (:977-982) */
 long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_central, double zEnd) {
   long ip, nKicks, ik;
   double length, omega, volt, tau, dtLight, phase, t0, To, timeOffset, dgamma = 0, dgammaMax = 0;
   double P, gamma, gamma1, beta_i, t, dt, x, xp, dc4, *coord;
   int matrixMethod = 0, same_dgamma = 0;
 
-  if (rfca->Q != 0 || rfca->change_t || rfca->change_p0 || rfca->linearize || rfca->lockPhase)
+  if (rfca->Q != 0 || rfca->linearize || rfca->lockPhase)
     return or_fail("RFCA option not supported");
   if (np <= 0)
     return np;
@@ -1813,7 +1864,6 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
     To = PIx2 / omega;
   } else
     t0 = To = 0;
-  (void)To;
   phase += rfca->phase * PI / 180.0;
 
   if (omega == 0 && tau == 0) {
@@ -1822,6 +1872,14 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
     same_dgamma = 1;
   }
   timeOffset = 0;
+  if (omega && rfca->change_t && np != 0) {
+    coord = part[0];
+    P = *P_central * (1 + coord[5]);
+    beta_i = P / (gamma = sqrt(sqr(P) + 1));
+    t = coord[4] / (c_mks * beta_i);
+    if (omega != 0 && t > (0.9 * To) && rfca->change_t)
+      timeOffset = ((long)(t / To + 0.5)) * To;
+  }
 
   for (ip = 0; ip < np; ip++) {
     coord = part[ip];
@@ -1843,6 +1901,8 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
         P = *P_central * (1 + coord[5]);
         beta_i = P / (gamma = sqrt(sqr(P) + 1));
         t = (coord[4] + dc4) / (c_mks * beta_i) - timeOffset;
+        if (ik == 0 && timeOffset && rfca->change_t)
+          coord[4] = t * c_mks * beta_i - dc4;
         if ((dt = t - t0) < 0)
           dt = 0;
         (void)dt;
@@ -1894,6 +1954,8 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
         beta_i = P / (gamma = sqrt(sqr(P) + 1));
         ds1 = length / 2 * sqrt(1 + sqr(coord[1]) + sqr(coord[3]));
         t = (coord[4] + ds1) / (c_mks * beta_i) - timeOffset;
+        if (timeOffset && rfca->change_t)
+          coord[4] = t * c_mks * beta_i - ds1;
         if ((dt = t - t0) < 0)
           dt = 0;
         if (!same_dgamma) {
@@ -1939,7 +2001,13 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
     coord[0] += rfca->dx;
     coord[2] += rfca->dy;
   }
-  return or_removeInvalidParticles(part, np, zEnd, *P_central);
+  np = or_removeInvalidParticles(part, np, zEnd, *P_central);
+  if (rfca->change_p0) {
+    long r = or_do_match_energy(part, np, P_central, 0);
+    if (r < 0)
+      return r;
+  }
+  return np;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -2122,6 +2190,13 @@ long or_element_flat(double *flat, long np, const EH_ELEMENT_LIST *eptr, double 
   default:
     r = or_fail("element type not supported");
   }
+  free(rows);
+  return r;
+}
+
+long or_do_match_energy_flat(double *flat, long np, double *P_central, long change_beam) {
+  double **rows = or_rows(flat, np);
+  long r = or_do_match_energy(rows, np, P_central, change_beam);
   free(rows);
   return r;
 }

@@ -51,6 +51,8 @@ def load_path(path):
     lib.or_elimit_amplitudes_flat.argtypes = [vp, C.c_double, C.c_double, C.c_long, C.c_double, C.c_double, C.c_long,
                                               C.c_long, C.c_long]
     lib.or_elimit_amplitudes_flat.restype = C.c_long
+    lib.or_do_match_energy_flat.argtypes = [vp, C.c_long, dp, C.c_long]
+    lib.or_do_match_energy_flat.restype = C.c_long
     lib.or_ipow.argtypes = [C.c_double, C.c_long]
     lib.or_ipow.restype = C.c_double
     lib.or_get_phase_reference.argtypes = [dp, C.c_long]
@@ -150,9 +152,10 @@ def get_phase_reference(ref, libm=False):
     return st, ph.value
 
 
-def element(coords, lattice, index, p_central, z_start, z_end, maxamp=None, libm=False, lib=None):
+def element(coords, lattice, index, p_central, z_start, z_end, maxamp=None, libm=False, lib=None, with_pc=False):
     """Apply one element routine (no sweeps) to a copy of coords (lib: a checker
-    build other than the default, e.g. from build_variant)."""
+    build other than the default, e.g. from build_variant); with_pc: also return
+    the reference momentum afterwards."""
     lib = lib or load(libm)
     rows = np.array(coords, dtype=np.float64, order="C", copy=True)
     arr, keep = lattice.c_beamline()
@@ -161,7 +164,17 @@ def element(coords, lattice, index, p_central, z_start, z_end, maxamp=None, libm
     el = C.cast(C.byref(arr, index * C.sizeof(arr._type_)), C.c_void_p)
     n = _check(lib, lib.or_element_flat(rows.ctypes.data, rows.shape[0], el, C.byref(pc), z_start, z_end, amp))
     del keep
-    return rows, n
+    return (rows, n, pc.value) if with_pc else (rows, n)
+
+
+def match_energy(coords, p_central, change_beam, libm=False):
+    """do_match_energy on a copy of coords (every row taken as a survivor);
+    returns (rows, p_central)"""
+    lib = load(libm)
+    rows = np.array(coords, dtype=np.float64, order="C", copy=True)
+    pc = C.c_double(p_central)
+    _check(lib, lib.or_do_match_energy_flat(rows.ctypes.data, rows.shape[0], C.byref(pc), change_beam))
+    return rows, pc.value
 
 
 def field_table(b, c, h, nonlinear=1, expansion_order=0):

@@ -13,7 +13,7 @@ struct eh_beam {
   double rows[64][EH_NPROP];
 };
 static struct {
-  long uploads, downloads, mp2, csb, rf, drift, tp, lim, elim, rip, off, scm, pass_set;
+  long uploads, downloads, mp2, csb, rf, drift, tp, lim, elim, rip, off, scm, match, pass_set;
   long last_pass;
   EH_MAXAMP last_amp; /* the aperture of the last element call */
   long linked;        /* cavity calls that found their group's phase set */
@@ -173,6 +173,14 @@ long eh_set_central_momentum(eh_beam *b, long n, double Pn, double *P) {
   *P = Pn;
   M.scm++;
   return track_stand_in(b, n);
+}
+long eh_do_match_energy(eh_beam *b, long n, double *P, long change_beam) {
+  if (n != b->alive)
+    return EH_ERR_ARG;
+  if (!change_beam)
+    *P += 1.0;
+  M.match++;
+  return n;
 }
 
 /* ---- recording stand-ins for elegant's routines ---- */
@@ -463,6 +471,13 @@ int ah_scenario_host_fallback(void) {
     return 10;
   if (gpu_findFiducialTime(8, 0, 0, 5000.0, FID_MODE_TMEAN) != 7.0 || E.fid != 1)
     return 11;
+  /* momentum matching runs on the device copy */
+  {
+    double P = 5000.0;
+    gpu_do_match_energy(8, &P, 0);
+    if (M.match != 1 || E.match != 0 || P != 5001.0)
+      return 13;
+  }
   gpuBaseDealloc();
   return E.bomb ? 12 : 0;
 }
