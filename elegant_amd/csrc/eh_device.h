@@ -74,7 +74,11 @@ typedef struct {
   /* the descriptor needs the full instantiation of the maps: an elliptical
    * aperture, a bend edge method 3 or an edge kick limit */
   int32_t full_only;
-  double z_start, z_end; /* last_z and z of do_tracking for this element */
+  /* last_z and z of do_tracking for this element (z_start, z_end).  The maps
+   * take them through a pointer (zp) and read them only for lost particles:
+   * the fused kernel passes a per-position table instead, so that positions
+   * with equal constants share one descriptor. */
+  double zpos[2];
   double Po;
 
   /* ---- drifts ---- */
@@ -597,7 +601,8 @@ EH_HD void eh_mface2(eh_p6 &p, const eh_desc &d, bool s_too) {
  * term.  RAD: the element radiates (d.rad); FULL: the options only the full
  * instantiation of the maps carries (elliptical apertures; for bends also edge
  * method 3 and edge kick limits) are compiled in; the lean one has neither. */
-template <bool RAD, bool FULL = RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh_desc &d, int pi) {
+template <bool RAD, bool FULL = RAD>
+EH_HD int eh_multipole_t(eh_p6 &p, const eh_desc &d, int pi, const double *__restrict__ zp) {
   eh_mface1(p, d);
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
   const int ell = FULL ? d.ap_ell[pi] : 0;
@@ -745,15 +750,15 @@ template <bool RAD, bool FULL = RAD> EH_HD int eh_multipole_t(eh_p6 &p, const eh
     p.dp = dp;
   }
   if (lost) {
-    p.s = d.z_start + dzLoss;
+    p.s = zp[0] + dzLoss;
     p.dp = d.Po * (1 + p.dp);
   }
   eh_mface2(p, d, !lost);
   return lost ? (entry ? 2 : 1) : 0;
 }
 
-EH_HD int eh_multipole(eh_p6 &p, const eh_desc &d, int pi) {
-  return d.rad ? eh_multipole_t<true, true>(p, d, pi) : eh_multipole_t<false, true>(p, d, pi);
+EH_HD int eh_multipole(eh_p6 &p, const eh_desc &d, int pi, const double *__restrict__ zp) {
+  return d.rad ? eh_multipole_t<true, true>(p, d, pi, zp) : eh_multipole_t<false, true>(p, d, pi, zp);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -827,7 +832,7 @@ EH_HD bool eh_mult_out(double x, double xp, double y, double yp) {
  * lost in the kicks keeps the values reached; both get s = z_start and
  * delta = Po*(1+delta).  Zero strength is an exact drift, set up by the host.
  * Returns true if lost. */
-template <bool RAD> EH_HD bool eh_mult_t(eh_p6 &p, const eh_desc &d) {
+template <bool RAD> EH_HD bool eh_mult_t(eh_p6 &p, const eh_desc &d, const double *__restrict__ zp) {
   eh_mface1(p, d);
   const double drift = d.mdrift;
   double x = p.x, xp = p.xp, y = p.y, yp = p.yp, s = 0, dp = p.dp, qx, qy;
@@ -902,14 +907,14 @@ template <bool RAD> EH_HD bool eh_mult_t(eh_p6 &p, const eh_desc &d) {
     lost = is_lost || eh_mult_out(x, xp, y, yp);
   }
   if (lost) {
-    p.s = d.z_start;
+    p.s = zp[0];
     p.dp = d.Po * (1 + p.dp);
   }
   eh_mface2(p, d, true);
   return lost;
 }
 
-EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d) { return eh_mult_t<true>(p, d); }
+EH_HD bool eh_mult(eh_p6 &p, const eh_desc &d, const double *__restrict__ zp) { return eh_mult_t<true>(p, d, zp); }
 
 /* one row of the order-1 loop of track_particles (src/matrix.c:424-460):
  * C[i] (zero here) plus R[i][j]*ini[j] for j = 5 down to 0, zero entries
@@ -1186,7 +1191,8 @@ EH_HD void eh_edge_simple(double &xp, double &yp, double x, double y, double dp,
  * integrate_csbend_ordn (src/csbend.c:1459-1720) and the canonical conversions
  * (:1436-1454).  Returns true if lost.  RAD and FULL as for eh_multipole_t. */
 template <bool RAD, bool FULL = RAD>
-EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms) {
+EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms,
+                       const double *__restrict__ zp) {
   const double apx = d.ap_xmax[pi], apy = d.ap_ymax[pi];
   const int ell = FULL ? d.ap_ell[pi] : 0;
   const double rho0 = d.rho0, rho_actual = d.rho_actual;
@@ -1326,7 +1332,7 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
     p.y = X * d.sin_tt + Y * d.cos_tt;
     p.xp = QX * d.cos_tt - QY * d.sin_tt;
     p.yp = QX * d.sin_tt + QY * d.cos_tt;
-    p.s = d.z_start + dz_lost;
+    p.s = zp[0] + dz_lost;
     p.dp = d.Po * (1 + DP);
     return true;
   }
@@ -1381,8 +1387,9 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   return false;
 }
 
-EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms) {
-  return d.rad ? eh_csbend_t<true, true>(p, d, pi, terms) : eh_csbend_t<false, true>(p, d, pi, terms);
+EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restrict__ terms,
+                     const double *__restrict__ zp) {
+  return d.rad ? eh_csbend_t<true, true>(p, d, pi, terms, zp) : eh_csbend_t<false, true>(p, d, pi, terms, zp);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1390,7 +1397,7 @@ EH_HD bool eh_csbend(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__restr
  * matrix method :824-967, kick method :567-660, add_to_particle_energy
  * :987-1006) followed by removeInvalidParticles (src/limit_amplitudes.c:289-330).
  * Returns true if lost. */
-EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d) {
+EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d, const double *__restrict__ zp) {
   const double c_mks = 2.99792458e8;
   const double Pc = d.Po;
   if (d.rf_zero_volt) {
@@ -1517,7 +1524,7 @@ EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d) {
   /* removeInvalidParticles */
   if (eh_isnan(p.x) || eh_isnan(p.xp) || eh_isnan(p.y) || eh_isnan(p.yp) || eh_isnan(p.s) || eh_isnan(p.dp) ||
       p.dp <= -1) {
-    p.s = d.z_end;
+    p.s = zp[1];
     p.dp = Pc * (1 + p.dp);
     return true;
   }
@@ -1529,7 +1536,8 @@ EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d) {
  * survives, else the loss stage (EH_STAGE_ELEMENT possibly with
  * EH_STAGE_INHERIT). */
 template <bool RAD>
-EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms) {
+EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms,
+                             const double *__restrict__ zp) {
   const int pi = pass > 0 ? 1 : 0;
   bool lost = false;
   int flag = 0;
@@ -1548,16 +1556,16 @@ EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fter
   case 4:
   case 5:
   case 9: { /* KQUSE */
-    const int m = RAD ? eh_multipole(p, d, pi) : eh_multipole_t<false>(p, d, pi);
+    const int m = RAD ? eh_multipole(p, d, pi, zp) : eh_multipole_t<false>(p, d, pi, zp);
     lost = m != 0;
     flag = m == 2 ? EH_STAGE_INHERIT : 0;
     break;
   }
   case 6:
-    lost = RAD ? eh_csbend(p, d, pi, terms) : eh_csbend_t<false>(p, d, pi, terms);
+    lost = RAD ? eh_csbend(p, d, pi, terms, zp) : eh_csbend_t<false>(p, d, pi, terms, zp);
     break;
   case 7:
-    lost = eh_rfca(p, d);
+    lost = eh_rfca(p, d, zp);
     break;
   /* the full instantiation (RAD) also carries the element types below; the
    * lean one refuses them (the host runs it only on lines without them) */
@@ -1565,7 +1573,7 @@ EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fter
     if constexpr (!RAD)
       return EH_STAGE_REFUSED;
     if (d.nslices)
-      lost = eh_mult_t<RAD>(p, d);
+      lost = eh_mult_t<RAD>(p, d, zp);
     break;
   case 11: /* ROTATE, src/do_tracking.c:1836-1852 */
     if constexpr (!RAD)
@@ -1595,13 +1603,13 @@ EH_HD int eh_apply_element_t(eh_p6 &p, const eh_desc &d, int pass, const eh_fter
   if (d.post_ap[pi]) {
     bool out;
     if (RAD && d.post_ell[pi])
-      out = eh_elimit_one(p, d.ell_a2[pi], d.ell_b2[pi], d.ell_xe[pi], d.ell_ye[pi], d.z_end, d.Po, d.extrap);
+      out = eh_elimit_one(p, d.ell_a2[pi], d.ell_b2[pi], d.ell_xe[pi], d.ell_ye[pi], zp[1], d.Po, d.extrap);
     else
-      out = eh_limit_one(p, d.post_xmax[pi], d.post_ymax[pi], d.z_end, d.Po, d.extrap);
+      out = eh_limit_one(p, d.post_xmax[pi], d.post_ymax[pi], zp[1], d.Po, d.extrap);
     if (out)
       return EH_STAGE_MAXAMP;
   }
-  if (eh_limit_one(p, EH_DBL_MAX, EH_DBL_MAX, d.z_end, d.Po, 0))
+  if (eh_limit_one(p, EH_DBL_MAX, EH_DBL_MAX, zp[1], d.Po, 0))
     return EH_STAGE_NAN;
   return -1;
 }
@@ -1630,7 +1638,7 @@ EH_HD bool eh_lean_capable(const eh_desc &d) { return !d.rad && d.type < 10 && !
 
 
 EH_HD int eh_apply_element(eh_p6 &p, const eh_desc &d, int pass, const eh_fterm *__restrict__ terms) {
-  return eh_apply_element_t<true>(p, d, pass, terms);
+  return eh_apply_element_t<true>(p, d, pass, terms, d.zpos);
 }
 
 #endif

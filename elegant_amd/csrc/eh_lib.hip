@@ -20,6 +20,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "elegant_hip.h"
@@ -753,8 +755,8 @@ static long build_desc(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c, st
   d->counted = !(e->type == EH_T_MARK || e->type == EH_T_MAXAMP);
   d->adj_pk = d->adj_pdx = d->adj_pdy = d->adj_pfse = d->adj_anchor = -1;
   d->Po = c.Po;
-  d->z_start = c.z_start;
-  d->z_end = c.z_end;
+  d->zpos[0] = c.z_start;
+  d->zpos[1] = c.z_end;
   for (int pi = 0; pi < 2; pi++) {
     const EH_MAXAMP *m = c.ap[pi];
     d->ap_xmax[pi] = m ? m->x_max : 0;
@@ -854,8 +856,8 @@ static eh_kmp kmp_of(const eh_desc &d) {
       if (d.mdrift != 0)
         dz += d.dsh[st];
   k.has_offset = d.has_offset;
-  k.z_start = d.z_start;
-  k.zfull = d.z_start + dz;
+  k.z_start = d.zpos[0];
+  k.zfull = d.zpos[0] + dz;
   k.mdz = d.mdz;
   return k;
 }
@@ -941,6 +943,14 @@ static void settle_losses(const std::vector<eh_evrec> &log, long variant, const 
 
 /* ------------------------------------------------------------------------- */
 /* device storage */
+/* one position of a fused line: its z pair (z_start, z_end) and the byte
+ * offset of its descriptor from the start of the position table */
+struct eh_pos {
+  double z[2];
+  int64_t off;
+  int64_t pad_;
+};
+
 struct eh_soa {
   double *x, *xp, *y, *yp, *s, *dp;
   int32_t *row0;   /* row index at upload: the particle's identity */
@@ -957,6 +967,10 @@ struct eh_beam {
   unsigned long long *d_cnt = nullptr; /* [0] alive out, [1] graveyard append, [2] lost in place, [3] effort */
   eh_desc *d_desc = nullptr;
   long desc_cap = 0;
+  /* fused tracking: the position table and the distinct descriptors it points
+   * into (upload_fused_descs) */
+  char *d_fused = nullptr;
+  long fused_cap = 0; /* bytes */
   eh_fterm *d_terms = nullptr; /* CSBEND field-term pool */
   long terms_cap = 0;
   EH_VMATRIX *d_mat = nullptr; /* transport map of eh_track_particles */
@@ -976,15 +990,19 @@ struct eh_beam {
 
 static int g_device = -1;
 
-/* The static scratch of eh_do_tracking_noalloc: per GPU, 8 MiB holding the
- * launch counters (first 256 B), up to EH_SCRATCH_NDESC element descriptors and
- * up to EH_SCRATCH_NTERMS CSBEND field terms.  In host code `scratch` names a
- * host-side stand-in, not device memory: its device address on each GPU is
+/* The static scratch of eh_do_tracking_noalloc: per GPU, about 8.1 MiB holding
+ * the launch counters (first 256 B), the fused line's position table and
+ * distinct descriptors for up to EH_SCRATCH_NDESC positions, one descriptor for
+ * the per-element sweeps and up to EH_SCRATCH_NTERMS CSBEND field terms.  In host code `scratch` names
+ * a host-side stand-in, not device memory: its device address on each GPU is
  * taken once with hipGetSymbolAddress (eh_init) and kept in g_scratch. */
-#define EH_SCRATCH_BYTES (8l << 20)
 #define EH_SCRATCH_NDESC 4096l
-#define EH_SCRATCH_DESC_OFF 256l
-#define EH_SCRATCH_TERMS_OFF (EH_SCRATCH_DESC_OFF + EH_SCRATCH_NDESC * (long)sizeof(eh_desc))
+#define EH_SCRATCH_SINGLE_BYTES (((long)sizeof(eh_desc) + 255l) / 256l * 256l)
+#define EH_SCRATCH_BYTES ((8l << 20) + EH_SCRATCH_NDESC * (long)sizeof(eh_pos) + EH_SCRATCH_SINGLE_BYTES)
+#define EH_SCRATCH_FUSED_OFF 256l
+#define EH_SCRATCH_FUSED_BYTES (EH_SCRATCH_NDESC * ((long)sizeof(eh_pos) + (long)sizeof(eh_desc)))
+#define EH_SCRATCH_DESC_OFF (EH_SCRATCH_FUSED_OFF + EH_SCRATCH_FUSED_BYTES)
+#define EH_SCRATCH_TERMS_OFF (EH_SCRATCH_DESC_OFF + EH_SCRATCH_SINGLE_BYTES)
 #define EH_SCRATCH_NTERMS ((EH_SCRATCH_BYTES - EH_SCRATCH_TERMS_OFF) / (long)sizeof(eh_fterm))
 /* field terms of one CSBEND at most: both sparse lists and both dense blocks */
 #define EH_MAX_TERMS_PER_BEND (2l * EH_MAXFTERMS + 2l * 36l)
@@ -1066,6 +1084,8 @@ extern "C" void eh_beam_free(eh_beam *b) {
     (void)hipFree(b->d_mat);
   if (b->d_desc)
     (void)hipFree(b->d_desc);
+  if (b->d_fused)
+    (void)hipFree(b->d_fused);
   if (b->d_terms)
     (void)hipFree(b->d_terms);
   if (b->ev0)
@@ -1188,20 +1208,20 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_plain(eh_soa a, lon
       case EH_T_KSEXT:
       case EH_T_KOCT:
       case EH_T_KQUSE: {
-        const int m = eh_multipole(p, d, 0);
+        const int m = eh_multipole(p, d, 0, d.zpos);
         lost = m != 0;
         inherit = m == 2;
         break;
       }
       case EH_T_CSBEND:
-        lost = eh_csbend(p, d, 0, terms);
+        lost = eh_csbend(p, d, 0, terms, d.zpos);
         break;
       case EH_T_RFCA:
-        lost = eh_rfca(p, d);
+        lost = eh_rfca(p, d, d.zpos);
         break;
       case EH_T_MULT:
         if (d.nslices)
-          lost = eh_mult(p, d);
+          lost = eh_mult(p, d, d.zpos);
         break;
       case EH_T_ROTATE:
         eh_rotate(p, d.ncos, d.nsin);
@@ -1315,7 +1335,7 @@ __global__ void __launch_bounds__(EH_BLOCK) k_compact(eh_soa a, long n, eh_soa b
  * where and how in one word (eh_loss_code, decoded by eh_loss_words): a single
  * loop-carried register. */
 template <bool RAD>
-__device__ __forceinline__ void track_span(eh_p6 &p, bool &alive, const eh_desc *__restrict__ descs,
+__device__ __forceinline__ void track_span(eh_p6 &p, bool &alive, const eh_pos *__restrict__ pos,
                                            const eh_fterm *__restrict__ terms, int nelem, int p0, int e0, int p1,
                                            int e1, int32_t &lcode, unsigned long long &units) {
   int pp = p0, ee = e0;
@@ -1323,10 +1343,11 @@ __device__ __forceinline__ void track_span(eh_p6 &p, bool &alive, const eh_desc 
     const unsigned long long live = __ballot(alive);
     if (live == 0)
       break;
-    const eh_desc *d = descs + ee;
+    const eh_pos *q = pos + ee;
+    const eh_desc *d = (const eh_desc *)((const char *)pos + q->off);
     units += (unsigned long long)__popcll(live) * (unsigned)d->counted;
     if (alive) {
-      int st = eh_apply_element_t<RAD>(p, *d, pp, terms);
+      int st = eh_apply_element_t<RAD>(p, *d, pp, terms, q->z);
       if (st >= 0) {
         alive = false;
         lcode = eh_loss_code(pp - p0, nelem, ee, st);
@@ -1374,10 +1395,12 @@ __device__ __forceinline__ void wave_store(const eh_p6 &p, bool in, bool alive, 
 }
 
 /* fused tracking of one lattice: each thread takes one particle through the span,
- * then survivors are compacted into b and lost particles appended to the graveyard */
+ * then survivors are compacted into b and lost particles appended to the graveyard.
+ * Position ee runs the descriptor at byte offset pos[ee].off from pos, with its
+ * z pair pos[ee].z. */
 template <bool RAD>
 __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_fused(eh_soa a, long n, eh_soa b, eh_soa g,
-                                                     const eh_desc *__restrict__ descs,
+                                                     const eh_pos *__restrict__ pos,
                                                      const eh_fterm *__restrict__ terms, int nelem, int p0, int e0,
                                                      int p1, int e1, int32_t event_base, unsigned long long *cnt) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1388,7 +1411,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_fused(eh_soa a, lon
     p = load_p(a, i);
   int32_t lcode = 0;
   unsigned long long units = 0;
-  track_span<RAD>(p, alive, descs, terms, nelem, p0, e0, p1, e1, lcode, units);
+  track_span<RAD>(p, alive, pos, terms, nelem, p0, e0, p1, e1, lcode, units);
   if (__lane_id() == 0 && units)
     atomicAdd(&cnt[3], units);
   /* identity and loss words are read and formed only now, so that none of them
@@ -1498,7 +1521,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_batch(eh_soa a, eh_
       const eh_desc *d = pool + __builtin_amdgcn_readfirstlane(idx[ee]);
       units += (unsigned long long)__popcll(live) * (unsigned)d->counted;
       if (alive) {
-        int st = eh_apply_element_t<RAD>(p, *d, pp, terms);
+        int st = eh_apply_element_t<RAD>(p, *d, pp, terms, d->zpos);
         if (st >= 0) {
           alive = false;
           lcode = eh_loss_code(pp - p0, nelem, ee, st);
@@ -1532,6 +1555,50 @@ static long ensure_desc(eh_beam *b, long n) {
   b->d_desc = nullptr;
   HIPCHK(hipMalloc(&b->d_desc, sizeof(eh_desc) * n));
   b->desc_cap = n;
+  return EH_OK;
+}
+
+/* the line's descriptors for the fused kernel, in one buffer: a table of
+ * positions (eh_pos: z pair, byte offset of the descriptor) followed by the
+ * distinct descriptors.  Positions whose descriptors are equal apart from their
+ * z share one, so a ring of identical cells reads a few dozen descriptors
+ * instead of one per position. */
+static long upload_fused_descs(eh_beam *b, const std::vector<eh_desc> &hd) {
+  const long ne = (long)hd.size();
+  std::vector<eh_desc> uniq;
+  std::vector<eh_pos> pos(ne);
+  std::unordered_map<std::string, int64_t> seen;
+  const int64_t base = (int64_t)sizeof(eh_pos) * ne;
+  for (long ie = 0; ie < ne; ie++) {
+    eh_desc k;
+    std::memcpy(&k, &hd[ie], sizeof k);
+    pos[ie].z[0] = k.zpos[0];
+    pos[ie].z[1] = k.zpos[1];
+    pos[ie].pad_ = 0;
+    k.zpos[0] = k.zpos[1] = 0;
+    std::string key(reinterpret_cast<const char *>(&k), sizeof k);
+    auto it = seen.find(key);
+    if (it == seen.end()) {
+      it = seen.emplace(std::move(key), base + (int64_t)sizeof(eh_desc) * (int64_t)uniq.size()).first;
+      uniq.push_back(k);
+    }
+    pos[ie].off = it->second;
+  }
+  const long bytes = (long)base + (long)(sizeof(eh_desc) * uniq.size());
+  if (b->fused_cap < bytes) {
+    if (b->noalloc)
+      return fail(EH_ERR_ARG, "the line's %ld positions exceed the static scratch", ne);
+    if (b->d_fused)
+      HIPCHK(hipFree(b->d_fused));
+    b->d_fused = nullptr;
+    b->fused_cap = 0;
+    HIPCHK(hipMalloc(&b->d_fused, bytes));
+    b->fused_cap = bytes;
+  }
+  if (ne) {
+    HIPCHK(hipMemcpy(b->d_fused, pos.data(), (size_t)base, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_fused + base, uniq.data(), sizeof(eh_desc) * uniq.size(), hipMemcpyHostToDevice));
+  }
   return EH_OK;
 }
 
@@ -1708,8 +1775,8 @@ extern "C" long eh_multipole_tracking(eh_beam *b, long n_part, const EH_MULT *mu
   std::memset(&d, 0, sizeof d);
   d.type = EH_T_MULT;
   d.Po = Po;
-  d.z_start = z_start;
-  d.z_end = z_start + mult->length;
+  d.zpos[0] = z_start;
+  d.zpos[1] = z_start + mult->length;
   r = build_mult(&d, mult, Po);
   if (r < 0)
     return r;
@@ -2487,10 +2554,12 @@ static long launch_fused(eh_beam *b, long ne, int p0, int e0, int p1, int e1, do
     return r;
   if (rad)
     hipLaunchKernelGGL(k_fused<true>, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, b->buf[b->cur], n,
-                       b->buf[1 - b->cur], b->grave, b->d_desc, b->d_terms, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
+                       b->buf[1 - b->cur], b->grave, (const eh_pos *)b->d_fused, b->d_terms, (int)ne, p0, e0, p1,
+                       e1, base, b->d_cnt);
   else
     hipLaunchKernelGGL(k_fused<false>, dim3(grid_for(n)), dim3(EH_BLOCK), 0, b->stream, b->buf[b->cur], n,
-                       b->buf[1 - b->cur], b->grave, b->d_desc, b->d_terms, (int)ne, p0, e0, p1, e1, base, b->d_cnt);
+                       b->buf[1 - b->cur], b->grave, (const eh_pos *)b->d_fused, b->d_terms, (int)ne, p0, e0, p1,
+                       e1, base, b->d_cnt);
   HIPCHK(hipGetLastError());
   r = timed_end(b);
   if (r < 0)
@@ -2513,9 +2582,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
                         double *effort) {
   if (tpl < 1)
     tpl = 1;
-  long r = ensure_desc(b, ne);
-  if (r < 0)
-    return r;
+  long r;
   std::vector<eh_desc> hd(ne);
   std::vector<eh_fterm> pool;
   std::vector<double> zs(ne), ze(ne);
@@ -2664,7 +2731,9 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
     if (r < 0)
       return r;
   }
-  HIPCHK(hipMemcpy(b->d_desc, hd.data(), sizeof(eh_desc) * ne, hipMemcpyHostToDevice));
+  r = upload_fused_descs(b, hd);
+  if (r < 0)
+    return r;
   r = upload_terms(b, pool);
   if (r < 0)
     return r;
@@ -2708,9 +2777,8 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       r2 = build(je);
       if (r2 < 0)
         return r2;
-      HIPCHK(hipMemcpy(b->d_desc + je, &hd[je], sizeof(eh_desc), hipMemcpyHostToDevice));
     }
-    return EH_OK;
+    return upload_fused_descs(b, hd);
   };
   int p = 0, e = 0;
   if (dynamic) {
@@ -2745,7 +2813,9 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
         if (r2 < 0)
           return r2;
       }
-      HIPCHK(hipMemcpy(b->d_desc, hd.data(), sizeof(eh_desc) * ne, hipMemcpyHostToDevice));
+      long r2 = upload_fused_descs(b, hd);
+      if (r2 < 0)
+        return r2;
       return upload_terms(b, pool);
     };
     while (p < n_passes) {
@@ -2778,7 +2848,9 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
         r = build(e);
         if (r < 0)
           return r;
-        HIPCHK(hipMemcpy(b->d_desc + e, &hd[e], sizeof(eh_desc), hipMemcpyHostToDevice));
+        r = upload_fused_descs(b, hd);
+        if (r < 0)
+          return r;
       }
       int e1 = e;
       while (e1 < ne) {
@@ -2808,8 +2880,11 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
                             : eh_limit_amplitudes(b, m->x_max, m->y_max, b->n_alive, ze[je], P, 0, 0);
           if (r < 0)
             return r;
-          if (m->elliptical) /* the per-element sweep used descriptor slot 0 */
-            HIPCHK(hipMemcpy(b->d_desc, &hd[0], sizeof(eh_desc), hipMemcpyHostToDevice));
+          if (m->elliptical) { /* the per-element sweep used descriptor slot 0 */
+            r = upload_fused_descs(b, hd);
+            if (r < 0)
+              return r;
+          }
         }
         r = repropagate(e1 % ne, P);
         if (r < 0)
@@ -2862,7 +2937,9 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
         if (r < 0)
           return r;
       }
-      HIPCHK(hipMemcpy(b->d_desc, hd.data(), sizeof(eh_desc) * ne, hipMemcpyHostToDevice));
+      r = upload_fused_descs(b, hd);
+      if (r < 0)
+        return r;
       r = upload_terms(b, pool);
       if (r < 0)
         return r;
@@ -2928,14 +3005,20 @@ extern "C" long eh_do_tracking_noalloc(eh_beam *b, EH_ELEMENT_LIST *beamline, lo
   eh_fterm *terms0 = b->d_terms;
   long dcap0 = b->desc_cap, tcap0 = b->terms_cap;
   b->d_cnt = (unsigned long long *)base;
+  char *fused0 = b->d_fused;
+  long fcap0 = b->fused_cap;
   b->d_desc = (eh_desc *)(base + EH_SCRATCH_DESC_OFF);
-  b->desc_cap = EH_SCRATCH_NDESC;
+  b->desc_cap = 1;
+  b->d_fused = base + EH_SCRATCH_FUSED_OFF;
+  b->fused_cap = EH_SCRATCH_FUSED_BYTES;
   b->d_terms = (eh_fterm *)(base + EH_SCRATCH_TERMS_OFF);
   b->terms_cap = EH_SCRATCH_NTERMS;
   b->noalloc = true;
   long r = eh_do_tracking(b, beamline, n_elements, n_passes, P_central, EH_MODE_FUSED, turns_per_launch, stats);
   b->d_cnt = cnt0;
   b->d_desc = desc0;
+  b->d_fused = fused0;
+  b->fused_cap = fcap0;
   b->d_terms = terms0;
   b->desc_cap = dcap0;
   b->terms_cap = tcap0;
@@ -3834,7 +3917,7 @@ __global__ void __launch_bounds__(EH_BLOCK) EH_TRACK_WAVES k_adj_fwd(double *z, 
           a[5 * a_ld] = p.dp;
         }
         u += d->counted;
-        if (eh_apply_element_t<false>(p, *d, t, terms) >= 0)
+        if (eh_apply_element_t<false>(p, *d, t, terms, d->zpos) >= 0)
           live = false;
       }
     }

@@ -50,7 +50,7 @@ extern "C" long bh_lean_stage(int type, int ell, int *capable) {
   }
   *capable = eh_lean_capable(d);
   eh_p6 p = {1e-3, 0, 0, 0, 0, 0};
-  return eh_apply_element_t<false>(p, d, 0, nullptr);
+  return eh_apply_element_t<false>(p, d, 0, nullptr, d.zpos);
 }
 
 /* the loss code of track_span and k_batch decoded back (host check of the
@@ -189,7 +189,7 @@ static void bh_span(const bh_soa &a, bh_soa &out, bh_soa &grave, long &n_grave, 
       int pp = p0, ee = e0;
       while (live && (pp < p1 || (pp == p1 && ee < e1))) {
         const eh_desc &d = pool[idx[(size_t)v * nelem + ee]];
-        int st = eh_apply_element_t<true>(p, d, pp, terms.data());
+        int st = eh_apply_element_t<true>(p, d, pp, terms.data(), d.zpos);
         if (st >= 0) {
           live = false;
           lcode = eh_loss_code(pp - p0, nelem, ee, st);
