@@ -22,6 +22,7 @@
 
 /* the per-particle maps compile for the GPU and, for test harnesses, the host */
 #define EH_HD __host__ __device__ __forceinline__
+#define EH_PI_D 3.14159265358979323846
 
 #define EH_MAXSUB 8     /* integrator sub-steps (order 6) */
 #define EH_MAXFTERMS 66 /* field terms per component (expansion order 10) */
@@ -106,6 +107,11 @@ typedef struct {
   int32_t csb_nfx, csb_nfy;
   int32_t csb_edge1, csb_edge2;  /* the edge applies (EDGE*_EFFECTS 1 or 3) */
   int32_t csb_esimple1, csb_esimple2; /* EDGE*_EFFECTS 3: applySimpleDipoleEdgeKick */
+  /* EDGE*_EFFECTS 2 (dipoleFringeKHwang): per edge 1/cos, tan, sin and cos of
+   * the edge angle, the full gap 2*HGAP, FINT and the pole-face curvature H;
+   * csb_hw_order2 for EDGE_ORDER > 1 */
+  int32_t csb_hwang1, csb_hwang2, csb_hw_order2;
+  double csb_hw1[7], csb_hw2[7];
   double csb_klim1, csb_klim2;   /* edge x' kick limits (<= 0: none) */
   double rho0, rho_actual;
   int32_t csb_recip;        /* 1: divide by rho0 and rho_actual through the reciprocals below */
@@ -1187,6 +1193,75 @@ EH_HD void eh_edge_simple(double &xp, double &yp, double x, double y, double dp,
   yp = qy * f;
 }
 
+/* dipoleFringeKHwang (src/csbend.c:4336-4410, EDGE_EFFECTS 2) on the canonical
+ * momenta: K. Hwang's dipole fringe map at the entrance (in_fringe -1) or the
+ * exit (+1), with c = {1/cos e, tan e, sin e, cos e, gap, FINT, H} and the
+ * higher-order terms when higher; the reference's operation order, SDDS's
+ * ipow2/ipow3 as x*x and (x*x)*x */
+EH_HD void eh_fringe_hwang(double &x, double &px, double &y, double &py, double dp0, double rho,
+                           const double *__restrict__ c, double in_fringe, bool higher) {
+  const double k0 = (EH_PI_D * EH_PI_D) / 6., k2 = c[5], k3 = 1.0 * 1. / 6., k4 = -1.0 * (EH_PI_D * EH_PI_D) / 3.,
+               k5 = 0.0, k6 = -1.0;
+  const double sec_e = c[0], tan_e = c[1], sin_e = c[2], cos_e = c[3], gap = c[4], Rhe = c[6];
+  const double x0 = x, px0 = px, y0 = y, py0 = py;
+  const double sec2 = sec_e * sec_e, tan2 = tan_e * tan_e, gap2 = gap * gap, cos3 = cos_e * cos_e * cos_e;
+  const double sin2 = sin_e * sin_e, rho2 = rho * rho;
+  double dx, dpx, dy, dpy;
+  if (higher) {
+    dx = in_fringe * sec2 * gap2 * k0 / rho / (1 + dp0) + in_fringe * (x0 * x0) * tan2 / 2 / rho / (1 + dp0) -
+         in_fringe * (y0 * y0) * sec2 / 2 / rho / (1 + dp0);
+    dy = -in_fringe * x0 * y0 * tan2 / rho / (1 + dp0);
+    if (in_fringe == -1.) {
+      dpx = -1. * (sec_e * sec_e * sec_e) * sin_e * gap2 * k0 / rho / rho / (1 + dp0) + tan_e * x0 / rho +
+            (y0 * y0) / 2 * (2 * (tan_e * tan_e * tan_e)) / rho2 / (1 + dp0) + (y0 * y0) / 2 * (tan_e) / rho2 / (1 + dp0) -
+            in_fringe * (x0 * px0 - y0 * py0) * tan2 / rho / (1 + dp0) + k4 * sin2 * gap2 / 2 / cos3 / rho * Rhe -
+            k5 * x0 * sin_e * gap / cos3 / rho * Rhe + k6 * (y0 * y0 - x0 * x0) / 2 / cos3 / rho * Rhe;
+      dpy = -1. * tan_e * y0 / rho + k2 * y0 * (1 + sin2) * gap / (1 + dp0) / rho2 / cos3 +
+            in_fringe * (x0 * py0 + y0 * px0) * tan2 / rho / (1 + dp0) + in_fringe * y0 * px0 / rho / (1 + dp0) +
+            k3 * (y0 * y0 * y0) * (2. / 3. / cos_e - 4. / 3. / cos3) / (1 + dp0) / rho / rho / gap +
+            k6 * x0 * y0 / cos3 / rho * Rhe;
+    } else {
+      dpx = tan_e * x0 / rho - (y0 * y0) / 2 * (1 * (tan_e * tan_e * tan_e)) / rho2 / (1 + dp0) -
+            (x0 * x0) / 2 * (1 * (tan_e * tan_e * tan_e)) / rho2 / (1 + dp0) -
+            in_fringe * (x0 * px0 - y0 * py0) * tan2 / rho / (1 + dp0) + k4 * sin2 * gap2 / 2 / cos3 / rho * Rhe -
+            k5 * x0 * sin_e * gap / cos3 / rho * Rhe + k6 * (y0 * y0 - x0 * x0) / 2 / cos3 / rho * Rhe;
+      dpy = -1. * tan_e * y0 / rho + k2 * y0 * (1 + sin2) * gap / (1 + dp0) / rho2 / cos3 +
+            in_fringe * (x0 * py0 + y0 * px0) * tan2 / rho / (1 + dp0) + in_fringe * y0 * px0 / rho / (1 + dp0) +
+            x0 * y0 * sec2 * tan_e / rho2 / (1 + dp0) +
+            k3 * (y0 * y0 * y0) * (2. / 3. / cos_e - 4. / 3. / cos3) / (1 + dp0) / rho / rho / gap -
+            k5 * y0 * sin_e * gap / cos3 / rho * Rhe + k6 * x0 * y0 / cos3 / rho * Rhe;
+    }
+  } else {
+    dx = in_fringe * sec2 * gap2 * k0 / rho / (1 + dp0);
+    dy = 0;
+    if (in_fringe == -1.) {
+      dpx = -1. * (sec_e * sec_e * sec_e) * sin_e * gap2 * k0 / rho / rho / (1 + dp0) + tan_e * x0 / rho +
+            k4 * sin2 * gap2 / 2 / cos3 / rho * Rhe - k5 * x0 * sin_e * gap / cos3 / rho * Rhe;
+      dpy = -1. * tan_e * y0 / rho + k2 * y0 * (1 + sin2) * gap / (1 + dp0) / rho2 / cos3;
+    } else {
+      dpx = tan_e * x0 / rho + k4 * sin2 * gap2 / 2 / cos3 / rho * Rhe - k5 * x0 * sin_e * gap / cos3 / rho * Rhe;
+      dpy = -1. * tan_e * y0 / rho + k2 * y0 * (1 + sin2) * gap / (1 + dp0) / rho2 / cos3 -
+            k5 * y0 * sin_e * gap / cos3 / rho * Rhe;
+    }
+  }
+  x = x0 + dx;
+  px = px0 + dpx;
+  y = y0 + dy;
+  py = py0 + dpy;
+}
+
+/* EDGE_EFFECTS 2 at one edge (src/csbend.c:1139-1159, :1319-1338): the slopes
+ * to canonical momenta, Hwang's map, and back */
+EH_HD void eh_edge_hwang(double &x, double &xp, double &y, double &yp, double dp, double rho_actual,
+                         const double *__restrict__ c, double in_fringe, bool higher) {
+  double f = (1 + dp) / eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
+  double qx = xp * f, qy = yp * f;
+  eh_fringe_hwang(x, qx, y, qy, dp, rho_actual, c, in_fringe, higher);
+  f = 1 / eh_sqrt(eh_sqr(1 + dp) - eh_sqr(qx) - eh_sqr(qy));
+  xp = qx * f;
+  yp = qy * f;
+}
+
 /* CSBEND: track_through_csbend (src/csbend.c:1088-1403) for one particle, with
  * integrate_csbend_ordn (src/csbend.c:1459-1720) and the canonical conversions
  * (:1436-1454).  Returns true if lost.  RAD and FULL as for eh_multipole_t. */
@@ -1209,7 +1284,9 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   const double dp0 = dp;
   if (d.csb_edge1) {
     double rho = (1 + dp) * rho_actual;
-    if (FULL && d.csb_esimple1) {
+    if (FULL && d.csb_hwang1) {
+      eh_edge_hwang(x, xp, y, yp, dp, rho_actual, d.csb_hw1, -1., d.csb_hw_order2);
+    } else if (FULL && d.csb_esimple1) {
       eh_edge_simple(xp, yp, x, y, dp, rho_actual, d.tan_e1, d.e1, d.psi1, d.csb_klim1);
     } else if (!d.csb_eorder2) {
       double delta_xp = d.tan_e1 / rho * x;
@@ -1363,7 +1440,9 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   dp = DP;
   if (d.csb_edge2) {
     double rho = (1 + dp) * rho_actual;
-    if (FULL && d.csb_esimple2) {
+    if (FULL && d.csb_hwang2) {
+      eh_edge_hwang(x, xp, y, yp, dp, rho_actual, d.csb_hw2, 1., d.csb_hw_order2);
+    } else if (FULL && d.csb_esimple2) {
       eh_edge_simple(xp, yp, x, y, dp, rho_actual, d.tan_e2, d.e2, d.psi2, d.csb_klim2);
     } else if (!d.csb_eorder2) {
       double delta_xp = d.tan_e2 / rho * x;

@@ -389,9 +389,11 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
     return fail(EH_ERR_UNSUPPORTED, "CSBEND integration_order is invalid");
   if (cb->malignMethod != 0 || cb->etilt != 0 || cb->expandHamiltonian)
     return fail(EH_ERR_UNSUPPORTED, "CSBEND option not supported");
-  if ((cb->edge1_effects != 0 && cb->edge1_effects != 1 && cb->edge1_effects != 3) ||
-      (cb->edge2_effects != 0 && cb->edge2_effects != 1 && cb->edge2_effects != 3))
+  if (cb->edge1_effects < 0 || cb->edge1_effects > 3 || cb->edge2_effects < 0 || cb->edge2_effects > 3)
     return fail(EH_ERR_UNSUPPORTED, "CSBEND edge method not supported");
+  if (cb->edge_order > 1 && (cb->edge1_effects == 2 || cb->edge2_effects == 2) && cb->hgap == 0)
+    return fail(EH_ERR_ARG, "CSBEND has EDGE_ORDER>1 and EDGE[12]_EFFECTS==2, but HGAP=0. This gives undefined "
+                            "results.");
   if (cb->nSlices < 1)
     return fail(EH_ERR_UNSUPPORTED, "invalid number of steps (integrate_csbend_ordn)");
   double b[9], cc[9];
@@ -544,10 +546,26 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
     d->csb_ds[j] = s * kickFrac[j];
   }
   d->csb_dzl = cb->nSlices * s;
-  d->csb_edge1 = (cb->edgeFlags & EH_BEND_EDGE1_EFFECTS) && (cb->edge1_effects == 1 || cb->edge1_effects == 3);
-  d->csb_edge2 = (cb->edgeFlags & EH_BEND_EDGE2_EFFECTS) && (cb->edge2_effects == 1 || cb->edge2_effects == 3);
+  d->csb_edge1 = (cb->edgeFlags & EH_BEND_EDGE1_EFFECTS) && cb->edge1_effects >= 1;
+  d->csb_edge2 = (cb->edgeFlags & EH_BEND_EDGE2_EFFECTS) && cb->edge2_effects >= 1;
   d->csb_esimple1 = d->csb_edge1 && cb->edge1_effects == 3;
   d->csb_esimple2 = d->csb_edge2 && cb->edge2_effects == 3;
+  /* EDGE_EFFECTS 2, dipoleFringeKHwang (src/csbend.c:1139-1159, :1319-1338):
+   * the edge angle's functions, the full gap, FINT (both edges) and H */
+  d->csb_hwang1 = d->csb_edge1 && cb->edge1_effects == 2;
+  d->csb_hwang2 = d->csb_edge2 && cb->edge2_effects == 2;
+  d->csb_hw_order2 = cb->edge_order > 1;
+  for (int k = 0; k < 2; k++) {
+    const double e = k ? e2 : e1;
+    double *hw = k ? d->csb_hw2 : d->csb_hw1;
+    hw[0] = 1. / eh_cos(e);
+    hw[1] = eh_tan(e);
+    hw[2] = eh_sin(e);
+    hw[3] = eh_cos(e);
+    hw[4] = 2 * cb->hgap;
+    hw[5] = cb->fint;
+    hw[6] = k ? cb->h2 : cb->h1;
+  }
   /* edge kick limits, src/csbend.c:932-937 (they act in methods 1 with
    * EDGE_ORDER 1, and 3) */
   d->csb_klim1 = cb->edge1_kick_limit;
@@ -556,7 +574,7 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
     d->csb_klim1 *= rho0 / rho_actual;
     d->csb_klim2 *= rho0 / rho_actual;
   }
-  d->full_only = d->full_only || d->csb_esimple1 || d->csb_esimple2 ||
+  d->full_only = d->full_only || d->csb_esimple1 || d->csb_esimple2 || d->csb_hwang1 || d->csb_hwang2 ||
                  (d->csb_edge1 && d->csb_klim1 > 0 && (d->csb_esimple1 || cb->edge_order <= 1)) ||
                  (d->csb_edge2 && d->csb_klim2 > 0 && (d->csb_esimple2 || cb->edge_order <= 1));
   d->csb_e1r = (cb->edgeFlags & EH_BEND_EDGE1_EFFECTS) && e1 != 0;
@@ -990,20 +1008,22 @@ struct eh_beam {
 
 static int g_device = -1;
 
-/* The static scratch of eh_do_tracking_noalloc: per GPU, about 8.1 MiB holding
+/* The static scratch of eh_do_tracking_noalloc: per GPU, about 9 MiB holding
  * the launch counters (first 256 B), the fused line's position table and
  * distinct descriptors for up to EH_SCRATCH_NDESC positions, one descriptor for
- * the per-element sweeps and up to EH_SCRATCH_NTERMS CSBEND field terms.  In host code `scratch` names
+ * the per-element sweeps and the field terms of EH_SCRATCH_NBENDS CSBENDs at
+ * their largest.  In host code `scratch` names
  * a host-side stand-in, not device memory: its device address on each GPU is
  * taken once with hipGetSymbolAddress (eh_init) and kept in g_scratch. */
 #define EH_SCRATCH_NDESC 4096l
 #define EH_SCRATCH_SINGLE_BYTES (((long)sizeof(eh_desc) + 255l) / 256l * 256l)
-#define EH_SCRATCH_BYTES ((8l << 20) + EH_SCRATCH_NDESC * (long)sizeof(eh_pos) + EH_SCRATCH_SINGLE_BYTES)
+#define EH_SCRATCH_NBENDS 421l
 #define EH_SCRATCH_FUSED_OFF 256l
 #define EH_SCRATCH_FUSED_BYTES (EH_SCRATCH_NDESC * ((long)sizeof(eh_pos) + (long)sizeof(eh_desc)))
 #define EH_SCRATCH_DESC_OFF (EH_SCRATCH_FUSED_OFF + EH_SCRATCH_FUSED_BYTES)
 #define EH_SCRATCH_TERMS_OFF (EH_SCRATCH_DESC_OFF + EH_SCRATCH_SINGLE_BYTES)
-#define EH_SCRATCH_NTERMS ((EH_SCRATCH_BYTES - EH_SCRATCH_TERMS_OFF) / (long)sizeof(eh_fterm))
+#define EH_SCRATCH_NTERMS (EH_SCRATCH_NBENDS * EH_MAX_TERMS_PER_BEND)
+#define EH_SCRATCH_BYTES (EH_SCRATCH_TERMS_OFF + EH_SCRATCH_NTERMS * (long)sizeof(eh_fterm))
 /* field terms of one CSBEND at most: both sparse lists and both dense blocks */
 #define EH_MAX_TERMS_PER_BEND (2l * EH_MAXFTERMS + 2l * 36l)
 static __device__ __align__(256) unsigned char scratch[EH_SCRATCH_BYTES];
@@ -4329,8 +4349,8 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
         return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support SYNCH_RAD", e);
       if (d.full_only)
         return fail(EH_ERR_UNSUPPORTED,
-                    "element %ld: the adjoint does not support elliptical MAXAMP, CSBEND edge method 3 or edge kick "
-                    "limits", e);
+                    "element %ld: the adjoint does not support elliptical MAXAMP, CSBEND edge methods 2 and 3 or "
+                    "edge kick limits", e);
       if (!eh_lean_capable(d))
         return fail(EH_ERR_STATE, "element %ld: not trackable by the adjoint's map", e);
       if ((t == EH_T_KQUAD || t == EH_T_KSEXT || t == EH_T_KOCT || t == EH_T_KQUSE) && d.expand)

@@ -164,7 +164,8 @@ typedef struct {
   double b1, b2, b3, b4, b5, b6, b7, b8; /* used when use_bn */
   int32_t nSlices;
   int32_t nonlinear;          /* default 1 */
-  int32_t edge1_effects;      /* default 1; 0, 1, or 3 (the symplectic kick on the canonical momenta) */
+  int32_t edge1_effects;      /* default 1; 0, 1, 2 (K. Hwang's fringe map, HGAP/FINT/H) or 3 (the
+                                 symplectic kick on the canonical momenta) */
   int32_t edge2_effects;
   int32_t edge_order;         /* default 1; >= 2: second-order edge map (apply_edge_effects) */
   int32_t integration_order;  /* 2, 4, 6 */
@@ -396,12 +397,11 @@ long eh_do_tracking(eh_beam *beam, EH_ELEMENT_LIST *beamline, long n_elements, l
 /* No counterpart in the reference: eh_do_tracking in EH_MODE_FUSED for callers
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  The element descriptors,
- * CSBEND field terms and launch counters live in a static 8 MiB scratch array
- * per GPU instead of per-beam allocations; the caller still creates the beam
- * (its particles are the inputs and outputs).  Limits: at most 4096 elements, and
- * as many CSBENDs as the field terms left in the scratch allow at the worst
- * case per bend (421 with the current descriptor size); larger lines return
- * EH_ERR_ARG before anything runs.  eh_init must have run on the device.  Calls
+ * CSBEND field terms and launch counters live in a static scratch array of
+ * about 9 MiB per GPU instead of per-beam allocations; the caller still creates
+ * the beam (its particles are the inputs and outputs).  Limits: at most 4096
+ * elements and 421 CSBENDs (the field terms of each at the worst case); larger
+ * lines return EH_ERR_ARG before anything runs.  eh_init must have run on the device.  Calls
  * that overlap on one GPU share the scratch and race. */
 long eh_do_tracking_noalloc(eh_beam *beam, EH_ELEMENT_LIST *beamline, long n_elements, long n_passes,
                             double *P_central, int turns_per_launch, eh_track_stats *stats);

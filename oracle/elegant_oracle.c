@@ -72,6 +72,15 @@ static double or_dual_log(double x) { return or_dual(eh_log(x), log(x)); }
 
 #include "eh_csbend_terms.h"
 
+/* OR_PLANT (tests only: checks build the oracle with a planted mistake and show
+ * that they reject it): 1 R12 with the momentum ratio inverted, 2 R22 growing
+ * with the energy, 3 exit path-length rescaling inverted, 4 the kick method's
+ * slope ratio inverted, 5 the FINT term of Hwang's first-order fringe map with
+ * cos^2 for cos^3.  0 (the default) is the reference's code. */
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
+
 /* the counting build (or_count.h) tallies the generic transport-matrix loops
  * by their non-trivial terms: these mark those loops, and do nothing here */
 #ifndef OR_COUNT_MATRIX_BEGIN
@@ -1327,6 +1336,100 @@ static void fromDipoleCanonical(double *Qi) {
   Qi[3] *= f;
 }
 
+/* dipoleFringeKHwang, src/csbend.c:4336-4410 (EDGE_EFFECTS 2): K. Hwang's
+ * dipole fringe map on the canonical coordinates Q, at the entrance
+ * (in_fringe -1) or the exit (+1), with the terms of higher order in the
+ * transverse coordinates for EDGE_ORDER > 1.  The SDDS toolkit's ipow1/2/3 (not
+ * vendored) are x, x*x and (x*x)*x here. */
+#define OR_P2(v) ((v) * (v))
+#define OR_P3(v) ((v) * (v) * (v))
+static void or_fringe_hwang(double *Q, double rho, double in_fringe, long edge_order, double edge, double gap,
+                            double fint, double Rhe) {
+  const double k0 = sqr(PI) / 6., k2 = fint, k3 = 1.0 * 1. / 6., k4 = -1.0 * sqr(PI) / 3., k5 = 0.0, k6 = -1.0;
+  const double x0 = Q[0], px0 = Q[1], y0 = Q[2], py0 = Q[3], dp0 = Q[5];
+  const double sec_e = 1. / OR_COS(edge), tan_e = OR_TAN(edge), sin_e = OR_SIN(edge), cos_e = OR_COS(edge);
+  double dx = 0, dpx = 0, dy = 0, dpy = 0;
+  if (edge_order > 1) {
+    if (in_fringe == -1.) {
+      dx = in_fringe * OR_P2(sec_e) * OR_P2(gap) * k0 / rho / (1 + dp0) +
+           in_fringe * OR_P2(x0) * OR_P2(tan_e) / 2 / rho / (1 + dp0) -
+           in_fringe * OR_P2(y0) * OR_P2(sec_e) / 2 / rho / (1 + dp0);
+      dy = -in_fringe * x0 * y0 * OR_P2(tan_e) / rho / (1 + dp0);
+      dpx = -1. * OR_P3(sec_e) * sin_e * OR_P2(gap) * k0 / rho / rho / (1 + dp0) + tan_e * x0 / rho +
+            OR_P2(y0) / 2 * (2 * OR_P3(tan_e)) / OR_P2(rho) / (1 + dp0) +
+            OR_P2(y0) / 2 * (tan_e) / OR_P2(rho) / (1 + dp0) -
+            in_fringe * (x0 * px0 - y0 * py0) * OR_P2(tan_e) / rho / (1 + dp0) +
+            k4 * OR_P2(sin_e) * OR_P2(gap) / 2 / OR_P3(cos_e) / rho * Rhe -
+            k5 * x0 * sin_e * gap / OR_P3(cos_e) / rho * Rhe +
+            k6 * (y0 * y0 - x0 * x0) / 2 / OR_P3(cos_e) / rho * Rhe;
+      dpy = -1. * tan_e * y0 / rho + k2 * y0 * (1 + OR_P2(sin_e)) * gap / (1 + dp0) / OR_P2(rho) / OR_P3(cos_e) +
+            in_fringe * (x0 * py0 + y0 * px0) * OR_P2(tan_e) / rho / (1 + dp0) +
+            in_fringe * y0 * px0 / rho / (1 + dp0) +
+            k3 * OR_P3(y0) * (2. / 3. / cos_e - 4. / 3. / OR_P3(cos_e)) / (1 + dp0) / rho / rho / gap +
+            k6 * x0 * y0 / OR_P3(cos_e) / rho * Rhe;
+    }
+    if (in_fringe == 1.) {
+      dx = in_fringe * OR_P2(sec_e) * OR_P2(gap) * k0 / rho / (1 + dp0) +
+           in_fringe * OR_P2(x0) * OR_P2(tan_e) / 2 / rho / (1 + dp0) -
+           in_fringe * OR_P2(y0) * OR_P2(sec_e) / 2 / rho / (1 + dp0);
+      dy = -in_fringe * x0 * y0 * OR_P2(tan_e) / rho / (1 + dp0);
+      dpx = tan_e * x0 / rho - OR_P2(y0) / 2 * (1 * OR_P3(tan_e)) / OR_P2(rho) / (1 + dp0) -
+            OR_P2(x0) / 2 * (1 * OR_P3(tan_e)) / OR_P2(rho) / (1 + dp0) -
+            in_fringe * (x0 * px0 - y0 * py0) * OR_P2(tan_e) / rho / (1 + dp0) +
+            k4 * OR_P2(sin_e) * OR_P2(gap) / 2 / OR_P3(cos_e) / rho * Rhe -
+            k5 * x0 * sin_e * gap / OR_P3(cos_e) / rho * Rhe +
+            k6 * (y0 * y0 - x0 * x0) / 2 / OR_P3(cos_e) / rho * Rhe;
+      dpy = -1. * tan_e * y0 / rho + k2 * y0 * (1 + OR_P2(sin_e)) * gap / (1 + dp0) / OR_P2(rho) / OR_P3(cos_e) +
+            in_fringe * (x0 * py0 + y0 * px0) * OR_P2(tan_e) / rho / (1 + dp0) +
+            in_fringe * y0 * px0 / rho / (1 + dp0) + x0 * y0 * OR_P2(sec_e) * tan_e / OR_P2(rho) / (1 + dp0) +
+            k3 * OR_P3(y0) * (2. / 3. / cos_e - 4. / 3. / OR_P3(cos_e)) / (1 + dp0) / rho / rho / gap -
+            k5 * y0 * sin_e * gap / OR_P3(cos_e) / rho * Rhe + k6 * x0 * y0 / OR_P3(cos_e) / rho * Rhe;
+    }
+  } else {
+    if (in_fringe == -1.) {
+      dx = in_fringe * OR_P2(sec_e) * OR_P2(gap) * k0 / rho / (1 + dp0);
+      dy = 0;
+      dpx = -1. * OR_P3(sec_e) * sin_e * OR_P2(gap) * k0 / rho / rho / (1 + dp0) + tan_e * x0 / rho +
+            k4 * OR_P2(sin_e) * OR_P2(gap) / 2 / OR_P3(cos_e) / rho * Rhe -
+            k5 * x0 * sin_e * gap / OR_P3(cos_e) / rho * Rhe;
+      dpy = -1. * tan_e * y0 / rho +
+            k2 * y0 * (1 + OR_P2(sin_e)) * gap / (1 + dp0) / OR_P2(rho) / (OR_PLANT == 5 ? OR_P2(cos_e) : OR_P3(cos_e));
+    }
+    if (in_fringe == 1.) {
+      dx = in_fringe * OR_P2(sec_e) * OR_P2(gap) * k0 / rho / (1 + dp0);
+      dy = 0;
+      dpx = tan_e * x0 / rho + k4 * OR_P2(sin_e) * OR_P2(gap) / 2 / OR_P3(cos_e) / rho * Rhe -
+            k5 * x0 * sin_e * gap / OR_P3(cos_e) / rho * Rhe;
+      dpy = -1. * tan_e * y0 / rho +
+            k2 * y0 * (1 + OR_P2(sin_e)) * gap / (1 + dp0) / OR_P2(rho) / (OR_PLANT == 5 ? OR_P2(cos_e) : OR_P3(cos_e)) -
+            k5 * y0 * sin_e * gap / OR_P3(cos_e) / rho * Rhe;
+    }
+  }
+  Q[0] = x0 + dx;
+  Q[1] = px0 + dpx;
+  Q[2] = y0 + dy;
+  Q[3] = py0 + dpy;
+}
+
+/* EDGE_EFFECTS 2 at one edge, src/csbend.c:1139-1159 and :1319-1338 */
+static void or_edge_hwang(double *x, double *xp, double *y, double *yp, double dp, double rho_actual,
+                          double in_fringe, long edge_order, double edge, double hgap, double fint, double h) {
+  double Q[6];
+  Q[0] = *x;
+  Q[1] = *xp;
+  Q[2] = *y;
+  Q[3] = *yp;
+  Q[4] = 0;
+  Q[5] = dp;
+  toDipoleCanonical(Q);
+  or_fringe_hwang(Q, rho_actual, in_fringe, edge_order, edge, 2 * hgap, fint, h);
+  fromDipoleCanonical(Q);
+  *x = Q[0];
+  *xp = Q[1];
+  *y = Q[2];
+  *yp = Q[3];
+}
+
 /* applySimpleDipoleEdgeKick, src/csbend.c:5664-5690 (EDGE_EFFECTS 3, the
  * Hamiltonian not expanded) */
 static void or_simple_edge_kick(double *xp, double *yp, double x, double y, double delta, double rho, double ea,
@@ -1351,7 +1454,7 @@ static void or_simple_edge_kick(double *xp, double *yp, double x, double y, doub
 }
 
 /* track_through_csbend, src/csbend.c:654-1434 (malignMethod 0, ETILT 0,
- * EDGE*_EFFECTS 0, 1 or 3, no reference correction) */
+ * EDGE*_EFFECTS 0 to 3, no reference correction) */
 /* apply_edge_effects, src/csbend.c:4291-4332: K. L. Brown's second-order edge
  * map (EDGE_ORDER >= 2); which_edge -1 at the entrance, 1 at the exit */
 static void or_apply_edge_effects(double *x, double *xp, double *y, double *yp, double rho, double n, double beta,
@@ -1407,9 +1510,11 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
     return or_fail("CSBEND integration_order is invalid");
   if (csbend->malignMethod != 0 || csbend->etilt != 0 || csbend->expandHamiltonian)
     return or_fail("CSBEND option not supported");
-  if ((csbend->edge1_effects != 0 && csbend->edge1_effects != 1 && csbend->edge1_effects != 3) ||
-      (csbend->edge2_effects != 0 && csbend->edge2_effects != 1 && csbend->edge2_effects != 3))
+  if ((csbend->edge1_effects < 0 || csbend->edge1_effects > 3) ||
+      (csbend->edge2_effects < 0 || csbend->edge2_effects > 3))
     return or_fail("CSBEND edge method not supported");
+  if (csbend->edge_order > 1 && (csbend->edge1_effects == 2 || csbend->edge2_effects == 2) && csbend->hgap == 0)
+    return or_fail("CSBEND has EDGE_ORDER>1 and EDGE[12]_EFFECTS==2, but HGAP=0");
 
   rho0 = csbend->length / csbend->angle;
   b[0] = 0;
@@ -1542,7 +1647,10 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
         yp -= OR_TAN(e1 - psi1 / (1 + dp)) / rho * y;
       } else
         or_apply_edge_effects(&x, &xp, &y, &yp, rho, n, e1, csbend->h1, psi1 * (1 + dp), -1);
-    } else if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) && csbend->edge1_effects == 3)
+    } else if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) && csbend->edge1_effects == 2)
+      or_edge_hwang(&x, &xp, &y, &yp, dp, rho_actual, -1., csbend->edge_order, e1, csbend->hgap, csbend->fint,
+                    csbend->h1);
+    else if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) && csbend->edge1_effects == 3)
       or_simple_edge_kick(&xp, &yp, x, y, dp, rho_actual, e1, psi1, e1_kick_limit);
     Qi[0] = x;
     Qi[1] = xp;
@@ -1608,7 +1716,10 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
         yp -= OR_TAN(e2 - psi2 / (1 + dp)) / rho * y;
       } else
         or_apply_edge_effects(&x, &xp, &y, &yp, rho, n, e2, csbend->h2, psi2 * (1 + dp), 1);
-    } else if ((csbend->edgeFlags & EH_BEND_EDGE2_EFFECTS) && csbend->edge2_effects == 3)
+    } else if ((csbend->edgeFlags & EH_BEND_EDGE2_EFFECTS) && csbend->edge2_effects == 2)
+      or_edge_hwang(&x, &xp, &y, &yp, dp, rho_actual, 1., csbend->edge_order, e2, csbend->hgap, csbend->fint,
+                    csbend->h2);
+    else if ((csbend->edgeFlags & EH_BEND_EDGE2_EFFECTS) && csbend->edge2_effects == 3)
       or_simple_edge_kick(&xp, &yp, x, y, dp, rho_actual, e2, psi2, e2_kick_limit);
     coord[0] = x * cos_ttilt - y * sin_ttilt + 0.0;
     coord[2] = x * sin_ttilt + y * cos_ttilt + 0.0;
@@ -1772,14 +1883,6 @@ void or_delete_phase_references(void) {
     or_refs[i].set = 0;
 }
 
-
-/* OR_PLANT (tests only: the RFCA checks build the oracle with a planted mistake
- * and show that they reject it): 1 R12 with the momentum ratio inverted, 2 R22
- * growing with the energy, 3 exit path-length rescaling inverted, 4 the kick
- * method's slope ratio inverted.  0 (the default) is the reference's code. */
-#ifndef OR_PLANT
-#define OR_PLANT 0
-#endif
 
 /* add_to_particle_energy, src/simple_rfca.c:987-1006 */
 static void add_to_particle_energy(double *coord, double timeOfFlight, double Po, double dgamma) {

@@ -605,7 +605,10 @@ int ah_scenario_fill(void) {
   if (fill_csbend(&cb, &db))
     return 9;
   cb.fint[0] = -1;
-  cb.edge_effects[0] = 2; /* another edge method */
+  cb.edge_effects[0] = 2; /* Hwang's fringe map is carried */
+  if (!fill_csbend(&cb, &db) || db.edge2_effects != 2)
+    return 10;
+  cb.edge_effects[0] = 4; /* Hwang-Lindberg is not */
   if (fill_csbend(&cb, &db))
     return 10;
   cb.edge_effects[0] = 1;

@@ -108,6 +108,10 @@ ELEMENT_CASES = [
                     EDGE2_EFFECTS=1, EDGE1_KICK_LIMIT=1e-4, EDGE2_KICK_LIMIT=5e-5, KICK_LIMIT_SCALING=1,
                     FSE=1e-2)),
     ("CSBEND", dict(L=0.8, ANGLE=0.1, N_KICKS=8, E1=0.1, E2=0.1, EDGE1_KICK_LIMIT=2e-4, EDGE2_KICK_LIMIT=2e-4)),
+    ("CSBEND", dict(L=0.8, ANGLE=0.1, K1=0.4, N_KICKS=8, E1=0.05, E2=0.03, HGAP=0.012, FINT=0.5, H1=0.3, H2=-0.2,
+                    EDGE1_EFFECTS=2, EDGE2_EFFECTS=2)),
+    ("CSBEND", dict(L=0.8, ANGLE=-0.07, K1=0.2, N_KICKS=8, E1=0.08, E2=0.04, HGAP=0.015, FINT=0.45, H1=-0.1,
+                    EDGE1_EFFECTS=2, EDGE2_EFFECTS=1, EDGE_ORDER=2, DX=2e-4, TILT=0.01)),
 ]
 
 
@@ -232,6 +236,21 @@ def test_more_elements_ring(mode, ehlib, oracle_mod):
     got, n, ref, nref, stats, eff = _track_both(lat, rows, 5, Po, mode, tpl=2)
     assert 0 < nref < len(rows)
     assert_rows_identical(got, ref, n, nref, "more elements " + mode)
+    assert stats["particle_element_turns"] == eff
+
+
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+@pytest.mark.parametrize("order", [1, 2])
+def test_edge_method2_ring(mode, order, ehlib, oracle_mod):
+    """bends with EDGE_EFFECTS 2 (Hwang's fringe map, first and higher order) in
+    a ring, bit for bit with the oracle, losses included"""
+    _say("edge method 2 ring %s order %d" % (mode, order))
+    from tests.test_csbend_edges import edge_beam, edge_ring
+    rows = edge_beam(3000, seed=43 + order)
+    Po = configs.p_central_from_mev(1500.0)
+    got, n, ref, nref, stats, eff = _track_both(edge_ring(2, order), rows, 4, Po, mode, tpl=2)
+    assert 0 < nref < len(rows)
+    assert_rows_identical(got, ref, n, nref, "edge method 2 ring " + mode)
     assert stats["particle_element_turns"] == eff
 
 
