@@ -308,7 +308,7 @@ static const char *const csbend_untracked[] __attribute__((unused)) = {
  * (src/csbend.c:654-1000).  e[], h[], edge_effects[], fint[] are read through
  * e1Index/e2Index (a reversed element swaps them, :859-876); Lindberg's fringe
  * integrals only act with edge method 5, which the library does not carry
- * (EDGE*_EFFECTS 4 and 5 are refused); the b/c arrays and the
+ * (and refuses); the b/c arrays and the
  * *_ref fields are caches of the strengths above; the reference-trajectory
  * fields act only with REFERENCE_CORRECTION and the FSE correction value only
  * with FSE_CORRECTION, both refused */
@@ -321,7 +321,7 @@ static int fill_csbend(const CSBEND *cb, EH_CSBEND *d) {
       !no_file(cb->photonOutputFile) || cb->referenceCorrection || cb->trackingMatrix || cb->fseCorrection ||
       cb->edgeFlip)
     return 0;
-  if (cb->edge_effects[i1] < 0 || cb->edge_effects[i2] < 0 || cb->edge_effects[i1] > 3 || cb->edge_effects[i2] > 3)
+  if (cb->edge_effects[i1] < 0 || cb->edge_effects[i2] < 0 || cb->edge_effects[i1] > 4 || cb->edge_effects[i2] > 4)
     return 0;
   if (cb->xReference != 0 || cb->f1 || cb->f2 || cb->f3 || cb->f4 || cb->f5 || cb->f6 || cb->f7 || cb->f8 || cb->g1 ||
       cb->g2 || cb->g3 || cb->g4 || cb->g5 || cb->g6 || cb->g7 || cb->g8)

@@ -111,6 +111,7 @@ typedef struct {
    * the edge angle, the full gap 2*HGAP, FINT and the pole-face curvature H;
    * csb_hw_order2 for EDGE_ORDER > 1 */
   int32_t csb_hwang1, csb_hwang2, csb_hw_order2;
+  int32_t csb_hl1, csb_hl2; /* EDGE*_EFFECTS 4: dipoleFringeKHwangRLindberg, same constants */
   double csb_hw1[7], csb_hw2[7];
   double csb_klim1, csb_klim2;   /* edge x' kick limits (<= 0: none) */
   double rho0, rho_actual;
@@ -1250,13 +1251,60 @@ EH_HD void eh_fringe_hwang(double &x, double &px, double &y, double &py, double 
   py = py0 + dpy;
 }
 
-/* EDGE_EFFECTS 2 at one edge (src/csbend.c:1139-1159, :1319-1338): the slopes
- * to canonical momenta, Hwang's map, and back */
+/* dipoleFringeKHwangRLindberg (src/csbend.c:4413-4520, EDGE_EFFECTS 4): Hwang's
+ * map made symplectic by R. Lindberg, a sequence of exact maps on the canonical
+ * coordinates (one exponential in x), with the constants of eh_fringe_hwang */
+EH_HD void eh_fringe_hl(double &x, double &px, double &y, double &py, double dp0, double rho,
+                        const double *__restrict__ c, double in_fringe) {
+  const double k0 = (EH_PI_D * EH_PI_D) / 6., k2 = c[5], k3 = 1.0 * 1. / 6., k4 = -1.0 * (EH_PI_D * EH_PI_D) / 3.,
+               k5 = 0.0, k6 = -1.0;
+  const double sec_e = c[0], tan_e = c[1], sin_e = c[2], cos_e = c[3], gap = c[4], Rhe = c[6];
+  const double rho2 = rho * rho, sec2 = sec_e * sec_e, cos3 = cos_e * cos_e * cos_e, tan2 = tan_e * tan_e;
+  const double tan3 = tan_e * tan_e * tan_e;
+  const double x0 = x, px0 = px, y0 = y, py0 = py;
+  double px1, py1, t1, t2;
+  const bool in = in_fringe == -1;
+  if (in)
+    px1 = px0 + tan_e / rho * x0 + tan_e / (2 * rho2 * (1 + dp0)) * (y0 * y0) - tan3 / (rho2 * (1 + dp0)) * (x0 * x0) -
+          gap * k5 * sin_e * Rhe / (rho * cos3) * x0 + k6 * sec2 * Rhe / (2 * rho) * ((y0 * y0) - (x0 * x0));
+  else
+    px1 = px0 + tan_e / rho * x0 + tan3 / (2 * rho2 * (1 + dp0)) * (y0 * y0) +
+          tan3 / (2 * rho2 * (1 + dp0)) * (x0 * x0) - gap * k5 * sin_e * Rhe / (rho * cos3) * x0 +
+          k6 * sec2 * Rhe / (2 * rho) * ((y0 * y0) - (x0 * x0));
+  py1 = py0 - tan_e / rho * y0 + (in ? tan_e : tan3) / (rho2 * (1 + dp0)) * x0 * y0 +
+        gap * k2 * (1 + (sin_e * sin_e)) / (rho2 * (1 + dp0) * cos3) * y0 +
+        2 * k3 * ((cos_e * cos_e) - 2) / (3 * gap * rho2 * cos3) * (y0 * y0 * y0) +
+        gap * k5 * sin_e * Rhe / (rho * cos3) * y0 + k6 * (sec_e * sec_e * sec_e) * Rhe / rho * x0 * y0;
+  t1 = in ? (1 + tan2 / (2 * rho * (1 + dp0)) * x0) : (1 - tan2 / (2 * rho * (1 + dp0)) * x0);
+  const double x2 = x0 / t1, px2 = px1 * (t1 * t1);
+  t1 = sec2 / (rho * (1 + dp0));
+  const double x3 = in ? x2 + t1 / 2 * (y0 * y0) : x2 - t1 / 2 * (y0 * y0);
+  const double py3 = in ? py1 - t1 * y0 * px2 : py1 + t1 * y0 * px2;
+  t1 = in ? tan2 / (rho * (1 + dp0)) : -tan2 / (rho * (1 + dp0));
+  t2 = eh_exp(t1 * x3);
+  const double px4 = px2 - t1 * y0 * py3, y4 = y0 * t2, py4 = py3 / t2;
+  if (in) {
+    t1 = (gap * gap) * k0 * sec2 / (rho * (1 + dp0));
+    x = x3 - t1;
+    px = px4 - t1 * tan_e / rho + (gap * gap) * k4 * (sin_e * sin_e) * Rhe / (2 * rho * cos3);
+  } else {
+    x = x3 + (gap * gap) * k0 * sec2 / (rho * (1 + dp0));
+    px = px4 + (gap * gap) * k4 * (sin_e * sin_e) * Rhe / (2 * rho * cos3);
+  }
+  y = y4;
+  py = py4;
+}
+
+/* EDGE_EFFECTS 2 or 4 at one edge (src/csbend.c:1139-1181, :1319-1360): the
+ * slopes to canonical momenta, Hwang's map (or Lindberg's form), and back */
 EH_HD void eh_edge_hwang(double &x, double &xp, double &y, double &yp, double dp, double rho_actual,
-                         const double *__restrict__ c, double in_fringe, bool higher) {
+                         const double *__restrict__ c, double in_fringe, bool higher, bool lindberg) {
   double f = (1 + dp) / eh_sqrt(1 + eh_sqr(xp) + eh_sqr(yp));
   double qx = xp * f, qy = yp * f;
-  eh_fringe_hwang(x, qx, y, qy, dp, rho_actual, c, in_fringe, higher);
+  if (lindberg)
+    eh_fringe_hl(x, qx, y, qy, dp, rho_actual, c, in_fringe);
+  else
+    eh_fringe_hwang(x, qx, y, qy, dp, rho_actual, c, in_fringe, higher);
   f = 1 / eh_sqrt(eh_sqr(1 + dp) - eh_sqr(qx) - eh_sqr(qy));
   xp = qx * f;
   yp = qy * f;
@@ -1284,8 +1332,8 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   const double dp0 = dp;
   if (d.csb_edge1) {
     double rho = (1 + dp) * rho_actual;
-    if (FULL && d.csb_hwang1) {
-      eh_edge_hwang(x, xp, y, yp, dp, rho_actual, d.csb_hw1, -1., d.csb_hw_order2);
+    if (FULL && (d.csb_hwang1 || d.csb_hl1)) {
+      eh_edge_hwang(x, xp, y, yp, dp, rho_actual, d.csb_hw1, -1., d.csb_hw_order2, d.csb_hl1);
     } else if (FULL && d.csb_esimple1) {
       eh_edge_simple(xp, yp, x, y, dp, rho_actual, d.tan_e1, d.e1, d.psi1, d.csb_klim1);
     } else if (!d.csb_eorder2) {
@@ -1440,8 +1488,8 @@ EH_HD bool eh_csbend_t(eh_p6 &p, const eh_desc &d, int pi, const eh_fterm *__res
   dp = DP;
   if (d.csb_edge2) {
     double rho = (1 + dp) * rho_actual;
-    if (FULL && d.csb_hwang2) {
-      eh_edge_hwang(x, xp, y, yp, dp, rho_actual, d.csb_hw2, 1., d.csb_hw_order2);
+    if (FULL && (d.csb_hwang2 || d.csb_hl2)) {
+      eh_edge_hwang(x, xp, y, yp, dp, rho_actual, d.csb_hw2, 1., d.csb_hw_order2, d.csb_hl2);
     } else if (FULL && d.csb_esimple2) {
       eh_edge_simple(xp, yp, x, y, dp, rho_actual, d.tan_e2, d.e2, d.psi2, d.csb_klim2);
     } else if (!d.csb_eorder2) {

@@ -389,7 +389,7 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
     return fail(EH_ERR_UNSUPPORTED, "CSBEND integration_order is invalid");
   if (cb->malignMethod != 0 || cb->etilt != 0 || cb->expandHamiltonian)
     return fail(EH_ERR_UNSUPPORTED, "CSBEND option not supported");
-  if (cb->edge1_effects < 0 || cb->edge1_effects > 3 || cb->edge2_effects < 0 || cb->edge2_effects > 3)
+  if (cb->edge1_effects < 0 || cb->edge1_effects > 4 || cb->edge2_effects < 0 || cb->edge2_effects > 4)
     return fail(EH_ERR_UNSUPPORTED, "CSBEND edge method not supported");
   if (cb->edge_order > 1 && (cb->edge1_effects == 2 || cb->edge2_effects == 2) && cb->hgap == 0)
     return fail(EH_ERR_ARG, "CSBEND has EDGE_ORDER>1 and EDGE[12]_EFFECTS==2, but HGAP=0. This gives undefined "
@@ -555,6 +555,9 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
   d->csb_hwang1 = d->csb_edge1 && cb->edge1_effects == 2;
   d->csb_hwang2 = d->csb_edge2 && cb->edge2_effects == 2;
   d->csb_hw_order2 = cb->edge_order > 1;
+  /* EDGE_EFFECTS 4, dipoleFringeKHwangRLindberg (:1160-1181, :1340-1360) */
+  d->csb_hl1 = d->csb_edge1 && cb->edge1_effects == 4;
+  d->csb_hl2 = d->csb_edge2 && cb->edge2_effects == 4;
   for (int k = 0; k < 2; k++) {
     const double e = k ? e2 : e1;
     double *hw = k ? d->csb_hw2 : d->csb_hw1;
@@ -575,6 +578,7 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
     d->csb_klim2 *= rho0 / rho_actual;
   }
   d->full_only = d->full_only || d->csb_esimple1 || d->csb_esimple2 || d->csb_hwang1 || d->csb_hwang2 ||
+                 d->csb_hl1 || d->csb_hl2 ||
                  (d->csb_edge1 && d->csb_klim1 > 0 && (d->csb_esimple1 || cb->edge_order <= 1)) ||
                  (d->csb_edge2 && d->csb_klim2 > 0 && (d->csb_esimple2 || cb->edge_order <= 1));
   d->csb_e1r = (cb->edgeFlags & EH_BEND_EDGE1_EFFECTS) && e1 != 0;
@@ -4349,7 +4353,7 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
         return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support SYNCH_RAD", e);
       if (d.full_only)
         return fail(EH_ERR_UNSUPPORTED,
-                    "element %ld: the adjoint does not support elliptical MAXAMP, CSBEND edge methods 2 and 3 or "
+                    "element %ld: the adjoint does not support elliptical MAXAMP, CSBEND edge methods 2 to 4 or "
                     "edge kick limits", e);
       if (!eh_lean_capable(d))
         return fail(EH_ERR_STATE, "element %ld: not trackable by the adjoint's map", e);

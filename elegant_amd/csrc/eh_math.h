@@ -414,6 +414,44 @@ EH_FN double eh_asin(double x) {
   return x < 0 ? -r : r;
 }
 
+/* exp: x = k ln2 + r with |r| <= ln2/2 (Cody-Waite, ln2 split so that k ln2_hi
+ * is exact), exp(r) = (1 + r) + r^2 (1/2! + r/3! + ... + r^12/14!) with the
+ * rounding error of 1 + r carried (Fast2Sum), so that only the last addition
+ * rounds at the scale of the result; then the exact scaling by 2^k. */
+EH_FN double eh_exp(double x) {
+  if (x != x)
+    return x;
+  if (x > 0x1.62e42fefa39efp+9) /* ln(DBL_MAX) */
+    return eh_from_bits(0x7ff0000000000000ULL);
+  if (x < -0x1.74910d52d3052p+9) /* below ln of half the least subnormal */
+    return 0.0;
+  const double kd = __builtin_rint(x * EH_INV_LN2);
+  const int k = (int)kd;
+  const double r = (x - kd * EH_LN2_HI) - kd * EH_LN2_LO;
+  double p = EH_EXP_14;
+  p = EH_EXP_13 + r * p;
+  p = EH_EXP_12 + r * p;
+  p = EH_EXP_11 + r * p;
+  p = EH_EXP_10 + r * p;
+  p = EH_EXP_9 + r * p;
+  p = EH_EXP_8 + r * p;
+  p = EH_EXP_7 + r * p;
+  p = EH_EXP_6 + r * p;
+  p = EH_EXP_5 + r * p;
+  p = EH_EXP_4 + r * p;
+  p = EH_EXP_3 + r * p;
+  p = EH_EXP_2 + r * p;
+  const double s = (r * r) * p;
+  const double hi = 1.0 + r;
+  const double lo = (1.0 - hi) + r;
+  const double e = hi + (lo + s);
+  if (k >= -1021 && k <= 1023)
+    return e * eh_from_bits((uint64_t)(k + 1023) << 52);
+  if (k > 1023)
+    return (e * 2.0) * eh_from_bits((uint64_t)(k - 1 + 1023) << 52);
+  return (e * eh_from_bits((uint64_t)(k + 1000 + 1023) << 52)) * 0x1p-1000;
+}
+
 EH_FN double eh_log(double x) {
   if (x != x || x < 0)
     return eh_nan();

@@ -164,8 +164,9 @@ typedef struct {
   double b1, b2, b3, b4, b5, b6, b7, b8; /* used when use_bn */
   int32_t nSlices;
   int32_t nonlinear;          /* default 1 */
-  int32_t edge1_effects;      /* default 1; 0, 1, 2 (K. Hwang's fringe map, HGAP/FINT/H) or 3 (the
-                                 symplectic kick on the canonical momenta) */
+  int32_t edge1_effects;      /* default 1; 0, 1, 2 (K. Hwang's fringe map, HGAP/FINT/H), 3 (the
+                                 symplectic kick on the canonical momenta) or 4 (Hwang's map in
+                                 R. Lindberg's symplectic form) */
   int32_t edge2_effects;
   int32_t edge_order;         /* default 1; >= 2: second-order edge map (apply_edge_effects) */
   int32_t integration_order;  /* 2, 4, 6 */

@@ -50,17 +50,20 @@ static double or_dual_cos(double x) { return or_dual(eh_cos(x), cos(x)); }
 static double or_dual_tan(double x) { return or_dual(eh_tan(x), tan(x)); }
 static double or_dual_asin(double x) { return or_dual(eh_asin(x), asin(x)); }
 static double or_dual_log(double x) { return or_dual(eh_log(x), log(x)); }
+static double or_dual_exp(double x) { return or_dual(eh_exp(x), exp(x)); }
 #define OR_SIN or_dual_sin
 #define OR_COS or_dual_cos
 #define OR_TAN or_dual_tan
 #define OR_ASIN or_dual_asin
 #define OR_LOG or_dual_log
+#define OR_EXP or_dual_exp
 #elif defined(OR_LIBM)
 #define OR_SIN sin
 #define OR_COS cos
 #define OR_TAN tan
 #define OR_ASIN asin
 #define OR_LOG log
+#define OR_EXP exp
 #else
 #include "eh_math.h"
 #define OR_SIN eh_sin
@@ -68,6 +71,7 @@ static double or_dual_log(double x) { return or_dual(eh_log(x), log(x)); }
 #define OR_TAN eh_tan
 #define OR_ASIN eh_asin
 #define OR_LOG eh_log
+#define OR_EXP eh_exp
 #endif
 
 #include "eh_csbend_terms.h"
@@ -1411,9 +1415,87 @@ static void or_fringe_hwang(double *Q, double rho, double in_fringe, long edge_o
   Q[3] = py0 + dpy;
 }
 
-/* EDGE_EFFECTS 2 at one edge, src/csbend.c:1139-1159 and :1319-1338 */
+/* dipoleFringeKHwangRLindberg, src/csbend.c:4413-4520 (EDGE_EFFECTS 4): Hwang's
+ * fringe map made symplectic by R. Lindberg, a sequence of exact maps on the
+ * canonical coordinates Q (one of them exponential in x), at the entrance
+ * (in_fringe -1) or the exit (+1) */
+static void or_fringe_hl(double *Q, double rho, double in_fringe, double edge, double gap, double fint, double Rhe) {
+  const double k0 = sqr(PI) / 6., k2 = fint, k3 = 1.0 * 1. / 6., k4 = -1.0 * sqr(PI) / 3., k5 = 0.0, k6 = -1.0;
+  const double rho2 = sqr(rho);
+  const double x0 = Q[0], px0 = Q[1], y0 = Q[2], py0 = Q[3], dp0 = Q[5];
+  const double sec_e = 1. / OR_COS(edge), tan_e = OR_TAN(edge), sin_e = OR_SIN(edge), cos_e = OR_COS(edge);
+  const double sec2 = OR_P2(sec_e), cos3 = OR_P3(cos_e), tan2 = OR_P2(tan_e), tan3 = OR_P3(tan_e);
+  double x1, px1, y1, py1, x2, px2, y2, py2, x3, px3, y3, py3, x4, px4, y4, py4, x5, px5, y5, py5, t1, t2;
+  if (in_fringe == -1) {
+    x1 = x0;
+    px1 = px0 + tan_e / rho * x0 + tan_e / (2 * rho2 * (1 + dp0)) * sqr(y0) - tan3 / (rho2 * (1 + dp0)) * sqr(x0) -
+          gap * k5 * sin_e * Rhe / (rho * cos3) * x0 + k6 * sec2 * Rhe / (2 * rho) * (sqr(y0) - sqr(x0));
+    y1 = y0;
+    py1 = py0 - tan_e / rho * y0 + tan_e / (rho2 * (1 + dp0)) * x0 * y0 +
+          gap * k2 * (1 + sqr(sin_e)) / (rho2 * (1 + dp0) * cos3) * y0 +
+          2 * k3 * (sqr(cos_e) - 2) / (3 * gap * rho2 * cos3) * OR_P3(y0) + gap * k5 * sin_e * Rhe / (rho * cos3) * y0 +
+          k6 * OR_P3(sec_e) * Rhe / rho * x0 * y0;
+    t1 = (1 + tan2 / (2 * rho * (1 + dp0)) * x1);
+    x2 = x1 / t1;
+    px2 = px1 * sqr(t1);
+    y2 = y1;
+    py2 = py1;
+    t1 = sec2 / (rho * (1 + dp0));
+    x3 = x2 + t1 / 2 * sqr(y2);
+    px3 = px2;
+    y3 = y2;
+    py3 = py2 - t1 * y2 * px2;
+    t1 = tan2 / (rho * (1 + dp0));
+    t2 = OR_EXP(t1 * x3);
+    x4 = x3;
+    px4 = px3 - t1 * y3 * py3;
+    y4 = y3 * t2;
+    py4 = py3 / t2;
+    t1 = sqr(gap) * k0 * sec2 / (rho * (1 + dp0));
+    x5 = x4 - t1;
+    px5 = px4 - t1 * tan_e / rho + sqr(gap) * k4 * sqr(sin_e) * Rhe / (2 * rho * cos3);
+    y5 = y4;
+    py5 = py4;
+  } else {
+    x1 = x0;
+    px1 = px0 + tan_e / rho * x0 + tan3 / (2 * rho2 * (1 + dp0)) * sqr(y0) + tan3 / (2 * rho2 * (1 + dp0)) * sqr(x0) -
+          gap * k5 * sin_e * Rhe / (rho * cos3) * x0 + k6 * sec2 * Rhe / (2 * rho) * (sqr(y0) - sqr(x0));
+    y1 = y0;
+    py1 = py0 - tan_e / rho * y0 + tan3 / (rho2 * (1 + dp0)) * x0 * y0 +
+          gap * k2 * (1 + sqr(sin_e)) / (rho2 * (1 + dp0) * cos3) * y0 +
+          2 * k3 * (sqr(cos_e) - 2) / (3 * gap * rho2 * cos3) * OR_P3(y0) + gap * k5 * sin_e * Rhe / (rho * cos3) * y0 +
+          k6 * OR_P3(sec_e) * Rhe / rho * x0 * y0;
+    t1 = (1 - tan2 / (2 * rho * (1 + dp0)) * x1);
+    x2 = x1 / t1;
+    px2 = px1 * sqr(t1);
+    y2 = y1;
+    py2 = py1;
+    t1 = sec2 / (rho * (1 + dp0));
+    x3 = x2 - t1 / 2 * sqr(y2);
+    px3 = px2;
+    y3 = y2;
+    py3 = py2 + t1 * y2 * px2;
+    t1 = -tan2 / (rho * (1 + dp0));
+    t2 = OR_EXP(t1 * x3);
+    x4 = x3;
+    px4 = px3 - t1 * y3 * py3;
+    y4 = y3 * t2;
+    py4 = py3 / t2;
+    x5 = x4 + sqr(gap) * k0 * sec2 / (rho * (1 + dp0));
+    px5 = px4 + sqr(gap) * k4 * sqr(sin_e) * Rhe / (2 * rho * cos3);
+    y5 = y4;
+    py5 = py4;
+  }
+  Q[0] = x5;
+  Q[1] = px5;
+  Q[2] = y5;
+  Q[3] = py5;
+}
+
+/* EDGE_EFFECTS 2 or 4 at one edge, src/csbend.c:1139-1181 and :1319-1360 */
 static void or_edge_hwang(double *x, double *xp, double *y, double *yp, double dp, double rho_actual,
-                          double in_fringe, long edge_order, double edge, double hgap, double fint, double h) {
+                          double in_fringe, long edge_order, double edge, double hgap, double fint, double h,
+                          int lindberg) {
   double Q[6];
   Q[0] = *x;
   Q[1] = *xp;
@@ -1422,7 +1504,10 @@ static void or_edge_hwang(double *x, double *xp, double *y, double *yp, double d
   Q[4] = 0;
   Q[5] = dp;
   toDipoleCanonical(Q);
-  or_fringe_hwang(Q, rho_actual, in_fringe, edge_order, edge, 2 * hgap, fint, h);
+  if (lindberg)
+    or_fringe_hl(Q, rho_actual, in_fringe, edge, 2 * hgap, fint, h);
+  else
+    or_fringe_hwang(Q, rho_actual, in_fringe, edge_order, edge, 2 * hgap, fint, h);
   fromDipoleCanonical(Q);
   *x = Q[0];
   *xp = Q[1];
@@ -1454,7 +1539,7 @@ static void or_simple_edge_kick(double *xp, double *yp, double x, double y, doub
 }
 
 /* track_through_csbend, src/csbend.c:654-1434 (malignMethod 0, ETILT 0,
- * EDGE*_EFFECTS 0 to 3, no reference correction) */
+ * EDGE*_EFFECTS 0 to 4, no reference correction) */
 /* apply_edge_effects, src/csbend.c:4291-4332: K. L. Brown's second-order edge
  * map (EDGE_ORDER >= 2); which_edge -1 at the entrance, 1 at the exit */
 static void or_apply_edge_effects(double *x, double *xp, double *y, double *yp, double rho, double n, double beta,
@@ -1510,8 +1595,8 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
     return or_fail("CSBEND integration_order is invalid");
   if (csbend->malignMethod != 0 || csbend->etilt != 0 || csbend->expandHamiltonian)
     return or_fail("CSBEND option not supported");
-  if ((csbend->edge1_effects < 0 || csbend->edge1_effects > 3) ||
-      (csbend->edge2_effects < 0 || csbend->edge2_effects > 3))
+  if ((csbend->edge1_effects < 0 || csbend->edge1_effects > 4) ||
+      (csbend->edge2_effects < 0 || csbend->edge2_effects > 4))
     return or_fail("CSBEND edge method not supported");
   if (csbend->edge_order > 1 && (csbend->edge1_effects == 2 || csbend->edge2_effects == 2) && csbend->hgap == 0)
     return or_fail("CSBEND has EDGE_ORDER>1 and EDGE[12]_EFFECTS==2, but HGAP=0");
@@ -1647,9 +1732,10 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
         yp -= OR_TAN(e1 - psi1 / (1 + dp)) / rho * y;
       } else
         or_apply_edge_effects(&x, &xp, &y, &yp, rho, n, e1, csbend->h1, psi1 * (1 + dp), -1);
-    } else if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) && csbend->edge1_effects == 2)
+    } else if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) &&
+               (csbend->edge1_effects == 2 || csbend->edge1_effects == 4))
       or_edge_hwang(&x, &xp, &y, &yp, dp, rho_actual, -1., csbend->edge_order, e1, csbend->hgap, csbend->fint,
-                    csbend->h1);
+                    csbend->h1, csbend->edge1_effects == 4);
     else if ((csbend->edgeFlags & EH_BEND_EDGE1_EFFECTS) && csbend->edge1_effects == 3)
       or_simple_edge_kick(&xp, &yp, x, y, dp, rho_actual, e1, psi1, e1_kick_limit);
     Qi[0] = x;
@@ -1716,9 +1802,10 @@ long or_track_through_csbend(double **part, long n_part, const EH_CSBEND *csbend
         yp -= OR_TAN(e2 - psi2 / (1 + dp)) / rho * y;
       } else
         or_apply_edge_effects(&x, &xp, &y, &yp, rho, n, e2, csbend->h2, psi2 * (1 + dp), 1);
-    } else if ((csbend->edgeFlags & EH_BEND_EDGE2_EFFECTS) && csbend->edge2_effects == 2)
+    } else if ((csbend->edgeFlags & EH_BEND_EDGE2_EFFECTS) &&
+               (csbend->edge2_effects == 2 || csbend->edge2_effects == 4))
       or_edge_hwang(&x, &xp, &y, &yp, dp, rho_actual, 1., csbend->edge_order, e2, csbend->hgap, csbend->fint,
-                    csbend->h2);
+                    csbend->h2, csbend->edge2_effects == 4);
     else if ((csbend->edgeFlags & EH_BEND_EDGE2_EFFECTS) && csbend->edge2_effects == 3)
       or_simple_edge_kick(&xp, &yp, x, y, dp, rho_actual, e2, psi2, e2_kick_limit);
     coord[0] = x * cos_ttilt - y * sin_ttilt + 0.0;
@@ -2323,4 +2410,5 @@ long or_elimit_amplitudes_flat(double *flat, double xmax, double ymax, long np, 
 double or_sin(double x) { return OR_SIN(x); }
 double or_cos(double x) { return OR_COS(x); }
 double or_asin(double x) { return OR_ASIN(x); }
+double or_exp(double x) { return OR_EXP(x); }
 double or_log(double x) { return OR_LOG(x); }

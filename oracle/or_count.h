@@ -131,6 +131,7 @@ OR_COUNT_T(cos, 1)
 OR_COUNT_T(tan, 2)
 OR_COUNT_T(asin, 3)
 OR_COUNT_T(log, 4)
+OR_COUNT_T(exp, 5)
 #undef OR_COUNT_T
 
 #define double or_cd

@@ -71,7 +71,7 @@ def load_path(path):
         lib.or_targ_start.argtypes = [C.c_long]
         lib.or_targ_read.argtypes = [vp, vp, C.c_long]
         lib.or_targ_read.restype = C.c_long
-    for f in ("or_sin", "or_cos", "or_asin", "or_log"):
+    for f in ("or_sin", "or_cos", "or_asin", "or_log", "or_exp"):
         getattr(lib, f).argtypes = [C.c_double]
         getattr(lib, f).restype = C.c_double
     return lib
@@ -102,7 +102,7 @@ def count_ops(coords, lattice, n_passes, p_central):
     return counts, rows, n, eff.value
 
 
-T_KINDS = ("sin", "cos", "tan", "asin", "log")
+T_KINDS = ("sin", "cos", "tan", "asin", "log", "exp")
 
 
 def transcendental_args(coords, lattice, n_passes, p_central, cap=10_000_000):

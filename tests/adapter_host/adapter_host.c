@@ -608,7 +608,7 @@ int ah_scenario_fill(void) {
   cb.edge_effects[0] = 2; /* Hwang's fringe map is carried */
   if (!fill_csbend(&cb, &db) || db.edge2_effects != 2)
     return 10;
-  cb.edge_effects[0] = 4; /* Hwang-Lindberg is not */
+  cb.edge_effects[0] = 5; /* Lindberg's curved-dipole fringe is not */
   if (fill_csbend(&cb, &db))
     return 10;
   cb.edge_effects[0] = 1;

@@ -205,7 +205,29 @@ def edge_beam(n, seed, sigma=6e-3):
     return b
 
 
-@pytest.mark.parametrize("method,order", [(2, 1), (2, 2), (3, 1)])
+def test_method4_is_symplectic_and_agrees_with_method2_to_first_order():
+    """EDGE_EFFECTS 4 (Hwang's map in Lindberg's symplectic form,
+    src/csbend.c:4413-4520) is a symplectic map of (x, q_x, y, q_y) at fixed
+    delta, which method 2 at EDGE_ORDER 2 is not (edges of a bend so short that
+    its body is the identity; without pole-face curvature, whose terms carry
+    sec^2 e in the q_x kick and sec^3 e in the q_y kick in the reference and so
+    derive from no common potential); the two share the linear terms (their
+    difference vanishes faster than the amplitude)"""
+    kw = dict(L=1e-9, ANGLE=1e-10, K1=0.0, N_KICKS=1, E1=0.3, E2=0.25, HGAP=0.03, FINT=0.5)
+    z0 = np.array([3e-2, 0.02, -2e-2, 0.01, 0.0, 2e-3])
+    e4 = symplectic_error(jacobian(bend(EDGE1_EFFECTS=4, EDGE2_EFFECTS=4, **kw), z0))
+    e2 = symplectic_error(jacobian(bend(EDGE1_EFFECTS=2, EDGE2_EFFECTS=2, EDGE_ORDER=2, **kw), z0))
+    assert e4 < 1e-7 and e2 > 1e-6, (e4, e2)
+    d = []
+    for a in (1e-3, 2e-3, 4e-3):
+        z = [[a, 0.05 * a, -0.7 * a, 0.03 * a, 0.0, 1e-3]]
+        r4 = run(bend(EDGE1_EFFECTS=4, EDGE2_EFFECTS=4), z)[0]
+        r2 = run(bend(EDGE1_EFFECTS=2, EDGE2_EFFECTS=2), z)[0]
+        d.append(np.max(np.abs(r4[:4] - r2[:4])))
+    assert d[1] / d[0] > 3.0 and d[2] / d[1] > 3.0, d
+
+
+@pytest.mark.parametrize("method,order", [(2, 1), (2, 2), (3, 1), (4, 1)])
 def test_host_build_of_the_maps_matches_the_oracle(method, order):
     from tests.test_batch_host import _compare_with_oracle, _host_track
     beams = [edge_beam(n, 11 + method + order + v) for v, n in enumerate((200, 130))]

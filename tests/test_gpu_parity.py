@@ -112,6 +112,8 @@ ELEMENT_CASES = [
                     EDGE1_EFFECTS=2, EDGE2_EFFECTS=2)),
     ("CSBEND", dict(L=0.8, ANGLE=-0.07, K1=0.2, N_KICKS=8, E1=0.08, E2=0.04, HGAP=0.015, FINT=0.45, H1=-0.1,
                     EDGE1_EFFECTS=2, EDGE2_EFFECTS=1, EDGE_ORDER=2, DX=2e-4, TILT=0.01)),
+    ("CSBEND", dict(L=0.8, ANGLE=0.09, K1=0.3, N_KICKS=8, E1=0.07, E2=0.05, HGAP=0.012, FINT=0.5, H1=0.2, H2=0.1,
+                    EDGE1_EFFECTS=4, EDGE2_EFFECTS=4)),
 ]
 
 
@@ -240,15 +242,16 @@ def test_more_elements_ring(mode, ehlib, oracle_mod):
 
 
 @pytest.mark.parametrize("mode", ["plain", "fused"])
-@pytest.mark.parametrize("order", [1, 2])
-def test_edge_method2_ring(mode, order, ehlib, oracle_mod):
-    """bends with EDGE_EFFECTS 2 (Hwang's fringe map, first and higher order) in
-    a ring, bit for bit with the oracle, losses included"""
-    _say("edge method 2 ring %s order %d" % (mode, order))
+@pytest.mark.parametrize("method,order", [(2, 1), (2, 2), (4, 1)])
+def test_edge_method2_ring(mode, method, order, ehlib, oracle_mod):
+    """bends with EDGE_EFFECTS 2 (Hwang's fringe map, first and higher order) or
+    4 (its symplectic form) in a ring, bit for bit with the oracle, losses
+    included"""
+    _say("edge method %d ring %s order %d" % (method, mode, order))
     from tests.test_csbend_edges import edge_beam, edge_ring
-    rows = edge_beam(3000, seed=43 + order)
+    rows = edge_beam(3000, seed=43 + order + method)
     Po = configs.p_central_from_mev(1500.0)
-    got, n, ref, nref, stats, eff = _track_both(edge_ring(2, order), rows, 4, Po, mode, tpl=2)
+    got, n, ref, nref, stats, eff = _track_both(edge_ring(method, order), rows, 4, Po, mode, tpl=2)
     assert 0 < nref < len(rows)
     assert_rows_identical(got, ref, n, nref, "edge method 2 ring " + mode)
     assert stats["particle_element_turns"] == eff

@@ -86,3 +86,36 @@ def test_generated_constants_match_python():
         assert (m * 2 ** 24) == int(m * 2 ** 24)
     assert vals["EH_SIN_1"] == -1.0 / 6 and vals["EH_COS_1"] == -0.5
     assert importlib.util.find_spec("numpy") is not None
+
+
+def test_exp_accuracy_and_special_values(lib):
+    """eh_exp against the correctly rounded value (60-digit decimal exp rounded
+    once) on the fringe map's small arguments and across the range; exact
+    special values; exp(log x) = x to a few ulp"""
+    from decimal import Decimal, localcontext
+
+    def exact(v):
+        with localcontext() as ctx:
+            ctx.prec = 60
+            return float(Decimal(v).exp())
+
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for lo, hi in ((-1e-4, 1e-4), (-0.35, 0.35), (-5, 5), (-700, 700)):
+        x = rng.uniform(lo, hi, 4000)
+        got = _vec(lib.or_exp, x)
+        ref = np.array([exact(float(v)) for v in x])
+        worst = max(worst, ulps(got, ref).max())
+        assert ulps(got, np.exp(x)).max() <= 1, (lo, hi)
+    assert worst <= 1, worst
+    # a planted Taylor series cut one term short misses the bar on |r| near ln2/2
+    r = np.linspace(0.3, 0.346, 200)
+    short = 1 + r + sum(r ** n / math.factorial(n) for n in range(2, 10))
+    assert ulps(short, [exact(float(v)) for v in r]).max() > 1
+    assert lib.or_exp(0.0) == 1.0 and lib.or_exp(-0.0) == 1.0
+    assert lib.or_exp(1000.0) == math.inf and lib.or_exp(-1000.0) == 0.0
+    assert math.isnan(lib.or_exp(math.nan))
+    assert lib.or_exp(709.78) == pytest.approx(math.exp(709.78), rel=4e-16)
+    assert lib.or_exp(-740.0) == math.exp(-740.0)  # subnormal result
+    x = np.exp(rng.uniform(-30, 30, 2000))
+    assert ulps(_vec(lib.or_exp, _vec(lib.or_log, x)), x).max() <= 64  # log's error times |log x|
