@@ -11,11 +11,16 @@ comparison.  Layout and conversions follow the reference:
 * read rows: s = t c p / sqrt(p^2 + 1), delta = (p - pCentral) / pCentral, loss
   pass, bunch = -1, weight = 1 (new_sdds_beam, src/sdds_beam.c:418-437).
 
-Only the ASCII flavour is produced here (SDDS1 header, one page, row count
-given); the reader accepts files with these columns in any order, extra columns
-and parameters ignored, several pages concatenated.
+Files are written as ASCII or, as elegant writes them by default, binary (SDDS1
+header with "!# little-endian", then per page the row count as a 32-bit
+integer, the parameter values and the rows, each value in its declared binary
+type, strings as a 32-bit length and the characters).  The reader takes either
+flavour (binary rows in row or column major order, either byte order), these
+columns in any order, extra columns and parameters ignored, several pages
+concatenated.
 """
 import math
+import struct
 
 import numpy as np
 
@@ -27,11 +32,14 @@ COLUMNS = (("x", "units=m, type=double"), ("xp", "symbol=\"x'\", type=double"),
            ("particleID", "type=ulong64"))
 
 
-def write_phase_space(path, rows, p_central, step=1, charge=0.0, n=None):
-    """rows: (n, 10) elegant-layout coordinates; the first n (default all) are written"""
+def write_phase_space(path, rows, p_central, step=1, charge=0.0, n=None, binary=False):
+    """rows: (n, 10) elegant-layout coordinates; the first n (default all) are
+    written, as ASCII or (binary=True) binary SDDS"""
     rows = np.asarray(rows, dtype=np.float64)
     n = len(rows) if n is None else int(n)
     lines = ["SDDS1"]
+    if binary:
+        lines.append("!# little-endian")
     lines.append('&parameter name=Step, type=long, description="Simulation step" &end')
     lines.append('&parameter name=pCentral, symbol="p$bcen$n", units="m$be$nc", type=double, '
                  'description="Reference beta*gamma" &end')
@@ -41,6 +49,20 @@ def write_phase_space(path, rows, p_central, step=1, charge=0.0, n=None):
                  'description="Number of particle ID slots reserved to a bunch" &end')
     for name, attrs in COLUMNS:
         lines.append("&column name=%s, %s &end" % (name, attrs))
+    if binary:
+        lines.append("&data mode=binary, &end")
+        p = p_central * (1 + rows[:n, 5])
+        t = rows[:n, 4] / (S.C_MKS * p / np.sqrt(p * p + 1))
+        rec = np.zeros(n, dtype=_ROW_DTYPE)
+        for k, c in enumerate(("x", "xp", "y", "yp")):
+            rec[c] = rows[:n, k]
+        rec["t"], rec["p"] = t, p
+        rec["particleID"] = rows[:n, 6].astype(np.uint64)
+        with open(path, "wb") as f:
+            f.write(("\n".join(lines) + "\n").encode())
+            f.write(struct.pack("<iiddii", n, int(step), float(p_central), float(charge), n, n))
+            f.write(rec.tobytes())
+        return
     lines.append("&data mode=ascii, &end")
     lines.append("! page number 1")
     lines += [str(int(step)), repr(float(p_central)), repr(float(charge)), str(n), str(n)]
@@ -52,6 +74,81 @@ def write_phase_space(path, rows, p_central, step=1, charge=0.0, n=None):
                                                repr(float(r[3])), repr(float(t)), repr(float(p)), int(r[6])))
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
+
+
+_ROW_DTYPE = np.dtype([("x", "<f8"), ("xp", "<f8"), ("y", "<f8"), ("yp", "<f8"), ("t", "<f8"), ("p", "<f8"),
+                       ("particleID", "<u8")])
+_TYPES = {"double": "f8", "float": "f4", "long": "i4", "ulong": "u4", "long64": "i8", "ulong64": "u8",
+          "short": "i2", "ushort": "u2", "character": "S1"}
+
+
+def _binary_header(raw):
+    """(params [(name, type)], cols [(name, type)], byte order, column major,
+    offset of the binary data) of a binary SDDS file"""
+    params, cols, order, colmajor, pos = [], [], "<", False, 0
+    first = True
+    while True:
+        end = raw.index(b"\n", pos)
+        ln = raw[pos:end].decode("latin-1").strip()
+        pos = end + 1
+        if first:
+            if not ln.startswith("SDDS"):
+                raise ValueError("not an SDDS file")
+            first = False
+            continue
+        if ln.startswith("!#"):
+            order = ">" if "big-endian" in ln else "<"
+        elif ln.startswith("&parameter"):
+            params.append((_field(ln, "name"), _field(ln, "type")))
+        elif ln.startswith("&column"):
+            cols.append((_field(ln, "name"), _field(ln, "type")))
+        elif ln.startswith("&data"):
+            compact = ln.replace(" ", "")
+            colmajor = "column_major_order=1" in compact
+            if "mode=binary" not in compact:
+                return None
+            return params, cols, order, colmajor, pos
+        elif ln.startswith("&array"):
+            raise ValueError("SDDS arrays are not supported")
+
+
+def _read_binary(raw, hdr):
+    """rows (x, xp, y, yp, t, p, id) and the first pCentral of a binary SDDS file"""
+    params, cols, order, colmajor, pos = hdr
+    names = [c for c, _ in cols]
+    need = ("x", "xp", "y", "yp", "t", "p")
+    out, pc_file = [], None
+
+    def scalar(typ, pos):
+        if typ == "string":
+            (ln,) = struct.unpack_from(order + "i", raw, pos)
+            return raw[pos + 4:pos + 4 + ln].decode("latin-1"), pos + 4 + ln
+        dt = np.dtype(order + _TYPES[typ])
+        return np.frombuffer(raw, dt, 1, pos)[0], pos + dt.itemsize
+
+    while pos < len(raw):
+        (nrows,) = struct.unpack_from(order + "i", raw, pos)
+        pos += 4
+        for name, typ in params:
+            v, pos = scalar(typ, pos)
+            if name == "pCentral" and pc_file is None:
+                pc_file = float(v)
+        if any(t == "string" for _, t in cols):
+            raise ValueError("string columns are not supported in binary SDDS")
+        dts = [np.dtype(order + _TYPES[t]) for _, t in cols]
+        if colmajor:
+            data = {}
+            for (name, _), dt in zip(cols, dts):
+                data[name] = np.frombuffer(raw, dt, nrows, pos)
+                pos += dt.itemsize * nrows
+        else:
+            rec = np.dtype([(name, dt) for (name, _), dt in zip(cols, dts)])
+            arr = np.frombuffer(raw, rec, nrows, pos)
+            pos += rec.itemsize * nrows
+            data = {name: arr[name] for name in names}
+        ids = data["particleID"].astype(np.float64) if "particleID" in data else np.arange(1, nrows + 1)
+        out.append(np.column_stack([np.asarray(data[c], dtype=np.float64) for c in need] + [ids]))
+    return (np.concatenate(out) if out else np.zeros((0, 7))), pc_file
 
 
 def _header(lines):
@@ -88,8 +185,13 @@ def _field(line, key):
 def read_phase_space(path, p_central=None):
     """returns (rows (n, 10), p_central) for all pages of the file; p_central
     defaults to the file's pCentral parameter (first page)"""
-    with open(path) as f:
-        lines = [ln for ln in f.read().splitlines()]
+    with open(path, "rb") as f:
+        raw = f.read()
+    hdr = _binary_header(raw)
+    if hdr is not None:
+        arr, pc_file = _read_binary(raw, hdr)
+        return _to_rows(arr, p_central if p_central is not None else pc_file)
+    lines = raw.decode("latin-1").splitlines()
     params, cols, i = _header(lines)
     need = ("x", "xp", "y", "yp", "t", "p")
     idx = [cols.index(c) for c in need]
@@ -107,10 +209,12 @@ def read_phase_space(path, p_central=None):
             tok = data[k + j].split()
             out.append([float(tok[c]) for c in idx] + [float(tok[id_col]) if id_col is not None else j + 1])
         k += nrows
-    pc = p_central if p_central is not None else pc_file
+    return _to_rows(np.array(out, dtype=np.float64).reshape(-1, 7), p_central if p_central is not None else pc_file)
+
+
+def _to_rows(arr, pc):
     if pc is None:
         raise ValueError("no pCentral in the file and none given")
-    arr = np.array(out, dtype=np.float64).reshape(-1, 7)
     rows = np.zeros((len(arr), S.NPROP))
     rows[:, :4] = arr[:, :4]
     p = arr[:, 5]
