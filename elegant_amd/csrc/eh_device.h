@@ -136,6 +136,7 @@ typedef struct {
   double rf_flen; /* full element length (volt == 0 drift) */
   double rf_volt, rf_omega, rf_phase, rf_dtLight, rf_t0, rf_dx, rf_dy, rf_dgamma_same;
   double rf_toff; /* CHANGE_T: whole RF periods taken off the arrival times at this visit (0: none) */
+  double rf_tau;  /* Q / omega, the cavity's fill time (0: none) */
 
   /* ---- MULT, ROTATE, MALIGN ---- */
   int32_t on_pass;                      /* ROTATE/MALIGN: -1 every pass, -2 none, else the one pass acted on */
@@ -1550,13 +1551,18 @@ EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d, const double *__restrict__ zp) {
         double t = (p.s + dc4) / (c_mks * beta_i) - d.rf_toff;
         if (ik == 0 && d.rf_toff != 0)
           p.s = t * c_mks * beta_i - dc4;
+        double dt = t - d.rf_t0;
+        if (dt < 0)
+          dt = 0;
+        const double tau = d.rf_tau;
         double dgamma = d.rf_dgamma_same;
         if (!d.rf_same) {
           if (d.rf_standing)
-            dgamma = volt * eh_sin(omega * (t - 0.0) + phase) * 1.0;
+            dgamma = volt * eh_sin(omega * (t - 0.0) + phase) * (tau ? eh_sqrt(1 - eh_exp(-dt / tau)) : 1);
           else {
             double phi = omega * (t - 0.0 - ik * d.rf_dtLight) + phase;
-            dgamma = volt * eh_sin(phi) * 1.0;
+            double dfactor = tau ? eh_sqrt(1 - eh_exp(-dt / tau)) : 1;
+            dgamma = volt * eh_sin(phi) * dfactor;
           }
         }
         if (length) {
@@ -1610,10 +1616,14 @@ EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d, const double *__restrict__ zp) {
     double t = (p.s + ds1) / (c_mks * beta_i) - d.rf_toff;
     if (d.rf_toff != 0)
       p.s = t * c_mks * beta_i - ds1;
+    double dt = t - d.rf_t0;
+    if (dt < 0)
+      dt = 0;
+    const double tau = d.rf_tau;
     double dgamma = d.rf_dgamma_same;
     if (!d.rf_same) {
       double sin_phase = eh_sin(omega * (t - 0.0 - 0 * d.rf_dtLight) + phase);
-      dgamma = (volt * 1.0) * sin_phase;
+      dgamma = (volt * (tau ? eh_sqrt(1 - eh_exp(-dt / tau)) : 1)) * sin_phase;
     }
     if (d.rf_end1 && length) {
       double inverseF = dgamma / (2 * gamma * length);

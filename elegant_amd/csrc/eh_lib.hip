@@ -602,8 +602,10 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
 
 /* RFCA constants, src/simple_rfca.c:378-470, once the fiducial phase is known */
 static long build_rfca(eh_desc *d, const EH_RFCA *rf, const eh_ctx &c) {
-  if (rf->Q != 0 || rf->linearize || rf->lockPhase)
+  if (rf->linearize || rf->lockPhase)
     return fail(EH_ERR_UNSUPPORTED, "RFCA option not supported");
+  if (rf->change_t && rf->Q)
+    return fail(EH_ERR_ARG, "incompatible RF cavity parameters: change_t!=0 && Q!=0");
   d->rf_flen = rf->length;
   if (rf->volt == 0) {
     d->rf_zero_volt = 1;
@@ -638,6 +640,7 @@ static long build_rfca(eh_desc *d, const EH_RFCA *rf, const eh_ctx &c) {
   d->rf_dx = rf->dx;
   d->rf_dy = rf->dy;
   d->rf_toff = rf->change_t ? c.rf_toff : 0.0;
+  d->rf_tau = omega ? rf->Q / omega : 0;
   return EH_OK;
 }
 
@@ -2238,7 +2241,7 @@ extern "C" long eh_simple_rf_cavity(eh_beam *b, long np, EH_RFCA *rfca, double *
     return r;
   if (np <= 0)
     return np;
-  if (rfca->Q != 0 || rfca->linearize || rfca->lockPhase)
+  if (rfca->linearize || rfca->lockPhase)
     return fail(EH_ERR_UNSUPPORTED, "RFCA option not supported");
   eh_ctx c = {};
   if (rfca->volt != 0) {
@@ -4335,8 +4338,8 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
         const EH_RFCA *rf = (const EH_RFCA *)beamlines[v][e].p_elem;
         if (rf->phase_reference)
           return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support PHASE_REFERENCE", e);
-        if (rf->change_t || rf->change_p0)
-          return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support CHANGE_T or CHANGE_P0", e);
+        if (rf->change_t || rf->change_p0 || rf->Q != 0)
+          return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support CHANGE_T, CHANGE_P0 or Q", e);
         /* a variant without particles never reads its descriptors */
         if (!rf->fiducial_seen && rf->volt != 0 && !(counts && counts[v] == 0))
           return fail(EH_ERR_ARG, "element %ld: RFCA fiducial unknown; track one pass first to fix it", e);

@@ -1989,7 +1989,9 @@ static void add_to_particle_energy(double *coord, double timeOfFlight, double Po
 }
 
 /* simple_rf_cavity -> trackRfCavityWithWakes, src/simple_rfca.c:264-985 (no wakes,
- * no LSC, Q = 0, no linearize/lockPhase, no backtrack, body focus model none);
+ * no LSC, no linearize/lockPhase, no backtrack, body focus model none; Q gives
+ * the fill factor sqrt(1 - exp(-dt/tau)), tau = Q/omega, dt the arrival after
+ * the fiducial time, :399-401, :594-599, :872);
  * CHANGE_T takes whole RF periods off the time coordinates, judged on the
  * particle in row 0 (:488-498, :586-588, :864-865); CHANGE_P0 matches the
  * reference momentum to the surviving beam (:977-982) */
@@ -1999,7 +2001,7 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
   double P, gamma, gamma1, beta_i, t, dt, x, xp, dc4, *coord;
   int matrixMethod = 0, same_dgamma = 0;
 
-  if (rfca->Q != 0 || rfca->linearize || rfca->lockPhase)
+  if (rfca->linearize || rfca->lockPhase)
     return or_fail("RFCA option not supported");
   if (np <= 0)
     return np;
@@ -2017,7 +2019,9 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
   }
   omega = PIx2 * rfca->freq;
   volt = rfca->volt / (1e6 * particleMassMV * particleRelSign);
-  tau = 0;
+  tau = omega ? rfca->Q / omega : 0;
+  if (rfca->change_t && rfca->Q)
+    return or_fail("incompatible RF cavity parameters: change_t!=0 && Q!=0");
   nKicks = length ? rfca->nKicks : 1;
   if (nKicks <= 0) {
     matrixMethod = 1;
@@ -2095,13 +2099,13 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
           coord[4] = t * c_mks * beta_i - dc4;
         if ((dt = t - t0) < 0)
           dt = 0;
-        (void)dt;
         if (!same_dgamma) {
           if (rfca->standingWave)
-            dgamma = volt * OR_SIN(omega * (t - 0.0) + phase) * 1.0;
+            dgamma = volt * OR_SIN(omega * (t - 0.0) + phase) * (tau ? sqrt(1 - OR_EXP(-dt / tau)) : 1);
           else {
             double phi = omega * (t - 0.0 - ik * dtLight) + phase;
-            dgamma = volt * OR_SIN(phi) * 1.0;
+            double dfactor = tau ? sqrt(1 - OR_EXP(-dt / tau)) : 1;
+            dgamma = volt * OR_SIN(phi) * dfactor;
           }
         }
         if (length) {
@@ -2150,7 +2154,7 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
           dt = 0;
         if (!same_dgamma) {
           sin_phase = OR_SIN(omega * (t - 0.0 - ik * dtLight) + phase);
-          dgamma = (dgammaMax = volt * 1.0) * sin_phase;
+          dgamma = (dgammaMax = volt * (tau ? sqrt(1 - OR_EXP(-dt / tau)) : 1)) * sin_phase;
         }
         if (rfca->end1Focus && length && ik == 0) {
           inverseF = dgamma / (2 * gamma * length);

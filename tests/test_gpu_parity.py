@@ -151,13 +151,17 @@ def test_single_element(case, ehlib, oracle_mod):
     assert_rows_identical(got, ref, n, nref, "%s case %d" % (etype, case))
 
 
-@pytest.mark.parametrize("method", ["matrix", "kick"])
+@pytest.mark.parametrize("method", ["matrix", "kick", "matrix Q", "kick Q", "standing Q"])
 def test_rfca(method, ehlib, oracle_mod):
     _say("rfca %s" % method)
     Po = configs.p_central_from_mev(3000.0)
     params = configs.mba_rf(Po)
-    if method == "kick":
+    if "kick" in method or "standing" in method:
         params["N_KICKS"] = 4
+    if "standing" in method:
+        params["STANDING_WAVE"] = 1
+    if "Q" in method:
+        params["Q"] = 3000.0  # a fill time of ~1 us against arrivals spread over ~10 ns
     lat = Lattice([Element("RF", "RFCA", **params)])
     rows = _beam_rows(5000, seed=11)
     rows[:, 4] += 123.456

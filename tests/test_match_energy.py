@@ -12,6 +12,8 @@ restating the formulas:
   mean momentum becomes the reference, each time of flight is kept;
 * CHANGE_P0: the cavity followed by the matching, the reference momentum then
   the survivors' mean momentum;
+* Q: the fill factor sqrt(1 - exp(-dt/tau)) on the energy gain, none before the
+  fiducial time, the whole gain many fill times after it;
 * CHANGE_T: the offset is the whole number of RF periods in row 0's arrival
   time (from 0.9 periods on), the cavity's kick is that of the unshifted beam
   (the sine is periodic), and the time coordinate drops by offset x c x beta
@@ -197,3 +199,35 @@ def test_energy_matching_modes_in_do_tracking():
     els[1] = Element("E", "ENERGY", MATCH_PARTICLES=1, MATCH_BEAMLINE=1)
     with pytest.raises(RuntimeError, match="match_beamline AND match_particles"):
         oracle.do_tracking(b, Lattice(els), 1, Po)
+
+
+def test_cavity_fill_time():
+    """Q: the energy gain of a particle arriving dt after the fiducial time is
+    the unloaded one times sqrt(1 - exp(-dt/tau)), tau = Q/omega
+    (src/simple_rfca.c:399-401, :872): a particle arriving before the fiducial
+    gains nothing, one arriving many fill times later the full gain"""
+    freq = 5e8
+    omega = 2 * math.pi * freq
+    beta0 = Po / math.sqrt(Po * Po + 1)
+    b = beam(6, 9, mean_dp=0.0, sigma_dp=0.0)
+    b[:, :4] = 0.0
+    c = 2.99792458e8
+    # arrival times at the cavity centre: before the fiducial, just after, long after
+    b[:, 4] = np.array([-3e-9, 0.0, 2e-10, 1e-9, 4e-6, 8e-6]) * c * beta0
+    out = {}
+    for q in (0.0, 300.0):
+        lat = rf_lattice(Q=q, PHASE=90.0, L=0.0, N_KICKS=0)
+        _set_fiducial(lat, 0.0)  # t0 = 0
+        out[q] = oracle.element(b, lat, 0, Po, 0.0, 0.0)[0]
+    g0 = out[0.0][:, 5] - b[:, 5]
+    gq = out[300.0][:, 5] - b[:, 5]
+    tau = 300.0 / omega
+    # arriving at or before the fiducial time: the cavity is empty (the momentum
+    # only goes through gamma and back, a rounding of 1e-16)
+    assert abs(gq[0]) < 1e-15 and abs(gq[1]) < 1e-15 and abs(g0[0]) > 1e-4
+    for k in (2, 3):
+        dt = b[k, 4] / (c * beta0)
+        assert abs(gq[k] / g0[k] - math.sqrt(1 - math.exp(-dt / tau))) < 1e-6
+    assert abs(gq[5] / g0[5] - 1) < 1e-12
+    with pytest.raises(RuntimeError, match="change_t!=0 && Q!=0"):
+        oracle.element(b, rf_lattice(Q=300.0, CHANGE_T=1), 0, Po, 0.0, 0.3)
