@@ -396,8 +396,6 @@ static const char *const rfca_untracked[] __attribute__((unused)) = {"backtrack"
 static int fill_rfca(const RFCA *rf, EH_RFCA *d) {
   unsigned long mode;
   int32_t fid;
-  if (rf->linearize || rf->lockPhase)
-    return 0;
   if (rf->bodyFocusModel && rf->bodyFocusModel[0] && strcmp(rf->bodyFocusModel, "none") != 0)
     return 0;
   mode = parseFiducialMode(rf->fiducial);

@@ -137,6 +137,11 @@ typedef struct {
   double rf_volt, rf_omega, rf_phase, rf_dtLight, rf_t0, rf_dx, rf_dy, rf_dgamma_same;
   double rf_toff; /* CHANGE_T: whole RF periods taken off the arrival times at this visit (0: none) */
   double rf_tau;  /* Q / omega, the cavity's fill time (0: none) */
+  /* LOCK_PHASE / LINEARIZE at this visit: the beam's mean arrival time tave,
+   * tlock = tave with LOCK_PHASE (else 0), the mean energy gain and the cosines
+   * of the linearized kick (kick and matrix methods) */
+  int32_t rf_lin, rf_pad_;
+  double rf_tlock, rf_tave, rf_dgave, rf_cosk, rf_cosm;
 
   /* ---- MULT, ROTATE, MALIGN ---- */
   int32_t on_pass;                      /* ROTATE/MALIGN: -1 every pass, -2 none, else the one pass acted on */
@@ -1557,13 +1562,16 @@ EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d, const double *__restrict__ zp) {
         const double tau = d.rf_tau;
         double dgamma = d.rf_dgamma_same;
         if (!d.rf_same) {
-          if (d.rf_standing)
-            dgamma = volt * eh_sin(omega * (t - 0.0) + phase) * (tau ? eh_sqrt(1 - eh_exp(-dt / tau)) : 1);
-          else {
-            double phi = omega * (t - 0.0 - ik * d.rf_dtLight) + phase;
-            double dfactor = tau ? eh_sqrt(1 - eh_exp(-dt / tau)) : 1;
-            dgamma = volt * eh_sin(phi) * dfactor;
-          }
+          if (!d.rf_lin) {
+            if (d.rf_standing)
+              dgamma = volt * eh_sin(omega * (t - d.rf_tlock) + phase) * (tau ? eh_sqrt(1 - eh_exp(-dt / tau)) : 1);
+            else {
+              double phi = omega * (t - d.rf_tlock - ik * d.rf_dtLight) + phase;
+              double dfactor = tau ? eh_sqrt(1 - eh_exp(-dt / tau)) : 1;
+              dgamma = volt * eh_sin(phi) * dfactor;
+            }
+          } else
+            dgamma = d.rf_dgave + volt * omega * (t - d.rf_tave) * d.rf_cosk;
         }
         if (length) {
           if (d.rf_end1 && ik == 0) {
@@ -1622,8 +1630,13 @@ EH_HD bool eh_rfca(eh_p6 &p, const eh_desc &d, const double *__restrict__ zp) {
     const double tau = d.rf_tau;
     double dgamma = d.rf_dgamma_same;
     if (!d.rf_same) {
-      double sin_phase = eh_sin(omega * (t - 0.0 - 0 * d.rf_dtLight) + phase);
-      dgamma = (volt * (tau ? eh_sqrt(1 - eh_exp(-dt / tau)) : 1)) * sin_phase;
+      if (!d.rf_lin) {
+        double sin_phase = eh_sin(omega * (t - d.rf_tlock - 0 * d.rf_dtLight) + phase);
+        dgamma = (volt * (tau ? eh_sqrt(1 - eh_exp(-dt / tau)) : 1)) * sin_phase;
+      } else {
+        double sin_phase = omega * (t - d.rf_tave) * d.rf_cosm;
+        dgamma = (volt * (tau ? eh_sqrt(1 - eh_exp(-dt / tau)) : 1)) * sin_phase + d.rf_dgave;
+      }
     }
     if (d.rf_end1 && length) {
       double inverseF = dgamma / (2 * gamma * length);

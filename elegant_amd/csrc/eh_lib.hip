@@ -186,6 +186,8 @@ struct eh_ctx {
   double rf_ref_phase;
   /* CHANGE_T: the time offset found at this visit (src/simple_rfca.c:488-498) */
   double rf_toff;
+  /* LINEARIZE / LOCK_PHASE: the beam's mean arrival time at this visit */
+  double rf_tave;
 };
 
 static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &c) {
@@ -602,8 +604,6 @@ static long build_csbend(eh_desc *d, const EH_CSBEND *cb, const eh_ctx &c, std::
 
 /* RFCA constants, src/simple_rfca.c:378-470, once the fiducial phase is known */
 static long build_rfca(eh_desc *d, const EH_RFCA *rf, const eh_ctx &c) {
-  if (rf->linearize || rf->lockPhase)
-    return fail(EH_ERR_UNSUPPORTED, "RFCA option not supported");
   if (rf->change_t && rf->Q)
     return fail(EH_ERR_ARG, "incompatible RF cavity parameters: change_t!=0 && Q!=0");
   d->rf_flen = rf->length;
@@ -641,6 +641,26 @@ static long build_rfca(eh_desc *d, const EH_RFCA *rf, const eh_ctx &c) {
   d->rf_dy = rf->dy;
   d->rf_toff = rf->change_t ? c.rf_toff : 0.0;
   d->rf_tau = omega ? rf->Q / omega : 0;
+  /* LINEARIZE / LOCK_PHASE, src/simple_rfca.c:500-556, with the beam's mean
+   * arrival time found at this visit */
+  if (rf->linearize || rf->lockPhase) {
+    if (nKicks != 1)
+      return fail(EH_ERR_ARG, "Must use n_kicks=1 for linearized rf cavity");
+    const double toff = d->rf_toff, tAve = c.rf_tave;
+    double dgammaAve;
+    if (rf->lockPhase) {
+      phase = EH_PI / 180 * rf->phase;
+      d->rf_phase = phase;
+      dgammaAve = volt * eh_sin(phase);
+    } else
+      dgammaAve = volt * eh_sin(omega * (tAve - toff) + phase);
+    d->rf_lin = rf->linearize;
+    d->rf_tlock = rf->lockPhase ? tAve : 0.0;
+    d->rf_tave = tAve;
+    d->rf_dgave = dgammaAve;
+    d->rf_cosk = eh_cos(omega * (tAve - toff) + phase);
+    d->rf_cosm = eh_cos(omega * tAve + phase);
+  }
   return EH_OK;
 }
 
@@ -2120,6 +2140,40 @@ static long rf_time_offset(eh_beam *b, const EH_RFCA *rf, double Pc, double *tof
   return EH_OK;
 }
 
+/* LINEARIZE / LOCK_PHASE's mean arrival time at a cavity visit
+ * (src/simple_rfca.c:503-521): a Kahan sum in the reference's row order of
+ * (s + L/2) / (c beta) - timeOffset over the alive particles */
+static long rf_mean_time(eh_beam *b, const EH_RFCA *rf, double Pc, double toff, double *tAve) {
+  *tAve = 0;
+  if (!(rf->linearize || rf->lockPhase) || rf->volt == 0 || b->n_alive == 0)
+    return EH_OK;
+  std::vector<int32_t> slot_at_row, alive_row0, idx;
+  long r = reference_order(b, slot_at_row, alive_row0, idx);
+  if (r < 0)
+    return r;
+  const long n = b->n_alive;
+  std::vector<double> s(n), dp(n);
+  eh_soa &a = b->buf[b->cur];
+  HIPCHK(hipMemcpy(s.data(), a.s, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(dp.data(), a.dp, sizeof(double) * n, hipMemcpyDeviceToHost));
+  const double length = rf->length; /* n_kicks is 1 */
+  double sum = 0, error = 0.0;
+  for (long row = 0; row < n; row++) {
+    const int32_t k = idx[slot_at_row[row]];
+    if (k < 0)
+      return fail(EH_ERR_STATE, "replay produced a lost particle among survivors");
+    double P = Pc * (1 + dp[k]);
+    double beta_i = P / std::sqrt(P * P + 1);
+    double t = (s[k] + length / 2) / (EH_C_MKS * beta_i) - toff;
+    double Y = t + error;
+    double s1 = sum + Y;
+    error = Y - (s1 - sum);
+    sum = s1;
+  }
+  *tAve = sum / n;
+  return EH_OK;
+}
+
 /* phase references, src/phase_reference.c:28-143: numbered groups of
  * time-dependent elements sharing one reference phase, set by the first cavity
  * of the group tracked with particles and kept until eh_delete_phase_references.
@@ -2241,8 +2295,6 @@ extern "C" long eh_simple_rf_cavity(eh_beam *b, long np, EH_RFCA *rfca, double *
     return r;
   if (np <= 0)
     return np;
-  if (rfca->linearize || rfca->lockPhase)
-    return fail(EH_ERR_UNSUPPORTED, "RFCA option not supported");
   eh_ctx c = {};
   if (rfca->volt != 0) {
     r = rf_resolve(b, rfca, *P_central, zEnd, c);
@@ -2251,6 +2303,9 @@ extern "C" long eh_simple_rf_cavity(eh_beam *b, long np, EH_RFCA *rfca, double *
     if (rfca->freq != 0)
       rfca->t_fiducial = -rfca->phase_fiducial / (2 * EH_PI * rfca->freq);
     r = rf_time_offset(b, rfca, *P_central, &c.rf_toff);
+    if (r < 0)
+      return r;
+    r = rf_mean_time(b, rfca, *P_central, c.rf_toff, &c.rf_tave);
     if (r < 0)
       return r;
   }
@@ -2666,12 +2721,12 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
    * a matching mode.  Lines holding them are tracked in launches that end there. */
   std::vector<char> stop_before(ne, 0), stop_after(ne, 0);
   std::vector<long> match_mode(ne, 0);
-  std::vector<double> toff(ne, 0.0);
+  std::vector<double> toff(ne, 0.0), tave(ne, 0.0);
   bool dynamic = false;
   for (long ie = 0; ie < ne; ie++) {
     if (bl[ie].type == EH_T_RFCA) {
       const EH_RFCA *rf = (const EH_RFCA *)bl[ie].p_elem;
-      stop_before[ie] = rf->volt != 0 && rf->change_t && rf->freq != 0;
+      stop_before[ie] = rf->volt != 0 && ((rf->change_t && rf->freq != 0) || rf->linearize || rf->lockPhase);
       stop_after[ie] = rf->volt != 0 && rf->change_p0;
     } else if (bl[ie].type == EH_T_ENERGY) {
       double Pn = 0;
@@ -2695,6 +2750,7 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
     c.rf_linked = link[ie];
     c.rf_ref_phase = lph[ie];
     c.rf_toff = toff[ie];
+    c.rf_tave = tave[ie];
     c.Po = (*Pe)[ie];
     c.z_start = zs[ie];
     c.z_end = ze[ie];
@@ -2870,6 +2926,10 @@ static long track_fused(eh_beam *b, EH_ELEMENT_LIST *bl, long ne, long n_passes,
       }
       if (stop_before[e]) {
         r = rf_time_offset(b, (const EH_RFCA *)bl[e].p_elem, P, &toff[e]);
+        if (r < 0)
+          return r;
+        const EH_RFCA *rf = (const EH_RFCA *)bl[e].p_elem;
+        r = rf_mean_time(b, rf, P, rf->change_t ? toff[e] : 0.0, &tave[e]);
         if (r < 0)
           return r;
         r = build(e);
@@ -3699,9 +3759,10 @@ extern "C" long eh_batch_track(eh_batch *b, EH_ELEMENT_LIST *const *beamlines, l
         const EH_RFCA *rf = (const EH_RFCA *)beamlines[v][e].p_elem;
         if (rf->phase_reference)
           return fail(EH_ERR_UNSUPPORTED, "element %ld: PHASE_REFERENCE is not supported in batched tracking", e);
-        if (rf->change_t || rf->change_p0)
-          return fail(EH_ERR_UNSUPPORTED, "element %ld: CHANGE_T and CHANGE_P0 are not supported in batched tracking",
-                      e);
+        if (rf->change_t || rf->change_p0 || rf->linearize || rf->lockPhase)
+          return fail(EH_ERR_UNSUPPORTED,
+                      "element %ld: CHANGE_T, CHANGE_P0, LINEARIZE and LOCK_PHASE are not supported in batched "
+                      "tracking", e);
         if (rf->fiducial_mode != EH_FID_TMEAN && rf->fiducial_mode != EH_FID_LIGHT && !rf->fiducial_seen)
           return fail(EH_ERR_UNSUPPORTED, "element %ld: batched tracking takes TMEAN and LIGHT fiducials only", e);
       }
@@ -4338,8 +4399,9 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
         const EH_RFCA *rf = (const EH_RFCA *)beamlines[v][e].p_elem;
         if (rf->phase_reference)
           return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support PHASE_REFERENCE", e);
-        if (rf->change_t || rf->change_p0 || rf->Q != 0)
-          return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support CHANGE_T, CHANGE_P0 or Q", e);
+        if (rf->change_t || rf->change_p0 || rf->Q != 0 || rf->linearize || rf->lockPhase)
+          return fail(EH_ERR_UNSUPPORTED,
+                      "element %ld: the adjoint does not support CHANGE_T, CHANGE_P0, Q, LINEARIZE or LOCK_PHASE", e);
         /* a variant without particles never reads its descriptors */
         if (!rf->fiducial_seen && rf->volt != 0 && !(counts && counts[v] == 0))
           return fail(EH_ERR_ARG, "element %ld: RFCA fiducial unknown; track one pass first to fix it", e);

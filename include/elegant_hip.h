@@ -201,6 +201,9 @@ typedef struct {
    * arrangement (:488-498, :586-588, :864-865).  Not in batched tracking or the
    * adjoint (EH_ERR_UNSUPPORTED). */
   int32_t change_p0, change_t, end1Focus, end2Focus, standingWave;
+  /* LINEARIZE, LOCK_PHASE: on the beam's mean arrival time at each visit
+   * (src/simple_rfca.c:500-556); N_KICKS must be 1 (EH_ERR_ARG otherwise).  Not
+   * in batched tracking or the adjoint. */
   int32_t linearize, lockPhase;
   int32_t fiducial_mode; /* eh_fid_mode */
   /* PHASE_REFERENCE: cavities with the same non-zero number share the phase

@@ -1989,7 +1989,8 @@ static void add_to_particle_energy(double *coord, double timeOfFlight, double Po
 }
 
 /* simple_rf_cavity -> trackRfCavityWithWakes, src/simple_rfca.c:264-985 (no wakes,
- * no LSC, no linearize/lockPhase, no backtrack, body focus model none; Q gives
+ * no LSC, no backtrack, body focus model none; LINEARIZE and LOCK_PHASE through
+ * the beam's mean arrival time; Q gives
  * the fill factor sqrt(1 - exp(-dt/tau)), tau = Q/omega, dt the arrival after
  * the fiducial time, :399-401, :594-599, :872);
  * CHANGE_T takes whole RF periods off the time coordinates, judged on the
@@ -2001,8 +2002,7 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
   double P, gamma, gamma1, beta_i, t, dt, x, xp, dc4, *coord;
   int matrixMethod = 0, same_dgamma = 0;
 
-  if (rfca->linearize || rfca->lockPhase)
-    return or_fail("RFCA option not supported");
+  double tAve = 0, dgammaAve = 0, cos_phase;
   if (np <= 0)
     return np;
   length = rfca->length;
@@ -2074,6 +2074,26 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
     if (omega != 0 && t > (0.9 * To) && rfca->change_t)
       timeOffset = ((long)(t / To + 0.5)) * To;
   }
+  /* LINEARIZE, LOCK_PHASE: the mean arrival time (a Kahan sum in row order),
+   * src/simple_rfca.c:500-556 */
+  if (rfca->linearize || rfca->lockPhase) {
+    double error = 0.0;
+    if (nKicks != 1)
+      return or_fail("Must use n_kicks=1 for linearized rf cavity");
+    for (ip = 0; ip < np; ip++) {
+      coord = part[ip];
+      P = *P_central * (1 + coord[5]);
+      beta_i = P / (gamma = sqrt(sqr(P) + 1));
+      t = (coord[4] + length / 2) / (c_mks * beta_i) - timeOffset;
+      tAve = KahanPlus(tAve, t, &error);
+    }
+    tAve /= np;
+    if (rfca->lockPhase) {
+      phase = PI / 180 * rfca->phase;
+      dgammaAve = volt * OR_SIN(phase);
+    } else
+      dgammaAve = volt * OR_SIN(omega * (tAve - timeOffset) + phase);
+  }
 
   for (ip = 0; ip < np; ip++) {
     coord = part[ip];
@@ -2100,13 +2120,17 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
         if ((dt = t - t0) < 0)
           dt = 0;
         if (!same_dgamma) {
-          if (rfca->standingWave)
-            dgamma = volt * OR_SIN(omega * (t - 0.0) + phase) * (tau ? sqrt(1 - OR_EXP(-dt / tau)) : 1);
-          else {
-            double phi = omega * (t - 0.0 - ik * dtLight) + phase;
-            double dfactor = tau ? sqrt(1 - OR_EXP(-dt / tau)) : 1;
-            dgamma = volt * OR_SIN(phi) * dfactor;
-          }
+          if (!rfca->linearize) {
+            if (rfca->standingWave)
+              dgamma = volt * OR_SIN(omega * (t - (rfca->lockPhase ? tAve : 0)) + phase) *
+                       (tau ? sqrt(1 - OR_EXP(-dt / tau)) : 1);
+            else {
+              double phi = omega * (t - (rfca->lockPhase ? tAve : 0) - ik * dtLight) + phase;
+              double dfactor = tau ? sqrt(1 - OR_EXP(-dt / tau)) : 1;
+              dgamma = volt * OR_SIN(phi) * dfactor;
+            }
+          } else
+            dgamma = dgammaAve + volt * omega * (t - tAve) * OR_COS(omega * (tAve - timeOffset) + phase);
         }
         if (length) {
           if (rfca->end1Focus && ik == 0) {
@@ -2153,8 +2177,14 @@ long or_simple_rf_cavity(double **part, long np, EH_RFCA *rfca, double *P_centra
         if ((dt = t - t0) < 0)
           dt = 0;
         if (!same_dgamma) {
-          sin_phase = OR_SIN(omega * (t - 0.0 - ik * dtLight) + phase);
-          dgamma = (dgammaMax = volt * (tau ? sqrt(1 - OR_EXP(-dt / tau)) : 1)) * sin_phase;
+          if (!rfca->linearize) {
+            sin_phase = OR_SIN(omega * (t - (rfca->lockPhase ? tAve : 0) - ik * dtLight) + phase);
+            dgamma = (dgammaMax = volt * (tau ? sqrt(1 - OR_EXP(-dt / tau)) : 1)) * sin_phase;
+          } else {
+            cos_phase = OR_COS(omega * tAve + phase);
+            sin_phase = omega * (t - tAve) * cos_phase;
+            dgamma = (dgammaMax = volt * (tau ? sqrt(1 - OR_EXP(-dt / tau)) : 1)) * sin_phase + dgammaAve;
+          }
         }
         if (rfca->end1Focus && length && ik == 0) {
           inverseF = dgamma / (2 * gamma * length);

@@ -77,6 +77,10 @@ CASES = {
                                                   rf(CHANGE_P0=1, PHASE=100.0),
                                                   energy=dict(CENTRAL_MOMENTUM=Po * 1.001)),
     "change_p0 elliptical": lambda: line(rf(CHANGE_P0=1, CHANGE_T=1, PHASE=95.0), elliptical=True),
+    "lock phase kick": lambda: line(rf(LOCK_PHASE=1, N_KICKS=1, PHASE=100.0)),
+    "linearize matrix": lambda: line(rf(LINEARIZE=1, PHASE=95.0)),
+    "linearize change_t Q": lambda: line(rf(LINEARIZE=1, CHANGE_T=1, N_KICKS=1, PHASE=95.0),
+                                         rf(LOCK_PHASE=1, Q=2000.0, PHASE=100.0)),
     "match beamline": lambda: line(rf(), energy=dict(MATCH_BEAMLINE=1)),
     "match particles": lambda: line(rf(CHANGE_T=1), energy=dict(MATCH_PARTICLES=1)),
 }
@@ -147,9 +151,9 @@ def test_cavity_entry_point_changes_p_central(ehlib):
 def test_batch_refuses_change_t_and_p0(ehlib):
     _say("batch refusals")
     rows = beam(100, 15)
-    for kw in (dict(CHANGE_T=1), dict(CHANGE_P0=1)):
+    for kw in (dict(CHANGE_T=1), dict(CHANGE_P0=1), dict(LOCK_PHASE=1, N_KICKS=1), dict(LINEARIZE=1)):
         lat = line(rf(**kw))
         batch = tracking.Batch([rows, rows])
-        with pytest.raises(_lib.EHError, match="CHANGE_T and CHANGE_P0"):
+        with pytest.raises(_lib.EHError, match="CHANGE_T, CHANGE_P0, LINEARIZE and LOCK_PHASE"):
             batch.track([lat, lat.copy()], 1, Po)
         batch.free()

@@ -14,6 +14,9 @@ restating the formulas:
   the survivors' mean momentum;
 * Q: the fill factor sqrt(1 - exp(-dt/tau)) on the energy gain, none before the
   fiducial time, the whole gain many fill times after it;
+* LOCK_PHASE: the cavity's phase is fixed on the bunch's mean arrival time, so
+  the result does not depend on the fiducial; LINEARIZE: the energy gain is
+  linear in the arrival time and departs from the full sine at second order;
 * CHANGE_T: the offset is the whole number of RF periods in row 0's arrival
   time (from 0.9 periods on), the cavity's kick is that of the unshifted beam
   (the sine is periodic), and the time coordinate drops by offset x c x beta
@@ -231,3 +234,39 @@ def test_cavity_fill_time():
     assert abs(gq[5] / g0[5] - 1) < 1e-12
     with pytest.raises(RuntimeError, match="change_t!=0 && Q!=0"):
         oracle.element(b, rf_lattice(Q=300.0, CHANGE_T=1), 0, Po, 0.0, 0.3)
+
+
+def _rf_out(b, fid, **kw):
+    lat = rf_lattice(**kw)
+    _set_fiducial(lat, fid)
+    return oracle.element(b, lat, 0, Po, 0.0, 0.3)
+
+
+def test_lock_phase_ignores_the_fiducial():
+    b = beam(400, 10, mean_dp=0.0)
+    b[:, 4] += 3.7
+    for nk in (0, 1):
+        r1, n1 = _rf_out(b, 0.0, LOCK_PHASE=1, N_KICKS=nk, PHASE=120.0)
+        r2, n2 = _rf_out(b, 1.234, LOCK_PHASE=1, N_KICKS=nk, PHASE=120.0)
+        assert n1 == n2 and np.array_equal(r1, r2)
+        r3, _ = _rf_out(b, 1.234, N_KICKS=nk, PHASE=120.0)  # unlocked: the fiducial matters
+        assert not np.array_equal(r1, r3)
+    with pytest.raises(RuntimeError, match="n_kicks=1"):
+        _rf_out(b, 0.0, LOCK_PHASE=1, N_KICKS=4)
+
+
+def test_linearize_is_linear_in_arrival_time():
+    """particles on a line in arrival time: the linearized gain has zero second
+    differences; it matches the full sine at the mean arrival time to first
+    order (the difference grows as the square of the spread)"""
+    b = np.zeros((41, S.NPROP))
+    b[:, 6] = np.arange(1, 42)
+    d = []
+    for spread in (1e-3, 2e-3):
+        b[:, 4] = 5.0 + spread * np.linspace(-1, 1, 41)
+        lin, _ = _rf_out(b, 0.0, LINEARIZE=1, PHASE=100.0, L=0.0)
+        full, _ = _rf_out(b, 0.0, PHASE=100.0, L=0.0)
+        g = lin[:, 5]
+        assert np.max(np.abs(np.diff(g, 2))) < 1e-15
+        d.append(np.max(np.abs(lin[:, 5] - full[:, 5])))
+    assert 3.5 < d[1] / d[0] < 4.5, d
