@@ -177,15 +177,15 @@ static const char *const kquad_untracked[] __attribute__((unused)) = {
     "xSteering", "ySteering", "synchRadInOrdinaryMatrix", "trackingBasedMatrix", "sqrtOrder",
     "systematicMultipoleFactor", "randomMultipoleFactor", "steeringMultipoleFactor", "minMultipoleOrder",
     "maxMultipoleOrder", "multipolesInitialized", "totalMultipolesComputed", "systematicMultipoleData",
-    "edgeMultipoleData", "randomMultipoleData", "totalMultipoleData", "steeringMultipoleData", "B", "isr1Particle",
+    "edgeMultipoleData", "randomMultipoleData", "totalMultipoleData", "steeringMultipoleData", "isr1Particle",
     "fringeIntP", "fringeIntM", "edge1Linear", "edge2Linear", "edge1NonlinearFactor", "edge2NonlinearFactor", NULL};
 
 /* KQUAD, src/track.h:2204-2228, as multipole_tracking2 reads it
- * (src/multipole.c:726-791): B is read only with a bore, the fringe integrals
+ * (src/multipole.c:726-791): the fringe integrals
  * and edge factors only with EDGE1/2_EFFECTS, the multipole factors and order
  * limits only with multipole files */
 static int fill_kquad(const KQUAD *q, EH_KQUAD *d) {
-  if (q->pitch != 0 || q->yaw != 0 || q->bore != 0 || q->isr || q->edge1_effects || q->edge2_effects ||
+  if (q->pitch != 0 || q->yaw != 0 || q->isr || q->edge1_effects || q->edge2_effects ||
       q->lEffective > 0 || q->radial || q->malignMethod != 0)
     return 0;
   if (!no_file(q->systematic_multipoles) || !no_file(q->edge_multipoles) || !no_file(q->random_multipoles) ||
@@ -199,6 +199,8 @@ static int fill_kquad(const KQUAD *q, EH_KQUAD *d) {
   d->dy = q->dy;
   d->dz = q->dz;
   d->fse = q->fse;
+  d->bore = q->bore;
+  d->B = q->B;
   d->xkick = q->xkick;
   d->ykick = q->ykick;
   d->xKickCalibration = q->xKickCalibration;
@@ -215,11 +217,11 @@ static const char *const ksext_untracked[] __attribute__((unused)) = {
     "xSteering", "ySteering", "sqrtOrder", "systematicMultipoleFactor", "randomMultipoleFactor",
     "steeringMultipoleFactor", "minMultipoleOrder", "maxMultipoleOrder", "multipolesInitialized",
     "totalMultipolesComputed", "systematicMultipoleData", "edgeMultipoleData", "randomMultipoleData",
-    "totalMultipoleData", "steeringMultipoleData", "B", "isr1Particle", NULL};
+    "totalMultipoleData", "steeringMultipoleData", "isr1Particle", NULL};
 
 /* KSEXT, src/track.h:2149-2166, src/multipole.c:796-862 */
 static int fill_ksext(const KSEXT *q, EH_KSEXT *d) {
-  if (q->pitch != 0 || q->yaw != 0 || q->bore != 0 || q->isr || q->malignMethod != 0)
+  if (q->pitch != 0 || q->yaw != 0 || q->isr || q->malignMethod != 0)
     return 0;
   if (!no_file(q->systematic_multipoles) || !no_file(q->edge_multipoles) || !no_file(q->random_multipoles) ||
       !no_file(q->steering_multipoles))
@@ -234,6 +236,8 @@ static int fill_ksext(const KSEXT *q, EH_KSEXT *d) {
   d->dy = q->dy;
   d->dz = q->dz;
   d->fse = q->fse;
+  d->bore = q->bore;
+  d->B = q->B;
   d->xkick = q->xkick;
   d->ykick = q->ykick;
   d->xKickCalibration = q->xKickCalibration;
@@ -248,11 +252,11 @@ static int fill_ksext(const KSEXT *q, EH_KSEXT *d) {
 
 static const char *const koct_untracked[] __attribute__((unused)) = {"sqrtOrder", "multipolesInitialized", "totalMultipolesComputed",
                                               "systematicMultipoleData", "randomMultipoleData",
-                                              "totalMultipoleData", "B", "isr1Particle", NULL};
+                                              "totalMultipoleData", "isr1Particle", NULL};
 
 /* KOCT, src/track.h:2171-2182, src/multipole.c:867-922 */
 static int fill_koct(const KOCT *q, EH_KOCT *d) {
-  if (q->pitch != 0 || q->yaw != 0 || q->bore != 0 || q->isr || q->malignMethod != 0)
+  if (q->pitch != 0 || q->yaw != 0 || q->isr || q->malignMethod != 0)
     return 0;
   if (!no_file(q->systematic_multipoles) || !no_file(q->random_multipoles))
     return 0;
@@ -264,6 +268,8 @@ static int fill_koct(const KOCT *q, EH_KOCT *d) {
   d->dy = q->dy;
   d->dz = q->dz;
   d->fse = q->fse;
+  d->bore = q->bore;
+  d->B = q->B;
   d->n_kicks = (int32_t)q->n_kicks;
   d->nSlices = (int32_t)q->nSlices;
   d->integration_order = q->integration_order;

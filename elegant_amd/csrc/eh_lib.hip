@@ -205,7 +205,9 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
     n_kicks = q->n_kicks;
     expand = q->expandHamiltonian;
     order[0] = 1;
-    KnL[0] = q->k1 * q->length * (1 + q->fse);
+    /* KnL = n! B / BORE^n e / (m c P) L (1 + FSE), src/multipole.c:739-742 */
+    KnL[0] = q->bore ? q->B / q->bore * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * c.Po)) * q->length * (1 + q->fse)
+                     : q->k1 * q->length * (1 + q->fse);
     drift = q->length;
     tilt = q->tilt;
     dx = q->dx;
@@ -223,7 +225,9 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
     n_kicks = q->n_kicks;
     expand = q->expandHamiltonian;
     order[0] = 2;
-    KnL[0] = q->k2 * q->length * (1 + q->fse);
+    KnL[0] = (q->bore ? 2 * q->B / (q->bore * q->bore) * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * c.Po)) * q->length *
+                                  (1 + q->fse)
+                           : q->k2 * q->length * (1 + q->fse));
     drift = q->length;
     tilt = q->tilt;
     dx = q->dx;
@@ -250,7 +254,9 @@ static long build_multipole(eh_desc *d, const EH_ELEMENT_LIST *e, const eh_ctx &
     n_kicks = q->n_kicks;
     expand = q->expandHamiltonian;
     order[0] = 3;
-    KnL[0] = q->k3 * q->length * (1 + q->fse);
+    KnL[0] = (q->bore ? 6 * q->B / (q->bore * q->bore * q->bore) * (EH_E_MKS / (EH_ME_MKS * EH_C_MKS * c.Po)) *
+                                  q->length * (1 + q->fse)
+                           : q->k3 * q->length * (1 + q->fse));
     drift = q->length;
     tilt = q->tilt;
     dx = q->dx;
@@ -4395,6 +4401,10 @@ static long adj_prepare(eh_adjoint *a, EH_ELEMENT_LIST *const *beamlines, long n
       const int t = beamlines[v][e].type;
       if (t != beamlines[0][e].type)
         return fail(EH_ERR_ARG, "variant %ld element %ld: types differ from variant 0", v, e);
+      if ((t == EH_T_KQUAD && ((const EH_KQUAD *)beamlines[v][e].p_elem)->bore) ||
+          (t == EH_T_KSEXT && ((const EH_KSEXT *)beamlines[v][e].p_elem)->bore) ||
+          (t == EH_T_KOCT && ((const EH_KOCT *)beamlines[v][e].p_elem)->bore))
+        return fail(EH_ERR_UNSUPPORTED, "element %ld: the adjoint does not support strengths set by BORE and B", e);
       if (t == EH_T_RFCA) {
         const EH_RFCA *rf = (const EH_RFCA *)beamlines[v][e].p_elem;
         if (rf->phase_reference)

@@ -40,18 +40,21 @@ class EH_EDRIFT(C.Structure):
 class EH_KQUAD(C.Structure):
     _fields_ = [(n, dbl) for n in ("length", "k1", "tilt", "dx", "dy", "dz", "fse", "xkick", "ykick",
                                    "xKickCalibration", "yKickCalibration")] + \
-               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian", "synch_rad")]
+               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian", "synch_rad")] + \
+               [(n, dbl) for n in ("bore", "B")]
 
 
 class EH_KSEXT(C.Structure):
     _fields_ = [(n, dbl) for n in ("length", "k2", "k1", "j1", "tilt", "dx", "dy", "dz", "fse", "xkick", "ykick",
                                    "xKickCalibration", "yKickCalibration")] + \
-               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian", "synch_rad")]
+               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian", "synch_rad")] + \
+               [(n, dbl) for n in ("bore", "B")]
 
 
 class EH_KOCT(C.Structure):
     _fields_ = [(n, dbl) for n in ("length", "k3", "tilt", "dx", "dy", "dz", "fse")] + \
-               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian", "synch_rad")]
+               [(n, i32) for n in ("n_kicks", "nSlices", "integration_order", "expandHamiltonian", "synch_rad")] + \
+               [(n, dbl) for n in ("bore", "B")]
 
 
 class EH_KQUSE(C.Structure):
@@ -158,13 +161,13 @@ DEFAULTS = {
 
 # elegant parameter names -> struct fields where they differ
 ALIASES = {
-    T_KQUAD: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "HKICK": "xkick", "VKICK": "ykick", "SYNCH_RAD": "synch_rad",
+    T_KQUAD: {"B": "B", "N_KICKS": "n_kicks", "N_SLICES": "nSlices", "HKICK": "xkick", "VKICK": "ykick", "SYNCH_RAD": "synch_rad",
               "HCALIBRATION": "xKickCalibration", "VCALIBRATION": "yKickCalibration",
               "INTEGRATION_ORDER": "integration_order", "EXPAND_HAMILTONIAN": "expandHamiltonian"},
-    T_KSEXT: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "HKICK": "xkick", "VKICK": "ykick", "SYNCH_RAD": "synch_rad",
+    T_KSEXT: {"B": "B", "N_KICKS": "n_kicks", "N_SLICES": "nSlices", "HKICK": "xkick", "VKICK": "ykick", "SYNCH_RAD": "synch_rad",
               "HCALIBRATION": "xKickCalibration", "VCALIBRATION": "yKickCalibration",
               "INTEGRATION_ORDER": "integration_order", "EXPAND_HAMILTONIAN": "expandHamiltonian"},
-    T_KOCT: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "INTEGRATION_ORDER": "integration_order", "SYNCH_RAD": "synch_rad",
+    T_KOCT: {"B": "B", "N_KICKS": "n_kicks", "N_SLICES": "nSlices", "INTEGRATION_ORDER": "integration_order", "SYNCH_RAD": "synch_rad",
              "EXPAND_HAMILTONIAN": "expandHamiltonian"},
     T_KQUSE: {"N_KICKS": "n_kicks", "N_SLICES": "nSlices", "INTEGRATION_ORDER": "integration_order", "SYNCH_RAD": "synch_rad",
               "EXPAND_HAMILTONIAN": "expandHamiltonian"},

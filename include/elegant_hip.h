@@ -84,6 +84,9 @@ typedef struct {
   int32_t integration_order;                              /* 2, 4 or 6 */
   int32_t expandHamiltonian;
   int32_t synch_rad; /* SYNCH_RAD: classical radiation (rad_coef, src/multipole.c:755-763) */
+  /* BORE and B (pole-tip field, T): with BORE != 0 the strength comes from them,
+   * n! B / BORE^n e / (m c P_central), instead of K<n> (src/multipole.c:739-804) */
+  double bore, B;
 } EH_KQUAD;
 
 /* KSEXT: src/track.h:2149-2166, defaults src/track_data.c:1125 */
@@ -94,6 +97,9 @@ typedef struct {
   int32_t integration_order;
   int32_t expandHamiltonian;
   int32_t synch_rad; /* SYNCH_RAD: classical radiation (rad_coef, src/multipole.c:755-763) */
+  /* BORE and B (pole-tip field, T): with BORE != 0 the strength comes from them,
+   * n! B / BORE^n e / (m c P_central), instead of K<n> (src/multipole.c:739-804) */
+  double bore, B;
 } EH_KSEXT;
 
 /* KOCT: src/track.h:2171-2182, defaults src/track_data.c:1169 (N_KICKS=4) */
@@ -103,6 +109,9 @@ typedef struct {
   int32_t integration_order;
   int32_t expandHamiltonian;
   int32_t synch_rad; /* SYNCH_RAD: classical radiation (rad_coef, src/multipole.c:755-763) */
+  /* BORE and B (pole-tip field, T): with BORE != 0 the strength comes from them,
+   * n! B / BORE^n e / (m c P_central), instead of K<n> (src/multipole.c:739-804) */
+  double bore, B;
 } EH_KOCT;
 
 /* KQUSE (quadrupole with a sextupole as secondary multipole): src/track.h:3498-3503,

@@ -765,7 +765,8 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
     n_kicks = q->n_kicks;
     expand = q->expandHamiltonian;
     order[0] = 1;
-    KnL[0] = q->k1 * q->length * (1 + q->fse);
+    KnL[0] = q->bore ? q->B / q->bore * (EH_E_MKS / (EH_ME_MKS * c_mks * Po)) * q->length * (1 + q->fse)
+                     : q->k1 * q->length * (1 + q->fse);
     drift = q->length;
     tilt = q->tilt;
     dx = q->dx;
@@ -783,7 +784,8 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
     nSlices = q->nSlices;
     expand = q->expandHamiltonian;
     order[0] = 2;
-    KnL[0] = q->k2 * q->length * (1 + q->fse);
+    KnL[0] = q->bore ? 2 * q->B / sqr(q->bore) * (EH_E_MKS / (EH_ME_MKS * c_mks * Po)) * q->length * (1 + q->fse)
+                     : q->k2 * q->length * (1 + q->fse);
     drift = q->length;
     tilt = q->tilt;
     dx = q->dx;
@@ -810,7 +812,8 @@ long or_multipole_tracking2(double **particle, long n_part, const EH_ELEMENT_LIS
     nSlices = q->nSlices;
     expand = q->expandHamiltonian;
     order[0] = 3;
-    KnL[0] = q->k3 * q->length * (1 + q->fse);
+    KnL[0] = q->bore ? 6 * q->B / (q->bore * q->bore * q->bore) * (EH_E_MKS / (EH_ME_MKS * c_mks * Po)) * q->length * (1 + q->fse)
+                     : q->k3 * q->length * (1 + q->fse);
     drift = q->length;
     tilt = q->tilt;
     dx = q->dx;

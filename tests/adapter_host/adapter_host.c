@@ -573,8 +573,9 @@ int ah_scenario_fill(void) {
   if (fill_kquad(&q, &dq))
     return 5;
   default_kquad(&q);
-  q.bore = 0.01;
-  if (fill_kquad(&q, &dq))
+  q.bore = 0.01; /* the pole-tip field sets the strength in the library */
+  q.B = 0.4;
+  if (!fill_kquad(&q, &dq) || dq.bore != 0.01 || dq.B != 0.4)
     return 6;
   memset(&cb, 0, sizeof cb);
   cb.length = 0.8;

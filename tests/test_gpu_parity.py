@@ -114,6 +114,9 @@ ELEMENT_CASES = [
                     EDGE1_EFFECTS=2, EDGE2_EFFECTS=1, EDGE_ORDER=2, DX=2e-4, TILT=0.01)),
     ("CSBEND", dict(L=0.8, ANGLE=0.09, K1=0.3, N_KICKS=8, E1=0.07, E2=0.05, HGAP=0.012, FINT=0.5, H1=0.2, H2=0.1,
                     EDGE1_EFFECTS=4, EDGE2_EFFECTS=4)),
+    ("KQUAD", dict(L=0.3, BORE=0.015, B=0.35, N_KICKS=8, FSE=1e-3, DX=1e-4)),
+    ("KSEXT", dict(L=0.2, BORE=0.02, B=0.12, N_KICKS=4, TILT=0.01)),
+    ("KOCT", dict(L=0.1, BORE=0.02, B=0.05, N_KICKS=4)),
 ]
 
 

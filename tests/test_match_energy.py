@@ -270,3 +270,25 @@ def test_linearize_is_linear_in_arrival_time():
         assert np.max(np.abs(np.diff(g, 2))) < 1e-15
         d.append(np.max(np.abs(lin[:, 5] - full[:, 5])))
     assert 3.5 < d[1] / d[0] < 4.5, d
+
+
+@pytest.mark.parametrize("etype,n,field", [("KQUAD", 1, "K1"), ("KSEXT", 2, "K2"), ("KOCT", 3, "K3")])
+def test_bore_field_sets_the_strength(etype, n, field):
+    """BORE and B (src/multipole.c:739-804): the strength is the field gradient
+    n! B / BORE^n over the magnetic rigidity P_central m c / e, so the element
+    tracks exactly as one given that K<n>"""
+    e_mks, me_mks, c = 1.602176634e-19, 9.1093837015e-31, 2.99792458e8
+    bore, B = 0.015, 0.3
+    k = math.factorial(n) * B / bore ** n if n > 1 else B / bore
+    k = k * (e_mks / (me_mks * c * Po))
+    b = beam(300, 12, mean_dp=0.0)
+    kw = dict(L=0.25, N_KICKS=8, FSE=1e-3)
+    r1, n1 = oracle.element(b, Lattice([Element("M", etype, BORE=bore, B=B, **kw)]), 0, Po, 0.0, 0.25)
+    r2, n2 = oracle.element(b, Lattice([Element("M", etype, **{field: k}, **kw)]), 0, Po, 0.0, 0.25)
+    assert n1 == n2
+    if n == 1:
+        assert np.array_equal(r1, r2)
+    else:  # n! B / BORE^n is formed in another order than here
+        np.testing.assert_allclose(r1[:, :4], r2[:, :4], rtol=1e-12, atol=1e-18)
+    r3, _ = oracle.element(b, Lattice([Element("M", etype, BORE=bore, B=2 * B, **kw)]), 0, Po, 0.0, 0.25)
+    assert not np.array_equal(r1, r3)
